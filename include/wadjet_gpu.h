@@ -1,0 +1,323 @@
+/*
+ * wadjet_gpu.h — C-ABI of the MI355X hash-aggregate / hash-join library.
+ *
+ * Plain pointers and sizes only; no C++ or framework types cross this line.
+ * Every entry point returns a wg_status; on failure wg_last_error() returns a
+ * thread-local message (the Go adapter maps it to an `error`, the way
+ * pipeline.go:38-75 turns operator failures into errors).
+ *
+ * Threading: a handle (wg_agg, wg_join, ...) is NOT thread-safe. The caller
+ * serialises calls per handle, as HashAggregate does with h.mu
+ * (internal/engine/exec/aggregate.go:1076); clones get their own handles.
+ * Host pointers passed in are read only for the duration of the call and never
+ * retained (the cgo rule; batch.go:113-131 ownership: a consumer may not keep a
+ * pooled batch or its Sel past the call).
+ *
+ * Reference interfaces replaced (paths relative to the wadjet repository):
+ *   Sink/SinkSource/MergeableSink      internal/engine/exec/operators.go:24-60
+ *   HashAggregate                      internal/engine/exec/aggregate.go:98,1023-1220,3855,4202
+ *   KernelFilter (selection vectors)   internal/engine/exec/filter.go:347-425
+ *   HashJoin.Build / HashJoinProbe     internal/engine/exec/join.go:771-1162,2072-2139,2306-2515
+ *   BoundedOutputOperator              internal/engine/exec/operators.go:86-90, join.go:2044-2060
+ */
+#ifndef WADJET_GPU_H
+#define WADJET_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+typedef enum wg_status {
+  WG_OK = 0,
+  WG_ERR_INVALID = 1,     /* bad argument / shape */
+  WG_ERR_UNSUPPORTED = 2, /* shape the GPU path does not take: caller keeps the CPU operator */
+  WG_ERR_DEVICE = 3,      /* HIP runtime error */
+  WG_ERR_OOM = 4,         /* device allocation failed or footprint over budget */
+  WG_ERR_STATE = 5,       /* call out of order (e.g. consume after finalize) */
+  WG_ERR_NULL_KEY = 6     /* NULL in a multi-column group key (reference migrates to its generic path) */
+} wg_status;
+
+/* Thread-local text of the last failure on this thread ("" if none). */
+const char* wg_last_error(void);
+/* Library version string. */
+const char* wg_version(void);
+
+/* ------------------------------------------------------------------- types */
+
+/* Column storage types. Semantics follow batch.Vector (internal/engine/batch/vector.go:302-465):
+ *   WG_INT32   int32 storage, sign-extended: TypeInt32/Port/Date/Protocol
+ *   WG_INT64   int64 storage: TypeInt64/Timestamp/Duration
+ *   WG_FLOAT64 float64 storage
+ *   WG_DECIMAL scaled int64 storage (device layout for DECIMAL(p<=18,s)); scale in wg_column.scale
+ *   WG_DECIMAL128 batch.Int128 storage {int64 Hi; uint64 Lo} per row (decimal.go:11-14)
+ *   WG_IPV4    int64 storage: TypeIPv4/MAC (keys, MIN/MAX; SUM/AVG emit NULL as in agg_scatter.go:519-578)
+ *   WG_UINT32  compact IPv4: uint32 storage, zero-extended, otherwise as WG_IPV4
+ *   WG_UINT16  compact Port: uint16 storage, zero-extended, otherwise as WG_INT32 */
+typedef enum wg_type {
+  WG_INT32 = 1,
+  WG_INT64 = 2,
+  WG_FLOAT64 = 3,
+  WG_DECIMAL = 4,
+  WG_DECIMAL128 = 5,
+  WG_IPV4 = 6,
+  WG_UINT32 = 7,
+  WG_UINT16 = 8
+} wg_type;
+
+/* Aggregate functions, numbered as exec.AggFunc (aggregate.go:26-31).
+ * COUNT(*) is WG_AGG_COUNT with input_col = -1 (aggColIdx < 0 in the reference). */
+typedef enum wg_agg_func {
+  WG_AGG_SUM = 0,
+  WG_AGG_COUNT = 1,
+  WG_AGG_MIN = 2,
+  WG_AGG_MAX = 3,
+  WG_AGG_AVG = 4
+} wg_agg_func;
+
+/* Comparison operators, numbered as kernel.CompareOp (kernel/compare.go:14-21). */
+typedef enum wg_cmp { WG_EQ = 0, WG_NE = 1, WG_LT = 2, WG_LE = 3, WG_GT = 4, WG_GE = 5 } wg_cmp;
+
+/* Where the column pointers of a call live. */
+typedef enum wg_mem { WG_MEM_HOST = 0, WG_MEM_DEVICE = 1 } wg_mem;
+
+/* One column of a batch: data (typed by `type`), optional validity bitmap
+ * (batch.Bitmap words: bit i%64 of word i/64, 1 = valid; NULL = no nulls). */
+typedef struct wg_column {
+  const void* data;
+  const uint64_t* validity;
+  int32_t type;  /* wg_type */
+  int32_t scale; /* DECIMAL scale, else 0 */
+} wg_column;
+
+/* One aggregate: exec.AggColumn{Func, InputCol} (aggregate.go:73-81).
+ * input_col indexes the batch's columns (-1 = COUNT(*)).
+ * input_col2 >= 0 makes the input the product input_col * input_col2 — the
+ * aggPreProject product (planner/physical/plan.go:7900-7990) formed exactly:
+ * INT x DECIMAL -> DECIMAL at the decimal's scale, INT x INT -> INT64. */
+typedef struct wg_agg_spec {
+  int32_t func;       /* wg_agg_func */
+  int32_t input_col;  /* -1 for COUNT(*) */
+  int32_t input_col2; /* -1 unless a product input */
+  int32_t reserved;
+} wg_agg_spec;
+
+/* A fused predicate `col <op> constant` (KernelFilter with a typed kernel,
+ * filter.go:373-425): rows that fail, or whose column is NULL, are dropped
+ * exactly as the Sel vector it would have produced drops them. */
+typedef struct wg_predicate {
+  int32_t col; /* -1 = no predicate */
+  int32_t op;  /* wg_cmp */
+  int64_t i64; /* constant for int columns */
+  double f64;  /* constant for float columns */
+} wg_predicate;
+
+/* ----------------------------------------------------------------- context */
+
+typedef struct wg_ctx wg_ctx;
+
+/* One context = one device + one HIP stream (default priority). */
+wg_status wg_ctx_create(int device, wg_ctx** out);
+wg_status wg_ctx_destroy(wg_ctx* ctx);
+wg_status wg_ctx_sync(wg_ctx* ctx);
+/* Device properties: compute units, HBM bytes, LDS bytes per workgroup. */
+wg_status wg_ctx_info(wg_ctx* ctx, int32_t* cus, uint64_t* hbm_bytes, int32_t* lds_bytes);
+/* Elapsed ms of the last timed region on the ctx stream (HIP events around
+ * the library's own kernels: see wg_ctx_timer_*). */
+wg_status wg_ctx_timer_start(wg_ctx* ctx);
+wg_status wg_ctx_timer_stop(wg_ctx* ctx, float* ms);
+
+/* Device memory helpers so callers need no other GPU runtime. */
+wg_status wg_malloc(wg_ctx* ctx, uint64_t bytes, void** out);
+wg_status wg_free(wg_ctx* ctx, void* ptr);
+wg_status wg_memcpy_h2d(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+wg_status wg_memcpy_d2h(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+wg_status wg_memset(wg_ctx* ctx, void* dst, int value, uint64_t bytes);
+
+/* ------------------------------------------------------------- aggregation */
+
+typedef struct wg_agg wg_agg;
+
+/* Create a grouped aggregation: GROUP BY key_cols[0..n_keys) over a batch
+ * schema col_types[0..n_cols) (wg_type) with col_nullable[i] != 0 when the
+ * column may carry a validity bitmap. Typed dispatch is resolved here, once,
+ * like resolveIndices (aggregate.go:1296-1764): one int-class key -> 64-bit
+ * key; 2-4 int-class keys -> packed composite key (packed_hash.go:386-472).
+ * group_hint sizes the table (InputRowHint/GroupNDVHint, aggregate.go:1464-1506);
+ * the table grows when it is exceeded.
+ * Keys wider than 8 bytes in total, string keys and other functions return
+ * WG_ERR_UNSUPPORTED (the caller keeps the CPU operator). */
+wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types,
+                        const int32_t* col_scales, const uint8_t* col_nullable,
+                        int32_t n_keys, const int32_t* key_cols, int32_t n_aggs,
+                        const wg_agg_spec* aggs, uint64_t group_hint, wg_agg** out);
+wg_status wg_agg_destroy(wg_agg* h);
+
+/* Sink.Consume (operators.go:27; aggregate.go:1075). cols[n_cols] follow the
+ * create-time schema; sel (ascending row indices, may be NULL) selects n_sel
+ * of the n_rows rows; pred is an optional fused filter (NULL = none).
+ * mem = WG_MEM_HOST: the call copies what it needs before returning and
+ * coalesces small batches into large device batches. mem = WG_MEM_DEVICE:
+ * pointers are device-resident; the launch is enqueued on the ctx stream. */
+wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows,
+                         const uint32_t* sel, uint64_t n_sel, const wg_predicate* pred,
+                         int32_t mem);
+/* MergeableSink.MergeSink (operators.go:56-60; aggregate.go:4682-4700):
+ * fold src's partial state into dst (same create-time spec). src is left empty. */
+wg_status wg_agg_merge(wg_agg* dst, wg_agg* src);
+/* Sink.Finalize (aggregate.go:3855): flush staged rows, compact groups. */
+wg_status wg_agg_finalize(wg_agg* h);
+/* Number of groups after finalize. */
+wg_status wg_agg_num_groups(wg_agg* h, uint64_t* out);
+/* Source.Next payload (aggregate.go:4226-4524), group order unspecified.
+ * Writes `count` groups starting at `offset` to host or device buffers:
+ *   key_out[k]      : key column k in its input storage type (int32/int64/uint32/uint16)
+ *   val_out[a]      : finalized value of aggregate a — float64 for SUM/AVG and
+ *                     float MIN/MAX, int64 for COUNT and int MIN/MAX
+ *                     (writeAccToColumn, aggregate_partial_spill.go:1855-1945)
+ *   valid_out[a]    : 1 byte per group, 0 = NULL (SUM/AVG with Count==0, MIN/MAX without a value)
+ * Any pointer may be NULL to skip that column. */
+wg_status wg_agg_fetch(wg_agg* h, uint64_t offset, uint64_t count, void* const* key_out,
+                       void* const* val_out, uint8_t* const* valid_out, int32_t mem);
+/* Raw accumulator state per aggregate (kernel.Accumulator fields, types.go:21-40):
+ * count[g] (int64), sum_lo/sum_hi[g] (int64 sum in sum_lo, or Int128 as lo/hi,
+ * or float64 bits of SumF64 in sum_lo), minmax[g] (int64, or float64 bits).
+ * Used for the exact parity check of Int128 sums. NULL pointers are skipped. */
+wg_status wg_agg_fetch_state(wg_agg* h, int32_t agg, uint64_t offset, uint64_t count,
+                             int64_t* count_out, uint64_t* sum_lo_out, int64_t* sum_hi_out,
+                             int64_t* minmax_out, int32_t mem);
+/* Validity of the group keys (1 byte per group, 0 = the NULL group of a
+ * single-column key, aggregate_null_group_test.go:48-289). */
+wg_status wg_agg_fetch_key_valid(wg_agg* h, uint64_t offset, uint64_t count, uint8_t* valid_out,
+                                 int32_t mem);
+/* Drop all groups and staged rows; keep the spec and the table memory. */
+wg_status wg_agg_reset(wg_agg* h);
+/* Device bytes held by this aggregate (table + state + staging). */
+wg_status wg_agg_footprint(wg_agg* h, uint64_t* bytes);
+/* Tuning knobs (autotuner output): lds_slots = 0 disables the LDS table,
+ * block = threads per workgroup (256, 512, 1024). */
+wg_status wg_agg_set_tuning(wg_agg* h, int32_t lds_slots, int32_t block);
+/* Counters from the last consume: rows that hit the LDS table, rows that went
+ * to the global table, LDS entries flushed, global table capacity. */
+wg_status wg_agg_stats(wg_agg* h, uint64_t* lds_rows, uint64_t* global_rows,
+                       uint64_t* flushed, uint64_t* capacity);
+
+/* Device time (HIP events on the ctx stream) of the consume kernel launches
+ * of the last consume call, and how many launches it took (1 unless the
+ * table had to grow). */
+wg_status wg_agg_kernel_time(wg_agg* h, float* ms, uint32_t* launches);
+
+/* ------------------------------------------------------------------ filter */
+
+/* KernelFilter.Execute (filter.go:373-425) on device-resident data: writes the
+ * ascending indices of rows in [0,n_rows) (or of sel[0..n_sel) when sel != NULL)
+ * whose column passes `col <op> constant` (NULL never passes) to out_sel and
+ * their number to *n_out. Compaction is per 64-lane wavefront (ballot + mbcnt
+ * prefix) with an order-preserving block scan. All pointers device-resident
+ * except n_out (host). */
+wg_status wg_filter(wg_ctx* ctx, const wg_column* col, uint64_t n_rows, const uint32_t* sel,
+                    uint64_t n_sel, int32_t op, int64_t i64, double f64, uint32_t* out_sel,
+                    uint64_t* n_out);
+
+/* -------------------------------------------------------------------- join */
+
+typedef struct wg_join wg_join;
+
+/* HashJoin{InnerJoin, LeftKeys=[probe key], RightKeys=[build key]} (join.go:42-240)
+ * with a single int-class key (tryEnableIntKey, join.go:520-546). The build
+ * side schema is build_types[0..n_build_cols); build_key is its key column.
+ * Columns listed in inline_cols are copied into the table slots so that a
+ * fused probe reads them with the key (at most 4). build_hint sizes the table. */
+wg_status wg_join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types,
+                         const int32_t* build_scales, int32_t build_key, int32_t n_inline,
+                         const int32_t* inline_cols, uint64_t build_hint, wg_join** out);
+wg_status wg_join_destroy(wg_join* j);
+/* HashJoin.Build per batch (join.go:1019-1065): rows with a NULL key are
+ * skipped. Build columns are retained on the device for the gathers. */
+wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_rows, int32_t mem);
+/* End of build (join.go:1126-1152). */
+wg_status wg_join_build_finish(wg_join* j);
+wg_status wg_join_build_rows(wg_join* j, uint64_t* rows);
+
+/* Resumable probe cursor (HashJoinProbe.res, join.go:2670-2836). */
+typedef struct wg_probe_cursor {
+  uint64_t pos;   /* next probe position (index into sel or rows) */
+  int64_t chain;  /* build row to resume a suspended chain at, -1 = none */
+  int32_t done;   /* 1 when the input is fully probed */
+  int32_t reserved;
+} wg_probe_cursor;
+
+/* Inner-join probe of one device-resident probe batch (inlineIntProbe,
+ * join.go:2670-2814): emits at most out_cap (probe_row, build_row) pairs,
+ * compacted per wavefront; resumes from *cursor and updates it. Rows with a
+ * NULL key never match. Pair order is unspecified; the pair multiset is the
+ * reference's. probe_key is a device column; out_* are device buffers;
+ * *n_out is written on the host. */
+wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
+                        const uint32_t* sel, uint64_t n_sel, uint32_t* out_probe_rows,
+                        uint32_t* out_build_rows, uint64_t out_cap, wg_probe_cursor* cursor,
+                        uint64_t* n_out);
+/* Gather a retained build column at build row indices (gatherBuildVector,
+ * join.go:4024) into a device buffer of the column's storage type. */
+wg_status wg_join_gather_build(wg_join* j, int32_t build_col, const uint32_t* build_rows,
+                               uint64_t n, void* out);
+/* Gather a device column at row indices (gatherVector, sort.go:611). */
+wg_status wg_gather(wg_ctx* ctx, const wg_column* col, const uint32_t* rows, uint64_t n,
+                    void* out);
+
+/* Fused probe -> aggregate: join the device-resident probe batch against the
+ * build, and feed every matched (probe row, build row) pair to `agg` as one
+ * joined row whose columns are the probe columns followed by the build
+ * columns (joinOutputSchemaWithMapping, join.go:3898). The aggregate's
+ * create-time schema must be exactly that concatenation. */
+wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_probe_cols,
+                            int32_t probe_key, uint64_t n_rows, const uint32_t* sel,
+                            uint64_t n_sel, wg_agg* agg);
+/* Order-free digest of the join output multiset over (probe key, probe col q,
+ * build col p1, build col p2): count plus two independent 64-bit sums of
+ * per-row mixes, computed on the device from a pairs list. */
+wg_status wg_join_digest(wg_join* j, const wg_column* probe_cols, const uint32_t* probe_rows,
+                         const uint32_t* build_rows, uint64_t n, const int32_t* probe_sel_cols,
+                         int32_t n_probe_sel, const int32_t* build_sel_cols, int32_t n_build_sel,
+                         uint64_t digest_out[2]);
+
+/* ------------------------------------------------------- batched execution */
+
+/* Many independent GROUP BY queries in one persistent launch. Each query has
+ * its own device-resident columns, spec and result buffers. The runner owns a
+ * work queue of (query, tile) descriptors, a hipGraph of
+ * {reset, persistent aggregate, finalize} and the autotuner's choices. */
+typedef struct wg_batch wg_batch;
+
+typedef struct wg_batch_query {
+  wg_column key;       /* one int-class key column (device) */
+  wg_column val;       /* value column for SUM (device); type INT64 or DECIMAL */
+  uint64_t n_rows;
+  uint64_t est_groups; /* cardinality estimate used to size the table */
+} wg_batch_query;
+
+wg_status wg_batch_create(wg_ctx* ctx, int32_t max_queries, uint64_t max_groups_total,
+                          wg_batch** out);
+wg_status wg_batch_destroy(wg_batch* b);
+/* Run the queries: one persistent launch (graph-replayed when use_graph) or,
+ * when plain != 0, one launch sequence per query. Results stay on the device
+ * until wg_batch_result. Per query: GROUP BY key SUM(val), COUNT(*). */
+wg_status wg_batch_run(wg_batch* b, const wg_batch_query* qs, int32_t n_queries,
+                       int32_t use_graph, int32_t plain);
+/* Autotune: time the candidate layouts/workgroup sizes on these queries and
+ * keep the fastest per (rows, groups, key width) bucket. */
+wg_status wg_batch_autotune(wg_batch* b, const wg_batch_query* qs, int32_t n_queries);
+/* Per-query result: number of groups, then keys/sums(lo,hi as Int128)/counts. */
+wg_status wg_batch_result(wg_batch* b, int32_t q, uint64_t* n_groups, int64_t* keys,
+                          uint64_t* sum_lo, int64_t* sum_hi, int64_t* counts, uint64_t cap);
+/* Chosen layout for query q: 1 = LDS table, 0 = global table; workgroup size. */
+wg_status wg_batch_choice(wg_batch* b, int32_t q, int32_t* use_lds, int32_t* block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WADJET_GPU_H */

@@ -1,0 +1,266 @@
+"""CPU oracle for the HIP hash aggregate / hash join (TEST INFRASTRUCTURE).
+
+A C restatement of wadjet's operator semantics (oracle/oracle.c, built into
+oracle/liboracle.so by oracle/Makefile) plus numpy helpers. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this package;
+the product library never does.
+
+Pinning (how each function is checked, see tests/test_oracle.py):
+  Int128 add/neg/less/to_f64/from_f64/format/parse   reference KATs (decimal_test.go:8-157)
+                                                     + Python-int arithmetic on random values
+  or_hash_agg integer/decimal/count/min/max          brute force in pure Python dicts on small
+                                                     inputs, closed forms on constructed data,
+                                                     reference KATs (exec_test.go:39-96, dual_int_test.go)
+  or_hash_agg float64 SUM/AVG                        math.fsum-based bound + row-order Python loop
+  or_filter                                          numpy boolean masks
+  or_hash_join                                       brute-force nested loop, reference KATs
+                                                     (join_test.go:10-56, join_coverage_test.go:101-141)
+  or_digest_rows                                     order invariance + a Python restatement
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+INT32, INT64, FLOAT64, DECIMAL, DECIMAL128, IPV4, UINT32, UINT16 = range(1, 9)
+SUM, COUNT, MIN, MAX, AVG = range(5)
+EQ, NE, LT, LE, GT, GE = range(6)
+
+
+class OrCol(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("validity", C.c_void_p), ("type", C.c_int32), ("scale", C.c_int32)]
+
+
+class OrAcc(C.Structure):
+    _fields_ = [("count", C.c_int64), ("sum_i64", C.c_int64), ("sum_f64", C.c_double), ("sum_lo", C.c_uint64),
+                ("sum_hi", C.c_int64), ("min_i64", C.c_int64), ("max_i64", C.c_int64), ("min_f64", C.c_double),
+                ("max_f64", C.c_double), ("has_min", C.c_int32), ("has_max", C.c_int32), ("is_float", C.c_int32),
+                ("is_decimal", C.c_int32)]
+
+
+ACC_DTYPE = np.dtype([("count", np.int64), ("sum_i64", np.int64), ("sum_f64", np.float64), ("sum_lo", np.uint64),
+                      ("sum_hi", np.int64), ("min_i64", np.int64), ("max_i64", np.int64), ("min_f64", np.float64),
+                      ("max_f64", np.float64), ("has_min", np.int32), ("has_max", np.int32),
+                      ("is_float", np.int32), ("is_decimal", np.int32)])
+assert ACC_DTYPE.itemsize == C.sizeof(OrAcc)
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P, U64, I64, I32, D = C.c_void_p, C.c_uint64, C.c_int64, C.c_int32, C.c_double
+        L.or_i128_from_i64.argtypes = [I64, C.POINTER(U64), C.POINTER(I64)]
+        L.or_i128_add.argtypes = [U64, I64, U64, I64, C.POINTER(U64), C.POINTER(I64)]
+        L.or_i128_neg.argtypes = [U64, I64, C.POINTER(U64), C.POINTER(I64)]
+        L.or_i128_less.argtypes = [U64, I64, U64, I64]
+        L.or_i128_less.restype = C.c_int
+        L.or_i128_to_f64.argtypes = [U64, I64, C.c_int]
+        L.or_i128_to_f64.restype = D
+        L.or_i128_from_f64.argtypes = [D, C.c_int, C.POINTER(U64), C.POINTER(I64)]
+        L.or_i128_format.argtypes = [U64, I64, C.c_int, C.c_char_p, C.c_int]
+        L.or_parse_decimal.argtypes = [C.c_char_p, C.c_int]
+        L.or_parse_decimal.restype = I64
+        L.or_hash_agg.argtypes = [P, I32, U64, P, U64, I32, I32, I64, D, P, I32, P, P, I32, P, P, P, I64]
+        L.or_hash_agg.restype = I64
+        L.or_hash_agg_parallel.argtypes = [P, I32, U64, I32, I32, I64, D, P, I32, P, P, I32, I32, P, P, I64]
+        L.or_hash_agg_parallel.restype = I64
+        L.or_finalize.argtypes = [P, I32, I32, I32, C.POINTER(D), C.POINTER(I64)]
+        L.or_finalize.restype = C.c_int
+        L.or_filter.argtypes = [P, U64, P, U64, I32, I64, D, P]
+        L.or_filter.restype = U64
+        L.or_hash_join.argtypes = [P, U64, P, U64, P, U64, P, P, I64]
+        L.or_hash_join.restype = I64
+        L.or_digest_rows.argtypes = [P, U64, I32, P]
+        L.or_mix64.argtypes = [U64]
+        L.or_mix64.restype = U64
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_cols(cols):
+    """cols: list of (np.ndarray, type[, validity words[, scale]])."""
+    keep = []
+    arr = (OrCol * len(cols))()
+    for i, c in enumerate(cols):
+        data, t = c[0], c[1]
+        valid = c[2] if len(c) > 2 else None
+        scale = c[3] if len(c) > 3 else 0
+        data = np.ascontiguousarray(data)
+        keep.append(data)
+        if valid is not None:
+            valid = np.ascontiguousarray(valid, dtype=np.uint64)
+            keep.append(valid)
+        arr[i] = OrCol(_ptr(data), _ptr(valid), t, scale)
+    return arr, keep
+
+
+# ------------------------------------------------------------------ Int128
+
+def i128_split(v: int):
+    v &= (1 << 128) - 1
+    lo = v & ((1 << 64) - 1)
+    hi = v >> 64
+    if hi >= 1 << 63:
+        hi -= 1 << 64
+    return lo, hi
+
+
+def i128_join(lo: int, hi: int) -> int:
+    v = (int(hi) << 64) | (int(lo) & ((1 << 64) - 1))
+    if v >= 1 << 127:
+        v -= 1 << 128
+    return v
+
+
+def i128_from_i64(v: int):
+    lo, hi = C.c_uint64(), C.c_int64()
+    lib().or_i128_from_i64(v, C.byref(lo), C.byref(hi))
+    return lo.value, hi.value
+
+
+def i128_add(a: int, b: int) -> int:
+    alo, ahi = i128_split(a)
+    blo, bhi = i128_split(b)
+    lo, hi = C.c_uint64(), C.c_int64()
+    lib().or_i128_add(alo, ahi, blo, bhi, C.byref(lo), C.byref(hi))
+    return i128_join(lo.value, hi.value)
+
+
+def i128_neg(a: int) -> int:
+    lo, hi = C.c_uint64(), C.c_int64()
+    alo, ahi = i128_split(a)
+    lib().or_i128_neg(alo, ahi, C.byref(lo), C.byref(hi))
+    return i128_join(lo.value, hi.value)
+
+
+def i128_less(a: int, b: int) -> bool:
+    alo, ahi = i128_split(a)
+    blo, bhi = i128_split(b)
+    return bool(lib().or_i128_less(alo, ahi, blo, bhi))
+
+
+def i128_to_f64(v: int, scale: int) -> float:
+    lo, hi = i128_split(v)
+    return lib().or_i128_to_f64(lo, hi, scale)
+
+
+def i128_from_f64(f: float, scale: int) -> int:
+    lo, hi = C.c_uint64(), C.c_int64()
+    lib().or_i128_from_f64(f, scale, C.byref(lo), C.byref(hi))
+    return i128_join(lo.value, hi.value)
+
+
+def i128_format(v: int, scale: int) -> str:
+    lo, hi = i128_split(v)
+    buf = C.create_string_buffer(128)
+    lib().or_i128_format(lo, hi, scale, buf, 128)
+    return buf.value.decode()
+
+
+def parse_decimal(s: str, scale: int) -> int:
+    return lib().or_parse_decimal(s.encode(), scale)
+
+
+# --------------------------------------------------------------- operators
+
+def hash_agg(cols, keys, aggs, sel=None, pred=None, max_groups=None, threads: int = 0):
+    """GROUP BY keys over cols with aggs [(func, col)]; pred = (col, op, i64, f64).
+
+    Returns dict(n, keys[n, k] int64, key_null[n, k] bool, acc structured[n, a]).
+    threads > 0 uses the partial-aggregate + merge baseline (no sel, no nulls).
+    """
+    arr, keep = make_cols(cols)
+    n_rows = len(cols[0][0]) if cols[0][1] != DECIMAL128 else len(cols[0][0]) // 2
+    kc = np.asarray(keys, dtype=np.int32)
+    af = np.asarray([a[0] for a in aggs], dtype=np.int32)
+    ac = np.asarray([a[1] for a in aggs], dtype=np.int32)
+    sel_a = None if sel is None else np.ascontiguousarray(sel, dtype=np.uint32)
+    n_sel = 0 if sel_a is None else len(sel_a)
+    mg = max_groups if max_groups is not None else max(n_sel if sel_a is not None else n_rows, 1)
+    keys_out = np.empty((mg, len(keys)), dtype=np.int64)
+    null_out = np.empty((mg, len(keys)), dtype=np.uint8)
+    acc_out = np.zeros((mg, max(len(aggs), 1)), dtype=ACC_DTYPE)
+    pc, po, pi, pf = (-1, 0, 0, 0.0) if pred is None else pred
+    if threads:
+        n = lib().or_hash_agg_parallel(arr, len(cols), n_rows, pc, po, pi, pf, _ptr(kc), len(keys), _ptr(af), _ptr(ac),
+                                       len(aggs), threads, _ptr(keys_out), _ptr(acc_out), mg)
+        null_out[:] = 0
+    else:
+        n = lib().or_hash_agg(arr, len(cols), n_rows, _ptr(sel_a), n_sel, pc, po, pi, pf, _ptr(kc), len(keys),
+                              _ptr(af), _ptr(ac), len(aggs), _ptr(keys_out), _ptr(null_out), _ptr(acc_out), mg)
+    if n < 0:
+        raise RuntimeError("oracle hash_agg: group limit exceeded")
+    return dict(n=n, keys=keys_out[:n], key_null=null_out[:n].astype(bool), acc=acc_out[:n])
+
+
+def finalize(acc_col, func: int, in_type: int, scale: int = 0):
+    """Finalized values for one aggregate column: (values, valid)."""
+    n = len(acc_col)
+    is_f = func in (SUM, AVG) or (func in (MIN, MAX) and in_type == FLOAT64)
+    vals = np.empty(n, dtype=np.float64 if is_f else np.int64)
+    valid = np.empty(n, dtype=bool)
+    f, i = C.c_double(), C.c_int64()
+    a = np.ascontiguousarray(acc_col)
+    base = a.ctypes.data
+    for g in range(n):
+        ok = lib().or_finalize(C.c_void_p(base + g * ACC_DTYPE.itemsize), func, in_type, scale, C.byref(f),
+                               C.byref(i))
+        valid[g] = bool(ok)
+        vals[g] = f.value if is_f else i.value
+    return vals, valid
+
+
+def filter_rows(col, n_rows, op, i64=0, f64=0.0, sel=None):
+    arr, keep = make_cols([col])
+    sel_a = None if sel is None else np.ascontiguousarray(sel, dtype=np.uint32)
+    out = np.empty(max(n_rows if sel_a is None else len(sel_a), 1), dtype=np.uint32)
+    k = lib().or_filter(arr, n_rows, _ptr(sel_a), 0 if sel_a is None else len(sel_a), op, i64, f64, _ptr(out))
+    return out[:k]
+
+
+def hash_join(build_key, probe_key, probe_sel=None, cap=None):
+    """build_key/probe_key: (array, type[, validity]). Returns (probe_rows, build_rows)."""
+    ba, k1 = make_cols([build_key])
+    pa, k2 = make_cols([probe_key])
+    nb = len(build_key[0])
+    npr = len(probe_key[0])
+    sel_a = None if probe_sel is None else np.ascontiguousarray(probe_sel, dtype=np.uint32)
+    cap = cap if cap is not None else max(npr * 4, 16)
+    op = np.empty(cap, dtype=np.uint32)
+    ob = np.empty(cap, dtype=np.uint32)
+    n = lib().or_hash_join(ba, nb, pa, npr, _ptr(sel_a), 0 if sel_a is None else len(sel_a), _ptr(op), _ptr(ob), cap)
+    if n < 0:
+        raise RuntimeError("oracle hash_join: output capacity exceeded")
+    return op[:n], ob[:n]
+
+
+def digest_rows(rows: np.ndarray):
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    n, w = rows.shape
+    d = np.zeros(2, dtype=np.uint64)
+    lib().or_digest_rows(_ptr(rows), n, w, _ptr(d))
+    return int(d[0]), int(d[1])
+
+
+def mix64(k: int) -> int:
+    return lib().or_mix64(k & ((1 << 64) - 1))

@@ -1,0 +1,204 @@
+"""GPU parity of the grouped aggregate (wg_agg_*) against the CPU oracle.
+
+Every test generates its inputs on the device from a fixed seed, copies them
+back, runs the oracle on the same rows and compares sorted by key: integer,
+COUNT, MIN/MAX and Int128 DECIMAL results bit-exact, float64 SUM/AVG within
+FLOAT_RTOL (tests/helpers.py).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import compare, gpu_groups, oracle_groups
+from wadjet_amd import _lib as L
+from wadjet_amd import workloads as W
+from wadjet_amd.gpu import AggSpec, Col, HashAggregate, Pred, bitmap_from_mask
+
+pytestmark = pytest.mark.gpu
+
+NP = {L.INT32: np.int32, L.INT64: np.int64, L.FLOAT64: np.float64, L.DECIMAL: np.int64, L.UINT32: np.uint32,
+      L.UINT16: np.uint16, L.IPV4: np.int64}
+
+
+def host_cols(cols, n):
+    out = []
+    for c in cols:
+        arr = c.data.to_host(NP[c.type], n)
+        v = None if c.validity is None else c.validity.to_host(np.uint64, (n + 63) // 64)
+        out.append((arr, c.type, v, c.scale))
+    return out
+
+
+def run_gpu(ctx, cols, n, types, keys, aggs, scales=None, nullable=None, pred=None, sel=None, hint=0, tuning=None):
+    agg = HashAggregate(ctx, types, keys, aggs, scales=scales, nullable=nullable, group_hint=hint)
+    if tuning:
+        agg.set_tuning(*tuning)
+    agg.consume(cols, n, sel=sel, pred=pred)
+    agg.finalize()
+    return agg
+
+
+def run_oracle(hcols, keys, aggs, pred=None, sel=None):
+    p = None if pred is None else (pred.col, pred.op, pred.i64, pred.f64)
+    return O.hash_agg(hcols, keys, [(a.func, a.col) for a in aggs], sel=sel, pred=p)
+
+
+def abs_sums(hcols, keys, aggs, pred=None, sel=None):
+    """per-group sum(|v|) for float SUM/AVG aggregates (oracle group order)."""
+    out = []
+    for a in aggs:
+        if a.func in (L.SUM, L.AVG) and hcols[a.col][1] == L.FLOAT64:
+            h2 = list(hcols) + [(np.abs(hcols[a.col][0]), L.FLOAT64, hcols[a.col][2])]
+            r = O.hash_agg(h2, keys, [(O.SUM, len(hcols))], sel=sel,
+                           pred=None if pred is None else (pred.col, pred.op, pred.i64, pred.f64))
+            out.append(oracle_groups(r)["acc"][:, 0]["sum_f64"])
+        else:
+            out.append(None)
+    return out
+
+
+def check(ctx, cols, n, types, keys, aggs, scales=None, nullable=None, pred=None, sel=None, hint=0, tuning=None):
+    print(f"[agg] n={n} keys={keys} types={types} pred={pred} sel={'yes' if sel is not None else 'no'} "
+          f"hint={hint} tuning={tuning}", flush=True)
+    agg = run_gpu(ctx, cols, n, types, keys, aggs, scales, nullable, pred, sel, hint, tuning)
+    hcols = host_cols(cols, n)
+    hsel = None if sel is None else (sel.to_host(np.uint32) if hasattr(sel, "to_host") else sel)
+    ref = oracle_groups(run_oracle(hcols, keys, aggs, pred, hsel))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in aggs], types, scales, abs_sums(hcols, keys, aggs, pred, hsel))
+    return agg
+
+
+def test_config1_shape(ctx):
+    """k int64 over 1000 values: SUM(v), COUNT(*), MIN(v), MAX(v) GROUP BY k."""
+    cols = W.gen_config1(ctx, 1_000_000)
+    agg = check(ctx, cols, 1_000_000, W.CFG1_TYPES, [0], W.CFG1_AGGS)
+    assert agg.num_groups() == 1000
+
+
+@pytest.mark.parametrize("tuning", [None, (0, 256), (256, 1024), (2048, 512)])
+def test_config2_shape_small(ctx, tuning):
+    """Composite Zipf key, Int128 DECIMAL sum, float min/max/avg, fused filter."""
+    n = 3_000_000
+    cols = W.gen_config2(ctx, n)
+    check(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED, tuning=tuning)
+
+
+def test_table_growth_from_tiny_hint(ctx):
+    """Start from the minimum table and let overflow + grow + replay do the work."""
+    n = 2_000_000
+    cols = W.gen_config2(ctx, n, seed=7)
+    agg = check(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED, hint=1)
+    assert agg.stats()["capacity"] > 1024
+
+
+def test_selection_vector_matches_predicate(ctx):
+    """A device Sel from the filter kernel gives the same groups as the fused predicate."""
+    from wadjet_amd.gpu import filter_sel
+    n = 1_000_000
+    cols = W.gen_config2(ctx, n, seed=11)
+    sel, k = filter_sel(ctx, cols[3], n, L.GT, f64=-0.5)
+    a1 = run_gpu(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, sel=(sel, k))
+    a2 = run_gpu(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED)
+    g1, g2 = gpu_groups(a1), gpu_groups(a2)
+    assert g1["n"] == g2["n"]
+    np.testing.assert_array_equal(g1["keys"], g2["keys"])
+    for a in range(len(W.CFG2_AGGS)):
+        if a == 4:
+            np.testing.assert_allclose(g1["values"][a], g2["values"][a], rtol=1e-12)
+        else:
+            np.testing.assert_array_equal(g1["values"][a], g2["values"][a])
+
+
+def test_all_types_and_nulls(ctx):
+    """Every column type, NULLs in values and in a single key (the NULL group)."""
+    rng = np.random.default_rng(5)
+    n = 200_000
+    key = rng.integers(-50, 50, n).astype(np.int64)
+    kvalid = rng.random(n) > 0.05
+    i32 = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    i64 = rng.integers(-2**62, 2**62, n).astype(np.int64)
+    f64 = rng.standard_normal(n)
+    dec = rng.integers(-10**15, 10**15, n).astype(np.int64)
+    u32 = rng.integers(0, 2**32, n).astype(np.uint32)
+    u16 = rng.integers(0, 2**16, n).astype(np.uint16)
+    ipv4 = rng.integers(0, 2**32, n).astype(np.int64)
+    masks = [kvalid] + [rng.random(n) > 0.1 for _ in range(7)]
+    arrays = [key, i32, i64, f64, dec, u32, u16, ipv4]
+    types = [L.INT64, L.INT32, L.INT64, L.FLOAT64, L.DECIMAL, L.UINT32, L.UINT16, L.IPV4]
+    scales = [0, 0, 0, 0, 3, 0, 0, 0]
+    cols = []
+    for arr, t, m, s in zip(arrays, types, masks, scales):
+        cols.append(Col(ctx.to_device(arr), t, ctx.to_device(bitmap_from_mask(m)), s))
+    for c in range(1, 8):
+        aggs = [AggSpec(L.COUNT, -1)] + [AggSpec(f, c) for f in (L.SUM, L.COUNT, L.MIN, L.MAX, L.AVG)]
+        check(ctx, cols, n, types, [0], aggs, scales, nullable=[True] * 8)
+
+
+def test_decimal_carries_exact(ctx):
+    """Int128 sums that cross 2^64 and go negative: carry-save limbs stay exact."""
+    rng = np.random.default_rng(9)
+    n = 500_000
+    k = rng.integers(0, 8, n).astype(np.int32)
+    d = rng.integers(2**62, 2**63 - 1, n).astype(np.int64)
+    d[rng.random(n) < 0.3] *= -1
+    cols = [Col(ctx.to_device(k), L.INT32), Col(ctx.to_device(d), L.DECIMAL, None, 4)]
+    check(ctx, cols, n, [L.INT32, L.DECIMAL], [0], [AggSpec(L.SUM, 1), AggSpec(L.AVG, 1)], [0, 4])
+
+
+def test_sentinel_key_and_extremes(ctx):
+    """The key value equal to the table's EMPTY sentinel and int64 extremes."""
+    k = np.array([-0x7FFFFFFFFFFFFFFF, 0, -2**63, 2**63 - 1, -0x7FFFFFFFFFFFFFFF, 5] * 1000, dtype=np.int64)
+    v = np.arange(len(k), dtype=np.int64) * 7 - 3000
+    cols = [Col(ctx.to_device(k), L.INT64), Col(ctx.to_device(v), L.INT64)]
+    agg = check(ctx, cols, len(k), [L.INT64, L.INT64], [0], W.CFG1_AGGS)
+    assert agg.num_groups() == 5
+
+
+def test_host_memory_morsels_and_merge(ctx):
+    """2048-row host morsels with Sel vectors (coalesced staging), two clone
+    sinks merged (MergeSink), equals one device pass."""
+    rng = np.random.default_rng(3)
+    n = 300_000
+    a = rng.integers(0, 5000, n).astype(np.int32)
+    b = rng.integers(-3, 3, n).astype(np.int32)
+    p = rng.integers(100, 10**7, n).astype(np.int64)
+    f = rng.standard_normal(n)
+    types = W.CFG2_TYPES
+    aggs = W.CFG2_AGGS
+    s1 = HashAggregate(ctx, types, W.CFG2_KEYS, aggs, scales=W.CFG2_SCALES)
+    s2 = HashAggregate(ctx, types, W.CFG2_KEYS, aggs, scales=W.CFG2_SCALES)
+    all_sel = []
+    for m, start in enumerate(range(0, n, 2048)):
+        end = min(n, start + 2048)
+        cols = [Col(x[start:end], t, None, sc) for x, t, sc in zip((a, b, p, f), types, W.CFG2_SCALES)]
+        sel = np.nonzero(f[start:end] > -0.5)[0].astype(np.uint32)
+        all_sel.append(sel + start)
+        (s1 if m % 2 == 0 else s2).consume(cols, end - start, sel=sel, device=False)
+    s1.merge(s2)
+    s1.finalize()
+    hcols = [(a, L.INT32), (b, L.INT32), (p, L.DECIMAL, None, 2), (f, L.FLOAT64)]
+    sel = np.concatenate(all_sel)
+    ref = oracle_groups(run_oracle(hcols, W.CFG2_KEYS, aggs, sel=sel))
+    compare(gpu_groups(s1), ref, [(x.func, x.col) for x in aggs], types, W.CFG2_SCALES,
+            abs_sums(hcols, W.CFG2_KEYS, aggs, sel=sel))
+
+
+def test_multi_key_null_is_rejected(ctx):
+    from wadjet_amd import NullKeyError
+    a = np.array([1, 2, 3], np.int32)
+    m = bitmap_from_mask(np.array([True, False, True]))
+    agg = HashAggregate(ctx, [L.INT32, L.INT32], [0, 1], [AggSpec(L.COUNT, -1)], nullable=[True, False])
+    with pytest.raises(NullKeyError):
+        agg.consume([Col(ctx.to_device(a), L.INT32, ctx.to_device(m)), Col(ctx.to_device(a), L.INT32)], 3)
+
+
+def test_empty_input_and_reset(ctx):
+    agg = HashAggregate(ctx, W.CFG1_TYPES, [0], W.CFG1_AGGS)
+    cols = W.gen_config1(ctx, 10)
+    agg.consume(cols, 0)
+    agg.finalize()
+    assert agg.num_groups() == 0
+    agg.reset()
+    agg.consume(cols, 10)
+    agg.finalize()
+    assert agg.num_groups() > 0

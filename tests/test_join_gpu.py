@@ -1,0 +1,164 @@
+"""GPU parity of the hash join (wg_join_*) against the CPU oracle: the joined
+row multiset (pairs and their digest), the bounded/resumable probe, NULL keys,
+duplicate build keys, and the fused probe -> GROUP BY with exact Int128 sums."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import compare, gpu_groups, oracle_groups
+from wadjet_amd import _lib as L
+from wadjet_amd import workloads as W
+from wadjet_amd.gpu import AggSpec, Col, HashAggregate, HashJoin, bitmap_from_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def probe_all(j, key, n, cap, total_cap):
+    ctx = j.ctx
+    out_p = ctx.alloc(total_cap * 4)
+    out_b = ctx.alloc(total_cap * 4)
+    tp = ctx.alloc(cap * 4)
+    tb = ctx.alloc(cap * 4)
+    cur = L.ProbeCursor(0, -1, 0, 0)
+    ps, bs = [], []
+    calls = 0
+    while not cur.done:
+        k, cur = j.probe(key, n, tp, tb, cap, cur)
+        assert k <= cap
+        ps.append(tp.to_host(np.uint32, k))
+        bs.append(tb.to_host(np.uint32, k))
+        calls += 1
+        assert calls < 10_000
+    return np.concatenate(ps), np.concatenate(bs), calls
+
+
+def pairs_set(p, b):
+    return np.sort(p.astype(np.uint64) << np.uint64(32) | b.astype(np.uint64))
+
+
+@pytest.mark.parametrize("cap", [32768, 1000, 7])
+def test_probe_multiset_with_duplicates_and_nulls(ctx, cap):
+    """Duplicate build keys (chains), NULL keys on both sides, int32 probe
+    key against an int64 build key; the bounded probe resumes mid-chain."""
+    print(f"[join] duplicates/nulls cap={cap}", flush=True)
+    rng = np.random.default_rng(cap)
+    nb, npr = 20_000, 50_000
+    bk = rng.integers(0, 8000, nb).astype(np.int64)
+    bk[:50] = 77  # one long chain
+    bm = rng.random(nb) > 0.02
+    pk = rng.integers(0, 12_000, npr).astype(np.int32)
+    pk[::100] = 77
+    pm = rng.random(npr) > 0.02
+    j = HashJoin(ctx, [L.INT64, L.INT32], 0, inline=[1], build_hint=1000)
+    half = nb // 2
+    for lo, hi in ((0, half), (half, nb)):  # two build batches, one host one device
+        m = bitmap_from_mask(bm[lo:hi])
+        if lo == 0:
+            j.build([Col(bk[lo:hi], L.INT64, m), Col(np.arange(lo, hi, dtype=np.int32), L.INT32)], hi - lo, device=False)
+        else:
+            j.build([Col(ctx.to_device(bk[lo:hi]), L.INT64, ctx.to_device(m)),
+                     Col(ctx.to_device(np.arange(lo, hi, dtype=np.int32)), L.INT32)], hi - lo)
+    j.finish()
+    assert j.rows() == nb
+    key = Col(ctx.to_device(pk), L.INT32, ctx.to_device(bitmap_from_mask(pm)))
+    want_p, want_b = O.hash_join((bk, O.INT64, bitmap_from_mask(bm)), (pk, O.INT32, bitmap_from_mask(pm)),
+                                 cap=npr * 60)
+    p, b, calls = probe_all(j, key, npr, cap, len(want_p) + 1)
+    np.testing.assert_array_equal(pairs_set(p, b), pairs_set(want_p, want_b))
+    if cap == 7:
+        assert calls >= len(want_p) // 7
+
+
+def test_config3_shape_join_multiset_and_digest(ctx):
+    """Unique permutation keys, 95 % matching foreign keys: pair multiset,
+    device digest == oracle digest, matched count closed form."""
+    cfg = W.Cfg3(n_build=200_000, n_probe=1_000_000)
+    print(f"[join] config3 shape {cfg}", flush=True)
+    bcols = W.gen_config3_build(ctx, cfg)
+    pcols = W.gen_config3_probe(ctx, cfg)
+    j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    j.build(bcols, cfg.n_build)
+    j.finish()
+    bk = bcols[0].data.to_host(np.int64)
+    assert len(np.unique(bk)) == cfg.n_build  # a permutation: unique keys
+    fk = pcols[0].data.to_host(np.int64)
+    qty = pcols[1].data.to_host(np.int32)
+    p1 = bcols[1].data.to_host(np.int32)
+    p2 = bcols[2].data.to_host(np.int64)
+    want_p, want_b = O.hash_join((bk, O.INT64), (fk, O.INT64), cap=cfg.n_probe)
+    p, b, _ = probe_all(j, pcols[0], cfg.n_probe, cfg.n_probe, cfg.n_probe)
+    assert len(p) == len(want_p)
+    np.testing.assert_array_equal(pairs_set(p, b), pairs_set(want_p, want_b))
+    # the matched count is the number of probe rows drawn from the build domain
+    assert abs(len(p) / cfg.n_probe - 0.95) < 0.005
+    dp, db = ctx.to_device(p), ctx.to_device(b)
+    d_gpu = j.digest(pcols, dp, db, len(p), [0, 1], [1, 2])
+    rows = np.stack([fk[want_p], qty[want_p].astype(np.int64), p1[want_b].astype(np.int64), p2[want_b]], axis=1)
+    assert d_gpu == O.digest_rows(rows)
+    # gathers
+    out = ctx.alloc(len(b) * 8)
+    j.gather_build(2, db, len(b), out)
+    np.testing.assert_array_equal(out.to_host(np.int64, len(b)), p2[b])
+
+
+@pytest.mark.parametrize("lds", [1024, 0])
+def test_config3_fused_probe_aggregate(ctx, lds):
+    """JOIN fk = key GROUP BY p1 SUM(qty * p2) (DECIMAL product -> Int128),
+    COUNT(*), MIN(qty), AVG(p2): fused probe vs oracle over the pair list."""
+    cfg = W.Cfg3(n_build=300_000, n_probe=2_000_000, seed=17)
+    print(f"[join] fused probe-agg lds={lds} {cfg}", flush=True)
+    bcols = W.gen_config3_build(ctx, cfg)
+    pcols = W.gen_config3_probe(ctx, cfg)
+    j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    j.build(bcols, cfg.n_build)
+    j.finish()
+    types = W.CFG3_PROBE_TYPES + W.CFG3_BUILD_TYPES  # fk, qty, key, p1, p2
+    scales = [0, 0, 0, 0, 2]
+    aggs = [AggSpec(L.SUM, 1, 4), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 1), AggSpec(L.AVG, 4)]
+    agg = HashAggregate(ctx, types, [3], aggs, scales=scales, group_hint=64)
+    agg.set_tuning(lds, 512)
+    j.probe_agg(pcols, 0, cfg.n_probe, agg)
+    agg.finalize()
+    g = gpu_groups(agg)
+    # oracle: join pairs, then aggregate the joined rows; the product is a
+    # DECIMAL128 column formed exactly in Python integers
+    fk = pcols[0].data.to_host(np.int64)
+    qty = pcols[1].data.to_host(np.int32)
+    bk = bcols[0].data.to_host(np.int64)
+    p1 = bcols[1].data.to_host(np.int32)
+    p2 = bcols[2].data.to_host(np.int64)
+    pp, bb = O.hash_join((bk, O.INT64), (fk, O.INT64), cap=cfg.n_probe)
+    prod = qty[pp].astype(np.int64) * p2[bb]  # < 50 * 10^12: exact in int64
+    prod128 = np.empty(2 * len(prod), np.uint64)
+    prod128[0::2] = np.where(prod < 0, np.uint64(2**64 - 1), np.uint64(0))
+    prod128[1::2] = prod.view(np.uint64)
+    jcols = [(fk[pp], O.INT64), (qty[pp], O.INT32), (bk[bb], O.INT64), (p1[bb], O.INT32), (p2[bb], O.DECIMAL, None, 2),
+             (prod128, O.DECIMAL128, None, 2)]
+    ref = oracle_groups(O.hash_agg(jcols, [3], [(O.SUM, 5), (O.COUNT, -1), (O.MIN, 1), (O.AVG, 4)]))
+    assert g["n"] == ref["n"] == 25
+    types_o = types + [O.DECIMAL128]
+    compare(g, ref, [(O.SUM, 5), (O.COUNT, -1), (O.MIN, 1), (O.AVG, 4)], types_o, scales + [2])
+
+
+def test_fused_probe_aggregate_many_to_many_with_overflow(ctx):
+    """Duplicate build keys, high-cardinality groups from a tiny table hint:
+    the fused kernel claims every group of a probe row before applying any,
+    so replays after a grow stay exact."""
+    rng = np.random.default_rng(21)
+    nb, npr = 30_000, 200_000
+    bk = rng.integers(0, 10_000, nb).astype(np.int64)
+    bv = rng.integers(-10**9, 10**9, nb).astype(np.int64)
+    pk = rng.integers(0, 12_000, npr).astype(np.int64)
+    pv = rng.integers(0, 100, npr).astype(np.int32)
+    j = HashJoin(ctx, [L.INT64, L.INT64], 0)
+    j.build([Col(ctx.to_device(bk), L.INT64), Col(ctx.to_device(bv), L.INT64)], nb)
+    j.finish()
+    types = [L.INT64, L.INT32, L.INT64, L.INT64]
+    aggs = [AggSpec(L.SUM, 3), AggSpec(L.COUNT, -1), AggSpec(L.MAX, 1)]
+    agg = HashAggregate(ctx, types, [3], aggs, group_hint=1)  # GROUP BY the build value
+    j.probe_agg([Col(ctx.to_device(pk), L.INT64), Col(ctx.to_device(pv), L.INT32)], 0, npr, agg)
+    agg.finalize()
+    pp, bb = O.hash_join((bk, O.INT64), (pk, O.INT64), cap=npr * 20)
+    jcols = [(pk[pp], O.INT64), (pv[pp], O.INT32), (bk[bb], O.INT64), (bv[bb], O.INT64)]
+    ref = oracle_groups(O.hash_agg(jcols, [3], [(O.SUM, 3), (O.COUNT, -1), (O.MAX, 1)]))
+    compare(gpu_groups(agg), ref, [(O.SUM, 3), (O.COUNT, -1), (O.MAX, 1)], types)

@@ -1,0 +1,105 @@
+// wg_agg_host.hpp — host-side state of a wg_agg handle (shared by the
+// aggregate, the fused join probe and the batched runner).
+#pragma once
+#include <vector>
+
+#include "wg_agg_dev.hpp"
+
+namespace wg {
+
+// Finalization recipe of one aggregate (which words decide NULL and value).
+struct AggFinal {
+  int32_t func;
+  int32_t count_word;
+  int32_t val_kind;  // OpKind or 0 (always NULL: MIN/MAX DECIMAL, SUM/AVG IPv4)
+  int32_t val_word;
+  int32_t is_float_in;
+  int32_t sumd_index;
+  int32_t scale;
+  int32_t limbs;
+  int32_t in_type;
+};
+
+// Staging of host batches (WG_MEM_HOST): rows are gathered through the
+// selection vector into pinned buffers and shipped in large device batches.
+struct HostStage {
+  uint64_t cap_rows = 0;
+  uint64_t rows = 0;
+  std::vector<void*> pinned;     // per column
+  std::vector<uint64_t*> pvalid;  // per nullable column (bitmap words)
+  std::vector<void*> dev;         // per column
+  std::vector<uint64_t*> dvalid;
+  hipEvent_t copied = nullptr;    // last H2D of the stage finished
+};
+
+}  // namespace wg
+
+struct wg_agg {
+  wg_ctx* ctx = nullptr;
+  // schema
+  int32_t n_cols = 0;
+  int32_t col_types[wg::kMaxCols] = {};
+  int32_t col_scales[wg::kMaxCols] = {};
+  uint8_t col_nullable[wg::kMaxCols] = {};
+  // compiled program
+  wg::KeyDesc key{};
+  std::vector<wg_agg_spec> aggs;
+  std::vector<wg::AggFinal> finals;
+  wg::AggProgram pg{};
+  int32_t n_words = 0;  // used words incl. key
+  // exact Int128 bases of the SUMD limb groups
+  int32_t n_sumd = 0;
+  std::vector<int32_t> sumd_word, sumd_limbs;
+  int32_t* d_sumd_word = nullptr;
+  int32_t* d_sumd_limbs = nullptr;
+  uint64_t* base = nullptr;  // (cap + side) * n_sumd * 2 words, lazily allocated
+  bool base_valid = false;
+  uint64_t rows_since_norm = 0;
+  // global table
+  uint64_t* slots = nullptr;
+  uint64_t cap = 0;
+  uint64_t groups = 0;  // as of the last read-back
+  uint64_t* counters = nullptr;  // device CTR_N words
+  uint32_t* ovf_rows = nullptr;
+  uint64_t ovf_rows_cap = 0;
+  uint64_t* ovf_part = nullptr;
+  uint64_t ovf_part_cap = 0;
+  bool dirty = false;  // table not in the freshly-filled state
+  // tuning
+  int32_t lds_slots = 1024;
+  int32_t block = 512;
+  int32_t grid = 0;
+  uint64_t tile_rows = 0;
+  // statistics of the last consume
+  uint64_t st_lds = 0, st_glb = 0, st_flushed = 0;
+  float st_kernel_ms = 0.f;  // consume-kernel time of the last consume (HIP events)
+  uint32_t st_launches = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // finalize output
+  bool finalized = false;
+  uint64_t out_n = 0;
+  uint64_t* dense = nullptr;
+  uint64_t* dense_base = nullptr;
+  uint8_t* dense_null = nullptr;
+  uint64_t dense_cap = 0;
+  // host staging
+  wg::HostStage stage;
+};
+
+namespace wg {
+// Internal entry points shared with the join and batch runners.
+wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n,
+                             const PredDesc& pred);
+wg_status agg_ensure_capacity(wg_agg* h, uint64_t extra_groups);
+wg_status agg_read_counters(wg_agg* h, uint64_t* c);
+wg_status agg_prepare_rows(wg_agg* h, uint64_t n);
+wg_status agg_flush_stage(wg_agg* h);
+uint64_t agg_footprint(const wg_agg* h);
+int grid_for(uint64_t n, int block, int max_blocks = 4096);
+wg_status zero_dev(wg_ctx* ctx, uint64_t* p, uint64_t n);
+GlobalTable table_of(wg_agg* h);
+wg_status grow_to(wg_agg* h, uint64_t new_cap);
+wg_status normalize(wg_agg* h);
+wg_status ensure_overflow_buffers(wg_agg* h);
+wg_status ensure_base(wg_agg* h);
+}  // namespace wg

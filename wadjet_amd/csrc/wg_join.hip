@@ -1,0 +1,935 @@
+// wg_join.hip — inner hash join on one int-class key: build, bounded
+// materializing probe, fused probe -> grouped aggregate, gathers and the
+// order-free output digest.
+//
+// Reference: HashJoin.Build flat path (internal/engine/exec/join.go:771-1162,
+// arenaAppendInt :551-560), inlineIntProbe / resumeIntChain (:2670-2836),
+// MaxProbeOutputRows (:2044-2060), gatherBuildVector (:4024), gatherVector
+// (sort.go:611).
+//
+// Table slot (power-of-two words):
+//   word 0   key (EMPTY sentinel = free)
+//   word 1   low 32 bits: head build row of the key's chain (-1 = none)
+//            high 32 bits: number of build rows with the key
+//   word 2.. inline payload: build columns copied into the slot so a probe of
+//            a unique key reads key and payload from one cache line
+// Chains live in next[build_row] (newest first, like arenaNext). A slot's
+// inline payload is the payload of its first build row; probes use it only
+// when the key has exactly one build row and walk the chain otherwise.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "wg_agg_dev.hpp"
+#include "wg_agg_host.hpp"
+
+namespace wg {
+
+constexpr int kMaxInline = 4;
+
+struct JoinTable {
+  uint64_t* slots;  // cap * stride words + 1 side slot (key == kEmpty)
+  uint64_t mask;
+  int32_t stride;
+  const int32_t* next;
+};
+
+__device__ __forceinline__ int32_t slot_head(const uint64_t* s) { return (int32_t)(uint32_t)s[1]; }
+__device__ __forceinline__ uint32_t slot_count(const uint64_t* s) { return (uint32_t)(s[1] >> 32); }
+
+// The slot of `key`, or null when the key has no build row.
+__device__ __forceinline__ const uint64_t* join_find(const JoinTable& J, uint64_t key) {
+  if (key == kEmpty) {
+    const uint64_t* s = J.slots + (J.mask + 1) * (uint64_t)J.stride;
+    return slot_count(s) ? s : nullptr;
+  }
+  uint64_t idx = mix64(key) & J.mask;
+  for (uint64_t p = 0; p <= J.mask; ++p) {
+    const uint64_t* s = J.slots + idx * (uint64_t)J.stride;
+    const uint64_t k = s[0];
+    if (k == key) return s;
+    if (k == kEmpty) return nullptr;
+    idx = (idx + 1) & J.mask;
+  }
+  return nullptr;
+}
+
+__global__ void join_fill_kernel(uint64_t* slots, uint64_t n_words, int stride) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += (uint64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(i % stride);
+    slots[i] = w == 0 ? kEmpty : (w == 1 ? 0x00000000FFFFFFFFull : 0ull);
+  }
+}
+
+struct BuildParams {
+  ColDesc key;       // retained build key column (global build row index)
+  ColDesc inl[kMaxInline];
+  int32_t n_inline;
+  uint64_t row0, n;  // build rows [row0, row0 + n)
+  uint64_t* slots;
+  uint64_t mask;
+  int32_t stride;
+  int32_t* next;
+  uint64_t* claimed;  // slots claimed (device counter)
+};
+
+// HashJoin.Build inner loop (join.go:1051-1065): NULL keys are skipped; each
+// row becomes the new head of its key's chain (arenaAppendInt, join.go:551-560).
+__global__ void join_build_kernel(BuildParams B) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < B.n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t row = B.row0 + j;
+    if (!col_valid(B.key, row)) {
+      B.next[row] = -1;
+      continue;
+    }
+    const uint64_t key = (uint64_t)load_int(B.key, row);
+    uint64_t* s = nullptr;
+    if (key == kEmpty) {
+      s = B.slots + (B.mask + 1) * (uint64_t)B.stride;
+    } else {
+      uint64_t idx = mix64(key) & B.mask;
+      for (uint64_t p = 0; p <= B.mask; ++p) {
+        uint64_t* c = B.slots + idx * (uint64_t)B.stride;
+        uint64_t k = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == kEmpty) {
+          const uint64_t prev = at_cas<GlbT>(c, kEmpty, key);
+          if (prev == kEmpty) at_add<GlbT>(B.claimed, 1);
+          k = prev == kEmpty ? key : prev;
+        }
+        if (k == key) {
+          s = c;
+          break;
+        }
+        idx = (idx + 1) & B.mask;
+      }
+    }
+    if (s == nullptr) continue;  // table full: the host sizes it so this never happens
+    int32_t* head = (int32_t*)(s + 1);
+    uint32_t* cnt = ((uint32_t*)(s + 1)) + 1;
+    const int32_t old = __hip_atomic_exchange(head, (int32_t)row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    B.next[row] = old;
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == -1) {
+      for (int i = 0; i < B.n_inline; ++i) s[2 + i] = (uint64_t)load_int(B.inl[i], row);
+    }
+  }
+}
+
+// Move a join table into a larger one (chains are per build row and stay).
+__global__ void join_rehash_kernel(const uint64_t* src, uint64_t src_cap, uint64_t* dst, uint64_t dst_mask, int stride) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < src_cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t* s = src + i * stride;
+    if (s[0] == kEmpty) continue;
+    uint64_t idx = mix64(s[0]) & dst_mask;
+    for (;;) {
+      uint64_t* d = dst + idx * stride;
+      if (at_cas<GlbT>(d, kEmpty, s[0]) == kEmpty) {
+        for (int w = 1; w < stride; ++w) d[w] = s[w];
+        break;
+      }
+      idx = (idx + 1) & dst_mask;
+    }
+  }
+}
+
+// ------------------------------------------------------- materializing probe
+
+__device__ __forceinline__ const uint64_t* probe_slot(const JoinTable& J, const ColDesc& pk, uint64_t row) {
+  if (!col_valid(pk, row)) return nullptr;  // NULL keys never match
+  return join_find(J, (uint64_t)load_int(pk, row));
+}
+
+__global__ void join_count_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, uint64_t pos0, uint64_t n,
+                                  uint64_t* cnt) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = pos0 + j;
+    const uint64_t row = sel ? sel[i] : i;
+    const uint64_t* s = probe_slot(J, pk, row);
+    cnt[j] = s ? slot_count(s) : 0;
+  }
+}
+
+// First position whose pairs do not fit in `cap` (atomicMin).
+__global__ void join_cut_kernel(const uint64_t* off, const uint64_t* cnt, uint64_t n, uint64_t cap, uint64_t* cut) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
+    if (off[j] + cnt[j] > cap) atomicMin((unsigned long long*)cut, (unsigned long long)j);
+}
+
+__global__ void join_emit_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, uint64_t pos0, uint64_t n,
+                                 const uint64_t* off, uint32_t* out_p, uint32_t* out_b) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = pos0 + j;
+    const uint64_t row = sel ? sel[i] : i;
+    const uint64_t* s = probe_slot(J, pk, row);
+    if (!s) continue;
+    uint64_t o = off[j];
+    for (int32_t b = slot_head(s); b >= 0; b = J.next[b]) {
+      out_p[o] = (uint32_t)row;
+      out_b[o] = (uint32_t)b;
+      ++o;
+    }
+  }
+}
+
+// One probe row's chain, from build row `start`, at most `limit` pairs;
+// res[0] = pairs written, res[1] = build row to resume at (-1 = chain done).
+__global__ void join_chain_kernel(const int32_t* next, int32_t start, uint32_t prow, uint64_t limit, uint32_t* out_p,
+                                  uint32_t* out_b, int64_t* res) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t k = 0;
+  int32_t b = start;
+  while (b >= 0 && k < limit) {
+    out_p[k] = prow;
+    out_b[k] = (uint32_t)b;
+    ++k;
+    b = next[b];
+  }
+  res[0] = (int64_t)k;
+  res[1] = b;
+}
+
+__global__ void join_head_kernel(JoinTable J, ColDesc pk, uint64_t row, int64_t* res) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const uint64_t* s = probe_slot(J, pk, row);
+  res[0] = s ? slot_head(s) : -1;
+}
+
+// ---------------------------------------------------------------- gathers
+
+__global__ void gather_kernel(ColDesc c, const uint32_t* rows, uint64_t n, void* out) {
+  const int w = type_bytes(c.type);
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = rows[j];
+    switch (w) {
+      case 2: ((uint16_t*)out)[j] = ((const uint16_t*)c.data)[r]; break;
+      case 4: ((uint32_t*)out)[j] = ((const uint32_t*)c.data)[r]; break;
+      case 16:
+        ((uint64_t*)out)[2 * j] = ((const uint64_t*)c.data)[2 * r];
+        ((uint64_t*)out)[2 * j + 1] = ((const uint64_t*)c.data)[2 * r + 1];
+        break;
+      default: ((uint64_t*)out)[j] = ((const uint64_t*)c.data)[r]; break;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- digest
+
+struct DigestParams {
+  ColDesc pc[8];
+  int32_t n_p;
+  ColDesc bc[8];
+  int32_t n_b;
+};
+
+__device__ __forceinline__ uint64_t raw64(const ColDesc& c, uint64_t row) {
+  if (c.type == WG_FLOAT64) return ((const uint64_t*)c.data)[row];
+  return (uint64_t)load_int(c, row);
+}
+
+// Same per-row mix as or_digest_rows (oracle/oracle.c): values in order
+// (probe columns, then build columns), two independent folds, summed mod 2^64.
+__global__ void join_digest_kernel(DigestParams D, const uint32_t* prow, const uint32_t* brow, uint64_t n,
+                                   unsigned long long* out) {
+  unsigned long long a = 0, b = 0;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t x = 0x243F6A8885A308D3ull, y = 0x13198A2E03707344ull;
+    for (int i = 0; i < D.n_p + D.n_b; ++i) {
+      const uint64_t v = i < D.n_p ? raw64(D.pc[i], prow[j]) : raw64(D.bc[i - D.n_p], brow[j]);
+      x = mix64(x ^ v);
+      y = mix64(y + v * 0x9E3779B97F4A7C15ull);
+    }
+    a += x;
+    b += y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o);
+    b += __shfl_down(b, o);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(out, a);
+    atomicAdd(out + 1, b);
+  }
+}
+
+// ------------------------------------------------- fused probe -> aggregate
+
+struct JoinAggParams {
+  JoinTable J;
+  ColDesc pkey;         // probe key column
+  ColSet cs;            // joined row: probe columns, then build columns
+  KeyDesc key;          // GROUP BY over joined-row columns
+  AggProgram pg;
+  GlobalTable T;
+  const uint32_t* sel;
+  uint64_t row_base;
+  uint64_t n;
+  uint64_t tile_rows;
+  uint64_t n_tiles;
+  uint32_t* ovf_rows;
+  uint64_t ovf_rows_cap;
+  uint64_t* ovf_part;
+  int32_t lds_slots;
+};
+
+// The group slot of a joined row: LDS first, then the global table; null
+// when the global table is over its budget (the probe row is replayed).
+template <bool USE_LDS>
+__device__ __forceinline__ uint64_t* group_slot(const JoinAggParams& P, uint64_t* lds, int L, const RowRef& r,
+                                                bool& in_lds) {
+  uint64_t key;
+  in_lds = false;
+  if (!build_key<true>(P.key, P.cs, r, key)) return P.T.slots + (P.T.mask + 2) * (uint64_t)P.pg.stride;
+  if (key == kEmpty) return P.T.slots + (P.T.mask + 1) * (uint64_t)P.pg.stride;
+  const uint64_t h = key_hash(key);
+  if (USE_LDS) {
+    uint64_t* ls = lds_find_or_claim(lds, L - 1, P.pg.stride, key, h);
+    if (ls) {
+      in_lds = true;
+      return ls;
+    }
+  }
+  return glb_find_or_claim(P.T, P.pg.stride, key, h);
+}
+
+template <bool USE_LDS>
+__global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
+  extern __shared__ uint64_t lds[];
+  __shared__ int64_t s_tile;
+  const int stride = P.pg.stride;
+  const int L = USE_LDS ? P.lds_slots : 0;
+  if (USE_LDS)
+    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = slot_init_word(P.pg, i % stride);
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int64_t t = -1;
+      const uint64_t o = __hip_atomic_load(P.T.counters + CTR_OVF_ROWS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (o + (uint64_t)gridDim.x * P.tile_rows > P.ovf_rows_cap) {
+        at_add<GlbT>(P.T.counters + CTR_STOPPED, 1);
+      } else {
+        const uint64_t tt = __hip_atomic_fetch_add(P.T.counters + CTR_TILE, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tt < P.n_tiles) t = (int64_t)tt;
+      }
+      s_tile = t;
+    }
+    __syncthreads();
+    const int64_t tile = s_tile;
+    __syncthreads();
+    if (tile < 0) break;
+    const uint64_t beg = (uint64_t)tile * P.tile_rows;
+    const uint64_t end = min(beg + P.tile_rows, P.n);
+    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+      const uint64_t prow = P.sel ? (uint64_t)P.sel[i] : P.row_base + i;
+      const uint64_t* js = probe_slot(P.J, P.pkey, prow);
+      if (!js) continue;
+      const uint32_t cnt = slot_count(js);
+      bool in_lds;
+      if (cnt == 1) {
+        const RowRef r{prow, (uint64_t)slot_head(js), js};
+        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds);
+        if (!g) {
+          const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          P.ovf_rows[at] = (uint32_t)prow;
+          continue;
+        }
+        if (in_lds) apply_row<LdsT, true>(g, P.pg, P.cs, r);
+        else apply_row<GlbT, true>(g, P.pg, P.cs, r);
+        continue;
+      }
+      // several build rows: claim every group first, apply only if all fit
+      bool ok = true;
+      for (int32_t b = slot_head(js); b >= 0 && ok; b = P.J.next[b]) {
+        const RowRef r{prow, (uint64_t)b, nullptr};
+        ok = group_slot<USE_LDS>(P, lds, L, r, in_lds) != nullptr;
+      }
+      if (!ok) {
+        const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        P.ovf_rows[at] = (uint32_t)prow;
+        continue;
+      }
+      for (int32_t b = slot_head(js); b >= 0; b = P.J.next[b]) {
+        const RowRef r{prow, (uint64_t)b, nullptr};
+        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds);
+        if (in_lds) apply_row<LdsT, true>(g, P.pg, P.cs, r);
+        else apply_row<GlbT, true>(g, P.pg, P.cs, r);
+      }
+    }
+  }
+  if (USE_LDS) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+      const uint64_t* ls = lds + (uint64_t)i * stride;
+      const uint64_t key = ls[0];
+      if (key == kEmpty) continue;
+      uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key));
+      if (slot != nullptr) {
+        merge_partial(slot, ls, P.pg);
+      } else {
+        const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t* rec = P.ovf_part + at * (uint64_t)stride;
+        for (int w = 0; w < stride; ++w) rec[w] = ls[w];
+      }
+    }
+  }
+}
+
+__global__ void agg_merge_part_kernel_j(const uint64_t* recs, uint64_t n, GlobalTable T, AggProgram pg) {
+  const int stride = pg.stride;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t* r = recs + i * stride;
+    uint64_t* d = glb_find_or_claim(T, stride, r[0], key_hash(r[0]));
+    if (d == nullptr) {
+      at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
+      continue;
+    }
+    merge_partial(d, r, pg);
+  }
+}
+
+}  // namespace wg
+
+// ================================================================ host side
+
+struct wg_join {
+  wg_ctx* ctx = nullptr;
+  int32_t n_cols = 0;
+  int32_t types[wg::kMaxCols] = {};
+  int32_t scales[wg::kMaxCols] = {};
+  int32_t key_col = 0;
+  int32_t n_inline = 0;
+  int32_t inline_cols[wg::kMaxInline] = {};
+  int32_t stride = 2;
+  // retained build columns (device), capacity in rows
+  void* col_data[wg::kMaxCols] = {};
+  uint64_t* col_valid[wg::kMaxCols] = {};
+  bool has_valid[wg::kMaxCols] = {};
+  uint64_t rows = 0, row_cap = 0;
+  int32_t* next = nullptr;
+  // table
+  uint64_t* slots = nullptr;
+  uint64_t cap = 0;
+  uint64_t* counter = nullptr;  // claimed slots + scratch words
+  bool finished = false;
+  // probe scratch
+  uint64_t* cnt = nullptr;
+  uint64_t* off = nullptr;
+  uint64_t scratch_rows = 0;
+  void* cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+};
+
+namespace wg {
+
+static wg_status join_alloc_table(wg_join* j, uint64_t cap) {
+  uint64_t* s = nullptr;
+  const uint64_t words = (cap + 1) * (uint64_t)j->stride;
+  if (hipMalloc(&s, words * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipMalloc for a %llu-slot join table failed", (unsigned long long)cap);
+    return WG_ERR_OOM;
+  }
+  hipLaunchKernelGGL(join_fill_kernel, dim3(grid_for(words, 256, 8192)), dim3(256), 0, j->ctx->stream, s, words, j->stride);
+  WG_HIP(hipGetLastError());
+  if (j->slots) {
+    hipLaunchKernelGGL(join_rehash_kernel, dim3(grid_for(j->cap, 256, 8192)), dim3(256), 0, j->ctx->stream,
+                       (const uint64_t*)j->slots, j->cap, s, cap - 1, j->stride);
+    WG_HIP(hipGetLastError());
+    // the side slot (key == sentinel) moves as is
+    WG_HIP(hipMemcpyAsync(s + cap * j->stride, j->slots + j->cap * j->stride, j->stride * 8, hipMemcpyDeviceToDevice,
+                          j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    WG_HIP(hipFree(j->slots));
+  }
+  j->slots = s;
+  j->cap = cap;
+  return WG_OK;
+}
+
+static wg_status join_reserve_rows(wg_join* j, uint64_t need) {
+  if (need <= j->row_cap) return WG_OK;
+  uint64_t cap = std::max<uint64_t>(need, j->row_cap * 2);
+  cap = std::max<uint64_t>(cap, 1 << 16);
+  for (int c = 0; c < j->n_cols; ++c) {
+    void* d = nullptr;
+    WG_HIP(hipMalloc(&d, cap * type_bytes(j->types[c])));
+    if (j->col_data[c]) {
+      WG_HIP(hipMemcpyAsync(d, j->col_data[c], j->rows * type_bytes(j->types[c]), hipMemcpyDeviceToDevice, j->ctx->stream));
+      WG_HIP(hipStreamSynchronize(j->ctx->stream));
+      WG_HIP(hipFree(j->col_data[c]));
+    }
+    j->col_data[c] = d;
+    uint64_t* v = nullptr;
+    WG_HIP(hipMalloc(&v, (cap + 63) / 64 * 8));
+    WG_HIP(hipMemsetAsync(v, 0xFF, (cap + 63) / 64 * 8, j->ctx->stream));
+    if (j->col_valid[c]) {
+      WG_HIP(hipMemcpyAsync(v, j->col_valid[c], (j->rows + 63) / 64 * 8, hipMemcpyDeviceToDevice, j->ctx->stream));
+      WG_HIP(hipStreamSynchronize(j->ctx->stream));
+      WG_HIP(hipFree(j->col_valid[c]));
+    }
+    j->col_valid[c] = v;
+  }
+  int32_t* nx = nullptr;
+  WG_HIP(hipMalloc(&nx, cap * 4));
+  if (j->next) {
+    WG_HIP(hipMemcpyAsync(nx, j->next, j->rows * 4, hipMemcpyDeviceToDevice, j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    WG_HIP(hipFree(j->next));
+  }
+  j->next = nx;
+  j->row_cap = cap;
+  return WG_OK;
+}
+
+// validity bits of appended rows land at row offset `at`: shift-copy on the host
+static wg_status append_validity(wg_join* j, int c, const uint64_t* src, uint64_t n, bool src_device) {
+  std::vector<uint64_t> words((n + 63) / 64);
+  if (src_device) {
+    WG_HIP(hipMemcpy(words.data(), src, words.size() * 8, hipMemcpyDeviceToHost));
+  } else {
+    memcpy(words.data(), src, words.size() * 8);
+  }
+  const uint64_t at = j->rows;
+  const uint64_t first = at / 64, last = (at + n + 63) / 64;
+  std::vector<uint64_t> dst(last - first);
+  WG_HIP(hipMemcpy(dst.data(), j->col_valid[c] + first, dst.size() * 8, hipMemcpyDeviceToHost));
+  for (uint64_t r = 0; r < n; ++r) {
+    const uint64_t g = at + r;
+    const bool v = (words[r >> 6] >> (r & 63)) & 1ull;
+    uint64_t& w = dst[(g >> 6) - first];
+    if (v) w |= 1ull << (g & 63);
+    else w &= ~(1ull << (g & 63));
+  }
+  WG_HIP(hipMemcpy(j->col_valid[c] + first, dst.data(), dst.size() * 8, hipMemcpyHostToDevice));
+  j->has_valid[c] = true;
+  return WG_OK;
+}
+
+static JoinTable jt_of(const wg_join* j) { return JoinTable{j->slots, j->cap - 1, j->stride, j->next}; }
+
+static ColDesc retained_col(const wg_join* j, int c) {
+  return ColDesc{j->col_data[c], j->has_valid[c] ? j->col_valid[c] : nullptr, j->types[c], j->scales[c]};
+}
+
+static wg_status ensure_probe_scratch(wg_join* j, uint64_t n) {
+  if (n <= j->scratch_rows) return WG_OK;
+  if (j->cnt) WG_HIP(hipFree(j->cnt));
+  if (j->off) WG_HIP(hipFree(j->off));
+  WG_HIP(hipMalloc(&j->cnt, n * 8));
+  WG_HIP(hipMalloc(&j->off, n * 8));
+  size_t bytes = 0;
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, j->cnt, j->off, (int)std::min<uint64_t>(n, 1u << 30), j->ctx->stream));
+  if (bytes > j->cub_bytes) {
+    if (j->cub_tmp) WG_HIP(hipFree(j->cub_tmp));
+    WG_HIP(hipMalloc(&j->cub_tmp, bytes));
+    j->cub_bytes = bytes;
+  }
+  j->scratch_rows = n;
+  return WG_OK;
+}
+
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" {
+
+wg_status wg_join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types, const int32_t* build_scales,
+                         int32_t build_key, int32_t n_inline, const int32_t* inline_cols, uint64_t build_hint,
+                         wg_join** out) {
+  clear_error();
+  WG_CHECK(ctx && out && build_types, WG_ERR_INVALID, "null argument");
+  WG_CHECK(n_build_cols > 0 && n_build_cols <= kMaxCols, WG_ERR_UNSUPPORTED, "%d build columns", n_build_cols);
+  WG_CHECK(build_key >= 0 && build_key < n_build_cols, WG_ERR_INVALID, "build key column %d", build_key);
+  WG_CHECK(is_int_class(build_types[build_key]), WG_ERR_UNSUPPORTED, "join key of type %d not on the GPU path",
+           build_types[build_key]);
+  WG_CHECK(n_inline >= 0 && n_inline <= kMaxInline, WG_ERR_INVALID, "%d inline columns (max %d)", n_inline, kMaxInline);
+  wg_join* j = new wg_join();
+  j->ctx = ctx;
+  j->n_cols = n_build_cols;
+  for (int c = 0; c < n_build_cols; ++c) {
+    WG_CHECK(build_types[c] >= WG_INT32 && build_types[c] <= WG_UINT16, WG_ERR_UNSUPPORTED, "build column type %d", build_types[c]);
+    j->types[c] = build_types[c];
+    j->scales[c] = build_scales ? build_scales[c] : 0;
+  }
+  j->key_col = build_key;
+  j->n_inline = n_inline;
+  for (int i = 0; i < n_inline; ++i) {
+    const int c = inline_cols[i];
+    WG_CHECK(c >= 0 && c < n_build_cols && j->types[c] != WG_DECIMAL128, WG_ERR_INVALID, "inline column %d", c);
+    j->inline_cols[i] = c;
+  }
+  j->stride = (int32_t)next_pow2(2 + n_inline);
+  WG_HIP(hipMalloc(&j->counter, 64));
+  WG_TRY(zero_dev(ctx, j->counter, 8));
+  wg_status st = join_alloc_table(j, next_pow2(std::max<uint64_t>(build_hint, 1024) * 2));
+  if (st != WG_OK) {
+    delete j;
+    return st;
+  }
+  WG_HIP(hipStreamSynchronize(ctx->stream));
+  *out = j;
+  return WG_OK;
+}
+
+wg_status wg_join_destroy(wg_join* j) {
+  if (!j) return WG_OK;
+  (void)hipStreamSynchronize(j->ctx->stream);
+  for (int c = 0; c < j->n_cols; ++c) {
+    (void)hipFree(j->col_data[c]);
+    (void)hipFree(j->col_valid[c]);
+  }
+  (void)hipFree(j->next);
+  (void)hipFree(j->slots);
+  (void)hipFree(j->counter);
+  (void)hipFree(j->cnt);
+  (void)hipFree(j->off);
+  (void)hipFree(j->cub_tmp);
+  delete j;
+  return WG_OK;
+}
+
+wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_rows, int32_t mem) {
+  clear_error();
+  WG_CHECK(j && cols, WG_ERR_INVALID, "null argument");
+  WG_CHECK(!j->finished, WG_ERR_STATE, "build_append after build_finish");
+  if (n_rows == 0) return WG_OK;
+  WG_CHECK(j->rows + n_rows < (1ull << 31), WG_ERR_UNSUPPORTED, "build sides of 2^31 rows or more");
+  for (int c = 0; c < j->n_cols; ++c)
+    WG_CHECK(cols[c].type == j->types[c] && cols[c].data, WG_ERR_INVALID, "build column %d: type %d, expected %d", c,
+             cols[c].type, j->types[c]);
+  WG_TRY(join_reserve_rows(j, j->rows + n_rows));
+  // keep the table at <= 50 % load
+  if ((j->rows + n_rows) * 2 > j->cap) WG_TRY(join_alloc_table(j, next_pow2((j->rows + n_rows) * 2)));
+  const hipMemcpyKind kind = mem == WG_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  for (int c = 0; c < j->n_cols; ++c) {
+    const int w = type_bytes(j->types[c]);
+    WG_HIP(hipMemcpyAsync((char*)j->col_data[c] + j->rows * w, cols[c].data, n_rows * w, kind, j->ctx->stream));
+    if (cols[c].validity) WG_TRY(append_validity(j, c, cols[c].validity, n_rows, mem != WG_MEM_HOST));
+  }
+  BuildParams B{};
+  B.key = retained_col(j, j->key_col);
+  for (int i = 0; i < j->n_inline; ++i) B.inl[i] = retained_col(j, j->inline_cols[i]);
+  B.n_inline = j->n_inline;
+  B.row0 = j->rows;
+  B.n = n_rows;
+  B.slots = j->slots;
+  B.mask = j->cap - 1;
+  B.stride = j->stride;
+  B.next = j->next;
+  B.claimed = j->counter;
+  hipLaunchKernelGGL(join_build_kernel, dim3(grid_for(n_rows, 256, 16384)), dim3(256), 0, j->ctx->stream, B);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  j->rows += n_rows;
+  return WG_OK;
+}
+
+wg_status wg_join_build_finish(wg_join* j) {
+  clear_error();
+  WG_CHECK(j, WG_ERR_INVALID, "null handle");
+  // build columns that carry no NULLs keep their all-valid bitmap unused
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  j->finished = true;
+  return WG_OK;
+}
+
+wg_status wg_join_build_rows(wg_join* j, uint64_t* rows) {
+  WG_CHECK(j && rows, WG_ERR_INVALID, "null argument");
+  *rows = j->rows;
+  return WG_OK;
+}
+
+wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
+                        uint32_t* out_probe_rows, uint32_t* out_build_rows, uint64_t out_cap, wg_probe_cursor* cursor,
+                        uint64_t* n_out) {
+  clear_error();
+  WG_CHECK(j && probe_key && cursor && n_out && out_probe_rows && out_build_rows, WG_ERR_INVALID, "null argument");
+  WG_CHECK(j->finished, WG_ERR_STATE, "probe before build_finish");
+  WG_CHECK(is_int_class(probe_key->type), WG_ERR_UNSUPPORTED, "probe key type %d", probe_key->type);
+  WG_CHECK(out_cap > 0, WG_ERR_INVALID, "out_cap must be positive");
+  const uint64_t n = sel ? n_sel : n_rows;
+  *n_out = 0;
+  if (cursor->done || cursor->pos >= n) {
+    cursor->done = 1;
+    return WG_OK;
+  }
+  WG_CHECK(n < (1ull << 31), WG_ERR_INVALID, "probe batches of 2^31 rows or more must be split");
+  const JoinTable J = jt_of(j);
+  const ColDesc pk{probe_key->data, probe_key->validity, probe_key->type, probe_key->scale};
+  int64_t* res = (int64_t*)(j->counter + 4);
+  uint64_t written = 0;
+  // resume a suspended chain first (resumeIntChain, join.go:2819-2836)
+  if (cursor->chain >= 0) {
+    const uint64_t row = sel ? 0 : cursor->pos;
+    uint32_t prow = (uint32_t)row;
+    if (sel) WG_HIP(hipMemcpy(&prow, sel + cursor->pos, 4, hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(join_chain_kernel, dim3(1), dim3(64), 0, j->ctx->stream, (const int32_t*)j->next,
+                       (int32_t)cursor->chain, prow, out_cap, out_probe_rows, out_build_rows, res);
+    WG_HIP(hipGetLastError());
+    int64_t r[2];
+    WG_HIP(hipMemcpyAsync(r, res, 16, hipMemcpyDeviceToHost, j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    written = (uint64_t)r[0];
+    cursor->chain = r[1];
+    if (cursor->chain >= 0) {
+      *n_out = written;
+      return WG_OK;
+    }
+    cursor->pos += 1;
+    if (cursor->pos >= n) {
+      cursor->done = 1;
+      *n_out = written;
+      return WG_OK;
+    }
+  }
+  const uint64_t pos = cursor->pos;
+  const uint64_t m = n - pos;
+  const uint64_t room = out_cap - written;
+  WG_TRY(ensure_probe_scratch(j, m));
+  const int g = grid_for(m, 256, 16384);
+  hipLaunchKernelGGL(join_count_kernel, dim3(g), dim3(256), 0, j->ctx->stream, J, pk, sel, pos, m, j->cnt);
+  WG_HIP(hipGetLastError());
+  size_t bytes = j->cub_bytes;
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(j->cub_tmp, bytes, j->cnt, j->off, (int)m, j->ctx->stream));
+  uint64_t* cut_d = j->counter + 6;
+  const uint64_t big = ~0ull;
+  WG_HIP(hipMemcpyAsync(cut_d, &big, 8, hipMemcpyHostToDevice, j->ctx->stream));
+  hipLaunchKernelGGL(join_cut_kernel, dim3(g), dim3(256), 0, j->ctx->stream, (const uint64_t*)j->off,
+                     (const uint64_t*)j->cnt, m, room, cut_d);
+  WG_HIP(hipGetLastError());
+  uint64_t cut = 0;
+  WG_HIP(hipMemcpyAsync(&cut, cut_d, 8, hipMemcpyDeviceToHost, j->ctx->stream));
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  const uint64_t fit = cut == ~0ull ? m : cut;
+  uint64_t total = 0;
+  if (fit > 0) {
+    uint64_t lastoff = 0, lastcnt = 0;
+    WG_HIP(hipMemcpy(&lastoff, j->off + fit - 1, 8, hipMemcpyDeviceToHost));
+    WG_HIP(hipMemcpy(&lastcnt, j->cnt + fit - 1, 8, hipMemcpyDeviceToHost));
+    total = lastoff + lastcnt;
+    hipLaunchKernelGGL(join_emit_kernel, dim3(grid_for(fit, 256, 16384)), dim3(256), 0, j->ctx->stream, J, pk, sel, pos,
+                       fit, (const uint64_t*)j->off, out_probe_rows + written, out_build_rows + written);
+    WG_HIP(hipGetLastError());
+  }
+  written += total;
+  if (fit == m) {
+    cursor->pos = n;
+    cursor->done = 1;
+  } else if (fit > 0) {
+    cursor->pos = pos + fit;
+  } else {
+    // the first remaining row alone exceeds the room: split its chain
+    uint32_t prow = (uint32_t)pos;
+    if (sel) WG_HIP(hipMemcpy(&prow, sel + pos, 4, hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(join_head_kernel, dim3(1), dim3(64), 0, j->ctx->stream, J, pk, (uint64_t)prow, res);
+    int64_t head = -1;
+    WG_HIP(hipMemcpyAsync(&head, res, 8, hipMemcpyDeviceToHost, j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    hipLaunchKernelGGL(join_chain_kernel, dim3(1), dim3(64), 0, j->ctx->stream, (const int32_t*)j->next, (int32_t)head,
+                       prow, room, out_probe_rows + written, out_build_rows + written, res);
+    int64_t r[2];
+    WG_HIP(hipMemcpyAsync(r, res, 16, hipMemcpyDeviceToHost, j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    written += (uint64_t)r[0];
+    cursor->pos = pos;
+    cursor->chain = r[1];
+    if (r[1] < 0) cursor->pos = pos + 1;
+  }
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  *n_out = written;
+  return WG_OK;
+}
+
+wg_status wg_join_gather_build(wg_join* j, int32_t build_col, const uint32_t* build_rows, uint64_t n, void* out) {
+  clear_error();
+  WG_CHECK(j && (n == 0 || (build_rows && out)), WG_ERR_INVALID, "null argument");
+  WG_CHECK(build_col >= 0 && build_col < j->n_cols, WG_ERR_INVALID, "build column %d", build_col);
+  if (n == 0) return WG_OK;
+  hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, j->ctx->stream, retained_col(j, build_col),
+                     build_rows, n, out);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  return WG_OK;
+}
+
+wg_status wg_gather(wg_ctx* ctx, const wg_column* col, const uint32_t* rows, uint64_t n, void* out) {
+  clear_error();
+  WG_CHECK(ctx && col && (n == 0 || (rows && out)), WG_ERR_INVALID, "null argument");
+  if (n == 0) return WG_OK;
+  ColDesc c{col->data, col->validity, col->type, col->scale};
+  hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, c, rows, n, out);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipStreamSynchronize(ctx->stream));
+  return WG_OK;
+}
+
+wg_status wg_join_digest(wg_join* j, const wg_column* probe_cols, const uint32_t* probe_rows, const uint32_t* build_rows,
+                         uint64_t n, const int32_t* probe_sel_cols, int32_t n_probe_sel, const int32_t* build_sel_cols,
+                         int32_t n_build_sel, uint64_t digest_out[2]) {
+  clear_error();
+  WG_CHECK(j && digest_out, WG_ERR_INVALID, "null argument");
+  WG_CHECK(n_probe_sel >= 0 && n_probe_sel <= 8 && n_build_sel >= 0 && n_build_sel <= 8, WG_ERR_INVALID, "at most 8 + 8 columns");
+  DigestParams D{};
+  D.n_p = n_probe_sel;
+  D.n_b = n_build_sel;
+  for (int i = 0; i < n_probe_sel; ++i) {
+    const wg_column& c = probe_cols[probe_sel_cols[i]];
+    WG_CHECK(c.type != WG_DECIMAL128, WG_ERR_UNSUPPORTED, "digest of DECIMAL128");
+    D.pc[i] = ColDesc{c.data, c.validity, c.type, c.scale};
+  }
+  for (int i = 0; i < n_build_sel; ++i) {
+    const int c = build_sel_cols[i];
+    WG_CHECK(c >= 0 && c < j->n_cols && j->types[c] != WG_DECIMAL128, WG_ERR_INVALID, "digest build column %d", c);
+    D.bc[i] = retained_col(j, c);
+  }
+  unsigned long long* acc = (unsigned long long*)(j->counter + 2);
+  WG_TRY(zero_dev(j->ctx, (uint64_t*)acc, 2));
+  if (n) {
+    hipLaunchKernelGGL(join_digest_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, j->ctx->stream, D, probe_rows,
+                       build_rows, n, acc);
+    WG_HIP(hipGetLastError());
+  }
+  WG_HIP(hipMemcpyAsync(digest_out, acc, 16, hipMemcpyDeviceToHost, j->ctx->stream));
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  return WG_OK;
+}
+
+wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_probe_cols, int32_t probe_key,
+                            uint64_t n_rows, const uint32_t* sel, uint64_t n_sel, wg_agg* agg) {
+  clear_error();
+  WG_CHECK(j && agg && probe_cols, WG_ERR_INVALID, "null argument");
+  WG_CHECK(j->finished, WG_ERR_STATE, "probe before build_finish");
+  WG_CHECK(!agg->finalized, WG_ERR_STATE, "aggregate already finalized");
+  WG_CHECK(agg->ctx == j->ctx, WG_ERR_INVALID, "join and aggregate on different contexts");
+  WG_CHECK(n_probe_cols > 0 && probe_key >= 0 && probe_key < n_probe_cols, WG_ERR_INVALID, "probe key column");
+  WG_CHECK(agg->n_cols == n_probe_cols + j->n_cols && agg->n_cols <= kMaxCols, WG_ERR_INVALID,
+           "aggregate schema must be the %d probe columns followed by the %d build columns", n_probe_cols, j->n_cols);
+  for (int c = 0; c < n_probe_cols; ++c)
+    WG_CHECK(probe_cols[c].type == agg->col_types[c], WG_ERR_INVALID, "probe column %d type", c);
+  for (int c = 0; c < j->n_cols; ++c)
+    WG_CHECK(j->types[c] == agg->col_types[n_probe_cols + c], WG_ERR_INVALID, "build column %d type", c);
+  WG_CHECK(is_int_class(probe_cols[probe_key].type), WG_ERR_UNSUPPORTED, "probe key type");
+  if (agg->key.n > 1)
+    for (int k = 0; k < agg->key.n; ++k)
+      WG_CHECK(!agg->col_nullable[agg->key.col[k]], WG_ERR_UNSUPPORTED, "nullable multi-column key after a join");
+  const uint64_t n = sel ? n_sel : n_rows;
+  if (n == 0) return WG_OK;
+  WG_CHECK(n < (1ull << 31), WG_ERR_INVALID, "probe batches of 2^31 rows or more must be split");
+  WG_TRY(agg_flush_stage(agg));
+  if (agg->rows_since_norm + n >= (1ull << 31)) WG_TRY(normalize(agg));
+  WG_TRY(agg_ensure_capacity(agg, 0));
+  WG_TRY(ensure_overflow_buffers(agg));
+
+  JoinAggParams P{};
+  P.J = jt_of(j);
+  P.pkey = ColDesc{probe_cols[probe_key].data, probe_cols[probe_key].validity, probe_cols[probe_key].type,
+                   probe_cols[probe_key].scale};
+  P.cs.n_cols = agg->n_cols;
+  P.cs.n_probe = n_probe_cols;
+  for (int c = 0; c < n_probe_cols; ++c) {
+    P.cs.cols[c] = ColDesc{probe_cols[c].data, probe_cols[c].validity, probe_cols[c].type, probe_cols[c].scale};
+    P.cs.inline_word[c] = -1;
+  }
+  for (int c = 0; c < j->n_cols; ++c) {
+    P.cs.cols[n_probe_cols + c] = retained_col(j, c);
+    int w = -1;
+    for (int i = 0; i < j->n_inline; ++i)
+      if (j->inline_cols[i] == c && !j->has_valid[c]) w = 2 + i;
+    P.cs.inline_word[n_probe_cols + c] = w;
+  }
+  P.key = agg->key;
+  P.pg = agg->pg;
+  P.lds_slots = agg->lds_slots;
+  P.tile_rows = agg->tile_rows;
+  agg->st_kernel_ms = 0.f;
+  agg->st_launches = 0;
+  if (!agg->ev0) {
+    WG_HIP(hipEventCreate(&agg->ev0));
+    WG_HIP(hipEventCreate(&agg->ev1));
+  }
+  // work list: (sel, n) segments still to process; overflow rows become new segments
+  struct Seg { const uint32_t* sel; uint64_t n; uint32_t* owned; };
+  std::vector<Seg> work{{sel, n, nullptr}};
+  wg_status st = WG_OK;
+  for (int guard = 0; !work.empty() && st == WG_OK && guard < 64; ++guard) {
+    Seg seg = work.back();
+    work.pop_back();
+    uint64_t pos = 0;
+    while (pos < seg.n && st == WG_OK) {
+      P.T = table_of(agg);
+      P.ovf_rows = agg->ovf_rows;
+      P.ovf_rows_cap = agg->ovf_rows_cap;
+      P.ovf_part = agg->ovf_part;
+      P.sel = seg.sel ? seg.sel + pos : nullptr;
+      P.row_base = seg.sel ? 0 : pos;
+      P.n = seg.n - pos;
+      P.n_tiles = (P.n + P.tile_rows - 1) / P.tile_rows;
+      WG_TRY(zero_dev(agg->ctx, agg->counters + CTR_OVF_ROWS, CTR_N - 1));
+      const int grid = (int)std::min<uint64_t>((uint64_t)agg->grid, std::max<uint64_t>(P.n_tiles, 1));
+      WG_HIP(hipEventRecord(agg->ev0, agg->ctx->stream));
+      if (P.lds_slots > 0) {
+        static bool attr_done = false;
+        if (!attr_done) {
+          WG_HIP(hipFuncSetAttribute((const void*)join_agg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     150 * 1024));
+          attr_done = true;
+        }
+        hipLaunchKernelGGL(join_agg_kernel<true>, dim3(grid), dim3(agg->block),
+                           (size_t)P.lds_slots * P.pg.stride * 8, agg->ctx->stream, P);
+      } else {
+        hipLaunchKernelGGL(join_agg_kernel<false>, dim3(grid), dim3(agg->block), 0, agg->ctx->stream, P);
+      }
+      WG_HIP(hipGetLastError());
+      WG_HIP(hipEventRecord(agg->ev1, agg->ctx->stream));
+      agg->dirty = true;
+      uint64_t c[CTR_N];
+      WG_TRY(agg_read_counters(agg, c));
+      float ms = 0.f;
+      WG_HIP(hipEventElapsedTime(&ms, agg->ev0, agg->ev1));
+      agg->st_kernel_ms += ms;
+      agg->st_launches += 1;
+      agg->groups = c[CTR_GROUPS];
+      const uint64_t tiles = std::min<uint64_t>(c[CTR_TILE], P.n_tiles);
+      if (c[CTR_OVF_ROWS] || c[CTR_OVF_PART]) {
+        const uint64_t nr = c[CTR_OVF_ROWS], np = c[CTR_OVF_PART];
+        WG_TRY(grow_to(agg, next_pow2(std::max<uint64_t>((agg->groups + nr * 4 + np) * 2, agg->cap * 2))));
+        if (np) {
+          uint64_t* recs = nullptr;
+          const uint64_t bytes = np * (uint64_t)agg->pg.stride * 8;
+          WG_HIP(hipMalloc(&recs, bytes));
+          WG_HIP(hipMemcpyAsync(recs, agg->ovf_part, bytes, hipMemcpyDeviceToDevice, agg->ctx->stream));
+          GlobalTable T = table_of(agg);
+          T.max_groups = agg->cap;
+          hipLaunchKernelGGL(agg_merge_part_kernel_j, dim3(grid_for(np, 256)), dim3(256), 0, agg->ctx->stream, recs, np,
+                             T, agg->pg);
+          WG_HIP(hipGetLastError());
+          uint64_t c2[CTR_N];
+          WG_TRY(agg_read_counters(agg, c2));
+          WG_HIP(hipFree(recs));
+          WG_CHECK((c2[CTR_STOPPED] >> 32) == 0, WG_ERR_DEVICE, "overflow partials did not fit the grown table");
+          agg->groups = c2[CTR_GROUPS];
+        }
+        if (nr) {
+          uint32_t* rows = nullptr;
+          WG_HIP(hipMalloc(&rows, nr * 4));
+          WG_HIP(hipMemcpyAsync(rows, agg->ovf_rows, nr * 4, hipMemcpyDeviceToDevice, agg->ctx->stream));
+          WG_HIP(hipStreamSynchronize(agg->ctx->stream));
+          work.push_back(Seg{rows, nr, rows});
+        }
+      }
+      WG_CHECK(tiles > 0 || c[CTR_OVF_ROWS] > 0, WG_ERR_DEVICE, "join probe launch made no progress");
+      pos = std::min<uint64_t>(seg.n, pos + tiles * P.tile_rows);
+      if (pos < seg.n) WG_TRY(agg_ensure_capacity(agg, 0));
+    }
+    if (seg.owned) {
+      WG_HIP(hipStreamSynchronize(agg->ctx->stream));
+      WG_HIP(hipFree(seg.owned));
+    }
+  }
+  WG_CHECK(work.empty(), WG_ERR_DEVICE, "join aggregate overflow did not converge");
+  agg->rows_since_norm += n;
+  return st;
+}
+
+}  // extern "C"

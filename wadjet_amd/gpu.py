@@ -1,0 +1,335 @@
+"""Thin Python handles over the C-ABI: device buffers, the grouped aggregate,
+the filter, the hash join and the generators. Used by the tests and bench.py;
+numpy arrays are the host-side representation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import call
+
+NP_OF = {L.INT32: np.int32, L.INT64: np.int64, L.FLOAT64: np.float64, L.DECIMAL: np.int64, L.IPV4: np.int64,
+         L.UINT32: np.uint32, L.UINT16: np.uint16}
+
+
+class Context:
+    """One device + one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        call("wg_ctx_create", device, C.byref(h))
+        self.h = h
+        cus, hbm, lds = C.c_int32(), C.c_uint64(), C.c_int32()
+        call("wg_ctx_info", h, C.byref(cus), C.byref(hbm), C.byref(lds))
+        self.cus, self.hbm, self.lds = cus.value, hbm.value, lds.value
+
+    def close(self):
+        if self.h:
+            call("wg_ctx_destroy", self.h)
+            self.h = None
+
+    def sync(self):
+        call("wg_ctx_sync", self.h)
+
+    def timer_start(self):
+        call("wg_ctx_timer_start", self.h)
+
+    def timer_stop(self) -> float:
+        ms = C.c_float()
+        call("wg_ctx_timer_stop", self.h, C.byref(ms))
+        return ms.value
+
+    # -- memory
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, max(arr.nbytes, 1))
+        if arr.nbytes:
+            call("wg_memcpy_h2d", self.h, b.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes)
+        return b
+
+    def gen(self, kind: int, dtype, n: int, seed: int, i0: int = 0, i1: int = 0, f0: float = 0.0, f1: float = 0.0,
+            row0: int = 0, out: Optional["DeviceBuffer"] = None) -> "DeviceBuffer":
+        nbytes = n * np.dtype(dtype).itemsize
+        b = out if out is not None else DeviceBuffer(self, max(nbytes, 1))
+        call("wg_gen", self.h, kind, b.ptr, row0, n, seed, i0, i1, f0, f1)
+        return b
+
+
+class DeviceBuffer:
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        call("wg_malloc", ctx.h, nbytes, C.byref(p))
+        self.ptr = p
+
+    def to_host(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dtype=dt)
+        if out.nbytes:
+            call("wg_memcpy_d2h", self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            call("wg_free", self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                L.lib().wg_free(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+
+@dataclass
+class Col:
+    """A column handed to the library: device buffer or host numpy array."""
+    data: object  # DeviceBuffer | np.ndarray
+    type: int
+    validity: object = None  # DeviceBuffer | np.ndarray(uint64) | None
+    scale: int = 0
+
+    def c(self) -> L.Column:
+        def ptr(x):
+            if x is None:
+                return None
+            if isinstance(x, DeviceBuffer):
+                return x.ptr
+            return x.ctypes.data_as(C.c_void_p)
+        return L.Column(ptr(self.data), ptr(self.validity), self.type, self.scale)
+
+
+def columns(cols: Sequence[Col]):
+    arr = (L.Column * len(cols))(*[c.c() for c in cols])
+    return arr
+
+
+@dataclass
+class AggSpec:
+    func: int
+    col: int = -1
+    col2: int = -1
+
+
+@dataclass
+class Pred:
+    col: int
+    op: int
+    i64: int = 0
+    f64: float = 0.0
+
+
+def bitmap_from_mask(valid: np.ndarray) -> np.ndarray:
+    """batch.Bitmap words (bit i%64 of word i/64, 1 = valid) from a bool mask."""
+    n = len(valid)
+    words = np.zeros((n + 63) // 64, dtype=np.uint64)
+    idx = np.nonzero(valid)[0]
+    np.bitwise_or.at(words, idx >> 6, (np.uint64(1) << (idx & 63).astype(np.uint64)))
+    return words
+
+
+class HashAggregate:
+    """GROUP BY keys with SUM/COUNT/MIN/MAX/AVG on the GPU (wg_agg_*)."""
+
+    def __init__(self, ctx: Context, types: Sequence[int], keys: Sequence[int], aggs: Sequence[AggSpec],
+                 scales: Optional[Sequence[int]] = None, nullable: Optional[Sequence[bool]] = None,
+                 group_hint: int = 0):
+        self.ctx = ctx
+        self.types = list(types)
+        self.keys = list(keys)
+        self.aggs = list(aggs)
+        n = len(types)
+        t = (C.c_int32 * n)(*types)
+        s = (C.c_int32 * n)(*(scales or [0] * n))
+        nl = (C.c_uint8 * n)(*[1 if x else 0 for x in (nullable or [False] * n)])
+        k = (C.c_int32 * len(keys))(*keys)
+        a = (L.AggSpecC * max(len(aggs), 1))(*[L.AggSpecC(x.func, x.col, x.col2, 0) for x in aggs])
+        h = C.c_void_p()
+        call("wg_agg_create", ctx.h, n, t, s, nl, len(keys), k, len(aggs), a, group_hint, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            call("wg_agg_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.lib().wg_agg_destroy(self.h)
+        except Exception:
+            pass
+
+    def consume(self, cols: Sequence[Col], n_rows: int, sel=None, pred: Optional[Pred] = None, device: bool = True):
+        """sel: None, a host uint32 array, a DeviceBuffer, or (DeviceBuffer, count)."""
+        arr = columns(cols)
+        keep = [arr]
+        if sel is None:
+            sp, ns = None, 0
+        elif isinstance(sel, tuple):
+            sp, ns = sel[0].ptr, int(sel[1])
+        elif isinstance(sel, DeviceBuffer):
+            sp, ns = sel.ptr, sel.nbytes // 4
+        else:
+            sel = np.ascontiguousarray(sel, dtype=np.uint32)
+            keep.append(sel)
+            sp, ns = sel.ctypes.data_as(C.c_void_p), len(sel)
+        pc = L.PredicateC(pred.col, pred.op, pred.i64, pred.f64) if pred else None
+        call("wg_agg_consume", self.h, arr, n_rows, sp, ns, C.byref(pc) if pc else None,
+             L.MEM_DEVICE if device else L.MEM_HOST)
+
+    def merge(self, other: "HashAggregate"):
+        call("wg_agg_merge", self.h, other.h)
+
+    def finalize(self):
+        call("wg_agg_finalize", self.h)
+
+    def reset(self):
+        call("wg_agg_reset", self.h)
+
+    def set_tuning(self, lds_slots: int, block: int):
+        call("wg_agg_set_tuning", self.h, lds_slots, block)
+
+    def num_groups(self) -> int:
+        n = C.c_uint64()
+        call("wg_agg_num_groups", self.h, C.byref(n))
+        return n.value
+
+    def footprint(self) -> int:
+        n = C.c_uint64()
+        call("wg_agg_footprint", self.h, C.byref(n))
+        return n.value
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        call("wg_agg_stats", self.h, *[C.byref(x) for x in v])
+        return dict(lds_rows=v[0].value, global_rows=v[1].value, flushed=v[2].value, capacity=v[3].value)
+
+    def kernel_time(self):
+        ms, k = C.c_float(), C.c_uint32()
+        call("wg_agg_kernel_time", self.h, C.byref(ms), C.byref(k))
+        return ms.value, k.value
+
+    def fetch(self) -> dict:
+        """All groups to host: keys (list of arrays), key_valid, values, valid."""
+        n = self.num_groups()
+        keys = [np.empty(n, dtype=NP_OF[self.types[k]]) for k in self.keys]
+        vals, valid = [], []
+        for a in self.aggs:
+            is_f = a.func in (L.SUM, L.AVG) or (a.func in (L.MIN, L.MAX) and self.types[a.col] == L.FLOAT64)
+            vals.append(np.empty(n, dtype=np.float64 if is_f else np.int64))
+            valid.append(np.empty(n, dtype=np.uint8))
+        kp = (C.c_void_p * max(len(keys), 1))(*[k.ctypes.data_as(C.c_void_p) for k in keys])
+        vp = (C.c_void_p * max(len(vals), 1))(*[v.ctypes.data_as(C.c_void_p) for v in vals])
+        vv = (C.c_void_p * max(len(valid), 1))(*[v.ctypes.data_as(C.c_void_p) for v in valid])
+        if n:
+            call("wg_agg_fetch", self.h, 0, n, kp, vp, vv, L.MEM_HOST)
+        kvalid = np.empty(n, dtype=np.uint8)
+        if n:
+            call("wg_agg_fetch_key_valid", self.h, 0, n, kvalid.ctypes.data_as(C.c_void_p), L.MEM_HOST)
+        return dict(n=n, keys=keys, key_valid=kvalid.astype(bool), values=vals, valid=[v.astype(bool) for v in valid])
+
+    def fetch_state(self, agg: int) -> dict:
+        n = self.num_groups()
+        cnt = np.empty(n, np.int64)
+        lo = np.empty(n, np.uint64)
+        hi = np.empty(n, np.int64)
+        mm = np.empty(n, np.int64)
+        if n:
+            call("wg_agg_fetch_state", self.h, agg, 0, n, *(x.ctypes.data_as(C.c_void_p) for x in (cnt, lo, hi, mm)),
+                 L.MEM_HOST)
+        return dict(count=cnt, sum_lo=lo, sum_hi=hi, minmax=mm)
+
+
+def filter_sel(ctx: Context, col: Col, n_rows: int, op: int, i64: int = 0, f64: float = 0.0,
+               sel: Optional[DeviceBuffer] = None, n_sel: int = 0, out: Optional[DeviceBuffer] = None):
+    """KernelFilter on the device: returns (device sel buffer, count)."""
+    n = n_sel if sel is not None else n_rows
+    out = out if out is not None else ctx.alloc(max(n, 1) * 4)
+    nout = C.c_uint64()
+    cc = col.c()
+    call("wg_filter", ctx.h, C.byref(cc), n_rows, sel.ptr if sel is not None else None, n_sel, op, i64, f64,
+         out.ptr, C.byref(nout))
+    return out, nout.value
+
+
+class HashJoin:
+    """Inner hash join on one int-class key (wg_join_*): build, then probe."""
+
+    def __init__(self, ctx: Context, build_types: Sequence[int], build_key: int, inline: Sequence[int] = (),
+                 scales: Optional[Sequence[int]] = None, build_hint: int = 0):
+        self.ctx = ctx
+        self.types = list(build_types)
+        n = len(build_types)
+        t = (C.c_int32 * n)(*build_types)
+        s = (C.c_int32 * n)(*(scales or [0] * n))
+        il = (C.c_int32 * max(len(inline), 1))(*inline)
+        h = C.c_void_p()
+        call("wg_join_create", ctx.h, n, t, s, build_key, len(inline), il, build_hint, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            call("wg_join_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.lib().wg_join_destroy(self.h)
+        except Exception:
+            pass
+
+    def build(self, cols: Sequence[Col], n_rows: int, device: bool = True):
+        call("wg_join_build_append", self.h, columns(cols), n_rows, L.MEM_DEVICE if device else L.MEM_HOST)
+
+    def finish(self):
+        call("wg_join_build_finish", self.h)
+
+    def rows(self) -> int:
+        n = C.c_uint64()
+        call("wg_join_build_rows", self.h, C.byref(n))
+        return n.value
+
+    def probe(self, key: Col, n_rows: int, out_p: DeviceBuffer, out_b: DeviceBuffer, out_cap: int,
+              cursor: Optional[L.ProbeCursor] = None, sel=None):
+        """One bounded probe call; returns (pairs written, cursor)."""
+        cur = cursor if cursor is not None else L.ProbeCursor(0, -1, 0, 0)
+        n = C.c_uint64()
+        kc = key.c()
+        sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
+        call("wg_join_probe", self.h, C.byref(kc), n_rows, sp, ns, out_p.ptr, out_b.ptr, out_cap, C.byref(cur),
+             C.byref(n))
+        return n.value, cur
+
+    def probe_agg(self, probe_cols: Sequence[Col], probe_key: int, n_rows: int, agg: HashAggregate, sel=None):
+        sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
+        call("wg_join_probe_agg", self.h, columns(probe_cols), len(probe_cols), probe_key, n_rows, sp, ns, agg.h)
+
+    def gather_build(self, col: int, rows: DeviceBuffer, n: int, out: DeviceBuffer):
+        call("wg_join_gather_build", self.h, col, rows.ptr, n, out.ptr)
+
+    def digest(self, probe_cols: Sequence[Col], prow: DeviceBuffer, brow: DeviceBuffer, n: int,
+               probe_sel: Sequence[int], build_sel: Sequence[int]):
+        d = (C.c_uint64 * 2)()
+        ps = (C.c_int32 * max(len(probe_sel), 1))(*probe_sel)
+        bs = (C.c_int32 * max(len(build_sel), 1))(*build_sel)
+        call("wg_join_digest", self.h, columns(probe_cols), prow.ptr, brow.ptr, n, ps, len(probe_sel), bs,
+             len(build_sel), d)
+        return int(d[0]), int(d[1])
+
+
+def gather(ctx: Context, col: Col, rows: DeviceBuffer, n: int, out: DeviceBuffer):
+    cc = col.c()
+    call("wg_gather", ctx.h, C.byref(cc), rows.ptr, n, out.ptr)
