@@ -27,7 +27,8 @@ typedef enum wg_gen_kind {
   WG_GEN_LOGNORMAL_I64 = 7, /* int64 round(exp(N(f0, f1))) */
   WG_GEN_PERM_KEY_I64 = 8,  /* int64 mix64(pi(row)) for a seeded permutation pi of [0, i0): unique keys */
   WG_GEN_FK_I64 = 9,        /* int64 foreign key: with prob i1/1000 mix64(u % i0) (a build key), else mix64(i0 + u % i0) */
-  WG_GEN_SEQ_I64 = 10       /* int64 i0 + row * i1 (monotonic timestamps) */
+  WG_GEN_SEQ_I64 = 10,      /* int64 i0 + row * i1 (monotonic timestamps) */
+  WG_GEN_RANK_U32 = 11      /* uint32: the address WG_GEN_ZIPF_U32 (same seed) gives rank row + 1 */
 } wg_gen_kind;
 
 /* Fill `out` (device) with rows [row0, row0 + n) of the generator. */

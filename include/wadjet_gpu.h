@@ -159,13 +159,15 @@ wg_status wg_agg_destroy(wg_agg* h);
 
 /* Sink.Consume (operators.go:27; aggregate.go:1075). cols[n_cols] follow the
  * create-time schema; sel (ascending row indices, may be NULL) selects n_sel
- * of the n_rows rows; pred is an optional fused filter (NULL = none).
+ * of the n_rows rows; preds[0..n_preds) (n_preds <= 2) are fused filters that
+ * must all pass — a chain of KernelFilters ANDed through one Sel
+ * (filter.go:408-418).
  * mem = WG_MEM_HOST: the call copies what it needs before returning and
  * coalesces small batches into large device batches. mem = WG_MEM_DEVICE:
  * pointers are device-resident; the launch is enqueued on the ctx stream. */
 wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows,
-                         const uint32_t* sel, uint64_t n_sel, const wg_predicate* pred,
-                         int32_t mem);
+                         const uint32_t* sel, uint64_t n_sel, const wg_predicate* preds,
+                         int32_t n_preds, int32_t mem);
 /* MergeableSink.MergeSink (operators.go:56-60; aggregate.go:4682-4700):
  * fold src's partial state into dst (same create-time spec). src is left empty. */
 wg_status wg_agg_merge(wg_agg* dst, wg_agg* src);
@@ -295,9 +297,20 @@ typedef struct wg_batch wg_batch;
 
 typedef struct wg_batch_query {
   wg_column key;       /* one int-class key column (device) */
-  wg_column val;       /* value column for SUM (device); type INT64 or DECIMAL */
-  uint64_t n_rows;
+  wg_column val;       /* value column for SUM (device); INT32, INT64 or DECIMAL */
+  uint64_t n_rows;     /* rows [row_begin, row_begin + n_rows) of the columns */
   uint64_t est_groups; /* cardinality estimate used to size the table */
+  uint64_t row_begin;
+  /* optional window filter: rows with range_lo <= range_col < range_hi
+   * (two ANDed KernelFilters); range_col.data == NULL disables it */
+  wg_column range_col;
+  int64_t range_lo, range_hi;
+  /* optional inner join to a dimension (unique build keys): rows whose key has
+   * no build row drop out; the group key becomes inline payload `dim_payload`
+   * of the matched build row (fused probe -> GROUP BY build column) */
+  const wg_join* dim;
+  int32_t dim_payload;
+  int32_t reserved;
 } wg_batch_query;
 
 wg_status wg_batch_create(wg_ctx* ctx, int32_t max_queries, uint64_t max_groups_total,
@@ -316,6 +329,67 @@ wg_status wg_batch_result(wg_batch* b, int32_t q, uint64_t* n_groups, int64_t* k
                           uint64_t* sum_lo, int64_t* sum_hi, int64_t* counts, uint64_t cap);
 /* Chosen layout for query q: 1 = LDS table, 0 = global table; workgroup size. */
 wg_status wg_batch_choice(wg_batch* b, int32_t q, int32_t* use_lds, int32_t* block);
+/* Current tuning (what wg_batch_autotune picked, or what was set): workgroup
+ * size, LDS cardinality threshold (queries estimated at or below it use an
+ * LDS table; 0 = never), rows per work item. */
+wg_status wg_batch_tuning(wg_batch* b, int32_t* block, int32_t* lds_threshold, uint64_t* tile_rows);
+wg_status wg_batch_set_tuning(wg_batch* b, int32_t block, int32_t lds_threshold, uint64_t tile_rows);
+
+/* ---------------------------------------------------------------- serving */
+
+/* A request loop over a device-resident table (continuous batching).
+ * The reference serves queries through its worker (internal/worker/executor.go,
+ * one pipeline per fragment); here requests are admitted by the HBM footprint
+ * of their hash tables into `lanes` concurrent batches (each lane its own HIP
+ * stream and persistent-launch runner): a request that arrives while other
+ * batches run joins the next lane that frees up, so work overlaps on the
+ * device. mode 1 runs the same requests one per launch, FIFO (the plain path). */
+typedef struct wg_serve wg_serve;
+
+typedef struct wg_serve_query {
+  int32_t kind;        /* 0: GROUP BY key_col; 1: JOIN dim ON key_col GROUP BY dim payload */
+  int32_t key_col;     /* table column: group key (kind 0) or join key (kind 1) */
+  int32_t val_col;     /* table column under SUM */
+  int32_t dim;         /* kind 1: id from wg_serve_add_dim */
+  int32_t dim_payload; /* kind 1: inline payload index of the dimension */
+  int32_t reserved;
+  int64_t ts_lo, ts_hi; /* window [ts_lo, ts_hi) on the table's ts column */
+  uint64_t est_groups;
+} wg_serve_query;
+
+typedef struct wg_serve_stats {
+  uint64_t completed;
+  uint64_t batches;       /* launches */
+  uint64_t max_in_flight; /* most lanes busy at once */
+  uint64_t peak_footprint;/* largest admitted hash-table bytes at once */
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double wall_s, qps;
+} wg_serve_stats;
+
+/* cols[n_cols] are device-resident columns of n_rows rows; ts_col is a
+ * non-decreasing int64 column used for windows. hbm_budget bounds the bytes of
+ * hash tables admitted at once (over all lanes). */
+wg_status wg_serve_create(wg_ctx* ctx, const wg_column* cols, int32_t n_cols, uint64_t n_rows,
+                          int32_t ts_col, int32_t lanes, uint64_t hbm_budget, int32_t max_batch,
+                          int32_t mode, wg_serve** out);
+wg_status wg_serve_destroy(wg_serve* s);
+/* Register a built dimension join (unique keys); returns its id. */
+wg_status wg_serve_add_dim(wg_serve* s, const wg_join* dim, int32_t* dim_id);
+/* Thread-safe submit; the arrival time is taken here. */
+wg_status wg_serve_submit(wg_serve* s, const wg_serve_query* q, uint64_t* id);
+/* Block until request `id` completed; its group count and latency. */
+wg_status wg_serve_wait(wg_serve* s, uint64_t id, uint64_t* n_groups, double* latency_ms);
+/* Result rows of a completed request (keys, Int128 sums as lo/hi, counts). */
+wg_status wg_serve_result(wg_serve* s, uint64_t id, int64_t* keys, uint64_t* sum_lo, int64_t* sum_hi,
+                          int64_t* counts, uint64_t cap);
+/* Open-loop load: submit qs[0..n) at Poisson arrival times of rate qps (an
+ * arrival thread in the library), wait for all, write per-request latencies
+ * (ms) and the request ids. */
+wg_status wg_serve_replay(wg_serve* s, const wg_serve_query* qs, int32_t n, double qps, uint64_t seed,
+                          double* latency_ms, uint64_t* ids);
+wg_status wg_serve_stats_get(wg_serve* s, wg_serve_stats* out);
+/* Forget completed requests and reset the statistics. */
+wg_status wg_serve_reset(wg_serve* s);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,8 @@ def abs_sums(hcols, keys, aggs, pred=None, sel=None):
     out = []
     for a in aggs:
         if a.func in (L.SUM, L.AVG) and hcols[a.col][1] == L.FLOAT64:
-            h2 = list(hcols) + [(np.abs(hcols[a.col][0]), L.FLOAT64, hcols[a.col][2])]
+            v = hcols[a.col][2] if len(hcols[a.col]) > 2 else None
+            h2 = list(hcols) + [(np.abs(hcols[a.col][0]), L.FLOAT64, v)]
             r = O.hash_agg(h2, keys, [(O.SUM, len(hcols))], sel=sel,
                            pred=None if pred is None else (pred.col, pred.op, pred.i64, pred.f64))
             out.append(oracle_groups(r)["acc"][:, 0]["sum_f64"])

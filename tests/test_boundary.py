@@ -58,7 +58,7 @@ def test_null_arguments_are_rejected_on_the_host():
         L.call("wg_ctx_create", 0, None)
     assert e.value.status == L.WG_ERR_INVALID
     with pytest.raises(L.WgError) as e:
-        L.call("wg_agg_consume", None, None, 0, None, 0, None, 1)
+        L.call("wg_agg_consume", None, None, 0, None, 0, None, 0, 1)
     assert e.value.status == L.WG_ERR_INVALID
     with pytest.raises(L.WgError):
         L.call("wg_agg_num_groups", None, C.byref(C.c_uint64()))

@@ -28,7 +28,7 @@ def probe_all(j, key, n, cap, total_cap):
         ps.append(tp.to_host(np.uint32, k))
         bs.append(tb.to_host(np.uint32, k))
         calls += 1
-        assert calls < 10_000
+        assert calls < total_cap // cap * 4 + 1000
     return np.concatenate(ps), np.concatenate(bs), calls
 
 
@@ -36,7 +36,7 @@ def pairs_set(p, b):
     return np.sort(p.astype(np.uint64) << np.uint64(32) | b.astype(np.uint64))
 
 
-@pytest.mark.parametrize("cap", [32768, 1000, 7])
+@pytest.mark.parametrize("cap", [32768, 1000, 97])
 def test_probe_multiset_with_duplicates_and_nulls(ctx, cap):
     """Duplicate build keys (chains), NULL keys on both sides, int32 probe
     key against an int64 build key; the bounded probe resumes mid-chain."""
@@ -44,7 +44,7 @@ def test_probe_multiset_with_duplicates_and_nulls(ctx, cap):
     rng = np.random.default_rng(cap)
     nb, npr = 20_000, 50_000
     bk = rng.integers(0, 8000, nb).astype(np.int64)
-    bk[:50] = 77  # one long chain
+    bk[:300] = 77  # one chain longer than the smallest output bound
     bm = rng.random(nb) > 0.02
     pk = rng.integers(0, 12_000, npr).astype(np.int32)
     pk[::100] = 77
@@ -65,8 +65,7 @@ def test_probe_multiset_with_duplicates_and_nulls(ctx, cap):
                                  cap=npr * 60)
     p, b, calls = probe_all(j, key, npr, cap, len(want_p) + 1)
     np.testing.assert_array_equal(pairs_set(p, b), pairs_set(want_p, want_b))
-    if cap == 7:
-        assert calls >= len(want_p) // 7
+    assert calls >= len(want_p) // cap
 
 
 def test_config3_shape_join_multiset_and_digest(ctx):

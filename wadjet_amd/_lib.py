@@ -19,7 +19,7 @@ EQ, NE, LT, LE, GT, GE = range(6)
 MEM_HOST, MEM_DEVICE = 0, 1
 
 GEN_UNIFORM_I64, GEN_UNIFORM_I32, GEN_UNIFORM_U16, GEN_ZIPF_I32, GEN_ZIPF_U32 = 1, 2, 3, 4, 5
-GEN_NORMAL_F64, GEN_LOGNORMAL_I64, GEN_PERM_KEY_I64, GEN_FK_I64, GEN_SEQ_I64 = 6, 7, 8, 9, 10
+GEN_NORMAL_F64, GEN_LOGNORMAL_I64, GEN_PERM_KEY_I64, GEN_FK_I64, GEN_SEQ_I64, GEN_RANK_U32 = 6, 7, 8, 9, 10, 11
 
 TYPE_BYTES = {INT32: 4, INT64: 8, FLOAT64: 8, DECIMAL: 8, DECIMAL128: 16, IPV4: 8, UINT32: 4, UINT16: 2}
 
@@ -41,7 +41,21 @@ class ProbeCursor(C.Structure):
 
 
 class BatchQuery(C.Structure):
-    _fields_ = [("key", Column), ("val", Column), ("n_rows", C.c_uint64), ("est_groups", C.c_uint64)]
+    _fields_ = [("key", Column), ("val", Column), ("n_rows", C.c_uint64), ("est_groups", C.c_uint64),
+                ("row_begin", C.c_uint64), ("range_col", Column), ("range_lo", C.c_int64), ("range_hi", C.c_int64),
+                ("dim", C.c_void_p), ("dim_payload", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServeQuery(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("key_col", C.c_int32), ("val_col", C.c_int32), ("dim", C.c_int32),
+                ("dim_payload", C.c_int32), ("reserved", C.c_int32), ("ts_lo", C.c_int64), ("ts_hi", C.c_int64),
+                ("est_groups", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("batches", C.c_uint64), ("max_in_flight", C.c_uint64),
+                ("peak_footprint", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("wall_s", C.c_double), ("qps", C.c_double)]
 
 
 P = C.c_void_p
@@ -66,7 +80,7 @@ SIGNATURES: dict[str, list] = {
     "wg_agg_create": [P, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(C.c_uint8), I32, C.POINTER(I32), I32,
                       C.POINTER(AggSpecC), U64, C.POINTER(P)],
     "wg_agg_destroy": [P],
-    "wg_agg_consume": [P, C.POINTER(Column), U64, P, U64, C.POINTER(PredicateC), I32],
+    "wg_agg_consume": [P, C.POINTER(Column), U64, P, U64, C.POINTER(PredicateC), I32, I32],
     "wg_agg_merge": [P, P],
     "wg_agg_finalize": [P],
     "wg_agg_num_groups": [P, C.POINTER(U64)],
@@ -95,6 +109,17 @@ SIGNATURES: dict[str, list] = {
     "wg_batch_autotune": [P, C.POINTER(BatchQuery), I32],
     "wg_batch_result": [P, I32, C.POINTER(U64), P, P, P, P, U64],
     "wg_batch_choice": [P, I32, C.POINTER(I32), C.POINTER(I32)],
+    "wg_batch_tuning": [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(U64)],
+    "wg_batch_set_tuning": [P, I32, I32, U64],
+    "wg_serve_create": [P, C.POINTER(Column), I32, U64, I32, I32, U64, I32, I32, C.POINTER(P)],
+    "wg_serve_destroy": [P],
+    "wg_serve_add_dim": [P, P, C.POINTER(I32)],
+    "wg_serve_submit": [P, C.POINTER(ServeQuery), C.POINTER(U64)],
+    "wg_serve_wait": [P, U64, C.POINTER(U64), C.POINTER(C.c_double)],
+    "wg_serve_result": [P, U64, P, P, P, P, U64],
+    "wg_serve_replay": [P, C.POINTER(ServeQuery), I32, C.c_double, U64, P, P],
+    "wg_serve_stats_get": [P, C.POINTER(ServeStats)],
+    "wg_serve_reset": [P],
     "wg_gen": [P, I32, P, U64, U64, U64, I64, I64, C.c_double, C.c_double],
 }
 STRING_FNS = ("wg_last_error", "wg_version")

@@ -29,7 +29,7 @@ namespace wg {
 struct ConsumeParams {
   ColSet cs;
   KeyDesc key;
-  PredDesc pred;
+  PredDesc pred[2];  // col = -1: unused
   AggProgram pg;
   GlobalTable T;
   const uint32_t* sel;  // null = rows [row_base, row_base + n)
@@ -85,7 +85,8 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
     for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
       const uint64_t row = P.sel ? (uint64_t)P.sel[i] : P.row_base + i;
       RowRef r{row, 0, nullptr};
-      if (P.pred.col >= 0 && !pred_pass(P.cs.cols[P.pred.col], row, P.pred.op, P.pred.i64, P.pred.f64)) continue;
+      if (P.pred[0].col >= 0 && !pred_pass(P.cs.cols[P.pred[0].col], row, P.pred[0].op, P.pred[0].i64, P.pred[0].f64)) continue;
+      if (P.pred[1].col >= 0 && !pred_pass(P.cs.cols[P.pred[1].col], row, P.pred[1].op, P.pred[1].i64, P.pred[1].f64)) continue;
       uint64_t key;
       uint64_t* slot = nullptr;
       if (!build_key<false>(P.key, P.cs, r, key)) {
@@ -739,11 +740,12 @@ static void pick_launch(wg_agg* h) {
 // One launch over rows [row_base, row_base + n) or sel[0..n); reads back the
 // counters (one stream sync).
 static wg_status consume_once(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t row_base, uint64_t n,
-                              const PredDesc& pred, uint64_t* c) {
+                              const PredDesc* pred, uint64_t* c) {
   ConsumeParams P{};
   P.cs = cs;
   P.key = h->key;
-  P.pred = pred;
+  P.pred[0] = pred[0];
+  P.pred[1] = pred[1];
   P.pg = h->pg;
   P.T = table_of(h);
   P.sel = sel;
@@ -786,13 +788,13 @@ static wg_status consume_once(wg_agg* h, const ColSet& cs, const uint32_t* sel, 
   return WG_OK;
 }
 
-static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc& pred,
+static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred,
                                int depth);
 
 // After a launch that overflowed: grow the table so that every group seen so
 // far plus every pending row fits at <= 50 % load, fold the overflow partials
 // in, then replay the overflow rows.
-static wg_status resolve_overflow(wg_agg* h, const ColSet& cs, const PredDesc& pred, const uint64_t* c, int depth) {
+static wg_status resolve_overflow(wg_agg* h, const ColSet& cs, const PredDesc* pred, const uint64_t* c, int depth) {
   WG_CHECK(depth < 8, WG_ERR_DEVICE, "group table overflow did not converge");
   const uint64_t n_rows = c[CTR_OVF_ROWS], n_part = c[CTR_OVF_PART];
   h->groups = c[CTR_GROUPS];
@@ -826,7 +828,7 @@ static wg_status resolve_overflow(wg_agg* h, const ColSet& cs, const PredDesc& p
   return WG_OK;
 }
 
-static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc& pred,
+static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred,
                                int depth) {
   uint64_t pos = 0;
   while (pos < n) {
@@ -846,7 +848,7 @@ static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel,
   return WG_OK;
 }
 
-wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc& pred) {
+wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred) {
   WG_CHECK(!h->finalized, WG_ERR_STATE, "consume after finalize");
   if (n == 0) return WG_OK;
   WG_CHECK(n < (1ull << 31), WG_ERR_INVALID, "batches of 2^31 rows or more must be split by the caller");
@@ -901,7 +903,7 @@ wg_status agg_flush_stage(wg_agg* h) {
     cs.cols[c] = ColDesc{S.dev[c], S.dvalid[c], h->col_types[c], h->col_scales[c]};
     cs.inline_word[c] = -1;
   }
-  PredDesc none{-1, 0, 0, 0.0};
+  const PredDesc none[2] = {{-1, 0, 0, 0.0}, {-1, 0, 0, 0.0}};
   const uint64_t n = S.rows;
   S.rows = 0;
   // consume syncs on its counter read-back, after which the pinned buffers are free
@@ -938,8 +940,16 @@ static int64_t host_int(const wg_column& c, uint64_t row) {
 
 // Host batch -> staged rows: applies sel and the predicate (the KernelFilter
 // the caller pushed down), copies the surviving rows and their validity bits.
+static bool host_pred(const wg_column* cols, const wg_predicate& p, uint64_t row) {
+  const wg_column& pc = cols[p.col];
+  if (!host_valid(pc, row)) return false;
+  if (pc.type == WG_FLOAT64) return cmp_host<double>(((const double*)pc.data)[row], p.f64, p.op);
+  if (pc.type == WG_INT32 || pc.type == WG_UINT16) return cmp_host<int32_t>((int32_t)host_int(pc, row), (int32_t)p.i64, p.op);
+  return cmp_host<int64_t>(host_int(pc, row), p.i64, p.op);
+}
+
 static wg_status consume_host(wg_agg* h, const wg_column* cols, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
-                              const wg_predicate* pred) {
+                              const wg_predicate* preds, int n_preds) {
   WG_TRY(stage_init(h));
   HostStage& S = h->stage;
   const uint64_t n = sel ? n_sel : n_rows;
@@ -957,15 +967,9 @@ static wg_status consume_host(wg_agg* h, const wg_column* cols, uint64_t n_rows,
   }
   for (uint64_t i = 0; i < n; ++i) {
     const uint64_t row = sel ? sel[i] : i;
-    if (pred && pred->col >= 0) {
-      const wg_column& pc = cols[pred->col];
-      if (!host_valid(pc, row)) continue;
-      bool ok;
-      if (pc.type == WG_FLOAT64) ok = cmp_host<double>(((const double*)pc.data)[row], pred->f64, pred->op);
-      else if (pc.type == WG_INT32 || pc.type == WG_UINT16) ok = cmp_host<int32_t>((int32_t)host_int(pc, row), (int32_t)pred->i64, pred->op);
-      else ok = cmp_host<int64_t>(host_int(pc, row), pred->i64, pred->op);
-      if (!ok) continue;
-    }
+    bool pass = true;
+    for (int k = 0; k < n_preds && pass; ++k) pass = host_pred(cols, preds[k], row);
+    if (!pass) continue;
     const uint64_t d = S.rows;
     for (int c = 0; c < h->n_cols; ++c) {
       const int w = type_bytes(h->col_types[c]);
@@ -1061,22 +1065,32 @@ wg_status wg_agg_destroy(wg_agg* h) {
 }
 
 wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
-                         const wg_predicate* pred, int32_t mem) {
+                         const wg_predicate* preds, int32_t n_preds, int32_t mem) {
   clear_error();
   WG_CHECK(h && cols, WG_ERR_INVALID, "null argument");
   WG_CHECK(!h->finalized, WG_ERR_STATE, "consume after finalize");
+  WG_CHECK(n_preds >= 0 && n_preds <= 2 && (n_preds == 0 || preds), WG_ERR_INVALID, "%d predicates (max 2)", n_preds);
   for (int c = 0; c < h->n_cols; ++c) {
     WG_CHECK(cols[c].type == h->col_types[c], WG_ERR_INVALID, "column %d: type %d, created as %d", c, cols[c].type, h->col_types[c]);
     WG_CHECK(cols[c].validity == nullptr || h->col_nullable[c], WG_ERR_INVALID, "column %d has a validity bitmap but was created non-nullable", c);
     WG_CHECK(cols[c].data != nullptr || (sel ? n_sel : n_rows) == 0, WG_ERR_INVALID, "column %d: null data", c);
   }
-  if (pred && pred->col >= 0) {
-    WG_CHECK(pred->col < h->n_cols, WG_ERR_INVALID, "predicate column out of range");
-    const int t = h->col_types[pred->col];
+  PredDesc pd[2] = {{-1, 0, 0, 0.0}, {-1, 0, 0, 0.0}};
+  int np = 0;
+  for (int k = 0; k < n_preds; ++k) {
+    const wg_predicate& p = preds[k];
+    if (p.col < 0) continue;
+    WG_CHECK(p.col < h->n_cols, WG_ERR_INVALID, "predicate column out of range");
+    const int t = h->col_types[p.col];
     WG_CHECK(t != WG_DECIMAL && t != WG_DECIMAL128, WG_ERR_UNSUPPORTED, "no filter kernel for DECIMAL");
-    WG_CHECK(pred->op >= WG_EQ && pred->op <= WG_GE, WG_ERR_INVALID, "bad compare op");
+    WG_CHECK(p.op >= WG_EQ && p.op <= WG_GE, WG_ERR_INVALID, "bad compare op");
+    pd[np++] = PredDesc{p.col, p.op, p.i64, p.f64};
   }
-  if (mem == WG_MEM_HOST) return consume_host(h, cols, n_rows, sel, n_sel, pred);
+  if (mem == WG_MEM_HOST) {
+    wg_predicate hp[2];
+    for (int k = 0; k < np; ++k) hp[k] = wg_predicate{pd[k].col, pd[k].op, pd[k].i64, pd[k].f64};
+    return consume_host(h, cols, n_rows, sel, n_sel, hp, np);
+  }
   WG_CHECK(mem == WG_MEM_DEVICE, WG_ERR_INVALID, "bad mem kind %d", mem);
   const uint64_t n = sel ? n_sel : n_rows;
   if (n == 0) return WG_OK;
@@ -1103,8 +1117,6 @@ wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, cons
     cs.cols[c] = ColDesc{cols[c].data, cols[c].validity, cols[c].type, cols[c].scale};
     cs.inline_word[c] = -1;
   }
-  PredDesc pd{-1, 0, 0, 0.0};
-  if (pred) pd = PredDesc{pred->col, pred->op, pred->i64, pred->f64};
   WG_TRY(agg_flush_stage(h));
   return agg_consume_device(h, cs, sel, n, pd);
 }

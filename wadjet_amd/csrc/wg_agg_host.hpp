@@ -89,7 +89,7 @@ struct wg_agg {
 namespace wg {
 // Internal entry points shared with the join and batch runners.
 wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n,
-                             const PredDesc& pred);
+                             const PredDesc* pred /* [2], col = -1 unused */);
 wg_status agg_ensure_capacity(wg_agg* h, uint64_t extra_groups);
 wg_status agg_read_counters(wg_agg* h, uint64_t* c);
 wg_status agg_prepare_rows(wg_agg* h, uint64_t n);

@@ -101,6 +101,7 @@ __global__ void gen_kernel(int kind, void* out, uint64_t row0, uint64_t n, uint6
         break;
       }
       case WG_GEN_SEQ_I64: ((int64_t*)out)[j] = i0 + (int64_t)row * i1; break;
+      case WG_GEN_RANK_U32: ((uint32_t*)out)[j] = fmix32((uint32_t)(row + 1) ^ (uint32_t)seed); break;
       default: break;
     }
   }
@@ -141,7 +142,7 @@ extern "C" wg_status wg_gen(wg_ctx* ctx, int32_t kind, void* out, uint64_t row0,
                             int64_t i0, int64_t i1, double f0, double f1) {
   clear_error();
   WG_CHECK(ctx && (out || n == 0), WG_ERR_INVALID, "null argument");
-  WG_CHECK(kind >= WG_GEN_UNIFORM_I64 && kind <= WG_GEN_SEQ_I64, WG_ERR_INVALID, "generator %d", kind);
+  WG_CHECK(kind >= WG_GEN_UNIFORM_I64 && kind <= WG_GEN_RANK_U32, WG_ERR_INVALID, "generator %d", kind);
   if (n == 0) return WG_OK;
   double* cdf = nullptr;
   int half = 1;

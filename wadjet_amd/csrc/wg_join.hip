@@ -23,37 +23,10 @@
 
 #include "wg_agg_dev.hpp"
 #include "wg_agg_host.hpp"
+#include "wg_join_dev.hpp"
+#include "wg_join_host.hpp"
 
 namespace wg {
-
-constexpr int kMaxInline = 4;
-
-struct JoinTable {
-  uint64_t* slots;  // cap * stride words + 1 side slot (key == kEmpty)
-  uint64_t mask;
-  int32_t stride;
-  const int32_t* next;
-};
-
-__device__ __forceinline__ int32_t slot_head(const uint64_t* s) { return (int32_t)(uint32_t)s[1]; }
-__device__ __forceinline__ uint32_t slot_count(const uint64_t* s) { return (uint32_t)(s[1] >> 32); }
-
-// The slot of `key`, or null when the key has no build row.
-__device__ __forceinline__ const uint64_t* join_find(const JoinTable& J, uint64_t key) {
-  if (key == kEmpty) {
-    const uint64_t* s = J.slots + (J.mask + 1) * (uint64_t)J.stride;
-    return slot_count(s) ? s : nullptr;
-  }
-  uint64_t idx = mix64(key) & J.mask;
-  for (uint64_t p = 0; p <= J.mask; ++p) {
-    const uint64_t* s = J.slots + idx * (uint64_t)J.stride;
-    const uint64_t k = s[0];
-    if (k == key) return s;
-    if (k == kEmpty) return nullptr;
-    idx = (idx + 1) & J.mask;
-  }
-  return nullptr;
-}
 
 __global__ void join_fill_kernel(uint64_t* slots, uint64_t n_words, int stride) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -77,9 +50,17 @@ struct BuildParams {
 // HashJoin.Build inner loop (join.go:1051-1065): NULL keys are skipped; each
 // row becomes the new head of its key's chain (arenaAppendInt, join.go:551-560).
 __global__ void join_build_kernel(BuildParams B) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < B.n; j += (uint64_t)gridDim.x * blockDim.x) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t iters = (B.n + step - 1) / step;
+  for (uint64_t it = 0; it < iters; ++it) {
+    const uint64_t j = it * step + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t row = B.row0 + j;
-    if (!col_valid(B.key, row)) {
+    const bool live = j < B.n && col_valid(B.key, row);
+    // non-NULL build rows, one atomic per wave (uniqueness check at finish)
+    const uint64_t bal = __ballot(live);
+    if (__lane_id() == 0 && bal) at_add<GlbT>(B.claimed + 1, (uint64_t)__popcll(bal));
+    if (j >= B.n) continue;
+    if (!live) {
       B.next[row] = -1;
       continue;
     }
@@ -391,34 +372,6 @@ __global__ void agg_merge_part_kernel_j(const uint64_t* recs, uint64_t n, Global
 
 // ================================================================ host side
 
-struct wg_join {
-  wg_ctx* ctx = nullptr;
-  int32_t n_cols = 0;
-  int32_t types[wg::kMaxCols] = {};
-  int32_t scales[wg::kMaxCols] = {};
-  int32_t key_col = 0;
-  int32_t n_inline = 0;
-  int32_t inline_cols[wg::kMaxInline] = {};
-  int32_t stride = 2;
-  // retained build columns (device), capacity in rows
-  void* col_data[wg::kMaxCols] = {};
-  uint64_t* col_valid[wg::kMaxCols] = {};
-  bool has_valid[wg::kMaxCols] = {};
-  uint64_t rows = 0, row_cap = 0;
-  int32_t* next = nullptr;
-  // table
-  uint64_t* slots = nullptr;
-  uint64_t cap = 0;
-  uint64_t* counter = nullptr;  // claimed slots + scratch words
-  bool finished = false;
-  // probe scratch
-  uint64_t* cnt = nullptr;
-  uint64_t* off = nullptr;
-  uint64_t scratch_rows = 0;
-  void* cub_tmp = nullptr;
-  size_t cub_bytes = 0;
-};
-
 namespace wg {
 
 static wg_status join_alloc_table(wg_join* j, uint64_t cap) {
@@ -628,8 +581,13 @@ wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_row
 wg_status wg_join_build_finish(wg_join* j) {
   clear_error();
   WG_CHECK(j, WG_ERR_INVALID, "null handle");
-  // build columns that carry no NULLs keep their all-valid bitmap unused
+  uint64_t c[2];
+  WG_HIP(hipMemcpyAsync(c, j->counter, 16, hipMemcpyDeviceToHost, j->ctx->stream));
+  uint64_t side = 0;
+  WG_HIP(hipMemcpyAsync(&side, j->slots + j->cap * j->stride + 1, 8, hipMemcpyDeviceToHost, j->ctx->stream));
   WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  const uint64_t distinct = c[0] + ((side >> 32) ? 1 : 0);
+  j->claimed_unique = distinct == c[1];
   j->finished = true;
   return WG_OK;
 }

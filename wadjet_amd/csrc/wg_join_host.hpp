@@ -1,0 +1,34 @@
+// wg_join_host.hpp — host-side state of a wg_join handle (shared with the
+// batched / served runners that probe a built dimension table).
+#pragma once
+#include "wg_join_dev.hpp"
+
+struct wg_join {
+  wg_ctx* ctx = nullptr;
+  int32_t n_cols = 0;
+  int32_t types[wg::kMaxCols] = {};
+  int32_t scales[wg::kMaxCols] = {};
+  int32_t key_col = 0;
+  int32_t n_inline = 0;
+  int32_t inline_cols[wg::kMaxInline] = {};
+  int32_t stride = 2;
+  // retained build columns (device), capacity in rows
+  void* col_data[wg::kMaxCols] = {};
+  uint64_t* col_valid[wg::kMaxCols] = {};
+  bool has_valid[wg::kMaxCols] = {};
+  uint64_t rows = 0, row_cap = 0;
+  int32_t* next = nullptr;
+  // table
+  uint64_t* slots = nullptr;
+  uint64_t cap = 0;
+  uint64_t* counter = nullptr;  // claimed slots + scratch words
+  bool finished = false;
+  bool claimed_unique = false;  // every non-NULL build key distinct (set at finish)
+  // probe scratch
+  uint64_t* cnt = nullptr;
+  uint64_t* off = nullptr;
+  uint64_t scratch_rows = 0;
+  void* cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+};
+

@@ -18,20 +18,37 @@ NP_OF = {L.INT32: np.int32, L.INT64: np.int64, L.FLOAT64: np.float64, L.DECIMAL:
 
 
 class Context:
-    """One device + one HIP stream."""
+    """One device + one HIP stream. Closing it first closes every handle and
+    buffer still alive on it (the library frees them against this stream)."""
 
     def __init__(self, device: int = 0):
+        import weakref
         h = C.c_void_p()
         call("wg_ctx_create", device, C.byref(h))
         self.h = h
+        self._children = weakref.WeakSet()
         cus, hbm, lds = C.c_int32(), C.c_uint64(), C.c_int32()
         call("wg_ctx_info", h, C.byref(cus), C.byref(hbm), C.byref(lds))
         self.cus, self.hbm, self.lds = cus.value, hbm.value, lds.value
 
     def close(self):
         if self.h:
+            for c in list(self._children):
+                try:
+                    c.close()
+                except Exception:
+                    pass
             call("wg_ctx_destroy", self.h)
             self.h = None
+
+    def _adopt(self, child):
+        self._children.add(child)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def sync(self):
         call("wg_ctx_sync", self.h)
@@ -70,6 +87,10 @@ class DeviceBuffer:
         p = C.c_void_p()
         call("wg_malloc", ctx.h, nbytes, C.byref(p))
         self.ptr = p
+        ctx._adopt(self)
+
+    def close(self):
+        self.free()
 
     def to_host(self, dtype, count: Optional[int] = None) -> np.ndarray:
         dt = np.dtype(dtype)
@@ -80,9 +101,9 @@ class DeviceBuffer:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             call("wg_free", self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -158,15 +179,16 @@ class HashAggregate:
         h = C.c_void_p()
         call("wg_agg_create", ctx.h, n, t, s, nl, len(keys), k, len(aggs), a, group_hint, C.byref(h))
         self.h = h
+        ctx._adopt(self)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             call("wg_agg_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and self.ctx.h:
                 L.lib().wg_agg_destroy(self.h)
         except Exception:
             pass
@@ -185,9 +207,9 @@ class HashAggregate:
             sel = np.ascontiguousarray(sel, dtype=np.uint32)
             keep.append(sel)
             sp, ns = sel.ctypes.data_as(C.c_void_p), len(sel)
-        pc = L.PredicateC(pred.col, pred.op, pred.i64, pred.f64) if pred else None
-        call("wg_agg_consume", self.h, arr, n_rows, sp, ns, C.byref(pc) if pc else None,
-             L.MEM_DEVICE if device else L.MEM_HOST)
+        preds = [] if pred is None else (list(pred) if isinstance(pred, (list, tuple)) else [pred])
+        pa = (L.PredicateC * max(len(preds), 1))(*[L.PredicateC(p.col, p.op, p.i64, p.f64) for p in preds])
+        call("wg_agg_consume", self.h, arr, n_rows, sp, ns, pa, len(preds), L.MEM_DEVICE if device else L.MEM_HOST)
 
     def merge(self, other: "HashAggregate"):
         call("wg_agg_merge", self.h, other.h)
@@ -278,15 +300,16 @@ class HashJoin:
         h = C.c_void_p()
         call("wg_join_create", ctx.h, n, t, s, build_key, len(inline), il, build_hint, C.byref(h))
         self.h = h
+        ctx._adopt(self)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             call("wg_join_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
-            if self.h:
+            if self.h and self.ctx.h:
                 L.lib().wg_join_destroy(self.h)
         except Exception:
             pass
@@ -333,3 +356,147 @@ class HashJoin:
 def gather(ctx: Context, col: Col, rows: DeviceBuffer, n: int, out: DeviceBuffer):
     cc = col.c()
     call("wg_gather", ctx.h, C.byref(cc), rows.ptr, n, out.ptr)
+
+
+class Batch:
+    """Many GROUP BY key SUM(val), COUNT(*) queries in one persistent launch."""
+
+    def __init__(self, ctx: Context, max_queries: int, max_groups_total: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("wg_batch_create", ctx.h, max_queries, max_groups_total, C.byref(h))
+        self.h = h
+        ctx._adopt(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            call("wg_batch_destroy", self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                L.lib().wg_batch_destroy(self.h)
+        except Exception:
+            pass
+
+    @staticmethod
+    def queries(qs):
+        """qs: list of (key Col, val Col, n_rows, est_groups[, extras dict]) where
+        extras may hold row_begin, range=(Col, lo, hi), dim=(HashJoin, payload)."""
+        arr = (L.BatchQuery * max(len(qs), 1))()
+        for i, q in enumerate(qs):
+            k, v, n, g = q[:4]
+            x = q[4] if len(q) > 4 else {}
+            bq = L.BatchQuery()
+            bq.key, bq.val, bq.n_rows, bq.est_groups = k.c(), v.c(), n, g
+            bq.row_begin = x.get("row_begin", 0)
+            if "range" in x:
+                rc, lo, hi = x["range"]
+                bq.range_col, bq.range_lo, bq.range_hi = rc.c(), lo, hi
+            if "dim" in x:
+                bq.dim, bq.dim_payload = x["dim"][0].h, x["dim"][1]
+            arr[i] = bq
+        return arr
+
+    def run(self, qarr, n: int, use_graph: bool = True, plain: bool = False):
+        call("wg_batch_run", self.h, qarr, n, 1 if use_graph else 0, 1 if plain else 0)
+
+    def autotune(self, qarr, n: int):
+        call("wg_batch_autotune", self.h, qarr, n)
+
+    def tuning(self):
+        b, t, r = C.c_int32(), C.c_int32(), C.c_uint64()
+        call("wg_batch_tuning", self.h, C.byref(b), C.byref(t), C.byref(r))
+        return dict(block=b.value, lds_threshold=t.value, tile_rows=r.value)
+
+    def set_tuning(self, block: int, lds_threshold: int, tile_rows: int):
+        call("wg_batch_set_tuning", self.h, block, lds_threshold, tile_rows)
+
+    def choice(self, q: int):
+        u, b = C.c_int32(), C.c_int32()
+        call("wg_batch_choice", self.h, q, C.byref(u), C.byref(b))
+        return bool(u.value), b.value
+
+    def result(self, q: int) -> dict:
+        n = C.c_uint64()
+        call("wg_batch_result", self.h, q, C.byref(n), None, None, None, None, 0)
+        k = np.empty(n.value, np.int64)
+        lo = np.empty(n.value, np.uint64)
+        hi = np.empty(n.value, np.int64)
+        c = np.empty(n.value, np.int64)
+        call("wg_batch_result", self.h, q, C.byref(n), *(x.ctypes.data_as(C.c_void_p) for x in (k, lo, hi, c)), n.value)
+        o = np.argsort(k, kind="stable")
+        return dict(n=n.value, keys=k[o], sum_lo=lo[o], sum_hi=hi[o], counts=c[o])
+
+
+class Server:
+    """Continuous-batching request loop over device-resident columns (wg_serve_*)."""
+
+    def __init__(self, ctx: Context, cols: Sequence[Col], n_rows: int, ts_col: int, lanes: int = 3,
+                 hbm_budget: int = 4 << 30, max_batch: int = 64, plain: bool = False):
+        self.ctx = ctx
+        self.cols = list(cols)
+        h = C.c_void_p()
+        call("wg_serve_create", ctx.h, columns(cols), len(cols), n_rows, ts_col, lanes, hbm_budget, max_batch,
+             1 if plain else 0, C.byref(h))
+        self.h = h
+        ctx._adopt(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            call("wg_serve_destroy", self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                L.lib().wg_serve_destroy(self.h)
+        except Exception:
+            pass
+
+    def add_dim(self, j: "HashJoin") -> int:
+        i = C.c_int32()
+        call("wg_serve_add_dim", self.h, j.h, C.byref(i))
+        return i.value
+
+    @staticmethod
+    def query(kind, key_col, val_col, ts_lo, ts_hi, est_groups, dim=0, dim_payload=0):
+        return L.ServeQuery(kind, key_col, val_col, dim, dim_payload, 0, ts_lo, ts_hi, est_groups)
+
+    def submit(self, q) -> int:
+        i = C.c_uint64()
+        call("wg_serve_submit", self.h, C.byref(q), C.byref(i))
+        return i.value
+
+    def wait(self, rid: int):
+        n, ms = C.c_uint64(), C.c_double()
+        call("wg_serve_wait", self.h, rid, C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+    def result(self, rid: int) -> dict:
+        n, _ = self.wait(rid)
+        k = np.empty(n, np.int64)
+        lo = np.empty(n, np.uint64)
+        hi = np.empty(n, np.int64)
+        c = np.empty(n, np.int64)
+        call("wg_serve_result", self.h, rid, *(x.ctypes.data_as(C.c_void_p) for x in (k, lo, hi, c)), n)
+        o = np.argsort(k, kind="stable")
+        return dict(n=n, keys=k[o], sum_lo=lo[o], sum_hi=hi[o], counts=c[o])
+
+    def replay(self, qs, qps: float, seed: int = 0):
+        n = len(qs)
+        arr = (L.ServeQuery * n)(*qs)
+        lat = np.empty(n, np.float64)
+        ids = np.empty(n, np.uint64)
+        call("wg_serve_replay", self.h, arr, n, qps, seed, lat.ctypes.data_as(C.c_void_p),
+             ids.ctypes.data_as(C.c_void_p))
+        return lat, ids
+
+    def stats(self) -> dict:
+        st = L.ServeStats()
+        call("wg_serve_stats_get", self.h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in L.ServeStats._fields_}
+
+    def reset(self):
+        call("wg_serve_reset", self.h)

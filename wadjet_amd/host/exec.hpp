@@ -1,0 +1,302 @@
+// exec.hpp — C++ mirror of wadjet's operator interface for the GPU path.
+//
+// The reference's drop-in boundary is a set of Go interfaces in
+// internal/engine/exec/operators.go:10-103 (Source, UnaryOperator, Sink,
+// SinkSource, Cloneable, MergeableSink, BoundedOutputOperator) over
+// batch.RecordBatch (internal/engine/batch/batch.go:9-28). This header restates
+// them in C++ with the same names, argument meaning and error behaviour, plus
+// the operators the GPU library backs:
+//   GpuHashAggregate   SinkSource + MergeableSink      (aggregate.go:98, :1075, :3855, :4202, :4646-4700)
+//   GpuHashJoin        Build(ctx, Source) / Probe()      (join.go:771, :1967)
+//   GpuHashJoinProbe   UnaryOperator + BoundedOutput     (join.go:2072-2139, :2306-2515)
+// and the CPU pieces a pipeline needs around them: KernelFilter (filter.go:347-425),
+// BatchSource, CollectSink and Pipeline{Source, Ops, Sink, Workers}
+// (pipeline.go:31-690). Errors are returned as Status (Go's `error`).
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wadjet_gpu.h"
+
+namespace wadjet {
+
+// ---------------------------------------------------------------- status
+
+struct Status {
+  bool ok_ = true;
+  std::string msg;
+  static Status OK() { return {}; }
+  static Status Error(std::string m) { return Status{false, std::move(m)}; }
+  bool ok() const { return ok_; }
+};
+
+struct Context {
+  bool cancelled = false;  // ctx.Err() != nil
+};
+
+namespace batch {
+
+// TypeID subset on the GPU path (vector.go:302-465 storage classes).
+enum class TypeID { Int32, Int64, Float64, Decimal, IPv4, Port, Date, Timestamp };
+
+constexpr int kDefaultBatchSize = 2048;  // batch.go:6
+
+struct Int128 {  // decimal.go:11-14 (Go field order: Hi, Lo)
+  int64_t hi = 0;
+  uint64_t lo = 0;
+};
+
+// Bitmap: bit i%64 of word i/64, 1 = valid (bitmap.go:7-26).
+struct Bitmap {
+  std::vector<uint64_t> words;  // empty = no nulls
+  bool has_nulls() const { return !words.empty(); }
+  bool valid(int i) const { return words.empty() || ((words[i >> 6] >> (i & 63)) & 1ull); }
+  void set_null(int i, int n);
+};
+
+struct Column {  // parquet.Column subset: name, type, nullability, decimal scale
+  std::string name;
+  TypeID type;
+  int scale = 0;
+  bool nullable = true;
+};
+
+struct Vector {
+  TypeID type = TypeID::Int64;
+  int len = 0;
+  int scale = 0;
+  Bitmap nulls;
+  std::vector<int32_t> i32;
+  std::vector<int64_t> i64;
+  std::vector<double> f64;
+  std::vector<Int128> dec;  // DecimalData
+  static std::shared_ptr<Vector> make(TypeID t, int n, int scale = 0);
+};
+
+struct RecordBatch {
+  std::vector<std::shared_ptr<Vector>> columns;
+  std::vector<Column> schema;
+  int len = 0;
+  std::vector<uint32_t> sel;  // selection vector; has_sel = false means all rows active
+  bool has_sel = false;
+  int column_index(const std::string& name) const;
+  int active() const { return has_sel ? (int)sel.size() : len; }
+};
+
+}  // namespace batch
+
+using batch::RecordBatch;
+using BatchPtr = std::shared_ptr<RecordBatch>;
+
+// ------------------------------------------------------------ interfaces
+
+struct Source {  // operators.go:11-15; Next returns nullptr at end of data
+  virtual ~Source() = default;
+  virtual Status Init(Context&) = 0;
+  virtual Status Next(Context&, BatchPtr* out) = 0;
+  virtual Status Close() = 0;
+};
+
+struct UnaryOperator {  // operators.go:18-22; nullptr output = fully filtered
+  virtual ~UnaryOperator() = default;
+  virtual Status Init(Context&) = 0;
+  virtual Status Execute(Context&, const BatchPtr& in, BatchPtr* out) = 0;
+  virtual Status Close() = 0;
+};
+
+struct Sink {  // operators.go:24-31: must handle concurrent Consume calls
+  virtual ~Sink() = default;
+  virtual Status Init(Context&) = 0;
+  virtual Status Consume(Context&, const BatchPtr& b) = 0;
+  virtual Status Finalize(Context&) = 0;
+  virtual Status Close() = 0;
+};
+
+struct SinkSource : Sink, Source {  // operators.go:34-37
+  Status Init(Context& c) override = 0;
+  Status Close() override = 0;
+};
+
+struct Cloneable {  // operators.go:42-44
+  virtual ~Cloneable() = default;
+  virtual std::unique_ptr<UnaryOperator> Clone() = 0;
+};
+
+struct MergeableSink : SinkSource {  // operators.go:56-60
+  virtual std::unique_ptr<SinkSource> CloneSink() = 0;
+  virtual Status MergeSink(SinkSource* other) = 0;
+};
+
+struct BoundedOutputOperator {  // operators.go:86-90
+  virtual ~BoundedOutputOperator() = default;
+  virtual void EnableBoundedOutput() = 0;
+  virtual bool HasPendingOutput() = 0;
+  virtual Status NextOutput(Context&, BatchPtr* out) = 0;
+};
+
+// ------------------------------------------------------------ CPU pieces
+
+enum class CompareOp { Eq, Ne, Lt, Le, Gt, Ge };  // kernel.CompareOp
+
+// KernelFilter: typed compare of one column against a constant; compacts Sel
+// in place when the input already has one (filter.go:373-425).
+class KernelFilter : public UnaryOperator, public Cloneable {
+ public:
+  KernelFilter(std::string col, CompareOp op, double value) : col_(std::move(col)), op_(op), value_(value) {}
+  Status Init(Context&) override { return Status::OK(); }
+  Status Execute(Context&, const BatchPtr& in, BatchPtr* out) override;
+  Status Close() override { return Status::OK(); }
+  std::unique_ptr<UnaryOperator> Clone() override { return std::make_unique<KernelFilter>(col_, op_, value_); }
+  const std::string& column() const { return col_; }
+  CompareOp op() const { return op_; }
+  double value() const { return value_; }
+
+ private:
+  std::string col_;
+  CompareOp op_;
+  double value_;
+};
+
+// A Source over prepared batches (NewSliceSource / BatchSource in tests).
+class BatchSource : public Source {
+ public:
+  explicit BatchSource(std::vector<BatchPtr> batches) : batches_(std::move(batches)) {}
+  Status Init(Context&) override { pos_ = 0; return Status::OK(); }
+  Status Next(Context&, BatchPtr* out) override;
+  Status Close() override { return Status::OK(); }
+
+ private:
+  std::vector<BatchPtr> batches_;
+  size_t pos_ = 0;
+  std::mutex mu_;
+};
+
+// Collects every row it receives (Sel applied).
+class CollectSink : public Sink {
+ public:
+  Status Init(Context&) override { return Status::OK(); }
+  Status Consume(Context&, const BatchPtr& b) override;
+  Status Finalize(Context&) override { return Status::OK(); }
+  Status Close() override { return Status::OK(); }
+  std::vector<BatchPtr> batches;  // compacted copies
+  int rows() const;
+
+ private:
+  std::mutex mu_;
+};
+
+// Pipeline{Source, Ops, Sink, Workers} (pipeline.go:31-36): serial when
+// Workers <= 1; otherwise Workers threads pull batches, run cloned op chains
+// and feed cloned sinks that are merged into the primary (MergeSink) before
+// Finalize (pipeline.go:351-690, the non-partitioned shape).
+struct Pipeline {
+  Source* source = nullptr;
+  std::vector<UnaryOperator*> ops;
+  Sink* sink = nullptr;
+  int workers = 0;
+  Status Run(Context& ctx);
+};
+
+// Planner hook (the role of plan.go:7233-7240 with a WADJET_GPU switch): when
+// a pipeline's trailing KernelFilters feed a GpuHashAggregate, push the
+// predicates into the aggregate so the GPU evaluates them fused.
+void PushDownFiltersIntoGpuAggregate(Pipeline& p);
+
+// ------------------------------------------------------- GPU operators
+
+struct AggColumn {  // exec.AggColumn (aggregate.go:73-81)
+  wg_agg_func func;
+  std::string input_col;  // "" = COUNT(*)
+  std::string output_col;
+  std::string input_col2;  // product input (aggPreProject's qty * price), optional
+};
+
+class GpuHashAggregate : public MergeableSink {
+ public:
+  GpuHashAggregate(wg_ctx* ctx, std::vector<std::string> group_by, std::vector<AggColumn> aggs,
+                   uint64_t group_hint = 0);
+  ~GpuHashAggregate() override;
+  Status Init(Context&) override { return Status::OK(); }
+  Status Consume(Context&, const BatchPtr& b) override;
+  Status Finalize(Context&) override;
+  Status Next(Context&, BatchPtr* out) override;
+  Status Close() override;
+  std::unique_ptr<SinkSource> CloneSink() override;
+  Status MergeSink(SinkSource* other) override;
+  // predicates pushed down from KernelFilters (at most 2)
+  Status AddPredicate(const KernelFilter& f);
+  std::vector<batch::Column> OutputSchema() const;
+  wg_agg* handle() const { return h_; }
+
+ private:
+  Status resolve(const RecordBatch& b);  // typed dispatch on the first batch (resolveIndices)
+  wg_ctx* ctx_;
+  std::vector<std::string> group_by_;
+  std::vector<AggColumn> aggs_;
+  uint64_t hint_;
+  std::vector<std::pair<std::string, wg_predicate>> pending_preds_;
+  wg_agg* h_ = nullptr;
+  std::vector<batch::Column> in_schema_;
+  std::vector<int32_t> key_idx_;
+  std::vector<wg_predicate> preds_;
+  uint64_t emitted_ = 0, groups_ = 0;
+  bool finalized_ = false;
+  std::mutex mu_;  // h.mu: Consume is serialised per handle
+};
+
+class GpuHashJoinProbe;
+
+class GpuHashJoin {
+ public:
+  GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint = 0);
+  ~GpuHashJoin();
+  Status Build(Context& ctx, Source* build);  // join.go:771
+  std::unique_ptr<GpuHashJoinProbe> Probe();   // join.go:1967
+  uint64_t BuildRows() const { return rows_; }
+  wg_join* handle() const { return j_; }
+
+ private:
+  friend class GpuHashJoinProbe;
+  wg_ctx* ctx_;
+  std::string probe_key_, build_key_;
+  uint64_t hint_;
+  wg_join* j_ = nullptr;
+  std::vector<batch::Column> build_schema_;
+  std::vector<std::shared_ptr<batch::Vector>> build_cols_;  // retained build side (host copy for gathers)
+  uint64_t rows_ = 0;
+};
+
+// MaxProbeOutputRows = 32768 (join.go:2044-2060)
+constexpr uint64_t kMaxProbeOutputRows = 32768;
+
+class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, public Cloneable {
+ public:
+  explicit GpuHashJoinProbe(GpuHashJoin* j);
+  ~GpuHashJoinProbe() override;
+  Status Init(Context&) override { return Status::OK(); }
+  Status Execute(Context&, const BatchPtr& in, BatchPtr* out) override;
+  Status Close() override;
+  void EnableBoundedOutput() override { bounded_ = true; }
+  bool HasPendingOutput() override { return pending_; }
+  Status NextOutput(Context&, BatchPtr* out) override;
+  std::unique_ptr<UnaryOperator> Clone() override { return std::make_unique<GpuHashJoinProbe>(join_); }
+
+ private:
+  Status emit(BatchPtr* out);  // one bounded chunk of the current input
+  GpuHashJoin* join_;
+  bool bounded_ = false, pending_ = false;
+  BatchPtr cur_;
+  int key_idx_ = -1;
+  wg_probe_cursor cursor_{};
+  void* d_key_ = nullptr;
+  uint64_t* d_valid_ = nullptr;
+  uint32_t* d_sel_ = nullptr;
+  uint32_t* d_out_p_ = nullptr;
+  uint32_t* d_out_b_ = nullptr;
+  uint64_t cap_rows_ = 0;
+};
+
+}  // namespace wadjet

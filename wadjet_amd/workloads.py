@@ -65,3 +65,90 @@ def gen_config3_probe(ctx: Context, cfg: Cfg3, n: int = None, row0: int = 0):
     fk = ctx.gen(L.GEN_FK_I64, "int64", n, cfg.seed + 10, cfg.n_build, cfg.match_permille, row0=row0)
     qty = ctx.gen(L.GEN_UNIFORM_I32, "int32", n, cfg.seed + 11, 1, 50, row0=row0)
     return [Col(fk, L.INT64), Col(qty, L.INT32)]
+
+
+# --- config 4: many concurrent GROUP BY queries of uneven shapes ---------------
+def plan_config4(n_queries: int = 256, seed: int = 4, max_rows: int = 4_000_000, min_rows: int = 1000,
+                 median_rows: int = 65_536, sigma: float = 1.39, min_groups: int = 10, max_groups: int = 100_000):
+    """Per query: rows ~ lognormal clipped to [min_rows, max_rows], groups
+    log-uniform in [min_groups, max_groups], key type cycling int64 / int32 /
+    uint32 (IPv4-like), value int64 or DECIMAL(18,2)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    rows = np.clip(np.exp(rng.normal(np.log(median_rows), sigma, n_queries)), min_rows, max_rows).astype(np.int64)
+    groups = np.exp(rng.uniform(np.log(min_groups), np.log(max_groups), n_queries)).astype(np.int64)
+    key_types = [(L.INT64, L.INT32, L.UINT32)[i % 3] for i in range(n_queries)]
+    val_types = [(L.INT64, L.DECIMAL)[(i // 3) % 2] for i in range(n_queries)]
+    return [dict(rows=int(r), groups=int(g), key_type=k, val_type=v, seed=seed * 1000 + i)
+            for i, (r, g, k, v) in enumerate(zip(rows, groups, key_types, val_types))]
+
+
+def gen_config4(ctx: Context, plan):
+    """Device columns of every planned query: list of (key Col, val Col, rows, est_groups)."""
+    out = []
+    for q in plan:
+        n, g, s = q["rows"], q["groups"], q["seed"]
+        if q["key_type"] == L.INT64:
+            k = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, s, 0, g - 1)
+        elif q["key_type"] == L.INT32:
+            k = ctx.gen(L.GEN_UNIFORM_I32, "int32", n, s, -(g // 2), g - 1 - g // 2)
+        else:
+            k = ctx.gen(L.GEN_ZIPF_U32, "uint32", n, s, g, 0, 0.0)  # s = 0: uniform over g scrambled addresses
+        if q["val_type"] == L.INT64:
+            v = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, s + 7, -10**9, 10**9)
+            vc = Col(v, L.INT64)
+        else:
+            v = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, s + 7, 0, 10**12)
+            vc = Col(v, L.DECIMAL, None, 2)
+        out.append((Col(k, q["key_type"]), vc, n, g))
+    return out
+
+
+# --- config 5: resident flow table served under Poisson load ------------------
+CFG5_TYPES = [L.UINT32, L.UINT16, L.INT64, L.INT64]  # src_ip, dst_port, bytes, ts
+CFG5_TS0 = 1_700_000_000_000_000_000
+CFG5_TS_STEP = 1000
+CFG5_DIM_TYPES = [L.UINT32, L.INT32]  # ip, asn
+
+
+def gen_config5(ctx: Context, n: int = 500_000_000, seed: int = 5):
+    src = ctx.gen(L.GEN_ZIPF_U32, "uint32", n, seed, 1_000_000, 0, 1.2)
+    port = ctx.gen(L.GEN_UNIFORM_U16, "uint16", n, seed + 1, 0, 1023)
+    nbytes = ctx.gen(L.GEN_LOGNORMAL_I64, "int64", n, seed + 2, 0, 0, 6.0, 1.5)
+    ts = ctx.gen(L.GEN_SEQ_I64, "int64", n, seed + 3, CFG5_TS0, CFG5_TS_STEP)
+    return [Col(src, L.UINT32), Col(port, L.UINT16), Col(nbytes, L.INT64), Col(ts, L.INT64)]
+
+
+def gen_config5_dim(ctx: Context, n: int = 50_000, seed: int = 5, n_asn: int = 500):
+    """Dimension keyed by the n most frequent source addresses, with an ASN."""
+    ip = ctx.gen(L.GEN_RANK_U32, "uint32", n, seed)
+    asn = ctx.gen(L.GEN_UNIFORM_I32, "int32", n, seed + 9, 0, n_asn - 1)
+    return [Col(ip, L.UINT32), Col(asn, L.INT32)]
+
+
+def plan_config5(n_table: int, n_queries: int, seed: int = 55, scale: float = 1.0):
+    """Mixed requests: GROUP BY dst_port / GROUP BY src_ip / JOIN dim GROUP BY asn,
+    each over a random ts window; `scale` shrinks the windows for small tables."""
+    import numpy as np
+    from .gpu import Server
+    rng = np.random.default_rng(seed)
+    qs, shapes = [], []
+    for i in range(n_queries):
+        kind = rng.choice(3, p=[0.5, 0.3, 0.2])
+        if kind == 0:
+            w = int(rng.integers(1_000_000, 4_000_000) * scale)
+            spec = (0, 1, 2, 1024)
+        elif kind == 1:
+            w = int(rng.integers(250_000, 1_000_000) * scale)
+            spec = (0, 0, 2, max(w // 2, 64))
+        else:
+            w = int(rng.integers(1_000_000, 4_000_000) * scale)
+            spec = (1, 0, 2, 512)
+        w = max(1000, min(w, n_table - 1))
+        start = int(rng.integers(0, n_table - w))
+        lo = CFG5_TS0 + start * CFG5_TS_STEP + int(rng.integers(0, CFG5_TS_STEP))
+        hi = lo + w * CFG5_TS_STEP
+        k, key_col, val_col, est = spec
+        qs.append(Server.query(int(k), key_col, val_col, lo, hi, est, dim=0, dim_payload=0))
+        shapes.append(dict(kind=int(k), rows=w, start=start))
+    return qs, shapes
