@@ -1,0 +1,256 @@
+// Operator-level tests of the C++ layer (wadjet_amd/host/exec.hpp), written
+// like the reference's own exec tests. `cpu` runs the host-only pieces
+// (KernelFilter, Pipeline, CollectSink); `gpu` runs GpuHashAggregate and
+// GpuHashJoin on device 0.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "../../wadjet_amd/host/exec.hpp"
+
+using namespace wadjet;
+using batch::Column;
+using batch::TypeID;
+using batch::Vector;
+
+static int g_fail = 0;
+#define EXPECT(cond, ...)                                  \
+  do {                                                     \
+    if (!(cond)) {                                         \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__);     \
+      std::printf(__VA_ARGS__);                            \
+      std::printf("\n");                                   \
+      ++g_fail;                                            \
+    }                                                      \
+  } while (0)
+#define OK(st) EXPECT((st).ok(), "%s", (st).msg.c_str())
+
+static BatchPtr make_batch(std::vector<Column> schema, int n) {
+  auto b = std::make_shared<RecordBatch>();
+  b->schema = schema;
+  b->len = n;
+  for (auto& c : schema) b->columns.push_back(Vector::make(c.type, n, c.scale));
+  return b;
+}
+
+// ----------------------------------------------------------------- CPU tests
+
+static void test_kernel_filter_and_pipeline() {
+  // KernelFilter compacts Sel; a second filter compacts it again (AND chain)
+  auto b = make_batch({{"k", TypeID::Int64}, {"f", TypeID::Float64}}, 10);
+  for (int i = 0; i < 10; ++i) {
+    b->columns[0]->i64[i] = i;
+    b->columns[1]->f64[i] = i - 4.5;
+  }
+  b->columns[1]->nulls.set_null(7, 10);  // NULL never passes
+  KernelFilter f1("f", CompareOp::Gt, -0.5), f2("k", CompareOp::Lt, 9);
+  Context ctx;
+  BatchPtr o1, o2;
+  OK(f1.Execute(ctx, b, &o1));
+  OK(f2.Execute(ctx, o1, &o2));
+  // f = i - 4.5 > -0.5 keeps rows 5..9 except the NULL row 7; then k < 9
+  EXPECT(o2 && o2->sel == std::vector<uint32_t>({5, 6, 8}), "sel mismatch");
+  BatchPtr none;
+  KernelFilter f3("f", CompareOp::Gt, 100.0);
+  OK(f3.Execute(ctx, b, &none));
+  EXPECT(!none, "fully filtered batch must be nullptr");
+  KernelFilter bad("nope", CompareOp::Eq, 1);
+  EXPECT(!bad.Execute(ctx, b, &none).ok(), "unknown filter column must error (filter.go:410)");
+
+  // Pipeline serial and parallel through CollectSink
+  std::vector<BatchPtr> src;
+  for (int m = 0; m < 8; ++m) {
+    auto x = make_batch({{"k", TypeID::Int64}, {"f", TypeID::Float64}}, 2048);
+    for (int i = 0; i < 2048; ++i) {
+      x->columns[0]->i64[i] = m * 2048 + i;
+      x->columns[1]->f64[i] = (i % 10) - 4.5;
+    }
+    src.push_back(x);
+  }
+  for (int workers : {0, 4}) {
+    BatchSource s(src);
+    CollectSink sink;
+    KernelFilter f("f", CompareOp::Gt, -0.5);
+    Pipeline p{&s, {&f}, &sink, workers};
+    OK(p.Run(ctx));
+    int want = 0;
+    for (int i = 0; i < 2048; ++i) want += (i % 10) - 4.5 > -0.5;
+    EXPECT(sink.rows() == 8 * want, "pipeline rows %d (workers %d)", sink.rows(), workers);
+  }
+  // cancellation between batches (pipeline.go:268-270)
+  {
+    BatchSource s(src);
+    CollectSink sink;
+    Context cancelled;
+    cancelled.cancelled = true;
+    Pipeline p{&s, {}, &sink, 0};
+    EXPECT(!p.Run(cancelled).ok(), "cancelled context must stop the pipeline");
+  }
+}
+
+// ----------------------------------------------------------------- GPU tests
+
+static void test_hash_aggregate_kat(wg_ctx* g) {
+  // exec_test.go:39-96 with int keys: group 1 = "a", 2 = "b"; SUM(amount) a = 40, COUNT(amount) a = 2
+  auto b = make_batch({{"group", TypeID::Int64}, {"amount", TypeID::Float64}}, 4);
+  const int64_t gk[] = {1, 2, 1, 2};
+  const double am[] = {10, 20, 30, 40};
+  for (int i = 0; i < 4; ++i) {
+    b->columns[0]->i64[i] = gk[i];
+    b->columns[1]->f64[i] = am[i];
+  }
+  GpuHashAggregate agg(g, {"group"}, {{WG_AGG_SUM, "amount", "total"}, {WG_AGG_COUNT, "amount", "cnt"}});
+  BatchSource src({b});
+  Context ctx;
+  Pipeline p{&src, {}, &agg, 0};
+  OK(p.Run(ctx));
+  BatchPtr out;
+  OK(agg.Next(ctx, &out));
+  EXPECT(out && out->len == 2, "expected 2 groups");
+  bool seen = false;
+  for (int i = 0; out && i < out->len; ++i)
+    if (out->columns[0]->i64[i] == 1) {
+      seen = true;
+      EXPECT(out->columns[1]->f64[i] == 40.0, "total %f", out->columns[1]->f64[i]);
+      EXPECT(out->columns[2]->i64[i] == 2, "cnt %lld", (long long)out->columns[2]->i64[i]);
+    }
+  EXPECT(seen, "group a missing");
+  BatchPtr eof;
+  OK(agg.Next(ctx, &eof));
+  EXPECT(!eof, "Next after the last batch returns nullptr");
+}
+
+static void test_parallel_pushdown_equivalence(wg_ctx* g) {
+  // Workers=4 with CloneSink/MergeSink and pushed-down filters == serial, CPU-filtered
+  std::vector<BatchPtr> src;
+  std::map<int64_t, std::pair<int64_t, int64_t>> want;  // k -> (sum v, count)
+  for (int m = 0; m < 40; ++m) {
+    auto x = make_batch({{"k", TypeID::Int32}, {"v", TypeID::Int64}, {"f", TypeID::Float64}}, 2048);
+    for (int i = 0; i < 2048; ++i) {
+      const int64_t r = m * 2048 + i;
+      x->columns[0]->i32[i] = (int32_t)((r * 7919) % 1000);
+      x->columns[1]->i64[i] = r * 3 - 1000;
+      x->columns[2]->f64[i] = std::sin((double)r);
+      if (x->columns[2]->f64[i] > -0.5) {
+        auto& w = want[x->columns[0]->i32[i]];
+        w.first += x->columns[1]->i64[i];
+        w.second += 1;
+      }
+    }
+    src.push_back(x);
+  }
+  for (int mode = 0; mode < 2; ++mode) {
+    BatchSource s(src);
+    GpuHashAggregate agg(g, {"k"}, {{WG_AGG_SUM, "v", "s"}, {WG_AGG_COUNT, "", "c"}, {WG_AGG_MIN, "k", "mn"}});
+    KernelFilter f("f", CompareOp::Gt, -0.5);
+    Pipeline p{&s, {&f}, &agg, mode == 0 ? 0 : 4};
+    if (mode == 1) PushDownFiltersIntoGpuAggregate(p);
+    Context ctx;
+    OK(p.Run(ctx));
+    int groups = 0;
+    for (;;) {
+      BatchPtr out;
+      OK(agg.Next(ctx, &out));
+      if (!out) break;
+      EXPECT(out->schema[3].type == TypeID::Int64, "MIN(int32) emits Int64 (aggregate.go:4636)");
+      for (int i = 0; i < out->len; ++i, ++groups) {
+        const int64_t k = out->columns[0]->i32[i];
+        EXPECT(want.count(k), "unexpected key %lld", (long long)k);
+        EXPECT((int64_t)out->columns[1]->f64[i] == want[k].first, "sum of %lld", (long long)k);
+        EXPECT(out->columns[2]->i64[i] == want[k].second, "count of %lld", (long long)k);
+        EXPECT(out->columns[3]->i64[i] == k, "min of %lld", (long long)k);
+      }
+    }
+    EXPECT(groups == (int)want.size(), "groups %d want %zu (mode %d)", groups, want.size(), mode);
+  }
+}
+
+static void test_hash_join_kats(wg_ctx* g) {
+  Context ctx;
+  // join_test.go:10-56 (strings coded as ints): 3 rows, alice twice
+  {
+    auto build = make_batch({{"user_id", TypeID::Int64}, {"name", TypeID::Int64}}, 3);
+    const int64_t bk[] = {1, 2, 3}, nm[] = {100, 200, 300};
+    for (int i = 0; i < 3; ++i) {
+      build->columns[0]->i64[i] = bk[i];
+      build->columns[1]->i64[i] = nm[i];
+    }
+    auto probe = make_batch({{"user_id", TypeID::Int64}, {"amount", TypeID::Float64}}, 4);
+    const int64_t pk[] = {1, 2, 1, 9};
+    for (int i = 0; i < 4; ++i) {
+      probe->columns[0]->i64[i] = pk[i];
+      probe->columns[1]->f64[i] = 10.0 * i;
+    }
+    GpuHashJoin hj(g, "user_id", "user_id");
+    BatchSource bs({build});
+    OK(hj.Build(ctx, &bs));
+    auto pr = hj.Probe();
+    BatchSource ps({probe});
+    CollectSink sink;
+    Pipeline p{&ps, {pr.get()}, &sink, 0};
+    OK(p.Run(ctx));
+    EXPECT(sink.rows() == 3, "inner join rows %d, want 3", sink.rows());
+    for (auto& ob : sink.batches)
+      for (int i = 0; i < ob->len; ++i)
+        EXPECT(ob->columns[0]->i64[i] == ob->columns[2]->i64[i], "joined keys differ");
+  }
+  // join_coverage_test.go:101-141: int key path, 1 row
+  {
+    auto build = make_batch({{"id", TypeID::Int64}}, 2);
+    build->columns[0]->i64 = {1, 2};
+    auto probe = make_batch({{"id", TypeID::Int64}, {"amount", TypeID::Float64}}, 2);
+    probe->columns[0]->i64 = {1, 3};
+    GpuHashJoin hj(g, "id", "id");
+    BatchSource bs({build});
+    OK(hj.Build(ctx, &bs));
+    auto pr = hj.Probe();
+    BatchSource ps({probe});
+    CollectSink sink;
+    Pipeline p{&ps, {pr.get()}, &sink, 0};
+    OK(p.Run(ctx));
+    EXPECT(sink.rows() == 1, "int key path rows %d, want 1", sink.rows());
+  }
+  // bounded output: one probe row fans out to 40000 build rows > MaxProbeOutputRows
+  {
+    auto build = make_batch({{"k", TypeID::Int32}, {"v", TypeID::Int64}}, 40000);
+    for (int i = 0; i < 40000; ++i) {
+      build->columns[0]->i32[i] = i < 39990 ? 7 : i;
+      build->columns[1]->i64[i] = i;
+    }
+    auto probe = make_batch({{"k", TypeID::Int64}}, 3);
+    probe->columns[0]->i64 = {7, 39995, 5};
+    GpuHashJoin hj(g, "k", "k");
+    BatchSource bs({build});
+    OK(hj.Build(ctx, &bs));
+    auto pr = hj.Probe();
+    BatchSource ps({probe});
+    CollectSink sink;
+    Pipeline p{&ps, {pr.get()}, &sink, 0};
+    OK(p.Run(ctx));
+    EXPECT(sink.rows() == 39991, "fan-out rows %d, want 39991", sink.rows());
+    size_t max_batch = 0;
+    for (auto& ob : sink.batches) max_batch = std::max<size_t>(max_batch, (size_t)ob->len);
+    EXPECT(max_batch <= kMaxProbeOutputRows, "probe output batch %zu exceeds the bound", max_batch);
+    EXPECT(sink.batches.size() >= 2, "bounded output must span several NextOutput calls");
+  }
+}
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  test_kernel_filter_and_pipeline();
+  if (mode == "gpu") {
+    wg_ctx* g = nullptr;
+    if (wg_ctx_create(0, &g) != WG_OK) {
+      std::printf("FAIL: no device: %s\n", wg_last_error());
+      return 2;
+    }
+    test_hash_aggregate_kat(g);
+    test_parallel_pushdown_equivalence(g);
+    test_hash_join_kats(g);
+    wg_ctx_destroy(g);
+  }
+  std::printf("%s: %d failure(s)\n", mode.c_str(), g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -56,6 +56,18 @@ def test_batched_equals_plain_equals_oracle(ctx):
         same(plain[q], ref, q)
 
 
+def test_graph_replay_after_the_work_queue_grows(ctx):
+    """A replayed graph must never see work-queue buffers that a bigger batch
+    reallocated: small batch, then a batch with many more tiles, both graphed."""
+    b = Batch(ctx, 96, 200_000)
+    for n_q, seed in ((4, 11), (80, 12), (8, 13)):
+        plan = W.plan_config4(n_queries=n_q, seed=seed, max_rows=200_000, max_groups=2_000)
+        data = W.gen_config4(ctx, plan)
+        b.run(Batch.queries(data), len(plan), use_graph=True)
+        for q, (kc, vc, n, g) in enumerate(data):
+            same(b.result(q), oracle_query(kc, vc, n), q)
+
+
 def test_autotune_keeps_results(ctx):
     plan = W.plan_config4(n_queries=24, seed=6, max_rows=200_000, max_groups=50_000)
     data = W.gen_config4(ctx, plan)

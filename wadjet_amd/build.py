@@ -55,9 +55,24 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host_layer()
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
     return LIB
+
+
+HOST_TEST = LIBDIR / "test_exec"
+
+
+def build_host_layer() -> None:
+    """The C++ operator layer (host/exec.cpp) and its test program, linked
+    against the library with an $ORIGIN rpath so the pair travels together."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-pthread",
+           str(PKG / "host" / "exec.cpp"), str(ROOT / "tests" / "cpp" / "test_exec.cpp"),
+           f"-L{LIBDIR}", "-lwadjet_gpu", "-Wl,-rpath,$ORIGIN", "-o", str(HOST_TEST)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host layer build failed:\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -49,20 +49,15 @@ template <bool USE_LDS>
 __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
   extern __shared__ uint64_t lds[];
   __shared__ int64_t s_tile;
-  __shared__ uint32_t s_lds_rows, s_glb_rows;
   const int stride = P.pg.stride;
   const int L = USE_LDS ? P.lds_slots : 0;
 
   if (USE_LDS) {
     for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = slot_init_word(P.pg, i % stride);
   }
-  if (threadIdx.x == 0) {
-    s_lds_rows = 0;
-    s_glb_rows = 0;
-  }
   __syncthreads();
 
-  uint32_t my_lds = 0, my_glb = 0;
+  uint32_t my_lds = 0, my_glb = 0, my_claims = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       int64_t t = -1;
@@ -103,7 +98,7 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
             continue;
           }
         }
-        slot = glb_find_or_claim(P.T, stride, key, h);
+        slot = glb_find_or_claim(P.T, stride, key, h, my_claims);
         if (slot == nullptr) {
           uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           P.ovf_rows[at] = (uint32_t)row;
@@ -123,7 +118,7 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
       const uint64_t key = ls[0];
       if (key == kEmpty) continue;
       ++flushed;
-      uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key));
+      uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key), my_claims);
       if (slot != nullptr) {
         merge_partial(slot, ls, P.pg);
       } else {
@@ -132,16 +127,12 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
         for (int w = 0; w < stride; ++w) rec[w] = ls[w];
       }
     }
-    if (flushed) at_add<GlbT>(P.T.counters + CTR_FLUSHED, flushed);
+    block_add(P.T.counters + CTR_FLUSHED, flushed);
   }
-  // wave-reduced statistics
-  atomicAdd(&s_lds_rows, my_lds);
-  atomicAdd(&s_glb_rows, my_glb);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_lds_rows) at_add<GlbT>(P.T.counters + CTR_LDS_ROWS, s_lds_rows);
-    if (s_glb_rows) at_add<GlbT>(P.T.counters + CTR_GLB_ROWS, s_glb_rows);
-  }
+  // one atomic per workgroup for each statistic and for the claimed groups
+  block_add(P.T.counters + CTR_GROUPS, my_claims);
+  block_add(P.T.counters + CTR_LDS_ROWS, my_lds);
+  block_add(P.T.counters + CTR_GLB_ROWS, my_glb);
 }
 
 // Fill every slot (and the side slots) with the empty pattern.
@@ -162,6 +153,7 @@ __device__ __forceinline__ bool slot_live(const uint64_t* s, bool side) {
 __global__ void agg_rehash_kernel(const uint64_t* src, uint64_t src_cap, const uint64_t* src_base,
                                   GlobalTable T, uint64_t* dst_base, int stride, int n_base_words) {
   const uint64_t total = src_cap + kSideSlots;
+  uint32_t claims = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
     const bool side = i >= src_cap;
     const uint64_t* s = src + i * stride;
@@ -172,22 +164,27 @@ __global__ void agg_rehash_kernel(const uint64_t* src, uint64_t src_cap, const u
       di = (T.mask + 1) + (i - src_cap);
       d = T.slots + di * stride;
     } else {
-      d = glb_find_or_claim(T, stride, s[0], key_hash(s[0]));
-      if (d == nullptr) continue;  // cannot happen: dst is sized for all of src
+      d = glb_find_or_claim(T, stride, s[0], key_hash(s[0]), claims);
+      if (d == nullptr) {  // the new table is sized for all of src: a probe run this long is a bug
+        at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
+        continue;
+      }
       di = (uint64_t)(d - T.slots) / stride;
     }
     for (int w = 1; w < stride; ++w) d[w] = s[w];
     if (src_base != nullptr)
       for (int w = 0; w < n_base_words; ++w) dst_base[di * n_base_words + w] = src_base[i * n_base_words + w];
   }
+  block_add(T.counters + CTR_GROUPS, claims);
 }
 
 // Merge overflow partial records (possibly several per key) with atomics.
 __global__ void agg_merge_part_kernel(const uint64_t* recs, uint64_t n, GlobalTable T, AggProgram pg) {
   const int stride = pg.stride;
+  uint32_t claims = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t* r = recs + i * stride;
-    uint64_t* d = glb_find_or_claim(T, stride, r[0], key_hash(r[0]));
+    uint64_t* d = glb_find_or_claim(T, stride, r[0], key_hash(r[0]), claims);
     if (d == nullptr) {
       // the host grows the table before replaying partials, so this is a bug
       at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
@@ -195,6 +192,7 @@ __global__ void agg_merge_part_kernel(const uint64_t* recs, uint64_t n, GlobalTa
     }
     merge_partial(d, r, pg);
   }
+  block_add(T.counters + CTR_GROUPS, claims);
 }
 
 // Exact Int128 value of a limb group (2 or 4 carry-save limbs).
@@ -241,6 +239,7 @@ __global__ void agg_merge_table_kernel(const uint64_t* src, uint64_t src_cap, co
                                        const int32_t* limb_word, const int32_t* limb_n, int n_sumd) {
   const int stride = pg.stride;
   const uint64_t total = src_cap + kSideSlots;
+  uint32_t claims = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
     const bool side = i >= src_cap;
     const uint64_t* s = src + i * stride;
@@ -251,7 +250,7 @@ __global__ void agg_merge_table_kernel(const uint64_t* src, uint64_t src_cap, co
       di = (T.mask + 1) + (i - src_cap);
       d = T.slots + di * stride;
     } else {
-      d = glb_find_or_claim(T, stride, s[0], key_hash(s[0]));
+      d = glb_find_or_claim(T, stride, s[0], key_hash(s[0]), claims);
       if (d == nullptr) {
         at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
         continue;
@@ -289,41 +288,63 @@ __global__ void agg_merge_table_kernel(const uint64_t* src, uint64_t src_cap, co
       b[1] = (uint64_t)(v >> 64);
     }
   }
+  block_add(T.counters + CTR_GROUPS, claims);
 }
 
 // Compact live slots into a dense array (slot words + base words), order
 // unspecified; out_null[pos] = 1 for the NULL-key group (side slot 1).
-// out_n is a device counter.
-__global__ void agg_compact_kernel(const uint64_t* slots, uint64_t cap, const uint64_t* base, int stride,
-                                   int n_base_words, uint64_t* out, uint64_t* out_base, uint8_t* out_null,
-                                   uint64_t* out_n) {
+// Each workgroup owns a contiguous slot range: it counts its live slots,
+// reserves its output range with ONE atomic, and places slots by a block
+// prefix sum (wave ballot + per-wave totals).
+constexpr int kCompactBlock = 256;
+
+__global__ void __launch_bounds__(kCompactBlock) agg_compact_kernel(const uint64_t* slots, uint64_t cap,
+                                                                   const uint64_t* base, int stride, int n_base_words,
+                                                                   uint64_t* out, uint64_t* out_base,
+                                                                   uint8_t* out_null, uint64_t* out_n,
+                                                                   uint64_t per_block) {
+  __shared__ uint32_t wave_cnt[kCompactBlock / 64];
+  __shared__ unsigned long long s_base;
   const uint64_t total = cap + kSideSlots;
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-  // every lane of a wave runs the same number of iterations (ballot below)
-  const uint64_t iters = (total + step - 1) / step;
-  for (uint64_t it = 0; it < iters; ++it) {
-    const uint64_t i = it * step + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t beg = (uint64_t)blockIdx.x * per_block;
+  const uint64_t end = min(beg + per_block, total);
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  // pass 1: count this block's live slots
+  uint32_t mine = 0;
+  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) mine += slot_live(slots + i * stride, i >= cap);
+  __shared__ unsigned long long s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  if (mine) atomicAdd(&s_cnt, (unsigned long long)mine);
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd((unsigned long long*)out_n, s_cnt) : 0;
+  __syncthreads();
+  // pass 2: place live slots in block order
+  uint64_t run = s_base;
+  for (uint64_t i0 = beg; i0 < end; i0 += blockDim.x) {
+    const uint64_t i = i0 + threadIdx.x;
     const bool side = i >= cap;
-    const uint64_t* s = slots + i * stride;
-    const bool live = i < total && slot_live(s, side);
-    // wave-aggregated reservation: one atomic per wave
-    const uint64_t ballot = __ballot(live);
-    if (ballot == 0) continue;
-    const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)ballot) - 1;
-    unsigned long long basepos = 0;
-    if (lane == leader) basepos = atomicAdd((unsigned long long*)out_n, (unsigned long long)__popcll(ballot));
-    basepos = __shfl(basepos, leader);
-    if (!live) continue;
-    const uint64_t pos = basepos + __popcll(ballot & ((1ull << lane) - 1));
-    uint64_t* o = out + pos * stride;
-    for (int w = 0; w < stride; ++w) o[w] = s[w];
-    if (side && i == cap) o[0] = kEmpty;  // the key whose value is the sentinel
-    out_null[pos] = (side && i == cap + 1) ? 1 : 0;
-    if (out_base != nullptr) {
-      for (int w = 0; w < n_base_words; ++w)
-        out_base[pos * n_base_words + w] = base ? base[i * n_base_words + w] : 0;
+    const bool live = i < end && slot_live(slots + i * stride, side);
+    const uint64_t b = __ballot(live);
+    if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int w = 0; w < kCompactBlock / 64; ++w) {
+      if (w < wave) before += wave_cnt[w];
+      tot += wave_cnt[w];
     }
+    if (live) {
+      const uint64_t pos = run + before + __popcll(b & ((1ull << lane) - 1));
+      const uint64_t* s = slots + i * stride;
+      uint64_t* o = out + pos * stride;
+      for (int w = 0; w < stride; ++w) o[w] = s[w];
+      if (side && i == cap) o[0] = kEmpty;  // the key whose value is the sentinel
+      out_null[pos] = (side && i == cap + 1) ? 1 : 0;
+      if (out_base != nullptr)
+        for (int w = 0; w < n_base_words; ++w) out_base[pos * n_base_words + w] = base ? base[i * n_base_words + w] : 0;
+    }
+    run += tot;
+    __syncthreads();
   }
 }
 
@@ -653,7 +674,7 @@ GlobalTable table_of(wg_agg* h) {
   T.slots = h->slots;
   T.mask = h->cap - 1;
   T.counters = h->counters;
-  T.max_groups = h->cap - h->cap / 8;  // 87.5 % hard load limit
+  T.max_probe = kMaxProbe;
   return T;
 }
 
@@ -670,7 +691,7 @@ wg_status grow_to(wg_agg* h, uint64_t new_cap) {
   // the rehash claims slots: reset the groups counter first
   WG_TRY(zero_dev(h->ctx, h->counters + CTR_GROUPS, 1));
   GlobalTable T = table_of(h);
-  T.max_groups = new_cap;  // never refuse while moving
+  T.max_probe = new_cap;  // never refuse while moving into the larger table
   const uint64_t total = old_cap + kSideSlots;
   hipLaunchKernelGGL(agg_rehash_kernel, dim3(grid_for(total, 256, 8192)), dim3(256), 0, h->ctx->stream, old, old_cap,
                      old_base, T, h->base, h->pg.stride, h->n_sumd * 2);
@@ -806,7 +827,7 @@ static wg_status resolve_overflow(wg_agg* h, const ColSet& cs, const PredDesc* p
     WG_HIP(hipMalloc(&recs, bytes));
     WG_HIP(hipMemcpyAsync(recs, h->ovf_part, bytes, hipMemcpyDeviceToDevice, h->ctx->stream));
     GlobalTable T = table_of(h);
-    T.max_groups = h->cap;
+    T.max_probe = h->cap;
     hipLaunchKernelGGL(agg_merge_part_kernel, dim3(grid_for(n_part, 256)), dim3(256), 0, h->ctx->stream, recs, n_part,
                        T, h->pg);
     WG_HIP(hipGetLastError());
@@ -1132,7 +1153,7 @@ wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
   WG_TRY(agg_ensure_capacity(dst, src->groups));
   WG_TRY(ensure_base(dst));
   GlobalTable T = table_of(dst);
-  T.max_groups = dst->cap;
+  T.max_probe = dst->cap;
   WG_TRY(zero_dev(dst->ctx, dst->counters + CTR_STOPPED, 1));
   const uint64_t total = src->cap + kSideSlots;
   hipLaunchKernelGGL(agg_merge_table_kernel, dim3(grid_for(total, 256, 8192)), dim3(256), 0, dst->ctx->stream,
@@ -1163,9 +1184,11 @@ wg_status wg_agg_finalize(wg_agg* h) {
   uint64_t* out_n = h->counters + CTR_LDS_ROWS;  // reuse a counter word
   WG_TRY(zero_dev(h->ctx, out_n, 1));
   const uint64_t total = h->cap + kSideSlots;
-  hipLaunchKernelGGL(agg_compact_kernel, dim3(grid_for(total, 256, 8192)), dim3(256), 0, h->ctx->stream, h->slots,
+  const uint64_t per_block = std::max<uint64_t>(4096, (total + 8191) / 8192);
+  const uint64_t blocks = (total + per_block - 1) / per_block;
+  hipLaunchKernelGGL(agg_compact_kernel, dim3((unsigned)blocks), dim3(kCompactBlock), 0, h->ctx->stream, h->slots,
                      h->cap, (const uint64_t*)h->base, h->pg.stride, h->n_sumd * 2, h->dense, h->dense_base,
-                     h->dense_null, out_n);
+                     h->dense_null, out_n, per_block);
   WG_HIP(hipGetLastError());
   uint64_t c[CTR_N];
   WG_TRY(agg_read_counters(h, c));

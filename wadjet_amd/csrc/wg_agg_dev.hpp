@@ -242,11 +242,11 @@ struct GlobalTable {
   uint64_t* slots;      // cap * stride words, then kSideSlots side slots
   uint64_t mask;        // cap - 1
   uint64_t* counters;   // see Ctr
-  uint64_t max_groups;  // claims beyond this overflow (the host grows the table)
+  uint64_t max_probe;   // a claim farther than this from the key's home slot overflows
 };
 
 enum Ctr : int {
-  CTR_GROUPS = 0,    // slots claimed
+  CTR_GROUPS = 0,    // slots claimed (summed once per workgroup)
   CTR_OVF_ROWS = 1,  // rows appended to the overflow row list
   CTR_OVF_PART = 2,  // partial records appended to the overflow partial list
   CTR_TILE = 3,      // next tile (dynamic scheduling)
@@ -257,24 +257,30 @@ enum Ctr : int {
   CTR_N = 8
 };
 
+// Linear probing stops after this many slots; a table that needs more is
+// past its load budget and the host grows it (the claim reports overflow).
+constexpr uint64_t kMaxProbe = 512;
+
 __device__ __forceinline__ uint64_t key_hash(uint64_t key) { return mix64(key); }
 
 // Find the slot of `key`, claiming a free one with a 64-bit CAS. Returns the
-// slot pointer, or nullptr when the group budget is exhausted / the table is
-// full (the caller records the row or partial as overflow).
-__device__ __forceinline__ uint64_t* glb_find_or_claim(const GlobalTable& T, int stride, uint64_t key,
-                                                      uint64_t h) {
+// slot pointer (claimed = 1 when this call took a free slot), or nullptr when
+// no free slot lies within max_probe (the caller records the row or partial
+// as overflow). Probe reads are workgroup-scope loads: a stale EMPTY is
+// resolved by the CAS, and a slot never changes once it holds a key.
+// No shared counter is touched here: callers sum `claimed` per workgroup.
+__device__ __forceinline__ uint64_t* glb_find_or_claim(const GlobalTable& T, int stride, uint64_t key, uint64_t h,
+                                                      uint32_t& claimed) {
   uint64_t idx = h & T.mask;
-  for (uint64_t p = 0; p <= T.mask; ++p) {
+  const uint64_t lim = T.max_probe < T.mask + 1 ? T.max_probe : T.mask + 1;
+  for (uint64_t p = 0; p < lim; ++p) {
     uint64_t* s = T.slots + idx * (uint64_t)stride;
-    uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == key) return s;
     if (k == kEmpty) {
-      if (__hip_atomic_load(T.counters + CTR_GROUPS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= T.max_groups)
-        return nullptr;
-      uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
+      const uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
       if (prev == kEmpty) {
-        at_add<GlbT>(T.counters + CTR_GROUPS, 1);
+        ++claimed;
         return s;
       }
       if (prev == key) return s;
@@ -282,6 +288,18 @@ __device__ __forceinline__ uint64_t* glb_find_or_claim(const GlobalTable& T, int
     idx = (idx + 1) & T.mask;
   }
   return nullptr;
+}
+
+// Sum a per-thread count over the workgroup and add it to *dst with one
+// atomic (a per-row atomic on one shared address serialises the whole grid).
+__device__ __forceinline__ void block_add(uint64_t* dst, uint64_t v) {
+  __shared__ unsigned long long s_sum;
+  __syncthreads();
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  if (v) atomicAdd(&s_sum, (unsigned long long)v);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sum) at_add<GlbT>(dst, s_sum);
 }
 
 // Per-workgroup LDS table: `lds_slots` slots of `stride` words.

@@ -34,7 +34,7 @@ struct BQ {
   ColDesc val;
   uint64_t* table;
   uint64_t mask;
-  uint64_t max_groups;
+  uint64_t max_probe;  // a claim farther than this from the key's home slot overflows
   uint32_t lds_slots;  // 0 = global table only
   int32_t dim_word;    // >= 0: group key = this word of the matched dimension slot
   ColDesc range;       // data == null: no window filter
@@ -65,21 +65,17 @@ struct BatchParams {
 
 __device__ __forceinline__ uint64_t bq_key(const BQ& q, uint64_t row) { return (uint64_t)load_int(q.key, row); }
 
-__device__ __forceinline__ uint64_t* bq_slot(const BQ& q, BCounters* c, uint64_t key) {
+__device__ __forceinline__ uint64_t* bq_slot(const BQ& q, uint64_t key) {
   if (key == kEmpty) return q.table + (q.mask + 1) * kBStride;  // side slot
   uint64_t idx = mix64(key) & q.mask;
-  for (uint64_t p = 0; p <= q.mask; ++p) {
+  const uint64_t lim = q.max_probe < q.mask + 1 ? q.max_probe : q.mask + 1;
+  for (uint64_t p = 0; p < lim; ++p) {
     uint64_t* s = q.table + idx * kBStride;
-    const uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == key) return s;
     if (k == kEmpty) {
-      if (__hip_atomic_load(&c->groups, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q.max_groups) return nullptr;
       const uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
-      if (prev == kEmpty) {
-        atomicAdd(&c->groups, 1ull);
-        return s;
-      }
-      if (prev == key) return s;
+      if (prev == kEmpty || prev == key) return s;
     }
     idx = (idx + 1) & q.mask;
   }
@@ -135,9 +131,9 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
           continue;
         }
       }
-      uint64_t* s = bq_slot(q, c, key);
+      uint64_t* s = bq_slot(q, key);
       if (!s) {
-        atomicAdd(&c->overflow, 1ull);
+        atomicAdd(&c->overflow, 1ull);  // rare: the query is re-run alone
         continue;
       }
       bq_apply<GlbT>(s, v);
@@ -147,7 +143,7 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
       for (uint32_t i = threadIdx.x; i < L; i += blockDim.x) {
         const uint64_t* ls = lds + (uint64_t)i * kBStride;
         if (ls[0] == kEmpty) continue;
-        uint64_t* s = bq_slot(q, c, ls[0]);
+        uint64_t* s = bq_slot(q, ls[0]);
         if (!s) {
           atomicAdd(&c->overflow, 1ull);
           continue;
@@ -180,35 +176,58 @@ struct CompactParams {
   int64_t* counts;
 };
 
-__global__ void batch_compact_kernel(CompactParams P) {
+__global__ void __launch_bounds__(256) batch_compact_kernel(CompactParams P) {
   __shared__ int64_t s_item;
+  __shared__ uint32_t wave_cnt[4];
+  __shared__ unsigned long long s_cnt, s_base;
   const uint64_t n_work = *P.n_work;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
   for (;;) {
     if (threadIdx.x == 0) {
       const unsigned long long w = atomicAdd(P.work_ctr, 1ull);
       s_item = w < n_work ? (int64_t)w : -1;
+      s_cnt = 0;
     }
     __syncthreads();
     const int64_t item = s_item;
-    __syncthreads();
     if (item < 0) break;
     const CWork W = P.work[item];
     const BQ q = P.qs[W.q];
-    for (uint64_t i = W.beg + threadIdx.x; i < W.end; i += blockDim.x) {
+    auto live_at = [&](uint64_t i) -> bool {
+      if (i >= W.end) return false;
       const uint64_t* s = q.table + i * kBStride;
-      const bool side = i == q.mask + 1;
-      if (side ? s[1] == 0 : s[0] == kEmpty) continue;
-      const uint64_t pos = W.out_off + atomicAdd(&P.ctr[W.q].out_n, 1ull);
-      P.keys[pos] = (int64_t)s[0];
-      uint64_t lo;
-      int64_t hi;
-      const uint64_t limbs[2] = {s[2], s[3]};
-      unsigned __int128 acc = (unsigned __int128)limbs[0] + ((unsigned __int128)(__int128)(int64_t)limbs[1] << 32);
-      lo = (uint64_t)acc;
-      hi = (int64_t)(uint64_t)(acc >> 64);
-      P.sum_lo[pos] = lo;
-      P.sum_hi[pos] = hi;
-      P.counts[pos] = (int64_t)s[1];
+      return i == q.mask + 1 ? s[1] != 0 : s[0] != kEmpty;
+    };
+    // count, reserve the item's output range with one atomic, then place
+    uint32_t mine = 0;
+    for (uint64_t i = W.beg + threadIdx.x; i < W.end; i += blockDim.x) mine += live_at(i);
+    if (mine) atomicAdd(&s_cnt, (unsigned long long)mine);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(&P.ctr[W.q].out_n, s_cnt) : 0;
+    __syncthreads();
+    uint64_t run = W.out_off + s_base;
+    for (uint64_t i0 = W.beg; i0 < W.end; i0 += blockDim.x) {
+      const uint64_t i = i0 + threadIdx.x;
+      const bool live = live_at(i);
+      const uint64_t b = __ballot(live);
+      if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(b);
+      __syncthreads();
+      uint32_t before = 0, tot = 0;
+      for (int w = 0; w < 4; ++w) {
+        if (w < wave) before += wave_cnt[w];
+        tot += wave_cnt[w];
+      }
+      if (live) {
+        const uint64_t pos = run + before + __popcll(b & ((1ull << lane) - 1));
+        const uint64_t* s = q.table + i * kBStride;
+        P.keys[pos] = (int64_t)s[0];
+        const unsigned __int128 acc = (unsigned __int128)s[2] + ((unsigned __int128)(__int128)(int64_t)s[3] << 32);
+        P.sum_lo[pos] = (uint64_t)acc;
+        P.sum_hi[pos] = (int64_t)(uint64_t)(acc >> 64);
+        P.counts[pos] = (int64_t)s[1];
+      }
+      run += tot;
+      __syncthreads();
     }
   }
 }
@@ -250,9 +269,14 @@ struct BatchArena {
   int64_t* d_counts = nullptr;
   void* pinned = nullptr;  // descriptor staging + counter read-back
   uint64_t pinned_bytes = 0;
+  // A captured graph holds the device pointers above as kernel arguments:
+  // every reallocation bumps `gen`, and a graph of an older generation is
+  // re-captured before it is replayed.
+  uint64_t gen = 0;
   hipGraphExec_t graph = nullptr;
   int32_t graph_block = 0;
   uint32_t graph_lds = 0;
+  uint64_t graph_gen = ~0ull;
 };
 
 struct Plan {
@@ -291,11 +315,12 @@ struct wg_batch {
 
 namespace wg {
 
-static wg_status ensure_dev(void** p, uint64_t& cap, uint64_t need, uint64_t elem) {
+static wg_status ensure_dev(void** p, uint64_t& cap, uint64_t need, uint64_t elem, uint64_t& gen) {
   if (need <= cap && *p) return WG_OK;
   if (*p) WG_HIP(hipFree(*p));
   cap = std::max<uint64_t>(need, cap * 2);
   WG_HIP(hipMalloc(p, cap * elem));
+  ++gen;  // invalidates any graph that captured the old pointer
   return WG_OK;
 }
 
@@ -308,7 +333,7 @@ static wg_status arena_init(BatchArena& A, uint64_t slots, uint64_t max_q) {
   WG_HIP(hipMalloc(&A.d_sum_lo, slots * 8));
   WG_HIP(hipMalloc(&A.d_sum_hi, slots * 8));
   WG_HIP(hipMalloc(&A.d_counts, slots * 8));
-  WG_TRY(ensure_dev((void**)&A.d_qs, A.q_cap, max_q, sizeof(BQ)));
+  WG_TRY(ensure_dev((void**)&A.d_qs, A.q_cap, max_q, sizeof(BQ), A.gen));
   WG_HIP(hipMalloc(&A.d_ctr, max_q * sizeof(BCounters)));
   return WG_OK;
 }
@@ -358,7 +383,7 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
       d.dim_word = 2 + q.dim_payload;
     }
     d.mask = cap - 1;
-    d.max_groups = cap - cap / 8;
+    d.max_probe = kMaxProbe;
     P.out_off[i] = slot_off;  // output region = the query's slot range
     slot_off += cap + 1;
     const bool lds = b->lds_threshold > 0 && est <= b->lds_threshold;
@@ -392,9 +417,9 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
 
 static wg_status upload(wg_ctx* ctx, BatchArena& A, const Plan& P) {
   const int n = (int)P.qs.size();
-  WG_TRY(ensure_dev((void**)&A.d_qs, A.q_cap, std::max(n, 1), sizeof(BQ)));
-  WG_TRY(ensure_dev((void**)&A.d_work, A.work_cap, std::max<size_t>(P.work.size(), 1), sizeof(BWork)));
-  WG_TRY(ensure_dev((void**)&A.d_cwork, A.cwork_cap, std::max<size_t>(P.cwork.size(), 1), sizeof(CWork)));
+  WG_TRY(ensure_dev((void**)&A.d_qs, A.q_cap, std::max(n, 1), sizeof(BQ), A.gen));
+  WG_TRY(ensure_dev((void**)&A.d_work, A.work_cap, std::max<size_t>(P.work.size(), 1), sizeof(BWork), A.gen));
+  WG_TRY(ensure_dev((void**)&A.d_cwork, A.cwork_cap, std::max<size_t>(P.cwork.size(), 1), sizeof(CWork), A.gen));
   const uint64_t bytes = n * sizeof(BQ) + P.work.size() * sizeof(BWork) + P.cwork.size() * sizeof(CWork) + 64 +
                          n * sizeof(BCounters);
   if (bytes > A.pinned_bytes) {
@@ -443,8 +468,9 @@ static wg_status launch(wg_ctx* ctx, BatchArena& A, int32_t block, uint32_t lds_
     attr_done = true;
   }
   if (use_graph) {
-    // the graph is specialised on (workgroup size, LDS bytes): re-capture on change
-    if (!A.graph || A.graph_block != block || A.graph_lds != lds_bytes) {
+    // the graph is specialised on (workgroup size, LDS bytes) and holds the
+    // descriptor pointers of its generation: re-capture when any changed
+    if (!A.graph || A.graph_block != block || A.graph_lds != lds_bytes || A.graph_gen != A.gen) {
       if (A.graph) WG_HIP(hipGraphExecDestroy(A.graph));
       A.graph = nullptr;
       hipGraph_t g;
@@ -455,6 +481,7 @@ static wg_status launch(wg_ctx* ctx, BatchArena& A, int32_t block, uint32_t lds_
       WG_HIP(hipGraphDestroy(g));
       A.graph_block = block;
       A.graph_lds = lds_bytes;
+      A.graph_gen = A.gen;
     }
     WG_HIP(hipGraphLaunch(A.graph, ctx->stream));
   } else {
@@ -545,14 +572,10 @@ wg_status wg_batch_poll(wg_batch* b, int32_t* finished, int32_t wait) {
     *finished = 1;
     return WG_OK;
   }
-  if (wait) {
-    WG_HIP(hipEventSynchronize(b->done));
-  } else {
-    const hipError_t e = hipEventQuery(b->done);
-    if (e == hipErrorNotReady) return WG_OK;
-    WG_HIP(e);
-  }
-  b->in_flight = false;
+  hipError_t e = wait ? hipEventSynchronize(b->done) : hipEventQuery(b->done);
+  if (e == hipErrorNotReady) return WG_OK;
+  b->in_flight = false;  // finished, or failed: either way nothing is in flight any more
+  WG_HIP(e);
   const BCounters* c = counters_staging(b->main, b->plan);
   const int n = (int)b->queries.size();
   for (int i = 0; i < n; ++i) {

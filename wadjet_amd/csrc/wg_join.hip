@@ -52,13 +52,12 @@ struct BuildParams {
 __global__ void join_build_kernel(BuildParams B) {
   const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t iters = (B.n + step - 1) / step;
+  uint32_t claims = 0, nonnull = 0;
   for (uint64_t it = 0; it < iters; ++it) {
     const uint64_t j = it * step + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t row = B.row0 + j;
     const bool live = j < B.n && col_valid(B.key, row);
-    // non-NULL build rows, one atomic per wave (uniqueness check at finish)
-    const uint64_t bal = __ballot(live);
-    if (__lane_id() == 0 && bal) at_add<GlbT>(B.claimed + 1, (uint64_t)__popcll(bal));
+    nonnull += live;
     if (j >= B.n) continue;
     if (!live) {
       B.next[row] = -1;
@@ -72,10 +71,10 @@ __global__ void join_build_kernel(BuildParams B) {
       uint64_t idx = mix64(key) & B.mask;
       for (uint64_t p = 0; p <= B.mask; ++p) {
         uint64_t* c = B.slots + idx * (uint64_t)B.stride;
-        uint64_t k = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t k = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (k == kEmpty) {
           const uint64_t prev = at_cas<GlbT>(c, kEmpty, key);
-          if (prev == kEmpty) at_add<GlbT>(B.claimed, 1);
+          claims += prev == kEmpty;
           k = prev == kEmpty ? key : prev;
         }
         if (k == key) {
@@ -95,6 +94,9 @@ __global__ void join_build_kernel(BuildParams B) {
       for (int i = 0; i < B.n_inline; ++i) s[2 + i] = (uint64_t)load_int(B.inl[i], row);
     }
   }
+  // distinct keys and non-NULL rows (uniqueness check at finish), one atomic each per workgroup
+  block_add(B.claimed, claims);
+  block_add(B.claimed + 1, nonnull);
 }
 
 // Move a join table into a larger one (chains are per build row and stay).
@@ -131,10 +133,11 @@ __global__ void join_count_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, 
   }
 }
 
-// First position whose pairs do not fit in `cap` (atomicMin).
+// First position whose pairs do not fit in `cap`. off is non-decreasing, so
+// that position is the only one with off <= cap < off + cnt: one writer.
 __global__ void join_cut_kernel(const uint64_t* off, const uint64_t* cnt, uint64_t n, uint64_t cap, uint64_t* cut) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x)
-    if (off[j] + cnt[j] > cap) atomicMin((unsigned long long*)cut, (unsigned long long)j);
+    if (off[j] <= cap && off[j] + cnt[j] > cap) *cut = j;
 }
 
 __global__ void join_emit_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, uint64_t pos0, uint64_t n,
@@ -223,14 +226,8 @@ __global__ void join_digest_kernel(DigestParams D, const uint32_t* prow, const u
     a += x;
     b += y;
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    a += __shfl_down(a, o);
-    b += __shfl_down(b, o);
-  }
-  if (__lane_id() == 0) {
-    atomicAdd(out, a);
-    atomicAdd(out + 1, b);
-  }
+  block_add((uint64_t*)out, a);
+  block_add((uint64_t*)out + 1, b);
 }
 
 // ------------------------------------------------- fused probe -> aggregate
@@ -257,7 +254,7 @@ struct JoinAggParams {
 // when the global table is over its budget (the probe row is replayed).
 template <bool USE_LDS>
 __device__ __forceinline__ uint64_t* group_slot(const JoinAggParams& P, uint64_t* lds, int L, const RowRef& r,
-                                                bool& in_lds) {
+                                                bool& in_lds, uint32_t& claims) {
   uint64_t key;
   in_lds = false;
   if (!build_key<true>(P.key, P.cs, r, key)) return P.T.slots + (P.T.mask + 2) * (uint64_t)P.pg.stride;
@@ -270,7 +267,7 @@ __device__ __forceinline__ uint64_t* group_slot(const JoinAggParams& P, uint64_t
       return ls;
     }
   }
-  return glb_find_or_claim(P.T, P.pg.stride, key, h);
+  return glb_find_or_claim(P.T, P.pg.stride, key, h, claims);
 }
 
 template <bool USE_LDS>
@@ -282,6 +279,7 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
   if (USE_LDS)
     for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = slot_init_word(P.pg, i % stride);
   __syncthreads();
+  uint32_t claims = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       int64_t t = -1;
@@ -308,7 +306,7 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
       bool in_lds;
       if (cnt == 1) {
         const RowRef r{prow, (uint64_t)slot_head(js), js};
-        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds);
+        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims);
         if (!g) {
           const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           P.ovf_rows[at] = (uint32_t)prow;
@@ -322,7 +320,7 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
       bool ok = true;
       for (int32_t b = slot_head(js); b >= 0 && ok; b = P.J.next[b]) {
         const RowRef r{prow, (uint64_t)b, nullptr};
-        ok = group_slot<USE_LDS>(P, lds, L, r, in_lds) != nullptr;
+        ok = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims) != nullptr;
       }
       if (!ok) {
         const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -331,7 +329,7 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
       }
       for (int32_t b = slot_head(js); b >= 0; b = P.J.next[b]) {
         const RowRef r{prow, (uint64_t)b, nullptr};
-        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds);
+        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims);
         if (in_lds) apply_row<LdsT, true>(g, P.pg, P.cs, r);
         else apply_row<GlbT, true>(g, P.pg, P.cs, r);
       }
@@ -343,7 +341,7 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
       const uint64_t* ls = lds + (uint64_t)i * stride;
       const uint64_t key = ls[0];
       if (key == kEmpty) continue;
-      uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key));
+      uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key), claims);
       if (slot != nullptr) {
         merge_partial(slot, ls, P.pg);
       } else {
@@ -353,19 +351,22 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
       }
     }
   }
+  block_add(P.T.counters + CTR_GROUPS, claims);
 }
 
 __global__ void agg_merge_part_kernel_j(const uint64_t* recs, uint64_t n, GlobalTable T, AggProgram pg) {
   const int stride = pg.stride;
+  uint32_t claims = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t* r = recs + i * stride;
-    uint64_t* d = glb_find_or_claim(T, stride, r[0], key_hash(r[0]));
+    uint64_t* d = glb_find_or_claim(T, stride, r[0], key_hash(r[0]), claims);
     if (d == nullptr) {
       at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
       continue;
     }
     merge_partial(d, r, pg);
   }
+  block_add(T.counters + CTR_GROUPS, claims);
 }
 
 }  // namespace wg
@@ -858,7 +859,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
           WG_HIP(hipMalloc(&recs, bytes));
           WG_HIP(hipMemcpyAsync(recs, agg->ovf_part, bytes, hipMemcpyDeviceToDevice, agg->ctx->stream));
           GlobalTable T = table_of(agg);
-          T.max_groups = agg->cap;
+          T.max_probe = agg->cap;
           hipLaunchKernelGGL(agg_merge_part_kernel_j, dim3(grid_for(np, 256)), dim3(256), 0, agg->ctx->stream, recs, np,
                              T, agg->pg);
           WG_HIP(hipGetLastError());
