@@ -90,11 +90,12 @@ def main():
         step()
         log(f"[bench] warmup {i} done")
     ctx.sync()
-    kern_ms = []
+    kern_ms, passes = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
         kern_ms.append(agg.kernel_time()[0])
+        passes.append(agg.pass_times())
     ctx.sync()
     wall = time.perf_counter() - t0
     groups = agg.num_groups()
@@ -103,7 +104,8 @@ def main():
     k_ms = sum(kern_ms) / len(kern_ms)
     alg_bytes = W.CFG2_BYTES_PER_ROW * n + W.CFG2_BYTES_PER_GROUP * groups
     achieved = alg_bytes / (k_ms / 1000) / 1e9
-    log(f"[bench] {ms_step:.3f} ms/step, consume kernel {k_ms:.3f} ms, groups {groups}, stats {st}")
+    log(f"[bench] {ms_step:.3f} ms/step, consume kernels {k_ms:.3f} ms, groups {groups}, stats {st}")
+    log(f"[bench] pass times (direct, count+scan, scatter, partition-agg) last step: {passes[-1]}")
 
     cpu = None
     if not args.no_cpu:

@@ -213,6 +213,16 @@ wg_status wg_agg_stats(wg_agg* h, uint64_t* lds_rows, uint64_t* global_rows,
  * table had to grow). */
 wg_status wg_agg_kernel_time(wg_agg* h, float* ms, uint32_t* launches);
 
+/* Device time (ms) of the last consume by phase: [0] direct consume kernel,
+ * [1] partition count + scan, [2] partition scatter, [3] partition aggregate. */
+wg_status wg_agg_pass_times(wg_agg* h, float* ms4);
+/* Consume strategy: 0 = pick by batch size and cardinality, 1 = direct
+ * (shared table with LDS pre-aggregation), 2 = partitioned (count, scatter,
+ * per-partition LDS aggregation); partition LDS table slots, workgroup size
+ * and the largest partition chunk one workgroup takes. */
+wg_status wg_agg_set_strategy(wg_agg* h, int32_t strategy, int32_t part_lds_slots, int32_t part_block,
+                              uint64_t part_chunk_rows);
+
 /* ------------------------------------------------------------------ filter */
 
 /* KernelFilter.Execute (filter.go:373-425) on device-resident data: writes the

@@ -84,6 +84,64 @@ def test_config2_shape_small(ctx, tuning):
     check(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED, tuning=tuning)
 
 
+@pytest.mark.parametrize("chunk", [1 << 16, 2048])
+def test_config2_partitioned(ctx, chunk):
+    """The partitioned consume (count, scatter, per-partition LDS aggregation);
+    a small chunk splits the hot partitions over several workgroups."""
+    n = 3_000_000
+    cols = W.gen_config2(ctx, n, seed=31)
+    print(f"[agg] partitioned chunk={chunk}", flush=True)
+    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES, group_hint=300_000)
+    agg.set_strategy(2, 2048, 1024, chunk)
+    agg.consume(cols, n, pred=W.CFG2_PRED)
+    t = agg.pass_times()
+    assert t[0] == 0 and t[2] > 0, t  # the partitioned passes ran
+    agg.finalize()
+    hcols = host_cols(cols, n)
+    ref = oracle_groups(run_oracle(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in W.CFG2_AGGS], W.CFG2_TYPES, W.CFG2_SCALES,
+            abs_sums(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
+
+
+def test_partitioned_overflow_grows_and_replays(ctx):
+    """Partitioned consume into a table far too small: items stop at the
+    overflow headroom, the table grows, overflowed positions replay."""
+    n = 2_000_000
+    cols = W.gen_config2(ctx, n, seed=37)
+    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES, group_hint=1)
+    agg.set_strategy(2, 1024, 512, 4096)
+    agg.consume(cols, n, pred=W.CFG2_PRED)
+    agg.finalize()
+    assert agg.stats()["capacity"] > 100_000
+    hcols = host_cols(cols, n)
+    ref = oracle_groups(run_oracle(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in W.CFG2_AGGS], W.CFG2_TYPES, W.CFG2_SCALES,
+            abs_sums(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
+
+
+def test_partitioned_two_consumes_and_sentinel_keys(ctx):
+    """Two partitioned consumes into one aggregate (keys already in the table
+    are merged by their owner), with sentinel-valued keys on the side path."""
+    rng = np.random.default_rng(41)
+    n = 1_500_000
+    k = rng.integers(0, 400_000, n).astype(np.int64)
+    k[::1000] = -0x7FFFFFFFFFFFFFFF
+    v = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    f = rng.standard_normal(n)
+    types = [L.INT64, L.INT64, L.FLOAT64]
+    aggs = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MAX, 2), AggSpec(L.AVG, 1)]
+    agg = HashAggregate(ctx, types, [0], aggs, group_hint=400_000)
+    agg.set_strategy(2, 2048, 1024, 1 << 15)
+    h = n // 2
+    for lo, hi in ((0, h), (h, n)):
+        agg.consume([Col(ctx.to_device(k[lo:hi]), L.INT64), Col(ctx.to_device(v[lo:hi]), L.INT64),
+                     Col(ctx.to_device(f[lo:hi]), L.FLOAT64)], hi - lo)
+    agg.finalize()
+    hcols = [(k, L.INT64), (v, L.INT64), (f, L.FLOAT64)]
+    ref = oracle_groups(run_oracle(hcols, [0], aggs))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in aggs], types, None, abs_sums(hcols, [0], aggs))
+
+
 def test_table_growth_from_tiny_hint(ctx):
     """Start from the minimum table and let overflow + grow + replay do the work."""
     n = 2_000_000

@@ -703,10 +703,10 @@ wg_status grow_to(wg_agg* h, uint64_t new_cap) {
 }
 
 wg_status agg_ensure_capacity(wg_agg* h, uint64_t extra_groups) {
-  // keep the table at <= 50 % load for what is known to be coming
+  // keep the table at <= 75 % load for what is known to be coming
   const uint64_t need = h->groups + extra_groups;
-  if (need * 2 <= h->cap) return WG_OK;
-  return grow_to(h, next_pow2(std::max<uint64_t>(need * 2, 1024)));
+  if (need + need / 3 <= h->cap) return WG_OK;
+  return grow_to(h, next_pow2(std::max<uint64_t>(need + need / 3, 1024)));
 }
 
 wg_status normalize(wg_agg* h) {
@@ -812,6 +812,26 @@ static wg_status consume_once(wg_agg* h, const ColSet& cs, const uint32_t* sel, 
 static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred,
                                int depth);
 
+// Fold overflow partial records (possibly several per key) into the table
+// with atomics; the table was grown to hold them.
+wg_status agg_merge_partials(wg_agg* h, const uint64_t* src, uint64_t n_part) {
+  uint64_t* recs = nullptr;
+  const uint64_t bytes = n_part * (uint64_t)h->pg.stride * 8;
+  WG_HIP(hipMalloc(&recs, bytes));  // out of the way of later launches
+  WG_HIP(hipMemcpyAsync(recs, src, bytes, hipMemcpyDeviceToDevice, h->ctx->stream));
+  GlobalTable T = table_of(h);
+  T.max_probe = h->cap;
+  hipLaunchKernelGGL(agg_merge_part_kernel, dim3(grid_for(n_part, 256)), dim3(256), 0, h->ctx->stream, recs, n_part,
+                     T, h->pg);
+  WG_HIP(hipGetLastError());
+  uint64_t c2[CTR_N];
+  WG_TRY(agg_read_counters(h, c2));
+  WG_HIP(hipFree(recs));
+  WG_CHECK((c2[CTR_STOPPED] >> 32) == 0, WG_ERR_DEVICE, "overflow partials did not fit the grown table");
+  h->groups = c2[CTR_GROUPS];
+  return WG_OK;
+}
+
 // After a launch that overflowed: grow the table so that every group seen so
 // far plus every pending row fits at <= 50 % load, fold the overflow partials
 // in, then replay the overflow rows.
@@ -820,23 +840,7 @@ static wg_status resolve_overflow(wg_agg* h, const ColSet& cs, const PredDesc* p
   const uint64_t n_rows = c[CTR_OVF_ROWS], n_part = c[CTR_OVF_PART];
   h->groups = c[CTR_GROUPS];
   WG_TRY(grow_to(h, next_pow2(std::max<uint64_t>((h->groups + n_rows + n_part) * 2, h->cap * 2))));
-  if (n_part) {
-    // move the records out of the way of later launches
-    uint64_t* recs = nullptr;
-    const uint64_t bytes = n_part * (uint64_t)h->pg.stride * 8;
-    WG_HIP(hipMalloc(&recs, bytes));
-    WG_HIP(hipMemcpyAsync(recs, h->ovf_part, bytes, hipMemcpyDeviceToDevice, h->ctx->stream));
-    GlobalTable T = table_of(h);
-    T.max_probe = h->cap;
-    hipLaunchKernelGGL(agg_merge_part_kernel, dim3(grid_for(n_part, 256)), dim3(256), 0, h->ctx->stream, recs, n_part,
-                       T, h->pg);
-    WG_HIP(hipGetLastError());
-    uint64_t c2[CTR_N];
-    WG_TRY(agg_read_counters(h, c2));
-    WG_HIP(hipFree(recs));
-    WG_CHECK((c2[CTR_STOPPED] >> 32) == 0, WG_ERR_DEVICE, "overflow partials did not fit the grown table");
-    h->groups = c2[CTR_GROUPS];
-  }
+  if (n_part) WG_TRY(agg_merge_partials(h, h->ovf_part, n_part));
   if (n_rows) {
     uint32_t* rows = nullptr;
     WG_HIP(hipMalloc(&rows, n_rows * 4));
@@ -879,7 +883,21 @@ wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, u
   h->st_lds = h->st_glb = h->st_flushed = 0;
   h->st_kernel_ms = 0.f;
   h->st_launches = 0;
-  WG_TRY(consume_range(h, cs, sel, n, pred, 0));
+  for (float& v : h->pass_ms) v = 0.f;
+  // high-cardinality batches take the partitioned consume: rows of a key
+  // meet in one workgroup instead of contending on shared slots
+  const uint64_t est = std::max(h->group_hint, h->groups);
+  const bool part = h->strategy == 2 ||
+                    (h->strategy == 0 && n >= (1ull << 22) && est >= 8ull * (uint64_t)std::max(h->lds_slots, 256));
+  if (part && part_eligible(h, cs)) {
+    WG_TRY(agg_ensure_capacity(h, est));
+    WG_TRY(agg_consume_partitioned(h, cs, sel, n, pred));
+    h->st_kernel_ms = h->pass_ms[1] + h->pass_ms[2] + h->pass_ms[3];
+    h->st_launches = 3;
+  } else {
+    WG_TRY(consume_range(h, cs, sel, n, pred, 0));
+    h->pass_ms[0] = h->st_kernel_ms;
+  }
   h->rows_since_norm += n;
   return WG_OK;
 }
@@ -1054,7 +1072,8 @@ wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types, c
   WG_HIP(hipMalloc(&h->counters, CTR_N * 8));
   WG_TRY(zero_dev(ctx, h->counters, CTR_N));
   const uint64_t hint = std::max<uint64_t>(group_hint, 1024);
-  st = alloc_table(h, next_pow2(hint * 2));
+  h->group_hint = group_hint;
+  st = alloc_table(h, next_pow2(hint + hint / 4));
   if (st != WG_OK) {
     wg_agg_destroy(h);
     return st;
@@ -1077,6 +1096,12 @@ wg_status wg_agg_destroy(wg_agg* h) {
   (void)free_dense(h);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (hipEvent_t e : h->pev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipFree(h->part_cnt);
+  (void)hipFree(h->part_scan);
+  (void)hipFree(h->part_buf);
+  (void)hipFree(h->part_items);
   for (void* p : h->stage.pinned) (void)hipHostFree(p);
   for (uint64_t* p : h->stage.pvalid) (void)hipHostFree(p);
   for (void* p : h->stage.dev) (void)hipFree(p);
@@ -1360,6 +1385,29 @@ wg_status wg_agg_set_tuning(wg_agg* h, int32_t lds_slots, int32_t block) {
   h->lds_slots = lds_slots;
   h->block = block;
   pick_launch(h);
+  return WG_OK;
+}
+
+wg_status wg_agg_pass_times(wg_agg* h, float* ms4) {
+  WG_CHECK(h && ms4, WG_ERR_INVALID, "null argument");
+  for (int i = 0; i < 4; ++i) ms4[i] = h->pass_ms[i];
+  return WG_OK;
+}
+
+wg_status wg_agg_set_strategy(wg_agg* h, int32_t strategy, int32_t part_lds_slots, int32_t part_block,
+                              uint64_t part_chunk_rows) {
+  clear_error();
+  WG_CHECK(h, WG_ERR_INVALID, "null handle");
+  WG_CHECK(strategy >= 0 && strategy <= 2, WG_ERR_INVALID, "strategy %d", strategy);
+  WG_CHECK(part_block == 256 || part_block == 512 || part_block == 1024, WG_ERR_INVALID, "block %d", part_block);
+  WG_CHECK(part_lds_slots >= 64 && (part_lds_slots & (part_lds_slots - 1)) == 0 &&
+               (uint64_t)part_lds_slots * h->pg.stride * 8 <= 150 * 1024,
+           WG_ERR_INVALID, "partition LDS table of %d slots does not fit", part_lds_slots);
+  WG_CHECK(part_chunk_rows >= 1024 && part_chunk_rows < (1ull << 31), WG_ERR_INVALID, "chunk rows");
+  h->strategy = strategy;
+  h->part_lds_slots = part_lds_slots;
+  h->part_block = part_block;
+  h->part_chunk_rows = part_chunk_rows;
   return WG_OK;
 }
 

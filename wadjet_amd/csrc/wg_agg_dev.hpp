@@ -224,6 +224,28 @@ __device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src
   }
 }
 
+// Fold a partial slot into a global slot this workgroup owns exclusively
+// (no other workgroup touches the key during the kernel): plain read-modify-write.
+__device__ __forceinline__ void merge_partial_owned(uint64_t* dst, const uint64_t* src, const AggProgram& pg) {
+  for (int o = 0; o < pg.n_ops; ++o) {
+    const OpDesc op = pg.ops[o];
+    uint64_t* t = dst + op.word;
+    const uint64_t* s = src + op.word;
+    switch (op.kind) {
+      case OP_ROWCOUNT: case OP_NNCOUNT: case OP_SUMI: t[0] += s[0]; break;
+      case OP_SUMF:
+        t[0] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)t[0]) +
+                                              __longlong_as_double((long long)s[0]));
+        break;
+      case OP_SUMD: t[0] += s[0]; t[1] += s[1]; break;
+      case OP_SUMD4: for (int k = 0; k < 4; ++k) t[k] += s[k]; break;
+      case OP_MIN: if (s[0] < t[0]) t[0] = s[0]; break;
+      case OP_MAX: if (s[0] > t[0]) t[0] = s[0]; break;
+      default: break;
+    }
+  }
+}
+
 // Initial value of every word of an empty slot.
 __host__ __device__ inline uint64_t slot_init_word(const AggProgram& pg, int word) {
   if (word == 0) return kEmpty;
@@ -306,8 +328,8 @@ __device__ __forceinline__ void block_add(uint64_t* dst, uint64_t v) {
 constexpr int kLdsProbes = 8;
 
 __device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_mask, int stride, uint64_t key,
-                                                      uint64_t h) {
-  uint32_t idx = (uint32_t)(h >> 40) & (uint32_t)lds_mask;
+                                                      uint64_t h, int shift = 40) {
+  uint32_t idx = (uint32_t)(h >> shift) & (uint32_t)lds_mask;
   for (int p = 0; p < kLdsProbes; ++p) {
     uint64_t* s = lds + (uint64_t)idx * stride;
     uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

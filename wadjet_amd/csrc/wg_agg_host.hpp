@@ -65,6 +65,22 @@ struct wg_agg {
   uint64_t* ovf_part = nullptr;
   uint64_t ovf_part_cap = 0;
   bool dirty = false;  // table not in the freshly-filled state
+  // partitioned consume (high-cardinality batches)
+  int32_t strategy = 0;  // 0 auto, 1 direct, 2 partitioned
+  uint64_t group_hint = 0;
+  int32_t part_lds_slots = 2048;
+  int32_t part_block = 1024;
+  uint64_t part_chunk_rows = 1u << 16;
+  uint32_t* part_cnt = nullptr;  // counts then offsets, 2 x cap u32
+  uint64_t part_cnt_cap = 0;
+  void* part_scan = nullptr;
+  size_t part_scan_bytes = 0;
+  uint64_t* part_buf = nullptr;  // partitioned key + value words
+  uint64_t part_buf_words = 0;
+  void* part_items = nullptr;
+  uint64_t part_items_bytes = 0;
+  float pass_ms[4] = {0, 0, 0, 0};  // direct consume, count+scan, scatter, partition aggregate
+  hipEvent_t pev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // tuning
   int32_t lds_slots = 1024;
   int32_t block = 512;
@@ -102,4 +118,7 @@ wg_status grow_to(wg_agg* h, uint64_t new_cap);
 wg_status normalize(wg_agg* h);
 wg_status ensure_overflow_buffers(wg_agg* h);
 wg_status ensure_base(wg_agg* h);
+wg_status agg_merge_partials(wg_agg* h, const uint64_t* recs, uint64_t n);
+bool part_eligible(const wg_agg* h, const ColSet& cs);
+wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred);
 }  // namespace wg

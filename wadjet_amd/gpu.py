@@ -238,6 +238,15 @@ class HashAggregate:
         call("wg_agg_stats", self.h, *[C.byref(x) for x in v])
         return dict(lds_rows=v[0].value, global_rows=v[1].value, flushed=v[2].value, capacity=v[3].value)
 
+    def set_strategy(self, strategy: int, part_lds_slots: int = 2048, part_block: int = 1024,
+                     part_chunk_rows: int = 1 << 16):
+        call("wg_agg_set_strategy", self.h, strategy, part_lds_slots, part_block, part_chunk_rows)
+
+    def pass_times(self):
+        a = (C.c_float * 4)()
+        call("wg_agg_pass_times", self.h, a)
+        return list(a)
+
     def kernel_time(self):
         ms, k = C.c_float(), C.c_uint32()
         call("wg_agg_kernel_time", self.h, C.byref(ms), C.byref(k))
