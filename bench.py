@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X hash aggregate (headline) on one GPU.
+"""Benchmark of the MI355X hash aggregate / hash join on one GPU.
 
-Workload: 100M-row composite GROUP BY (a int32 Zipf(1.1) over 1M values,
-b int32 uniform over 16) with SUM(price DECIMAL(15,2)) into Int128, COUNT(*),
-MIN/MAX/AVG(f float64), WHERE f > -0.5 fused as a predicate. Inputs are
-generated into HBM before timing. One step = reset + consume + finalize.
+Headline (the JSON line's `value`): the 100M-row composite GROUP BY
+(a int32 Zipf(1.1) over 1M values, b int32 uniform over 16) with
+SUM(price DECIMAL(15,2)) into Int128, COUNT(*), MIN/MAX/AVG(f float64) and
+WHERE f > -0.5 fused. Inputs are generated into HBM before timing; one step =
+reset + consume + finalize.
 
-Prints ONE JSON line. `roofline` uses the consume kernel's device time (HIP
-events on the library stream) and its algorithmic bytes (24 B/row + 56 B/group);
-`cpu_baseline` times the C restatement (oracle/) on this host's cores on a
-bounded prefix of the same rows.
+Beside it, in the same line:
+  join     10M-row build, 200M-row probe, fused JOIN fk = key GROUP BY p1
+           SUM(qty * p2) (exact Int128), probe rows/s
+  batched  256 GROUP BY queries of uneven shapes: one persistent graph-replayed
+           launch (autotuned) vs one launch per query: q/s and latency
+  served   request loop over a resident flow table: Poisson arrivals at fixed
+           rates, continuous batching vs one request per launch: p50/p99, q/s
+`roofline` is the headline's consume kernels: algorithmic bytes
+(24 B/row + 56 B/group) over their device time (HIP events on the library
+stream). `cpu_baseline` times the C restatement (oracle/) on this host's cores
+over a prefix of the same rows.
 """
 from __future__ import annotations
 
@@ -30,12 +38,56 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cols, n_rows, sample_rows, threads):
+def pct(xs, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
+
+
+# ------------------------------------------------------------------- config 2
+
+def bench_agg(ctx, args):
+    from wadjet_amd import workloads as W
+    from wadjet_amd.gpu import HashAggregate
+    n = args.rows
+    log(f"[agg] generating {n} rows on device")
+    cols = W.gen_config2(ctx, n)
+    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES,
+                        group_hint=args.group_hint)
+    if args.strategy >= 0:
+        agg.set_strategy(args.strategy)
+
+    def step():
+        agg.reset()
+        agg.consume(cols, n, pred=W.CFG2_PRED)
+        agg.finalize()
+
+    for i in range(args.warmup):
+        step()
+    ctx.sync()
+    kern_ms, passes = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        kern_ms.append(agg.kernel_time()[0])
+        passes.append(agg.pass_times())
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    groups = agg.num_groups()
+    ms_step = wall * 1000 / args.steps
+    k_ms = sum(kern_ms) / len(kern_ms)
+    alg = W.CFG2_BYTES_PER_ROW * n + W.CFG2_BYTES_PER_GROUP * groups
+    log(f"[agg] {ms_step:.3f} ms/step, consume kernels {k_ms:.3f} ms, groups {groups}; "
+        f"passes (direct, count+scan, scatter, partition-agg) {passes[-1]}")
+    return dict(cols=cols, n=n, groups=groups, ms_step=ms_step, kernel_ms=k_ms, alg_bytes=alg,
+                passes=passes[-1], agg=agg)
+
+
+def cpu_agg(cols, n, sample_rows, threads):
     import numpy as np
 
     import oracle as O
     from wadjet_amd import workloads as W
-    m = min(n_rows, sample_rows)
+    m = min(n, sample_rows)
     host = [(cols[0].data.to_host(np.int32, m), O.INT32), (cols[1].data.to_host(np.int32, m), O.INT32),
             (cols[2].data.to_host(np.int64, m), O.DECIMAL, None, 2), (cols[3].data.to_host(np.float64, m), O.FLOAT64)]
     aggs = [(a.func, a.col) for a in W.CFG2_AGGS]
@@ -47,8 +99,140 @@ def cpu_baseline(cols, n_rows, sample_rows, threads):
         best = dt if best is None else min(best, dt)
         log(f"[cpu] rep {rep}: {m} rows, {r['n']} groups, {dt:.2f} s on {threads} threads")
     return dict(value=m / best, unit="rows/s", cores=threads, kind="port",
-                sample=f"first {m} rows of the same 100M-row input (C restatement, {threads} threads: "
-                       f"per-thread partial hash aggregates merged in thread order, like CloneSink/MergeSink)")
+                sample=f"first {m} rows of the same 100M-row input; C restatement of HashAggregate, {threads} "
+                       f"threads each aggregating a contiguous row range, partials merged in thread order "
+                       f"(the CloneSink/MergeSink shape); generation excluded, merge included")
+
+
+# ------------------------------------------------------------------- config 3
+
+def bench_join(ctx, args):
+    from wadjet_amd import _lib as L
+    from wadjet_amd import workloads as W
+    from wadjet_amd.gpu import AggSpec, HashAggregate, HashJoin
+    cfg = W.Cfg3(n_build=args.join_build, n_probe=args.join_probe)
+    log(f"[join] generating build {cfg.n_build} / probe {cfg.n_probe}")
+    bcols = W.gen_config3_build(ctx, cfg)
+    pcols = W.gen_config3_probe(ctx, cfg)
+    j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    ctx.timer_start()
+    j.build(bcols, cfg.n_build)
+    j.finish()
+    build_ms = ctx.timer_stop()
+    types = W.CFG3_PROBE_TYPES + W.CFG3_BUILD_TYPES
+    aggs = [AggSpec(L.SUM, 1, 4), AggSpec(L.COUNT, -1)]
+    agg = HashAggregate(ctx, types, [3], aggs, scales=[0, 0, 0, 0, 2], group_hint=64)
+
+    def step():
+        agg.reset()
+        j.probe_agg(pcols, 0, cfg.n_probe, agg)
+        agg.finalize()
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    kms = []
+    t0 = time.perf_counter()
+    steps = max(1, args.steps)
+    for _ in range(steps):
+        step()
+        kms.append(agg.kernel_time()[0])
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1000 / steps
+    k = sum(kms) / len(kms)
+    alg = W.CFG3_BYTES_PER_PROBE_ROW * cfg.n_probe
+    log(f"[join] build {build_ms:.2f} ms, probe+agg {ms:.3f} ms/step (kernel {k:.3f} ms), groups {agg.num_groups()}")
+    return dict(rows_per_s=cfg.n_probe / (ms / 1000), ms_per_step=ms, kernel_ms=k, build_ms=build_ms,
+                build_rows_per_s=cfg.n_build / (build_ms / 1000), groups=agg.num_groups(),
+                roofline=dict(bound="hbm", achieved=alg / (k / 1000) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=alg / (k / 1000) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
+                              kernel="join_agg_kernel"),
+                _keep=(j, agg, bcols, pcols))
+
+
+# ------------------------------------------------------------------- config 4
+
+def bench_batched(ctx, args):
+    import numpy as np
+
+    from wadjet_amd import workloads as W
+    from wadjet_amd.gpu import Batch
+    plan = W.plan_config4(n_queries=args.batch_queries, seed=4)
+    data = W.gen_config4(ctx, plan)
+    total_rows = sum(q["rows"] for q in plan)
+    qarr = Batch.queries(data)
+    b = Batch(ctx, args.batch_queries, sum(q["groups"] for q in plan))
+    t0 = time.perf_counter()
+    b.autotune(qarr, len(plan))
+    tune_s = time.perf_counter() - t0
+    tuning = b.tuning()
+    b.run(qarr, len(plan), use_graph=True)  # warm the graph
+    reps = max(3, args.steps)
+    bt = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.run(qarr, len(plan), use_graph=True)
+        bt.append(time.perf_counter() - t0)
+    n_lds = sum(1 for q in range(len(plan)) if b.choice(q)[0])
+    # one launch per query, serialised; latency of query i = its completion time
+    b.run(qarr, len(plan), plain=True)  # warm
+    pt = []
+    for _ in range(max(1, reps // 2)):
+        t0 = time.perf_counter()
+        b.run(qarr, len(plan), plain=True)
+        pt.append(time.perf_counter() - t0)
+    # per-query plain latencies (all submitted at t=0, run one after another)
+    lat = []
+    t0 = time.perf_counter()
+    for q in range(len(plan)):
+        b.run(Batch.queries([data[q]]), 1, plain=True)
+        lat.append((time.perf_counter() - t0) * 1000)
+    batched_s = min(bt)
+    plain_s = min(pt)
+    log(f"[batched] {len(plan)} queries, {total_rows} rows: batched {batched_s*1000:.2f} ms, "
+        f"plain {plain_s*1000:.2f} ms, tuning {tuning} ({tune_s:.1f} s)")
+    return dict(queries=len(plan), rows=total_rows, batched_ms=batched_s * 1000, plain_ms=plain_s * 1000,
+                batched_qps=len(plan) / batched_s, plain_qps=len(plan) / plain_s,
+                batched_p50_ms=batched_s * 1000, batched_p99_ms=batched_s * 1000,
+                plain_p50_ms=pct(lat, 50), plain_p99_ms=pct(lat, 99),
+                batched_rows_per_s=total_rows / batched_s, speedup=plain_s / batched_s,
+                tuning=tuning, lds_queries=n_lds, autotune_s=tune_s)
+
+
+# ------------------------------------------------------------------- config 5
+
+def bench_served(ctx, args):
+    from wadjet_amd import workloads as W
+    from wadjet_amd.gpu import HashJoin, Server
+    n = args.serve_rows
+    log(f"[served] generating a {n}-row flow table")
+    cols = W.gen_config5(ctx, n)
+    dim = W.gen_config5_dim(ctx, 50_000)
+    j = HashJoin(ctx, W.CFG5_DIM_TYPES, 0, inline=[1], build_hint=50_000)
+    j.build(dim, 50_000)
+    j.finish()
+    served = Server(ctx, cols, n, 3, lanes=3, hbm_budget=8 << 30, max_batch=64)
+    plain = Server(ctx, cols, n, 3, plain=True)
+    served.add_dim(j)
+    plain.add_dim(j)
+    out = []
+    for qps in args.qps:
+        n_req = max(50, min(args.serve_requests, int(qps * args.serve_seconds)))
+        qs, shapes = W.plan_config5(n, n_req, seed=int(qps))
+        res = {"qps_offered": qps, "requests": n_req}
+        for name, s in (("served", served), ("plain", plain)):
+            s.reset()
+            lat, _ = s.replay(qs, qps=qps, seed=1)
+            st = s.stats()
+            res[name] = dict(p50_ms=pct(lat, 50), p99_ms=pct(lat, 99), qps=st["qps"], launches=st["batches"],
+                             max_in_flight=st["max_in_flight"])
+        log(f"[served] {qps} q/s offered: served p50 {res['served']['p50_ms']:.2f} p99 {res['served']['p99_ms']:.2f} "
+            f"ms ({res['served']['qps']:.0f} q/s), plain p50 {res['plain']['p50_ms']:.2f} p99 "
+            f"{res['plain']['p99_ms']:.2f} ms ({res['plain']['qps']:.0f} q/s)")
+        out.append(res)
+    served.close()
+    plain.close()
+    return dict(table_rows=n, sweep=out)
 
 
 def main():
@@ -57,88 +241,74 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
-    ap.add_argument("--group-hint", type=int, default=8_000_000)
-    ap.add_argument("--lds-slots", type=int, default=-1)
-    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--group-hint", type=int, default=6_000_000)
+    ap.add_argument("--strategy", type=int, default=-1)
     ap.add_argument("--cpu-rows", type=int, default=25_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--join-build", type=int, default=10_000_000)
+    ap.add_argument("--join-probe", type=int, default=200_000_000)
+    ap.add_argument("--batch-queries", type=int, default=256)
+    ap.add_argument("--serve-rows", type=int, default=500_000_000)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--qps", type=float, nargs="*", default=[100, 1000, 5000])
+    ap.add_argument("--only", choices=["agg", "join", "batched", "served"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU "
               f"design is described, not built)", file=sys.stderr)
         sys.exit(2)
 
-    from wadjet_amd import workloads as W
-    from wadjet_amd.gpu import Context, HashAggregate
-
+    from wadjet_amd.gpu import Context
     ctx = Context(0)
-    n = args.rows
-    log(f"[bench] generating {n} rows on device")
-    cols = W.gen_config2(ctx, n)
-    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES,
-                        group_hint=args.group_hint)
-    if args.lds_slots >= 0 or args.block:
-        agg.set_tuning(args.lds_slots if args.lds_slots >= 0 else 1024, args.block or 512)
-
-    def step():
-        agg.reset()
-        agg.consume(cols, n, pred=W.CFG2_PRED)
-        agg.finalize()
-
-    for i in range(args.warmup):
-        step()
-        log(f"[bench] warmup {i} done")
-    ctx.sync()
-    kern_ms, passes = [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        kern_ms.append(agg.kernel_time()[0])
-        passes.append(agg.pass_times())
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    groups = agg.num_groups()
-    st = agg.stats()
-    ms_step = wall * 1000 / args.steps
-    k_ms = sum(kern_ms) / len(kern_ms)
-    alg_bytes = W.CFG2_BYTES_PER_ROW * n + W.CFG2_BYTES_PER_GROUP * groups
-    achieved = alg_bytes / (k_ms / 1000) / 1e9
-    log(f"[bench] {ms_step:.3f} ms/step, consume kernels {k_ms:.3f} ms, groups {groups}, stats {st}")
-    log(f"[bench] pass times (direct, count+scan, scatter, partition-agg) last step: {passes[-1]}")
-
+    want = (lambda s: args.only in (None, s))
+    A = bench_agg(ctx, args) if want("agg") else None
+    extra = {}
+    if want("join"):
+        J = bench_join(ctx, args)
+        J.pop("_keep")
+        extra["join"] = J
+    if want("batched"):
+        extra["batched"] = bench_batched(ctx, args)
+    if want("served"):
+        extra["served"] = bench_served(ctx, args)
     cpu = None
-    if not args.no_cpu:
+    if A is not None and not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(cols, n, args.cpu_rows, threads)
+        cpu = cpu_agg(A["cols"], A["n"], args.cpu_rows, threads)
 
-    out = {
-        "metric": "rows/s & %HBM roofline for hash-agg/join probe; batched vs plain qps + p99",
-        "value": n / (ms_step / 1000),
-        "unit": "rows/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int64+int128+f64",
-        "data": "synthetic (seeded device generators)",
-        "config": {"workload": "hash-aggregate 100M rows: GROUP BY (a int32 Zipf1.1/1M, b int32 u16) "
-                               "SUM(price DECIMAL(15,2))->Int128, COUNT(*), MIN/MAX/AVG(f f64), WHERE f > -0.5",
-                   "rows": n, "groups": groups, "lds_slots": agg_tuning(agg), "parallelism": "1 GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "agg_consume_kernel", "kernel_ms": k_ms,
-                     "alg_bytes": alg_bytes},
-        "cpu_baseline": cpu,
-    }
+    if A is not None:
+        achieved = A["alg_bytes"] / (A["kernel_ms"] / 1000) / 1e9
+        out = {
+            "metric": "rows/s & %HBM roofline for hash-agg/join probe; batched vs plain qps + p99",
+            "value": A["n"] / (A["ms_step"] / 1000),
+            "unit": "rows/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": A["ms_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "int64+int128+f64",
+            "data": "synthetic (seeded device generators)",
+            "config": {"workload": "hash-aggregate, 100M rows: GROUP BY (a int32 Zipf(1.1) over 1M, b int32 "
+                                   "uniform 16) SUM(price DECIMAL(15,2)) -> Int128, COUNT(*), MIN/MAX/AVG(f "
+                                   "float64) WHERE f > -0.5",
+                       "rows": A["n"], "groups": A["groups"], "parallelism": "1 GPU"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernels": "consume kernels of one step (direct, or count+scan, scatter, "
+                                    "partition-aggregate)",
+                         "kernel_ms": A["kernel_ms"], "pass_ms": A["passes"], "alg_bytes": A["alg_bytes"]},
+            "cpu_baseline": cpu,
+        }
+    else:
+        out = {"metric": "rows/s & %HBM roofline for hash-agg/join probe; batched vs plain qps + p99",
+               "value": None, "unit": "rows/s", "n_gpus": 1}
+    out.update(extra)
     print(json.dumps(out), flush=True)
-
-
-def agg_tuning(agg):
-    return None
 
 
 if __name__ == "__main__":

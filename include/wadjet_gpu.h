@@ -137,6 +137,10 @@ wg_status wg_free(wg_ctx* ctx, void* ptr);
 wg_status wg_memcpy_h2d(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 wg_status wg_memcpy_d2h(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 wg_status wg_memset(wg_ctx* ctx, void* dst, int value, uint64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and
+ * stores). Both pointers and `bytes` must be multiples of 16. Also the
+ * known-size kernel against which HBM traffic counters are calibrated. */
+wg_status wg_memcpy_d2d(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ------------------------------------------------------------- aggregation */
 

@@ -77,6 +77,7 @@ SIGNATURES: dict[str, list] = {
     "wg_memcpy_h2d": [P, P, P, U64],
     "wg_memcpy_d2h": [P, P, P, U64],
     "wg_memset": [P, P, C.c_int, U64],
+    "wg_memcpy_d2d": [P, P, P, U64],
     "wg_agg_create": [P, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(C.c_uint8), I32, C.POINTER(I32), I32,
                       C.POINTER(AggSpecC), U64, C.POINTER(P)],
     "wg_agg_destroy": [P],

@@ -1102,6 +1102,7 @@ wg_status wg_agg_destroy(wg_agg* h) {
   (void)hipFree(h->part_scan);
   (void)hipFree(h->part_buf);
   (void)hipFree(h->part_items);
+  if (h->part_host) (void)hipHostFree(h->part_host);
   for (void* p : h->stage.pinned) (void)hipHostFree(p);
   for (uint64_t* p : h->stage.pvalid) (void)hipHostFree(p);
   for (void* p : h->stage.dev) (void)hipFree(p);

@@ -71,7 +71,7 @@ struct wg_agg {
   int32_t part_lds_slots = 2048;
   int32_t part_block = 1024;
   uint64_t part_chunk_rows = 1u << 16;
-  uint32_t* part_cnt = nullptr;  // counts then offsets, 2 x cap u32
+  uint32_t* part_cnt = nullptr;  // counts, offsets, boundaries (u32)
   uint64_t part_cnt_cap = 0;
   void* part_scan = nullptr;
   size_t part_scan_bytes = 0;
@@ -79,6 +79,8 @@ struct wg_agg {
   uint64_t part_buf_words = 0;
   void* part_items = nullptr;
   uint64_t part_items_bytes = 0;
+  void* part_host = nullptr;  // pinned: partition boundaries + work items
+  uint64_t part_host_bytes = 0;
   float pass_ms[4] = {0, 0, 0, 0};  // direct consume, count+scan, scatter, partition aggregate
   hipEvent_t pev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // tuning

@@ -29,7 +29,10 @@
 
 namespace wg {
 
-constexpr int kMaxVals = 8;
+constexpr int kMaxVals = 7;      // value words carried per record (64-byte records)
+constexpr int kPartU = 4;        // rows per thread per step of the count/scatter loops
+constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
+constexpr int kPartBlock = 512;  // count/scatter workgroup size
 
 struct PartParams {
   ColSet cs;
@@ -44,50 +47,125 @@ struct PartParams {
   int32_t n_vals;
   uint32_t* counts;      // [P][G]
   const uint32_t* offs;  // [P][G] exclusive scan of counts
-  uint64_t* pkey;
-  uint64_t* pval[kMaxVals];
+  uint64_t* recs;        // partitioned records: key word, value words, zero padding
   int32_t val_col[kMaxVals];
 };
 
-__device__ __forceinline__ bool part_row(const PartParams& P, uint64_t row, uint64_t& key, int& kind) {
-  // kind: 0 = partitioned, 1 = sentinel-valued key, 2 = NULL single key
-  if (P.pred[0].col >= 0 && !pred_pass(P.cs.cols[P.pred[0].col], row, P.pred[0].op, P.pred[0].i64, P.pred[0].f64))
-    return false;
-  if (P.pred[1].col >= 0 && !pred_pass(P.cs.cols[P.pred[1].col], row, P.pred[1].op, P.pred[1].i64, P.pred[1].f64))
-    return false;
-  const RowRef r{row, 0, nullptr};
-  if (!build_key<false>(P.key, P.cs, r, key)) {
-    kind = 2;
-    return true;
+// Raw 64-bit word of a column value: float64 bits, integers widened to int64.
+__device__ __forceinline__ uint64_t raw_word(const ColDesc& c, uint64_t row) {
+  if (c.type == WG_FLOAT64) return ((const uint64_t*)c.data)[row];
+  return (uint64_t)load_int(c, row);
+}
+__device__ __forceinline__ uint64_t valid_word(const ColDesc& c, uint64_t row) {
+  return c.valid ? c.valid[row >> 6] : ~0ull;
+}
+// pred_pass on a value already loaded (same typed comparison).
+__device__ __forceinline__ bool pred_raw(const ColDesc& c, uint64_t raw, uint64_t vw, uint64_t row, int op,
+                                         int64_t ci, double cf) {
+  if (!((vw >> (row & 63)) & 1ull)) return false;
+  switch (c.type) {
+    case WG_FLOAT64: return cmp_op<double>(__longlong_as_double((long long)raw), cf, op);
+    case WG_INT32: case WG_UINT16: return cmp_op<int32_t>((int32_t)(int64_t)raw, (int32_t)ci, op);
+    default: return cmp_op<int64_t>((int64_t)raw, ci, op);
   }
-  kind = key == kEmpty ? 1 : 0;
-  return true;
 }
 
-__global__ void __launch_bounds__(256) part_count_kernel(PartParams P) {
+// Load and classify kPartU rows per thread (positions i0 + u * step). Every
+// load of every row is issued before any value is used, so a wave keeps
+// kPartU rows of memory traffic in flight.
+// kind: -1 filtered out or past the end, 0 partitioned, 1 sentinel-valued
+// key, 2 NULL single-column key.
+template <int NV>
+__device__ __forceinline__ void part_load(const PartParams& P, uint64_t i0, uint64_t step, uint64_t end,
+                                          uint64_t (&row)[kPartU], uint64_t (&key)[kPartU], int (&kind)[kPartU],
+                                          uint64_t (&val)[kPartU][NV + 1]) {
+  bool in[kPartU];
+#pragma unroll
+  for (int u = 0; u < kPartU; ++u) {
+    const uint64_t i = i0 + u * step;
+    in[u] = i < end;
+    const uint64_t ii = in[u] ? i : end - 1;
+    row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
+  }
+  uint64_t pr[2][kPartU], pv[2][kPartU], kw[2][kPartU], kv[kPartU];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+    if (P.pred[q].col >= 0) {
+      const ColDesc& c = P.cs.cols[P.pred[q].col];
+#pragma unroll
+      for (int u = 0; u < kPartU; ++u) {
+        pr[q][u] = raw_word(c, row[u]);
+        pv[q][u] = valid_word(c, row[u]);
+      }
+    }
+  // packed keys are at most 8 bytes: one or two columns
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k < P.key.n) {
+      const ColDesc& c = P.cs.cols[P.key.col[k]];
+#pragma unroll
+      for (int u = 0; u < kPartU; ++u) kw[k][u] = (uint64_t)load_int(c, row[u]);
+    }
+#pragma unroll
+  for (int u = 0; u < kPartU; ++u) kv[u] = valid_word(P.cs.cols[P.key.col[0]], row[u]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j < P.n_vals) {
+      const ColDesc& c = P.cs.cols[P.val_col[j]];
+#pragma unroll
+      for (int u = 0; u < kPartU; ++u) val[u][j] = raw_word(c, row[u]);
+    }
+#pragma unroll
+  for (int u = 0; u < kPartU; ++u) {
+    bool ok = in[u];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (P.pred[q].col >= 0)
+        ok = ok && pred_raw(P.cs.cols[P.pred[q].col], pr[q][u], pv[q][u], row[u], P.pred[q].op, P.pred[q].i64,
+                            P.pred[q].f64);
+    uint64_t k;
+    if (P.key.n == 1) {
+      k = kw[0][u];
+    } else {
+      k = 0;
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        if (c < P.key.n) {
+          uint64_t v = kw[c][u];
+          if (P.key.narrow[c]) v = (v & 0xFFFFFFFFull) << P.key.shift[c];
+          k |= v;
+        }
+    }
+    key[u] = k;
+    // multi-column keys carry no validity on this path (checked by the host)
+    const bool null_key = P.key.n == 1 && !((kv[u] >> (row[u] & 63)) & 1ull);
+    kind[u] = !ok ? -1 : null_key ? 2 : (k == kEmpty ? 1 : 0);
+  }
+}
+
+__global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   extern __shared__ uint32_t hist[];
   const uint32_t np = 1u << P.pbits;
   for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-    const uint64_t row = P.sel ? (uint64_t)P.sel[i] : i;
-    uint64_t key;
-    int kind;
-    if (!part_row(P, row, key, kind) || kind != 0) continue;
-    atomicAdd(&hist[key_hash(key) >> (64 - P.pbits)], 1u);
+  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kPartU) {
+    uint64_t row[kPartU], key[kPartU], val[kPartU][1];
+    int kind[kPartU];
+    part_load<0>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val);
+#pragma unroll
+    for (int u = 0; u < kPartU; ++u)
+      if (kind[u] == 0) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], 1u);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
 }
 
-__device__ __forceinline__ uint64_t raw_word(const ColDesc& c, uint64_t row) {
-  if (c.type == WG_FLOAT64) return ((const uint64_t*)c.data)[row];
-  return (uint64_t)load_int(c, row);
-}
-
-__global__ void __launch_bounds__(256) part_scatter_kernel(PartParams P) {
+// RW = record words (key + values, padded to a power of two).
+template <int RW>
+__global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) {
+  constexpr int NV = RW - 1;
   extern __shared__ uint32_t cur[];
   const uint32_t np = 1u << P.pbits;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) cur[p] = P.offs[(uint64_t)p * gridDim.x + blockIdx.x];
@@ -95,34 +173,50 @@ __global__ void __launch_bounds__(256) part_scatter_kernel(PartParams P) {
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
   const int stride = P.pg.stride;
-  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-    const uint64_t row = P.sel ? (uint64_t)P.sel[i] : i;
-    uint64_t key;
-    int kind;
-    if (!part_row(P, row, key, kind)) continue;
-    if (kind != 0) {
-      // side slots: [0] the sentinel-valued key, [1] the NULL key
-      uint64_t* s = P.T.slots + (P.T.mask + (kind == 1 ? 1 : 2)) * (uint64_t)stride;
-      apply_row<GlbT, false>(s, P.pg, P.cs, RowRef{row, 0, nullptr});
-      continue;
+  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kPartU) {
+    uint64_t row[kPartU], key[kPartU], val[kPartU][NV + 1];
+    int kind[kPartU];
+    part_load<NV>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val);
+#pragma unroll
+    for (int u = 0; u < kPartU; ++u) {
+      if (kind[u] < 0) continue;
+      if (kind[u] != 0) {
+        // side slots: [0] the sentinel-valued key, [1] the NULL key
+        uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
+        apply_row<GlbT, false>(s, P.pg, P.cs, RowRef{row[u], 0, nullptr});
+        continue;
+      }
+      const uint32_t pos = atomicAdd(&cur[key_hash(key[u]) >> (64 - P.pbits)], 1u);
+      uint64_t w[RW];
+      w[0] = key[u];
+#pragma unroll
+      for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
+      ulonglong2* dst = (ulonglong2*)(P.recs + (uint64_t)pos * RW);
+#pragma unroll
+      for (int j = 0; j < RW / 2; ++j) dst[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
     }
-    const uint32_t pos = atomicAdd(&cur[key_hash(key) >> (64 - P.pbits)], 1u);
-    P.pkey[pos] = key;
-    for (int j = 0; j < P.n_vals; ++j) P.pval[j][pos] = raw_word(P.cs.cols[P.val_col[j]], row);
   }
 }
 
+// Device copy of the partition boundaries: bound[p] = first position of
+// partition p, bound[np] = total records.
+__global__ void part_bounds_kernel(const uint32_t* counts, const uint32_t* offs, uint32_t np, uint32_t G,
+                                   uint32_t* bound) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) bound[p] = offs[(uint64_t)p * G];
+  if (p == np) bound[np] = offs[(uint64_t)np * G - 1] + counts[(uint64_t)np * G - 1];
+}
+
 struct PartItem {
-  uint32_t beg, end;  // positions in the partitioned arrays
+  uint32_t beg, end;  // positions in the partitioned records
   uint32_t split;     // 1: the partition is shared with other items
   uint32_t pad;
 };
 
 struct PartAggParams {
-  ColSet cs;        // value arrays (partitioned layout), one column per value word
-  AggProgram pg;    // ops remapped onto cs
+  AggProgram pg;  // ops remapped onto the record's value words
   GlobalTable T;
-  const uint64_t* pkey;
+  const uint64_t* recs;
   const PartItem* items;
   uint32_t n_items;
   int32_t lds_slots;
@@ -135,13 +229,107 @@ struct PartAggParams {
   uint64_t ovf_rows_cap;
   uint64_t* ovf_part;
   uint64_t ovf_part_cap;
+  uint32_t vfloat;  // bit j: value word j holds float64 bits (else an int64)
 };
 
+template <int NV>
+__device__ __forceinline__ uint64_t pick(const uint64_t (&v)[NV], int c) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j == c) r = v[j];
+  return r;
+}
+
+// apply_row for a partitioned record: the op's columns index the value words.
+// Value columns on this path carry no NULLs (checked by the host), so
+// COUNT(col) counts every row as COUNT(*) does.
+template <typename S, int NV>
+__device__ __forceinline__ void apply_vals(uint64_t* w, const AggProgram& pg, const uint64_t (&v)[NV],
+                                           uint32_t vfloat) {
+  for (int o = 0; o < pg.n_ops; ++o) {
+    const OpDesc op = pg.ops[o];
+    uint64_t* t = w + op.word;
+    switch (op.kind) {
+      case OP_ROWCOUNT: case OP_NNCOUNT:
+        at_add<S>(t, 1);
+        break;
+      case OP_SUMI: {
+        uint64_t x = pick(v, op.col);
+        if (op.col2 >= 0) x *= pick(v, op.col2);
+        at_add<S>(t, x);
+        break;
+      }
+      case OP_SUMF: {
+        const uint64_t b = pick(v, op.col);
+        at_addf<S>(t, ((vfloat >> op.col) & 1u) ? __longlong_as_double((long long)b) : (double)(int64_t)b);
+        break;
+      }
+      case OP_SUMD:
+        add_limbs2<S>(t, (int64_t)pick(v, op.col));
+        break;
+      case OP_SUMD4: {
+        const int64_t a = (int64_t)pick(v, op.col);
+        uint64_t lo;
+        int64_t hi;
+        if (op.col2 >= 0) {
+          const int64_t b = (int64_t)pick(v, op.col2);
+          lo = (uint64_t)a * (uint64_t)b;
+          hi = (int64_t)__mul64hi((long long)a, (long long)b);
+        } else {
+          lo = (uint64_t)a;
+          hi = a < 0 ? -1 : 0;
+        }
+        add_limbs4<S>(t, lo, hi);
+        break;
+      }
+      case OP_MIN: case OP_MAX: {
+        const uint64_t b = pick(v, op.col);
+        uint64_t e;
+        if ((vfloat >> op.col) & 1u) {
+          if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) break;  // NaN never wins
+          e = enc_f64_bits(b);
+        } else {
+          e = enc_i64((int64_t)b);
+        }
+        if (op.kind == OP_MIN) at_min<S>(t, e);
+        else at_max<S>(t, e);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+// Claim-or-find with the CAS as the probe: one round trip per probed slot.
+__device__ __forceinline__ uint64_t* glb_claim_from(const GlobalTable& T, int stride, uint64_t key, uint64_t idx,
+                                                    uint64_t first_p, bool& fresh) {
+  const uint64_t lim = T.max_probe < T.mask + 1 ? T.max_probe : T.mask + 1;
+  for (uint64_t p = first_p; p < lim; ++p) {
+    uint64_t* s = T.slots + idx * (uint64_t)stride;
+    const uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
+    if (prev == kEmpty) {
+      fresh = true;
+      return s;
+    }
+    if (prev == key) return s;
+    idx = (idx + 1) & T.mask;
+  }
+  return nullptr;
+}
+
+constexpr int kFlushU = 2;
+
+template <int RW>
 __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
+  constexpr int NV = RW - 1;
   extern __shared__ uint64_t lds[];
   __shared__ int64_t s_item;
+  __shared__ uint64_t s_init[16];
   const int stride = P.pg.stride;
   const int L = P.lds_slots;
+  if (threadIdx.x < (unsigned)stride) s_init[threadIdx.x] = slot_init_word(P.pg, threadIdx.x);
   uint32_t claims = 0;
   const uint64_t n_work = P.sel ? (P.n_sel + P.sel_chunk - 1) / P.sel_chunk : P.n_items;
   for (;;) {
@@ -174,41 +362,93 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
       end = it.end;
       split = it.split != 0;
     }
-    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = slot_init_word(P.pg, i % stride);
+    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = s_init[i & (stride - 1)];
     __syncthreads();
-    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-      const uint64_t pos = P.sel ? (uint64_t)P.sel[i] : i;
-      const uint64_t key = P.pkey[pos];
-      const uint64_t h = key_hash(key);
-      const RowRef r{pos, 0, nullptr};
-      uint64_t* ls = lds_find_or_claim(lds, L - 1, stride, key, h, 16);
-      if (ls) {
-        apply_row<LdsT, false>(ls, P.pg, P.cs, r);
-        continue;
+    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kAggU) {
+      uint64_t rec[kAggU][RW];
+      uint64_t pos[kAggU];
+      bool in[kAggU];
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u) {
+        const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+        in[u] = i < end;
+        const uint64_t ii = in[u] ? i : beg;
+        pos[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
       }
-      uint64_t* g = glb_find_or_claim(P.T, stride, key, h, claims);
-      if (!g) {
-        const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        P.ovf_rows[at] = (uint32_t)pos;  // in bounds: items are taken only with headroom
-        continue;
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u) {
+        const ulonglong2* src = (const ulonglong2*)(P.recs + pos[u] * RW);
+#pragma unroll
+        for (int j = 0; j < RW / 2; ++j) {
+          const ulonglong2 x = src[j];
+          rec[u][2 * j] = x.x;
+          rec[u][2 * j + 1] = x.y;
+        }
       }
-      apply_row<GlbT, false>(g, P.pg, P.cs, r);
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u) {
+        if (!in[u]) continue;
+        const uint64_t key = rec[u][0];
+        uint64_t v[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] = rec[u][j + 1];
+        const uint64_t h = key_hash(key);
+        uint64_t* ls = lds_find_or_claim(lds, L - 1, stride, key, h, 16);
+        if (ls) {
+          apply_vals<LdsT, NV>(ls, P.pg, v, P.vfloat);
+          continue;
+        }
+        uint64_t* g = glb_find_or_claim(P.T, stride, key, h, claims);
+        if (!g) {
+          const uint64_t at =
+              __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          P.ovf_rows[at] = (uint32_t)pos[u];  // in bounds: items are taken only with headroom
+          continue;
+        }
+        apply_vals<GlbT, NV>(g, P.pg, v, P.vfloat);
+      }
     }
     __syncthreads();
-    // merge the LDS table into the global table
-    for (int i = threadIdx.x; i < L; i += blockDim.x) {
-      const uint64_t* ls = lds + (uint64_t)i * stride;
-      const uint64_t key = ls[0];
-      if (key == kEmpty) continue;
-      uint64_t* g = glb_find_or_claim(P.T, stride, key, key_hash(key), claims);
-      if (!g) {
-        const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t* rec = P.ovf_part + at * (uint64_t)stride;  // in bounds: headroom as above
-        for (int w = 0; w < stride; ++w) rec[w] = ls[w];
-        continue;
+    // merge the LDS table into the global table; first probes of kFlushU
+    // slots per thread are in flight together
+    for (int i0 = threadIdx.x; i0 < L; i0 += blockDim.x * kFlushU) {
+      uint64_t key[kFlushU], prev[kFlushU] = {}, home[kFlushU];
+#pragma unroll
+      for (int u = 0; u < kFlushU; ++u) {
+        const int i = i0 + u * blockDim.x;
+        key[u] = i < L ? lds[(uint64_t)i * stride] : kEmpty;
+        home[u] = key_hash(key[u]) & P.T.mask;
       }
-      if (split) merge_partial(g, ls, P.pg);
-      else merge_partial_owned(g, ls, P.pg);
+#pragma unroll
+      for (int u = 0; u < kFlushU; ++u)
+        if (key[u] != kEmpty) prev[u] = at_cas<GlbT>(P.T.slots + home[u] * (uint64_t)stride, kEmpty, key[u]);
+#pragma unroll
+      for (int u = 0; u < kFlushU; ++u) {
+        if (key[u] == kEmpty) continue;
+        const uint64_t* ls = lds + (uint64_t)(i0 + u * blockDim.x) * stride;
+        bool fresh = prev[u] == kEmpty;
+        uint64_t* g = (fresh || prev[u] == key[u])
+                          ? P.T.slots + home[u] * (uint64_t)stride
+                          : glb_claim_from(P.T, stride, key[u], (home[u] + 1) & P.T.mask, 1, fresh);
+        if (!g) {
+          const uint64_t at =
+              __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          uint64_t* rec = P.ovf_part + at * (uint64_t)stride;  // in bounds: headroom as above
+          for (int w = 0; w < stride; ++w) rec[w] = ls[w];
+          continue;
+        }
+        if (fresh) ++claims;
+        if (split) {
+          merge_partial(g, ls, P.pg);
+        } else if (fresh) {
+          // a slot this workgroup just claimed for a key no one else holds:
+          // its words are still the initial values, store the partial
+          g[1] = ls[1];
+          for (int w = 2; w < stride; w += 2) *(ulonglong2*)(g + w) = make_ulonglong2(ls[w], ls[w + 1]);
+        } else {
+          merge_partial_owned(g, ls, P.pg);
+        }
+      }
     }
     __syncthreads();
   }
@@ -233,6 +473,40 @@ bool part_eligible(const wg_agg* h, const ColSet& cs) {
   return true;
 }
 
+static wg_status allow_lds(const void* kernel, size_t bytes) {
+  if (bytes > 64 * 1024) WG_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return WG_OK;
+}
+
+static wg_status launch_scatter(int rw, dim3 grid, size_t lds, hipStream_t st, const PartParams& P) {
+  switch (rw) {
+    case 2: hipLaunchKernelGGL(part_scatter_kernel<2>, grid, dim3(kPartBlock), lds, st, P); break;
+    case 4: hipLaunchKernelGGL(part_scatter_kernel<4>, grid, dim3(kPartBlock), lds, st, P); break;
+    default: hipLaunchKernelGGL(part_scatter_kernel<8>, grid, dim3(kPartBlock), lds, st, P); break;
+  }
+  WG_HIP(hipGetLastError());
+  return WG_OK;
+}
+
+static wg_status launch_part_agg(int rw, int grid, int block, size_t lds, hipStream_t st, const PartAggParams& Q) {
+  switch (rw) {
+    case 2:
+      WG_TRY(allow_lds((const void*)part_agg_kernel<2>, lds));
+      hipLaunchKernelGGL(part_agg_kernel<2>, dim3(grid), dim3(block), lds, st, Q);
+      break;
+    case 4:
+      WG_TRY(allow_lds((const void*)part_agg_kernel<4>, lds));
+      hipLaunchKernelGGL(part_agg_kernel<4>, dim3(grid), dim3(block), lds, st, Q);
+      break;
+    default:
+      WG_TRY(allow_lds((const void*)part_agg_kernel<8>, lds));
+      hipLaunchKernelGGL(part_agg_kernel<8>, dim3(grid), dim3(block), lds, st, Q);
+      break;
+  }
+  WG_HIP(hipGetLastError());
+  return WG_OK;
+}
+
 wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred) {
   wg_ctx* ctx = h->ctx;
   // value columns the program reads, each carried as one 8-byte word
@@ -253,13 +527,18 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   }
   WG_CHECK((int)vcols.size() <= kMaxVals, WG_ERR_UNSUPPORTED, "partitioned consume carries at most %d value columns",
            kMaxVals);
+  WG_CHECK(h->key.n <= 2, WG_ERR_UNSUPPORTED, "partitioned consume takes keys of at most two columns");
+  const int RW = vcols.size() < 2 ? 2 : vcols.size() < 4 ? 4 : 8;
   const int L = h->part_lds_slots;
+  // partitions of about L/2 groups each (the LDS table at half load)
   const uint64_t est = std::max<uint64_t>(std::max<uint64_t>(h->group_hint, h->groups), 1);
   int pbits = 6;
-  while (pbits < 14 && (1ull << pbits) * (uint64_t)(L / 3) < est) ++pbits;
+  while (pbits < 14 && (1ull << pbits) * (uint64_t)(L / 2) < est) ++pbits;
   const uint32_t np = 1u << pbits;
-  const int G = ctx->cus * 4;
-  const uint64_t R = (n + G - 1) / G;
+  const size_t hist_bytes = (size_t)np * 4;
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / hist_bytes));
+  const int G = ctx->cus * per_cu;
+  const uint64_t R = ((n + G - 1) / G + 255) / 256 * 256;
 
   PartParams P{};
   P.cs = cs;
@@ -274,108 +553,105 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   P.n_vals = (int)vcols.size();
   for (size_t j = 0; j < vcols.size(); ++j) P.val_col[j] = vcols[j];
 
-  // scratch: counts + offsets
+  // scratch: counts, offsets, boundaries (device) and boundaries (pinned host)
   const uint64_t ncnt = (uint64_t)np * G;
-  if (h->part_cnt_cap < ncnt) {
+  const uint64_t cnt_words = 2 * ncnt + np + 1;
+  if (h->part_cnt_cap < cnt_words) {
     if (h->part_cnt) WG_HIP(hipFree(h->part_cnt));
-    WG_HIP(hipMalloc(&h->part_cnt, ncnt * 8));
-    h->part_cnt_cap = ncnt;
-    size_t bytes = 0;
-    WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, h->part_cnt, h->part_cnt + ncnt, (int)ncnt, ctx->stream));
-    if (bytes > h->part_scan_bytes) {
-      if (h->part_scan) WG_HIP(hipFree(h->part_scan));
-      WG_HIP(hipMalloc(&h->part_scan, bytes));
-      h->part_scan_bytes = bytes;
-    }
+    WG_HIP(hipMalloc(&h->part_cnt, cnt_words * 4));
+    h->part_cnt_cap = cnt_words;
+  }
+  size_t scan_bytes = 0;
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->part_cnt, h->part_cnt + ncnt, (int)ncnt,
+                                          ctx->stream));
+  if (scan_bytes > h->part_scan_bytes) {
+    if (h->part_scan) WG_HIP(hipFree(h->part_scan));
+    WG_HIP(hipMalloc(&h->part_scan, scan_bytes));
+    h->part_scan_bytes = scan_bytes;
   }
   uint32_t* counts = h->part_cnt;
   uint32_t* offs = h->part_cnt + ncnt;
+  uint32_t* dbound = h->part_cnt + 2 * ncnt;
   P.counts = counts;
   P.offs = offs;
   P.T = table_of(h);
+  // host side: boundaries, then the work items (largest possible count)
+  const uint64_t chunk = h->part_chunk_rows;
+  const uint64_t max_items = np + n / chunk + 1;
+  const size_t host_bytes = (np + 1) * 4 + max_items * sizeof(PartItem) + 64;
+  if (h->part_host_bytes < host_bytes) {
+    if (h->part_host) WG_HIP(hipHostFree(h->part_host));
+    h->part_host = nullptr;
+    WG_HIP(hipHostMalloc(&h->part_host, host_bytes, hipHostMallocDefault));
+    h->part_host_bytes = host_bytes;
+  }
+  uint32_t* bound = (uint32_t*)h->part_host;
+  PartItem* hitems = (PartItem*)((char*)h->part_host + ((np + 1) * 4 + 15) / 16 * 16);
 
   if (!h->pev[0])
     for (hipEvent_t& e : h->pev) WG_HIP(hipEventCreate(&e));
-  // A: count
+  // A: count, B: scan, boundaries
   WG_HIP(hipEventRecord(h->pev[0], ctx->stream));
-  hipLaunchKernelGGL(part_count_kernel, dim3(G), dim3(256), np * 4, ctx->stream, P);
+  hipLaunchKernelGGL(part_count_kernel, dim3(G), dim3(kPartBlock), hist_bytes, ctx->stream, P);
   WG_HIP(hipGetLastError());
-  size_t bytes = h->part_scan_bytes;
-  WG_HIP(hipcub::DeviceScan::ExclusiveSum(h->part_scan, bytes, counts, offs, (int)ncnt, ctx->stream));
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(h->part_scan, scan_bytes, counts, offs, (int)ncnt, ctx->stream));
+  hipLaunchKernelGGL(part_bounds_kernel, dim3((np + 256) / 256), dim3(256), 0, ctx->stream, counts, offs, np,
+                     (uint32_t)G, dbound);
+  WG_HIP(hipGetLastError());
   WG_HIP(hipEventRecord(h->pev[1], ctx->stream));
-  // partition boundaries (column 0 of each partition row) and the total
-  std::vector<uint32_t> bound(np + 1);
-  WG_HIP(hipMemcpy2DAsync(bound.data(), 4, offs, (size_t)G * 4, 4, np, hipMemcpyDeviceToHost, ctx->stream));
-  uint32_t last_off = 0, last_cnt = 0;
-  WG_HIP(hipMemcpyAsync(&last_off, offs + ncnt - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-  WG_HIP(hipMemcpyAsync(&last_cnt, counts + ncnt - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  WG_HIP(hipMemcpyAsync(bound, dbound, (np + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
   WG_HIP(hipStreamSynchronize(ctx->stream));
-  const uint64_t total = (uint64_t)last_off + last_cnt;
-  bound[np] = (uint32_t)total;
+  const uint64_t total = bound[np];
 
-  // scratch: partitioned key + value arrays
-  const uint64_t need = std::max<uint64_t>(total, 1) * (1 + vcols.size());
+  // scratch: partitioned records
+  const uint64_t need = std::max<uint64_t>(total, 1) * RW;
   if (h->part_buf_words < need) {
     if (h->part_buf) WG_HIP(hipFree(h->part_buf));
     WG_HIP(hipMalloc(&h->part_buf, need * 8));
     h->part_buf_words = need;
   }
-  P.pkey = h->part_buf;
-  for (size_t j = 0; j < vcols.size(); ++j) P.pval[j] = h->part_buf + (1 + j) * std::max<uint64_t>(total, 1);
+  P.recs = h->part_buf;
 
   // C: scatter (side-slot rows are folded here)
   h->dirty = true;
   WG_HIP(hipEventRecord(h->pev[2], ctx->stream));
-  hipLaunchKernelGGL(part_scatter_kernel, dim3(G), dim3(256), np * 4, ctx->stream, P);
-  WG_HIP(hipGetLastError());
+  WG_TRY(launch_scatter(RW, dim3(G), hist_bytes, ctx->stream, P));
   WG_HIP(hipEventRecord(h->pev[3], ctx->stream));
 
-  // D: work items, largest first; partitions above the chunk size are split
-  const uint64_t chunk = h->part_chunk_rows;
-  std::vector<PartItem> items;
-  items.reserve(np + total / chunk + 1);
+  // D: work items (built while the scatter runs), largest first; partitions
+  // above the chunk size are split
+  uint64_t n_items = 0;
   for (uint32_t p = 0; p < np; ++p) {
     const uint64_t b = bound[p], e = bound[p + 1];
     if (e <= b) continue;
     const bool split = e - b > chunk;
     for (uint64_t x = b; x < e; x += chunk)
-      items.push_back(PartItem{(uint32_t)x, (uint32_t)std::min<uint64_t>(e, x + chunk), split ? 1u : 0u, 0});
+      hitems[n_items++] = PartItem{(uint32_t)x, (uint32_t)std::min<uint64_t>(e, x + chunk), split ? 1u : 0u, 0};
   }
-  std::stable_sort(items.begin(), items.end(),
+  std::stable_sort(hitems, hitems + n_items,
                    [](const PartItem& a, const PartItem& b) { return a.end - a.beg > b.end - b.beg; });
-  const uint64_t ibytes = std::max<size_t>(items.size(), 1) * sizeof(PartItem) + 64;
+  const uint64_t ibytes = std::max<uint64_t>(n_items, 1) * sizeof(PartItem) + 64;
   if (h->part_items_bytes < ibytes) {
     if (h->part_items) WG_HIP(hipFree(h->part_items));
     WG_HIP(hipMalloc(&h->part_items, ibytes));
     h->part_items_bytes = ibytes;
   }
   unsigned long long* item_ctr = (unsigned long long*)((char*)h->part_items + ibytes - 64);
-  if (!items.empty())
-    WG_HIP(hipMemcpyAsync(h->part_items, items.data(), items.size() * sizeof(PartItem), hipMemcpyHostToDevice, ctx->stream));
+  if (n_items)
+    WG_HIP(hipMemcpyAsync(h->part_items, hitems, n_items * sizeof(PartItem), hipMemcpyHostToDevice, ctx->stream));
 
   PartAggParams Q{};
-  Q.cs.n_cols = (int)vcols.size();
-  Q.cs.n_probe = Q.cs.n_cols;
-  for (size_t j = 0; j < vcols.size(); ++j) {
-    const int t = cs.cols[vcols[j]].type;
-    const int t2 = t == WG_FLOAT64 ? WG_FLOAT64 : (t == WG_DECIMAL ? WG_DECIMAL : WG_INT64);
-    Q.cs.cols[j] = ColDesc{P.pval[j], nullptr, t2, cs.cols[vcols[j]].scale};
-    Q.cs.inline_word[j] = -1;
-  }
   Q.pg = pg2;
-  Q.pkey = P.pkey;
+  Q.recs = P.recs;
   Q.items = (const PartItem*)h->part_items;
-  Q.n_items = (uint32_t)items.size();
+  Q.n_items = (uint32_t)n_items;
   Q.lds_slots = L;
   Q.item_ctr = item_ctr;
+  for (size_t j = 0; j < vcols.size(); ++j)
+    if (cs.cols[vcols[j]].type == WG_FLOAT64) Q.vfloat |= 1u << j;
   const size_t lds_bytes = (size_t)L * h->pg.stride * 8;
-  static bool attr_done = false;
-  if (!attr_done) {
-    WG_HIP(hipFuncSetAttribute((const void*)part_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    attr_done = true;
-  }
-  const int per_cu = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
-  const int grid_max = ctx->cus * per_cu;
+  const int per_cu_agg = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
+  const int grid_max = ctx->cus * per_cu_agg;
   // overflow lists with headroom for every in-flight item (see the kernel)
   const uint64_t need_rows = (uint64_t)grid_max * chunk * 2;
   if (h->ovf_rows_cap < need_rows) {
@@ -403,8 +679,8 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     const bool replay = !replays.empty();
     const uint32_t* rsel = replay ? replays.back().rows : nullptr;
     const uint64_t rn = replay ? replays.back().n : 0;
-    const uint64_t total_work = replay ? (rn + chunk - 1) / chunk : items.size();
-    if (!replay && items_done >= items.size()) break;
+    const uint64_t total_work = replay ? (rn + chunk - 1) / chunk : n_items;
+    if (!replay && items_done >= n_items) break;
     WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
     const uint64_t start_item = replay ? 0 : items_done;
     uint64_t* pin = ctx->pinned + 8;  // [8] start item, [9] items taken (counters use [0, 8))
@@ -420,8 +696,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     const uint64_t left = total_work - start_item;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)grid_max, left));
     WG_HIP(hipEventRecord(h->pev[4], ctx->stream));
-    if (left) hipLaunchKernelGGL(part_agg_kernel, dim3(grid), dim3(h->part_block), lds_bytes, ctx->stream, Q);
-    WG_HIP(hipGetLastError());
+    if (left) WG_TRY(launch_part_agg(RW, grid, h->part_block, lds_bytes, ctx->stream, Q));
     WG_HIP(hipEventRecord(h->pev[5], ctx->stream));
     WG_HIP(hipMemcpyAsync(pin + 1, item_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
     uint64_t c[CTR_N];
@@ -472,7 +747,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     }
   }
   for (auto& r : replays) (void)hipFree(r.rows);
-  WG_CHECK(replays.empty() && items_done >= items.size(), WG_ERR_DEVICE, "partitioned consume did not converge");
+  WG_CHECK(replays.empty() && items_done >= n_items, WG_ERR_DEVICE, "partitioned consume did not converge");
   return WG_OK;
 }
 

@@ -34,6 +34,12 @@ __device__ __forceinline__ bool filter_pass(const ColDesc& c, uint64_t row, int 
   return pred_pass(c, row, op, ci, cf);
 }
 
+// Grid-stride streaming copy of 16-byte words.
+__global__ void __launch_bounds__(256) copy_words_kernel(ulonglong2* dst, const ulonglong2* src, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 template <bool WRITE>
 __global__ void __launch_bounds__(kFilterBlock) filter_kernel(ColDesc col, uint64_t n, const uint32_t* sel, int op,
                                                               int64_t ci, double cf, uint32_t* tile_counts,
@@ -175,6 +181,19 @@ wg_status wg_memcpy_d2h(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes)
   if (bytes == 0) return WG_OK;
   WG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   WG_HIP(hipStreamSynchronize(ctx->stream));
+  return WG_OK;
+}
+
+wg_status wg_memcpy_d2d(wg_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  WG_CHECK(ctx, WG_ERR_INVALID, "ctx is null");
+  WG_CHECK(((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0, WG_ERR_INVALID,
+           "wg_memcpy_d2d needs 16-byte aligned pointers and size");
+  if (bytes == 0) return WG_OK;
+  const uint64_t n = bytes / 16;
+  const int grid = (int)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->cus * 16);
+  hipLaunchKernelGGL(copy_words_kernel, dim3(grid), dim3(256), 0, ctx->stream, (ulonglong2*)dst,
+                     (const ulonglong2*)src, n);
+  WG_HIP(hipGetLastError());
   return WG_OK;
 }
 

@@ -72,6 +72,10 @@ class Context:
             call("wg_memcpy_h2d", self.h, b.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes)
         return b
 
+    def copy(self, dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int):
+        """Device-to-device copy through the library's streaming copy kernel."""
+        call("wg_memcpy_d2d", self.h, dst.ptr, src.ptr, nbytes)
+
     def gen(self, kind: int, dtype, n: int, seed: int, i0: int = 0, i1: int = 0, f0: float = 0.0, f1: float = 0.0,
             row0: int = 0, out: Optional["DeviceBuffer"] = None) -> "DeviceBuffer":
         nbytes = n * np.dtype(dtype).itemsize
