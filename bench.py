@@ -32,6 +32,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of the consume kernels (see the file's header)
+AGG_PMC_SUMMARY = ROOT / "profiles" / "agg_cfg2_100M_pmc.txt"
+JOIN_PMC_SUMMARY = ROOT / "profiles" / "join_cfg3_200M_pmc.txt"
+
+
+def is_consume_kernel(name):
+    return any(s in name for s in ("part_count", "part_scatter", "part_agg", "part_bounds", "rocprim",
+                                   "agg_consume_kernel"))
 
 
 def log(msg):
@@ -141,12 +149,15 @@ def bench_join(ctx, args):
     ms = (time.perf_counter() - t0) * 1000 / steps
     k = sum(kms) / len(kms)
     alg = W.CFG3_BYTES_PER_PROBE_ROW * cfg.n_probe
+    from wadjet_amd.profiling import load_traffic
+    traffic = load_traffic(str(JOIN_PMC_SUMMARY), lambda k: "join_agg_kernel" in k)
     log(f"[join] build {build_ms:.2f} ms, probe+agg {ms:.3f} ms/step (kernel {k:.3f} ms), groups {agg.num_groups()}")
     return dict(rows_per_s=cfg.n_probe / (ms / 1000), ms_per_step=ms, kernel_ms=k, build_ms=build_ms,
                 build_rows_per_s=cfg.n_build / (build_ms / 1000), groups=agg.num_groups(),
                 roofline=dict(bound="hbm", achieved=alg / (k / 1000) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=alg / (k / 1000) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
-                              kernel="join_agg_kernel"),
+                              kernel="join_agg_kernel", traffic=traffic,
+                              traffic_source=JOIN_PMC_SUMMARY.name if traffic else None),
                 _keep=(j, agg, bcols, pcols))
 
 
@@ -254,6 +265,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--qps", type=float, nargs="*", default=[100, 1000, 5000])
     ap.add_argument("--only", choices=["agg", "join", "batched", "served"], default=None)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU "
@@ -262,6 +275,14 @@ def main():
 
     from wadjet_amd.gpu import Context
     ctx = Context(0)
+    if args.calib_copy_mib:
+        # known-size copy for calibrating HBM traffic counters (profiled runs only)
+        nb = args.calib_copy_mib << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy(b, a, nb)
+        ctx.sync()
+        a.free()
+        b.free()
     want = (lambda s: args.only in (None, s))
     A = bench_agg(ctx, args) if want("agg") else None
     extra = {}
@@ -280,6 +301,8 @@ def main():
 
     if A is not None:
         achieved = A["alg_bytes"] / (A["kernel_ms"] / 1000) / 1e9
+        from wadjet_amd.profiling import load_traffic
+        traffic = load_traffic(str(AGG_PMC_SUMMARY), is_consume_kernel)
         out = {
             "metric": "rows/s & %HBM roofline for hash-agg/join probe; batched vs plain qps + p99",
             "value": A["n"] / (A["ms_step"] / 1000),
@@ -298,7 +321,8 @@ def main():
                                    "float64) WHERE f > -0.5",
                        "rows": A["n"], "groups": A["groups"], "parallelism": "1 GPU"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                         "traffic_source": AGG_PMC_SUMMARY.name if traffic else None,
                          "kernels": "consume kernels of one step (direct, or count+scan, scatter, "
                                     "partition-aggregate)",
                          "kernel_ms": A["kernel_ms"], "pass_ms": A["passes"], "alg_bytes": A["alg_bytes"]},

@@ -42,19 +42,24 @@ struct ConsumeParams {
   uint64_t* ovf_part;
   uint64_t ovf_part_cap;
   int32_t lds_slots;  // power of two, 0 = no LDS table
-  int32_t pad;
+  // preloaded values (NV > 0 kernels): pg's op columns index val_col
+  int32_t n_vals;
+  int32_t val_col[kMaxVals];
+  uint32_t vfloat;  // bit j: value j is float64 bits
 };
 
-template <bool USE_LDS>
+// NV > 0: every value a row needs is preloaded with its keys (kRowU rows per
+// thread in flight) and applied with apply_vals; NV = 0: values are read by
+// apply_row when the row is applied (DECIMAL128 inputs, more than three value
+// columns).
+template <bool USE_LDS, int NV>
 __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
   extern __shared__ uint64_t lds[];
   __shared__ int64_t s_tile;
   const int stride = P.pg.stride;
   const int L = USE_LDS ? P.lds_slots : 0;
 
-  if (USE_LDS) {
-    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = slot_init_word(P.pg, i % stride);
-  }
+  if (USE_LDS) lds_table_init(lds, L, P.pg);
   __syncthreads();
 
   uint32_t my_lds = 0, my_glb = 0, my_claims = 0;
@@ -77,36 +82,53 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
 
     const uint64_t beg = (uint64_t)tile * P.tile_rows;
     const uint64_t end = min(beg + P.tile_rows, P.n);
-    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-      const uint64_t row = P.sel ? (uint64_t)P.sel[i] : P.row_base + i;
-      RowRef r{row, 0, nullptr};
-      if (P.pred[0].col >= 0 && !pred_pass(P.cs.cols[P.pred[0].col], row, P.pred[0].op, P.pred[0].i64, P.pred[0].f64)) continue;
-      if (P.pred[1].col >= 0 && !pred_pass(P.cs.cols[P.pred[1].col], row, P.pred[1].op, P.pred[1].i64, P.pred[1].f64)) continue;
-      uint64_t key;
-      uint64_t* slot = nullptr;
-      if (!build_key<false>(P.key, P.cs, r, key)) {
-        slot = P.T.slots + (P.T.mask + 1 + 1) * (uint64_t)stride;  // NULL-key side slot
-      } else if (key == kEmpty) {
-        slot = P.T.slots + (P.T.mask + 1) * (uint64_t)stride;  // sentinel-valued key side slot
-      } else {
-        const uint64_t h = key_hash(key);
-        if (USE_LDS) {
-          uint64_t* ls = lds_find_or_claim(lds, L - 1, stride, key, h);
-          if (ls != nullptr) {
-            apply_row<LdsT, false>(ls, P.pg, P.cs, r);
-            ++my_lds;
-            continue;
+    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kRowU) {
+      uint64_t row[kRowU], key[kRowU], val[kRowU][NV > 0 ? NV : 1];
+      uint32_t vvalid[kRowU];
+      int kind[kRowU];
+      load_rows<NV>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+#pragma unroll
+      for (int u = 0; u < kRowU; ++u) {
+        if (kind[u] < 0) continue;
+        uint64_t* slot = nullptr;
+        int ws = 1;
+        bool in_lds = false;
+        if (kind[u] == 2) {
+          slot = P.T.slots + (P.T.mask + 1 + 1) * (uint64_t)stride;  // NULL-key side slot
+        } else if (kind[u] == 1) {
+          slot = P.T.slots + (P.T.mask + 1) * (uint64_t)stride;  // sentinel-valued key side slot
+        } else {
+          const uint64_t h = key_hash(key[u]);
+          if (USE_LDS) {
+            slot = lds_find_or_claim(lds, L - 1, key[u], h);
+            if (slot) {
+              ws = L;
+              in_lds = true;
+            }
+          }
+          if (!slot) {
+            slot = glb_find_or_claim(P.T, stride, key[u], h, my_claims);
+            if (!slot) {
+              uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              P.ovf_rows[at] = (uint32_t)row[u];
+              continue;
+            }
           }
         }
-        slot = glb_find_or_claim(P.T, stride, key, h, my_claims);
-        if (slot == nullptr) {
-          uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          P.ovf_rows[at] = (uint32_t)row;
-          continue;
+        if (NV > 0) {
+          uint64_t v[NV > 0 ? NV : 1];
+#pragma unroll
+          for (int j = 0; j < (NV > 0 ? NV : 1); ++j) v[j] = val[u][j];
+          if (in_lds) apply_vals<LdsT>(slot, ws, P.pg, v, P.vfloat, vvalid[u]);
+          else apply_vals<GlbT>(slot, 1, P.pg, v, P.vfloat, vvalid[u]);
+        } else {
+          const RowRef r{row[u], 0, nullptr};
+          if (in_lds) apply_row<LdsT, false>(slot, ws, P.pg, P.cs, r);
+          else apply_row<GlbT, false>(slot, 1, P.pg, P.cs, r);
         }
+        if (in_lds) ++my_lds;
+        else ++my_glb;
       }
-      apply_row<GlbT, false>(slot, P.pg, P.cs, r);
-      ++my_glb;
     }
   }
 
@@ -114,17 +136,17 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
     __syncthreads();
     uint32_t flushed = 0;
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
-      const uint64_t* ls = lds + (uint64_t)i * stride;
+      const uint64_t* ls = lds + i;
       const uint64_t key = ls[0];
       if (key == kEmpty) continue;
       ++flushed;
       uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key), my_claims);
       if (slot != nullptr) {
-        merge_partial(slot, ls, P.pg);
+        merge_partial(slot, ls, L, P.pg);
       } else {
         uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t* rec = P.ovf_part + at * (uint64_t)stride;
-        for (int w = 0; w < stride; ++w) rec[w] = ls[w];
+        for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * L];
       }
     }
     block_add(P.T.counters + CTR_FLUSHED, flushed);
@@ -135,11 +157,19 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
   block_add(P.T.counters + CTR_GLB_ROWS, my_glb);
 }
 
-// Fill every slot (and the side slots) with the empty pattern.
-__global__ void agg_fill_kernel(uint64_t* slots, uint64_t n_words, AggProgram pg) {
+// Fill every slot (and the side slots) with the empty pattern: 16-byte
+// stores of the precomputed slot words (the stride is a power of two >= 2).
+__global__ void __launch_bounds__(256) agg_fill_kernel(uint64_t* slots, uint64_t n_words, AggProgram pg) {
+  __shared__ uint64_t s_init[16];
   const int stride = pg.stride;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += (uint64_t)gridDim.x * blockDim.x)
-    slots[i] = slot_init_word(pg, (int)(i % stride));
+  if (threadIdx.x < (unsigned)stride) s_init[threadIdx.x] = slot_init_word(pg, threadIdx.x);
+  __syncthreads();
+  ulonglong2* d = (ulonglong2*)slots;
+  const uint64_t n2 = n_words / 2;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * blockDim.x) {
+    const int w = (int)((2 * i) & (uint64_t)(stride - 1));
+    d[i] = make_ulonglong2(s_init[w], s_init[w + 1]);
+  }
 }
 
 // Is the slot a live group? Table slots: key != EMPTY; side slots: rowcount > 0.
@@ -190,7 +220,7 @@ __global__ void agg_merge_part_kernel(const uint64_t* recs, uint64_t n, GlobalTa
       at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
       continue;
     }
-    merge_partial(d, r, pg);
+    merge_partial(d, r, 1, pg);
   }
   block_add(T.counters + CTR_GROUPS, claims);
 }
@@ -732,14 +762,19 @@ wg_status agg_read_counters(wg_agg* h, uint64_t* c) {
   return WG_OK;
 }
 
-wg_status ensure_overflow_buffers(wg_agg* h) {
-  const uint64_t rows_cap = std::max<uint64_t>((uint64_t)h->grid * h->tile_rows * 4, 1ull << 20);
+wg_status ensure_overflow_buffers(wg_agg* h) { return ensure_overflow_for(h, h->grid, h->lds_slots); }
+
+// Overflow lists for a launch of `grid` workgroups with `lds_slots`-slot LDS
+// tables: row headroom for every in-flight tile, and room for every LDS entry
+// a flush may spill (flushes write without a further bound check).
+wg_status ensure_overflow_for(wg_agg* h, uint64_t grid, int32_t lds_slots) {
+  const uint64_t rows_cap = std::max<uint64_t>(grid * h->tile_rows * 4, 1ull << 20);
   if (h->ovf_rows_cap < rows_cap) {
     if (h->ovf_rows) WG_HIP(hipFree(h->ovf_rows));
     WG_HIP(hipMalloc(&h->ovf_rows, rows_cap * 4));
     h->ovf_rows_cap = rows_cap;
   }
-  const uint64_t part_cap = (uint64_t)h->grid * (uint64_t)std::max(h->lds_slots, 1);
+  const uint64_t part_cap = grid * (uint64_t)std::max(lds_slots, 1);
   if (h->ovf_part_cap < part_cap) {
     if (h->ovf_part) WG_HIP(hipFree(h->ovf_part));
     WG_HIP(hipMalloc(&h->ovf_part, part_cap * (uint64_t)h->pg.stride * 8));
@@ -756,6 +791,27 @@ static void pick_launch(wg_agg* h) {
   else per_cu = std::max(1, std::min(160 * 1024 / std::max(lds_bytes, 1), 2048 / h->block));
   h->grid = h->ctx->cus * per_cu;
   h->tile_rows = (uint64_t)h->block * 16;
+}
+
+template <bool USE_LDS, int NV>
+static wg_status launch_consume_t(int grid, int block, size_t lds, hipStream_t st, const ConsumeParams& P) {
+  if (USE_LDS && lds > 64 * 1024)
+    WG_HIP(hipFuncSetAttribute((const void*)agg_consume_kernel<USE_LDS, NV>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((agg_consume_kernel<USE_LDS, NV>), dim3(grid), dim3(block), lds, st, P);
+  return WG_OK;
+}
+
+static wg_status launch_consume(bool use_lds, int nv, int grid, int block, size_t lds, hipStream_t st,
+                                const ConsumeParams& P) {
+  switch (nv) {
+    case 0: return use_lds ? launch_consume_t<true, 0>(grid, block, lds, st, P)
+                           : launch_consume_t<false, 0>(grid, block, lds, st, P);
+    case 1: return use_lds ? launch_consume_t<true, 1>(grid, block, lds, st, P)
+                           : launch_consume_t<false, 1>(grid, block, lds, st, P);
+    default: return use_lds ? launch_consume_t<true, 3>(grid, block, lds, st, P)
+                            : launch_consume_t<false, 3>(grid, block, lds, st, P);
+  }
 }
 
 // One launch over rows [row_base, row_base + n) or sel[0..n); reads back the
@@ -785,19 +841,38 @@ static wg_status consume_once(wg_agg* h, const ColSet& cs, const uint32_t* sel, 
     WG_HIP(hipEventCreate(&h->ev0));
     WG_HIP(hipEventCreate(&h->ev1));
   }
-  WG_HIP(hipEventRecord(h->ev0, h->ctx->stream));
-  if (h->lds_slots > 0) {
-    const size_t lds = (size_t)h->lds_slots * h->pg.stride * 8;
-    static bool attr_done = false;
-    if (!attr_done) {
-      WG_HIP(hipFuncSetAttribute((const void*)agg_consume_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 150 * 1024));
-      attr_done = true;
+  // preload the values a row needs unless a column is DECIMAL128 or there
+  // are more than kMaxVals of them
+  std::vector<int> vcols;
+  bool fast = true;
+  AggProgram pg2 = h->pg;
+  for (int o = 0; o < pg2.n_ops && fast; ++o)
+    for (int32_t* c : {&pg2.ops[o].col, &pg2.ops[o].col2}) {
+      if (*c < 0) continue;
+      if (cs.cols[*c].type == WG_DECIMAL128) fast = false;
+      int j = -1;
+      for (size_t k = 0; k < vcols.size(); ++k)
+        if (vcols[k] == *c) j = (int)k;
+      if (j < 0) {
+        j = (int)vcols.size();
+        vcols.push_back(*c);
+      }
+      *c = j;
     }
-    hipLaunchKernelGGL(agg_consume_kernel<true>, dim3(grid), dim3(h->block), lds, h->ctx->stream, P);
-  } else {
-    hipLaunchKernelGGL(agg_consume_kernel<false>, dim3(grid), dim3(h->block), 0, h->ctx->stream, P);
+  if (vcols.size() > 3) fast = false;  // more preloaded values would spill registers
+  int nv = 0;
+  if (fast) {
+    P.pg = pg2;
+    P.n_vals = (int32_t)vcols.size();
+    for (size_t j = 0; j < vcols.size(); ++j) {
+      P.val_col[j] = vcols[j];
+      if (cs.cols[vcols[j]].type == WG_FLOAT64) P.vfloat |= 1u << j;
+    }
+    nv = vcols.size() <= 1 ? 1 : 3;
   }
+  WG_HIP(hipEventRecord(h->ev0, h->ctx->stream));
+  const size_t lds = h->lds_slots > 0 ? (size_t)h->lds_slots * h->pg.stride * 8 : 0;
+  WG_TRY(launch_consume(h->lds_slots > 0, nv, grid, h->block, lds, h->ctx->stream, P));
   WG_HIP(hipGetLastError());
   WG_HIP(hipEventRecord(h->ev1, h->ctx->stream));
   h->dirty = true;

@@ -35,7 +35,19 @@ struct RowRef {
   uint64_t prow;
   uint64_t brow;
   const uint64_t* bslot;  // join slot of the matched key (inline payload), or null
+  uint64_t pay2, pay3;    // with bslot: slot words 2 and 3, already loaded
 };
+
+// Inline payload word `w` of the matched join slot. Words 2 and 3 come from
+// registers (scalar fields, not an array, so nothing is placed in scratch);
+// `w` is uniform, so this is a scalar branch.
+__device__ __forceinline__ uint64_t slot_payload(const RowRef& r, int w) {
+  switch (w) {
+    case 2: return r.pay2;
+    case 3: return r.pay3;
+    default: return r.bslot[w];
+  }
+}
 
 template <bool JOINED>
 __device__ __forceinline__ bool rv_valid(const ColSet& cs, int c, const RowRef& r) {
@@ -45,14 +57,14 @@ __device__ __forceinline__ bool rv_valid(const ColSet& cs, int c, const RowRef& 
 template <bool JOINED>
 __device__ __forceinline__ int64_t rv_int(const ColSet& cs, int c, const RowRef& r) {
   if (!JOINED || c < cs.n_probe) return load_int(cs.cols[c], r.prow);
-  if (r.bslot != nullptr && cs.inline_word[c] >= 0) return (int64_t)r.bslot[cs.inline_word[c]];
+  if (r.bslot != nullptr && cs.inline_word[c] >= 0) return (int64_t)slot_payload(r, cs.inline_word[c]);
   return load_int(cs.cols[c], r.brow);
 }
 template <bool JOINED>
 __device__ __forceinline__ uint64_t rv_bits(const ColSet& cs, int c, const RowRef& r) {
   // raw 64-bit payload of a float64 column
   if (!JOINED || c < cs.n_probe) return ((const uint64_t*)cs.cols[c].data)[r.prow];
-  if (r.bslot != nullptr && cs.inline_word[c] >= 0) return r.bslot[cs.inline_word[c]];
+  if (r.bslot != nullptr && cs.inline_word[c] >= 0) return slot_payload(r, cs.inline_word[c]);
   return ((const uint64_t*)cs.cols[c].data)[r.brow];
 }
 // Int128 value of a DECIMAL128 column: {Hi, Lo} per row.
@@ -99,28 +111,31 @@ __device__ __forceinline__ void at_max(uint64_t* p, uint64_t v) {
 
 // Add a signed 128-bit value as four carry-save limbs (32-bit pieces; the top
 // limb signed). Each limb word absorbs < 2^32 additions without overflow.
+// `ws` is the distance between consecutive words of a slot: 1 in the global
+// table (array of structs), the slot count in an LDS table (struct of arrays).
 template <typename S>
-__device__ __forceinline__ void add_limbs4(uint64_t* w, uint64_t lo, int64_t hi) {
-  at_add<S>(w + 0, lo & 0xFFFFFFFFull);
-  at_add<S>(w + 1, lo >> 32);
-  at_add<S>(w + 2, (uint64_t)hi & 0xFFFFFFFFull);
-  at_add<S>(w + 3, (uint64_t)(int64_t)(hi >> 32));
+__device__ __forceinline__ void add_limbs4(uint64_t* w, int ws, uint64_t lo, int64_t hi) {
+  at_add<S>(w, lo & 0xFFFFFFFFull);
+  at_add<S>(w + ws, lo >> 32);
+  at_add<S>(w + 2 * ws, (uint64_t)hi & 0xFFFFFFFFull);
+  at_add<S>(w + 3 * ws, (uint64_t)(int64_t)(hi >> 32));
 }
 template <typename S>
-__device__ __forceinline__ void add_limbs2(uint64_t* w, int64_t x) {
-  at_add<S>(w + 0, (uint64_t)x & 0xFFFFFFFFull);
-  at_add<S>(w + 1, (uint64_t)(x >> 32));  // arithmetic shift: signed high limb
+__device__ __forceinline__ void add_limbs2(uint64_t* w, int ws, int64_t x) {
+  at_add<S>(w, (uint64_t)x & 0xFFFFFFFFull);
+  at_add<S>(w + ws, (uint64_t)(x >> 32));  // arithmetic shift: signed high limb
 }
 
 // ------------------------------------------------------ row update program
 
-// Apply one row to a group slot (LDS or global). `w` points at word 0 (key).
+// Apply one row to a group slot (LDS or global). `w` points at word 0 (key);
+// word k is at w[k * ws].
 template <typename S, bool JOINED>
-__device__ __forceinline__ void apply_row(uint64_t* w, const AggProgram& pg, const ColSet& cs,
+__device__ __forceinline__ void apply_row(uint64_t* w, int ws, const AggProgram& pg, const ColSet& cs,
                                           const RowRef& r) {
   for (int o = 0; o < pg.n_ops; ++o) {
     const OpDesc op = pg.ops[o];
-    uint64_t* t = w + op.word;
+    uint64_t* t = w + op.word * ws;
     switch (op.kind) {
       case OP_ROWCOUNT:
         at_add<S>(t, 1);
@@ -145,7 +160,7 @@ __device__ __forceinline__ void apply_row(uint64_t* w, const AggProgram& pg, con
         }
         break;
       case OP_SUMD:
-        if (rv_valid<JOINED>(cs, op.col, r)) add_limbs2<S>(t, rv_int<JOINED>(cs, op.col, r));
+        if (rv_valid<JOINED>(cs, op.col, r)) add_limbs2<S>(t, ws, rv_int<JOINED>(cs, op.col, r));
         break;
       case OP_SUMD4:
         if (rv_valid<JOINED>(cs, op.col, r) && (op.col2 < 0 || rv_valid<JOINED>(cs, op.col2, r))) {
@@ -164,7 +179,7 @@ __device__ __forceinline__ void apply_row(uint64_t* w, const AggProgram& pg, con
             lo = (uint64_t)a;
             hi = a < 0 ? -1 : 0;
           }
-          add_limbs4<S>(t, lo, hi);
+          add_limbs4<S>(t, ws, lo, hi);
         }
         break;
       case OP_MIN:
@@ -191,12 +206,13 @@ __device__ __forceinline__ void apply_row(uint64_t* w, const AggProgram& pg, con
   }
 }
 
-// Fold a partial slot (LDS entry or overflow record) into a global slot.
-__device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src, const AggProgram& pg) {
+// Fold a partial slot (LDS entry with word stride `sws`, or an overflow
+// record with sws = 1) into a global slot.
+__device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src, int sws, const AggProgram& pg) {
   for (int o = 0; o < pg.n_ops; ++o) {
     const OpDesc op = pg.ops[o];
     uint64_t* t = dst + op.word;
-    const uint64_t* s = src + op.word;
+    const uint64_t* s = src + op.word * sws;
     switch (op.kind) {
       case OP_ROWCOUNT: case OP_NNCOUNT: case OP_SUMI:
         if (s[0]) at_add<GlbT>(t, s[0]);
@@ -206,11 +222,11 @@ __device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src
         break;
       case OP_SUMD:
         if (s[0]) at_add<GlbT>(t, s[0]);
-        if (s[1]) at_add<GlbT>(t + 1, s[1]);
+        if (s[sws]) at_add<GlbT>(t + 1, s[sws]);
         break;
       case OP_SUMD4:
         for (int k = 0; k < 4; ++k)
-          if (s[k]) at_add<GlbT>(t + k, s[k]);
+          if (s[k * sws]) at_add<GlbT>(t + k, s[k * sws]);
         break;
       case OP_MIN:
         if (s[0] != kMinInit) at_min<GlbT>(t, s[0]);
@@ -226,19 +242,20 @@ __device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src
 
 // Fold a partial slot into a global slot this workgroup owns exclusively
 // (no other workgroup touches the key during the kernel): plain read-modify-write.
-__device__ __forceinline__ void merge_partial_owned(uint64_t* dst, const uint64_t* src, const AggProgram& pg) {
+__device__ __forceinline__ void merge_partial_owned(uint64_t* dst, const uint64_t* src, int sws,
+                                                    const AggProgram& pg) {
   for (int o = 0; o < pg.n_ops; ++o) {
     const OpDesc op = pg.ops[o];
     uint64_t* t = dst + op.word;
-    const uint64_t* s = src + op.word;
+    const uint64_t* s = src + op.word * sws;
     switch (op.kind) {
       case OP_ROWCOUNT: case OP_NNCOUNT: case OP_SUMI: t[0] += s[0]; break;
       case OP_SUMF:
         t[0] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)t[0]) +
                                               __longlong_as_double((long long)s[0]));
         break;
-      case OP_SUMD: t[0] += s[0]; t[1] += s[1]; break;
-      case OP_SUMD4: for (int k = 0; k < 4; ++k) t[k] += s[k]; break;
+      case OP_SUMD: t[0] += s[0]; t[1] += s[sws]; break;
+      case OP_SUMD4: for (int k = 0; k < 4; ++k) t[k] += s[k * sws]; break;
       case OP_MIN: if (s[0] < t[0]) t[0] = s[0]; break;
       case OP_MAX: if (s[0] > t[0]) t[0] = s[0]; break;
       default: break;
@@ -256,6 +273,12 @@ __host__ __device__ inline uint64_t slot_init_word(const AggProgram& pg, int wor
     }
   }
   return 0;
+}
+
+// Fill an LDS table (struct of arrays, L slots) with the empty-slot words.
+__device__ __forceinline__ void lds_table_init(uint64_t* lds, int L, const AggProgram& pg) {
+  const int lsh = __ffs(L) - 1;
+  for (int i = threadIdx.x; i < L * pg.stride; i += blockDim.x) lds[i] = slot_init_word(pg, i >> lsh);
 }
 
 // ------------------------------------------------------------- the tables
@@ -324,14 +347,17 @@ __device__ __forceinline__ void block_add(uint64_t* dst, uint64_t v) {
   if (threadIdx.x == 0 && s_sum) at_add<GlbT>(dst, s_sum);
 }
 
-// Per-workgroup LDS table: `lds_slots` slots of `stride` words.
+// Per-workgroup LDS table of `lds_slots` slots, struct of arrays: word k of
+// slot i is lds[k * lds_slots + i]. Random slots then spread over the LDS
+// banks (a power-of-two slot stride would put every key in the same few).
 constexpr int kLdsProbes = 8;
 
-__device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_mask, int stride, uint64_t key,
-                                                      uint64_t h, int shift = 40) {
+// Returns a pointer to the slot's key word (word k at ptr[k * lds_slots]).
+__device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_mask, uint64_t key, uint64_t h,
+                                                      int shift = 40) {
   uint32_t idx = (uint32_t)(h >> shift) & (uint32_t)lds_mask;
   for (int p = 0; p < kLdsProbes; ++p) {
-    uint64_t* s = lds + (uint64_t)idx * stride;
+    uint64_t* s = lds + idx;
     uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == key) return s;
     if (k == kEmpty) {
@@ -341,6 +367,192 @@ __device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_ma
     idx = (idx + 1) & (uint32_t)lds_mask;
   }
   return nullptr;
+}
+
+// ------------------------------------------------- batched row loading
+
+constexpr int kMaxVals = 7;  // value columns a preloaded row carries
+constexpr int kRowU = 4;     // rows per thread per step of the streaming loops
+
+// Raw 64-bit word of a column value: float64 bits, integers widened to int64.
+__device__ __forceinline__ uint64_t raw_word(const ColDesc& c, uint64_t row) {
+  if (c.type == WG_FLOAT64) return ((const uint64_t*)c.data)[row];
+  return (uint64_t)load_int(c, row);
+}
+__device__ __forceinline__ uint64_t valid_word(const ColDesc& c, uint64_t row) {
+  return c.valid ? c.valid[row >> 6] : ~0ull;
+}
+// pred_pass on a value already loaded (same typed comparison).
+__device__ __forceinline__ bool pred_raw(const ColDesc& c, uint64_t raw, uint64_t vw, uint64_t row, int op,
+                                         int64_t ci, double cf) {
+  if (!((vw >> (row & 63)) & 1ull)) return false;
+  switch (c.type) {
+    case WG_FLOAT64: return cmp_op<double>(__longlong_as_double((long long)raw), cf, op);
+    case WG_INT32: case WG_UINT16: return cmp_op<int32_t>((int32_t)(int64_t)raw, (int32_t)ci, op);
+    default: return cmp_op<int64_t>((int64_t)raw, ci, op);
+  }
+}
+
+// Load and classify kRowU rows per thread (positions i0 + u * step < end;
+// the row is sel[i] or row_base + i). Every load of every row is issued
+// before any value is used, so a wave keeps kRowU rows of traffic in flight.
+// kind: -1 filtered out or past the end, 0 keyed row, 1 sentinel-valued key,
+// 2 NULL single-column key. NV value columns (P.val_col[j < P.n_vals]) are
+// loaded as raw words with their validity bits in vvalid (bit j).
+// PP provides cs, key (at most two packed columns), pred[2], sel, row_base,
+// n_vals, val_col.
+template <int NV, typename PP>
+__device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t step, uint64_t end,
+                                          uint64_t (&row)[kRowU], uint64_t (&key)[kRowU], int (&kind)[kRowU],
+                                          uint64_t (&val)[kRowU][NV > 0 ? NV : 1], uint32_t (&vvalid)[kRowU]) {
+  bool in[kRowU];
+#pragma unroll
+  for (int u = 0; u < kRowU; ++u) {
+    const uint64_t i = i0 + u * step;
+    in[u] = i < end;
+    const uint64_t ii = in[u] ? i : end - 1;
+    row[u] = P.sel ? (uint64_t)P.sel[ii] : P.row_base + ii;
+  }
+  uint64_t pr[2][kRowU], pv[2][kRowU], kw[2][kRowU], kv[kRowU];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+    if (P.pred[q].col >= 0) {
+      const ColDesc& c = P.cs.cols[P.pred[q].col];
+#pragma unroll
+      for (int u = 0; u < kRowU; ++u) {
+        pr[q][u] = raw_word(c, row[u]);
+        pv[q][u] = valid_word(c, row[u]);
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k < P.key.n) {
+      const ColDesc& c = P.cs.cols[P.key.col[k]];
+#pragma unroll
+      for (int u = 0; u < kRowU; ++u) kw[k][u] = (uint64_t)load_int(c, row[u]);
+    }
+#pragma unroll
+  for (int u = 0; u < kRowU; ++u) kv[u] = valid_word(P.cs.cols[P.key.col[0]], row[u]);
+#pragma unroll
+  for (int u = 0; u < kRowU; ++u) vvalid[u] = ~0u;
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j < P.n_vals) {
+      const ColDesc& c = P.cs.cols[P.val_col[j]];
+#pragma unroll
+      for (int u = 0; u < kRowU; ++u) val[u][j] = raw_word(c, row[u]);
+      if (c.valid) {
+#pragma unroll
+        for (int u = 0; u < kRowU; ++u)
+          if (!((c.valid[row[u] >> 6] >> (row[u] & 63)) & 1ull)) vvalid[u] &= ~(1u << j);
+      }
+    }
+#pragma unroll
+  for (int u = 0; u < kRowU; ++u) {
+    bool ok = in[u];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (P.pred[q].col >= 0)
+        ok = ok && pred_raw(P.cs.cols[P.pred[q].col], pr[q][u], pv[q][u], row[u], P.pred[q].op, P.pred[q].i64,
+                            P.pred[q].f64);
+    uint64_t k;
+    if (P.key.n == 1) {
+      k = kw[0][u];
+    } else {
+      k = 0;
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        if (c < P.key.n) {
+          uint64_t v = kw[c][u];
+          if (P.key.narrow[c]) v = (v & 0xFFFFFFFFull) << P.key.shift[c];
+          k |= v;
+        }
+    }
+    key[u] = k;
+    // multi-column keys carry no validity (the host rejects them up front)
+    const bool null_key = P.key.n == 1 && !((kv[u] >> (row[u] & 63)) & 1ull);
+    kind[u] = !ok ? -1 : null_key ? 2 : (k == kEmpty ? 1 : 0);
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ uint64_t pick(const uint64_t (&v)[NV], int c) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+    if (j == c) r = v[j];
+  return r;
+}
+
+// apply_row for a preloaded row: the op's columns index the value words v
+// (float64 bits where vfloat bit j is set, else int64), vvalid bit j = value
+// j is not NULL. Same semantics as apply_row for every column type except
+// DECIMAL128 (16-byte values are not preloaded).
+template <typename S, int NV>
+__device__ __forceinline__ void apply_vals(uint64_t* w, int ws, const AggProgram& pg, const uint64_t (&v)[NV],
+                                           uint32_t vfloat, uint32_t vvalid) {
+  for (int o = 0; o < pg.n_ops; ++o) {
+    const OpDesc op = pg.ops[o];
+    uint64_t* t = w + op.word * ws;
+    const bool ok1 = op.col >= 0 && ((vvalid >> op.col) & 1u);
+    const bool ok2 = ok1 && (op.col2 < 0 || ((vvalid >> op.col2) & 1u));
+    switch (op.kind) {
+      case OP_ROWCOUNT:
+        at_add<S>(t, 1);
+        break;
+      case OP_NNCOUNT:
+        if (ok2) at_add<S>(t, 1);
+        break;
+      case OP_SUMI:
+        if (ok2) {
+          uint64_t x = pick(v, op.col);
+          if (op.col2 >= 0) x *= pick(v, op.col2);
+          at_add<S>(t, x);
+        }
+        break;
+      case OP_SUMF:
+        if (ok1) {
+          const uint64_t b = pick(v, op.col);
+          at_addf<S>(t, ((vfloat >> op.col) & 1u) ? __longlong_as_double((long long)b) : (double)(int64_t)b);
+        }
+        break;
+      case OP_SUMD:
+        if (ok1) add_limbs2<S>(t, ws, (int64_t)pick(v, op.col));
+        break;
+      case OP_SUMD4:
+        if (ok2) {
+          const int64_t a = (int64_t)pick(v, op.col);
+          uint64_t lo;
+          int64_t hi;
+          if (op.col2 >= 0) {
+            const int64_t b = (int64_t)pick(v, op.col2);
+            lo = (uint64_t)a * (uint64_t)b;
+            hi = (int64_t)__mul64hi((long long)a, (long long)b);
+          } else {
+            lo = (uint64_t)a;
+            hi = a < 0 ? -1 : 0;
+          }
+          add_limbs4<S>(t, ws, lo, hi);
+        }
+        break;
+      case OP_MIN: case OP_MAX:
+        if (ok1) {
+          const uint64_t b = pick(v, op.col);
+          uint64_t e;
+          if ((vfloat >> op.col) & 1u) {
+            if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) break;  // NaN never wins
+            e = enc_f64_bits(b);
+          } else {
+            e = enc_i64((int64_t)b);
+          }
+          if (op.kind == OP_MIN) at_min<S>(t, e);
+          else at_max<S>(t, e);
+        }
+        break;
+      default:
+        break;
+    }
+  }
 }
 
 // ----------------------------------------------------------- key building

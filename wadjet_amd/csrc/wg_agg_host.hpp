@@ -119,6 +119,7 @@ GlobalTable table_of(wg_agg* h);
 wg_status grow_to(wg_agg* h, uint64_t new_cap);
 wg_status normalize(wg_agg* h);
 wg_status ensure_overflow_buffers(wg_agg* h);
+wg_status ensure_overflow_for(wg_agg* h, uint64_t grid, int32_t lds_slots);
 wg_status ensure_base(wg_agg* h);
 wg_status agg_merge_partials(wg_agg* h, const uint64_t* recs, uint64_t n);
 bool part_eligible(const wg_agg* h, const ColSet& cs);

@@ -29,8 +29,7 @@
 
 namespace wg {
 
-constexpr int kMaxVals = 7;      // value words carried per record (64-byte records)
-constexpr int kPartU = 4;        // rows per thread per step of the count/scatter loops
+constexpr int kPartU = kRowU;    // rows per thread per step of the count/scatter loops
 constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
 constexpr int kPartBlock = 512;  // count/scatter workgroup size
 
@@ -41,6 +40,7 @@ struct PartParams {
   AggProgram pg;  // for side-slot rows in pass C (original columns)
   GlobalTable T;
   const uint32_t* sel;
+  uint64_t row_base;  // always 0: rows are sel[i] or i
   uint64_t n;
   uint64_t rows_per_wg;
   int32_t pbits;
@@ -51,98 +51,6 @@ struct PartParams {
   int32_t val_col[kMaxVals];
 };
 
-// Raw 64-bit word of a column value: float64 bits, integers widened to int64.
-__device__ __forceinline__ uint64_t raw_word(const ColDesc& c, uint64_t row) {
-  if (c.type == WG_FLOAT64) return ((const uint64_t*)c.data)[row];
-  return (uint64_t)load_int(c, row);
-}
-__device__ __forceinline__ uint64_t valid_word(const ColDesc& c, uint64_t row) {
-  return c.valid ? c.valid[row >> 6] : ~0ull;
-}
-// pred_pass on a value already loaded (same typed comparison).
-__device__ __forceinline__ bool pred_raw(const ColDesc& c, uint64_t raw, uint64_t vw, uint64_t row, int op,
-                                         int64_t ci, double cf) {
-  if (!((vw >> (row & 63)) & 1ull)) return false;
-  switch (c.type) {
-    case WG_FLOAT64: return cmp_op<double>(__longlong_as_double((long long)raw), cf, op);
-    case WG_INT32: case WG_UINT16: return cmp_op<int32_t>((int32_t)(int64_t)raw, (int32_t)ci, op);
-    default: return cmp_op<int64_t>((int64_t)raw, ci, op);
-  }
-}
-
-// Load and classify kPartU rows per thread (positions i0 + u * step). Every
-// load of every row is issued before any value is used, so a wave keeps
-// kPartU rows of memory traffic in flight.
-// kind: -1 filtered out or past the end, 0 partitioned, 1 sentinel-valued
-// key, 2 NULL single-column key.
-template <int NV>
-__device__ __forceinline__ void part_load(const PartParams& P, uint64_t i0, uint64_t step, uint64_t end,
-                                          uint64_t (&row)[kPartU], uint64_t (&key)[kPartU], int (&kind)[kPartU],
-                                          uint64_t (&val)[kPartU][NV + 1]) {
-  bool in[kPartU];
-#pragma unroll
-  for (int u = 0; u < kPartU; ++u) {
-    const uint64_t i = i0 + u * step;
-    in[u] = i < end;
-    const uint64_t ii = in[u] ? i : end - 1;
-    row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
-  }
-  uint64_t pr[2][kPartU], pv[2][kPartU], kw[2][kPartU], kv[kPartU];
-#pragma unroll
-  for (int q = 0; q < 2; ++q)
-    if (P.pred[q].col >= 0) {
-      const ColDesc& c = P.cs.cols[P.pred[q].col];
-#pragma unroll
-      for (int u = 0; u < kPartU; ++u) {
-        pr[q][u] = raw_word(c, row[u]);
-        pv[q][u] = valid_word(c, row[u]);
-      }
-    }
-  // packed keys are at most 8 bytes: one or two columns
-#pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (k < P.key.n) {
-      const ColDesc& c = P.cs.cols[P.key.col[k]];
-#pragma unroll
-      for (int u = 0; u < kPartU; ++u) kw[k][u] = (uint64_t)load_int(c, row[u]);
-    }
-#pragma unroll
-  for (int u = 0; u < kPartU; ++u) kv[u] = valid_word(P.cs.cols[P.key.col[0]], row[u]);
-#pragma unroll
-  for (int j = 0; j < NV; ++j)
-    if (j < P.n_vals) {
-      const ColDesc& c = P.cs.cols[P.val_col[j]];
-#pragma unroll
-      for (int u = 0; u < kPartU; ++u) val[u][j] = raw_word(c, row[u]);
-    }
-#pragma unroll
-  for (int u = 0; u < kPartU; ++u) {
-    bool ok = in[u];
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-      if (P.pred[q].col >= 0)
-        ok = ok && pred_raw(P.cs.cols[P.pred[q].col], pr[q][u], pv[q][u], row[u], P.pred[q].op, P.pred[q].i64,
-                            P.pred[q].f64);
-    uint64_t k;
-    if (P.key.n == 1) {
-      k = kw[0][u];
-    } else {
-      k = 0;
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-        if (c < P.key.n) {
-          uint64_t v = kw[c][u];
-          if (P.key.narrow[c]) v = (v & 0xFFFFFFFFull) << P.key.shift[c];
-          k |= v;
-        }
-    }
-    key[u] = k;
-    // multi-column keys carry no validity on this path (checked by the host)
-    const bool null_key = P.key.n == 1 && !((kv[u] >> (row[u] & 63)) & 1ull);
-    kind[u] = !ok ? -1 : null_key ? 2 : (k == kEmpty ? 1 : 0);
-  }
-}
-
 __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   extern __shared__ uint32_t hist[];
   const uint32_t np = 1u << P.pbits;
@@ -152,8 +60,9 @@ __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
   for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kPartU) {
     uint64_t row[kPartU], key[kPartU], val[kPartU][1];
+    uint32_t vvalid[kPartU];
     int kind[kPartU];
-    part_load<0>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val);
+    load_rows<0>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
     for (int u = 0; u < kPartU; ++u)
       if (kind[u] == 0) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], 1u);
@@ -174,16 +83,17 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
   const int stride = P.pg.stride;
   for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kPartU) {
-    uint64_t row[kPartU], key[kPartU], val[kPartU][NV + 1];
+    uint64_t row[kPartU], key[kPartU], val[kPartU][NV];
+    uint32_t vvalid[kPartU];
     int kind[kPartU];
-    part_load<NV>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val);
+    load_rows<NV>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
     for (int u = 0; u < kPartU; ++u) {
       if (kind[u] < 0) continue;
       if (kind[u] != 0) {
         // side slots: [0] the sentinel-valued key, [1] the NULL key
         uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
-        apply_row<GlbT, false>(s, P.pg, P.cs, RowRef{row[u], 0, nullptr});
+        apply_row<GlbT, false>(s, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
         continue;
       }
       const uint32_t pos = atomicAdd(&cur[key_hash(key[u]) >> (64 - P.pbits)], 1u);
@@ -232,76 +142,6 @@ struct PartAggParams {
   uint32_t vfloat;  // bit j: value word j holds float64 bits (else an int64)
 };
 
-template <int NV>
-__device__ __forceinline__ uint64_t pick(const uint64_t (&v)[NV], int c) {
-  uint64_t r = 0;
-#pragma unroll
-  for (int j = 0; j < NV; ++j)
-    if (j == c) r = v[j];
-  return r;
-}
-
-// apply_row for a partitioned record: the op's columns index the value words.
-// Value columns on this path carry no NULLs (checked by the host), so
-// COUNT(col) counts every row as COUNT(*) does.
-template <typename S, int NV>
-__device__ __forceinline__ void apply_vals(uint64_t* w, const AggProgram& pg, const uint64_t (&v)[NV],
-                                           uint32_t vfloat) {
-  for (int o = 0; o < pg.n_ops; ++o) {
-    const OpDesc op = pg.ops[o];
-    uint64_t* t = w + op.word;
-    switch (op.kind) {
-      case OP_ROWCOUNT: case OP_NNCOUNT:
-        at_add<S>(t, 1);
-        break;
-      case OP_SUMI: {
-        uint64_t x = pick(v, op.col);
-        if (op.col2 >= 0) x *= pick(v, op.col2);
-        at_add<S>(t, x);
-        break;
-      }
-      case OP_SUMF: {
-        const uint64_t b = pick(v, op.col);
-        at_addf<S>(t, ((vfloat >> op.col) & 1u) ? __longlong_as_double((long long)b) : (double)(int64_t)b);
-        break;
-      }
-      case OP_SUMD:
-        add_limbs2<S>(t, (int64_t)pick(v, op.col));
-        break;
-      case OP_SUMD4: {
-        const int64_t a = (int64_t)pick(v, op.col);
-        uint64_t lo;
-        int64_t hi;
-        if (op.col2 >= 0) {
-          const int64_t b = (int64_t)pick(v, op.col2);
-          lo = (uint64_t)a * (uint64_t)b;
-          hi = (int64_t)__mul64hi((long long)a, (long long)b);
-        } else {
-          lo = (uint64_t)a;
-          hi = a < 0 ? -1 : 0;
-        }
-        add_limbs4<S>(t, lo, hi);
-        break;
-      }
-      case OP_MIN: case OP_MAX: {
-        const uint64_t b = pick(v, op.col);
-        uint64_t e;
-        if ((vfloat >> op.col) & 1u) {
-          if ((b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) break;  // NaN never wins
-          e = enc_f64_bits(b);
-        } else {
-          e = enc_i64((int64_t)b);
-        }
-        if (op.kind == OP_MIN) at_min<S>(t, e);
-        else at_max<S>(t, e);
-        break;
-      }
-      default:
-        break;
-    }
-  }
-}
-
 // Claim-or-find with the CAS as the probe: one round trip per probed slot.
 __device__ __forceinline__ uint64_t* glb_claim_from(const GlobalTable& T, int stride, uint64_t key, uint64_t idx,
                                                     uint64_t first_p, bool& fresh) {
@@ -329,6 +169,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
   __shared__ uint64_t s_init[16];
   const int stride = P.pg.stride;
   const int L = P.lds_slots;
+  const int lsh = __ffs(L) - 1;  // the LDS table is struct of arrays: word k of slot i at lds[k * L + i]
   if (threadIdx.x < (unsigned)stride) s_init[threadIdx.x] = slot_init_word(P.pg, threadIdx.x);
   uint32_t claims = 0;
   const uint64_t n_work = P.sel ? (P.n_sel + P.sel_chunk - 1) / P.sel_chunk : P.n_items;
@@ -362,7 +203,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
       end = it.end;
       split = it.split != 0;
     }
-    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = s_init[i & (stride - 1)];
+    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = s_init[i >> lsh];
     __syncthreads();
     for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kAggU) {
       uint64_t rec[kAggU][RW];
@@ -393,9 +234,9 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
 #pragma unroll
         for (int j = 0; j < NV; ++j) v[j] = rec[u][j + 1];
         const uint64_t h = key_hash(key);
-        uint64_t* ls = lds_find_or_claim(lds, L - 1, stride, key, h, 16);
+        uint64_t* ls = lds_find_or_claim(lds, L - 1, key, h, 16);
         if (ls) {
-          apply_vals<LdsT, NV>(ls, P.pg, v, P.vfloat);
+          apply_vals<LdsT, NV>(ls, L, P.pg, v, P.vfloat, ~0u);
           continue;
         }
         uint64_t* g = glb_find_or_claim(P.T, stride, key, h, claims);
@@ -405,7 +246,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
           P.ovf_rows[at] = (uint32_t)pos[u];  // in bounds: items are taken only with headroom
           continue;
         }
-        apply_vals<GlbT, NV>(g, P.pg, v, P.vfloat);
+        apply_vals<GlbT, NV>(g, 1, P.pg, v, P.vfloat, ~0u);
       }
     }
     __syncthreads();
@@ -416,7 +257,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
 #pragma unroll
       for (int u = 0; u < kFlushU; ++u) {
         const int i = i0 + u * blockDim.x;
-        key[u] = i < L ? lds[(uint64_t)i * stride] : kEmpty;
+        key[u] = i < L ? lds[i] : kEmpty;
         home[u] = key_hash(key[u]) & P.T.mask;
       }
 #pragma unroll
@@ -425,7 +266,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
 #pragma unroll
       for (int u = 0; u < kFlushU; ++u) {
         if (key[u] == kEmpty) continue;
-        const uint64_t* ls = lds + (uint64_t)(i0 + u * blockDim.x) * stride;
+        const uint64_t* ls = lds + (i0 + u * blockDim.x);
         bool fresh = prev[u] == kEmpty;
         uint64_t* g = (fresh || prev[u] == key[u])
                           ? P.T.slots + home[u] * (uint64_t)stride
@@ -434,19 +275,20 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
           const uint64_t at =
               __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           uint64_t* rec = P.ovf_part + at * (uint64_t)stride;  // in bounds: headroom as above
-          for (int w = 0; w < stride; ++w) rec[w] = ls[w];
+          for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * L];
           continue;
         }
         if (fresh) ++claims;
         if (split) {
-          merge_partial(g, ls, P.pg);
+          merge_partial(g, ls, L, P.pg);
         } else if (fresh) {
           // a slot this workgroup just claimed for a key no one else holds:
           // its words are still the initial values, store the partial
-          g[1] = ls[1];
-          for (int w = 2; w < stride; w += 2) *(ulonglong2*)(g + w) = make_ulonglong2(ls[w], ls[w + 1]);
+          g[1] = ls[L];
+          for (int w = 2; w < stride; w += 2)
+            *(ulonglong2*)(g + w) = make_ulonglong2(ls[(uint64_t)w * L], ls[(uint64_t)(w + 1) * L]);
         } else {
-          merge_partial_owned(g, ls, P.pg);
+          merge_partial_owned(g, ls, L, P.pg);
         }
       }
     }
@@ -478,12 +320,13 @@ static wg_status allow_lds(const void* kernel, size_t bytes) {
   return WG_OK;
 }
 
+using ScatterFn = void (*)(PartParams);
+static ScatterFn scatter_kernel_for(int rw) {
+  return rw == 2 ? part_scatter_kernel<2> : rw == 4 ? part_scatter_kernel<4> : part_scatter_kernel<8>;
+}
+
 static wg_status launch_scatter(int rw, dim3 grid, size_t lds, hipStream_t st, const PartParams& P) {
-  switch (rw) {
-    case 2: hipLaunchKernelGGL(part_scatter_kernel<2>, grid, dim3(kPartBlock), lds, st, P); break;
-    case 4: hipLaunchKernelGGL(part_scatter_kernel<4>, grid, dim3(kPartBlock), lds, st, P); break;
-    default: hipLaunchKernelGGL(part_scatter_kernel<8>, grid, dim3(kPartBlock), lds, st, P); break;
-  }
+  hipLaunchKernelGGL(scatter_kernel_for(rw), grid, dim3(kPartBlock), lds, st, P);
   WG_HIP(hipGetLastError());
   return WG_OK;
 }
@@ -536,7 +379,12 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   while (pbits < 14 && (1ull << pbits) * (uint64_t)(L / 2) < est) ++pbits;
   const uint32_t np = 1u << pbits;
   const size_t hist_bytes = (size_t)np * 4;
-  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / hist_bytes));
+  // one resident round of count/scatter workgroups (the two passes must use
+  // the same row ranges, so both run at the occupancy of the lower one)
+  int occ_c = 0, occ_s = 0;
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, part_count_kernel, kPartBlock, hist_bytes));
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_kernel_for(RW), kPartBlock, hist_bytes));
+  const int per_cu = std::max(1, std::min(occ_c, occ_s));
   const int G = ctx->cus * per_cu;
   const uint64_t R = ((n + G - 1) / G + 255) / 256 * 256;
 

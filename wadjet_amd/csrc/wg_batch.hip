@@ -82,11 +82,13 @@ __device__ __forceinline__ uint64_t* bq_slot(const BQ& q, uint64_t key) {
   return nullptr;
 }
 
+// Word k of the slot is s[k * ws]: ws = 1 in the arena (array of structs),
+// the slot count in the LDS table (struct of arrays).
 template <typename S>
-__device__ __forceinline__ void bq_apply(uint64_t* s, int64_t v) {
-  at_add<S>(s + 1, 1);
-  at_add<S>(s + 2, (uint64_t)v & 0xFFFFFFFFull);
-  at_add<S>(s + 3, (uint64_t)(v >> 32));
+__device__ __forceinline__ void bq_apply(uint64_t* s, uint32_t ws, int64_t v) {
+  at_add<S>(s + ws, 1);
+  at_add<S>(s + 2 * ws, (uint64_t)v & 0xFFFFFFFFull);
+  at_add<S>(s + 3 * ws, (uint64_t)(v >> 32));
 }
 
 __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
     BCounters* c = P.ctr + W.q;
     const uint32_t L = q.lds_slots;
     if (L) {
-      for (uint32_t i = threadIdx.x; i < L * kBStride; i += blockDim.x) lds[i] = (i % kBStride) == 0 ? kEmpty : 0;
+      for (uint32_t i = threadIdx.x; i < L * kBStride; i += blockDim.x) lds[i] = i < L ? kEmpty : 0;
       __syncthreads();
     }
     for (uint64_t r = W.beg + threadIdx.x; r < W.end; r += blockDim.x) {
@@ -125,9 +127,9 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
       }
       const int64_t v = load_int(q.val, r);
       if (L && key != kEmpty) {
-        uint64_t* ls = lds_find_or_claim(lds, (int)L - 1, kBStride, key, mix64(key));
+        uint64_t* ls = lds_find_or_claim(lds, (int)L - 1, key, mix64(key));
         if (ls) {
-          bq_apply<LdsT>(ls, v);
+          bq_apply<LdsT>(ls, L, v);
           continue;
         }
       }
@@ -136,21 +138,21 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
         atomicAdd(&c->overflow, 1ull);  // rare: the query is re-run alone
         continue;
       }
-      bq_apply<GlbT>(s, v);
+      bq_apply<GlbT>(s, 1, v);
     }
     if (L) {
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < L; i += blockDim.x) {
-        const uint64_t* ls = lds + (uint64_t)i * kBStride;
+        const uint64_t* ls = lds + i;
         if (ls[0] == kEmpty) continue;
         uint64_t* s = bq_slot(q, ls[0]);
         if (!s) {
           atomicAdd(&c->overflow, 1ull);
           continue;
         }
-        at_add<GlbT>(s + 1, ls[1]);
-        at_add<GlbT>(s + 2, ls[2]);
-        at_add<GlbT>(s + 3, ls[3]);
+        at_add<GlbT>(s + 1, ls[L]);
+        at_add<GlbT>(s + 2, ls[2 * L]);
+        at_add<GlbT>(s + 3, ls[3 * L]);
       }
       __syncthreads();
     }

@@ -261,7 +261,7 @@ __device__ __forceinline__ uint64_t* group_slot(const JoinAggParams& P, uint64_t
   if (key == kEmpty) return P.T.slots + (P.T.mask + 1) * (uint64_t)P.pg.stride;
   const uint64_t h = key_hash(key);
   if (USE_LDS) {
-    uint64_t* ls = lds_find_or_claim(lds, L - 1, P.pg.stride, key, h);
+    uint64_t* ls = lds_find_or_claim(lds, L - 1, key, h);
     if (ls) {
       in_lds = true;
       return ls;
@@ -270,14 +270,108 @@ __device__ __forceinline__ uint64_t* group_slot(const JoinAggParams& P, uint64_t
   return glb_find_or_claim(P.T, P.pg.stride, key, h, claims);
 }
 
+// The probe row loop is software-pipelined: while row k is aggregated, the
+// home slot of row k+1 and the key of row k+2 are already being loaded.
+struct ProbeKey {
+  uint64_t row, key, vw;
+  bool live;
+};
+struct ProbeHome {
+  ProbeKey k;
+  uint64_t home;
+  uint64_t w0, w1, w2, w3;  // the home slot: key, head|count, two payload words
+};
+
+__device__ __forceinline__ ProbeKey probe_key(const JoinAggParams& P, uint64_t i, uint64_t beg, uint64_t end) {
+  ProbeKey r;
+  r.live = i < end;
+  const uint64_t ii = r.live ? i : beg;  // past the end: load a row in range, then ignore it
+  r.row = P.sel ? (uint64_t)P.sel[ii] : P.row_base + ii;
+  r.key = (uint64_t)load_int(P.pkey, r.row);
+  r.vw = P.pkey.valid ? P.pkey.valid[r.row >> 6] : ~0ull;
+  return r;
+}
+
+__device__ __forceinline__ ProbeHome probe_home(const JoinAggParams& P, const ProbeKey& k) {
+  ProbeHome h;
+  h.k = k;
+  h.home = mix64(k.key) & P.J.mask;
+  const ulonglong2* s = (const ulonglong2*)(P.J.slots + h.home * (uint64_t)P.J.stride);
+  const ulonglong2 a = s[0];
+  h.w0 = a.x;
+  h.w1 = a.y;
+  h.w2 = h.w3 = 0;
+  if (P.J.stride >= 4) {
+    const ulonglong2 b = s[1];
+    h.w2 = b.x;
+    h.w3 = b.y;
+  }
+  return h;
+}
+
+// Aggregate one probe row's matches (the home slot already loaded).
+template <bool USE_LDS>
+__device__ __forceinline__ void join_agg_row(const JoinAggParams& P, uint64_t* lds, int L, const ProbeHome& h,
+                                             uint32_t& claims) {
+  const ProbeKey& k = h.k;
+  if (!k.live || !((k.vw >> (k.row & 63)) & 1ull)) return;  // NULL keys never match
+  const uint64_t* js;
+  bool at_home = false;
+  if (k.key == kEmpty) {
+    js = join_find(P.J, k.key);
+  } else if (h.w0 == k.key) {
+    js = P.J.slots + h.home * (uint64_t)P.J.stride;
+    at_home = true;
+  } else if (h.w0 == kEmpty) {
+    return;
+  } else {
+    js = join_find_from(P.J, k.key, (h.home + 1) & P.J.mask);
+  }
+  if (!js) return;
+  const uint32_t cnt = at_home ? (uint32_t)(h.w1 >> 32) : slot_count(js);
+  bool in_lds;
+  if (cnt == 1) {
+    RowRef r{k.row, (uint64_t)(at_home ? (int32_t)(uint32_t)h.w1 : slot_head(js)), js, h.w2, h.w3};
+    if (!at_home && P.J.stride >= 4) {
+      r.pay2 = js[2];
+      r.pay3 = js[3];
+    }
+    uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims);
+    if (!g) {
+      const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      P.ovf_rows[at] = (uint32_t)k.row;
+      return;
+    }
+    if (in_lds) apply_row<LdsT, true>(g, L, P.pg, P.cs, r);
+    else apply_row<GlbT, true>(g, 1, P.pg, P.cs, r);
+    return;
+  }
+  // several build rows: claim every group first, apply only if all fit
+  bool ok = true;
+  for (int32_t b = slot_head(js); b >= 0 && ok; b = P.J.next[b]) {
+    const RowRef r{k.row, (uint64_t)b, nullptr};
+    ok = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims) != nullptr;
+  }
+  if (!ok) {
+    const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    P.ovf_rows[at] = (uint32_t)k.row;
+    return;
+  }
+  for (int32_t b = slot_head(js); b >= 0; b = P.J.next[b]) {
+    const RowRef r{k.row, (uint64_t)b, nullptr};
+    uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims);
+    if (in_lds) apply_row<LdsT, true>(g, L, P.pg, P.cs, r);
+    else apply_row<GlbT, true>(g, 1, P.pg, P.cs, r);
+  }
+}
+
 template <bool USE_LDS>
 __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
   extern __shared__ uint64_t lds[];
   __shared__ int64_t s_tile;
   const int stride = P.pg.stride;
   const int L = USE_LDS ? P.lds_slots : 0;
-  if (USE_LDS)
-    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = slot_init_word(P.pg, i % stride);
+  if (USE_LDS) lds_table_init(lds, L, P.pg);
   __syncthreads();
   uint32_t claims = 0;
   for (;;) {
@@ -298,56 +392,31 @@ __global__ void __launch_bounds__(1024) join_agg_kernel(JoinAggParams P) {
     if (tile < 0) break;
     const uint64_t beg = (uint64_t)tile * P.tile_rows;
     const uint64_t end = min(beg + P.tile_rows, P.n);
-    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-      const uint64_t prow = P.sel ? (uint64_t)P.sel[i] : P.row_base + i;
-      const uint64_t* js = probe_slot(P.J, P.pkey, prow);
-      if (!js) continue;
-      const uint32_t cnt = slot_count(js);
-      bool in_lds;
-      if (cnt == 1) {
-        const RowRef r{prow, (uint64_t)slot_head(js), js};
-        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims);
-        if (!g) {
-          const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          P.ovf_rows[at] = (uint32_t)prow;
-          continue;
-        }
-        if (in_lds) apply_row<LdsT, true>(g, P.pg, P.cs, r);
-        else apply_row<GlbT, true>(g, P.pg, P.cs, r);
-        continue;
-      }
-      // several build rows: claim every group first, apply only if all fit
-      bool ok = true;
-      for (int32_t b = slot_head(js); b >= 0 && ok; b = P.J.next[b]) {
-        const RowRef r{prow, (uint64_t)b, nullptr};
-        ok = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims) != nullptr;
-      }
-      if (!ok) {
-        const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        P.ovf_rows[at] = (uint32_t)prow;
-        continue;
-      }
-      for (int32_t b = slot_head(js); b >= 0; b = P.J.next[b]) {
-        const RowRef r{prow, (uint64_t)b, nullptr};
-        uint64_t* g = group_slot<USE_LDS>(P, lds, L, r, in_lds, claims);
-        if (in_lds) apply_row<LdsT, true>(g, P.pg, P.cs, r);
-        else apply_row<GlbT, true>(g, P.pg, P.cs, r);
-      }
+    const uint64_t B = blockDim.x;
+    uint64_t i = beg + threadIdx.x;
+    ProbeKey k1 = probe_key(P, i + B, beg, end);
+    ProbeHome h0 = probe_home(P, probe_key(P, i, beg, end));
+    for (; i < end; i += B) {
+      const ProbeKey k2 = probe_key(P, i + 2 * B, beg, end);
+      const ProbeHome h1 = probe_home(P, k1);
+      join_agg_row<USE_LDS>(P, lds, L, h0, claims);
+      h0 = h1;
+      k1 = k2;
     }
   }
   if (USE_LDS) {
     __syncthreads();
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
-      const uint64_t* ls = lds + (uint64_t)i * stride;
+      const uint64_t* ls = lds + i;
       const uint64_t key = ls[0];
       if (key == kEmpty) continue;
       uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key), claims);
       if (slot != nullptr) {
-        merge_partial(slot, ls, P.pg);
+        merge_partial(slot, ls, L, P.pg);
       } else {
         const uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t* rec = P.ovf_part + at * (uint64_t)stride;
-        for (int w = 0; w < stride; ++w) rec[w] = ls[w];
+        for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * L];
       }
     }
   }
@@ -364,7 +433,7 @@ __global__ void agg_merge_part_kernel_j(const uint64_t* recs, uint64_t n, Global
       at_add<GlbT>(T.counters + CTR_STOPPED, 1ull << 32);
       continue;
     }
-    merge_partial(d, r, pg);
+    merge_partial(d, r, 1, pg);
   }
   block_add(T.counters + CTR_GROUPS, claims);
 }
@@ -799,8 +868,19 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
   }
   P.key = agg->key;
   P.pg = agg->pg;
+  // a join usually feeds a small GROUP BY: size the LDS table to the expected
+  // groups (at most the aggregate's setting) so more workgroups fit per CU
   P.lds_slots = agg->lds_slots;
+  if (P.lds_slots > 0 && agg->group_hint > 0)
+    P.lds_slots = (int32_t)std::min<uint64_t>((uint64_t)P.lds_slots,
+                                              next_pow2(std::max<uint64_t>(64, 2 * agg->group_hint)));
   P.tile_rows = agg->tile_rows;
+  const int lds_bytes_j = P.lds_slots * P.pg.stride * 8;
+  const int per_cu_j = P.lds_slots > 0
+                           ? std::max(1, std::min(160 * 1024 / std::max(lds_bytes_j, 1), 2048 / agg->block))
+                           : std::max(1, 2048 / agg->block);
+  const int grid_j = agg->ctx->cus * per_cu_j;
+  WG_TRY(ensure_overflow_for(agg, (uint64_t)grid_j, P.lds_slots));
   agg->st_kernel_ms = 0.f;
   agg->st_launches = 0;
   if (!agg->ev0) {
@@ -825,7 +905,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
       P.n = seg.n - pos;
       P.n_tiles = (P.n + P.tile_rows - 1) / P.tile_rows;
       WG_TRY(zero_dev(agg->ctx, agg->counters + CTR_OVF_ROWS, CTR_N - 1));
-      const int grid = (int)std::min<uint64_t>((uint64_t)agg->grid, std::max<uint64_t>(P.n_tiles, 1));
+      const int grid = (int)std::min<uint64_t>((uint64_t)grid_j, std::max<uint64_t>(P.n_tiles, 1));
       WG_HIP(hipEventRecord(agg->ev0, agg->ctx->stream));
       if (P.lds_slots > 0) {
         static bool attr_done = false;

@@ -19,13 +19,8 @@ struct JoinTable {
 __device__ __forceinline__ int32_t slot_head(const uint64_t* s) { return (int32_t)(uint32_t)s[1]; }
 __device__ __forceinline__ uint32_t slot_count(const uint64_t* s) { return (uint32_t)(s[1] >> 32); }
 
-// The slot of `key`, or null when the key has no build row.
-__device__ __forceinline__ const uint64_t* join_find(const JoinTable& J, uint64_t key) {
-  if (key == kEmpty) {
-    const uint64_t* s = J.slots + (J.mask + 1) * (uint64_t)J.stride;
-    return slot_count(s) ? s : nullptr;
-  }
-  uint64_t idx = mix64(key) & J.mask;
+// Continue a probe for `key` at slot `idx` (after a miss at the slot before).
+__device__ __forceinline__ const uint64_t* join_find_from(const JoinTable& J, uint64_t key, uint64_t idx) {
   for (uint64_t p = 0; p <= J.mask; ++p) {
     const uint64_t* s = J.slots + idx * (uint64_t)J.stride;
     const uint64_t k = s[0];
@@ -34,6 +29,15 @@ __device__ __forceinline__ const uint64_t* join_find(const JoinTable& J, uint64_
     idx = (idx + 1) & J.mask;
   }
   return nullptr;
+}
+
+// The slot of `key`, or null when the key has no build row.
+__device__ __forceinline__ const uint64_t* join_find(const JoinTable& J, uint64_t key) {
+  if (key == kEmpty) {
+    const uint64_t* s = J.slots + (J.mask + 1) * (uint64_t)J.stride;
+    return slot_count(s) ? s : nullptr;
+  }
+  return join_find_from(J, key, mix64(key) & J.mask);
 }
 
 }  // namespace wg
