@@ -48,7 +48,7 @@ struct ConsumeParams {
   uint32_t vfloat;  // bit j: value j is float64 bits
 };
 
-// NV > 0: every value a row needs is preloaded with its keys (kRowU rows per
+// NV > 0: every value a row needs is preloaded with its keys (kRowU2 rows per
 // thread in flight) and applied with apply_vals; NV = 0: values are read by
 // apply_row when the row is applied (DECIMAL128 inputs, more than three value
 // columns).
@@ -82,13 +82,13 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
 
     const uint64_t beg = (uint64_t)tile * P.tile_rows;
     const uint64_t end = min(beg + P.tile_rows, P.n);
-    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kRowU) {
-      uint64_t row[kRowU], key[kRowU], val[kRowU][NV > 0 ? NV : 1];
-      uint32_t vvalid[kRowU];
-      int kind[kRowU];
-      load_rows<NV>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kRowU2) {
+      uint64_t row[kRowU2], key[kRowU2], val[kRowU2][NV > 0 ? NV : 1];
+      uint32_t vvalid[kRowU2];
+      int kind[kRowU2];
+      load_rows<NV, kRowU2>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
-      for (int u = 0; u < kRowU; ++u) {
+      for (int u = 0; u < kRowU2; ++u) {
         if (kind[u] < 0) continue;
         uint64_t* slot = nullptr;
         int ws = 1;

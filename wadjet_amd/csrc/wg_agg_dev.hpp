@@ -372,16 +372,65 @@ __device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_ma
 // ------------------------------------------------- batched row loading
 
 constexpr int kMaxVals = 7;  // value columns a preloaded row carries
-constexpr int kRowU = 4;     // rows per thread per step of the streaming loops
+constexpr int kRowU = 4;     // rows per thread per step of the streaming loops (count pass)
+constexpr int kRowU2 = 2;    // the same where more registers are live per row
 
 // Raw 64-bit word of a column value: float64 bits, integers widened to int64.
 __device__ __forceinline__ uint64_t raw_word(const ColDesc& c, uint64_t row) {
   if (c.type == WG_FLOAT64) return ((const uint64_t*)c.data)[row];
   return (uint64_t)load_int(c, row);
 }
+
+// A column value loaded with no branch on the column type. The type is
+// uniform, but loads of different widths in the arms of a branch merge into
+// one register with a copy, and the copy waits for the load right there: the
+// loads of a batch of rows then run one at a time. Here every value takes the
+// same three loads: the two 32-bit halves of a 4- or 8-byte element (the high
+// half re-reads the low one for 4-byte types) and a 16-bit load for uint16;
+// loads a type does not need read a fixed word in kSafeWord instead, so no
+// access leaves the element or that word. value_of() assembles the value
+// once every load of the batch is in flight.
+__device__ const uint64_t kSafeWord = 0;
+
+struct RawVal {
+  uint32_t lo, hi, h16;
+};
+
+// Column data is device (global) memory: say so, so the selected pointers
+// below still load with global (not flat) instructions.
+typedef const __attribute__((address_space(1))) uint32_t* gptr_u32;
+typedef const __attribute__((address_space(1))) uint16_t* gptr_u16;
+typedef const __attribute__((address_space(1))) char* gptr_c;
+
+// `active` is uniform; an inactive load (a column the query does not have)
+// reads kSafeWord three times, so a batch's load sequence has no branches.
+__device__ __forceinline__ RawVal load_raw(const ColDesc& c, uint64_t row, bool active = true) {
+  const int t = c.type;
+  const bool w2 = active && t == WG_UINT16;
+  const bool w8 = active && !(t == WG_UINT16 || t == WG_INT32 || t == WG_UINT32);
+  const uint64_t es = w2 ? 2 : w8 ? (t == WG_DECIMAL128 ? 16 : 8) : 4;
+  const gptr_c e = (gptr_c)c.data + row * es + (t == WG_DECIMAL128 ? 8 : 0);  // DECIMAL128: the Lo word
+  const gptr_u32 safe = (gptr_u32)&kSafeWord;
+  const gptr_u32 plo = (w2 || !active) ? safe : (gptr_u32)e;
+  const gptr_u32 phi = w8 ? plo + 1 : plo;
+  const gptr_u16 p16 = w2 ? (gptr_u16)e : (gptr_u16)safe;
+  return RawVal{*plo, *phi, *p16};
+}
+
+// raw_word's form: float64 bits, integers widened to int64.
+__device__ __forceinline__ uint64_t value_of(int type, const RawVal& r) {
+  switch (type) {
+    case WG_INT32: return (uint64_t)(int64_t)(int32_t)r.lo;
+    case WG_UINT32: return r.lo;
+    case WG_UINT16: return r.h16;
+    default: return ((uint64_t)r.hi << 32) | r.lo;
+  }
+}
+
 __device__ __forceinline__ uint64_t valid_word(const ColDesc& c, uint64_t row) {
   return c.valid ? c.valid[row >> 6] : ~0ull;
 }
+
 // pred_pass on a value already loaded (same typed comparison).
 __device__ __forceinline__ bool pred_raw(const ColDesc& c, uint64_t raw, uint64_t vw, uint64_t row, int op,
                                          int64_t ci, double cf) {
@@ -393,77 +442,92 @@ __device__ __forceinline__ bool pred_raw(const ColDesc& c, uint64_t raw, uint64_
   }
 }
 
-// Load and classify kRowU rows per thread (positions i0 + u * step < end;
+// Load and classify U rows per thread (positions i0 + u * step < end;
 // the row is sel[i] or row_base + i). Every load of every row is issued
-// before any value is used, so a wave keeps kRowU rows of traffic in flight.
+// before any value is used, so a wave keeps U rows of traffic in flight.
 // kind: -1 filtered out or past the end, 0 keyed row, 1 sentinel-valued key,
 // 2 NULL single-column key. NV value columns (P.val_col[j < P.n_vals]) are
 // loaded as raw words with their validity bits in vvalid (bit j).
 // PP provides cs, key (at most two packed columns), pred[2], sel, row_base,
 // n_vals, val_col.
-template <int NV, typename PP>
+template <int NV, int U, typename PP>
 __device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t step, uint64_t end,
-                                          uint64_t (&row)[kRowU], uint64_t (&key)[kRowU], int (&kind)[kRowU],
-                                          uint64_t (&val)[kRowU][NV > 0 ? NV : 1], uint32_t (&vvalid)[kRowU]) {
-  bool in[kRowU];
+                                          uint64_t (&row)[U], uint64_t (&key)[U], int (&kind)[U],
+                                          uint64_t (&val)[U][NV > 0 ? NV : 1], uint32_t (&vvalid)[U]) {
+  bool in[U];
 #pragma unroll
-  for (int u = 0; u < kRowU; ++u) {
+  for (int u = 0; u < U; ++u) {
     const uint64_t i = i0 + u * step;
     in[u] = i < end;
     const uint64_t ii = in[u] ? i : end - 1;
     row[u] = P.sel ? (uint64_t)P.sel[ii] : P.row_base + ii;
   }
-  uint64_t pr[2][kRowU], pv[2][kRowU], kw[2][kRowU], kv[kRowU];
+  // loads: no use of a loaded value until every load of the batch is issued
+  RawVal pr[2][U], kw[2][U], vr[U][NV > 0 ? NV : 1];
+  uint64_t pv[2][U], kv[U], vv[U][NV > 0 ? NV : 1];
 #pragma unroll
-  for (int q = 0; q < 2; ++q)
-    if (P.pred[q].col >= 0) {
-      const ColDesc& c = P.cs.cols[P.pred[q].col];
+  for (int q = 0; q < 2; ++q) {
+    const bool on = P.pred[q].col >= 0;
+    const ColDesc& c = P.cs.cols[on ? P.pred[q].col : 0];
 #pragma unroll
-      for (int u = 0; u < kRowU; ++u) {
-        pr[q][u] = raw_word(c, row[u]);
-        pv[q][u] = valid_word(c, row[u]);
-      }
+    for (int u = 0; u < U; ++u) {
+      pr[q][u] = load_raw(c, row[u], on);
+      pv[q][u] = on ? valid_word(c, row[u]) : ~0ull;
     }
+  }
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
-    if (k < P.key.n) {
-      const ColDesc& c = P.cs.cols[P.key.col[k]];
+  for (int k = 0; k < 2; ++k) {
+    const bool on = k < P.key.n;
+    const ColDesc& c = P.cs.cols[P.key.col[on ? k : 0]];
 #pragma unroll
-      for (int u = 0; u < kRowU; ++u) kw[k][u] = (uint64_t)load_int(c, row[u]);
+    for (int u = 0; u < U; ++u) kw[k][u] = load_raw(c, row[u], on);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) kv[u] = valid_word(P.cs.cols[P.key.col[0]], row[u]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const bool on = j < P.n_vals;
+    const ColDesc& c = P.cs.cols[on ? P.val_col[j] : 0];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      vr[u][j] = load_raw(c, row[u], on);
+      vv[u][j] = on ? valid_word(c, row[u]) : ~0ull;
     }
+  }
+  // every load above is in flight; from here on the values are used
 #pragma unroll
-  for (int u = 0; u < kRowU; ++u) kv[u] = valid_word(P.cs.cols[P.key.col[0]], row[u]);
+  for (int u = 0; u < U; ++u) {
+    uint32_t vb = ~0u;
 #pragma unroll
-  for (int u = 0; u < kRowU; ++u) vvalid[u] = ~0u;
+    for (int j = 0; j < NV; ++j)
+      if (!((vv[u][j] >> (row[u] & 63)) & 1ull)) vb &= ~(1u << j);
+    vvalid[u] = vb;
+  }
 #pragma unroll
   for (int j = 0; j < NV; ++j)
     if (j < P.n_vals) {
-      const ColDesc& c = P.cs.cols[P.val_col[j]];
+      const int t = P.cs.cols[P.val_col[j]].type;
 #pragma unroll
-      for (int u = 0; u < kRowU; ++u) val[u][j] = raw_word(c, row[u]);
-      if (c.valid) {
-#pragma unroll
-        for (int u = 0; u < kRowU; ++u)
-          if (!((c.valid[row[u] >> 6] >> (row[u] & 63)) & 1ull)) vvalid[u] &= ~(1u << j);
-      }
+      for (int u = 0; u < U; ++u) val[u][j] = value_of(t, vr[u][j]);
     }
 #pragma unroll
-  for (int u = 0; u < kRowU; ++u) {
+  for (int u = 0; u < U; ++u) {
     bool ok = in[u];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
-      if (P.pred[q].col >= 0)
-        ok = ok && pred_raw(P.cs.cols[P.pred[q].col], pr[q][u], pv[q][u], row[u], P.pred[q].op, P.pred[q].i64,
-                            P.pred[q].f64);
+      if (P.pred[q].col >= 0) {
+        const ColDesc& c = P.cs.cols[P.pred[q].col];
+        ok = ok && pred_raw(c, value_of(c.type, pr[q][u]), pv[q][u], row[u], P.pred[q].op, P.pred[q].i64, P.pred[q].f64);
+      }
     uint64_t k;
     if (P.key.n == 1) {
-      k = kw[0][u];
+      k = value_of(P.cs.cols[P.key.col[0]].type, kw[0][u]);
     } else {
       k = 0;
 #pragma unroll
       for (int c = 0; c < 2; ++c)
         if (c < P.key.n) {
-          uint64_t v = kw[c][u];
+          uint64_t v = value_of(P.cs.cols[P.key.col[c]].type, kw[c][u]);
           if (P.key.narrow[c]) v = (v & 0xFFFFFFFFull) << P.key.shift[c];
           k |= v;
         }

@@ -29,7 +29,8 @@
 
 namespace wg {
 
-constexpr int kPartU = kRowU;    // rows per thread per step of the count/scatter loops
+constexpr int kCountU = kRowU;    // rows per thread per step of the count loop
+constexpr int kScatterU = kRowU2;  // and of the scatter loop (more values live per row)
 constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
 constexpr int kPartBlock = 512;  // count/scatter workgroup size
 
@@ -58,13 +59,13 @@ __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   __syncthreads();
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kPartU) {
-    uint64_t row[kPartU], key[kPartU], val[kPartU][1];
-    uint32_t vvalid[kPartU];
-    int kind[kPartU];
-    load_rows<0>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kCountU) {
+    uint64_t row[kCountU], key[kCountU], val[kCountU][1];
+    uint32_t vvalid[kCountU];
+    int kind[kCountU];
+    load_rows<0, kCountU>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
-    for (int u = 0; u < kPartU; ++u)
+    for (int u = 0; u < kCountU; ++u)
       if (kind[u] == 0) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], 1u);
   }
   __syncthreads();
@@ -82,13 +83,13 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
   const int stride = P.pg.stride;
-  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kPartU) {
-    uint64_t row[kPartU], key[kPartU], val[kPartU][NV];
-    uint32_t vvalid[kPartU];
-    int kind[kPartU];
-    load_rows<NV>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kScatterU) {
+    uint64_t row[kScatterU], key[kScatterU], val[kScatterU][NV];
+    uint32_t vvalid[kScatterU];
+    int kind[kScatterU];
+    load_rows<NV, kScatterU>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
-    for (int u = 0; u < kPartU; ++u) {
+    for (int u = 0; u < kScatterU; ++u) {
       if (kind[u] < 0) continue;
       if (kind[u] != 0) {
         // side slots: [0] the sentinel-valued key, [1] the NULL key
