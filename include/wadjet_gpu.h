@@ -277,6 +277,15 @@ wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
                         const uint32_t* sel, uint64_t n_sel, uint32_t* out_probe_rows,
                         uint32_t* out_build_rows, uint64_t out_cap, wg_probe_cursor* cursor,
                         uint64_t* n_out);
+/* Semi / anti join probe of one device-resident probe batch
+ * (executeSemiAntiJoin, join.go:3245-3310; semiProbeInt64 / antiProbeInt64,
+ * join.go:4486-4650): writes to out_sel the probe positions (ascending, from
+ * sel or 0..n_rows) whose key has a build row (anti = 0) or has none
+ * (anti = 1); a NULL probe key never matches, so only the anti join emits it.
+ * out_sel is a device buffer of at least n entries; *n_out on the host. */
+wg_status wg_join_semi(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
+                       const uint32_t* sel, uint64_t n_sel, int32_t anti, uint32_t* out_sel,
+                       uint64_t* n_out);
 /* Gather a retained build column at build row indices (gatherBuildVector,
  * join.go:4024) into a device buffer of the column's storage type. */
 wg_status wg_join_gather_build(wg_join* j, int32_t build_col, const uint32_t* build_rows,

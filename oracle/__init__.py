@@ -15,6 +15,8 @@ Pinning (how each function is checked, see tests/test_oracle.py):
   or_filter                                          numpy boolean masks
   or_hash_join                                       brute-force nested loop, reference KATs
                                                      (join_test.go:10-56, join_coverage_test.go:101-141)
+  or_semi_join                                       brute-force membership loop with NULLs, reference
+                                                     KATs (join_coverage_test.go:272-351, :711-735)
   or_digest_rows                                     order invariance + a Python restatement
 """
 from __future__ import annotations
@@ -85,6 +87,8 @@ def lib():
         L.or_filter.argtypes = [P, U64, P, U64, I32, I64, D, P]
         L.or_filter.restype = U64
         L.or_hash_join.argtypes = [P, U64, P, U64, P, U64, P, P, I64]
+        L.or_semi_join.argtypes = [P, U64, P, U64, P, U64, C.c_int, P]
+        L.or_semi_join.restype = I64
         L.or_hash_join.restype = I64
         L.or_digest_rows.argtypes = [P, U64, I32, P]
         L.or_mix64.argtypes = [U64]
@@ -252,6 +256,21 @@ def hash_join(build_key, probe_key, probe_sel=None, cap=None):
     if n < 0:
         raise RuntimeError("oracle hash_join: output capacity exceeded")
     return op[:n], ob[:n]
+
+
+def semi_join(build_key, probe_key, probe_sel=None, anti=False):
+    """build_key/probe_key: (array, type[, validity]). Probe positions, in
+    input order, whose key is (semi) / is not (anti) among the non-NULL build
+    keys; NULL probe keys never match."""
+    ba, k1 = make_cols([build_key])
+    pa, k2 = make_cols([probe_key])
+    nb = len(build_key[0])
+    npr = len(probe_key[0])
+    sel_a = None if probe_sel is None else np.ascontiguousarray(probe_sel, dtype=np.uint32)
+    out = np.empty(max(npr if sel_a is None else len(sel_a), 1), dtype=np.uint32)
+    n = lib().or_semi_join(ba, nb, pa, npr, _ptr(sel_a), 0 if sel_a is None else len(sel_a), 1 if anti else 0,
+                           _ptr(out))
+    return out[:n]
 
 
 def digest_rows(rows: np.ndarray):

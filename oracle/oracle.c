@@ -636,3 +636,42 @@ done:
   free(used);
   return out;
 }
+
+/* ------------------------------------------------------- semi / anti join */
+
+/* executeSemiAntiJoin without a residual filter (join.go:3245-3310) and its
+ * typed loops semiProbeInt64 / antiProbeInt64 (join.go:4486-4650): the probe
+ * positions, in input order, whose key is (semi) / is not (anti) among the
+ * non-NULL build keys; a NULL probe key never matches (the anti join keeps
+ * it, join.go:3290-3296). Returns the number of positions written. */
+int64_t or_semi_join(const or_col* build_key, uint64_t n_build, const or_col* probe_key, uint64_t n_probe,
+                     const uint32_t* probe_sel, uint64_t n_probe_sel, int anti, uint32_t* out) {
+  uint64_t tcap = 16;
+  while (tcap < n_build * 2 + 2) tcap <<= 1;
+  int64_t* tkey = (int64_t*)malloc(tcap * 8);
+  uint8_t* used = (uint8_t*)calloc(tcap, 1);
+  for (uint64_t r = 0; r < n_build; ++r) {
+    if (!valid_at(build_key, r)) continue;
+    int64_t k = int_at(build_key, r);
+    uint64_t i = or_mix64((uint64_t)k) & (tcap - 1);
+    while (used[i] && tkey[i] != k) i = (i + 1) & (tcap - 1);
+    used[i] = 1;
+    tkey[i] = k;
+  }
+  int64_t nout = 0;
+  uint64_t n = probe_sel ? n_probe_sel : n_probe;
+  for (uint64_t p = 0; p < n; ++p) {
+    uint64_t row = probe_sel ? probe_sel[p] : p;
+    int found = 0;
+    if (valid_at(probe_key, row)) {
+      int64_t k = int_at(probe_key, row);
+      uint64_t i = or_mix64((uint64_t)k) & (tcap - 1);
+      while (used[i] && tkey[i] != k) i = (i + 1) & (tcap - 1);
+      found = used[i];
+    }
+    if (anti ? !found : found) out[nout++] = (uint32_t)row;
+  }
+  free(tkey);
+  free(used);
+  return nout;
+}

@@ -73,6 +73,11 @@ uint64_t or_filter(const or_col* col, uint64_t n_rows, const uint32_t* sel, uint
 int64_t or_hash_join(const or_col* build_key, uint64_t n_build, const or_col* probe_key, uint64_t n_probe,
                      const uint32_t* probe_sel, uint64_t n_probe_sel, uint32_t* out_probe, uint32_t* out_build,
                      int64_t cap);
+/* Semi / anti join probe (join.go:3245-3310, :4486-4650): probe positions in
+ * input order whose key is (anti = 0) / is not (anti = 1) among the non-NULL
+ * build keys; NULL probe keys never match. Returns the count written. */
+int64_t or_semi_join(const or_col* build_key, uint64_t n_build, const or_col* probe_key, uint64_t n_probe,
+                     const uint32_t* probe_sel, uint64_t n_probe_sel, int anti, uint32_t* out);
 
 /* Order-free digest of a multiset of int64 tuples (rows of `width` values):
  * d[0] = sum mix(tuple; seed 1), d[1] = sum mix(tuple; seed 2), mod 2^64. */

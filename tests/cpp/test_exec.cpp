@@ -237,6 +237,52 @@ static void test_hash_join_kats(wg_ctx* g) {
   }
 }
 
+// join_coverage_test.go:272-351 (TestSemiJoin / TestAntiJoin) and :711-735
+// (TestAntiJoinNoMatches), the string ids coded as ints; plus a NULL probe key.
+static void test_semi_anti_kats(wg_ctx* g) {
+  Context ctx;
+  auto run = [&](JoinType type, std::vector<int64_t> bk, std::vector<int64_t> pk, std::vector<bool> pnull,
+                 std::vector<int64_t>* kept) {
+    GpuHashJoin hj(g, "id", "id", 0, type);
+    std::vector<BatchPtr> bb;
+    if (!bk.empty()) {
+      auto build = make_batch({{"id", TypeID::Int64}}, (int)bk.size());
+      build->columns[0]->i64 = bk;
+      bb.push_back(build);
+    }
+    BatchSource bs(bb);
+    OK(hj.Build(ctx, &bs));
+    auto probe = make_batch({{"id", TypeID::Int64}, {"val", TypeID::Float64}}, (int)pk.size());
+    probe->columns[0]->i64 = pk;
+    for (size_t i = 0; i < pnull.size(); ++i)
+      if (pnull[i]) probe->columns[0]->nulls.set_null((int)i, (int)pk.size());
+    auto pr = hj.Probe();
+    BatchSource ps({probe});
+    CollectSink sink;
+    Pipeline p{&ps, {pr.get()}, &sink, 0};
+    OK(p.Run(ctx));
+    kept->clear();
+    for (auto& ob : sink.batches) {
+      EXPECT(ob->columns.size() == 2, "semi/anti output must keep the probe schema only");
+      if (ob->has_sel)
+        for (uint32_t r : ob->sel) kept->push_back(ob->columns[0]->i64[r]);
+      else
+        for (int r = 0; r < ob->len; ++r) kept->push_back(ob->columns[0]->i64[r]);
+    }
+  };
+  std::vector<int64_t> kept;
+  run(JoinType::Semi, {1, 2}, {1, 2, 3}, {}, &kept);
+  EXPECT(kept == std::vector<int64_t>({1, 2}), "semi join kept %zu rows, want 2", kept.size());
+  run(JoinType::Anti, {1, 2}, {1, 2, 3}, {}, &kept);
+  EXPECT(kept == std::vector<int64_t>({3}), "anti join kept %zu rows, want the row with id 3", kept.size());
+  run(JoinType::Anti, {}, {1, 2}, {}, &kept);
+  EXPECT(kept.size() == 2, "anti join with an empty build kept %zu rows, want 2", kept.size());
+  run(JoinType::Semi, {1, 2}, {1, 0, 2}, {false, true, false}, &kept);
+  EXPECT(kept.size() == 2, "semi join with a NULL probe key kept %zu rows, want 2", kept.size());
+  run(JoinType::Anti, {1, 2}, {1, 0, 2}, {false, true, false}, &kept);
+  EXPECT(kept.size() == 1, "anti join keeps the NULL probe key: kept %zu rows, want 1", kept.size());
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   test_kernel_filter_and_pipeline();
@@ -249,6 +295,7 @@ int main(int argc, char** argv) {
     test_hash_aggregate_kat(g);
     test_parallel_pushdown_equivalence(g);
     test_hash_join_kats(g);
+    test_semi_anti_kats(g);
     wg_ctx_destroy(g);
   }
   std::printf("%s: %d failure(s)\n", mode.c_str(), g_fail);

@@ -161,3 +161,32 @@ def test_fused_probe_aggregate_many_to_many_with_overflow(ctx):
     jcols = [(pk[pp], O.INT64), (pv[pp], O.INT32), (bk[bb], O.INT64), (bv[bb], O.INT64)]
     ref = oracle_groups(O.hash_agg(jcols, [3], [(O.SUM, 3), (O.COUNT, -1), (O.MAX, 1)]))
     compare(gpu_groups(agg), ref, [(O.SUM, 3), (O.COUNT, -1), (O.MAX, 1)], types)
+
+
+@pytest.mark.parametrize("anti", [False, True])
+def test_semi_anti_probe(ctx, anti):
+    """Semi / anti probe: ascending probe positions whose key has (semi) / has
+    no (anti) build row; NULL probe keys never match; duplicate build keys,
+    NULL build keys, a selection vector, int32 and int64 keys."""
+    rng = np.random.default_rng(41 if anti else 42)
+    nb, npr = 50_000, 400_000
+    for dt, t in ((np.int64, L.INT64), (np.int32, L.INT32)):
+        bk = rng.integers(-60_000, 60_000, nb).astype(dt)
+        pk = rng.integers(-90_000, 90_000, npr).astype(dt)
+        bvalid = rng.random(nb) > 0.05
+        pvalid = rng.random(npr) > 0.05
+        bw, pw = bitmap_from_mask(bvalid), bitmap_from_mask(pvalid)
+        j = HashJoin(ctx, [t], 0)
+        j.build([Col(ctx.to_device(bk), t, ctx.to_device(bw))], nb)
+        j.finish()
+        key = Col(ctx.to_device(pk), t, ctx.to_device(pw))
+        sel = np.sort(rng.choice(npr, npr // 3, replace=False)).astype(np.uint32)
+        dsel = ctx.to_device(sel)
+        out = ctx.alloc(npr * 4)
+        print(f"[join] semi anti={anti} key={dt.__name__}", flush=True)
+        for s in (None, (dsel, len(sel))):
+            n = j.semi(key, npr, out, anti=anti, sel=s)
+            got = out.to_host(np.uint32, n)
+            ref = O.semi_join((bk, t, bw), (pk, t, pw), probe_sel=None if s is None else sel, anti=anti)
+            np.testing.assert_array_equal(got, ref)
+        j.close()

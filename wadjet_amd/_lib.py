@@ -101,6 +101,7 @@ SIGNATURES: dict[str, list] = {
     "wg_join_build_append": [P, C.POINTER(Column), U64, I32],
     "wg_join_build_finish": [P],
     "wg_join_build_rows": [P, C.POINTER(U64)],
+    "wg_join_semi": [P, C.POINTER(Column), U64, P, U64, I32, P, C.POINTER(U64)],
     "wg_join_probe": [P, C.POINTER(Column), U64, P, U64, P, P, U64, C.POINTER(ProbeCursor), C.POINTER(U64)],
     "wg_join_gather_build": [P, I32, P, U64, P],
     "wg_gather": [P, C.POINTER(Column), P, U64, P],

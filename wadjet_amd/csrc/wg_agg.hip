@@ -58,8 +58,10 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
   __shared__ int64_t s_tile;
   const int stride = P.pg.stride;
   const int L = USE_LDS ? P.lds_slots : 0;
+  const int LW = USE_LDS ? lds_word_stride(L) : 0;  // L hashed slots, then one spare slot per lane
+  uint64_t* const spare = lds + L + (threadIdx.x & 63);
 
-  if (USE_LDS) lds_table_init(lds, L, P.pg);
+  if (USE_LDS) lds_table_init(lds, LW, P.pg);
   __syncthreads();
 
   uint32_t my_lds = 0, my_glb = 0, my_claims = 0;
@@ -87,47 +89,59 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
       uint32_t vvalid[kRowU2];
       int kind[kRowU2];
       load_rows<NV, kRowU2>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+      // LDS first: rows whose key is (or gets) a slot in the workgroup table
+      uint64_t* ls[kRowU2];
+#pragma unroll
+      for (int u = 0; u < kRowU2; ++u) {
+        ls[u] = nullptr;
+        if (USE_LDS && kind[u] == 0) ls[u] = lds_find_or_claim(lds, L - 1, key[u], key_hash(key[u]));
+      }
+      if (USE_LDS && NV > 0) {
+        // every lane runs the same LDS atomics (identity operands where the
+        // row does not aggregate here): no per-row branch on the common path
+#pragma unroll
+        for (int u = 0; u < kRowU2; ++u) {
+          const bool hit = ls[u] != nullptr;
+          uint64_t v[NV > 0 ? NV : 1];
+#pragma unroll
+          for (int j = 0; j < (NV > 0 ? NV : 1); ++j) v[j] = val[u][j];
+          apply_vals_lds(hit ? ls[u] : spare, LW, P.pg, v, P.vfloat, vvalid[u], hit);
+          my_lds += hit ? 1u : 0u;
+        }
+      }
+      // the rest: side slots (NULL key, sentinel-valued key) and the global table
 #pragma unroll
       for (int u = 0; u < kRowU2; ++u) {
         if (kind[u] < 0) continue;
-        uint64_t* slot = nullptr;
-        int ws = 1;
-        bool in_lds = false;
+        if (USE_LDS && ls[u] != nullptr) {
+          if (NV == 0) {
+            apply_row<LdsT, false>(ls[u], LW, P.pg, P.cs, RowRef{row[u], 0, nullptr});
+            ++my_lds;
+          }
+          continue;
+        }
+        uint64_t* slot;
         if (kind[u] == 2) {
           slot = P.T.slots + (P.T.mask + 1 + 1) * (uint64_t)stride;  // NULL-key side slot
         } else if (kind[u] == 1) {
           slot = P.T.slots + (P.T.mask + 1) * (uint64_t)stride;  // sentinel-valued key side slot
         } else {
-          const uint64_t h = key_hash(key[u]);
-          if (USE_LDS) {
-            slot = lds_find_or_claim(lds, L - 1, key[u], h);
-            if (slot) {
-              ws = L;
-              in_lds = true;
-            }
-          }
+          slot = glb_find_or_claim(P.T, stride, key[u], key_hash(key[u]), my_claims);
           if (!slot) {
-            slot = glb_find_or_claim(P.T, stride, key[u], h, my_claims);
-            if (!slot) {
-              uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              P.ovf_rows[at] = (uint32_t)row[u];
-              continue;
-            }
+            uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            P.ovf_rows[at] = (uint32_t)row[u];
+            continue;
           }
         }
         if (NV > 0) {
           uint64_t v[NV > 0 ? NV : 1];
 #pragma unroll
           for (int j = 0; j < (NV > 0 ? NV : 1); ++j) v[j] = val[u][j];
-          if (in_lds) apply_vals<LdsT>(slot, ws, P.pg, v, P.vfloat, vvalid[u]);
-          else apply_vals<GlbT>(slot, 1, P.pg, v, P.vfloat, vvalid[u]);
+          apply_vals<GlbT>(slot, 1, P.pg, v, P.vfloat, vvalid[u]);
         } else {
-          const RowRef r{row[u], 0, nullptr};
-          if (in_lds) apply_row<LdsT, false>(slot, ws, P.pg, P.cs, r);
-          else apply_row<GlbT, false>(slot, 1, P.pg, P.cs, r);
+          apply_row<GlbT, false>(slot, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
         }
-        if (in_lds) ++my_lds;
-        else ++my_glb;
+        ++my_glb;
       }
     }
   }
@@ -142,11 +156,11 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
       ++flushed;
       uint64_t* slot = glb_find_or_claim(P.T, stride, key, key_hash(key), my_claims);
       if (slot != nullptr) {
-        merge_partial(slot, ls, L, P.pg);
+        merge_partial(slot, ls, LW, P.pg);
       } else {
         uint64_t at = __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t* rec = P.ovf_part + at * (uint64_t)stride;
-        for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * L];
+        for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * LW];
       }
     }
     block_add(P.T.counters + CTR_FLUSHED, flushed);
@@ -785,7 +799,7 @@ wg_status ensure_overflow_for(wg_agg* h, uint64_t grid, int32_t lds_slots) {
 
 static void pick_launch(wg_agg* h) {
   // persistent grid: enough resident workgroups to fill every CU
-  const int lds_bytes = h->lds_slots * h->pg.stride * 8;
+  const int lds_bytes = (int)lds_table_bytes(h->lds_slots, h->pg.stride);
   int per_cu = 1;
   if (h->lds_slots == 0) per_cu = std::max(1, 2048 / h->block);
   else per_cu = std::max(1, std::min(160 * 1024 / std::max(lds_bytes, 1), 2048 / h->block));
@@ -871,7 +885,7 @@ static wg_status consume_once(wg_agg* h, const ColSet& cs, const uint32_t* sel, 
     nv = vcols.size() <= 1 ? 1 : 3;
   }
   WG_HIP(hipEventRecord(h->ev0, h->ctx->stream));
-  const size_t lds = h->lds_slots > 0 ? (size_t)h->lds_slots * h->pg.stride * 8 : 0;
+  const size_t lds = h->lds_slots > 0 ? lds_table_bytes(h->lds_slots, h->pg.stride) : 0;
   WG_TRY(launch_consume(h->lds_slots > 0, nv, grid, h->block, lds, h->ctx->stream, P));
   WG_HIP(hipGetLastError());
   WG_HIP(hipEventRecord(h->ev1, h->ctx->stream));
@@ -1456,7 +1470,7 @@ wg_status wg_agg_set_tuning(wg_agg* h, int32_t lds_slots, int32_t block) {
   WG_CHECK(block == 256 || block == 512 || block == 1024, WG_ERR_INVALID, "block %d", block);
   WG_CHECK(lds_slots == 0 || (lds_slots >= 64 && (lds_slots & (lds_slots - 1)) == 0), WG_ERR_INVALID,
            "lds_slots %d must be 0 or a power of two >= 64", lds_slots);
-  WG_CHECK((uint64_t)lds_slots * h->pg.stride * 8 <= 160 * 1024 - 1024, WG_ERR_INVALID,
+  WG_CHECK(lds_slots == 0 || lds_table_bytes(lds_slots, h->pg.stride) <= 160 * 1024 - 1024, WG_ERR_INVALID,
            "LDS table of %d slots x %d words exceeds the workgroup LDS", lds_slots, h->pg.stride);
   h->lds_slots = lds_slots;
   h->block = block;
@@ -1477,7 +1491,7 @@ wg_status wg_agg_set_strategy(wg_agg* h, int32_t strategy, int32_t part_lds_slot
   WG_CHECK(strategy >= 0 && strategy <= 2, WG_ERR_INVALID, "strategy %d", strategy);
   WG_CHECK(part_block == 256 || part_block == 512 || part_block == 1024, WG_ERR_INVALID, "block %d", part_block);
   WG_CHECK(part_lds_slots >= 64 && (part_lds_slots & (part_lds_slots - 1)) == 0 &&
-               (uint64_t)part_lds_slots * h->pg.stride * 8 <= 150 * 1024,
+               lds_table_bytes(part_lds_slots, h->pg.stride) <= 150 * 1024,
            WG_ERR_INVALID, "partition LDS table of %d slots does not fit", part_lds_slots);
   WG_CHECK(part_chunk_rows >= 1024 && part_chunk_rows < (1ull << 31), WG_ERR_INVALID, "chunk rows");
   h->strategy = strategy;

@@ -275,10 +275,23 @@ __host__ __device__ inline uint64_t slot_init_word(const AggProgram& pg, int wor
   return 0;
 }
 
-// Fill an LDS table (struct of arrays, L slots) with the empty-slot words.
-__device__ __forceinline__ void lds_table_init(uint64_t* lds, int L, const AggProgram& pg) {
-  const int lsh = __ffs(L) - 1;
-  for (int i = threadIdx.x; i < L * pg.stride; i += blockDim.x) lds[i] = slot_init_word(pg, i >> lsh);
+// Fill an LDS table (struct of arrays: word k of slot i at lds[k * LW + i],
+// LW >= L slots) with the empty-slot words.
+__device__ __forceinline__ void lds_table_init(uint64_t* lds, int LW, const AggProgram& pg) {
+  for (int w = 0; w < pg.stride; ++w) {
+    const uint64_t v = slot_init_word(pg, w);
+    for (int i = threadIdx.x; i < LW; i += blockDim.x) lds[(uint64_t)w * LW + i] = v;
+  }
+}
+
+// An LDS table of L hashed slots is followed by kTrashSlots spare slots, one
+// per lane: rows that do not aggregate in LDS still run the same (identity)
+// LDS atomics on their lane's spare slot, so the common path has no per-row
+// branch. Word stride of such a table: L + kTrashSlots.
+constexpr int kTrashSlots = 64;
+__host__ __device__ inline int lds_word_stride(int L) { return L + kTrashSlots; }
+__host__ __device__ inline size_t lds_table_bytes(int L, int stride) {
+  return (size_t)lds_word_stride(L) * stride * 8;
 }
 
 // ------------------------------------------------------------- the tables
@@ -350,7 +363,7 @@ __device__ __forceinline__ void block_add(uint64_t* dst, uint64_t v) {
 // Per-workgroup LDS table of `lds_slots` slots, struct of arrays: word k of
 // slot i is lds[k * lds_slots + i]. Random slots then spread over the LDS
 // banks (a power-of-two slot stride would put every key in the same few).
-constexpr int kLdsProbes = 8;
+constexpr int kLdsProbes = 16;  // a key beyond this run of probes aggregates in the global table
 
 // Returns a pointer to the slot's key word (word k at ptr[k * lds_slots]).
 __device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_mask, uint64_t key, uint64_t h,
@@ -442,6 +455,96 @@ __device__ __forceinline__ bool pred_raw(const ColDesc& c, uint64_t raw, uint64_
   }
 }
 
+// value_of for U rows of one column: the type switch once, outside the row loop.
+template <int U>
+__device__ __forceinline__ void values_of(int type, const RawVal (&r)[U], uint64_t (&out)[U]) {
+  switch (type) {
+    case WG_INT32:
+#pragma unroll
+      for (int u = 0; u < U; ++u) out[u] = (uint64_t)(int64_t)(int32_t)r[u].lo;
+      break;
+    case WG_UINT32:
+#pragma unroll
+      for (int u = 0; u < U; ++u) out[u] = r[u].lo;
+      break;
+    case WG_UINT16:
+#pragma unroll
+      for (int u = 0; u < U; ++u) out[u] = r[u].h16;
+      break;
+    default:
+#pragma unroll
+      for (int u = 0; u < U; ++u) out[u] = ((uint64_t)r[u].hi << 32) | r[u].lo;
+      break;
+  }
+}
+
+// pred_raw for U rows (bit u of the result): one dispatch on the column type
+// and the operator, then a branch-free compare per row.
+template <int U>
+__device__ __forceinline__ uint32_t pred_mask(const ColDesc& c, const RawVal (&raw)[U], const uint64_t (&vw)[U],
+                                              const uint64_t (&row)[U], int op, int64_t ci, double cf) {
+  uint64_t v[U];
+  values_of<U>(c.type, raw, v);
+  uint32_t m = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) m |= (uint32_t)((vw[u] >> (row[u] & 63)) & 1ull) << u;
+  uint32_t pass = 0;
+  if (c.type == WG_FLOAT64) {
+    switch (op) {
+#define WG_PRED_CASE(OPC, EXPR)                                                           \
+  case OPC:                                                                               \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) {                                       \
+      const double x = __longlong_as_double((long long)v[u]);                            \
+      pass |= (uint32_t)(EXPR) << u;                                                      \
+    }                                                                                     \
+    break;
+      WG_PRED_CASE(WG_EQ, x == cf)
+      WG_PRED_CASE(WG_NE, x != cf)
+      WG_PRED_CASE(WG_LT, x < cf)
+      WG_PRED_CASE(WG_LE, x <= cf)
+      WG_PRED_CASE(WG_GT, x > cf)
+      default: WG_PRED_CASE(WG_GE, x >= cf)
+#undef WG_PRED_CASE
+    }
+  } else if (c.type == WG_INT32 || c.type == WG_UINT16) {
+    const int32_t k = (int32_t)ci;
+    switch (op) {
+#define WG_PRED_CASE(OPC, EXPR)                                                           \
+  case OPC:                                                                               \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) {                                       \
+      const int32_t x = (int32_t)(int64_t)v[u];                                           \
+      pass |= (uint32_t)(EXPR) << u;                                                      \
+    }                                                                                     \
+    break;
+      WG_PRED_CASE(WG_EQ, x == k)
+      WG_PRED_CASE(WG_NE, x != k)
+      WG_PRED_CASE(WG_LT, x < k)
+      WG_PRED_CASE(WG_LE, x <= k)
+      WG_PRED_CASE(WG_GT, x > k)
+      default: WG_PRED_CASE(WG_GE, x >= k)
+#undef WG_PRED_CASE
+    }
+  } else {
+    switch (op) {
+#define WG_PRED_CASE(OPC, EXPR)                                                           \
+  case OPC:                                                                               \
+    _Pragma("unroll") for (int u = 0; u < U; ++u) {                                       \
+      const int64_t x = (int64_t)v[u];                                                    \
+      pass |= (uint32_t)(EXPR) << u;                                                      \
+    }                                                                                     \
+    break;
+      WG_PRED_CASE(WG_EQ, x == ci)
+      WG_PRED_CASE(WG_NE, x != ci)
+      WG_PRED_CASE(WG_LT, x < ci)
+      WG_PRED_CASE(WG_LE, x <= ci)
+      WG_PRED_CASE(WG_GT, x > ci)
+      default: WG_PRED_CASE(WG_GE, x >= ci)
+#undef WG_PRED_CASE
+    }
+  }
+  return m & pass;
+}
+
 // Load and classify U rows per thread (positions i0 + u * step < end;
 // the row is sel[i] or row_base + i). Every load of every row is issued
 // before any value is used, so a wave keeps U rows of traffic in flight.
@@ -494,48 +597,50 @@ __device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t ste
       vv[u][j] = on ? valid_word(c, row[u]) : ~0ull;
     }
   }
-  // every load above is in flight; from here on the values are used
+  // every load above is in flight; from here on the values are used, one
+  // type / operator dispatch per column for all U rows (not one per row)
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     uint32_t vb = ~0u;
 #pragma unroll
-    for (int j = 0; j < NV; ++j)
-      if (!((vv[u][j] >> (row[u] & 63)) & 1ull)) vb &= ~(1u << j);
+    for (int j = 0; j < NV; ++j) vb &= ~((uint32_t)!((vv[u][j] >> (row[u] & 63)) & 1ull) << j);
     vvalid[u] = vb;
   }
 #pragma unroll
   for (int j = 0; j < NV; ++j)
     if (j < P.n_vals) {
-      const int t = P.cs.cols[P.val_col[j]].type;
+      uint64_t tmp[U];
+      RawVal col[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) val[u][j] = value_of(t, vr[u][j]);
+      for (int u = 0; u < U; ++u) col[u] = vr[u][j];
+      values_of<U>(P.cs.cols[P.val_col[j]].type, col, tmp);
+#pragma unroll
+      for (int u = 0; u < U; ++u) val[u][j] = tmp[u];
     }
+  uint32_t ok = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) ok |= (uint32_t)in[u] << u;
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+    if (P.pred[q].col >= 0)
+      ok &= pred_mask<U>(P.cs.cols[P.pred[q].col], pr[q], pv[q], row, P.pred[q].op, P.pred[q].i64, P.pred[q].f64);
+  values_of<U>(P.cs.cols[P.key.col[0]].type, kw[0], key);
+  if (P.key.n > 1) {
+    uint64_t k1[U];
+    values_of<U>(P.cs.cols[P.key.col[1]].type, kw[1], k1);
+    const int n0 = P.key.narrow[0], n1 = P.key.narrow[1], s0 = P.key.shift[0], s1 = P.key.shift[1];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t a = n0 ? (key[u] & 0xFFFFFFFFull) << s0 : key[u];
+      const uint64_t b = n1 ? (k1[u] & 0xFFFFFFFFull) << s1 : k1[u];
+      key[u] = a | b;
+    }
+  }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    bool ok = in[u];
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-      if (P.pred[q].col >= 0) {
-        const ColDesc& c = P.cs.cols[P.pred[q].col];
-        ok = ok && pred_raw(c, value_of(c.type, pr[q][u]), pv[q][u], row[u], P.pred[q].op, P.pred[q].i64, P.pred[q].f64);
-      }
-    uint64_t k;
-    if (P.key.n == 1) {
-      k = value_of(P.cs.cols[P.key.col[0]].type, kw[0][u]);
-    } else {
-      k = 0;
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-        if (c < P.key.n) {
-          uint64_t v = value_of(P.cs.cols[P.key.col[c]].type, kw[c][u]);
-          if (P.key.narrow[c]) v = (v & 0xFFFFFFFFull) << P.key.shift[c];
-          k |= v;
-        }
-    }
-    key[u] = k;
     // multi-column keys carry no validity (the host rejects them up front)
     const bool null_key = P.key.n == 1 && !((kv[u] >> (row[u] & 63)) & 1ull);
-    kind[u] = !ok ? -1 : null_key ? 2 : (k == kEmpty ? 1 : 0);
+    kind[u] = !((ok >> u) & 1u) ? -1 : null_key ? 2 : (key[u] == kEmpty ? 1 : 0);
   }
 }
 
@@ -613,6 +718,75 @@ __device__ __forceinline__ void apply_vals(uint64_t* w, int ws, const AggProgram
           else at_max<S>(t, e);
         }
         break;
+      default:
+        break;
+    }
+  }
+}
+
+// apply_vals on an LDS slot with no per-row branch: where the row does not
+// apply (dead row, NULL input, NaN for MIN/MAX) the operand is the
+// operation's identity (0, -0.0, the MIN/MAX initial word), so all lanes run
+// the same atomics. Dead rows pass their lane's spare slot.
+template <int NV>
+__device__ __forceinline__ void apply_vals_lds(uint64_t* w, int ws, const AggProgram& pg, const uint64_t (&v)[NV],
+                                               uint32_t vfloat, uint32_t vvalid, bool live) {
+  for (int o = 0; o < pg.n_ops; ++o) {
+    const OpDesc op = pg.ops[o];
+    uint64_t* t = w + op.word * ws;
+    const bool ok1 = live && op.col >= 0 && ((vvalid >> op.col) & 1u);
+    const bool ok2 = ok1 && (op.col2 < 0 || ((vvalid >> op.col2) & 1u));
+    switch (op.kind) {
+      case OP_ROWCOUNT:
+        at_add<LdsT>(t, live ? 1 : 0);
+        break;
+      case OP_NNCOUNT:
+        at_add<LdsT>(t, ok2 ? 1 : 0);
+        break;
+      case OP_SUMI: {
+        uint64_t x = pick(v, op.col);
+        if (op.col2 >= 0) x *= pick(v, op.col2);
+        at_add<LdsT>(t, ok2 ? x : 0);
+        break;
+      }
+      case OP_SUMF: {
+        const uint64_t b = pick(v, op.col);
+        const double x = ((vfloat >> op.col) & 1u) ? __longlong_as_double((long long)b) : (double)(int64_t)b;
+        at_addf<LdsT>(t, ok1 ? x : -0.0);
+        break;
+      }
+      case OP_SUMD:
+        add_limbs2<LdsT>(t, ws, ok1 ? (int64_t)pick(v, op.col) : 0);
+        break;
+      case OP_SUMD4: {
+        const int64_t a = (int64_t)pick(v, op.col);
+        uint64_t lo;
+        int64_t hi;
+        if (op.col2 >= 0) {
+          const int64_t b = (int64_t)pick(v, op.col2);
+          lo = (uint64_t)a * (uint64_t)b;
+          hi = (int64_t)__mul64hi((long long)a, (long long)b);
+        } else {
+          lo = (uint64_t)a;
+          hi = a < 0 ? -1 : 0;
+        }
+        add_limbs4<LdsT>(t, ws, ok2 ? lo : 0, ok2 ? hi : 0);
+        break;
+      }
+      case OP_MIN: case OP_MAX: {
+        const uint64_t b = pick(v, op.col);
+        uint64_t e;
+        bool ok = ok1;
+        if ((vfloat >> op.col) & 1u) {
+          ok = ok && (b & 0x7FFFFFFFFFFFFFFFull) <= 0x7FF0000000000000ull;  // NaN never wins
+          e = enc_f64_bits(b);
+        } else {
+          e = enc_i64((int64_t)b);
+        }
+        if (op.kind == OP_MIN) at_min<LdsT>(t, ok ? e : kMinInit);
+        else at_max<LdsT>(t, ok ? e : kMaxInit);
+        break;
+      }
       default:
         break;
     }

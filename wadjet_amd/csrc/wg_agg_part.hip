@@ -64,15 +64,16 @@ __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
     uint32_t vvalid[kCountU];
     int kind[kCountU];
     load_rows<0, kCountU>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+    // no per-row branch: rows that do not count add 0 to their bucket
 #pragma unroll
-    for (int u = 0; u < kCountU; ++u)
-      if (kind[u] == 0) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], 1u);
+    for (int u = 0; u < kCountU; ++u) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], kind[u] == 0 ? 1u : 0u);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
 }
 
-// RW = record words (key + values, padded to a power of two).
+// RW = record words: key + values (2, 3, 4 or 8 words; 3 and 4 differ by a
+// zero pad word, 8 pads up to seven values).
 template <int RW>
 __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) {
   constexpr int NV = RW - 1;
@@ -102,9 +103,14 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
       w[0] = key[u];
 #pragma unroll
       for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
-      ulonglong2* dst = (ulonglong2*)(P.recs + (uint64_t)pos * RW);
+      uint64_t* rec = P.recs + (uint64_t)pos * RW;
+      if (RW % 2 == 0) {
 #pragma unroll
-      for (int j = 0; j < RW / 2; ++j) dst[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+        for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)rec)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < RW; ++j) rec[j] = w[j];  // 24-byte records are only 8-byte aligned
+      }
     }
   }
 }
@@ -170,7 +176,8 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
   __shared__ uint64_t s_init[16];
   const int stride = P.pg.stride;
   const int L = P.lds_slots;
-  const int lsh = __ffs(L) - 1;  // the LDS table is struct of arrays: word k of slot i at lds[k * L + i]
+  const int LW = lds_word_stride(L);  // L hashed slots, then one spare slot per lane
+  uint64_t* const spare = lds + L + (threadIdx.x & 63);
   if (threadIdx.x < (unsigned)stride) s_init[threadIdx.x] = slot_init_word(P.pg, threadIdx.x);
   uint32_t claims = 0;
   const uint64_t n_work = P.sel ? (P.n_sel + P.sel_chunk - 1) / P.sel_chunk : P.n_items;
@@ -204,7 +211,10 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
       end = it.end;
       split = it.split != 0;
     }
-    for (int i = threadIdx.x; i < L * stride; i += blockDim.x) lds[i] = s_init[i >> lsh];
+    for (int w = 0; w < stride; ++w) {
+      const uint64_t v = s_init[w];
+      for (int i = threadIdx.x; i < LW; i += blockDim.x) lds[(uint64_t)w * LW + i] = v;
+    }
     __syncthreads();
     for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kAggU) {
       uint64_t rec[kAggU][RW];
@@ -219,28 +229,40 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
       }
 #pragma unroll
       for (int u = 0; u < kAggU; ++u) {
-        const ulonglong2* src = (const ulonglong2*)(P.recs + pos[u] * RW);
+        const uint64_t* src = P.recs + pos[u] * RW;
+        if (RW % 2 == 0) {
 #pragma unroll
-        for (int j = 0; j < RW / 2; ++j) {
-          const ulonglong2 x = src[j];
-          rec[u][2 * j] = x.x;
-          rec[u][2 * j + 1] = x.y;
+          for (int j = 0; j < RW / 2; ++j) {
+            const ulonglong2 x = ((const ulonglong2*)src)[j];
+            rec[u][2 * j] = x.x;
+            rec[u][2 * j + 1] = x.y;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < RW; ++j) rec[u][j] = src[j];
         }
+      }
+      uint64_t* ls[kAggU];
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u)
+        ls[u] = in[u] ? lds_find_or_claim(lds, L - 1, rec[u][0], key_hash(rec[u][0]), 16) : nullptr;
+      // same LDS atomics on every lane (identity operands on the spare slot
+      // where the row does not aggregate here)
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u) {
+        uint64_t v[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] = rec[u][j + 1];
+        apply_vals_lds(ls[u] ? ls[u] : spare, LW, P.pg, v, P.vfloat, ~0u, ls[u] != nullptr);
       }
 #pragma unroll
       for (int u = 0; u < kAggU; ++u) {
-        if (!in[u]) continue;
+        if (!in[u] || ls[u]) continue;
         const uint64_t key = rec[u][0];
         uint64_t v[NV];
 #pragma unroll
         for (int j = 0; j < NV; ++j) v[j] = rec[u][j + 1];
-        const uint64_t h = key_hash(key);
-        uint64_t* ls = lds_find_or_claim(lds, L - 1, key, h, 16);
-        if (ls) {
-          apply_vals<LdsT, NV>(ls, L, P.pg, v, P.vfloat, ~0u);
-          continue;
-        }
-        uint64_t* g = glb_find_or_claim(P.T, stride, key, h, claims);
+        uint64_t* g = glb_find_or_claim(P.T, stride, key, key_hash(key), claims);
         if (!g) {
           const uint64_t at =
               __hip_atomic_fetch_add(P.T.counters + CTR_OVF_ROWS, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -276,20 +298,20 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
           const uint64_t at =
               __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           uint64_t* rec = P.ovf_part + at * (uint64_t)stride;  // in bounds: headroom as above
-          for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * L];
+          for (int w = 0; w < stride; ++w) rec[w] = ls[(uint64_t)w * LW];
           continue;
         }
         if (fresh) ++claims;
         if (split) {
-          merge_partial(g, ls, L, P.pg);
+          merge_partial(g, ls, LW, P.pg);
         } else if (fresh) {
           // a slot this workgroup just claimed for a key no one else holds:
           // its words are still the initial values, store the partial
-          g[1] = ls[L];
+          g[1] = ls[LW];
           for (int w = 2; w < stride; w += 2)
-            *(ulonglong2*)(g + w) = make_ulonglong2(ls[(uint64_t)w * L], ls[(uint64_t)(w + 1) * L]);
+            *(ulonglong2*)(g + w) = make_ulonglong2(ls[(uint64_t)w * LW], ls[(uint64_t)(w + 1) * LW]);
         } else {
-          merge_partial_owned(g, ls, L, P.pg);
+          merge_partial_owned(g, ls, LW, P.pg);
         }
       }
     }
@@ -323,7 +345,8 @@ static wg_status allow_lds(const void* kernel, size_t bytes) {
 
 using ScatterFn = void (*)(PartParams);
 static ScatterFn scatter_kernel_for(int rw) {
-  return rw == 2 ? part_scatter_kernel<2> : rw == 4 ? part_scatter_kernel<4> : part_scatter_kernel<8>;
+  return rw == 2 ? part_scatter_kernel<2> : rw == 3 ? part_scatter_kernel<3> : rw == 4 ? part_scatter_kernel<4>
+                                                                                       : part_scatter_kernel<8>;
 }
 
 static wg_status launch_scatter(int rw, dim3 grid, size_t lds, hipStream_t st, const PartParams& P) {
@@ -337,6 +360,10 @@ static wg_status launch_part_agg(int rw, int grid, int block, size_t lds, hipStr
     case 2:
       WG_TRY(allow_lds((const void*)part_agg_kernel<2>, lds));
       hipLaunchKernelGGL(part_agg_kernel<2>, dim3(grid), dim3(block), lds, st, Q);
+      break;
+    case 3:
+      WG_TRY(allow_lds((const void*)part_agg_kernel<3>, lds));
+      hipLaunchKernelGGL(part_agg_kernel<3>, dim3(grid), dim3(block), lds, st, Q);
       break;
     case 4:
       WG_TRY(allow_lds((const void*)part_agg_kernel<4>, lds));
@@ -372,12 +399,14 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   WG_CHECK((int)vcols.size() <= kMaxVals, WG_ERR_UNSUPPORTED, "partitioned consume carries at most %d value columns",
            kMaxVals);
   WG_CHECK(h->key.n <= 2, WG_ERR_UNSUPPORTED, "partitioned consume takes keys of at most two columns");
-  const int RW = vcols.size() < 2 ? 2 : vcols.size() < 4 ? 4 : 8;
+  const int RW = vcols.size() < 2 ? 2 : vcols.size() < 3 ? 3 : vcols.size() < 4 ? 4 : 8;
   const int L = h->part_lds_slots;
-  // partitions of about L/2 groups each (the LDS table at half load)
+  // partitions of at most about 3/4 L groups each: fewer, larger partitions
+  // make the scatter's writes denser, while linear probing in the LDS table
+  // still mostly ends within its probe limit
   const uint64_t est = std::max<uint64_t>(std::max<uint64_t>(h->group_hint, h->groups), 1);
   int pbits = 6;
-  while (pbits < 14 && (1ull << pbits) * (uint64_t)(L / 2) < est) ++pbits;
+  while (pbits < 14 && (1ull << pbits) * (uint64_t)(3 * L / 4) < est) ++pbits;
   const uint32_t np = 1u << pbits;
   const size_t hist_bytes = (size_t)np * 4;
   // one resident round of count/scatter workgroups (the two passes must use
@@ -498,7 +527,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   Q.item_ctr = item_ctr;
   for (size_t j = 0; j < vcols.size(); ++j)
     if (cs.cols[vcols[j]].type == WG_FLOAT64) Q.vfloat |= 1u << j;
-  const size_t lds_bytes = (size_t)L * h->pg.stride * 8;
+  const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
   const int per_cu_agg = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
   const int grid_max = ctx->cus * per_cu_agg;
   // overflow lists with headroom for every in-flight item (see the kernel)

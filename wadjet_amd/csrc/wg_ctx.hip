@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "wg_core.hpp"
+#include "wg_select.hpp"
 
 namespace wg {
 
@@ -26,55 +27,19 @@ void clear_error() { g_err[0] = 0; }
 // their output offset from a decoupled pass: pass 1 counts per tile, the host
 // scans the (few) tile counts, pass 2 writes. Output stays ascending like the
 // reference's Sel.
-constexpr int kFilterBlock = 256;
-constexpr int kFilterRowsPerThread = 16;
-constexpr int kFilterTile = kFilterBlock * kFilterRowsPerThread;
-
-__device__ __forceinline__ bool filter_pass(const ColDesc& c, uint64_t row, int op, int64_t ci, double cf) {
-  return pred_pass(c, row, op, ci, cf);
-}
+// Predicate of the filter kernel: the typed comparison of KernelFilter.
+struct FilterPred {
+  ColDesc col;
+  int op;
+  int64_t ci;
+  double cf;
+  __device__ bool operator()(uint64_t row) const { return pred_pass(col, row, op, ci, cf); }
+};
 
 // Grid-stride streaming copy of 16-byte words.
 __global__ void __launch_bounds__(256) copy_words_kernel(ulonglong2* dst, const ulonglong2* src, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
-}
-
-template <bool WRITE>
-__global__ void __launch_bounds__(kFilterBlock) filter_kernel(ColDesc col, uint64_t n, const uint32_t* sel, int op,
-                                                              int64_t ci, double cf, uint32_t* tile_counts,
-                                                              const uint32_t* tile_offsets, uint32_t* out) {
-  __shared__ uint32_t wave_tot[kFilterBlock / 64];
-  __shared__ uint32_t running;
-  const uint64_t tile_beg = (uint64_t)blockIdx.x * kFilterTile;
-  const int lane = __lane_id();
-  const int wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) running = WRITE ? tile_offsets[blockIdx.x] : 0;
-  __syncthreads();
-  // process the tile in kFilterRowsPerThread rounds of blockDim rows, in order
-  for (int r = 0; r < kFilterRowsPerThread; ++r) {
-    const uint64_t i = tile_beg + (uint64_t)r * kFilterBlock + threadIdx.x;
-    bool pass = false;
-    uint64_t row = 0;
-    if (i < n) {
-      row = sel ? (uint64_t)sel[i] : i;
-      pass = filter_pass(col, row, op, ci, cf);
-    }
-    const uint64_t b = __ballot(pass);
-    const uint32_t rank = (uint32_t)__popcll(b & ((1ull << lane) - 1));
-    if (lane == 0) wave_tot[wave] = (uint32_t)__popcll(b);
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (int w = 0; w < kFilterBlock / 64; ++w) {
-      if (w < wave) before += wave_tot[w];
-      total += wave_tot[w];
-    }
-    if (WRITE && pass) out[running + before + rank] = (uint32_t)row;
-    __syncthreads();
-    if (threadIdx.x == 0) running += total;
-    __syncthreads();
-  }
-  if (!WRITE && threadIdx.x == 0) tile_counts[blockIdx.x] = running;
 }
 
 }  // namespace wg
@@ -213,34 +178,8 @@ wg_status wg_filter(wg_ctx* ctx, const wg_column* col, uint64_t n_rows, const ui
                col->type == WG_UINT32 || col->type == WG_IPV4,
            WG_ERR_UNSUPPORTED, "no filter kernel for type %d", col->type);
   const uint64_t n = sel ? n_sel : n_rows;
-  *n_out = 0;
-  if (n == 0) return WG_OK;
-  WG_CHECK(n < (1ull << 32), WG_ERR_INVALID, "selection vectors index < 2^32 rows");
-  ColDesc c{col->data, col->validity, col->type, col->scale};
-  const uint64_t tiles = (n + kFilterTile - 1) / kFilterTile;
-  uint32_t* counts = nullptr;
-  WG_HIP(hipMallocAsync((void**)&counts, tiles * 8, ctx->stream));
-  uint32_t* offsets = counts + tiles;
-  hipLaunchKernelGGL(filter_kernel<false>, dim3(tiles), dim3(kFilterBlock), 0, ctx->stream, c, n, sel, op, i64, f64,
-                     counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
-  WG_HIP(hipGetLastError());
-  std::vector<uint32_t> h(tiles);
-  WG_HIP(hipMemcpyAsync(h.data(), counts, tiles * 4, hipMemcpyDeviceToHost, ctx->stream));
-  WG_HIP(hipStreamSynchronize(ctx->stream));
-  uint64_t acc = 0;
-  for (uint64_t t = 0; t < tiles; ++t) {
-    const uint32_t v = h[t];
-    h[t] = (uint32_t)acc;
-    acc += v;
-  }
-  WG_HIP(hipMemcpyAsync(offsets, h.data(), tiles * 4, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(filter_kernel<true>, dim3(tiles), dim3(kFilterBlock), 0, ctx->stream, c, n, sel, op, i64, f64,
-                     (uint32_t*)nullptr, (const uint32_t*)offsets, out_sel);
-  WG_HIP(hipGetLastError());
-  WG_HIP(hipFreeAsync(counts, ctx->stream));
-  WG_HIP(hipStreamSynchronize(ctx->stream));
-  *n_out = acc;
-  return WG_OK;
+  return select_rows(ctx, FilterPred{ColDesc{col->data, col->validity, col->type, col->scale}, op, i64, f64}, n, sel,
+                     out_sel, n_out);
 }
 
 }  // extern "C"

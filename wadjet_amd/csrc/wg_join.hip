@@ -25,6 +25,7 @@
 #include "wg_agg_host.hpp"
 #include "wg_join_dev.hpp"
 #include "wg_join_host.hpp"
+#include "wg_select.hpp"
 
 namespace wg {
 
@@ -528,6 +529,20 @@ static wg_status append_validity(wg_join* j, int c, const uint64_t* src, uint64_
   return WG_OK;
 }
 
+// Semi / anti probe predicate (executeSemiAntiJoin, join.go:3245-3310 and
+// semiProbeInt64 / antiProbeInt64 :4486-4650): the key has a build row, or
+// (anti) has none. A NULL probe key never matches, so the anti join emits it.
+struct SemiPred {
+  JoinTable J;
+  ColDesc key;
+  int anti;
+  __device__ bool operator()(uint64_t row) const {
+    if (!col_valid(key, row)) return anti != 0;
+    const bool found = join_find(J, (uint64_t)load_int(key, row)) != nullptr;
+    return anti ? !found : found;
+  }
+};
+
 static JoinTable jt_of(const wg_join* j) { return JoinTable{j->slots, j->cap - 1, j->stride, j->next}; }
 
 static ColDesc retained_col(const wg_join* j, int c) {
@@ -666,6 +681,18 @@ wg_status wg_join_build_rows(wg_join* j, uint64_t* rows) {
   WG_CHECK(j && rows, WG_ERR_INVALID, "null argument");
   *rows = j->rows;
   return WG_OK;
+}
+
+wg_status wg_join_semi(wg_join* j, const wg_column* probe_key, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
+                       int32_t anti, uint32_t* out_sel, uint64_t* n_out) {
+  clear_error();
+  WG_CHECK(j && probe_key && out_sel && n_out, WG_ERR_INVALID, "null argument");
+  WG_CHECK(j->finished, WG_ERR_STATE, "probe before build_finish");
+  WG_CHECK(is_int_class(probe_key->type), WG_ERR_UNSUPPORTED, "probe key type %d", probe_key->type);
+  const uint64_t n = sel ? n_sel : n_rows;
+  const SemiPred pred{jt_of(j), ColDesc{probe_key->data, probe_key->validity, probe_key->type, probe_key->scale},
+                      anti ? 1 : 0};
+  return select_rows(j->ctx, pred, n, sel, out_sel, n_out);
 }
 
 wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,

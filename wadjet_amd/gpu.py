@@ -353,6 +353,14 @@ class HashJoin:
         sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
         call("wg_join_probe_agg", self.h, columns(probe_cols), len(probe_cols), probe_key, n_rows, sp, ns, agg.h)
 
+    def semi(self, key: Col, n_rows: int, out: DeviceBuffer, anti: bool = False, sel=None) -> int:
+        """Semi (anti=False) / anti join: positions written to `out`, returns the count."""
+        n = C.c_uint64()
+        kc = key.c()
+        sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
+        call("wg_join_semi", self.h, C.byref(kc), n_rows, sp, ns, 1 if anti else 0, out.ptr, C.byref(n))
+        return n.value
+
     def gather_build(self, col: int, rows: DeviceBuffer, n: int, out: DeviceBuffer):
         call("wg_join_gather_build", self.h, col, rows.ptr, n, out.ptr)
 

@@ -448,8 +448,9 @@ void PushDownFiltersIntoGpuAggregate(Pipeline& p) {
 
 // ------------------------------------------------------------ GpuHashJoin
 
-GpuHashJoin::GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint)
-    : ctx_(ctx), probe_key_(std::move(probe_key)), build_key_(std::move(build_key)), hint_(build_hint) {}
+GpuHashJoin::GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint,
+                         JoinType type)
+    : ctx_(ctx), probe_key_(std::move(probe_key)), build_key_(std::move(build_key)), hint_(build_hint), type_(type) {}
 
 GpuHashJoin::~GpuHashJoin() {
   if (j_) wg_join_destroy(j_);
@@ -530,6 +531,7 @@ Status GpuHashJoinProbe::Close() {
 Status GpuHashJoinProbe::Execute(Context& ctx, const BatchPtr& in, BatchPtr* out) {
   *out = nullptr;
   pending_ = false;
+  if (join_->type_ != JoinType::Inner) return semi_anti(in, out);
   if (!join_->j_) return Status::OK();  // empty build
   key_idx_ = in->column_index(join_->probe_key_);
   if (key_idx_ < 0) return Status::Error("probe key \"" + join_->probe_key_ + "\" not in the probe schema");
@@ -601,6 +603,50 @@ Status GpuHashJoinProbe::NextOutput(Context&, BatchPtr* out) {
 
 // One bounded chunk: probe on the device, then gather the probe columns and the
 // retained build columns on the host (gatherVector / gatherBuildVector).
+// Semi / anti: the input batch with a new Sel of the probe rows that have
+// (semi) / have no (anti) build row; nullptr when no row remains
+// (join.go:3468-3478). An empty build keeps no row (semi) / every row (anti).
+Status GpuHashJoinProbe::semi_anti(const BatchPtr& in, BatchPtr* out) {
+  const bool anti = join_->type_ == JoinType::Anti;
+  const uint64_t n = in->has_sel ? in->sel.size() : (uint64_t)in->len;
+  if (n == 0) return Status::OK();
+  if (!join_->j_) {
+    if (anti) *out = in;
+    return Status::OK();
+  }
+  key_idx_ = in->column_index(join_->probe_key_);
+  if (key_idx_ < 0) return Status::Error("probe key \"" + join_->probe_key_ + "\" not in the probe schema");
+  const Vector& k = *in->columns[key_idx_];
+  wg_ctx* c = join_->ctx_;
+  if ((uint64_t)in->len > cap_rows_) {
+    (void)Close();
+    cap_rows_ = std::max<uint64_t>((uint64_t)in->len, batch::kDefaultBatchSize);
+    TRY(wrap(wg_malloc(c, cap_rows_ * 8, &d_key_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, (cap_rows_ + 63) / 64 * 8, (void**)&d_valid_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_sel_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, std::max<uint64_t>(cap_rows_, kMaxProbeOutputRows) * 4, (void**)&d_out_p_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, kMaxProbeOutputRows * 4, (void**)&d_out_b_), "wg_malloc"));
+  }
+  TRY(wrap(wg_memcpy_h2d(c, d_key_, data_of(k), (uint64_t)in->len * elem_bytes(k.type)), "upload probe key"));
+  if (k.nulls.has_nulls())
+    TRY(wrap(wg_memcpy_h2d(c, d_valid_, k.nulls.words.data(), k.nulls.words.size() * 8), "upload validity"));
+  if (in->has_sel) TRY(wrap(wg_memcpy_h2d(c, d_sel_, in->sel.data(), in->sel.size() * 4), "upload sel"));
+  wg_column key = col_of(k);
+  key.data = d_key_;
+  key.validity = k.nulls.has_nulls() ? d_valid_ : nullptr;
+  uint64_t got = 0;
+  TRY(wrap(wg_join_semi(join_->j_, &key, (uint64_t)in->len, in->has_sel ? d_sel_ : nullptr,
+                        in->has_sel ? in->sel.size() : 0, anti ? 1 : 0, d_out_p_, &got),
+           "wg_join_semi"));
+  if (got == 0) return Status::OK();
+  auto o = std::make_shared<RecordBatch>(*in);  // columns shared, Sel replaced
+  o->sel.resize(got);
+  TRY(wrap(wg_memcpy_d2h(c, o->sel.data(), d_out_p_, got * 4), "download sel"));
+  o->has_sel = true;
+  *out = o;
+  return Status::OK();
+}
+
 Status GpuHashJoinProbe::emit(BatchPtr* out) {
   *out = nullptr;
   const RecordBatch& in = *cur_;

@@ -249,9 +249,13 @@ class GpuHashAggregate : public MergeableSink {
 
 class GpuHashJoinProbe;
 
+// JoinType subset on the GPU path (join.go JoinType): inner, semi, anti.
+enum class JoinType { Inner, Semi, Anti };
+
 class GpuHashJoin {
  public:
-  GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint = 0);
+  GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint = 0,
+              JoinType type = JoinType::Inner);
   ~GpuHashJoin();
   Status Build(Context& ctx, Source* build);  // join.go:771
   std::unique_ptr<GpuHashJoinProbe> Probe();   // join.go:1967
@@ -263,6 +267,7 @@ class GpuHashJoin {
   wg_ctx* ctx_;
   std::string probe_key_, build_key_;
   uint64_t hint_;
+  JoinType type_;
   wg_join* j_ = nullptr;
   std::vector<batch::Column> build_schema_;
   std::vector<std::shared_ptr<batch::Vector>> build_cols_;  // retained build side (host copy for gathers)
@@ -286,6 +291,7 @@ class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, pub
 
  private:
   Status emit(BatchPtr* out);  // one bounded chunk of the current input
+  Status semi_anti(const BatchPtr& in, BatchPtr* out);  // executeSemiAntiJoin, join.go:3245-3479
   GpuHashJoin* join_;
   bool bounded_ = false, pending_ = false;
   BatchPtr cur_;
