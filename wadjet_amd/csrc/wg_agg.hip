@@ -60,6 +60,7 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
   const int L = USE_LDS ? P.lds_slots : 0;
   const int LW = USE_LDS ? lds_word_stride(L) : 0;  // L hashed slots, then one spare slot per lane
   uint64_t* const spare = lds + L + (threadIdx.x & 63);
+  const RowSpec<NV> S = make_spec<NV>(P);
 
   if (USE_LDS) lds_table_init(lds, LW, P.pg);
   __syncthreads();
@@ -88,7 +89,7 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
       uint64_t row[kRowU2], key[kRowU2], val[kRowU2][NV > 0 ? NV : 1];
       uint32_t vvalid[kRowU2];
       int kind[kRowU2];
-      load_rows<NV, kRowU2>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+      load_rows<NV, kRowU2>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
       // LDS first: rows whose key is (or gets) a slot in the workgroup table
       uint64_t* ls[kRowU2];
 #pragma unroll
@@ -99,15 +100,16 @@ __global__ void __launch_bounds__(1024) agg_consume_kernel(ConsumeParams P) {
       if (USE_LDS && NV > 0) {
         // every lane runs the same LDS atomics (identity operands where the
         // row does not aggregate here): no per-row branch on the common path
+        uint64_t* w[kRowU2];
+        uint32_t live = 0;
 #pragma unroll
         for (int u = 0; u < kRowU2; ++u) {
           const bool hit = ls[u] != nullptr;
-          uint64_t v[NV > 0 ? NV : 1];
-#pragma unroll
-          for (int j = 0; j < (NV > 0 ? NV : 1); ++j) v[j] = val[u][j];
-          apply_vals_lds(hit ? ls[u] : spare, LW, P.pg, v, P.vfloat, vvalid[u], hit);
+          w[u] = hit ? ls[u] : spare;
+          live |= (uint32_t)hit << u;
           my_lds += hit ? 1u : 0u;
         }
+        apply_vals_lds_batch<(NV > 0 ? NV : 1), kRowU2>(w, LW, P.pg, val, P.vfloat, vvalid, live);
       }
       // the rest: side slots (NULL key, sentinel-valued key) and the global table
 #pragma unroll

@@ -430,6 +430,41 @@ __device__ __forceinline__ RawVal load_raw(const ColDesc& c, uint64_t row, bool 
   return RawVal{*plo, *phi, *p16};
 }
 
+// A column resolved once per kernel for the streaming loops: the three loads
+// of load_raw as base + row * stride each, with the type decisions taken here
+// rather than per row (a load a type does not need reads kSafeWord with
+// stride 0). Uniform values, so they live in scalar registers.
+struct ColRes {
+  const char* lo;
+  const char* hi;
+  const char* h16;
+  uint32_t s_lo, s_hi, s16;
+  const uint64_t* valid;
+  int32_t type;
+};
+
+__device__ __forceinline__ ColRes resolve_col(const ColDesc& c, bool active) {
+  const char* safe = (const char*)&kSafeWord;
+  const char* d = (const char*)c.data;
+  ColRes r{safe, safe, safe, 0, 0, 0, active ? c.valid : nullptr, c.type};
+  if (!active) return r;
+  switch (c.type) {
+    case WG_UINT16: r.h16 = d; r.s16 = 2; break;
+    case WG_INT32: case WG_UINT32: r.lo = r.hi = d; r.s_lo = r.s_hi = 4; break;
+    case WG_DECIMAL128: r.lo = d + 8; r.hi = d + 12; r.s_lo = r.s_hi = 16; break;  // the Lo word
+    default: r.lo = d; r.hi = d + 4; r.s_lo = r.s_hi = 8; break;
+  }
+  return r;
+}
+
+__device__ __forceinline__ RawVal load_res(const ColRes& c, uint64_t row) {
+  return RawVal{*(gptr_u32)(c.lo + row * c.s_lo), *(gptr_u32)(c.hi + row * c.s_hi),
+                *(gptr_u16)(c.h16 + row * c.s16)};
+}
+__device__ __forceinline__ uint64_t valid_res(const ColRes& c, uint64_t row) {
+  return c.valid ? c.valid[row >> 6] : ~0ull;
+}
+
 // raw_word's form: float64 bits, integers widened to int64.
 __device__ __forceinline__ uint64_t value_of(int type, const RawVal& r) {
   switch (type) {
@@ -481,15 +516,15 @@ __device__ __forceinline__ void values_of(int type, const RawVal (&r)[U], uint64
 // pred_raw for U rows (bit u of the result): one dispatch on the column type
 // and the operator, then a branch-free compare per row.
 template <int U>
-__device__ __forceinline__ uint32_t pred_mask(const ColDesc& c, const RawVal (&raw)[U], const uint64_t (&vw)[U],
+__device__ __forceinline__ uint32_t pred_mask(int type, const RawVal (&raw)[U], const uint64_t (&vw)[U],
                                               const uint64_t (&row)[U], int op, int64_t ci, double cf) {
   uint64_t v[U];
-  values_of<U>(c.type, raw, v);
+  values_of<U>(type, raw, v);
   uint32_t m = 0;
 #pragma unroll
   for (int u = 0; u < U; ++u) m |= (uint32_t)((vw[u] >> (row[u] & 63)) & 1ull) << u;
   uint32_t pass = 0;
-  if (c.type == WG_FLOAT64) {
+  if (type == WG_FLOAT64) {
     switch (op) {
 #define WG_PRED_CASE(OPC, EXPR)                                                           \
   case OPC:                                                                               \
@@ -506,7 +541,7 @@ __device__ __forceinline__ uint32_t pred_mask(const ColDesc& c, const RawVal (&r
       default: WG_PRED_CASE(WG_GE, x >= cf)
 #undef WG_PRED_CASE
     }
-  } else if (c.type == WG_INT32 || c.type == WG_UINT16) {
+  } else if (type == WG_INT32 || type == WG_UINT16) {
     const int32_t k = (int32_t)ci;
     switch (op) {
 #define WG_PRED_CASE(OPC, EXPR)                                                           \
@@ -545,16 +580,63 @@ __device__ __forceinline__ uint32_t pred_mask(const ColDesc& c, const RawVal (&r
   return m & pass;
 }
 
-// Load and classify U rows per thread (positions i0 + u * step < end;
-// the row is sel[i] or row_base + i). Every load of every row is issued
-// before any value is used, so a wave keeps U rows of traffic in flight.
-// kind: -1 filtered out or past the end, 0 keyed row, 1 sentinel-valued key,
-// 2 NULL single-column key. NV value columns (P.val_col[j < P.n_vals]) are
-// loaded as raw words with their validity bits in vvalid (bit j).
+// Everything the streaming loops read per row, resolved once per kernel.
+template <int NV>
+struct RowSpec {
+  ColRes pred[2];
+  int32_t pop[2];
+  int64_t pci[2];
+  double pcf[2];
+  bool pon[2];
+  ColRes key[2];
+  int32_t nkeys;
+  int32_t narrow[2], shift[2];
+  ColRes val[NV > 0 ? NV : 1];
+  int32_t nvals;
+  const uint32_t* sel;
+  uint64_t row_base;
+};
+
 // PP provides cs, key (at most two packed columns), pred[2], sel, row_base,
 // n_vals, val_col.
-template <int NV, int U, typename PP>
-__device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t step, uint64_t end,
+template <int NV, typename PP>
+__device__ __forceinline__ RowSpec<NV> make_spec(const PP& P) {
+  RowSpec<NV> S;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    S.pon[q] = P.pred[q].col >= 0;
+    S.pred[q] = resolve_col(P.cs.cols[S.pon[q] ? P.pred[q].col : 0], S.pon[q]);
+    S.pop[q] = P.pred[q].op;
+    S.pci[q] = P.pred[q].i64;
+    S.pcf[q] = P.pred[q].f64;
+  }
+  S.nkeys = P.key.n;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const bool on = k < P.key.n;
+    S.key[k] = resolve_col(P.cs.cols[P.key.col[on ? k : 0]], on);
+    S.narrow[k] = on ? P.key.narrow[k] : 0;
+    S.shift[k] = on ? P.key.shift[k] : 0;
+  }
+  S.nvals = P.n_vals;
+#pragma unroll
+  for (int j = 0; j < (NV > 0 ? NV : 1); ++j) {
+    const bool on = j < P.n_vals;
+    S.val[j] = resolve_col(P.cs.cols[on ? P.val_col[j] : 0], on);
+  }
+  S.sel = P.sel;
+  S.row_base = P.row_base;
+  return S;
+}
+
+// Load and classify U rows per thread (positions i0 + u * step < end; the
+// row is sel[i] or row_base + i). Every load of every row is issued before
+// any value is used, so a wave keeps U rows of traffic in flight.
+// kind: -1 filtered out or past the end, 0 keyed row, 1 sentinel-valued key,
+// 2 NULL single-column key. NV value columns are loaded as raw words with
+// their validity bits in vvalid (bit j).
+template <int NV, int U>
+__device__ __forceinline__ void load_rows(const RowSpec<NV>& S, uint64_t i0, uint64_t step, uint64_t end,
                                           uint64_t (&row)[U], uint64_t (&key)[U], int (&kind)[U],
                                           uint64_t (&val)[U][NV > 0 ? NV : 1], uint32_t (&vvalid)[U]) {
   bool in[U];
@@ -563,40 +645,31 @@ __device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t ste
     const uint64_t i = i0 + u * step;
     in[u] = i < end;
     const uint64_t ii = in[u] ? i : end - 1;
-    row[u] = P.sel ? (uint64_t)P.sel[ii] : P.row_base + ii;
+    row[u] = S.sel ? (uint64_t)S.sel[ii] : S.row_base + ii;
   }
   // loads: no use of a loaded value until every load of the batch is issued
   RawVal pr[2][U], kw[2][U], vr[U][NV > 0 ? NV : 1];
   uint64_t pv[2][U], kv[U], vv[U][NV > 0 ? NV : 1];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const bool on = P.pred[q].col >= 0;
-    const ColDesc& c = P.cs.cols[on ? P.pred[q].col : 0];
+  for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      pr[q][u] = load_raw(c, row[u], on);
-      pv[q][u] = on ? valid_word(c, row[u]) : ~0ull;
+      pr[q][u] = load_res(S.pred[q], row[u]);
+      pv[q][u] = valid_res(S.pred[q], row[u]);
     }
-  }
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const bool on = k < P.key.n;
-    const ColDesc& c = P.cs.cols[P.key.col[on ? k : 0]];
+  for (int k = 0; k < 2; ++k)
 #pragma unroll
-    for (int u = 0; u < U; ++u) kw[k][u] = load_raw(c, row[u], on);
-  }
+    for (int u = 0; u < U; ++u) kw[k][u] = load_res(S.key[k], row[u]);
 #pragma unroll
-  for (int u = 0; u < U; ++u) kv[u] = valid_word(P.cs.cols[P.key.col[0]], row[u]);
+  for (int u = 0; u < U; ++u) kv[u] = valid_res(S.key[0], row[u]);
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    const bool on = j < P.n_vals;
-    const ColDesc& c = P.cs.cols[on ? P.val_col[j] : 0];
+  for (int j = 0; j < NV; ++j)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      vr[u][j] = load_raw(c, row[u], on);
-      vv[u][j] = on ? valid_word(c, row[u]) : ~0ull;
+      vr[u][j] = load_res(S.val[j], row[u]);
+      vv[u][j] = valid_res(S.val[j], row[u]);
     }
-  }
   // every load above is in flight; from here on the values are used, one
   // type / operator dispatch per column for all U rows (not one per row)
 #pragma unroll
@@ -608,12 +681,12 @@ __device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t ste
   }
 #pragma unroll
   for (int j = 0; j < NV; ++j)
-    if (j < P.n_vals) {
+    if (j < S.nvals) {
       uint64_t tmp[U];
       RawVal col[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) col[u] = vr[u][j];
-      values_of<U>(P.cs.cols[P.val_col[j]].type, col, tmp);
+      values_of<U>(S.val[j].type, col, tmp);
 #pragma unroll
       for (int u = 0; u < U; ++u) val[u][j] = tmp[u];
     }
@@ -622,13 +695,12 @@ __device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t ste
   for (int u = 0; u < U; ++u) ok |= (uint32_t)in[u] << u;
 #pragma unroll
   for (int q = 0; q < 2; ++q)
-    if (P.pred[q].col >= 0)
-      ok &= pred_mask<U>(P.cs.cols[P.pred[q].col], pr[q], pv[q], row, P.pred[q].op, P.pred[q].i64, P.pred[q].f64);
-  values_of<U>(P.cs.cols[P.key.col[0]].type, kw[0], key);
-  if (P.key.n > 1) {
+    if (S.pon[q]) ok &= pred_mask<U>(S.pred[q].type, pr[q], pv[q], row, S.pop[q], S.pci[q], S.pcf[q]);
+  values_of<U>(S.key[0].type, kw[0], key);
+  if (S.nkeys > 1) {
     uint64_t k1[U];
-    values_of<U>(P.cs.cols[P.key.col[1]].type, kw[1], k1);
-    const int n0 = P.key.narrow[0], n1 = P.key.narrow[1], s0 = P.key.shift[0], s1 = P.key.shift[1];
+    values_of<U>(S.key[1].type, kw[1], k1);
+    const int n0 = S.narrow[0], n1 = S.narrow[1], s0 = S.shift[0], s1 = S.shift[1];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t a = n0 ? (key[u] & 0xFFFFFFFFull) << s0 : key[u];
@@ -639,7 +711,7 @@ __device__ __forceinline__ void load_rows(const PP& P, uint64_t i0, uint64_t ste
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     // multi-column keys carry no validity (the host rejects them up front)
-    const bool null_key = P.key.n == 1 && !((kv[u] >> (row[u] & 63)) & 1ull);
+    const bool null_key = S.nkeys == 1 && !((kv[u] >> (row[u] & 63)) & 1ull);
     kind[u] = !((ok >> u) & 1u) ? -1 : null_key ? 2 : (key[u] == kEmpty ? 1 : 0);
   }
 }
@@ -787,6 +859,93 @@ __device__ __forceinline__ void apply_vals_lds(uint64_t* w, int ws, const AggPro
         else at_max<LdsT>(t, ok ? e : kMaxInit);
         break;
       }
+      default:
+        break;
+    }
+  }
+}
+
+// apply_vals_lds for U rows at once: each op is decoded once for the U rows
+// (the op loop is the outer loop), so the per-op dispatch is shared.
+template <int NV, int U>
+__device__ __forceinline__ void apply_vals_lds_batch(uint64_t* const (&w)[U], int ws, const AggProgram& pg,
+                                                     const uint64_t (&v)[U][NV], uint32_t vfloat,
+                                                     const uint32_t (&vvalid)[U], uint32_t live) {
+  for (int o = 0; o < pg.n_ops; ++o) {
+    const OpDesc op = pg.ops[o];
+    const int c1 = op.col, c2 = op.col2;
+    const bool f1 = c1 >= 0 && ((vfloat >> c1) & 1u);
+    switch (op.kind) {
+      case OP_ROWCOUNT:
+#pragma unroll
+        for (int u = 0; u < U; ++u) at_add<LdsT>(w[u] + op.word * ws, (live >> u) & 1u);
+        break;
+      case OP_NNCOUNT:
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u) && (c2 < 0 || ((vvalid[u] >> c2) & 1u));
+          at_add<LdsT>(w[u] + op.word * ws, ok ? 1 : 0);
+        }
+        break;
+      case OP_SUMI:
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u) && (c2 < 0 || ((vvalid[u] >> c2) & 1u));
+          uint64_t x = pick(v[u], c1);
+          if (c2 >= 0) x *= pick(v[u], c2);
+          at_add<LdsT>(w[u] + op.word * ws, ok ? x : 0);
+        }
+        break;
+      case OP_SUMF:
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
+          const uint64_t b = pick(v[u], c1);
+          const double x = f1 ? __longlong_as_double((long long)b) : (double)(int64_t)b;
+          at_addf<LdsT>(w[u] + op.word * ws, ok ? x : -0.0);
+        }
+        break;
+      case OP_SUMD:
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
+          add_limbs2<LdsT>(w[u] + op.word * ws, ws, ok ? (int64_t)pick(v[u], c1) : 0);
+        }
+        break;
+      case OP_SUMD4:
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u) && (c2 < 0 || ((vvalid[u] >> c2) & 1u));
+          const int64_t a = (int64_t)pick(v[u], c1);
+          uint64_t lo;
+          int64_t hi;
+          if (c2 >= 0) {
+            const int64_t b = (int64_t)pick(v[u], c2);
+            lo = (uint64_t)a * (uint64_t)b;
+            hi = (int64_t)__mul64hi((long long)a, (long long)b);
+          } else {
+            lo = (uint64_t)a;
+            hi = a < 0 ? -1 : 0;
+          }
+          add_limbs4<LdsT>(w[u] + op.word * ws, ws, ok ? lo : 0, ok ? hi : 0);
+        }
+        break;
+      case OP_MIN: case OP_MAX:
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint64_t b = pick(v[u], c1);
+          bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
+          uint64_t e;
+          if (f1) {
+            ok = ok && (b & 0x7FFFFFFFFFFFFFFFull) <= 0x7FF0000000000000ull;  // NaN never wins
+            e = enc_f64_bits(b);
+          } else {
+            e = enc_i64((int64_t)b);
+          }
+          if (op.kind == OP_MIN) at_min<LdsT>(w[u] + op.word * ws, ok ? e : kMinInit);
+          else at_max<LdsT>(w[u] + op.word * ws, ok ? e : kMaxInit);
+        }
+        break;
       default:
         break;
     }

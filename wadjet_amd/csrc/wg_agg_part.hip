@@ -54,6 +54,7 @@ struct PartParams {
 
 __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   extern __shared__ uint32_t hist[];
+  const RowSpec<0> S = make_spec<0>(P);
   const uint32_t np = 1u << P.pbits;
   for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) hist[i] = 0;
   __syncthreads();
@@ -63,7 +64,7 @@ __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
     uint64_t row[kCountU], key[kCountU], val[kCountU][1];
     uint32_t vvalid[kCountU];
     int kind[kCountU];
-    load_rows<0, kCountU>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+    load_rows<0, kCountU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
     // no per-row branch: rows that do not count add 0 to their bucket
 #pragma unroll
     for (int u = 0; u < kCountU; ++u) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], kind[u] == 0 ? 1u : 0u);
@@ -78,6 +79,7 @@ template <int RW>
 __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) {
   constexpr int NV = RW - 1;
   extern __shared__ uint32_t cur[];
+  const RowSpec<NV> S = make_spec<NV>(P);
   const uint32_t np = 1u << P.pbits;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) cur[p] = P.offs[(uint64_t)p * gridDim.x + blockIdx.x];
   __syncthreads();
@@ -88,7 +90,7 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
     uint64_t row[kScatterU], key[kScatterU], val[kScatterU][NV];
     uint32_t vvalid[kScatterU];
     int kind[kScatterU];
-    load_rows<NV, kScatterU>(P, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+    load_rows<NV, kScatterU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
     for (int u = 0; u < kScatterU; ++u) {
       if (kind[u] < 0) continue;
@@ -248,12 +250,20 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
         ls[u] = in[u] ? lds_find_or_claim(lds, L - 1, rec[u][0], key_hash(rec[u][0]), 16) : nullptr;
       // same LDS atomics on every lane (identity operands on the spare slot
       // where the row does not aggregate here)
+      {
+        uint64_t* w[kAggU];
+        uint64_t v[kAggU][NV];
+        uint32_t vvalid[kAggU];
+        uint32_t live = 0;
 #pragma unroll
-      for (int u = 0; u < kAggU; ++u) {
-        uint64_t v[NV];
+        for (int u = 0; u < kAggU; ++u) {
+          w[u] = ls[u] ? ls[u] : spare;
+          live |= (uint32_t)(ls[u] != nullptr) << u;
+          vvalid[u] = ~0u;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) v[j] = rec[u][j + 1];
-        apply_vals_lds(ls[u] ? ls[u] : spare, LW, P.pg, v, P.vfloat, ~0u, ls[u] != nullptr);
+          for (int j = 0; j < NV; ++j) v[u][j] = rec[u][j + 1];
+        }
+        apply_vals_lds_batch<NV, kAggU>(w, LW, P.pg, v, P.vfloat, vvalid, live);
       }
 #pragma unroll
       for (int u = 0; u < kAggU; ++u) {
