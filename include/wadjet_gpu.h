@@ -226,6 +226,11 @@ wg_status wg_agg_pass_times(wg_agg* h, float* ms4);
  * and the largest partition chunk one workgroup takes. */
 wg_status wg_agg_set_strategy(wg_agg* h, int32_t strategy, int32_t part_lds_slots, int32_t part_block,
                               uint64_t part_chunk_rows);
+/* Heavy hitters of the partitioned consume (no reference counterpart; results
+ * do not depend on it): up to hot_slots keys that a sample of the batch sees
+ * most often are aggregated in LDS during the scatter pass instead of being
+ * partitioned. 0 disables; otherwise a power of two in [64, 2048]. */
+wg_status wg_agg_set_heavy_hitters(wg_agg* h, int32_t hot_slots);
 
 /* ------------------------------------------------------------------ filter */
 
@@ -302,6 +307,13 @@ wg_status wg_gather(wg_ctx* ctx, const wg_column* col, const uint32_t* rows, uin
 wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_probe_cols,
                             int32_t probe_key, uint64_t n_rows, const uint32_t* sel,
                             uint64_t n_sel, wg_agg* agg);
+/* Tuning of wg_join_probe_agg (no reference counterpart; results do not
+ * depend on it). lean = 1 lets a query whose single group key and values come
+ * from probe columns or the first two inline payload columns, none nullable,
+ * take the streamlined kernel; lean = 0 keeps the general one. block: 512 or
+ * 1024 threads; replicas: accumulator copies per LDS group slot (power of two
+ * <= 64, 0 = as many as the LDS budget allows). */
+wg_status wg_join_set_tuning(wg_join* j, int32_t lean, int32_t block, int32_t replicas);
 /* Order-free digest of the join output multiset over (probe key, probe col q,
  * build col p1, build col p2): count plus two independent 64-bit sums of
  * per-row mixes, computed on the device from a pairs list. */

@@ -84,15 +84,17 @@ def test_config2_shape_small(ctx, tuning):
     check(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED, tuning=tuning)
 
 
-@pytest.mark.parametrize("chunk", [1 << 16, 2048])
-def test_config2_partitioned(ctx, chunk):
+@pytest.mark.parametrize("chunk,hot", [(1 << 16, 512), (2048, 512), (1 << 16, 0), (2048, 0), (1 << 16, 2048)])
+def test_config2_partitioned(ctx, chunk, hot):
     """The partitioned consume (count, scatter, per-partition LDS aggregation);
-    a small chunk splits the hot partitions over several workgroups."""
+    a small chunk splits the large partitions over several workgroups; the
+    heavy hitters of the Zipf key aggregate in the scatter pass (hot > 0)."""
     n = 3_000_000
     cols = W.gen_config2(ctx, n, seed=31)
-    print(f"[agg] partitioned chunk={chunk}", flush=True)
+    print(f"[agg] partitioned chunk={chunk} hot={hot}", flush=True)
     agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES, group_hint=300_000)
     agg.set_strategy(2, 2048, 1024, chunk)
+    agg.set_heavy_hitters(hot)
     agg.consume(cols, n, pred=W.CFG2_PRED)
     t = agg.pass_times()
     assert t[0] == 0 and t[2] > 0, t  # the partitioned passes ran
@@ -119,25 +121,33 @@ def test_partitioned_overflow_grows_and_replays(ctx):
             abs_sums(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
 
 
-def test_partitioned_two_consumes_and_sentinel_keys(ctx):
+@pytest.mark.parametrize("hot", [512, 64, 0])
+def test_partitioned_two_consumes_and_sentinel_keys(ctx, hot):
     """Two partitioned consumes into one aggregate (keys already in the table
-    are merged by their owner), with sentinel-valued keys on the side path."""
-    rng = np.random.default_rng(41)
-    n = 1_500_000
+    are merged by their owner), with sentinel-valued and NULL keys on the
+    side path and 30 % of the rows on 50 heavy keys (hot set)."""
+    rng = np.random.default_rng(41 + hot)
+    n = 2_600_000
     k = rng.integers(0, 400_000, n).astype(np.int64)
+    heavy = rng.random(n) < 0.3
+    k[heavy] = rng.integers(0, 50, int(heavy.sum())) * 7919
     k[::1000] = -0x7FFFFFFFFFFFFFFF
+    kvalid = rng.random(n) > 0.01
     v = rng.integers(-10**12, 10**12, n).astype(np.int64)
     f = rng.standard_normal(n)
     types = [L.INT64, L.INT64, L.FLOAT64]
-    aggs = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MAX, 2), AggSpec(L.AVG, 1)]
-    agg = HashAggregate(ctx, types, [0], aggs, group_hint=400_000)
+    aggs = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MAX, 2), AggSpec(L.AVG, 1), AggSpec(L.MIN, 1)]
+    print(f"[agg] partitioned two consumes hot={hot}", flush=True)
+    agg = HashAggregate(ctx, types, [0], aggs, nullable=[True, False, False], group_hint=400_000)
     agg.set_strategy(2, 2048, 1024, 1 << 15)
+    agg.set_heavy_hitters(hot)
     h = n // 2
     for lo, hi in ((0, h), (h, n)):
-        agg.consume([Col(ctx.to_device(k[lo:hi]), L.INT64), Col(ctx.to_device(v[lo:hi]), L.INT64),
+        kv = ctx.to_device(bitmap_from_mask(kvalid[lo:hi]))
+        agg.consume([Col(ctx.to_device(k[lo:hi]), L.INT64, kv), Col(ctx.to_device(v[lo:hi]), L.INT64),
                      Col(ctx.to_device(f[lo:hi]), L.FLOAT64)], hi - lo)
     agg.finalize()
-    hcols = [(k, L.INT64), (v, L.INT64), (f, L.FLOAT64)]
+    hcols = [(k, L.INT64, bitmap_from_mask(kvalid)), (v, L.INT64), (f, L.FLOAT64)]
     ref = oracle_groups(run_oracle(hcols, [0], aggs))
     compare(gpu_groups(agg), ref, [(a.func, a.col) for a in aggs], types, None, abs_sums(hcols, [0], aggs))
 

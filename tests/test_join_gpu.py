@@ -100,15 +100,17 @@ def test_config3_shape_join_multiset_and_digest(ctx):
     np.testing.assert_array_equal(out.to_host(np.int64, len(b)), p2[b])
 
 
-@pytest.mark.parametrize("lds", [1024, 0])
-def test_config3_fused_probe_aggregate(ctx, lds):
+@pytest.mark.parametrize("lds,lean", [(1024, True), (1024, False), (0, False)])
+def test_config3_fused_probe_aggregate(ctx, lds, lean):
     """JOIN fk = key GROUP BY p1 SUM(qty * p2) (DECIMAL product -> Int128),
-    COUNT(*), MIN(qty), AVG(p2): fused probe vs oracle over the pair list."""
+    COUNT(*), MIN(qty), AVG(p2): fused probe vs oracle over the pair list,
+    through the streamlined kernel and the general one."""
     cfg = W.Cfg3(n_build=300_000, n_probe=2_000_000, seed=17)
-    print(f"[join] fused probe-agg lds={lds} {cfg}", flush=True)
+    print(f"[join] fused probe-agg lds={lds} lean={lean} {cfg}", flush=True)
     bcols = W.gen_config3_build(ctx, cfg)
     pcols = W.gen_config3_probe(ctx, cfg)
     j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    j.set_tuning(lean=lean)
     j.build(bcols, cfg.n_build)
     j.finish()
     types = W.CFG3_PROBE_TYPES + W.CFG3_BUILD_TYPES  # fk, qty, key, p1, p2
@@ -161,6 +163,71 @@ def test_fused_probe_aggregate_many_to_many_with_overflow(ctx):
     jcols = [(pk[pp], O.INT64), (pv[pp], O.INT32), (bk[bb], O.INT64), (bv[bb], O.INT64)]
     ref = oracle_groups(O.hash_agg(jcols, [3], [(O.SUM, 3), (O.COUNT, -1), (O.MAX, 1)]))
     compare(gpu_groups(agg), ref, [(O.SUM, 3), (O.COUNT, -1), (O.MAX, 1)], types)
+
+
+SENTINEL = -0x7FFFFFFFFFFFFFFF  # the tables' EMPTY key word, a legal key value
+
+
+@pytest.mark.parametrize("block,replicas,hint", [(512, 0, 40), (1024, 0, 40), (512, 1, 40), (512, 0, 4)])
+def test_fused_probe_aggregate_streamlined_edges(ctx, block, replicas, hint):
+    """The streamlined fused kernel and its hand-offs to the general one:
+    duplicate build keys (chains), NULL probe keys, the sentinel value as a
+    probe/build key and as a group key, a selection vector, float and int
+    probe values, a group key from an inline build column and from a probe
+    column, and a group hint below the real group count (LDS table full)."""
+    print(f"[join] streamlined edges block={block} replicas={replicas} hint={hint}", flush=True)
+    rng = np.random.default_rng(block + replicas + hint)
+    nb, npr = 40_000, 300_000
+    bk = rng.permutation(60_000)[:nb].astype(np.int64) - 30_000
+    bk[:500] = bk[500:1000]  # 500 keys with two build rows
+    bk[1000] = SENTINEL
+    bg = rng.integers(0, 20, nb).astype(np.int64)  # inline group column
+    bg[2000:2010] = SENTINEL
+    bv = rng.integers(-10**12, 10**12, nb).astype(np.int64)  # inline DECIMAL(18,2) value
+    pk = rng.integers(-31_000, 31_000, npr).astype(np.int64)
+    pk[::1000] = SENTINEL
+    pk[1::97] = bk[2000]  # rows that meet the sentinel group
+    pvalid = rng.random(npr) > 0.03
+    pq = rng.integers(1, 50, npr).astype(np.int32)
+    pf = rng.normal(0, 1, npr)
+    pgrp = rng.integers(0, 30, npr).astype(np.int32)
+    sel = np.sort(rng.choice(npr, npr // 2, replace=False)).astype(np.uint32)
+    j = HashJoin(ctx, [L.INT64, L.INT64, L.DECIMAL], 0, inline=[1, 2], scales=[0, 0, 2], build_hint=nb)
+    j.set_tuning(lean=True, block=block, replicas=replicas)
+    j.build([Col(ctx.to_device(bk), L.INT64), Col(ctx.to_device(bg), L.INT64),
+             Col(ctx.to_device(bv), L.DECIMAL, None, 2)], nb)
+    j.finish()
+    pw = bitmap_from_mask(pvalid)
+    pcols = [Col(ctx.to_device(pk), L.INT64, ctx.to_device(pw)), Col(ctx.to_device(pq), L.INT32),
+             Col(ctx.to_device(pf), L.FLOAT64), Col(ctx.to_device(pgrp), L.INT32)]
+    types = [L.INT64, L.INT32, L.FLOAT64, L.INT32, L.INT64, L.INT64, L.DECIMAL]
+    scales = [0, 0, 0, 0, 0, 0, 2]
+    pp, bb = O.hash_join((bk, O.INT64), (pk, O.INT64, pw), probe_sel=sel, cap=npr * 4)
+    for gcol, aggs in ((5, [AggSpec(L.SUM, 1, 6), AggSpec(L.COUNT, -1), AggSpec(L.MAX, 2), AggSpec(L.SUM, 2)]),
+                       (3, [AggSpec(L.SUM, 6), AggSpec(L.MIN, 2), AggSpec(L.AVG, 1), AggSpec(L.COUNT, 1)])):
+        agg = HashAggregate(ctx, types, [gcol], aggs, scales=scales, group_hint=hint)
+        j.probe_agg(pcols, 0, npr, agg, sel=(ctx.to_device(sel), len(sel)))
+        agg.finalize()
+        prod = pq[pp].astype(np.int64) * bv[bb]  # |.| < 50 * 10^12
+        prod128 = np.empty(2 * len(prod), np.uint64)
+        prod128[0::2] = np.where(prod < 0, np.uint64(2**64 - 1), np.uint64(0))
+        prod128[1::2] = prod.view(np.uint64)
+        jcols = [(pk[pp], O.INT64), (pq[pp], O.INT32), (pf[pp], O.FLOAT64), (pgrp[pp], O.INT32), (bk[bb], O.INT64),
+                 (bg[bb], O.INT64), (bv[bb], O.DECIMAL, None, 2), (prod128, O.DECIMAL128, None, 2)]
+        oaggs = [(a.func, 7 if a.col2 >= 0 else a.col) for a in aggs]
+        res = O.hash_agg(jcols, [gcol], oaggs)
+        ref = oracle_groups(res)
+        # per-group sum(|f|) for the float-sum tolerance
+        absf = None
+        if any(a.func in (L.SUM, L.AVG) and a.col == 2 for a in aggs):
+            g = np.unique(ref["keys"][:, 0])
+            idx = np.searchsorted(g, bg[bb] if gcol == 5 else pgrp[pp].astype(np.int64))
+            absf = np.zeros(len(g))
+            np.add.at(absf, idx, np.abs(pf[pp]))
+        abs_sums = [absf if (a.func in (L.SUM, L.AVG) and a.col == 2) else None for a in aggs]
+        compare(gpu_groups(agg), ref, oaggs, types + [O.DECIMAL128], scales + [2], abs_sums=abs_sums)
+        agg.close()
+    j.close()
 
 
 @pytest.mark.parametrize("anti", [False, True])

@@ -95,6 +95,7 @@ SIGNATURES: dict[str, list] = {
     "wg_agg_kernel_time": [P, C.POINTER(C.c_float), C.POINTER(C.c_uint32)],
     "wg_agg_pass_times": [P, C.POINTER(C.c_float)],
     "wg_agg_set_strategy": [P, I32, I32, I32, U64],
+    "wg_agg_set_heavy_hitters": [P, I32],
     "wg_filter": [P, C.POINTER(Column), U64, P, U64, I32, I64, C.c_double, P, C.POINTER(U64)],
     "wg_join_create": [P, I32, C.POINTER(I32), C.POINTER(I32), I32, I32, C.POINTER(I32), U64, C.POINTER(P)],
     "wg_join_destroy": [P],
@@ -106,6 +107,7 @@ SIGNATURES: dict[str, list] = {
     "wg_join_gather_build": [P, I32, P, U64, P],
     "wg_gather": [P, C.POINTER(Column), P, U64, P],
     "wg_join_probe_agg": [P, C.POINTER(Column), I32, I32, U64, P, U64, P],
+    "wg_join_set_tuning": [P, I32, I32, I32],
     "wg_join_digest": [P, C.POINTER(Column), P, P, U64, C.POINTER(I32), I32, C.POINTER(I32), I32, C.POINTER(U64)],
     "wg_batch_create": [P, I32, U64, C.POINTER(P)],
     "wg_batch_destroy": [P],

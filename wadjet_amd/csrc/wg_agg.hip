@@ -350,6 +350,7 @@ __global__ void __launch_bounds__(kCompactBlock) agg_compact_kernel(const uint64
                                                                    uint8_t* out_null, uint64_t* out_n,
                                                                    uint64_t per_block) {
   __shared__ uint32_t wave_cnt[kCompactBlock / 64];
+  __shared__ uint8_t live_lane[kCompactBlock];  // per wave: lanes of its live slots, in order
   __shared__ unsigned long long s_base;
   const uint64_t total = cap + kSideSlots;
   const uint64_t beg = (uint64_t)blockIdx.x * per_block;
@@ -365,26 +366,35 @@ __global__ void __launch_bounds__(kCompactBlock) agg_compact_kernel(const uint64
   __syncthreads();
   if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd((unsigned long long*)out_n, s_cnt) : 0;
   __syncthreads();
-  // pass 2: place live slots in block order
+  // pass 2: place live slots in block order; a wave's live slots land in
+  // consecutive output slots, which its lanes copy word by word (coalesced)
+  const int sh = __ffs(stride) - 1;  // stride is a power of two
   uint64_t run = s_base;
   for (uint64_t i0 = beg; i0 < end; i0 += blockDim.x) {
     const uint64_t i = i0 + threadIdx.x;
     const bool side = i >= cap;
     const bool live = i < end && slot_live(slots + i * stride, side);
     const uint64_t b = __ballot(live);
+    const uint32_t rank = (uint32_t)__popcll(b & ((1ull << lane) - 1));
     if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(b);
+    if (live) live_lane[wave * 64 + rank] = (uint8_t)lane;
     __syncthreads();
     uint32_t before = 0, tot = 0;
     for (int w = 0; w < kCompactBlock / 64; ++w) {
       if (w < wave) before += wave_cnt[w];
       tot += wave_cnt[w];
     }
+    const uint64_t pos0 = run + before;
+    const uint32_t nl = (uint32_t)__popcll(b);
+    const uint64_t wave_first = i0 + (uint64_t)wave * 64;
+    for (uint32_t k = lane; k < (nl << sh); k += 64) {
+      const uint32_t j = k >> sh, w = k & (uint32_t)(stride - 1);
+      const uint64_t si = wave_first + live_lane[wave * 64 + j];
+      // side slot 0 holds the key whose value is the sentinel
+      out[(pos0 + j) * stride + w] = (w == 0 && si == cap) ? kEmpty : slots[si * stride + w];
+    }
     if (live) {
-      const uint64_t pos = run + before + __popcll(b & ((1ull << lane) - 1));
-      const uint64_t* s = slots + i * stride;
-      uint64_t* o = out + pos * stride;
-      for (int w = 0; w < stride; ++w) o[w] = s[w];
-      if (side && i == cap) o[0] = kEmpty;  // the key whose value is the sentinel
+      const uint64_t pos = pos0 + rank;
       out_null[pos] = (side && i == cap + 1) ? 1 : 0;
       if (out_base != nullptr)
         for (int w = 0; w < n_base_words; ++w) out_base[pos * n_base_words + w] = base ? base[i * n_base_words + w] : 0;
@@ -1193,6 +1203,7 @@ wg_status wg_agg_destroy(wg_agg* h) {
   (void)hipFree(h->part_scan);
   (void)hipFree(h->part_buf);
   (void)hipFree(h->part_items);
+  (void)hipFree(h->hot_buf);
   if (h->part_host) (void)hipHostFree(h->part_host);
   for (void* p : h->stage.pinned) (void)hipHostFree(p);
   for (uint64_t* p : h->stage.pvalid) (void)hipHostFree(p);
@@ -1500,6 +1511,15 @@ wg_status wg_agg_set_strategy(wg_agg* h, int32_t strategy, int32_t part_lds_slot
   h->part_lds_slots = part_lds_slots;
   h->part_block = part_block;
   h->part_chunk_rows = part_chunk_rows;
+  return WG_OK;
+}
+
+wg_status wg_agg_set_heavy_hitters(wg_agg* h, int32_t hot_slots) {
+  clear_error();
+  WG_CHECK(h, WG_ERR_INVALID, "null handle");
+  WG_CHECK(hot_slots == 0 || (hot_slots >= 64 && hot_slots <= 2048 && (hot_slots & (hot_slots - 1)) == 0),
+           WG_ERR_INVALID, "hot slots %d: 0 or a power of two in [64, 2048]", hot_slots);
+  h->part_hot = hot_slots;
   return WG_OK;
 }
 

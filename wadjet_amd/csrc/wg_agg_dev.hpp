@@ -207,7 +207,8 @@ __device__ __forceinline__ void apply_row(uint64_t* w, int ws, const AggProgram&
 }
 
 // Fold a partial slot (LDS entry with word stride `sws`, or an overflow
-// record with sws = 1) into a global slot.
+// record with sws = 1) into a global slot (S = GlbT) or an LDS slot (LdsT).
+template <typename S = GlbT>
 __device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src, int sws, const AggProgram& pg) {
   for (int o = 0; o < pg.n_ops; ++o) {
     const OpDesc op = pg.ops[o];
@@ -215,24 +216,24 @@ __device__ __forceinline__ void merge_partial(uint64_t* dst, const uint64_t* src
     const uint64_t* s = src + op.word * sws;
     switch (op.kind) {
       case OP_ROWCOUNT: case OP_NNCOUNT: case OP_SUMI:
-        if (s[0]) at_add<GlbT>(t, s[0]);
+        if (s[0]) at_add<S>(t, s[0]);
         break;
       case OP_SUMF:
-        if (s[0]) at_addf<GlbT>(t, __longlong_as_double((long long)s[0]));
+        if (s[0]) at_addf<S>(t, __longlong_as_double((long long)s[0]));
         break;
       case OP_SUMD:
-        if (s[0]) at_add<GlbT>(t, s[0]);
-        if (s[sws]) at_add<GlbT>(t + 1, s[sws]);
+        if (s[0]) at_add<S>(t, s[0]);
+        if (s[sws]) at_add<S>(t + 1, s[sws]);
         break;
       case OP_SUMD4:
         for (int k = 0; k < 4; ++k)
-          if (s[k * sws]) at_add<GlbT>(t + k, s[k * sws]);
+          if (s[k * sws]) at_add<S>(t + k, s[k * sws]);
         break;
       case OP_MIN:
-        if (s[0] != kMinInit) at_min<GlbT>(t, s[0]);
+        if (s[0] != kMinInit) at_min<S>(t, s[0]);
         break;
       case OP_MAX:
-        if (s[0] != kMaxInit) at_max<GlbT>(t, s[0]);
+        if (s[0] != kMaxInit) at_max<S>(t, s[0]);
         break;
       default:
         break;

@@ -81,6 +81,9 @@ struct wg_agg {
   uint64_t part_items_bytes = 0;
   void* part_host = nullptr;  // pinned: partition boundaries + work items
   uint64_t part_host_bytes = 0;
+  int32_t part_hot = 512;          // heavy-hitter slots of the partitioned consume (0: none)
+  uint64_t* hot_buf = nullptr;     // hot-set selection scratch + per-workgroup partial slots
+  uint64_t hot_buf_words = 0;
   float pass_ms[4] = {0, 0, 0, 0};  // direct consume, count+scan, scatter, partition aggregate
   hipEvent_t pev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // tuning

@@ -19,6 +19,16 @@
 //                           partition split over several workgroups
 // Rows whose key is NULL or equals the table sentinel go straight to the side
 // slots in pass C. Results are the same accumulator words as the direct path.
+//
+// Heavy hitters: under a skewed key (Zipf) a few hundred keys carry half the
+// rows. Before pass A a sample of the batch (kSampleTiles spread tiles) is
+// counted and the keys seen most often (at most H of them) become the hot
+// set. Passes A and C look every row up in an LDS copy of the hot set: hot
+// rows are not counted or scattered but aggregated in an LDS table of H
+// slots inside pass C; each workgroup writes its H partial slots out, and
+// hot_merge_kernel folds them into the global table. Only the cold rows pay
+// for the scattered record writes and for pass D. The hot set is fixed before
+// pass A, so passes A and C classify every row the same way.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -50,13 +60,83 @@ struct PartParams {
   const uint32_t* offs;  // [P][G] exclusive scan of counts
   uint64_t* recs;        // partitioned records: key word, value words, zero padding
   int32_t val_col[kMaxVals];
+  // heavy hitters (hot_slots = 0: none)
+  AggProgram pgv;            // pg with op columns remapped onto the value words
+  uint32_t vfloat;           // bit j: value word j holds float64 bits
+  const uint64_t* hot_keys;  // the hot set, *n_hot keys
+  const uint32_t* n_hot;
+  int32_t hot_slots;         // H: LDS accumulator slots of pass C (>= *n_hot)
+  int32_t hot_bits;          // the LDS lookup of the hot set has 2^hot_bits entries
+  uint64_t* hot_part;        // [G][H][stride] per-workgroup partial slots of pass C
 };
+
+// LDS of passes A and C: the partition counters (np u32), then the hot-set
+// lookup (2^hot_bits keys and their hot indices), then in pass C the hot
+// accumulators (struct of arrays, H slots + one spare slot per lane).
+// A bit filter over 16 hash bits answers most cold rows with one LDS read.
+struct HotLds {
+  uint32_t* bits;  // kHotFilterBits bits: set for the hash of every hot key
+  uint64_t* key;
+  uint32_t* idx;
+  uint64_t* acc;
+  uint32_t mask;
+  int HW;  // word stride of acc
+};
+
+constexpr uint32_t kHotFilterBits = 1u << 16;
+
+__host__ __device__ inline size_t hot_lookup_bytes(int hot_bits) {
+  return kHotFilterBits / 8 + ((size_t)12 << hot_bits);
+}
+
+__device__ __forceinline__ HotLds hot_lds(uint32_t* base, uint32_t np, int hot_bits, int H) {
+  HotLds L;
+  L.bits = base + np;
+  L.key = (uint64_t*)(L.bits + kHotFilterBits / 32);  // np >= 64: 8-byte aligned
+  L.idx = (uint32_t*)(L.key + (1u << hot_bits));
+  L.acc = (uint64_t*)(L.idx + (1u << hot_bits));  // 2^hot_bits >= 2: 8-byte aligned
+  L.mask = (1u << hot_bits) - 1;
+  L.HW = lds_word_stride(H);
+  return L;
+}
+
+// Fill the LDS lookup with the hot keys (distinct, never the sentinel; the
+// table is at least twice the key count, so every insertion ends).
+__device__ __forceinline__ void hot_lookup_build(const HotLds& L, const uint64_t* keys, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i <= L.mask; i += blockDim.x) L.key[i] = kEmpty;
+  for (uint32_t i = threadIdx.x; i < kHotFilterBits / 32; i += blockDim.x) L.bits[i] = 0;
+  __syncthreads();
+  for (uint32_t h = threadIdx.x; h < n; h += blockDim.x) {
+    const uint64_t k = keys[h];
+    const uint64_t hk = key_hash(k);
+    atomicOr(&L.bits[(uint32_t)(hk & (kHotFilterBits - 1)) >> 5], 1u << (hk & 31));
+    uint32_t i = (uint32_t)(hk >> 24) & L.mask;
+    while (at_cas<LdsT>(L.key + i, kEmpty, k) != kEmpty) i = (i + 1) & L.mask;
+    L.idx[i] = h;
+  }
+  __syncthreads();
+}
+
+// Hot index of a key (hk = key_hash(key)), or -1.
+__device__ __forceinline__ int hot_find(const HotLds& L, uint64_t key, uint64_t hk) {
+  if (!((L.bits[(uint32_t)(hk & (kHotFilterBits - 1)) >> 5] >> (hk & 31)) & 1u)) return -1;
+  uint32_t i = (uint32_t)(hk >> 24) & L.mask;
+  for (uint32_t p = 0; p <= L.mask; ++p) {
+    const uint64_t k = L.key[i];
+    if (k == key) return (int)L.idx[i];
+    if (k == kEmpty) return -1;
+    i = (i + 1) & L.mask;
+  }
+  return -1;
+}
 
 __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   extern __shared__ uint32_t hist[];
   const RowSpec<0> S = make_spec<0>(P);
   const uint32_t np = 1u << P.pbits;
   for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) hist[i] = 0;
+  const HotLds HL = hot_lds(hist, np, P.hot_bits, P.hot_slots);
+  if (P.hot_slots) hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)P.hot_slots));
   __syncthreads();
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
@@ -65,9 +145,14 @@ __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
     uint32_t vvalid[kCountU];
     int kind[kCountU];
     load_rows<0, kCountU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
-    // no per-row branch: rows that do not count add 0 to their bucket
+    // no per-row branch: rows that do not count (filtered, side slot, hot)
+    // add 0 to their bucket
 #pragma unroll
-    for (int u = 0; u < kCountU; ++u) atomicAdd(&hist[key_hash(key[u]) >> (64 - P.pbits)], kind[u] == 0 ? 1u : 0u);
+    for (int u = 0; u < kCountU; ++u) {
+      const uint64_t hk = key_hash(key[u]);
+      const bool cold = kind[u] == 0 && (P.hot_slots == 0 || hot_find(HL, key[u], hk) < 0);
+      atomicAdd(&hist[hk >> (64 - P.pbits)], cold ? 1u : 0u);
+    }
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
@@ -82,6 +167,16 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
   const RowSpec<NV> S = make_spec<NV>(P);
   const uint32_t np = 1u << P.pbits;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) cur[p] = P.offs[(uint64_t)p * gridDim.x + blockIdx.x];
+  const HotLds HL = hot_lds(cur, np, P.hot_bits, P.hot_slots);
+  const int H = P.hot_slots;
+  if (H) {
+    for (int w = 0; w < P.pg.stride; ++w) {
+      const uint64_t v = slot_init_word(P.pg, w);
+      for (int i = threadIdx.x; i < HL.HW; i += blockDim.x) HL.acc[(uint64_t)w * HL.HW + i] = v;
+    }
+    hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)H));
+  }
+  uint64_t* const spare = HL.acc + H + (threadIdx.x & 63);
   __syncthreads();
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
@@ -91,16 +186,32 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
     uint32_t vvalid[kScatterU];
     int kind[kScatterU];
     load_rows<NV, kScatterU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+    uint64_t hk[kScatterU];
+    uint32_t hot = 0;
+#pragma unroll
+    for (int u = 0; u < kScatterU; ++u) hk[u] = key_hash(key[u]);
+    if (H) {
+      // hot rows aggregate here: the same LDS atomics on every lane
+      // (identity operands on the lane's spare slot for the others)
+      uint64_t* w[kScatterU];
+#pragma unroll
+      for (int u = 0; u < kScatterU; ++u) {
+        const int hi = kind[u] == 0 ? hot_find(HL, key[u], hk[u]) : -1;
+        w[u] = hi >= 0 ? HL.acc + hi : spare;
+        hot |= (uint32_t)(hi >= 0) << u;
+      }
+      apply_vals_lds_batch<NV, kScatterU>(w, HL.HW, P.pgv, val, P.vfloat, vvalid, hot);
+    }
 #pragma unroll
     for (int u = 0; u < kScatterU; ++u) {
-      if (kind[u] < 0) continue;
+      if (kind[u] < 0 || ((hot >> u) & 1u)) continue;
       if (kind[u] != 0) {
         // side slots: [0] the sentinel-valued key, [1] the NULL key
         uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
         apply_row<GlbT, false>(s, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
         continue;
       }
-      const uint32_t pos = atomicAdd(&cur[key_hash(key[u]) >> (64 - P.pbits)], 1u);
+      const uint32_t pos = atomicAdd(&cur[hk[u] >> (64 - P.pbits)], 1u);
       uint64_t w[RW];
       w[0] = key[u];
 #pragma unroll
@@ -115,6 +226,157 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
       }
     }
   }
+  if (H) {
+    // this workgroup's partial hot slots, dense: [H][stride], key in word 0
+    __syncthreads();
+    const uint32_t n_hot = min(*P.n_hot, (uint32_t)H);
+    uint64_t* dst = P.hot_part + (uint64_t)blockIdx.x * H * stride;
+    for (int i = threadIdx.x; i < H * stride; i += blockDim.x) {
+      const int h = i / stride, w = i % stride;
+      dst[i] = w ? HL.acc[(uint64_t)w * HL.HW + h] : ((uint32_t)h < n_hot ? P.hot_keys[h] : kEmpty);
+    }
+  }
+}
+
+// ------------------------------------------------------------ hot set
+
+constexpr uint64_t kSampleTiles = 128;     // spread tiles of the batch counted for the hot set
+constexpr uint64_t kSampleTileRows = 1024;
+constexpr int kSampleBits = 18;            // sample count table: 2^18 {key, count} slots
+constexpr int kHotHistBins = 1024;
+constexpr uint32_t kHotMinCount = 4;       // a hot key is seen at least this often in the sample
+
+// Scratch of the hot-set selection: the sample table, the count histogram,
+// the threshold and the key count, then the hot keys.
+struct HotScratch {
+  uint64_t* tab;
+  uint32_t* hist;
+  uint32_t* thr;  // [0] threshold count, [1] keys selected
+  uint64_t* keys;
+};
+
+__global__ void hot_reset_kernel(HotScratch X) {
+  const uint64_t n = 1ull << kSampleBits;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    X.tab[2 * i] = kEmpty;
+    X.tab[2 * i + 1] = 0;
+    if (i < kHotHistBins) X.hist[i] = 0;
+    if (i < 2) X.thr[i] = 0;
+  }
+}
+
+// Count the keys of kSampleTiles tiles of kSampleTileRows rows, spread over
+// the batch (same filtering and key packing as the passes).
+__global__ void __launch_bounds__(256) hot_sample_kernel(PartParams P, HotScratch X) {
+  const RowSpec<0> S = make_spec<0>(P);
+  const uint64_t mask = (1ull << kSampleBits) - 1;
+  const uint64_t gap = P.n / kSampleTiles;
+  for (uint64_t t = blockIdx.x; t < kSampleTiles; t += gridDim.x) {
+    const uint64_t beg = t * gap, end = min(beg + kSampleTileRows, P.n);
+    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kCountU) {
+      uint64_t row[kCountU], key[kCountU], val[kCountU][1];
+      uint32_t vvalid[kCountU];
+      int kind[kCountU];
+      load_rows<0, kCountU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+#pragma unroll
+      for (int u = 0; u < kCountU; ++u) {
+        if (kind[u] != 0) continue;
+        // at most kSampleTiles * kSampleTileRows keys in a table twice that size
+        uint64_t idx = key_hash(key[u]) & mask;
+        for (uint64_t p = 0; p <= mask; ++p) {
+          uint64_t* s = X.tab + 2 * idx;
+          const uint64_t prev = at_cas<GlbT>(s, kEmpty, key[u]);
+          if (prev == kEmpty || prev == key[u]) {
+            at_add<GlbT>(s + 1, 1);
+            break;
+          }
+          idx = (idx + 1) & mask;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) hot_hist_kernel(HotScratch X) {
+  __shared__ uint32_t h[kHotHistBins];
+  for (int i = threadIdx.x; i < kHotHistBins; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint64_t n = 1ull << kSampleBits;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (X.tab[2 * i] != kEmpty) {
+      const uint64_t c = X.tab[2 * i + 1];
+      atomicAdd(&h[c < (uint64_t)kHotHistBins ? (uint32_t)c : (uint32_t)kHotHistBins - 1], 1u);
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHotHistBins; i += blockDim.x)
+    if (h[i]) atomicAdd(&X.hist[i], h[i]);
+}
+
+// The smallest count T >= kHotMinCount such that at most H keys were seen T
+// times or more (kHotHistBins: none): a suffix sum over the histogram, then
+// the first bin whose suffix fits.
+__global__ void __launch_bounds__(kHotHistBins) hot_threshold_kernel(HotScratch X, uint32_t H) {
+  __shared__ uint32_t sfx[kHotHistBins];
+  __shared__ uint32_t s_t;
+  const uint32_t c = threadIdx.x;
+  sfx[c] = X.hist[c];
+  if (c == 0) s_t = kHotHistBins;
+  __syncthreads();
+  for (uint32_t d = 1; d < kHotHistBins; d <<= 1) {
+    const uint32_t add = c + d < kHotHistBins ? sfx[c + d] : 0;
+    __syncthreads();
+    sfx[c] += add;
+    __syncthreads();
+  }
+  // keys seen c times or more: sfx[c], non-increasing in c
+  if (c >= kHotMinCount && sfx[c] <= H) atomicMin(&s_t, c);
+  __syncthreads();
+  if (c == 0) X.thr[0] = s_t;
+}
+
+__global__ void __launch_bounds__(256) hot_collect_kernel(HotScratch X, uint32_t H) {
+  const uint64_t n = 1ull << kSampleBits;
+  const uint32_t T = X.thr[0];
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = X.tab[2 * i];
+    if (k == kEmpty || X.tab[2 * i + 1] < T) continue;
+    const uint32_t at = atomicAdd(&X.thr[1], 1u);
+    if (at < H) X.keys[at] = k;  // at most H by the choice of T
+  }
+}
+
+// Fold the per-workgroup hot partials of one hot key (block h) in LDS, then
+// into the global table (overflow partial list if the table is past budget).
+__global__ void __launch_bounds__(256) hot_merge_kernel(const uint64_t* part, uint32_t G, uint32_t H,
+                                                       const uint32_t* n_hot, AggProgram pg, GlobalTable T,
+                                                       uint64_t* ovf_part) {
+  __shared__ uint64_t s[64];
+  const int stride = pg.stride;
+  const uint32_t h = blockIdx.x;
+  uint32_t claims = 0;
+  if (h < min(*n_hot, H)) {  // uniform per block
+    if (threadIdx.x < (unsigned)stride) s[threadIdx.x] = slot_init_word(pg, threadIdx.x);
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < G; g += blockDim.x) {
+      const uint64_t* src = part + ((uint64_t)g * H + h) * stride;
+      if (src[1]) merge_partial<LdsT>(s, src, 1, pg);  // word 1: the slot's row count
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s[1]) {
+      const uint64_t key = part[(uint64_t)h * stride];
+      uint64_t* d = glb_find_or_claim(T, stride, key, key_hash(key), claims);
+      if (d) {
+        merge_partial(d, s, 1, pg);
+      } else {
+        const uint64_t at =
+            __hip_atomic_fetch_add(T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t* rec = ovf_part + at * (uint64_t)stride;  // in bounds: the list holds far more than H partials
+        s[0] = key;
+        for (int w = 0; w < stride; ++w) rec[w] = s[w];
+      }
+    }
+  }
+  block_add(T.counters + CTR_GROUPS, claims);
 }
 
 // Device copy of the partition boundaries: bound[p] = first position of
@@ -418,12 +680,32 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   int pbits = 6;
   while (pbits < 14 && (1ull << pbits) * (uint64_t)(3 * L / 4) < est) ++pbits;
   const uint32_t np = 1u << pbits;
-  const size_t hist_bytes = (size_t)np * 4;
+  // heavy hitters: batches large enough for a sample to find them
+  // (as many of the requested slots as fit the scatter pass's LDS)
+  int H = (h->part_hot > 0 && n >= (1ull << 20) && h->pg.stride <= 64) ? h->part_hot : 0;
+  auto hot_bits_for = [](int hs) {
+    int b = 0;
+    while (hs && (1 << b) < 2 * hs) ++b;
+    return b;
+  };
+  auto scatter_lds = [&](int hs) {
+    return (size_t)np * 4 + (hs ? hot_lookup_bytes(hot_bits_for(hs)) + lds_table_bytes(hs, h->pg.stride) : 0);
+  };
+  while (H > 64 && scatter_lds(H) > 150 * 1024) H >>= 1;
+  if (scatter_lds(H) > 150 * 1024) H = 0;
+  const int hot_bits = hot_bits_for(H);
+  const size_t hist_bytes = (size_t)np * 4 + (H ? hot_lookup_bytes(hot_bits) : 0);
+  const size_t scat_bytes = scatter_lds(H);
+  if (scat_bytes > 64 * 1024) {
+    WG_HIP(hipFuncSetAttribute((const void*)part_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    WG_HIP(hipFuncSetAttribute((const void*)scatter_kernel_for(RW), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               150 * 1024));
+  }
   // one resident round of count/scatter workgroups (the two passes must use
   // the same row ranges, so both run at the occupancy of the lower one)
   int occ_c = 0, occ_s = 0;
   WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, part_count_kernel, kPartBlock, hist_bytes));
-  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_kernel_for(RW), kPartBlock, hist_bytes));
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_kernel_for(RW), kPartBlock, scat_bytes));
   const int per_cu = std::max(1, std::min(occ_c, occ_s));
   const int G = ctx->cus * per_cu;
   const uint64_t R = ((n + G - 1) / G + 255) / 256 * 256;
@@ -440,6 +722,29 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   P.pbits = pbits;
   P.n_vals = (int)vcols.size();
   for (size_t j = 0; j < vcols.size(); ++j) P.val_col[j] = vcols[j];
+  P.pgv = pg2;
+  for (size_t j = 0; j < vcols.size(); ++j)
+    if (cs.cols[vcols[j]].type == WG_FLOAT64) P.vfloat |= 1u << j;
+  HotScratch X{};
+  if (H) {
+    // scratch: sample table, histogram, threshold/count, keys, partial slots
+    const uint64_t words = (2ull << kSampleBits) + kHotHistBins / 2 + 1 + (uint64_t)H +
+                           (uint64_t)G * H * h->pg.stride;
+    if (h->hot_buf_words < words) {
+      if (h->hot_buf) WG_HIP(hipFree(h->hot_buf));
+      WG_HIP(hipMalloc(&h->hot_buf, words * 8));
+      h->hot_buf_words = words;
+    }
+    X.tab = h->hot_buf;
+    X.hist = (uint32_t*)(X.tab + (2ull << kSampleBits));
+    X.thr = X.hist + kHotHistBins;
+    X.keys = (uint64_t*)(X.thr + 2);
+    P.hot_keys = X.keys;
+    P.n_hot = X.thr + 1;
+    P.hot_slots = H;
+    P.hot_bits = hot_bits;
+    P.hot_part = X.keys + H;
+  }
 
   // scratch: counts, offsets, boundaries (device) and boundaries (pinned host)
   const uint64_t ncnt = (uint64_t)np * G;
@@ -480,6 +785,14 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     for (hipEvent_t& e : h->pev) WG_HIP(hipEventCreate(&e));
   // A: count, B: scan, boundaries
   WG_HIP(hipEventRecord(h->pev[0], ctx->stream));
+  if (H) {
+    hipLaunchKernelGGL(hot_reset_kernel, dim3(256), dim3(256), 0, ctx->stream, X);
+    hipLaunchKernelGGL(hot_sample_kernel, dim3(kSampleTiles), dim3(256), 0, ctx->stream, P, X);
+    hipLaunchKernelGGL(hot_hist_kernel, dim3(256), dim3(256), 0, ctx->stream, X);
+    hipLaunchKernelGGL(hot_threshold_kernel, dim3(1), dim3(kHotHistBins), 0, ctx->stream, X, (uint32_t)H);
+    hipLaunchKernelGGL(hot_collect_kernel, dim3(256), dim3(256), 0, ctx->stream, X, (uint32_t)H);
+    WG_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(part_count_kernel, dim3(G), dim3(kPartBlock), hist_bytes, ctx->stream, P);
   WG_HIP(hipGetLastError());
   WG_HIP(hipcub::DeviceScan::ExclusiveSum(h->part_scan, scan_bytes, counts, offs, (int)ncnt, ctx->stream));
@@ -503,7 +816,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   // C: scatter (side-slot rows are folded here)
   h->dirty = true;
   WG_HIP(hipEventRecord(h->pev[2], ctx->stream));
-  WG_TRY(launch_scatter(RW, dim3(G), hist_bytes, ctx->stream, P));
+  WG_TRY(launch_scatter(RW, dim3(G), scat_bytes, ctx->stream, P));
   WG_HIP(hipEventRecord(h->pev[3], ctx->stream));
 
   // D: work items (built while the scatter runs), largest first; partitions
@@ -535,8 +848,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   Q.n_items = (uint32_t)n_items;
   Q.lds_slots = L;
   Q.item_ctr = item_ctr;
-  for (size_t j = 0; j < vcols.size(); ++j)
-    if (cs.cols[vcols[j]].type == WG_FLOAT64) Q.vfloat |= 1u << j;
+  Q.vfloat = P.vfloat;
   const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
   const int per_cu_agg = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
   const int grid_max = ctx->cus * per_cu_agg;
@@ -636,6 +948,25 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   }
   for (auto& r : replays) (void)hipFree(r.rows);
   WG_CHECK(replays.empty() && items_done >= n_items, WG_ERR_DEVICE, "partitioned consume did not converge");
+  if (H) {
+    // E: the hot keys' per-workgroup partials into the global table
+    WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
+    WG_HIP(hipEventRecord(h->pev[4], ctx->stream));
+    hipLaunchKernelGGL(hot_merge_kernel, dim3(H), dim3(256), 0, ctx->stream, (const uint64_t*)P.hot_part, (uint32_t)G,
+                       (uint32_t)H, P.n_hot, h->pg, table_of(h), h->ovf_part);
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipEventRecord(h->pev[5], ctx->stream));
+    uint64_t c[CTR_N];
+    WG_TRY(agg_read_counters(h, c));
+    float ms = 0.f;
+    WG_HIP(hipEventElapsedTime(&ms, h->pev[4], h->pev[5]));
+    h->pass_ms[3] += ms;
+    h->groups = c[CTR_GROUPS];
+    if (c[CTR_OVF_PART]) {
+      WG_TRY(grow_to(h, next_pow2(std::max<uint64_t>((h->groups + c[CTR_OVF_PART]) * 2, h->cap * 2))));
+      WG_TRY(agg_merge_partials(h, h->ovf_part, c[CTR_OVF_PART]));
+    }
+  }
   return WG_OK;
 }
 

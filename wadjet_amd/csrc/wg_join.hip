@@ -23,6 +23,7 @@
 
 #include "wg_agg_dev.hpp"
 #include "wg_agg_host.hpp"
+#include "wg_join_agg.hpp"
 #include "wg_join_dev.hpp"
 #include "wg_join_host.hpp"
 #include "wg_select.hpp"
@@ -549,6 +550,59 @@ static ColDesc retained_col(const wg_join* j, int c) {
   return ColDesc{j->col_data[c], j->has_valid[c] ? j->col_valid[c] : nullptr, j->types[c], j->scales[c]};
 }
 
+// Does the fused probe -> aggregate have the streamlined kernel's shape
+// (wg_join_agg.hpp)? Fills the parts of Q that describe the query.
+static bool lean_plan(const wg_join* j, const wg_agg* agg, const wg_column* probe_cols, int n_probe,
+                      int probe_key, JoinLeanParams& Q) {
+  if (!j->lean || agg->key.n != 1 || j->stride < 4) return false;
+  // where a joined-row column comes from: a probe column or inline word 2/3
+  auto source = [&](int c, ColDesc& cd, int32_t& word) -> bool {
+    if (c < n_probe) {
+      const wg_column& pc = probe_cols[c];
+      if (pc.validity || pc.type == WG_DECIMAL128) return false;
+      cd = ColDesc{pc.data, nullptr, pc.type, pc.scale};
+      word = -1;
+      return true;
+    }
+    const int b = c - n_probe;
+    if (j->has_valid[b] || j->types[b] == WG_DECIMAL128) return false;
+    for (int i = 0; i < j->n_inline && i < 2; ++i)
+      if (j->inline_cols[i] == b) {
+        cd = ColDesc{};
+        word = 2 + i;
+        return true;
+      }
+    return false;
+  };
+  const int gc = agg->key.col[0];
+  if (!is_int_class(agg->col_types[gc]) || !source(gc, Q.gcol, Q.gword)) return false;
+  Q.pg = agg->pg;
+  std::vector<int> vcols;
+  for (int o = 0; o < Q.pg.n_ops; ++o)
+    for (int32_t* c : {&Q.pg.ops[o].col, &Q.pg.ops[o].col2}) {
+      if (*c < 0) continue;
+      int k = -1;
+      for (size_t i = 0; i < vcols.size(); ++i)
+        if (vcols[i] == *c) k = (int)i;
+      if (k < 0) {
+        k = (int)vcols.size();
+        vcols.push_back(*c);
+      }
+      *c = k;
+    }
+  if ((int)vcols.size() > kLeanMaxVals) return false;
+  Q.n_vals = (int32_t)vcols.size();
+  Q.vfloat = 0;
+  for (int k = 0; k < Q.n_vals; ++k) {
+    if (!source(vcols[k], Q.vcol[k], Q.vword[k])) return false;
+    if (agg->col_types[vcols[k]] == WG_FLOAT64) Q.vfloat |= 1u << k;
+  }
+  const wg_column& pk = probe_cols[probe_key];
+  Q.pkey = ColDesc{pk.data, pk.validity, pk.type, pk.scale};
+  Q.J = jt_of(j);
+  return true;
+}
+
 static wg_status ensure_probe_scratch(wg_join* j, uint64_t n) {
   if (n <= j->scratch_rows) return WG_OK;
   if (j->cnt) WG_HIP(hipFree(j->cnt));
@@ -674,6 +728,19 @@ wg_status wg_join_build_finish(wg_join* j) {
   const uint64_t distinct = c[0] + ((side >> 32) ? 1 : 0);
   j->claimed_unique = distinct == c[1];
   j->finished = true;
+  return WG_OK;
+}
+
+wg_status wg_join_set_tuning(wg_join* j, int32_t lean, int32_t block, int32_t replicas) {
+  clear_error();
+  WG_CHECK(j, WG_ERR_INVALID, "null handle");
+  WG_CHECK(lean == 0 || lean == 1, WG_ERR_INVALID, "lean %d", lean);
+  WG_CHECK(block == 512 || block == 1024, WG_ERR_INVALID, "block %d", block);
+  WG_CHECK(replicas >= 0 && replicas <= kLeanMaxReplicas && (replicas & (replicas - 1)) == 0, WG_ERR_INVALID,
+           "replicas %d", replicas);
+  j->lean = lean;
+  j->lean_block = block;
+  j->lean_replicas = replicas;
   return WG_OK;
 }
 
@@ -907,7 +974,24 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
                            ? std::max(1, std::min(160 * 1024 / std::max(lds_bytes_j, 1), 2048 / agg->block))
                            : std::max(1, 2048 / agg->block);
   const int grid_j = agg->ctx->cus * per_cu_j;
-  WG_TRY(ensure_overflow_for(agg, (uint64_t)grid_j, P.lds_slots));
+  // streamlined kernel, for a GROUP BY declared small: group slots for twice
+  // the expected groups, then as many accumulator replicas as fit the LDS of
+  // the workgroups a CU holds
+  JoinLeanParams Q{};
+  const int per_cu_l = j->lean_block == 512 ? 2 : 1;
+  const size_t lds_budget = (size_t)(160 * 1024 / per_cu_l) - 2048;
+  Q.L = (int32_t)next_pow2(std::max<uint64_t>(32, 2 * agg->group_hint));
+  const bool lean = agg->group_hint > 0 && lean_lds_bytes(Q.L, 1, agg->pg.stride) <= lds_budget &&
+                    lean_plan(j, agg, probe_cols, n_probe_cols, probe_key, Q);
+  int grid_l = 0;
+  if (lean) {
+    int R = j->lean_replicas ? j->lean_replicas : kLeanMaxReplicas;
+    while (R > 1 && lean_lds_bytes(Q.L, R, Q.pg.stride) > lds_budget) R >>= 1;
+    Q.R = R;
+    Q.tile_rows = agg->tile_rows;
+    grid_l = agg->ctx->cus * per_cu_l;
+  }
+  WG_TRY(ensure_overflow_for(agg, (uint64_t)std::max(grid_j, grid_l), std::max<int32_t>(P.lds_slots, lean ? Q.L : 0)));
   agg->st_kernel_ms = 0.f;
   agg->st_launches = 0;
   if (!agg->ev0) {
@@ -915,8 +999,8 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
     WG_HIP(hipEventCreate(&agg->ev1));
   }
   // work list: (sel, n) segments still to process; overflow rows become new segments
-  struct Seg { const uint32_t* sel; uint64_t n; uint32_t* owned; };
-  std::vector<Seg> work{{sel, n, nullptr}};
+  struct Seg { const uint32_t* sel; uint64_t n; uint32_t* owned; bool lean; };
+  std::vector<Seg> work{{sel, n, nullptr, lean}};
   wg_status st = WG_OK;
   for (int guard = 0; !work.empty() && st == WG_OK && guard < 64; ++guard) {
     Seg seg = work.back();
@@ -932,9 +1016,19 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
       P.n = seg.n - pos;
       P.n_tiles = (P.n + P.tile_rows - 1) / P.tile_rows;
       WG_TRY(zero_dev(agg->ctx, agg->counters + CTR_OVF_ROWS, CTR_N - 1));
-      const int grid = (int)std::min<uint64_t>((uint64_t)grid_j, std::max<uint64_t>(P.n_tiles, 1));
+      const int grid = (int)std::min<uint64_t>((uint64_t)(seg.lean ? grid_l : grid_j), std::max<uint64_t>(P.n_tiles, 1));
       WG_HIP(hipEventRecord(agg->ev0, agg->ctx->stream));
-      if (P.lds_slots > 0) {
+      if (seg.lean) {
+        Q.T = P.T;
+        Q.sel = P.sel;
+        Q.row_base = P.row_base;
+        Q.n = P.n;
+        Q.n_tiles = P.n_tiles;
+        Q.ovf_rows = P.ovf_rows;
+        Q.ovf_rows_cap = P.ovf_rows_cap;
+        Q.ovf_part = P.ovf_part;
+        WG_TRY(launch_join_agg_lean(Q, grid, j->lean_block, agg->ctx->stream));
+      } else if (P.lds_slots > 0) {
         static bool attr_done = false;
         if (!attr_done) {
           WG_HIP(hipFuncSetAttribute((const void*)join_agg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -959,7 +1053,10 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
       const uint64_t tiles = std::min<uint64_t>(c[CTR_TILE], P.n_tiles);
       if (c[CTR_OVF_ROWS] || c[CTR_OVF_PART]) {
         const uint64_t nr = c[CTR_OVF_ROWS], np = c[CTR_OVF_PART];
-        WG_TRY(grow_to(agg, next_pow2(std::max<uint64_t>((agg->groups + nr * 4 + np) * 2, agg->cap * 2))));
+        // rows the streamlined kernel leaves to the general one need no
+        // room: grow only when a partial found the table past its budget
+        if (!seg.lean || np)
+          WG_TRY(grow_to(agg, next_pow2(std::max<uint64_t>((agg->groups + nr * 4 + np) * 2, agg->cap * 2))));
         if (np) {
           uint64_t* recs = nullptr;
           const uint64_t bytes = np * (uint64_t)agg->pg.stride * 8;
@@ -981,7 +1078,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
           WG_HIP(hipMalloc(&rows, nr * 4));
           WG_HIP(hipMemcpyAsync(rows, agg->ovf_rows, nr * 4, hipMemcpyDeviceToDevice, agg->ctx->stream));
           WG_HIP(hipStreamSynchronize(agg->ctx->stream));
-          work.push_back(Seg{rows, nr, rows});
+          work.push_back(Seg{rows, nr, rows, false});  // the general kernel takes every row shape
         }
       }
       WG_CHECK(tiles > 0 || c[CTR_OVF_ROWS] > 0, WG_ERR_DEVICE, "join probe launch made no progress");

@@ -24,6 +24,10 @@ struct wg_join {
   uint64_t* counter = nullptr;  // claimed slots + scratch words
   bool finished = false;
   bool claimed_unique = false;  // every non-NULL build key distinct (set at finish)
+  // fused probe -> aggregate: streamlined kernel when the query has its shape
+  int32_t lean = 1;           // 0: always the general kernel
+  int32_t lean_block = 512;   // 512 or 1024 threads
+  int32_t lean_replicas = 0;  // accumulator replicas (0: as many as the LDS budget allows)
   // probe scratch
   uint64_t* cnt = nullptr;
   uint64_t* off = nullptr;

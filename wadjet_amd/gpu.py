@@ -246,6 +246,10 @@ class HashAggregate:
                      part_chunk_rows: int = 1 << 16):
         call("wg_agg_set_strategy", self.h, strategy, part_lds_slots, part_block, part_chunk_rows)
 
+    def set_heavy_hitters(self, hot_slots: int):
+        """Heavy-hitter slots of the partitioned consume (0: none)."""
+        call("wg_agg_set_heavy_hitters", self.h, hot_slots)
+
     def pass_times(self):
         a = (C.c_float * 4)()
         call("wg_agg_pass_times", self.h, a)
@@ -352,6 +356,10 @@ class HashJoin:
     def probe_agg(self, probe_cols: Sequence[Col], probe_key: int, n_rows: int, agg: HashAggregate, sel=None):
         sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
         call("wg_join_probe_agg", self.h, columns(probe_cols), len(probe_cols), probe_key, n_rows, sp, ns, agg.h)
+
+    def set_tuning(self, lean: bool = True, block: int = 512, replicas: int = 0):
+        """Kernel choice for probe_agg (results do not depend on it)."""
+        call("wg_join_set_tuning", self.h, 1 if lean else 0, block, replicas)
 
     def semi(self, key: Col, n_rows: int, out: DeviceBuffer, anti: bool = False, sel=None) -> int:
         """Semi (anti=False) / anti join: positions written to `out`, returns the count."""
