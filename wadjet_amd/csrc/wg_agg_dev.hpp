@@ -328,25 +328,46 @@ __device__ __forceinline__ uint64_t key_hash(uint64_t key) { return mix64(key); 
 // as overflow). Probe reads are workgroup-scope loads: a stale EMPTY is
 // resolved by the CAS, and a slot never changes once it holds a key.
 // No shared counter is touched here: callers sum `claimed` per workgroup.
-__device__ __forceinline__ uint64_t* glb_find_or_claim(const GlobalTable& T, int stride, uint64_t key, uint64_t h,
-                                                      uint32_t& claimed) {
-  uint64_t idx = h & T.mask;
+// Linear probing from slot `idx`, kProbeWindow slots per round trip: the key
+// words of the window are loaded together, then the first slot that holds
+// the key, or is free and is won by the CAS, is the answer (a CAS lost to
+// another key moves on to the next slot of the window). At the table's load
+// budget most probe runs end inside the first window, so a claim costs one
+// load round trip and one CAS instead of one round trip per slot.
+constexpr int kProbeWindow = 4;
+
+__device__ __forceinline__ uint64_t* glb_probe_from(const GlobalTable& T, int stride, uint64_t key, uint64_t idx,
+                                                   uint64_t first_p, bool& fresh) {
   const uint64_t lim = T.max_probe < T.mask + 1 ? T.max_probe : T.mask + 1;
-  for (uint64_t p = 0; p < lim; ++p) {
-    uint64_t* s = T.slots + idx * (uint64_t)stride;
-    const uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (k == key) return s;
-    if (k == kEmpty) {
-      const uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
-      if (prev == kEmpty) {
-        ++claimed;
-        return s;
+  for (uint64_t p = first_p; p < lim; p += kProbeWindow) {
+    uint64_t k[kProbeWindow];
+#pragma unroll
+    for (int j = 0; j < kProbeWindow; ++j)
+      k[j] = __hip_atomic_load(T.slots + ((idx + j) & T.mask) * (uint64_t)stride, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int j = 0; j < kProbeWindow && p + j < lim; ++j) {
+      uint64_t* s = T.slots + ((idx + j) & T.mask) * (uint64_t)stride;
+      if (k[j] == key) return s;
+      if (k[j] == kEmpty) {
+        const uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
+        if (prev == kEmpty) {
+          fresh = true;
+          return s;
+        }
+        if (prev == key) return s;
       }
-      if (prev == key) return s;
     }
-    idx = (idx + 1) & T.mask;
+    idx = (idx + kProbeWindow) & T.mask;
   }
   return nullptr;
+}
+
+__device__ __forceinline__ uint64_t* glb_find_or_claim(const GlobalTable& T, int stride, uint64_t key, uint64_t h,
+                                                      uint32_t& claimed) {
+  bool fresh = false;
+  uint64_t* s = glb_probe_from(T, stride, key, h & T.mask, 0, fresh);
+  claimed += fresh ? 1u : 0u;
+  return s;
 }
 
 // Sum a per-thread count over the workgroup and add it to *dst with one

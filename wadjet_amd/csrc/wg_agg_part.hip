@@ -413,23 +413,6 @@ struct PartAggParams {
   uint32_t vfloat;  // bit j: value word j holds float64 bits (else an int64)
 };
 
-// Claim-or-find with the CAS as the probe: one round trip per probed slot.
-__device__ __forceinline__ uint64_t* glb_claim_from(const GlobalTable& T, int stride, uint64_t key, uint64_t idx,
-                                                    uint64_t first_p, bool& fresh) {
-  const uint64_t lim = T.max_probe < T.mask + 1 ? T.max_probe : T.mask + 1;
-  for (uint64_t p = first_p; p < lim; ++p) {
-    uint64_t* s = T.slots + idx * (uint64_t)stride;
-    const uint64_t prev = at_cas<GlbT>(s, kEmpty, key);
-    if (prev == kEmpty) {
-      fresh = true;
-      return s;
-    }
-    if (prev == key) return s;
-    idx = (idx + 1) & T.mask;
-  }
-  return nullptr;
-}
-
 constexpr int kFlushU = 2;
 
 template <int RW>
@@ -565,7 +548,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
         bool fresh = prev[u] == kEmpty;
         uint64_t* g = (fresh || prev[u] == key[u])
                           ? P.T.slots + home[u] * (uint64_t)stride
-                          : glb_claim_from(P.T, stride, key[u], (home[u] + 1) & P.T.mask, 1, fresh);
+                          : glb_probe_from(P.T, stride, key[u], (home[u] + 1) & P.T.mask, 1, fresh);
         if (!g) {
           const uint64_t at =
               __hip_atomic_fetch_add(P.T.counters + CTR_OVF_PART, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
