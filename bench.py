@@ -33,13 +33,17 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of the consume kernels (see the file's header)
-AGG_PMC_SUMMARY = ROOT / "profiles" / "agg_cfg2_100M_pmc.txt"
-JOIN_PMC_SUMMARY = ROOT / "profiles" / "join_cfg3_200M_pmc.txt"
+AGG_PMC_SUMMARY = ROOT / "profiles" / "groupby_zipf_100M_pmc.txt"
+JOIN_PMC_SUMMARY = ROOT / "profiles" / "join_probe_agg_200M_pmc.txt"
 
 
 def is_consume_kernel(name):
     return any(s in name for s in ("part_count", "part_scatter", "part_agg", "part_bounds", "rocprim",
-                                   "agg_consume_kernel"))
+                                   "agg_consume_kernel", "hot_"))
+
+
+def is_join_agg_kernel(name):
+    return "join_agg" in name
 
 
 def log(msg):
@@ -51,22 +55,22 @@ def pct(xs, p):
     return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
 
 
-# ------------------------------------------------------------------- config 2
+# ----------------------------------------------- Zipf composite GROUP BY, 100M rows
 
 def bench_agg(ctx, args):
     from wadjet_amd import workloads as W
     from wadjet_amd.gpu import HashAggregate
     n = args.rows
     log(f"[agg] generating {n} rows on device")
-    cols = W.gen_config2(ctx, n)
-    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES,
+    cols = W.gen_zipf_groupby(ctx, n)
+    agg = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES,
                         group_hint=args.group_hint)
     if args.strategy >= 0:
         agg.set_strategy(args.strategy)
 
     def step():
         agg.reset()
-        agg.consume(cols, n, pred=W.CFG2_PRED)
+        agg.consume(cols, n, pred=W.ZIPF_PRED)
         agg.finalize()
 
     for i in range(args.warmup):
@@ -83,7 +87,7 @@ def bench_agg(ctx, args):
     groups = agg.num_groups()
     ms_step = wall * 1000 / args.steps
     k_ms = sum(kern_ms) / len(kern_ms)
-    alg = W.CFG2_BYTES_PER_ROW * n + W.CFG2_BYTES_PER_GROUP * groups
+    alg = W.ZIPF_BYTES_PER_ROW * n + W.ZIPF_BYTES_PER_GROUP * groups
     log(f"[agg] {ms_step:.3f} ms/step, consume kernels {k_ms:.3f} ms, groups {groups}; "
         f"passes (direct, count+scan, scatter, partition-agg) {passes[-1]}")
     return dict(cols=cols, n=n, groups=groups, ms_step=ms_step, kernel_ms=k_ms, alg_bytes=alg,
@@ -98,11 +102,11 @@ def cpu_agg(cols, n, sample_rows, threads):
     m = min(n, sample_rows)
     host = [(cols[0].data.to_host(np.int32, m), O.INT32), (cols[1].data.to_host(np.int32, m), O.INT32),
             (cols[2].data.to_host(np.int64, m), O.DECIMAL, None, 2), (cols[3].data.to_host(np.float64, m), O.FLOAT64)]
-    aggs = [(a.func, a.col) for a in W.CFG2_AGGS]
+    aggs = [(a.func, a.col) for a in W.ZIPF_AGGS]
     best = None
     for rep in range(2):
         t0 = time.perf_counter()
-        r = O.hash_agg(host, W.CFG2_KEYS, aggs, pred=(3, O.GT, 0, -0.5), threads=threads, max_groups=m)
+        r = O.hash_agg(host, W.ZIPF_KEYS, aggs, pred=(3, O.GT, 0, -0.5), threads=threads, max_groups=m)
         dt = time.perf_counter() - t0
         best = dt if best is None else min(best, dt)
         log(f"[cpu] rep {rep}: {m} rows, {r['n']} groups, {dt:.2f} s on {threads} threads")
@@ -112,22 +116,22 @@ def cpu_agg(cols, n, sample_rows, threads):
                        f"(the CloneSink/MergeSink shape); generation excluded, merge included")
 
 
-# ------------------------------------------------------------------- config 3
+# ------------------------------------- join 10M build x 200M probe, then GROUP BY
 
 def bench_join(ctx, args):
     from wadjet_amd import _lib as L
     from wadjet_amd import workloads as W
     from wadjet_amd.gpu import AggSpec, HashAggregate, HashJoin
-    cfg = W.Cfg3(n_build=args.join_build, n_probe=args.join_probe)
+    cfg = W.FkJoin(n_build=args.join_build, n_probe=args.join_probe)
     log(f"[join] generating build {cfg.n_build} / probe {cfg.n_probe}")
-    bcols = W.gen_config3_build(ctx, cfg)
-    pcols = W.gen_config3_probe(ctx, cfg)
-    j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    bcols = W.gen_fk_join_build(ctx, cfg)
+    pcols = W.gen_fk_join_probe(ctx, cfg)
+    j = HashJoin(ctx, W.FKJ_BUILD_TYPES, 0, inline=[1, 2], scales=W.FKJ_BUILD_SCALES, build_hint=cfg.n_build)
     ctx.timer_start()
     j.build(bcols, cfg.n_build)
     j.finish()
     build_ms = ctx.timer_stop()
-    types = W.CFG3_PROBE_TYPES + W.CFG3_BUILD_TYPES
+    types = W.FKJ_PROBE_TYPES + W.FKJ_BUILD_TYPES
     aggs = [AggSpec(L.SUM, 1, 4), AggSpec(L.COUNT, -1)]
     agg = HashAggregate(ctx, types, [3], aggs, scales=[0, 0, 0, 0, 2], group_hint=64)
 
@@ -148,28 +152,28 @@ def bench_join(ctx, args):
     ctx.sync()
     ms = (time.perf_counter() - t0) * 1000 / steps
     k = sum(kms) / len(kms)
-    alg = W.CFG3_BYTES_PER_PROBE_ROW * cfg.n_probe
+    alg = W.FKJ_BYTES_PER_PROBE_ROW * cfg.n_probe
     from wadjet_amd.profiling import load_traffic
-    traffic = load_traffic(str(JOIN_PMC_SUMMARY), lambda k: "join_agg_kernel" in k)
+    traffic = load_traffic(str(JOIN_PMC_SUMMARY), is_join_agg_kernel)
     log(f"[join] build {build_ms:.2f} ms, probe+agg {ms:.3f} ms/step (kernel {k:.3f} ms), groups {agg.num_groups()}")
     return dict(rows_per_s=cfg.n_probe / (ms / 1000), ms_per_step=ms, kernel_ms=k, build_ms=build_ms,
                 build_rows_per_s=cfg.n_build / (build_ms / 1000), groups=agg.num_groups(),
                 roofline=dict(bound="hbm", achieved=alg / (k / 1000) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=alg / (k / 1000) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
-                              kernel="join_agg_kernel", traffic=traffic,
+                              kernel="join_agg_lean_kernel (fused probe -> GROUP BY)", traffic=traffic,
                               traffic_source=JOIN_PMC_SUMMARY.name if traffic else None),
                 _keep=(j, agg, bcols, pcols))
 
 
-# ------------------------------------------------------------------- config 4
+# -------------------------------------------- 256 concurrent queries, one launch
 
 def bench_batched(ctx, args):
     import numpy as np
 
     from wadjet_amd import workloads as W
     from wadjet_amd.gpu import Batch
-    plan = W.plan_config4(n_queries=args.batch_queries, seed=4)
-    data = W.gen_config4(ctx, plan)
+    plan = W.plan_query_mix(n_queries=args.batch_queries, seed=4)
+    data = W.gen_query_mix(ctx, plan)
     total_rows = sum(q["rows"] for q in plan)
     qarr = Batch.queries(data)
     b = Batch(ctx, args.batch_queries, sum(q["groups"] for q in plan))
@@ -210,16 +214,16 @@ def bench_batched(ctx, args):
                 tuning=tuning, lds_queries=n_lds, autotune_s=tune_s)
 
 
-# ------------------------------------------------------------------- config 5
+# ------------------------------------------------ served flow table, Poisson load
 
 def bench_served(ctx, args):
     from wadjet_amd import workloads as W
     from wadjet_amd.gpu import HashJoin, Server
     n = args.serve_rows
     log(f"[served] generating a {n}-row flow table")
-    cols = W.gen_config5(ctx, n)
-    dim = W.gen_config5_dim(ctx, 50_000)
-    j = HashJoin(ctx, W.CFG5_DIM_TYPES, 0, inline=[1], build_hint=50_000)
+    cols = W.gen_flow_table(ctx, n)
+    dim = W.gen_flow_dim(ctx, 50_000)
+    j = HashJoin(ctx, W.FLOW_DIM_TYPES, 0, inline=[1], build_hint=50_000)
     j.build(dim, 50_000)
     j.finish()
     served = Server(ctx, cols, n, 3, lanes=3, hbm_budget=8 << 30, max_batch=64)
@@ -229,7 +233,7 @@ def bench_served(ctx, args):
     out = []
     for qps in args.qps:
         n_req = max(50, min(args.serve_requests, int(qps * args.serve_seconds)))
-        qs, shapes = W.plan_config5(n, n_req, seed=int(qps))
+        qs, shapes = W.plan_flow_requests(n, n_req, seed=int(qps))
         res = {"qps_offered": qps, "requests": n_req}
         for name, s in (("served", served), ("plain", plain)):
             s.reset()

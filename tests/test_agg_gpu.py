@@ -69,56 +69,56 @@ def check(ctx, cols, n, types, keys, aggs, scales=None, nullable=None, pred=None
     return agg
 
 
-def test_config1_shape(ctx):
+def test_kv1k_shape(ctx):
     """k int64 over 1000 values: SUM(v), COUNT(*), MIN(v), MAX(v) GROUP BY k."""
-    cols = W.gen_config1(ctx, 1_000_000)
-    agg = check(ctx, cols, 1_000_000, W.CFG1_TYPES, [0], W.CFG1_AGGS)
+    cols = W.gen_kv1k(ctx, 1_000_000)
+    agg = check(ctx, cols, 1_000_000, W.KV1K_TYPES, [0], W.KV1K_AGGS)
     assert agg.num_groups() == 1000
 
 
 @pytest.mark.parametrize("tuning", [None, (0, 256), (256, 1024), (2048, 512)])
-def test_config2_shape_small(ctx, tuning):
+def test_zipf_groupby_small(ctx, tuning):
     """Composite Zipf key, Int128 DECIMAL sum, float min/max/avg, fused filter."""
     n = 3_000_000
-    cols = W.gen_config2(ctx, n)
-    check(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED, tuning=tuning)
+    cols = W.gen_zipf_groupby(ctx, n)
+    check(ctx, cols, n, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_SCALES, pred=W.ZIPF_PRED, tuning=tuning)
 
 
 @pytest.mark.parametrize("chunk,hot", [(1 << 16, 512), (2048, 512), (1 << 16, 0), (2048, 0), (1 << 16, 2048)])
-def test_config2_partitioned(ctx, chunk, hot):
+def test_zipf_groupby_partitioned(ctx, chunk, hot):
     """The partitioned consume (count, scatter, per-partition LDS aggregation);
     a small chunk splits the large partitions over several workgroups; the
     heavy hitters of the Zipf key aggregate in the scatter pass (hot > 0)."""
     n = 3_000_000
-    cols = W.gen_config2(ctx, n, seed=31)
+    cols = W.gen_zipf_groupby(ctx, n, seed=31)
     print(f"[agg] partitioned chunk={chunk} hot={hot}", flush=True)
-    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES, group_hint=300_000)
+    agg = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=300_000)
     agg.set_strategy(2, 2048, 1024, chunk)
     agg.set_heavy_hitters(hot)
-    agg.consume(cols, n, pred=W.CFG2_PRED)
+    agg.consume(cols, n, pred=W.ZIPF_PRED)
     t = agg.pass_times()
     assert t[0] == 0 and t[2] > 0, t  # the partitioned passes ran
     agg.finalize()
     hcols = host_cols(cols, n)
-    ref = oracle_groups(run_oracle(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
-    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in W.CFG2_AGGS], W.CFG2_TYPES, W.CFG2_SCALES,
-            abs_sums(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
+    ref = oracle_groups(run_oracle(hcols, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_PRED))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in W.ZIPF_AGGS], W.ZIPF_TYPES, W.ZIPF_SCALES,
+            abs_sums(hcols, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_PRED))
 
 
 def test_partitioned_overflow_grows_and_replays(ctx):
     """Partitioned consume into a table far too small: items stop at the
     overflow headroom, the table grows, overflowed positions replay."""
     n = 2_000_000
-    cols = W.gen_config2(ctx, n, seed=37)
-    agg = HashAggregate(ctx, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, scales=W.CFG2_SCALES, group_hint=1)
+    cols = W.gen_zipf_groupby(ctx, n, seed=37)
+    agg = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=1)
     agg.set_strategy(2, 1024, 512, 4096)
-    agg.consume(cols, n, pred=W.CFG2_PRED)
+    agg.consume(cols, n, pred=W.ZIPF_PRED)
     agg.finalize()
     assert agg.stats()["capacity"] > 100_000
     hcols = host_cols(cols, n)
-    ref = oracle_groups(run_oracle(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
-    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in W.CFG2_AGGS], W.CFG2_TYPES, W.CFG2_SCALES,
-            abs_sums(hcols, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_PRED))
+    ref = oracle_groups(run_oracle(hcols, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_PRED))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in W.ZIPF_AGGS], W.ZIPF_TYPES, W.ZIPF_SCALES,
+            abs_sums(hcols, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_PRED))
 
 
 @pytest.mark.parametrize("hot", [512, 64, 0])
@@ -155,8 +155,8 @@ def test_partitioned_two_consumes_and_sentinel_keys(ctx, hot):
 def test_table_growth_from_tiny_hint(ctx):
     """Start from the minimum table and let overflow + grow + replay do the work."""
     n = 2_000_000
-    cols = W.gen_config2(ctx, n, seed=7)
-    agg = check(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED, hint=1)
+    cols = W.gen_zipf_groupby(ctx, n, seed=7)
+    agg = check(ctx, cols, n, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_SCALES, pred=W.ZIPF_PRED, hint=1)
     assert agg.stats()["capacity"] > 1024
 
 
@@ -164,14 +164,14 @@ def test_selection_vector_matches_predicate(ctx):
     """A device Sel from the filter kernel gives the same groups as the fused predicate."""
     from wadjet_amd.gpu import filter_sel
     n = 1_000_000
-    cols = W.gen_config2(ctx, n, seed=11)
+    cols = W.gen_zipf_groupby(ctx, n, seed=11)
     sel, k = filter_sel(ctx, cols[3], n, L.GT, f64=-0.5)
-    a1 = run_gpu(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, sel=(sel, k))
-    a2 = run_gpu(ctx, cols, n, W.CFG2_TYPES, W.CFG2_KEYS, W.CFG2_AGGS, W.CFG2_SCALES, pred=W.CFG2_PRED)
+    a1 = run_gpu(ctx, cols, n, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_SCALES, sel=(sel, k))
+    a2 = run_gpu(ctx, cols, n, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_SCALES, pred=W.ZIPF_PRED)
     g1, g2 = gpu_groups(a1), gpu_groups(a2)
     assert g1["n"] == g2["n"]
     np.testing.assert_array_equal(g1["keys"], g2["keys"])
-    for a in range(len(W.CFG2_AGGS)):
+    for a in range(len(W.ZIPF_AGGS)):
         if a == 4:
             np.testing.assert_allclose(g1["values"][a], g2["values"][a], rtol=1e-12)
         else:
@@ -219,7 +219,7 @@ def test_sentinel_key_and_extremes(ctx):
     k = np.array([-0x7FFFFFFFFFFFFFFF, 0, -2**63, 2**63 - 1, -0x7FFFFFFFFFFFFFFF, 5] * 1000, dtype=np.int64)
     v = np.arange(len(k), dtype=np.int64) * 7 - 3000
     cols = [Col(ctx.to_device(k), L.INT64), Col(ctx.to_device(v), L.INT64)]
-    agg = check(ctx, cols, len(k), [L.INT64, L.INT64], [0], W.CFG1_AGGS)
+    agg = check(ctx, cols, len(k), [L.INT64, L.INT64], [0], W.KV1K_AGGS)
     assert agg.num_groups() == 5
 
 
@@ -232,14 +232,14 @@ def test_host_memory_morsels_and_merge(ctx):
     b = rng.integers(-3, 3, n).astype(np.int32)
     p = rng.integers(100, 10**7, n).astype(np.int64)
     f = rng.standard_normal(n)
-    types = W.CFG2_TYPES
-    aggs = W.CFG2_AGGS
-    s1 = HashAggregate(ctx, types, W.CFG2_KEYS, aggs, scales=W.CFG2_SCALES)
-    s2 = HashAggregate(ctx, types, W.CFG2_KEYS, aggs, scales=W.CFG2_SCALES)
+    types = W.ZIPF_TYPES
+    aggs = W.ZIPF_AGGS
+    s1 = HashAggregate(ctx, types, W.ZIPF_KEYS, aggs, scales=W.ZIPF_SCALES)
+    s2 = HashAggregate(ctx, types, W.ZIPF_KEYS, aggs, scales=W.ZIPF_SCALES)
     all_sel = []
     for m, start in enumerate(range(0, n, 2048)):
         end = min(n, start + 2048)
-        cols = [Col(x[start:end], t, None, sc) for x, t, sc in zip((a, b, p, f), types, W.CFG2_SCALES)]
+        cols = [Col(x[start:end], t, None, sc) for x, t, sc in zip((a, b, p, f), types, W.ZIPF_SCALES)]
         sel = np.nonzero(f[start:end] > -0.5)[0].astype(np.uint32)
         all_sel.append(sel + start)
         (s1 if m % 2 == 0 else s2).consume(cols, end - start, sel=sel, device=False)
@@ -247,9 +247,9 @@ def test_host_memory_morsels_and_merge(ctx):
     s1.finalize()
     hcols = [(a, L.INT32), (b, L.INT32), (p, L.DECIMAL, None, 2), (f, L.FLOAT64)]
     sel = np.concatenate(all_sel)
-    ref = oracle_groups(run_oracle(hcols, W.CFG2_KEYS, aggs, sel=sel))
-    compare(gpu_groups(s1), ref, [(x.func, x.col) for x in aggs], types, W.CFG2_SCALES,
-            abs_sums(hcols, W.CFG2_KEYS, aggs, sel=sel))
+    ref = oracle_groups(run_oracle(hcols, W.ZIPF_KEYS, aggs, sel=sel))
+    compare(gpu_groups(s1), ref, [(x.func, x.col) for x in aggs], types, W.ZIPF_SCALES,
+            abs_sums(hcols, W.ZIPF_KEYS, aggs, sel=sel))
 
 
 def test_multi_key_null_is_rejected(ctx):
@@ -262,8 +262,8 @@ def test_multi_key_null_is_rejected(ctx):
 
 
 def test_empty_input_and_reset(ctx):
-    agg = HashAggregate(ctx, W.CFG1_TYPES, [0], W.CFG1_AGGS)
-    cols = W.gen_config1(ctx, 10)
+    agg = HashAggregate(ctx, W.KV1K_TYPES, [0], W.KV1K_AGGS)
+    cols = W.gen_kv1k(ctx, 10)
     agg.consume(cols, 0)
     agg.finalize()
     assert agg.num_groups() == 0

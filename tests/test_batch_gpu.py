@@ -35,8 +35,8 @@ def same(a, b, q):
 
 
 def test_batched_equals_plain_equals_oracle(ctx):
-    plan = W.plan_config4(n_queries=40, seed=5, max_rows=300_000, max_groups=20_000)
-    data = W.gen_config4(ctx, plan)
+    plan = W.plan_query_mix(n_queries=40, seed=5, max_rows=300_000, max_groups=20_000)
+    data = W.gen_query_mix(ctx, plan)
     qarr = Batch.queries(data)
     b = Batch(ctx, 64, sum(q["groups"] for q in plan))
     print(f"[batch] {len(plan)} queries, {sum(q['rows'] for q in plan)} rows", flush=True)
@@ -61,16 +61,16 @@ def test_graph_replay_after_the_work_queue_grows(ctx):
     reallocated: small batch, then a batch with many more tiles, both graphed."""
     b = Batch(ctx, 96, 200_000)
     for n_q, seed in ((4, 11), (80, 12), (8, 13)):
-        plan = W.plan_config4(n_queries=n_q, seed=seed, max_rows=200_000, max_groups=2_000)
-        data = W.gen_config4(ctx, plan)
+        plan = W.plan_query_mix(n_queries=n_q, seed=seed, max_rows=200_000, max_groups=2_000)
+        data = W.gen_query_mix(ctx, plan)
         b.run(Batch.queries(data), len(plan), use_graph=True)
         for q, (kc, vc, n, g) in enumerate(data):
             same(b.result(q), oracle_query(kc, vc, n), q)
 
 
 def test_autotune_keeps_results(ctx):
-    plan = W.plan_config4(n_queries=24, seed=6, max_rows=200_000, max_groups=50_000)
-    data = W.gen_config4(ctx, plan)
+    plan = W.plan_query_mix(n_queries=24, seed=6, max_rows=200_000, max_groups=50_000)
+    data = W.gen_query_mix(ctx, plan)
     qarr = Batch.queries(data)
     b = Batch(ctx, 32, sum(q["groups"] for q in plan))
     b.autotune(qarr, len(plan))
@@ -86,8 +86,8 @@ def test_autotune_keeps_results(ctx):
 
 
 def test_underestimated_cardinality_falls_back_exactly(ctx):
-    plan = W.plan_config4(n_queries=6, seed=8, max_rows=100_000, min_groups=5000, max_groups=20_000)
-    data = W.gen_config4(ctx, plan)
+    plan = W.plan_query_mix(n_queries=6, seed=8, max_rows=100_000, min_groups=5000, max_groups=20_000)
+    data = W.gen_query_mix(ctx, plan)
     lying = [(k, v, n, 10) for (k, v, n, g) in data]  # estimate 10 groups
     b = Batch(ctx, 8, 1000)
     b.run(Batch.queries(lying), len(lying))

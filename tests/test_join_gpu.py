@@ -68,14 +68,14 @@ def test_probe_multiset_with_duplicates_and_nulls(ctx, cap):
     assert calls >= len(want_p) // cap
 
 
-def test_config3_shape_join_multiset_and_digest(ctx):
+def test_fk_join_multiset_and_digest(ctx):
     """Unique permutation keys, 95 % matching foreign keys: pair multiset,
     device digest == oracle digest, matched count closed form."""
-    cfg = W.Cfg3(n_build=200_000, n_probe=1_000_000)
-    print(f"[join] config3 shape {cfg}", flush=True)
-    bcols = W.gen_config3_build(ctx, cfg)
-    pcols = W.gen_config3_probe(ctx, cfg)
-    j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    cfg = W.FkJoin(n_build=200_000, n_probe=1_000_000)
+    print(f"[join] fk-join shape {cfg}", flush=True)
+    bcols = W.gen_fk_join_build(ctx, cfg)
+    pcols = W.gen_fk_join_probe(ctx, cfg)
+    j = HashJoin(ctx, W.FKJ_BUILD_TYPES, 0, inline=[1, 2], scales=W.FKJ_BUILD_SCALES, build_hint=cfg.n_build)
     j.build(bcols, cfg.n_build)
     j.finish()
     bk = bcols[0].data.to_host(np.int64)
@@ -101,19 +101,19 @@ def test_config3_shape_join_multiset_and_digest(ctx):
 
 
 @pytest.mark.parametrize("lds,lean", [(1024, True), (1024, False), (0, False)])
-def test_config3_fused_probe_aggregate(ctx, lds, lean):
+def test_fk_join_fused_probe_aggregate(ctx, lds, lean):
     """JOIN fk = key GROUP BY p1 SUM(qty * p2) (DECIMAL product -> Int128),
     COUNT(*), MIN(qty), AVG(p2): fused probe vs oracle over the pair list,
     through the streamlined kernel and the general one."""
-    cfg = W.Cfg3(n_build=300_000, n_probe=2_000_000, seed=17)
+    cfg = W.FkJoin(n_build=300_000, n_probe=2_000_000, seed=17)
     print(f"[join] fused probe-agg lds={lds} lean={lean} {cfg}", flush=True)
-    bcols = W.gen_config3_build(ctx, cfg)
-    pcols = W.gen_config3_probe(ctx, cfg)
-    j = HashJoin(ctx, W.CFG3_BUILD_TYPES, 0, inline=[1, 2], scales=W.CFG3_BUILD_SCALES, build_hint=cfg.n_build)
+    bcols = W.gen_fk_join_build(ctx, cfg)
+    pcols = W.gen_fk_join_probe(ctx, cfg)
+    j = HashJoin(ctx, W.FKJ_BUILD_TYPES, 0, inline=[1, 2], scales=W.FKJ_BUILD_SCALES, build_hint=cfg.n_build)
     j.set_tuning(lean=lean)
     j.build(bcols, cfg.n_build)
     j.finish()
-    types = W.CFG3_PROBE_TYPES + W.CFG3_BUILD_TYPES  # fk, qty, key, p1, p2
+    types = W.FKJ_PROBE_TYPES + W.FKJ_BUILD_TYPES  # fk, qty, key, p1, p2
     scales = [0, 0, 0, 0, 2]
     aggs = [AggSpec(L.SUM, 1, 4), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 1), AggSpec(L.AVG, 4)]
     agg = HashAggregate(ctx, types, [3], aggs, scales=scales, group_hint=64)

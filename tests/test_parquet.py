@@ -13,21 +13,21 @@ pa = pytest.importorskip("pyarrow")
 pq = pytest.importorskip("pyarrow.parquet")
 
 # SHA-256 of the generated (k, v) columns, little-endian: pins the generator
-CFG1_DIGEST = "18c4e25125938c227192433c92a64a3de69fe8cb9fa84418dc9bae8aeb40fc3e"
+KV1K_DIGEST = "18c4e25125938c227192433c92a64a3de69fe8cb9fa84418dc9bae8aeb40fc3e"
 
 
-def test_config1_generator_is_pinned():
-    k, v = Q.gen_config1_host()
-    assert Q.columns_digest([k, v]) == CFG1_DIGEST
+def test_kv1k_generator_is_pinned():
+    k, v = Q.gen_kv1k_host()
+    assert Q.columns_digest([k, v]) == KV1K_DIGEST
     assert len(k) == 1_000_000 and k.dtype == np.int64 and v.dtype == np.int64
     assert len(np.unique(k)) == 1000
     assert v.min() >= 0 and v.max() < 1_000_000
 
 
 def test_round_trip_in_batches(tmp_path):
-    path = str(tmp_path / "cfg1.parquet")
-    Q.write_config1(path, n=300_000, row_group_rows=65_536)
-    k, v = Q.gen_config1_host(300_000)
+    path = str(tmp_path / "kv1k.parquet")
+    Q.write_kv1k(path, n=300_000, row_group_rows=65_536)
+    k, v = Q.gen_kv1k_host(300_000)
     got_k, got_v, rows = [], [], 0
     for n, batch in Q.iter_parquet(path, ["k", "v"], batch_rows=50_000):
         (kv, kt, ks, kval), (vv, vt, vs, vval) = batch
@@ -40,11 +40,11 @@ def test_round_trip_in_batches(tmp_path):
     np.testing.assert_array_equal(np.concatenate(got_v), v)
 
 
-def test_oracle_on_config1_matches_pyarrow_group_by(tmp_path):
+def test_oracle_on_kv1k_matches_pyarrow_group_by(tmp_path):
     """SELECT k, SUM(v), COUNT(*), MIN(v), MAX(v) GROUP BY k: the oracle over
     the decoded file equals pyarrow's hash aggregation of the same file."""
-    path = str(tmp_path / "cfg1.parquet")
-    Q.write_config1(path)
+    path = str(tmp_path / "kv1k.parquet")
+    Q.write_kv1k(path)
     cols = []
     for n, batch in Q.iter_parquet(path, ["k", "v"], batch_rows=1 << 20):
         cols.append(batch)

@@ -26,15 +26,15 @@ def decoded(path, names):
     return out
 
 
-def test_config1_read_parquet_aggregate(ctx, tmp_path):
-    path = str(tmp_path / "cfg1.parquet")
-    Q.write_config1(path)
-    print("[parquet] config-1 file, 1M rows, 256K-row batches", flush=True)
+def test_kv1k_read_parquet_aggregate(ctx, tmp_path):
+    path = str(tmp_path / "kv1k.parquet")
+    Q.write_kv1k(path)
+    print("[parquet] k-over-1000 file, 1M rows, 256K-row batches", flush=True)
     aggs = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 1), AggSpec(L.MAX, 1)]
     agg = HashAggregate(ctx, [L.INT64, L.INT64], [0], aggs)
     assert Q.consume_parquet(agg, path, ["k", "v"], batch_rows=1 << 18) == 1_000_000
     agg.finalize()
-    k, v = Q.gen_config1_host()
+    k, v = Q.gen_kv1k_host()
     ref = oracle_groups(O.hash_agg([(k, O.INT64), (v, O.INT64)], [0], [(a.func, a.col) for a in aggs]))
     compare(gpu_groups(agg), ref, [(a.func, a.col) for a in aggs], [L.INT64, L.INT64])
     assert agg.num_groups() == 1000

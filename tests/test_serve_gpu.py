@@ -42,12 +42,12 @@ def same(a, b, i):
 
 def test_served_equals_alone_equals_oracle(ctx):
     n = 6_000_000
-    cols = W.gen_config5(ctx, n)
-    dim = W.gen_config5_dim(ctx, 50_000)
-    j = HashJoin(ctx, W.CFG5_DIM_TYPES, 0, inline=[1], build_hint=50_000)
+    cols = W.gen_flow_table(ctx, n)
+    dim = W.gen_flow_dim(ctx, 50_000)
+    j = HashJoin(ctx, W.FLOW_DIM_TYPES, 0, inline=[1], build_hint=50_000)
     j.build(dim, 50_000)
     j.finish()
-    qs, shapes = W.plan_config5(n, 80, seed=3, scale=0.1)
+    qs, shapes = W.plan_flow_requests(n, 80, seed=3, scale=0.1)
     print(f"[serve] {len(qs)} requests over {n} rows", flush=True)
     served = Server(ctx, cols, n, 3, lanes=3, hbm_budget=1 << 30, max_batch=16)
     served.add_dim(j)
@@ -75,8 +75,8 @@ def test_served_equals_alone_equals_oracle(ctx):
 def test_admission_respects_the_budget(ctx):
     """A tiny HBM budget forces requests through one at a time; results stay exact."""
     n = 1_000_000
-    cols = W.gen_config5(ctx, n, seed=9)
-    qs, shapes = W.plan_config5(n, 12, seed=4, scale=0.05)
+    cols = W.gen_flow_table(ctx, n, seed=9)
+    qs, shapes = W.plan_flow_requests(n, 12, seed=4, scale=0.05)
     qs = [q for q, s in zip(qs, shapes) if s["kind"] == 0]
     s1 = Server(ctx, cols, n, 3, lanes=2, hbm_budget=1, max_batch=8)
     lat, ids = s1.replay(qs, qps=5000, seed=2)

@@ -27,8 +27,8 @@ import numpy as np
 
 from . import _lib as L
 
-CFG1_ROWS = 1_000_000
-CFG1_SEED = 1
+KV1K_ROWS = 1_000_000
+KV1K_SEED = 1
 
 
 def _mix64(x: np.ndarray) -> np.ndarray:
@@ -43,14 +43,14 @@ def _mix64(x: np.ndarray) -> np.ndarray:
 
 
 # the 1000 distinct group keys: spread over the whole int64 range
-CFG1_DISTINCT = _mix64(np.arange(1000, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)).view(np.int64)
+KV1K_DISTINCT = _mix64(np.arange(1000, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)).view(np.int64)
 
 
-def gen_config1_host(n: int = CFG1_ROWS, seed: int = CFG1_SEED) -> Tuple[np.ndarray, np.ndarray]:
-    """Seeded (k, v): k uniform over CFG1_DISTINCT, v uniform in [0, 1e6)."""
+def gen_kv1k_host(n: int = KV1K_ROWS, seed: int = KV1K_SEED) -> Tuple[np.ndarray, np.ndarray]:
+    """Seeded (k, v): k uniform over KV1K_DISTINCT, v uniform in [0, 1e6)."""
     i = np.arange(n, dtype=np.uint64)
     s = np.uint64(seed) * np.uint64(0x632BE59BD9B4E019)
-    k = CFG1_DISTINCT[(_mix64(i + s) % np.uint64(1000)).astype(np.int64)]
+    k = KV1K_DISTINCT[(_mix64(i + s) % np.uint64(1000)).astype(np.int64)]
     v = (_mix64(i + s + np.uint64(0x3C6EF372FE94F82B)) % np.uint64(1_000_000)).astype(np.int64)
     return k, v
 
@@ -63,10 +63,10 @@ def columns_digest(arrays: Sequence[np.ndarray]) -> str:
     return h.hexdigest()
 
 
-def write_config1(path: str, n: int = CFG1_ROWS, seed: int = CFG1_SEED, row_group_rows: int = 131_072) -> None:
+def write_kv1k(path: str, n: int = KV1K_ROWS, seed: int = KV1K_SEED, row_group_rows: int = 131_072) -> None:
     import pyarrow as pa
     import pyarrow.parquet as pq
-    k, v = gen_config1_host(n, seed)
+    k, v = gen_kv1k_host(n, seed)
     pq.write_table(pa.table({"k": k, "v": v}), path, row_group_size=row_group_rows)
 
 

@@ -7,29 +7,29 @@ from dataclasses import dataclass
 from . import _lib as L
 from .gpu import AggSpec, Col, Context, Pred
 
-# --- config 1: k int64 over 1000 values, v int64 in [0, 1e6) -------------------
-CFG1_TYPES = [L.INT64, L.INT64]
-CFG1_AGGS = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 1), AggSpec(L.MAX, 1)]
+# --- k int64 over 1000 values, v int64 in [0, 1e6) -------------------
+KV1K_TYPES = [L.INT64, L.INT64]
+KV1K_AGGS = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 1), AggSpec(L.MAX, 1)]
 
 
-def gen_config1(ctx: Context, n: int = 1_000_000, seed: int = 1):
+def gen_kv1k(ctx: Context, n: int = 1_000_000, seed: int = 1):
     k = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, seed, 0, 999)
     v = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, seed + 1, 0, 999_999)
     return [Col(k, L.INT64), Col(v, L.INT64)]
 
 
-# --- config 2: composite (a Zipf(1.1) over 1M, b uniform 16) GROUP BY ---------
+# --- composite (a Zipf(1.1) over 1M, b uniform 16) GROUP BY ---------
 # SUM(price DECIMAL(15,2)) into Int128, COUNT(*), MIN/MAX/AVG(f), WHERE f > -0.5
-CFG2_TYPES = [L.INT32, L.INT32, L.DECIMAL, L.FLOAT64]
-CFG2_SCALES = [0, 0, 2, 0]
-CFG2_KEYS = [0, 1]
-CFG2_AGGS = [AggSpec(L.SUM, 2), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 3), AggSpec(L.MAX, 3), AggSpec(L.AVG, 3)]
-CFG2_PRED = Pred(3, L.GT, 0, -0.5)
-CFG2_BYTES_PER_ROW = 4 + 4 + 8 + 8
-CFG2_BYTES_PER_GROUP = 56
+ZIPF_TYPES = [L.INT32, L.INT32, L.DECIMAL, L.FLOAT64]
+ZIPF_SCALES = [0, 0, 2, 0]
+ZIPF_KEYS = [0, 1]
+ZIPF_AGGS = [AggSpec(L.SUM, 2), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 3), AggSpec(L.MAX, 3), AggSpec(L.AVG, 3)]
+ZIPF_PRED = Pred(3, L.GT, 0, -0.5)
+ZIPF_BYTES_PER_ROW = 4 + 4 + 8 + 8
+ZIPF_BYTES_PER_GROUP = 56
 
 
-def gen_config2(ctx: Context, n: int = 100_000_000, seed: int = 2, row0: int = 0):
+def gen_zipf_groupby(ctx: Context, n: int = 100_000_000, seed: int = 2, row0: int = 0):
     a = ctx.gen(L.GEN_ZIPF_I32, "int32", n, seed, 1_000_000, 0, 1.1, row0=row0)
     b = ctx.gen(L.GEN_UNIFORM_I32, "int32", n, seed + 1, 0, 15, row0=row0)
     price = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, seed + 2, 100, 10_000_000, row0=row0)
@@ -37,38 +37,38 @@ def gen_config2(ctx: Context, n: int = 100_000_000, seed: int = 2, row0: int = 0
     return [Col(a, L.INT32), Col(b, L.INT32), Col(price, L.DECIMAL, None, 2), Col(f, L.FLOAT64)]
 
 
-# --- config 3: build 10M unique int64 keys, probe 200M fk (95% matching) -------
+# --- build 10M unique int64 keys, probe 200M fk (95% matching) -------
 @dataclass
-class Cfg3:
+class FkJoin:
     n_build: int = 10_000_000
     n_probe: int = 200_000_000
     match_permille: int = 950
     seed: int = 3
 
 
-CFG3_BUILD_TYPES = [L.INT64, L.INT32, L.DECIMAL]  # key, p1, p2 DECIMAL(12,2)
-CFG3_BUILD_SCALES = [0, 0, 2]
-CFG3_PROBE_TYPES = [L.INT64, L.INT32]  # fk, qty
-CFG3_BYTES_PER_PROBE_ROW = 12
-CFG3_BYTES_PER_BUILD_ROW = 20
+FKJ_BUILD_TYPES = [L.INT64, L.INT32, L.DECIMAL]  # key, p1, p2 DECIMAL(12,2)
+FKJ_BUILD_SCALES = [0, 0, 2]
+FKJ_PROBE_TYPES = [L.INT64, L.INT32]  # fk, qty
+FKJ_BYTES_PER_PROBE_ROW = 12
+FKJ_BYTES_PER_BUILD_ROW = 20
 
 
-def gen_config3_build(ctx: Context, cfg: Cfg3):
+def gen_fk_join_build(ctx: Context, cfg: FkJoin):
     key = ctx.gen(L.GEN_PERM_KEY_I64, "int64", cfg.n_build, cfg.seed, cfg.n_build)
     p1 = ctx.gen(L.GEN_UNIFORM_I32, "int32", cfg.n_build, cfg.seed + 1, 0, 24)
     p2 = ctx.gen(L.GEN_UNIFORM_I64, "int64", cfg.n_build, cfg.seed + 2, 0, 10**12 - 1)
     return [Col(key, L.INT64), Col(p1, L.INT32), Col(p2, L.DECIMAL, None, 2)]
 
 
-def gen_config3_probe(ctx: Context, cfg: Cfg3, n: int = None, row0: int = 0):
+def gen_fk_join_probe(ctx: Context, cfg: FkJoin, n: int = None, row0: int = 0):
     n = cfg.n_probe if n is None else n
     fk = ctx.gen(L.GEN_FK_I64, "int64", n, cfg.seed + 10, cfg.n_build, cfg.match_permille, row0=row0)
     qty = ctx.gen(L.GEN_UNIFORM_I32, "int32", n, cfg.seed + 11, 1, 50, row0=row0)
     return [Col(fk, L.INT64), Col(qty, L.INT32)]
 
 
-# --- config 4: many concurrent GROUP BY queries of uneven shapes ---------------
-def plan_config4(n_queries: int = 256, seed: int = 4, max_rows: int = 4_000_000, min_rows: int = 1000,
+# --- many concurrent GROUP BY queries of uneven shapes ---------------
+def plan_query_mix(n_queries: int = 256, seed: int = 4, max_rows: int = 4_000_000, min_rows: int = 1000,
                  median_rows: int = 65_536, sigma: float = 1.39, min_groups: int = 10, max_groups: int = 100_000):
     """Per query: rows ~ lognormal clipped to [min_rows, max_rows], groups
     log-uniform in [min_groups, max_groups], key type cycling int64 / int32 /
@@ -83,7 +83,7 @@ def plan_config4(n_queries: int = 256, seed: int = 4, max_rows: int = 4_000_000,
             for i, (r, g, k, v) in enumerate(zip(rows, groups, key_types, val_types))]
 
 
-def gen_config4(ctx: Context, plan):
+def gen_query_mix(ctx: Context, plan):
     """Device columns of every planned query: list of (key Col, val Col, rows, est_groups)."""
     out = []
     for q in plan:
@@ -104,29 +104,29 @@ def gen_config4(ctx: Context, plan):
     return out
 
 
-# --- config 5: resident flow table served under Poisson load ------------------
-CFG5_TYPES = [L.UINT32, L.UINT16, L.INT64, L.INT64]  # src_ip, dst_port, bytes, ts
-CFG5_TS0 = 1_700_000_000_000_000_000
-CFG5_TS_STEP = 1000
-CFG5_DIM_TYPES = [L.UINT32, L.INT32]  # ip, asn
+# --- resident flow table served under Poisson load ------------------
+FLOW_TYPES = [L.UINT32, L.UINT16, L.INT64, L.INT64]  # src_ip, dst_port, bytes, ts
+FLOW_TS0 = 1_700_000_000_000_000_000
+FLOW_TS_STEP = 1000
+FLOW_DIM_TYPES = [L.UINT32, L.INT32]  # ip, asn
 
 
-def gen_config5(ctx: Context, n: int = 500_000_000, seed: int = 5):
+def gen_flow_table(ctx: Context, n: int = 500_000_000, seed: int = 5):
     src = ctx.gen(L.GEN_ZIPF_U32, "uint32", n, seed, 1_000_000, 0, 1.2)
     port = ctx.gen(L.GEN_UNIFORM_U16, "uint16", n, seed + 1, 0, 1023)
     nbytes = ctx.gen(L.GEN_LOGNORMAL_I64, "int64", n, seed + 2, 0, 0, 6.0, 1.5)
-    ts = ctx.gen(L.GEN_SEQ_I64, "int64", n, seed + 3, CFG5_TS0, CFG5_TS_STEP)
+    ts = ctx.gen(L.GEN_SEQ_I64, "int64", n, seed + 3, FLOW_TS0, FLOW_TS_STEP)
     return [Col(src, L.UINT32), Col(port, L.UINT16), Col(nbytes, L.INT64), Col(ts, L.INT64)]
 
 
-def gen_config5_dim(ctx: Context, n: int = 50_000, seed: int = 5, n_asn: int = 500):
+def gen_flow_dim(ctx: Context, n: int = 50_000, seed: int = 5, n_asn: int = 500):
     """Dimension keyed by the n most frequent source addresses, with an ASN."""
     ip = ctx.gen(L.GEN_RANK_U32, "uint32", n, seed)
     asn = ctx.gen(L.GEN_UNIFORM_I32, "int32", n, seed + 9, 0, n_asn - 1)
     return [Col(ip, L.UINT32), Col(asn, L.INT32)]
 
 
-def plan_config5(n_table: int, n_queries: int, seed: int = 55, scale: float = 1.0):
+def plan_flow_requests(n_table: int, n_queries: int, seed: int = 55, scale: float = 1.0):
     """Mixed requests: GROUP BY dst_port / GROUP BY src_ip / JOIN dim GROUP BY asn,
     each over a random ts window; `scale` shrinks the windows for small tables."""
     import numpy as np
@@ -146,8 +146,8 @@ def plan_config5(n_table: int, n_queries: int, seed: int = 55, scale: float = 1.
             spec = (1, 0, 2, 512)
         w = max(1000, min(w, n_table - 1))
         start = int(rng.integers(0, n_table - w))
-        lo = CFG5_TS0 + start * CFG5_TS_STEP + int(rng.integers(0, CFG5_TS_STEP))
-        hi = lo + w * CFG5_TS_STEP
+        lo = FLOW_TS0 + start * FLOW_TS_STEP + int(rng.integers(0, FLOW_TS_STEP))
+        hi = lo + w * FLOW_TS_STEP
         k, key_col, val_col, est = spec
         qs.append(Server.query(int(k), key_col, val_col, lo, hi, est, dim=0, dim_payload=0))
         shapes.append(dict(kind=int(k), rows=w, start=start))
