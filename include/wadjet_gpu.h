@@ -200,6 +200,17 @@ wg_status wg_agg_fetch_state(wg_agg* h, int32_t agg, uint64_t offset, uint64_t c
  * single-column key, aggregate_null_group_test.go:48-289). */
 wg_status wg_agg_fetch_key_valid(wg_agg* h, uint64_t offset, uint64_t count, uint8_t* valid_out,
                                  int32_t mem);
+/* The finalized groups as ONE partial-state spill run in the reference's
+ * external-merge format ("WAGS\x01", aggregate_partial_spill.go:19-52: the run
+ * drainStateToRuns writes, :1414-1470), groups ascending by their binary sort
+ * key, so the CPU operator's k-way merge (finalizeViaPartialMerge) can combine
+ * GPU partials with its own runs. group_names / group_type_ids (parquet.TypeID)
+ * fill the header; NULL type ids derive them from the column types. Call with
+ * out = NULL to learn *size; out is a host buffer of cap bytes. The aggregate
+ * is not changed (wg_agg_reset afterwards drains it, as the reference's spill
+ * resets its state). */
+wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint16_t* group_type_ids,
+                           void* out, uint64_t cap, uint64_t* size);
 /* Drop all groups and staged rows; keep the spec and the table memory. */
 wg_status wg_agg_reset(wg_agg* h);
 /* Device bytes held by this aggregate (table + state + staging). */

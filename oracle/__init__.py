@@ -18,6 +18,10 @@ Pinning (how each function is checked, see tests/test_oracle.py):
   or_semi_join                                       brute-force membership loop with NULLs, reference
                                                      KATs (join_coverage_test.go:272-351, :711-735)
   or_digest_rows                                     order invariance + a Python restatement
+  partial_run / read_partial_run                     the reference's round-trip case
+                                                     (aggregate_partial_spill_test.go:289-392), the
+                                                     numGroups offset Close() patches (:259-290),
+                                                     bytes.Compare order of decimal-text keys
 """
 from __future__ import annotations
 
@@ -283,3 +287,158 @@ def digest_rows(rows: np.ndarray):
 
 def mix64(k: int) -> int:
     return lib().or_mix64(k & ((1 << 64) - 1))
+
+
+# ------------------------------------------------ partial-state spill runs
+
+# The external-merge spill run of HashAggregate (aggregate_partial_spill.go:19-52):
+# header, groups in bytes.Compare order of their sort keys, end marker.
+PARTIAL_MAGIC = b"WAGS\x01"
+TAG_NULL, TAG_INT64, TAG_INT32 = 0, 3, 6  # partialTag* (:64-75)
+PARQUET_INT32, PARQUET_INT64, PARQUET_IPV4, PARQUET_PORT = 1, 2, 8, 12  # parquet.TypeID (schema.go:12-35)
+
+
+def partial_spec(func: int, in_type, scale: int = 0):
+    """(func, is_float, is_decimal, dec_scale) of one aggregate, as
+    initFlatAccums sets the flat accumulator flags (aggregate.go:6030-6119)."""
+    if func == COUNT or in_type is None:
+        return (func, False, False, 0)
+    if func in (SUM, AVG):
+        if in_type == FLOAT64:
+            return (func, True, False, 0)
+        if in_type in (DECIMAL, DECIMAL128):
+            return (func, False, True, scale)
+        if in_type == INT64 and func == AVG:  # overflow-safe AVG: float64 sum
+            return (func, True, False, 0)
+        return (func, False, False, 0)
+    if in_type == FLOAT64:
+        return (func, True, False, 0)
+    if in_type in (DECIMAL, DECIMAL128):
+        return (func, False, True, 0)
+    return (func, False, False, 0)
+
+
+def partial_sort_key(vals, nulls) -> bytes:
+    """appendIntModeSortKey (aggregate_partial_drain_cursor.go:421-452): decimal
+    text per column, 0-separated; a NULL value is "<null>" (sort.go:992-995)."""
+    return b"\x00".join(b"<null>" if nl else str(int(v)).encode() for v, nl in zip(vals, nulls))
+
+
+def partial_run(res, specs, key_types, names=None, type_ids=None) -> bytes:
+    """One spill run of hash_agg's groups: writeHeader / Write / emitAcc /
+    Close (aggregate_partial_spill.go:171-351). specs: partial_spec per agg."""
+    import struct
+    k = len(key_types)
+    names = names or [""] * k
+    if type_ids is None:
+        type_ids = [PARQUET_INT32 if t == INT32 else PARQUET_PORT if t == UINT16 else
+                    PARQUET_IPV4 if t in (IPV4, UINT32) else PARQUET_INT64 for t in key_types]
+    out = bytearray(PARTIAL_MAGIC)
+    out += struct.pack("<I", k)
+    for nm, tid in zip(names, type_ids):
+        b = nm.encode()
+        out += struct.pack("<H", len(b)) + b + struct.pack("<H", tid)
+    out += struct.pack("<I", len(specs))
+    for func, is_f, is_d, sc in specs:
+        out += struct.pack("<IBBi", func, int(is_f), int(is_d), sc)
+    n = res["n"]
+    out += struct.pack("<I", n)
+    keys, nulls, acc = res["keys"], res["key_null"], res["acc"]
+    order = sorted(range(n), key=lambda g: partial_sort_key(keys[g], nulls[g]))
+    for g in order:
+        sk = partial_sort_key(keys[g], nulls[g])
+        out += b"\x01" + struct.pack("<I", len(sk)) + sk
+        for c in range(k):
+            if nulls[g][c]:
+                out += bytes([TAG_NULL])
+            elif key_types[c] in (INT32, UINT16):
+                out += bytes([TAG_INT32]) + struct.pack("<i", int(keys[g][c]))
+            else:
+                out += bytes([TAG_INT64]) + struct.pack("<q", int(keys[g][c]))
+        for a, (func, is_f, is_d, sc) in enumerate(specs):
+            x = acc[g, a]
+            if func == COUNT:
+                out += struct.pack("<q", int(x["count"]))
+            elif func in (SUM, AVG):
+                out += struct.pack("<q", int(x["count"]))
+                if is_d:
+                    out += struct.pack("<qQ", int(x["sum_hi"]), int(x["sum_lo"]))
+                elif is_f:
+                    out += struct.pack("<d", float(x["sum_f64"]))
+                else:
+                    out += struct.pack("<q", int(x["sum_i64"]))
+            else:
+                has = bool(x["has_min"] if func == MIN else x["has_max"])
+                out += bytes([1 if has else 0])
+                if has:
+                    if is_f:
+                        out += struct.pack("<d", float(x["min_f64"] if func == MIN else x["max_f64"]))
+                    else:
+                        out += struct.pack("<q", int(x["min_i64"] if func == MIN else x["max_i64"]))
+    out += b"\x00"
+    return bytes(out)
+
+
+def read_partial_run(buf: bytes) -> dict:
+    """readHeader / Next / readAcc (aggregate_partial_spill.go:759-877, :353-463):
+    dict(cols=[(name, type_id)], specs=[(func, is_float, is_decimal, scale)],
+    n, groups=[(sort_key, [key values, None = NULL], [acc dicts])])."""
+    import struct
+    p = 0
+
+    def take(fmt):
+        nonlocal p
+        v = struct.unpack_from(fmt, buf, p)
+        p += struct.calcsize(fmt)
+        return v
+
+    assert buf[:5] == PARTIAL_MAGIC, "bad magic"
+    p = 5
+    (k,) = take("<I")
+    cols = []
+    for _ in range(k):
+        (ln,) = take("<H")
+        nm = buf[p:p + ln].decode()
+        p += ln
+        (tid,) = take("<H")
+        cols.append((nm, tid))
+    (na,) = take("<I")
+    specs = [tuple(take("<IBBi")) for _ in range(na)]
+    specs = [(f, bool(i), bool(d), s) for f, i, d, s in specs]
+    (n,) = take("<I")
+    groups = []
+    while True:
+        (marker,) = take("<B")
+        if marker == 0:
+            break
+        assert marker == 1, f"bad row marker {marker}"
+        (sl,) = take("<I")
+        sk = bytes(buf[p:p + sl])
+        p += sl
+        kv = []
+        for _ in range(k):
+            (tag,) = take("<B")
+            kv.append(None if tag == TAG_NULL else take("<i")[0] if tag == TAG_INT32 else take("<q")[0])
+        accs = []
+        for func, is_f, is_d, sc in specs:
+            a = {}
+            if func == COUNT:
+                (a["count"],) = take("<q")
+            elif func in (SUM, AVG):
+                (a["count"],) = take("<q")
+                if is_d:
+                    hi, lo = take("<qQ")
+                    a["sum"] = (hi << 64) | lo  # the signed Int128
+                elif is_f:
+                    (a["sum"],) = take("<d")
+                else:
+                    (a["sum"],) = take("<q")
+            else:
+                (has,) = take("<B")
+                a["has"] = bool(has)
+                if has:
+                    (a["value"],) = take("<d") if is_f else take("<q")
+            accs.append(a)
+        groups.append((sk, kv, accs))
+    assert p == len(buf), "trailing bytes after the end marker"
+    return dict(cols=cols, specs=specs, n=n, groups=groups)

@@ -1,0 +1,102 @@
+"""The GPU aggregate's partial-state spill run (wg_agg_spill_run) against the
+oracle's run of the same rows: byte-identical when every accumulator is an
+integer or Int128, and equal up to float-sum order otherwise."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import FLOAT_RTOL
+from wadjet_amd import _lib as L
+from wadjet_amd.gpu import AggSpec, Col, HashAggregate, bitmap_from_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(ctx, host_cols, types, keys, aggs, nullable=None, scales=None, names=None):
+    dev = []
+    for arr, t, *rest in host_cols:
+        v = rest[0] if rest else None
+        dev.append(Col(ctx.to_device(arr), t, None if v is None else ctx.to_device(v), rest[1] if len(rest) > 1 else 0))
+    n = len(host_cols[0][0])
+    agg = HashAggregate(ctx, types, keys, aggs, scales=scales, nullable=nullable)
+    agg.consume(dev, n)
+    agg.finalize()
+    got = agg.spill_run(names)
+    oaggs = [(a.func, a.col) for a in aggs]
+    res = O.hash_agg(host_cols, keys, oaggs)
+    specs = [O.partial_spec(a.func, None if a.col < 0 else types[a.col], (scales or [0] * len(types))[a.col])
+             for a in aggs]
+    want = O.partial_run(res, specs, [types[k] for k in keys], names=names)
+    agg.close()
+    return got, want
+
+
+def test_integer_and_decimal_run_is_byte_identical(ctx):
+    """int32 composite key; SUM int64, COUNT(*), COUNT(col) with NULLs,
+    MIN/MAX int, SUM DECIMAL(18,2) whose group sums pass 2^64."""
+    print("[spill] integer/decimal run", flush=True)
+    rng = np.random.default_rng(11)
+    n = 400_000
+    a = rng.integers(-3000, 3000, n).astype(np.int32)
+    b = rng.integers(0, 7, n).astype(np.int32)
+    v = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    vvalid = rng.random(n) > 0.1
+    d = rng.integers(9 * 10**17, 9 * 10**18, n).astype(np.int64)
+    types = [L.INT32, L.INT32, L.INT64, L.DECIMAL]
+    hcols = [(a, L.INT32), (b, L.INT32), (v, L.INT64, bitmap_from_mask(vvalid)), (d, L.DECIMAL, None, 2)]
+    aggs = [AggSpec(L.SUM, 2), AggSpec(L.COUNT, -1), AggSpec(L.COUNT, 2), AggSpec(L.MIN, 2), AggSpec(L.MAX, 2),
+            AggSpec(L.SUM, 3)]
+    got, want = run_both(ctx, hcols, types, [0, 1], aggs, nullable=[False, False, True, False],
+                         scales=[0, 0, 0, 2], names=["a", "b"])
+    assert O.read_partial_run(got)["n"] == 6000 * 7
+    assert got == want
+
+
+def test_null_and_sentinel_groups_int64_key(ctx):
+    """A nullable int64 key: the NULL group ("<null>", tag NULL) and the
+    sentinel-valued key sort and encode like every other group."""
+    print("[spill] NULL and sentinel groups", flush=True)
+    rng = np.random.default_rng(12)
+    n = 200_000
+    k = rng.integers(-500, 500, n).astype(np.int64)
+    k[::997] = -0x7FFFFFFFFFFFFFFF
+    kvalid = rng.random(n) > 0.05
+    v = rng.integers(0, 1000, n).astype(np.int32)
+    types = [L.INT64, L.INT32]
+    hcols = [(k, L.INT64, bitmap_from_mask(kvalid)), (v, L.INT32)]
+    aggs = [AggSpec(L.SUM, 1), AggSpec(L.AVG, 1), AggSpec(L.MAX, 1)]
+    got, want = run_both(ctx, hcols, types, [0], aggs, nullable=[True, False], names=["k"])
+    r = O.read_partial_run(got)
+    assert any(kv == [None] for _, kv, _ in r["groups"])
+    assert got == want
+
+
+def test_float_sums_equal_up_to_order(ctx):
+    """SUM/AVG over float64 and AVG over int64 are float sums: the runs agree
+    in every byte but those sums, which agree within the stated tolerance."""
+    print("[spill] float sums", flush=True)
+    rng = np.random.default_rng(13)
+    n = 300_000
+    k = rng.integers(0, 2000, n).astype(np.int32)
+    f = rng.normal(0, 1, n)
+    w = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    types = [L.INT32, L.FLOAT64, L.INT64]
+    hcols = [(k, L.INT32), (f, L.FLOAT64), (w, L.INT64)]
+    aggs = [AggSpec(L.SUM, 1), AggSpec(L.MIN, 1), AggSpec(L.MAX, 1), AggSpec(L.AVG, 2), AggSpec(L.COUNT, -1)]
+    got, want = run_both(ctx, hcols, types, [0], aggs)
+    rg, rw = O.read_partial_run(got), O.read_partial_run(want)
+    assert rg["cols"] == rw["cols"] and rg["specs"] == rw["specs"] and rg["n"] == rw["n"] == 2000
+    absf = np.zeros(2000)
+    np.add.at(absf, k, np.abs(f))
+    absw = np.zeros(2000)
+    np.add.at(absw, k, np.abs(w.astype(np.float64)))
+    for (skg, kvg, ag), (skw, kvw, aw) in zip(rg["groups"], rw["groups"]):
+        assert skg == skw and kvg == kvw
+        key = kvg[0]
+        for j, (x, y) in enumerate(zip(ag, aw)):
+            if j in (0, 3):
+                assert x["count"] == y["count"]
+                lim = FLOAT_RTOL * (absf[key] if j == 0 else absw[key])
+                assert abs(x["sum"] - y["sum"]) <= lim, (key, j, x, y)
+            else:
+                assert x == y, (key, j)

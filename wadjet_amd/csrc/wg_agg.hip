@@ -241,22 +241,6 @@ __global__ void agg_merge_part_kernel(const uint64_t* recs, uint64_t n, GlobalTa
   block_add(T.counters + CTR_GROUPS, claims);
 }
 
-// Exact Int128 value of a limb group (2 or 4 carry-save limbs).
-__host__ __device__ inline void limbs_value(const uint64_t* l, int nl, uint64_t& lo, int64_t& hi) {
-  // value = l0 + l1*2^32 (+ l2*2^64 + l3*2^96), limbs read as signed where
-  // they may carry negative sums (the top limb), unsigned otherwise.
-  unsigned __int128 acc = (unsigned __int128)l[0];
-  if (nl == 2) {
-    acc += (unsigned __int128)(__int128)(int64_t)l[1] << 32;
-  } else {
-    acc += (unsigned __int128)l[1] << 32;
-    acc += (unsigned __int128)l[2] << 64;
-    acc += (unsigned __int128)(__int128)(int64_t)l[3] << 96;
-  }
-  lo = (uint64_t)acc;
-  hi = (int64_t)(uint64_t)(acc >> 64);
-}
-
 // Fold each SUMD limb group into the exact Int128 base and zero the limbs
 // (before the limbs could reach 2^32 additions).
 __global__ void agg_normalize_kernel(uint64_t* slots, uint64_t total_slots, uint64_t* base, int stride,

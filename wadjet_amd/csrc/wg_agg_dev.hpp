@@ -76,6 +76,22 @@ __device__ __forceinline__ void rv_i128(const ColSet& cs, int c, const RowRef& r
   lo = d[2 * row + 1];
 }
 
+// Exact Int128 value of a limb group (2 or 4 carry-save limbs).
+__host__ __device__ inline void limbs_value(const uint64_t* l, int nl, uint64_t& lo, int64_t& hi) {
+  // value = l0 + l1*2^32 (+ l2*2^64 + l3*2^96), limbs read as signed where
+  // they may carry negative sums (the top limb), unsigned otherwise.
+  unsigned __int128 acc = (unsigned __int128)l[0];
+  if (nl == 2) {
+    acc += (unsigned __int128)(__int128)(int64_t)l[1] << 32;
+  } else {
+    acc += (unsigned __int128)l[1] << 32;
+    acc += (unsigned __int128)l[2] << 64;
+    acc += (unsigned __int128)(__int128)(int64_t)l[3] << 96;
+  }
+  lo = (uint64_t)acc;
+  hi = (int64_t)(uint64_t)(acc >> 64);
+}
+
 // ------------------------------------------------------------ atomics
 
 // Scope of the target table: LDS (workgroup) or global (agent).

@@ -1,0 +1,214 @@
+// wg_spill.hip — the aggregate's groups as one partial-state spill run in
+// wadjet's external-merge format, so a GPU aggregate can hand its partial
+// state to the CPU operator's k-way merge (finalizeViaPartialMerge) or park it
+// outside HBM.
+//
+// Format (internal/engine/exec/aggregate_partial_spill.go:19-52, little-endian):
+//   "WAGS\x01" | u32 group cols | per col: u16 name len, name, u16 parquet.TypeID
+//   | u32 aggs | per agg: u32 AggFunc, u8 isFloat, u8 isDecimal, i32 decScale
+//   | u32 groups | per group, ascending by sort key (bytes.Compare):
+//       u8 0x01, u32 sort key len, sort key, per group col a tagged value
+//       (writeKeyValue :465-536), per agg the accumulator (emitAcc :302-351)
+//   | u8 0x00
+// The sort key of an int-class group is the decimal text of each column value
+// joined by a 0 byte (appendIntModeSortKey / appendTypedIntKey,
+// aggregate_partial_drain_cursor.go:421-452); a NULL group value is "<null>"
+// (appendKeyValue, sort.go:992-995). The accumulator flags follow
+// initFlatAccums (aggregate.go:6030-6119): SUM/AVG over float64 and AVG over
+// int64 are float sums, SUM/AVG over DECIMAL are Int128 sums at the column's
+// scale, MIN/MAX over DECIMAL have no scatter kernel and stay empty.
+// Everything here is host code over the finalized dense state.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wg_agg_host.hpp"
+
+namespace wg {
+
+namespace {
+
+// parquet.TypeID (internal/storage/parquet/schema.go:12-35) of a group column
+int parquet_type_of(int t) {
+  switch (t) {
+    case WG_INT32: return 1;   // TypeInt32
+    case WG_UINT16: return 12; // TypePort
+    case WG_IPV4: case WG_UINT32: return 8;  // TypeIPv4
+    default: return 2;         // TypeInt64
+  }
+}
+
+struct Out {
+  std::vector<uint8_t> b;
+  void u8(uint8_t v) { b.push_back(v); }
+  void u16(uint16_t v) { for (int i = 0; i < 2; ++i) b.push_back((uint8_t)(v >> (8 * i))); }
+  void u32(uint32_t v) { for (int i = 0; i < 4; ++i) b.push_back((uint8_t)(v >> (8 * i))); }
+  void u64(uint64_t v) { for (int i = 0; i < 8; ++i) b.push_back((uint8_t)(v >> (8 * i))); }
+  void bytes(const void* p, size_t n) { b.insert(b.end(), (const uint8_t*)p, (const uint8_t*)p + n); }
+};
+
+// The reference accumulator flags of aggregate a.
+struct SpillSpec {
+  int func;
+  bool is_float, is_decimal;
+  int32_t scale;
+};
+
+SpillSpec spec_of(const wg_agg* h, int a) {
+  const AggFinal& f = h->finals[a];
+  const wg_agg_spec& s = h->aggs[a];
+  SpillSpec r{s.func, false, false, 0};
+  if (s.input_col < 0 || s.func == WG_AGG_COUNT) return r;
+  const int t = f.in_type;
+  switch (s.func) {
+    case WG_AGG_SUM:
+    case WG_AGG_AVG:
+      if (t == WG_FLOAT64) r.is_float = true;
+      else if (t == WG_DECIMAL || t == WG_DECIMAL128) { r.is_decimal = true; r.scale = f.scale; }
+      else if (t == WG_INT64 && s.func == WG_AGG_AVG) r.is_float = true;
+      break;
+    default:  // MIN / MAX
+      if (t == WG_FLOAT64) r.is_float = true;
+      else if (t == WG_DECIMAL || t == WG_DECIMAL128) r.is_decimal = true;
+      break;
+  }
+  return r;
+}
+
+// Value of group column k of a dense slot's packed key (agg_key_kernel's rule).
+int64_t key_value(const KeyDesc& K, uint64_t w, int k) {
+  if (K.n == 1) return (int64_t)w;
+  if (K.narrow[k] == 1) return (int64_t)(int32_t)(uint32_t)(w >> K.shift[k]);
+  if (K.narrow[k] == 2) return (int64_t)(uint32_t)(w >> K.shift[k]);
+  return (int64_t)w;
+}
+
+bool int32_tagged(int t) { return t == WG_INT32 || t == WG_UINT16; }
+
+}  // namespace
+
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" {
+
+wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint16_t* group_type_ids, void* out,
+                           uint64_t cap, uint64_t* size) {
+  clear_error();
+  WG_CHECK(h && size, WG_ERR_INVALID, "null argument");
+  WG_CHECK(h->finalized, WG_ERR_STATE, "spill run before finalize");
+  const uint64_t n = h->out_n;
+  WG_CHECK(n < (1ull << 32), WG_ERR_UNSUPPORTED, "a run holds fewer than 2^32 groups");
+  const int S = h->pg.stride;
+  std::vector<uint64_t> slots(n * S), base;
+  std::vector<uint8_t> nulls(n);
+  if (n) {
+    WG_HIP(hipMemcpyAsync(slots.data(), h->dense, n * S * 8, hipMemcpyDeviceToHost, h->ctx->stream));
+    WG_HIP(hipMemcpyAsync(nulls.data(), h->dense_null, n, hipMemcpyDeviceToHost, h->ctx->stream));
+    if (h->n_sumd) {
+      base.resize(n * h->n_sumd * 2);
+      WG_HIP(hipMemcpyAsync(base.data(), h->dense_base, base.size() * 8, hipMemcpyDeviceToHost, h->ctx->stream));
+    }
+    WG_HIP(hipStreamSynchronize(h->ctx->stream));
+  }
+  const KeyDesc& K = h->key;
+  const int na = (int)h->aggs.size();
+  // sort keys, then the order of the groups
+  std::vector<std::string> sk(n);
+  for (uint64_t j = 0; j < n; ++j) {
+    std::string& s = sk[j];
+    const uint64_t w = slots[j * S];
+    for (int k = 0; k < K.n; ++k) {
+      if (k) s.push_back('\0');
+      if (nulls[j]) s += "<null>";
+      else s += std::to_string(key_value(K, w, k));
+    }
+  }
+  std::vector<uint32_t> order(n);
+  for (uint64_t j = 0; j < n; ++j) order[j] = (uint32_t)j;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sk[a] < sk[b]; });
+
+  Out o;
+  o.bytes("WAGS\x01", 5);
+  o.u32((uint32_t)K.n);
+  for (int k = 0; k < K.n; ++k) {
+    const char* nm = group_names && group_names[k] ? group_names[k] : "";
+    const size_t len = strlen(nm);
+    WG_CHECK(len < 65536, WG_ERR_INVALID, "group column name of %zu bytes", len);
+    o.u16((uint16_t)len);
+    o.bytes(nm, len);
+    o.u16(group_type_ids ? group_type_ids[k] : (uint16_t)parquet_type_of(K.key_type[k]));
+  }
+  std::vector<SpillSpec> specs(na);
+  o.u32((uint32_t)na);
+  for (int a = 0; a < na; ++a) {
+    specs[a] = spec_of(h, a);
+    o.u32((uint32_t)specs[a].func);
+    o.u8(specs[a].is_float ? 1 : 0);
+    o.u8(specs[a].is_decimal ? 1 : 0);
+    o.u32((uint32_t)specs[a].scale);
+  }
+  o.u32((uint32_t)n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t j = order[i];
+    const uint64_t* s = slots.data() + (uint64_t)j * S;
+    o.u8(0x01);
+    o.u32((uint32_t)sk[j].size());
+    o.bytes(sk[j].data(), sk[j].size());
+    for (int k = 0; k < K.n; ++k) {
+      if (nulls[j]) {
+        o.u8(0);  // partialTagNull
+      } else if (int32_tagged(K.key_type[k])) {
+        o.u8(6);  // partialTagInt32
+        o.u32((uint32_t)(int32_t)key_value(K, s[0], k));
+      } else {
+        o.u8(3);  // partialTagInt64
+        o.u64((uint64_t)key_value(K, s[0], k));
+      }
+    }
+    for (int a = 0; a < na; ++a) {
+      const AggFinal& f = h->finals[a];
+      const SpillSpec& sp = specs[a];
+      // a sum with no scatter kernel (IPv4 inputs) never counted a row
+      const int64_t cnt = (sp.func == WG_AGG_COUNT || f.val_kind) ? (int64_t)s[f.count_word] : 0;
+      if (sp.func == WG_AGG_COUNT) {
+        o.u64((uint64_t)cnt);
+      } else if (sp.func == WG_AGG_SUM || sp.func == WG_AGG_AVG) {
+        o.u64((uint64_t)cnt);
+        if (sp.is_decimal) {
+          uint64_t lo = 0;
+          int64_t hi = 0;
+          if (f.val_kind == OP_SUMD || f.val_kind == OP_SUMD4) {
+            limbs_value(s + f.val_word, f.limbs, lo, hi);
+            const uint64_t* b = base.data() + ((uint64_t)j * h->n_sumd + f.sumd_index) * 2;
+            const unsigned __int128 v =
+                (((unsigned __int128)(uint64_t)hi << 64) | lo) + (((unsigned __int128)b[1] << 64) | b[0]);
+            lo = (uint64_t)v;
+            hi = (int64_t)(uint64_t)(v >> 64);
+          }
+          o.u64((uint64_t)hi);  // writeInt128: Hi, then Lo
+          o.u64(lo);
+        } else {
+          o.u64(f.val_kind ? s[f.val_word] : 0);  // SumF64 bits or SumI64
+        }
+      } else {  // MIN / MAX
+        const uint64_t e = f.val_kind ? s[f.val_word] : 0;
+        const uint64_t empty = sp.func == WG_AGG_MIN ? kMinInit : kMaxInit;
+        const bool has = f.val_kind != 0 && !sp.is_decimal && e != empty;
+        o.u8(has ? 1 : 0);
+        if (has) o.u64(f.is_float_in ? dec_f64_bits(e) : (uint64_t)dec_i64(e));
+      }
+    }
+  }
+  o.u8(0x00);
+  *size = o.b.size();
+  if (out == nullptr) return WG_OK;
+  WG_CHECK(cap >= o.b.size(), WG_ERR_INVALID, "run of %zu bytes does not fit %llu", o.b.size(),
+           (unsigned long long)cap);
+  memcpy(out, o.b.data(), o.b.size());
+  return WG_OK;
+}
+
+}  // extern "C"

@@ -250,6 +250,17 @@ class HashAggregate:
         """Heavy-hitter slots of the partitioned consume (0: none)."""
         call("wg_agg_set_heavy_hitters", self.h, hot_slots)
 
+    def spill_run(self, names=None, type_ids=None) -> bytes:
+        """The finalized groups as one partial-state spill run (bytes)."""
+        k = len(self.keys)
+        nm = None if names is None else (C.c_char_p * k)(*[x.encode() for x in names])
+        ti = None if type_ids is None else (C.c_uint16 * k)(*type_ids)
+        size = C.c_uint64()
+        call("wg_agg_spill_run", self.h, nm, ti, None, 0, C.byref(size))
+        buf = C.create_string_buffer(max(size.value, 1))
+        call("wg_agg_spill_run", self.h, nm, ti, buf, size.value, C.byref(size))
+        return buf.raw[:size.value]
+
     def pass_times(self):
         a = (C.c_float * 4)()
         call("wg_agg_pass_times", self.h, a)
