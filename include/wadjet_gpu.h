@@ -211,6 +211,12 @@ wg_status wg_agg_fetch_key_valid(wg_agg* h, uint64_t offset, uint64_t count, uin
  * resets its state). */
 wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint16_t* group_type_ids,
                            void* out, uint64_t cap, uint64_t* size);
+/* Merge one partial-state spill run (the same format, written by this library
+ * or by the CPU operator's drainStateToRuns for the same GROUP BY and
+ * aggregates) into a not yet finalized aggregate: the k-way merge's combine
+ * rules (mergeFlatAccumRow, aggregate.go:5155-5228) applied on the device.
+ * WG_ERR_INVALID when the run's aggregates or group columns differ. */
+wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size);
 /* Drop all groups and staged rows; keep the spec and the table memory. */
 wg_status wg_agg_reset(wg_agg* h);
 /* Device bytes held by this aggregate (table + state + staging). */

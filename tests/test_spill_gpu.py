@@ -100,3 +100,103 @@ def test_float_sums_equal_up_to_order(ctx):
                 assert abs(x["sum"] - y["sum"]) <= lim, (key, j, x, y)
             else:
                 assert x == y, (key, j)
+
+
+def _mixed_rows(seed, n):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(-400, 400, n).astype(np.int64)
+    k[::1009] = -0x7FFFFFFFFFFFFFFF
+    kvalid = rng.random(n) > 0.03
+    v = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    d = rng.integers(9 * 10**17, 9 * 10**18, n).astype(np.int64)
+    f = rng.normal(0, 1, n)
+    return k, kvalid, v, d, f
+
+
+MIX_TYPES = [L.INT64, L.INT64, L.DECIMAL, L.FLOAT64]
+MIX_AGGS = [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1), AggSpec(L.MIN, 1), AggSpec(L.MAX, 3), AggSpec(L.SUM, 2),
+            AggSpec(L.AVG, 1), AggSpec(L.SUM, 3)]
+
+
+def _dev_cols(ctx, k, kvalid, v, d, f):
+    return [Col(ctx.to_device(k), L.INT64, ctx.to_device(bitmap_from_mask(kvalid))), Col(ctx.to_device(v), L.INT64),
+            Col(ctx.to_device(d), L.DECIMAL, None, 2), Col(ctx.to_device(f), L.FLOAT64)]
+
+
+def _compare_all(ctx, agg, parts):
+    from helpers import compare, gpu_groups, oracle_groups
+    k, kvalid, v, d, f = (np.concatenate(x) for x in zip(*parts))
+    hcols = [(k, L.INT64, bitmap_from_mask(kvalid)), (v, L.INT64), (d, L.DECIMAL, None, 2), (f, L.FLOAT64)]
+    oaggs = [(a.func, a.col) for a in MIX_AGGS]
+    ref = oracle_groups(O.hash_agg(hcols, [0], oaggs))
+    keys = np.where(kvalid, k, 0)
+    gk = np.unique(ref["keys"][:, 0][~ref["nulls"][:, 0]])
+    absf = [None] * len(MIX_AGGS)
+    for j, a in enumerate(MIX_AGGS):
+        if a.func in (L.SUM, L.AVG) and MIX_TYPES[a.col] in (L.FLOAT64,) or (a.func == L.AVG and a.col == 1):
+            vals = np.abs(f) if a.col == 3 else np.abs(v.astype(np.float64))
+            s = np.zeros(len(ref["keys"]))
+            # oracle group order: key-sorted with the NULL group where _order puts it
+            order = {(bool(nl), int(kk)): g for g, (kk, nl) in enumerate(zip(ref["keys"][:, 0], ref["nulls"][:, 0]))}
+            for kk, ok, x in zip(keys, kvalid, vals):
+                s[order[(not ok, int(kk) if ok else 0)]] += x
+            absf[j] = s
+    compare(gpu_groups(agg), ref, oaggs, MIX_TYPES, [0, 0, 2, 0], abs_sums=absf)
+    assert len(gk) > 0
+
+
+def test_spill_then_merge_round_trip(ctx):
+    """Aggregate A spills its run; aggregate B consumes other rows, merges the
+    run, consumes more and finalizes: equal to the oracle over all rows
+    (NULL and sentinel-valued groups, Int128 sums, float sums in tolerance)."""
+    print("[spill] spill -> merge round trip", flush=True)
+    parts = [_mixed_rows(21 + i, 60_000) for i in range(3)]
+    a = HashAggregate(ctx, MIX_TYPES, [0], MIX_AGGS, scales=[0, 0, 2, 0], nullable=[True, False, False, False])
+    a.consume(_dev_cols(ctx, *parts[0]), 60_000)
+    a.finalize()
+    run = a.spill_run(["k"])
+    a.reset()  # the spill drains the state, as drainStateToRuns does
+    b = HashAggregate(ctx, MIX_TYPES, [0], MIX_AGGS, scales=[0, 0, 2, 0], nullable=[True, False, False, False])
+    b.consume(_dev_cols(ctx, *parts[1]), 60_000)
+    b.merge_run(run)
+    b.consume(_dev_cols(ctx, *parts[2]), 60_000)
+    b.finalize()
+    _compare_all(ctx, b, parts)
+    a.close()
+    b.close()
+
+
+def test_merge_a_run_written_by_the_cpu_operator(ctx):
+    """A partial computed on the CPU (the oracle's aggregate written as a run)
+    merges into a GPU aggregate: the result equals the oracle over all rows."""
+    print("[spill] merge a CPU-written run", flush=True)
+    parts = [_mixed_rows(31 + i, 50_000) for i in range(2)]
+    k, kvalid, v, d, f = parts[0]
+    hcols = [(k, L.INT64, bitmap_from_mask(kvalid)), (v, L.INT64), (d, L.DECIMAL, None, 2), (f, L.FLOAT64)]
+    res = O.hash_agg(hcols, [0], [(x.func, x.col) for x in MIX_AGGS])
+    specs = [O.partial_spec(x.func, None if x.col < 0 else MIX_TYPES[x.col], 2 if x.col == 2 else 0)
+             for x in MIX_AGGS]
+    run = O.partial_run(res, specs, [L.INT64], names=["k"])
+    g = HashAggregate(ctx, MIX_TYPES, [0], MIX_AGGS, scales=[0, 0, 2, 0], nullable=[True, False, False, False])
+    g.merge_run(run)
+    g.consume(_dev_cols(ctx, *parts[1]), 50_000)
+    g.finalize()
+    _compare_all(ctx, g, parts)
+    g.close()
+
+
+def test_merge_run_rejects_another_query(ctx):
+    """A run of different aggregates is refused before any state changes."""
+    from wadjet_amd._lib import WgError
+    k = np.arange(1000, dtype=np.int64) % 10
+    a = HashAggregate(ctx, [L.INT64, L.INT64], [0], [AggSpec(L.SUM, 1)])
+    a.consume([Col(ctx.to_device(k), L.INT64), Col(ctx.to_device(k), L.INT64)], 1000)
+    a.finalize()
+    run = a.spill_run()
+    b = HashAggregate(ctx, [L.INT64, L.INT64], [0], [AggSpec(L.MAX, 1)])
+    with pytest.raises(WgError):
+        b.merge_run(run)
+    with pytest.raises(WgError):
+        b.merge_run(run[:-3])
+    a.close()
+    b.close()

@@ -899,6 +899,28 @@ static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel,
 
 // Fold overflow partial records (possibly several per key) into the table
 // with atomics; the table was grown to hold them.
+// Merge a table of unique keys (cap slots + the side slots, exact Int128
+// bases or null) into h: one claim per key, then plain combines.
+wg_status agg_merge_slots(wg_agg* h, const uint64_t* src, uint64_t src_cap, const uint64_t* src_base,
+                          uint64_t src_groups) {
+  WG_TRY(agg_ensure_capacity(h, src_groups));
+  WG_TRY(ensure_base(h));
+  GlobalTable T = table_of(h);
+  T.max_probe = h->cap;
+  WG_TRY(zero_dev(h->ctx, h->counters + CTR_STOPPED, 1));
+  const uint64_t total = src_cap + kSideSlots;
+  hipLaunchKernelGGL(agg_merge_table_kernel, dim3(grid_for(total, 256, 8192)), dim3(256), 0, h->ctx->stream, src,
+                     src_cap, src_base, T, h->base, h->pg, (const int32_t*)h->d_sumd_word,
+                     (const int32_t*)h->d_sumd_limbs, h->n_sumd);
+  WG_HIP(hipGetLastError());
+  uint64_t c[CTR_N];
+  WG_TRY(agg_read_counters(h, c));
+  WG_CHECK((c[CTR_STOPPED] >> 32) == 0, WG_ERR_DEVICE, "merge did not fit the destination table");
+  h->groups = c[CTR_GROUPS];
+  h->dirty = true;
+  return WG_OK;
+}
+
 wg_status agg_merge_partials(wg_agg* h, const uint64_t* src, uint64_t n_part) {
   uint64_t* recs = nullptr;
   const uint64_t bytes = n_part * (uint64_t)h->pg.stride * 8;
@@ -1262,21 +1284,7 @@ wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
            WG_ERR_INVALID, "merge of aggregates with different specs");
   WG_TRY(agg_flush_stage(dst));
   WG_TRY(agg_flush_stage(src));
-  WG_TRY(agg_ensure_capacity(dst, src->groups));
-  WG_TRY(ensure_base(dst));
-  GlobalTable T = table_of(dst);
-  T.max_probe = dst->cap;
-  WG_TRY(zero_dev(dst->ctx, dst->counters + CTR_STOPPED, 1));
-  const uint64_t total = src->cap + kSideSlots;
-  hipLaunchKernelGGL(agg_merge_table_kernel, dim3(grid_for(total, 256, 8192)), dim3(256), 0, dst->ctx->stream,
-                     src->slots, src->cap, (const uint64_t*)src->base, T, dst->base, dst->pg,
-                     (const int32_t*)dst->d_sumd_word, (const int32_t*)dst->d_sumd_limbs, dst->n_sumd);
-  WG_HIP(hipGetLastError());
-  uint64_t c[CTR_N];
-  WG_TRY(agg_read_counters(dst, c));
-  WG_CHECK((c[CTR_STOPPED] >> 32) == 0, WG_ERR_DEVICE, "merge did not fit the destination table");
-  dst->groups = c[CTR_GROUPS];
-  dst->dirty = true;
+  WG_TRY(agg_merge_slots(dst, src->slots, src->cap, src->base, src->groups));
   return wg_agg_reset(src);
 }
 

@@ -125,6 +125,8 @@ wg_status ensure_overflow_buffers(wg_agg* h);
 wg_status ensure_overflow_for(wg_agg* h, uint64_t grid, int32_t lds_slots);
 wg_status ensure_base(wg_agg* h);
 wg_status agg_merge_partials(wg_agg* h, const uint64_t* recs, uint64_t n);
+wg_status agg_merge_slots(wg_agg* h, const uint64_t* src, uint64_t src_cap, const uint64_t* src_base,
+                          uint64_t src_groups);
 bool part_eligible(const wg_agg* h, const ColSet& cs);
 wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred);
 }  // namespace wg

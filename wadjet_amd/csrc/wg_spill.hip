@@ -211,4 +211,130 @@ wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint
   return WG_OK;
 }
 
+wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
+  clear_error();
+  WG_CHECK(h && (run || size == 0), WG_ERR_INVALID, "null argument");
+  WG_CHECK(!h->finalized, WG_ERR_STATE, "merge after finalize");
+  const uint8_t* p = (const uint8_t*)run;
+  const uint8_t* const end = p + size;
+  bool ok = true;
+  auto need = [&](uint64_t n) { ok = ok && (uint64_t)(end - p) >= n; return ok; };
+  auto rd = [&](int bytes) -> uint64_t {
+    uint64_t v = 0;
+    if (!need(bytes)) return 0;
+    for (int i = 0; i < bytes; ++i) v |= (uint64_t)p[i] << (8 * i);
+    p += bytes;
+    return v;
+  };
+  WG_CHECK(need(5) && memcmp(p, "WAGS\x01", 5) == 0, WG_ERR_INVALID, "not a partial-state spill run");
+  p += 5;
+  const KeyDesc& K = h->key;
+  const int na = (int)h->aggs.size();
+  WG_CHECK(rd(4) == (uint64_t)K.n && ok, WG_ERR_INVALID, "run has another number of group columns");
+  for (int k = 0; k < K.n && ok; ++k) {
+    const uint64_t len = rd(2);
+    if (need(len)) p += len;
+    (void)rd(2);
+  }
+  WG_CHECK(ok && rd(4) == (uint64_t)na && ok, WG_ERR_INVALID, "run has another number of aggregates");
+  std::vector<SpillSpec> specs(na);
+  for (int a = 0; a < na; ++a) {
+    const SpillSpec sp = specs[a] = spec_of(h, a);
+    const uint64_t func = rd(4), is_f = rd(1), is_d = rd(1), scale = rd(4);
+    WG_CHECK(ok && func == (uint64_t)sp.func && (is_f != 0) == sp.is_float && (is_d != 0) == sp.is_decimal &&
+                 (int32_t)scale == sp.scale,
+             WG_ERR_INVALID, "aggregate %d of the run differs from this aggregate", a);
+  }
+  const uint64_t n = rd(4);
+  WG_CHECK(ok, WG_ERR_INVALID, "truncated run header");
+  // the run as a table of unique keys: n slots, then the two side slots
+  const int S = h->pg.stride;
+  const int nb = h->n_sumd * 2;
+  std::vector<uint64_t> slots((n + kSideSlots) * S), base((n + kSideSlots) * (uint64_t)std::max(nb, 1));
+  for (uint64_t j = 0; j < n + kSideSlots; ++j)
+    for (int w = 0; w < S; ++w) slots[j * S + w] = slot_init_word(h->pg, w);
+  for (uint64_t j = n; j < n + kSideSlots; ++j) slots[j * S + 1] = 0;  // a side slot is live once it has rows
+  uint64_t g = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    WG_CHECK(rd(1) == 1 && ok, WG_ERR_INVALID, "bad row marker in group %llu", (unsigned long long)i);
+    const uint64_t sl = rd(4);
+    WG_CHECK(need(sl), WG_ERR_INVALID, "truncated sort key");
+    p += sl;
+    uint64_t key = 0;
+    bool null_key = false;
+    for (int k = 0; k < K.n; ++k) {
+      const uint64_t tag = rd(1);
+      int64_t v = 0;
+      if (tag == 0) null_key = true;
+      else if (tag == 6) v = (int64_t)(int32_t)(uint32_t)rd(4);
+      else if (tag == 3) v = (int64_t)rd(8);
+      else ok = false;
+      WG_CHECK(ok, WG_ERR_INVALID, "group value tag %llu not on the GPU path", (unsigned long long)tag);
+      uint64_t w = (uint64_t)v;
+      if (K.n > 1 && K.narrow[k]) w = (w & 0xFFFFFFFFull) << K.shift[k];
+      key |= w;
+    }
+    WG_CHECK(!null_key || K.n == 1, WG_ERR_NULL_KEY, "NULL in a multi-column group key");
+    // side slots: [n] the sentinel-valued key, [n + 1] the NULL key
+    const uint64_t j = null_key ? n + 1 : key == kEmpty ? n : g++;
+    uint64_t* s = slots.data() + j * S;
+    s[0] = null_key || key == kEmpty ? kEmpty : key;
+    uint64_t* b = base.data() + j * (uint64_t)std::max(nb, 1);
+    int64_t rows = -1;  // the group's row count, from an aggregate counted on word 1
+    for (int a = 0; a < na; ++a) {
+      const AggFinal& f = h->finals[a];
+      const SpillSpec& sp = specs[a];
+      if (sp.func == WG_AGG_COUNT || sp.func == WG_AGG_SUM || sp.func == WG_AGG_AVG) {
+        const int64_t cnt = (int64_t)rd(8);
+        if (f.val_kind || sp.func == WG_AGG_COUNT) {
+          s[f.count_word] = (uint64_t)cnt;
+          if (f.count_word == 1) rows = cnt;
+        }
+        if (sp.func == WG_AGG_COUNT) continue;
+        if (sp.is_decimal) {
+          const uint64_t hi = rd(8), lo = rd(8);
+          if (f.val_kind == OP_SUMD || f.val_kind == OP_SUMD4) {
+            b[f.sumd_index * 2] = lo;
+            b[f.sumd_index * 2 + 1] = hi;
+          }
+        } else {
+          const uint64_t v = rd(8);
+          if (f.val_kind == OP_SUMI || f.val_kind == OP_SUMF) s[f.val_word] = v;
+        }
+      } else {
+        const uint64_t has = rd(1);
+        if (has) {
+          const uint64_t v = rd(8);
+          if (f.val_kind == OP_MIN || f.val_kind == OP_MAX) s[f.val_word] = f.is_float_in ? enc_f64_bits(v) : enc_i64((int64_t)v);
+        }
+      }
+    }
+    WG_CHECK(ok, WG_ERR_INVALID, "truncated group %llu", (unsigned long long)i);
+    s[1] = (uint64_t)(rows >= 0 ? rows : 1);  // liveness when no aggregate carries the row count
+  }
+  WG_CHECK(rd(1) == 0 && ok && p == end, WG_ERR_INVALID, "run does not end after %llu groups", (unsigned long long)n);
+  if (n == 0) return WG_OK;
+  // the side-slot records move to just after the g table slots
+  for (uint64_t k = 0; k < kSideSlots; ++k) {
+    std::copy(slots.begin() + (n + k) * S, slots.begin() + (n + k + 1) * S, slots.begin() + (g + k) * S);
+    if (nb) std::copy(base.begin() + (n + k) * nb, base.begin() + (n + k + 1) * nb, base.begin() + (g + k) * nb);
+  }
+  WG_TRY(agg_flush_stage(h));
+  uint64_t* d_slots = nullptr;
+  uint64_t* d_base = nullptr;
+  WG_HIP(hipMalloc(&d_slots, (g + kSideSlots) * S * 8));
+  wg_status st = WG_OK;
+  if (nb && hipMalloc(&d_base, (g + kSideSlots) * nb * 8) != hipSuccess) st = WG_ERR_OOM;
+  if (st == WG_OK && hipMemcpy(d_slots, slots.data(), (g + kSideSlots) * S * 8, hipMemcpyHostToDevice) != hipSuccess)
+    st = WG_ERR_DEVICE;
+  if (st == WG_OK && nb && hipMemcpy(d_base, base.data(), (g + kSideSlots) * nb * 8, hipMemcpyHostToDevice) != hipSuccess)
+    st = WG_ERR_DEVICE;
+  if (st == WG_OK) st = agg_merge_slots(h, d_slots, g, d_base, g);
+  (void)hipStreamSynchronize(h->ctx->stream);
+  (void)hipFree(d_slots);
+  if (d_base) (void)hipFree(d_base);
+  if (st != WG_OK && !wg_last_error()[0]) set_error("merging a spill run failed");
+  return st;
+}
+
 }  // extern "C"

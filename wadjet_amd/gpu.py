@@ -261,6 +261,11 @@ class HashAggregate:
         call("wg_agg_spill_run", self.h, nm, ti, buf, size.value, C.byref(size))
         return buf.raw[:size.value]
 
+    def merge_run(self, run: bytes):
+        """Merge a partial-state spill run into this (not finalized) aggregate."""
+        buf = C.create_string_buffer(run, len(run))
+        call("wg_agg_merge_run", self.h, buf, len(run))
+
     def pass_times(self):
         a = (C.c_float * 4)()
         call("wg_agg_pass_times", self.h, a)
