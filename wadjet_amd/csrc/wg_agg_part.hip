@@ -158,6 +158,125 @@ __global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
 }
 
+// Pass A for the common column shapes, with the type decisions taken at
+// compile time: a key of one or two non-null 4- or 8-byte integer columns and
+// at most one predicate on a non-null 4- or 8-byte column. W0 / W1 / PW are the
+// byte widths of key column 0, key column 1 (0: none) and the predicate column
+// (0: none); signedness, the operator and the key packing stay runtime values
+// (uniform, resolved outside the row loop). Counts exactly the rows
+// part_count_kernel counts.
+constexpr int kCountTU = 8;  // rows per thread per step
+
+template <int W>
+__device__ __forceinline__ uint64_t load_w(const char* base, uint64_t row) {
+  if (W == 4) return (uint64_t)*(const __attribute__((address_space(1))) uint32_t*)(base + row * 4);
+  return *(const __attribute__((address_space(1))) uint64_t*)(base + row * 8);
+}
+
+template <int PW, int U>
+__device__ __forceinline__ uint32_t pred_bits(const uint64_t (&v)[U], bool is_float, int op, int64_t ci, double cf) {
+  uint32_t m = 0;
+  if (PW == 0) return (1u << U) - 1;
+  if (PW == 8 && is_float) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) m |= (uint32_t)cmp_op<double>(__longlong_as_double((long long)v[u]), cf, op) << u;
+  } else if (PW == 8) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) m |= (uint32_t)cmp_op<int64_t>((int64_t)v[u], ci, op) << u;
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; ++u) m |= (uint32_t)cmp_op<int32_t>((int32_t)(uint32_t)v[u], (int32_t)ci, op) << u;
+  }
+  return m;
+}
+
+template <int W0, int W1, int PW>
+__global__ void __launch_bounds__(kPartBlock) part_count_typed_kernel(PartParams P) {
+  constexpr int U = kCountTU;
+  extern __shared__ uint32_t hist[];
+  const uint32_t np = 1u << P.pbits;
+  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) hist[i] = 0;
+  const HotLds HL = hot_lds(hist, np, P.hot_bits, P.hot_slots);
+  if (P.hot_slots) hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)P.hot_slots));
+  __syncthreads();
+  const char* d0 = (const char*)P.cs.cols[P.key.col[0]].data;
+  const char* d1 = (const char*)P.cs.cols[P.key.col[W1 ? 1 : 0]].data;
+  const char* dp = (const char*)P.cs.cols[PW ? P.pred[0].col : 0].data;
+  const bool sx0 = P.cs.cols[P.key.col[0]].type == WG_INT32;
+  const bool sx1 = W1 && P.cs.cols[P.key.col[1]].type == WG_INT32;
+  const bool pf = PW && P.cs.cols[P.pred[0].col].type == WG_FLOAT64;
+  const int sh0 = P.key.shift[0], sh1 = P.key.shift[1];
+  const int op = P.pred[0].op;
+  const int64_t ci = P.pred[0].i64;
+  const double cf = P.pred[0].f64;
+  const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
+  const uint64_t end = min(beg + P.rows_per_wg, P.n);
+  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * U) {
+    uint64_t row[U], a[U], b[U], pv[U];
+    uint32_t in = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+      in |= (uint32_t)(i < end) << u;
+      const uint64_t ii = i < end ? i : end - 1;
+      row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
+    }
+    // every load of the step in flight before any is used
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = load_w<W0>(d0, row[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[u] = W1 ? load_w<W1 ? W1 : 4>(d1, row[u]) : 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) pv[u] = PW ? load_w<PW ? PW : 4>(dp, row[u]) : 0;
+    const uint32_t pass = in & pred_bits<PW, U>(pv, pf, op, ci, cf);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint64_t key;
+      if (W1) {  // two 4-byte columns packed into one word
+        key = ((a[u] & 0xFFFFFFFFull) << sh0) | ((b[u] & 0xFFFFFFFFull) << sh1);
+      } else {
+        key = (W0 == 4 && sx0) ? (uint64_t)(int64_t)(int32_t)(uint32_t)a[u] : a[u];
+      }
+      const uint64_t hk = key_hash(key);
+      // rows that do not count (filtered, sentinel-valued key, hot) add 0
+      const bool cold = ((pass >> u) & 1u) && key != kEmpty && (P.hot_slots == 0 || hot_find(HL, key, hk) < 0);
+      atomicAdd(&hist[hk >> (64 - P.pbits)], cold ? 1u : 0u);
+    }
+    (void)sx1;
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
+}
+
+using CountFn = void (*)(PartParams);
+
+// The typed pass A for this batch's columns, or null (the general kernel).
+static CountFn typed_count_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pred) {
+  auto width = [](int t) { return (t == WG_INT32 || t == WG_UINT32) ? 4 : (t == WG_INT64 || t == WG_IPV4) ? 8 : 0; };
+  if (K.n > 2 || pred[1].col >= 0) return nullptr;
+  int w[2] = {0, 0};
+  for (int k = 0; k < K.n; ++k) {
+    const ColDesc& c = cs.cols[K.col[k]];
+    if (c.valid) return nullptr;
+    w[k] = width(c.type);
+    if (!w[k]) return nullptr;
+  }
+  if (K.n == 2 && (w[0] != 4 || w[1] != 4)) return nullptr;
+  int pw = 0;
+  if (pred[0].col >= 0) {
+    const ColDesc& c = cs.cols[pred[0].col];
+    if (c.valid) return nullptr;
+    pw = c.type == WG_FLOAT64 ? 8 : c.type == WG_INT32 ? 4 : (c.type == WG_INT64 || c.type == WG_IPV4) ? 8 : 0;
+    if (!pw) return nullptr;
+  }
+#define WG_TC(A, B, C) if (w[0] == A && w[1] == B && pw == C) return part_count_typed_kernel<A, B, C>;
+  WG_TC(4, 4, 0) WG_TC(4, 4, 4) WG_TC(4, 4, 8)
+  WG_TC(4, 0, 0) WG_TC(4, 0, 4) WG_TC(4, 0, 8)
+  WG_TC(8, 0, 0) WG_TC(8, 0, 4) WG_TC(8, 0, 8)
+#undef WG_TC
+  return nullptr;
+}
+
 // RW = record words: key + values (2, 3, 4 or 8 words; 3 and 4 differ by a
 // zero pad word, 8 pads up to seven values).
 template <int RW>
@@ -679,15 +798,17 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   const int hot_bits = hot_bits_for(H);
   const size_t hist_bytes = (size_t)np * 4 + (H ? hot_lookup_bytes(hot_bits) : 0);
   const size_t scat_bytes = scatter_lds(H);
+  CountFn count_fn = typed_count_for(cs, h->key, pred);
+  if (!count_fn) count_fn = part_count_kernel;
   if (scat_bytes > 64 * 1024) {
-    WG_HIP(hipFuncSetAttribute((const void*)part_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    WG_HIP(hipFuncSetAttribute((const void*)count_fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     WG_HIP(hipFuncSetAttribute((const void*)scatter_kernel_for(RW), hipFuncAttributeMaxDynamicSharedMemorySize,
                                150 * 1024));
   }
   // one resident round of count/scatter workgroups (the two passes must use
   // the same row ranges, so both run at the occupancy of the lower one)
   int occ_c = 0, occ_s = 0;
-  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, part_count_kernel, kPartBlock, hist_bytes));
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, count_fn, kPartBlock, hist_bytes));
   WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_kernel_for(RW), kPartBlock, scat_bytes));
   const int per_cu = std::max(1, std::min(occ_c, occ_s));
   const int G = ctx->cus * per_cu;
@@ -776,7 +897,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     hipLaunchKernelGGL(hot_collect_kernel, dim3(256), dim3(256), 0, ctx->stream, X, (uint32_t)H);
     WG_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(part_count_kernel, dim3(G), dim3(kPartBlock), hist_bytes, ctx->stream, P);
+  hipLaunchKernelGGL(count_fn, dim3(G), dim3(kPartBlock), hist_bytes, ctx->stream, P);
   WG_HIP(hipGetLastError());
   WG_HIP(hipcub::DeviceScan::ExclusiveSum(h->part_scan, scan_bytes, counts, offs, (int)ncnt, ctx->stream));
   hipLaunchKernelGGL(part_bounds_kernel, dim3((np + 256) / 256), dim3(256), 0, ctx->stream, counts, offs, np,
