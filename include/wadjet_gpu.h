@@ -308,6 +308,19 @@ wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
 wg_status wg_join_semi(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
                        const uint32_t* sel, uint64_t n_sel, int32_t anti, uint32_t* out_sel,
                        uint64_t* n_out);
+/* Bloom pre-filter of the probe side (HashJoin.buildBloom, join.go:1441-1483;
+ * BloomFilterOp.Execute's int-key path, bloom_filter_op.go:39-219): build
+ * sets two bits per distinct build key (bloomHashInt, join.go:1650-1654) in a
+ * power-of-two array of ~10 bits per key, at least 8 words. The filter writes
+ * to out_sel (device, n entries) the ascending positions of sel (or 0..n_rows)
+ * whose non-NULL key may be in the build, *n_out on the host. The
+ * 32-batch / 5 % adaptive bypass is the caller's (host operator state). */
+wg_status wg_join_bloom_build(wg_join* j);
+wg_status wg_join_bloom_filter(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
+                               const uint32_t* sel, uint64_t n_sel, uint32_t* out_sel,
+                               uint64_t* n_out);
+/* The filter's words (*words = count; copied to host `out` when cap >= count). */
+wg_status wg_join_bloom_words(wg_join* j, uint64_t* words, uint64_t* out, uint64_t cap);
 /* Gather a retained build column at build row indices (gatherBuildVector,
  * join.go:4024) into a device buffer of the column's storage type. */
 wg_status wg_join_gather_build(wg_join* j, int32_t build_col, const uint32_t* build_rows,

@@ -442,3 +442,54 @@ def read_partial_run(buf: bytes) -> dict:
         groups.append((sk, kv, accs))
     assert p == len(buf), "trailing bytes after the end marker"
     return dict(cols=cols, specs=specs, n=n, groups=groups)
+
+
+# ------------------------------------------------------------ Bloom filter
+
+BLOOM_MUL = 0x9E3779B97F4A7C15
+
+
+def bloom_hash(keys: np.ndarray) -> np.ndarray:
+    """bloomHashInt (join.go:1650-1654) over int64 keys, wrapping uint64."""
+    with np.errstate(over="ignore"):
+        x = keys.astype(np.int64).view(np.uint64) * np.uint64(BLOOM_MUL)
+    return x ^ (x >> np.uint64(32))
+
+
+def bloom_words_for(n_keys: int) -> int:
+    """buildBloom's size (join.go:1452-1459): the smallest power of two of
+    64-bit words holding 10 bits per key, at least 8."""
+    w = 1
+    while w * 64 < n_keys * 10:
+        w *= 2
+    return max(w, 8)
+
+
+def bloom_build(build_keys: np.ndarray, valid=None) -> np.ndarray:
+    """buildBloom + bloomSet over the distinct non-NULL build keys."""
+    k = build_keys if valid is None else build_keys[valid]
+    k = np.unique(k.astype(np.int64))
+    words = np.zeros(bloom_words_for(len(k)), np.uint64)
+    mask = np.uint64(len(words) - 1)
+    h = bloom_hash(k)
+    one = np.uint64(1)
+    np.bitwise_or.at(words, (h & mask).astype(np.int64), one << (h & np.uint64(63)))
+    np.bitwise_or.at(words, ((h >> np.uint64(17)) & mask).astype(np.int64), one << ((h >> np.uint64(6)) & np.uint64(63)))
+    return words
+
+
+def bloom_filter(words: np.ndarray, probe_keys: np.ndarray, valid=None, sel=None) -> np.ndarray:
+    """BloomFilterOp.Execute's int-key path (bloom_filter_op.go:78-137):
+    ascending positions (of sel, or of every row) whose non-NULL key passes
+    bloomContains (:268-274)."""
+    pos = np.arange(len(probe_keys), dtype=np.uint32) if sel is None else np.asarray(sel, np.uint32)
+    keys = probe_keys.astype(np.int64)[pos]
+    mask = np.uint64(len(words) - 1)
+    h = bloom_hash(keys)
+    one = np.uint64(1)
+    b1 = (words[(h & mask).astype(np.int64)] >> (h & np.uint64(63))) & one
+    b2 = (words[((h >> np.uint64(17)) & mask).astype(np.int64)] >> ((h >> np.uint64(6)) & np.uint64(63))) & one
+    ok = (b1 == one) & (b2 == one)
+    if valid is not None:
+        ok &= np.asarray(valid, bool)[pos]
+    return pos[ok]

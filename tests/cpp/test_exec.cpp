@@ -283,6 +283,48 @@ static void test_semi_anti_kats(wg_ctx* g) {
   EXPECT(kept.size() == 1, "anti join keeps the NULL probe key: kept %zu rows, want 1", kept.size());
 }
 
+// BloomFilterOp over a built join: never drops a build key, keeps Sel order,
+// and bypasses itself after 32 batches that rejected under 5 % of their rows
+// (bloom_filter_op.go:201-214), while a filter that rejects half keeps going.
+static void test_bloom_filter_op(wg_ctx* g) {
+  Context ctx;
+  GpuHashJoin hj(g, "id", "id", 0, JoinType::Inner);
+  auto build = make_batch({{"id", TypeID::Int64}}, 1000);
+  for (int i = 0; i < 1000; ++i) build->columns[0]->i64[i] = (int64_t)i * 7;
+  BatchSource bs({build});
+  OK(hj.Build(ctx, &bs));
+  auto probe_batch = [](int64_t base, bool members) {
+    auto b = make_batch({{"id", TypeID::Int64}}, 2048);
+    for (int i = 0; i < 2048; ++i)
+      b->columns[0]->i64[i] = members ? ((base + i) % 1000) * 7 : (i % 2 ? ((base + i) % 1000) * 7 : 1000000 + base + i);
+    return b;
+  };
+  GpuBloomFilterOp all_in(&hj);
+  for (int bi = 0; bi < 40; ++bi) {
+    auto in = probe_batch(bi * 2048, true);
+    BatchPtr out;
+    OK(all_in.Execute(ctx, in, &out));
+    EXPECT(out != nullptr, "a batch of build keys was dropped");
+    if (!out) continue;
+    if (out.get() == in.get() && !out->has_sel) continue;  // bypassed
+    EXPECT(out->has_sel && out->sel.size() == 2048, "a build key was filtered out (%zu kept)", out->sel.size());
+  }
+  EXPECT(all_in.Disabled(), "a filter that rejects nothing must bypass itself after 32 batches");
+  GpuBloomFilterOp half(&hj);
+  for (int bi = 0; bi < 40; ++bi) {
+    auto in = probe_batch(bi * 2048, false);
+    BatchPtr out;
+    OK(half.Execute(ctx, in, &out));
+    EXPECT(out && out->has_sel, "half-member batch: no selection");
+    if (!out) continue;
+    size_t members = 0;
+    for (size_t i = 1; i < out->sel.size(); ++i) EXPECT(out->sel[i] > out->sel[i - 1], "selection out of order");
+    for (uint32_t r : out->sel) members += (r % 2) == 1;
+    EXPECT(members == 1024, "every build key of the batch is kept (%zu of 1024)", members);
+  }
+  EXPECT(!half.Disabled(), "a filter rejecting ~half the rows must stay on");
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   test_kernel_filter_and_pipeline();
@@ -296,6 +338,7 @@ int main(int argc, char** argv) {
     test_parallel_pushdown_equivalence(g);
     test_hash_join_kats(g);
     test_semi_anti_kats(g);
+    test_bloom_filter_op(g);
     wg_ctx_destroy(g);
   }
   std::printf("%s: %d failure(s)\n", mode.c_str(), g_fail);

@@ -257,3 +257,38 @@ def test_semi_anti_probe(ctx, anti):
             ref = O.semi_join((bk, t, bw), (pk, t, pw), probe_sel=None if s is None else sel, anti=anti)
             np.testing.assert_array_equal(got, ref)
         j.close()
+
+
+@pytest.mark.parametrize("n_build", [0, 1, 40_000, 2_000_000])
+def test_bloom_prefilter_matches_oracle(ctx, n_build):
+    """Device Bloom build == the oracle's words bit for bit; the filter's
+    ordered selection == the oracle's (selection vector, NULL probe keys,
+    duplicate and NULL build keys, int32 and int64 probe keys)."""
+    print(f"[join] bloom n_build={n_build}", flush=True)
+    rng = np.random.default_rng(n_build + 7)
+    bk = rng.integers(-10**12, 10**12, max(n_build, 1)).astype(np.int64)[:n_build]
+    if n_build > 10:
+        bk[: n_build // 10] = bk[n_build // 10: 2 * (n_build // 10)]  # duplicates
+    bvalid = rng.random(n_build) > 0.02
+    j = HashJoin(ctx, [L.INT64], 0, build_hint=max(n_build, 1))
+    if n_build:
+        j.build([Col(ctx.to_device(bk), L.INT64, ctx.to_device(bitmap_from_mask(bvalid)))], n_build)
+    j.finish()
+    j.bloom_build()
+    want_words = O.bloom_build(bk, bvalid)
+    np.testing.assert_array_equal(j.bloom_words(), want_words)
+    npr = 500_000
+    pk = np.where(rng.random(npr) < 0.5, bk[rng.integers(0, max(n_build, 1), npr)] if n_build else 0,
+                  rng.integers(-10**12, 10**12, npr)).astype(np.int64)
+    pvalid = rng.random(npr) > 0.03
+    sel = np.sort(rng.choice(npr, npr // 2, replace=False)).astype(np.uint32)
+    out = ctx.alloc(npr * 4)
+    key = Col(ctx.to_device(pk), L.INT64, ctx.to_device(bitmap_from_mask(pvalid)))
+    for s_ in (None, sel):
+        n = j.bloom_filter(key, npr, out, sel=None if s_ is None else (ctx.to_device(s_), len(s_)))
+        np.testing.assert_array_equal(out.to_host(np.uint32, n), O.bloom_filter(want_words, pk, pvalid, s_))
+    # int32 probe keys are widened like intKeyFromVector
+    pk32 = (pk % 1000).astype(np.int32)
+    n = j.bloom_filter(Col(ctx.to_device(pk32), L.INT32), npr, out)
+    np.testing.assert_array_equal(out.to_host(np.uint32, n), O.bloom_filter(want_words, pk32.astype(np.int64)))
+    j.close()

@@ -110,6 +110,9 @@ SIGNATURES: dict[str, list] = {
     "wg_gather": [P, C.POINTER(Column), P, U64, P],
     "wg_join_probe_agg": [P, C.POINTER(Column), I32, I32, U64, P, U64, P],
     "wg_join_set_tuning": [P, I32, I32, I32],
+    "wg_join_bloom_build": [P],
+    "wg_join_bloom_filter": [P, C.POINTER(Column), U64, P, U64, P, C.POINTER(U64)],
+    "wg_join_bloom_words": [P, C.POINTER(U64), P, U64],
     "wg_join_digest": [P, C.POINTER(Column), P, P, U64, C.POINTER(I32), I32, C.POINTER(I32), I32, C.POINTER(U64)],
     "wg_batch_create": [P, I32, U64, C.POINTER(P)],
     "wg_batch_destroy": [P],

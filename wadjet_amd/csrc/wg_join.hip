@@ -544,6 +544,39 @@ struct SemiPred {
   }
 };
 
+// ------------------------------------------------------ Bloom pre-filter
+// HashJoin.buildBloom / bloomSet / bloomHashInt (join.go:1441-1483, :1650-1654)
+// and bloomContains (bloom_filter_op.go:268-274): two bits per distinct build
+// key in a power-of-two array of 64-bit words (~10 bits per key, at least 8
+// words), both taken from one multiply-xorshift hash.
+__host__ __device__ __forceinline__ uint64_t bloom_hash(int64_t key) {
+  const uint64_t x = (uint64_t)key * 0x9E3779B97F4A7C15ull;
+  return x ^ (x >> 32);
+}
+
+__global__ void bloom_build_kernel(const uint64_t* slots, uint64_t cap, int stride, uint64_t* bloom, uint64_t mask) {
+  // table slots, then the side slot of the sentinel-valued key
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= cap; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t* s = slots + i * stride;
+    const bool live = i < cap ? s[0] != kEmpty : (s[1] >> 32) != 0;
+    if (!live) continue;
+    const uint64_t h = bloom_hash((int64_t)(i < cap ? s[0] : kEmpty));
+    atomicOr((unsigned long long*)&bloom[h & mask], 1ull << (h & 63));
+    atomicOr((unsigned long long*)&bloom[(h >> 17) & mask], 1ull << ((h >> 6) & 63));
+  }
+}
+
+struct BloomPred {
+  const uint64_t* bloom;
+  uint64_t mask;
+  ColDesc key;
+  __device__ bool operator()(uint64_t row) const {
+    if (!col_valid(key, row)) return false;  // a NULL key never matches
+    const uint64_t h = bloom_hash(load_int(key, row));
+    return ((bloom[h & mask] >> (h & 63)) & 1ull) && ((bloom[(h >> 17) & mask] >> ((h >> 6) & 63)) & 1ull);
+  }
+};
+
 static JoinTable jt_of(const wg_join* j) { return JoinTable{j->slots, j->cap - 1, j->stride, j->next}; }
 
 static ColDesc retained_col(const wg_join* j, int c) {
@@ -677,6 +710,7 @@ wg_status wg_join_destroy(wg_join* j) {
   (void)hipFree(j->cnt);
   (void)hipFree(j->off);
   (void)hipFree(j->cub_tmp);
+  (void)hipFree(j->bloom);
   delete j;
   return WG_OK;
 }
@@ -727,6 +761,7 @@ wg_status wg_join_build_finish(wg_join* j) {
   WG_HIP(hipStreamSynchronize(j->ctx->stream));
   const uint64_t distinct = c[0] + ((side >> 32) ? 1 : 0);
   j->claimed_unique = distinct == c[1];
+  j->distinct = distinct;
   j->finished = true;
   return WG_OK;
 }
@@ -760,6 +795,47 @@ wg_status wg_join_semi(wg_join* j, const wg_column* probe_key, uint64_t n_rows, 
   const SemiPred pred{jt_of(j), ColDesc{probe_key->data, probe_key->validity, probe_key->type, probe_key->scale},
                       anti ? 1 : 0};
   return select_rows(j->ctx, pred, n, sel, out_sel, n_out);
+}
+
+wg_status wg_join_bloom_build(wg_join* j) {
+  clear_error();
+  WG_CHECK(j, WG_ERR_INVALID, "null handle");
+  WG_CHECK(j->finished, WG_ERR_STATE, "bloom before build_finish");
+  // nSlots: the power of two with nSlots * 64 >= 10 bits per key, at least 8
+  uint64_t words = 1;
+  while (words * 64 < j->distinct * 10) words *= 2;
+  if (words < 8) words = 8;
+  if (j->bloom) WG_HIP(hipFree(j->bloom));
+  j->bloom = nullptr;
+  WG_HIP(hipMalloc(&j->bloom, words * 8));
+  WG_TRY(zero_dev(j->ctx, j->bloom, words));
+  j->bloom_mask = words - 1;
+  hipLaunchKernelGGL(bloom_build_kernel, dim3(grid_for(j->cap + 1, 256, 8192)), dim3(256), 0, j->ctx->stream,
+                     (const uint64_t*)j->slots, j->cap, j->stride, j->bloom, j->bloom_mask);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
+  return WG_OK;
+}
+
+wg_status wg_join_bloom_filter(wg_join* j, const wg_column* probe_key, uint64_t n_rows, const uint32_t* sel,
+                               uint64_t n_sel, uint32_t* out_sel, uint64_t* n_out) {
+  clear_error();
+  WG_CHECK(j && probe_key && out_sel && n_out, WG_ERR_INVALID, "null argument");
+  WG_CHECK(j->bloom, WG_ERR_STATE, "bloom filter before wg_join_bloom_build");
+  WG_CHECK(is_int_class(probe_key->type), WG_ERR_UNSUPPORTED, "probe key type %d", probe_key->type);
+  const uint64_t n = sel ? n_sel : n_rows;
+  const BloomPred pred{j->bloom, j->bloom_mask,
+                       ColDesc{probe_key->data, probe_key->validity, probe_key->type, probe_key->scale}};
+  return select_rows(j->ctx, pred, n, sel, out_sel, n_out);
+}
+
+wg_status wg_join_bloom_words(wg_join* j, uint64_t* words, uint64_t* out, uint64_t cap) {
+  clear_error();
+  WG_CHECK(j && words, WG_ERR_INVALID, "null argument");
+  WG_CHECK(j->bloom, WG_ERR_STATE, "bloom filter before wg_join_bloom_build");
+  *words = j->bloom_mask + 1;
+  if (out && cap >= *words) WG_HIP(hipMemcpy(out, j->bloom, *words * 8, hipMemcpyDeviceToHost));
+  return WG_OK;
 }
 
 wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,

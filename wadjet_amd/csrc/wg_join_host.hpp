@@ -24,6 +24,10 @@ struct wg_join {
   uint64_t* counter = nullptr;  // claimed slots + scratch words
   bool finished = false;
   bool claimed_unique = false;  // every non-NULL build key distinct (set at finish)
+  uint64_t distinct = 0;        // distinct non-NULL build keys (set at finish)
+  // Bloom pre-filter (wg_join_bloom_build): bloom_mask + 1 words of 64 bits
+  uint64_t* bloom = nullptr;
+  uint64_t bloom_mask = 0;
   // fused probe -> aggregate: streamlined kernel when the query has its shape
   int32_t lean = 1;           // 0: always the general kernel
   int32_t lean_block = 512;   // 512 or 1024 threads

@@ -385,6 +385,24 @@ class HashJoin:
         call("wg_join_semi", self.h, C.byref(kc), n_rows, sp, ns, 1 if anti else 0, out.ptr, C.byref(n))
         return n.value
 
+    def bloom_build(self):
+        call("wg_join_bloom_build", self.h)
+
+    def bloom_filter(self, key: Col, n_rows: int, out: DeviceBuffer, sel=None) -> int:
+        """Probe positions whose key may be in the build (device sel in `out`)."""
+        n = C.c_uint64()
+        kc = key.c()
+        sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
+        call("wg_join_bloom_filter", self.h, C.byref(kc), n_rows, sp, ns, out.ptr, C.byref(n))
+        return n.value
+
+    def bloom_words(self) -> np.ndarray:
+        w = C.c_uint64()
+        call("wg_join_bloom_words", self.h, C.byref(w), None, 0)
+        out = np.empty(w.value, np.uint64)
+        call("wg_join_bloom_words", self.h, C.byref(w), out.ctypes.data_as(C.c_void_p), w.value)
+        return out
+
     def gather_build(self, col: int, rows: DeviceBuffer, n: int, out: DeviceBuffer):
         call("wg_join_gather_build", self.h, col, rows.ptr, n, out.ptr)
 

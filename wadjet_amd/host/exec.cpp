@@ -681,4 +681,69 @@ Status GpuHashJoinProbe::emit(BatchPtr* out) {
   return Status::OK();
 }
 
+// --------------------------------------------------------- Bloom filter
+
+GpuBloomFilterOp::GpuBloomFilterOp(GpuHashJoin* j) : join_(j) {}
+
+GpuBloomFilterOp::~GpuBloomFilterOp() { (void)Close(); }
+
+Status GpuBloomFilterOp::Close() {
+  wg_ctx* c = join_->ctx_;
+  for (void* p : {d_key_, (void*)d_valid_, (void*)d_sel_, (void*)d_out_})
+    if (p) (void)wg_free(c, p);
+  d_key_ = nullptr;
+  d_valid_ = nullptr;
+  d_sel_ = nullptr;
+  d_out_ = nullptr;
+  cap_rows_ = 0;
+  return Status::OK();
+}
+
+Status GpuBloomFilterOp::Execute(Context&, const BatchPtr& in, BatchPtr* out) {
+  *out = in;
+  const uint64_t active = in ? (in->has_sel ? in->sel.size() : (uint64_t)in->len) : 0;
+  if (active == 0 || disabled_ || !join_->j_) return Status::OK();
+  if (!join_->bloom_built_) {
+    TRY(wrap(wg_join_bloom_build(join_->j_), "wg_join_bloom_build"));
+    join_->bloom_built_ = true;
+  }
+  const int ki = in->column_index(join_->probe_key_);
+  if (ki < 0) return Status::OK();  // key column not in this batch: pass every row (:66-76)
+  const Vector& k = *in->columns[ki];
+  wg_ctx* c = join_->ctx_;
+  if ((uint64_t)in->len > cap_rows_) {
+    (void)Close();
+    cap_rows_ = std::max<uint64_t>((uint64_t)in->len, batch::kDefaultBatchSize);
+    TRY(wrap(wg_malloc(c, cap_rows_ * 8, &d_key_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, (cap_rows_ + 63) / 64 * 8, (void**)&d_valid_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_sel_), "wg_malloc"));
+    TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_out_), "wg_malloc"));
+  }
+  TRY(wrap(wg_memcpy_h2d(c, d_key_, data_of(k), (uint64_t)in->len * elem_bytes(k.type)), "upload probe key"));
+  if (k.nulls.has_nulls())
+    TRY(wrap(wg_memcpy_h2d(c, d_valid_, k.nulls.words.data(), k.nulls.words.size() * 8), "upload validity"));
+  if (in->has_sel) TRY(wrap(wg_memcpy_h2d(c, d_sel_, in->sel.data(), in->sel.size() * 4), "upload sel"));
+  wg_column key = col_of(k);
+  key.data = d_key_;
+  key.validity = k.nulls.has_nulls() ? d_valid_ : nullptr;
+  uint64_t got = 0;
+  TRY(wrap(wg_join_bloom_filter(join_->j_, &key, (uint64_t)in->len, in->has_sel ? d_sel_ : nullptr,
+                                in->has_sel ? in->sel.size() : 0, d_out_, &got),
+           "wg_join_bloom_filter"));
+  if (got == 0) {
+    *out = nullptr;  // every row rejected (:183-185); not counted toward the bypass
+    return Status::OK();
+  }
+  total_checked_ += active;
+  total_passed_ += got;
+  if (++batches_seen_ >= 32 && total_checked_ > 0 && (total_checked_ - total_passed_) * 20 < total_checked_)
+    disabled_ = true;  // under 5 % rejected
+  auto o = std::make_shared<RecordBatch>(*in);  // columns shared, Sel replaced
+  o->sel.resize(got);
+  TRY(wrap(wg_memcpy_d2h(c, o->sel.data(), d_out_, got * 4), "download sel"));
+  o->has_sel = true;
+  *out = o;
+  return Status::OK();
+}
+
 }  // namespace wadjet

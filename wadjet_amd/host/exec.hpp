@@ -9,6 +9,7 @@
 //   GpuHashAggregate   SinkSource + MergeableSink      (aggregate.go:98, :1075, :3855, :4202, :4646-4700)
 //   GpuHashJoin        Build(ctx, Source) / Probe()      (join.go:771, :1967)
 //   GpuHashJoinProbe   UnaryOperator + BoundedOutput     (join.go:2072-2139, :2306-2515)
+//   GpuBloomFilterOp   UnaryOperator + Cloneable         (bloom_filter_op.go:13-274)
 // and the CPU pieces a pipeline needs around them: KernelFilter (filter.go:347-425),
 // BatchSource, CollectSink and Pipeline{Source, Ops, Sink, Workers}
 // (pipeline.go:31-690). Errors are returned as Status (Go's `error`).
@@ -272,6 +273,8 @@ class GpuHashJoin {
   std::vector<batch::Column> build_schema_;
   std::vector<std::shared_ptr<batch::Vector>> build_cols_;  // retained build side (host copy for gathers)
   uint64_t rows_ = 0;
+  bool bloom_built_ = false;
+  friend class GpuBloomFilterOp;
 };
 
 // MaxProbeOutputRows = 32768 (join.go:2044-2060)
@@ -302,6 +305,33 @@ class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, pub
   uint32_t* d_sel_ = nullptr;
   uint32_t* d_out_p_ = nullptr;
   uint32_t* d_out_b_ = nullptr;
+  uint64_t cap_rows_ = 0;
+};
+
+// BloomFilterOp (bloom_filter_op.go:13-274): drops probe rows whose key cannot
+// be in the build (device filter of the join's Bloom bits) before the probe.
+// Clones share the read-only filter; each keeps its own adaptive state: after
+// 32 batches a filter that rejected under 5 % of the rows it checked passes
+// every later batch through untouched (:201-214).
+class GpuBloomFilterOp : public UnaryOperator, public Cloneable {
+ public:
+  explicit GpuBloomFilterOp(GpuHashJoin* j);  // builds the join's filter on first use
+  ~GpuBloomFilterOp() override;
+  Status Init(Context&) override { return Status::OK(); }
+  Status Execute(Context&, const BatchPtr& in, BatchPtr* out) override;
+  Status Close() override;
+  std::unique_ptr<UnaryOperator> Clone() override { return std::make_unique<GpuBloomFilterOp>(join_); }
+  bool Disabled() const { return disabled_; }
+
+ private:
+  GpuHashJoin* join_;
+  uint64_t total_checked_ = 0, total_passed_ = 0;
+  int batches_seen_ = 0;
+  bool disabled_ = false;
+  void* d_key_ = nullptr;
+  uint64_t* d_valid_ = nullptr;
+  uint32_t* d_sel_ = nullptr;
+  uint32_t* d_out_ = nullptr;
   uint64_t cap_rows_ = 0;
 };
 
