@@ -88,10 +88,21 @@ def bench_agg(ctx, args):
     ms_step = wall * 1000 / args.steps
     k_ms = sum(kern_ms) / len(kern_ms)
     alg = W.ZIPF_BYTES_PER_ROW * n + W.ZIPF_BYTES_PER_GROUP * groups
+    cold = agg.stats()["global_rows"]  # rows the scatter pass wrote to the partitions
     log(f"[agg] {ms_step:.3f} ms/step, consume kernels {k_ms:.3f} ms, groups {groups}; "
-        f"passes (direct, count+scan, scatter, partition-agg) {passes[-1]}")
+        f"passes (direct, count+scan, scatter, partition-agg) {passes[-1]}; scattered rows {cold}")
+    # each pass against its own algorithmic bytes: count reads key + predicate
+    # (16 B/row); scatter reads the 24 B row and writes a 24 B record per cold
+    # row; the partition aggregate reads the records and writes 56 B per group
+    p = passes[-1]
+    per_pass = {}
+    for name, ms, b in (("count", p[1], 16 * n), ("scatter", p[2], 24 * n + 24 * cold),
+                        ("partition_agg", p[3], 24 * cold + W.ZIPF_BYTES_PER_GROUP * groups)):
+        if ms > 0:
+            per_pass[name] = dict(ms=ms, alg_bytes=b, achieved=b / (ms / 1000) / 1e9,
+                                  frac=b / (ms / 1000) / 1e9 / HBM_PEAK_GBS)
     return dict(cols=cols, n=n, groups=groups, ms_step=ms_step, kernel_ms=k_ms, alg_bytes=alg,
-                passes=passes[-1], agg=agg)
+                passes=passes[-1], per_pass=per_pass, cold_rows=cold, agg=agg)
 
 
 def cpu_agg(cols, n, sample_rows, threads):
@@ -329,7 +340,8 @@ def main():
                          "traffic_source": AGG_PMC_SUMMARY.name if traffic else None,
                          "kernels": "consume kernels of one step (direct, or count+scan, scatter, "
                                     "partition-aggregate)",
-                         "kernel_ms": A["kernel_ms"], "pass_ms": A["passes"], "alg_bytes": A["alg_bytes"]},
+                         "kernel_ms": A["kernel_ms"], "pass_ms": A["passes"], "alg_bytes": A["alg_bytes"],
+                         "per_pass": A["per_pass"], "scattered_rows": A["cold_rows"]},
             "cpu_baseline": cpu,
         }
     else:

@@ -225,7 +225,8 @@ wg_status wg_agg_footprint(wg_agg* h, uint64_t* bytes);
  * block = threads per workgroup (256, 512, 1024). */
 wg_status wg_agg_set_tuning(wg_agg* h, int32_t lds_slots, int32_t block);
 /* Counters from the last consume: rows that hit the LDS table, rows that went
- * to the global table, LDS entries flushed, global table capacity. */
+ * to the global table (partitioned consume: rows written to the partitions),
+ * LDS entries flushed, global table capacity. */
 wg_status wg_agg_stats(wg_agg* h, uint64_t* lds_rows, uint64_t* global_rows,
                        uint64_t* flushed, uint64_t* capacity);
 

@@ -907,6 +907,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   WG_HIP(hipMemcpyAsync(bound, dbound, (np + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
   WG_HIP(hipStreamSynchronize(ctx->stream));
   const uint64_t total = bound[np];
+  h->st_glb = total;  // rows scattered to the partitions (the cold rows)
 
   // scratch: partitioned records
   const uint64_t need = std::max<uint64_t>(total, 1) * RW;
