@@ -353,6 +353,26 @@ wg_status wg_join_digest(wg_join* j, const wg_column* probe_cols, const uint32_t
                          int32_t n_probe_sel, const int32_t* build_sel_cols, int32_t n_build_sel,
                          uint64_t digest_out[2]);
 
+/* ----------------------------------------------------------------- shuffle */
+
+/* Routing of a partitioned shuffle sink (partitionedShuffleSink.Consume,
+ * internal/worker/partitioned_shuffle_sink.go:199-290): every active row goes
+ * to partition FNV-1a-64(key bytes) % num_parts, as hashRowsIntoPartitions
+ * computes it (:1039-1171: each key column's storage bytes in the reference's
+ * vectors, little-endian — 4 for Int32/Port, 8 for Int64/IPv4/Float64 —,
+ * 0xff for a NULL, one 0x00 byte for a decimal column; no key columns = all
+ * rows to partition 0; num_parts 0 counts as 1), and the rows of each
+ * partition keep their input order (perPartRows, :272-290).
+ * Device columns; sel (device, ascending, < n_rows) selects the active rows,
+ * NULL = all n_rows. Outputs: part (device, n active entries, or NULL) =
+ * partition of the i-th active row (the `parts` array); rows (device, n
+ * active entries) = the active rows' indices (b.Sel[i] or i) grouped by
+ * partition; offsets (host, num_parts + 1 entries) = where each partition's
+ * rows start in `rows`, then the total. num_parts <= 16384. Synchronises. */
+wg_status wg_shuffle_partition(wg_ctx* ctx, const wg_column* cols, int32_t n_cols, const int32_t* key_cols,
+                               int32_t n_keys, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
+                               int32_t num_parts, uint32_t* part, uint32_t* rows, uint64_t* offsets);
+
 /* ------------------------------------------------------- batched execution */
 
 /* Many independent GROUP BY queries in one persistent launch. Each query has

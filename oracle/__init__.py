@@ -493,3 +493,105 @@ def bloom_filter(words: np.ndarray, probe_keys: np.ndarray, valid=None, sel=None
     if valid is not None:
         ok &= np.asarray(valid, bool)[pos]
     return pos[ok]
+
+
+# -------------------------------------------------------- shuffle routing
+
+FNV_OFFSET64 = 14695981039346656037  # partitioned_shuffle_sink.go:1023
+FNV_PRIME64 = 1099511628211          # partitioned_shuffle_sink.go:1024
+_M64 = (1 << 64) - 1
+
+
+def fnv1a64(data: bytes, h: int = FNV_OFFSET64) -> int:
+    """FNV-1a 64 over a byte string (Go's hash/fnv New64a), one byte at a time."""
+    for c in data:
+        h = ((h ^ c) * FNV_PRIME64) & _M64
+    return h
+
+
+def shuffle_key_width(t: int) -> int:
+    """Bytes a key column contributes to the row hash: the width of its storage
+    in the reference's vectors (hashRowsIntoPartitions' type switch,
+    partitioned_shuffle_sink.go:1057-1162). Int32/Port/Date hash their
+    Int32Data (4 bytes); Int64/Timestamp/IPv4 their Int64Data (8 bytes, so a
+    compact uint32 IPv4 hashes zero-extended to 8); Float64 its bits (8).
+    Anything else (decimals here) takes the default case: 0."""
+    return {INT32: 4, UINT16: 4, INT64: 8, IPV4: 8, UINT32: 8, FLOAT64: 8}.get(t, 0)
+
+
+def _key_words(data: np.ndarray, t: int) -> np.ndarray:
+    """Each row's storage bits as uint64 (little-endian byte order = bit order)."""
+    if t in (INT32, UINT16):
+        return data.astype(np.int64).astype(np.uint32).astype(np.uint64)
+    if t == UINT32:
+        return data.astype(np.uint32).astype(np.uint64)
+    if t == FLOAT64:
+        return np.ascontiguousarray(data, np.float64).view(np.uint64)
+    return data.astype(np.int64).view(np.uint64)
+
+
+def shuffle_hash(cols, key_cols, sel=None, n_rows=None) -> np.ndarray:
+    """hashRowsIntoPartitions' per-row hash (partitioned_shuffle_sink.go:1039-1162)
+    for every active row: FNV-1a 64 over the key columns' little-endian storage
+    bytes, column after column; a NULL value mixes one 0xff byte; a column of
+    another type mixes one 0x00 byte per row whether or not it is NULL.
+    cols: list of (values, type[, validity words]); returns uint64 per active row."""
+    n = len(cols[0][0]) if n_rows is None else n_rows
+    rows = np.arange(n, dtype=np.int64) if sel is None else np.asarray(sel, np.int64)
+    h = np.full(len(rows), FNV_OFFSET64, np.uint64)
+    prime = np.uint64(FNV_PRIME64)
+    with np.errstate(over="ignore"):
+        for c in key_cols:
+            data, t = cols[c][0], cols[c][1]
+            valid = cols[c][2] if len(cols[c]) > 2 else None
+            w = shuffle_key_width(t)
+            if w == 0:
+                h = (h ^ np.uint64(0)) * prime
+                continue
+            v = _key_words(np.asarray(data)[rows], t)
+            hv = h.copy()
+            for b in range(w):
+                hv = (hv ^ ((v >> np.uint64(8 * b)) & np.uint64(0xFF))) * prime
+            if valid is not None:
+                ok = ((np.asarray(valid, np.uint64)[rows >> 6] >> (rows & 63).astype(np.uint64)) & np.uint64(1)) == 1
+                hv = np.where(ok, hv, (h ^ np.uint64(0xFF)) * prime)
+            h = hv
+    return h
+
+
+def shuffle_partition(cols, key_cols, num_parts: int, sel=None, n_rows=None):
+    """Routing of partitionedShuffleSink.Consume (partitioned_shuffle_sink.go:
+    266-290): parts[i] = row hash % num_parts for the i-th active row (all 0
+    with no key columns, :1045-1050), then the rows of each partition in input
+    order (perPartRows: b.Sel[i] or i). Returns (parts uint32, rows uint32
+    grouped by partition, offsets int64 of num_parts + 1)."""
+    n = len(cols[0][0]) if n_rows is None else n_rows
+    active = np.arange(n, dtype=np.uint32) if sel is None else np.asarray(sel, np.uint32)
+    mod = max(int(num_parts), 1)
+    if len(key_cols) == 0:
+        parts = np.zeros(len(active), np.uint32)
+    else:
+        parts = (shuffle_hash(cols, key_cols, sel, n) % np.uint64(mod)).astype(np.uint32)
+    order = np.argsort(parts, kind="stable")
+    counts = np.bincount(parts, minlength=mod).astype(np.int64)
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return parts, active[order], offsets
+
+
+def shuffle_row_bytes(vals, types, nulls=None) -> bytes:
+    """hashVectorValue's byte stream for one row (partitioned_shuffle_sink.go:
+    1177-1195): each key column's little-endian storage bytes in order, 0xff
+    for a NULL; the reference's own routing test hashes rows this way."""
+    out = bytearray()
+    for i, (v, t) in enumerate(zip(vals, types)):
+        if nulls is not None and nulls[i]:
+            out += b"\xff"
+            continue
+        w = shuffle_key_width(t)
+        if w == 0:
+            out += b"\x00"
+        elif t == FLOAT64:
+            out += np.float64(v).tobytes()
+        else:
+            out += (int(v) & ((1 << (8 * w)) - 1)).to_bytes(w, "little")
+    return bytes(out)

@@ -114,6 +114,7 @@ SIGNATURES: dict[str, list] = {
     "wg_join_bloom_filter": [P, C.POINTER(Column), U64, P, U64, P, C.POINTER(U64)],
     "wg_join_bloom_words": [P, C.POINTER(U64), P, U64],
     "wg_join_digest": [P, C.POINTER(Column), P, P, U64, C.POINTER(I32), I32, C.POINTER(I32), I32, C.POINTER(U64)],
+    "wg_shuffle_partition": [P, C.POINTER(Column), I32, C.POINTER(I32), I32, U64, P, U64, I32, P, P, P],
     "wg_batch_create": [P, I32, U64, C.POINTER(P)],
     "wg_batch_destroy": [P],
     "wg_batch_run": [P, C.POINTER(BatchQuery), I32, I32, I32],

@@ -421,6 +421,23 @@ def gather(ctx: Context, col: Col, rows: DeviceBuffer, n: int, out: DeviceBuffer
     call("wg_gather", ctx.h, C.byref(cc), rows.ptr, n, out.ptr)
 
 
+def shuffle_partition(ctx: Context, cols: Sequence[Col], key_cols: Sequence[int], num_parts: int, n_rows: int,
+                      sel: Optional[DeviceBuffer] = None, n_sel: int = 0, want_parts: bool = True):
+    """Routing of a partitioned shuffle sink on the device (wg_shuffle_partition):
+    returns (parts device buffer or None, rows device buffer, offsets host
+    int64 array of num_parts + 1)."""
+    n = n_sel if sel is not None else n_rows
+    parts = ctx.alloc(max(n, 1) * 4) if want_parts else None
+    rows = ctx.alloc(max(n, 1) * 4)
+    offsets = np.zeros(max(num_parts, 1) + 1, np.uint64)
+    arr = columns(cols)
+    k = (C.c_int32 * max(len(key_cols), 1))(*key_cols)
+    call("wg_shuffle_partition", ctx.h, arr, len(cols), k, len(key_cols), n_rows,
+         sel.ptr if sel is not None else None, n_sel, num_parts, parts.ptr if parts is not None else None, rows.ptr,
+         offsets.ctypes.data_as(C.c_void_p))
+    return parts, rows, offsets.astype(np.int64)
+
+
 class Batch:
     """Many GROUP BY key SUM(val), COUNT(*) queries in one persistent launch."""
 
