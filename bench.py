@@ -176,6 +176,38 @@ def bench_join(ctx, args):
                 _keep=(j, agg, bcols, pcols))
 
 
+# -------------------------------------------- shuffle routing of the GROUP BY rows
+
+def bench_shuffle(ctx, args, cols=None):
+    """Partitioned-shuffle routing (wg_shuffle_partition) of the 100M-row
+    (a int32, b int32) keys into 256 partitions: FNV-1a per row, stable
+    per-partition row lists. Algorithmic bytes: the 8 key bytes read, the
+    4-byte partition id and the 4-byte row id written."""
+    from wadjet_amd import workloads as W
+    from wadjet_amd.gpu import shuffle_partition
+    n = args.rows
+    if cols is None:
+        log(f"[shuffle] generating {n} rows on device")
+        cols = W.gen_zipf_groupby(ctx, n)
+    parts_n = args.shuffle_parts
+    pbuf, rbuf = ctx.alloc(n * 4), ctx.alloc(n * 4)
+    for _ in range(max(args.warmup, 1)):
+        shuffle_partition(ctx, cols[:2], [0, 1], parts_n, n, parts=pbuf, rows=rbuf)
+    ctx.sync()
+    ms = []
+    for _ in range(max(1, args.steps)):
+        ctx.timer_start()
+        shuffle_partition(ctx, cols[:2], [0, 1], parts_n, n, parts=pbuf, rows=rbuf)
+        ms.append(ctx.timer_stop())
+    pbuf.free()
+    rbuf.free()
+    m = sum(ms) / len(ms)
+    alg = 16 * n
+    log(f"[shuffle] {n} rows -> {parts_n} partitions: {m:.3f} ms ({n / m / 1e6:.1f} G rows/s)")
+    return dict(rows=n, partitions=parts_n, ms=m, rows_per_s=n / (m / 1000), alg_bytes=alg,
+                achieved_gbs=alg / (m / 1000) / 1e9, frac=alg / (m / 1000) / 1e9 / HBM_PEAK_GBS)
+
+
 # -------------------------------------------- 256 concurrent queries, one launch
 
 def bench_batched(ctx, args):
@@ -279,7 +311,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--qps", type=float, nargs="*", default=[100, 1000, 5000])
-    ap.add_argument("--only", choices=["agg", "join", "batched", "served"], default=None)
+    ap.add_argument("--shuffle-parts", type=int, default=256)
+    ap.add_argument("--only", choices=["agg", "join", "batched", "served", "shuffle"], default=None)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -305,6 +338,8 @@ def main():
         J = bench_join(ctx, args)
         J.pop("_keep")
         extra["join"] = J
+    if want("shuffle"):
+        extra["shuffle"] = bench_shuffle(ctx, args, A["cols"] if A is not None else None)
     if want("batched"):
         extra["batched"] = bench_batched(ctx, args)
     if want("served"):

@@ -221,6 +221,12 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size);
 wg_status wg_agg_reset(wg_agg* h);
 /* Device bytes held by this aggregate (table + state + staging). */
 wg_status wg_agg_footprint(wg_agg* h, uint64_t* bytes);
+/* HashAggregate.StateBytes (aggregate.go:4050-4054, the bound
+ * cappedPartialAgg flushes on, worker/shuffle_partial_agg.go:186-188): bytes
+ * of live group state = groups so far x the bytes of one group's state
+ * (slot words + exact Int128 bases). Rows staged from host batches are
+ * aggregated first, so the figure covers every row consumed. */
+wg_status wg_agg_state_bytes(wg_agg* h, uint64_t* bytes);
 /* Tuning knobs (autotuner output): lds_slots = 0 disables the LDS table,
  * block = threads per workgroup (256, 512, 1024). */
 wg_status wg_agg_set_tuning(wg_agg* h, int32_t lds_slots, int32_t block);

@@ -325,6 +325,176 @@ static void test_bloom_filter_op(wg_ctx* g) {
   EXPECT(!half.Disabled(), "a filter rejecting ~half the rows must stay on");
 }
 
+// ------------------------------------------------------------ shuffle tests
+
+static uint64_t fnv1a64(const unsigned char* p, size_t n, uint64_t h = 14695981039346656037ull) {
+  for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+  return h;
+}
+
+// float64 sum of `val` per int64 `key` over flushed partial batches (the
+// consumer's merge, as sumByIntKey in shuffle_partial_agg_test.go:50-60)
+static std::map<int64_t, double> sum_by_key(const std::vector<BatchPtr>& bs, const char* key, const char* val) {
+  std::map<int64_t, double> out;
+  for (auto& b : bs) {
+    const int k = b->column_index(key), v = b->column_index(val);
+    for (int i = 0; i < b->len; ++i) out[b->columns[k]->i64[i]] += b->columns[v]->f64[i];
+  }
+  return out;
+}
+
+static BatchPtr kq_batch(std::vector<int64_t> k, std::vector<double> q) {
+  auto b = make_batch({{"k", TypeID::Int64}, {"q", TypeID::Float64}}, (int)k.size());
+  for (size_t i = 0; i < k.size(); ++i) {
+    b->columns[0]->i64[i] = k[i];
+    b->columns[1]->f64[i] = q[i];
+  }
+  return b;
+}
+
+static std::vector<BatchPtr> drain_all(CappedPartialAgg& p, std::vector<BatchPtr> in) {
+  Context ctx;
+  std::vector<BatchPtr> out, f;
+  for (auto& b : in) {
+    OK(p.Consume(ctx, b, &f));
+    out.insert(out.end(), f.begin(), f.end());
+  }
+  OK(p.Drain(ctx, &f));
+  out.insert(out.end(), f.begin(), f.end());
+  return out;
+}
+
+static void test_capped_partial_agg(wg_ctx* g) {
+  {  // TestCappedPartialAgg_SumMergesAndPreservesNames (shuffle_partial_agg_test.go:62-100)
+    CappedPartialAgg p(g, {"k"}, {{"sum", "q", "q"}});
+    auto rows = drain_all(p, {kq_batch({1, 1, 2}, {2, 3, 10}), kq_batch({1, 3}, {5, 7})});
+    int n = 0;
+    for (auto& b : rows) n += b->len;
+    EXPECT(n == 3, "got %d partial rows, want 3 groups", n);
+    for (auto& b : rows) EXPECT(b->column_index("k") >= 0 && b->column_index("q") >= 0, "names not preserved");
+    auto s = sum_by_key(rows, "k", "q");
+    EXPECT(s[1] == 10 && s[2] == 10 && s[3] == 7, "sums %g %g %g", s[1], s[2], s[3]);
+    EXPECT(p.in_rows == 5 && p.out_rows == 3, "counters in=%lld out=%lld", (long long)p.in_rows,
+           (long long)p.out_rows);
+  }
+  {  // TestCappedPartialAgg_CapFlushesEpochsAndStaysMergeable (:102-129): a 1-byte cap
+    CappedPartialAgg p(g, {"k"}, {{"sum", "q", "q"}}, {}, 1);
+    std::vector<BatchPtr> in;
+    for (int i = 0; i < 4; ++i) in.push_back(kq_batch({1, 2}, {1, 2}));
+    auto rows = drain_all(p, in);
+    EXPECT(p.flushes >= 4, "flushes = %lld, want >= 4", (long long)p.flushes);
+    auto s = sum_by_key(rows, "k", "q");
+    EXPECT(s[1] == 4 && s[2] == 8, "merged sums %g %g", s[1], s[2]);
+  }
+  {  // TestCappedPartialAgg_MinMax (:131-158)
+    auto b = make_batch({{"k", TypeID::Int64}, {"lo", TypeID::Float64}, {"hi", TypeID::Float64}}, 3);
+    const double lo[] = {5, 2, 7}, hi[] = {5, 9, 1};
+    for (int i = 0; i < 3; ++i) {
+      b->columns[0]->i64[i] = 1;
+      b->columns[1]->f64[i] = lo[i];
+      b->columns[2]->f64[i] = hi[i];
+    }
+    CappedPartialAgg p(g, {"k"}, {{"min", "lo", "lo"}, {"max", "hi", "hi"}});
+    auto rows = drain_all(p, {b});
+    EXPECT(rows.size() == 1 && rows[0]->len == 1, "want one row");
+    if (!rows.empty() && rows[0]->len == 1) {
+      const auto& r = rows[0];
+      EXPECT(r->columns[r->column_index("lo")]->f64[0] == 2.0 && r->columns[r->column_index("hi")]->f64[0] == 9.0,
+             "min/max = %g/%g, want 2/9", r->columns[r->column_index("lo")]->f64[0],
+             r->columns[r->column_index("hi")]->f64[0]);
+    }
+  }
+  {  // TestCappedPartialAgg_SelectionVectorRespected (:160-176)
+    auto b = kq_batch({1, 1, 2}, {100, 3, 4});
+    b->sel = {1, 2};
+    b->has_sel = true;
+    CappedPartialAgg p(g, {"k"}, {{"sum", "q", "q"}});
+    auto s = sum_by_key(drain_all(p, {b}), "k", "q");
+    EXPECT(s[1] == 3 && s[2] == 4, "sums %g %g (deselected row must not count)", s[1], s[2]);
+  }
+  {  // resolveAgainst (shuffle_partial_agg.go:117-151): absent keys dropped,
+     // absent partition key or nothing combinable -> raw passthrough
+    CappedPartialAgg absent_part(g, {"k"}, {{"sum", "q", "q"}}, {"nope"});
+    auto in = kq_batch({1, 1}, {1, 2});
+    auto rows = drain_all(absent_part, {in});
+    EXPECT(absent_part.Disabled() && rows.size() == 1 && rows[0].get() == in.get(), "absent partition key");
+    CappedPartialAgg dropped(g, {"k", "ghost"}, {{"sum", "q", "q"}});
+    auto s = sum_by_key(drain_all(dropped, {kq_batch({1, 1, 2}, {1, 2, 3})}), "k", "q");
+    EXPECT(!dropped.Disabled() && dropped.DroppedKeys() == 1 && s[1] == 3 && s[2] == 3, "declared-but-absent key");
+    CappedPartialAgg no_aggs(g, {"k"}, {{"sum", "ghost", "ghost"}});
+    drain_all(no_aggs, {kq_batch({1}, {1})});
+    EXPECT(no_aggs.Disabled(), "nothing combinable must pass through");
+    CappedPartialAgg unknown(g, {"k"}, {{"median", "q", "q"}});
+    auto raw = kq_batch({1, 2}, {1, 2});
+    auto r2 = drain_all(unknown, {raw});
+    EXPECT(unknown.Disabled() && r2.size() == 1 && r2[0].get() == raw.get(), "unknown function passes through");
+  }
+}
+
+static void test_shuffle_router(wg_ctx* g) {
+  Context ctx;
+  {  // TestPartitionedShuffleSink_RoundTrip (partitioned_shuffle_sink_test.go:19-78)
+    GpuShuffleRouter r(g, {"k"}, 4);
+    auto b = make_batch({{"k", TypeID::Int64}}, 1000);
+    for (int i = 0; i < 1000; ++i) b->columns[0]->i64[i] = i;
+    std::vector<std::vector<uint32_t>> per;
+    OK(r.Route(ctx, b, &per));
+    int64_t total = 0;
+    for (int p = 0; p < 4; ++p) {
+      for (size_t j = 0; j < per[p].size(); ++j) {
+        const int64_t k = b->columns[0]->i64[per[p][j]];
+        unsigned char le[8];
+        for (int x = 0; x < 8; ++x) le[x] = (unsigned char)((uint64_t)k >> (8 * x));
+        EXPECT((int)(fnv1a64(le, 8) % 4) == p, "row k=%lld in partition %d", (long long)k, p);
+        if (j) EXPECT(per[p][j] > per[p][j - 1], "partition %d out of input order", p);
+      }
+      total += (int64_t)per[p].size();
+      EXPECT(r.PartitionRowCounts()[p] == (int64_t)per[p].size(), "row counts");
+    }
+    EXPECT(total == 1000, "total rows %lld, want 1000", (long long)total);
+  }
+  {  // multi-column keys with NULLs and a Sel: hashVectorValue's byte stream
+    GpuShuffleRouter r(g, {"a", "p"}, 7);
+    auto b = make_batch({{"a", TypeID::Int64}, {"p", TypeID::Port}, {"x", TypeID::Float64}}, 300);
+    for (int i = 0; i < 300; ++i) {
+      b->columns[0]->i64[i] = (int64_t)i * 2654435761ll - 99;
+      b->columns[1]->i32[i] = (i * 37) % 65536;
+    }
+    for (int i = 0; i < 300; i += 11) b->columns[1]->nulls.set_null(i, 300);
+    for (int i = 0; i < 300; i += 3) b->sel.push_back((uint32_t)i);
+    b->has_sel = true;
+    std::vector<std::vector<uint32_t>> per;
+    std::vector<int> parts;
+    OK(r.Route(ctx, b, &per, &parts));
+    EXPECT(parts.size() == b->sel.size(), "one partition per active row");
+    size_t seen = 0;
+    for (int p = 0; p < 7; ++p) {
+      seen += per[p].size();
+      for (uint32_t row : per[p]) {
+        unsigned char buf[12];
+        size_t len = 0;
+        const int64_t a = b->columns[0]->i64[row];
+        for (int x = 0; x < 8; ++x) buf[len++] = (unsigned char)((uint64_t)a >> (8 * x));
+        if (!b->columns[1]->nulls.valid((int)row)) {
+          buf[len++] = 0xff;
+        } else {
+          const uint32_t v = (uint32_t)b->columns[1]->i32[row];
+          for (int x = 0; x < 4; ++x) buf[len++] = (unsigned char)(v >> (8 * x));
+        }
+        EXPECT((int)(fnv1a64(buf, len) % 7) == p, "row %u in partition %d", row, p);
+        EXPECT(row % 3 == 0, "row %u is not in the Sel", row);
+        EXPECT(parts[row / 3] == p, "parts[%u] = %d, want %d", row / 3, parts[row / 3], p);
+      }
+    }
+    EXPECT(seen == b->sel.size(), "rows lost");
+  }
+  {
+    GpuShuffleRouter r(g, {"missing"}, 4);
+    std::vector<std::vector<uint32_t>> per;
+    EXPECT(!r.Route(ctx, kq_batch({1}, {1}), &per).ok(), "an absent key must error (:212-216)");
+  }
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   test_kernel_filter_and_pipeline();
@@ -339,6 +509,8 @@ int main(int argc, char** argv) {
     test_hash_join_kats(g);
     test_semi_anti_kats(g);
     test_bloom_filter_op(g);
+    test_capped_partial_agg(g);
+    test_shuffle_router(g);
     wg_ctx_destroy(g);
   }
   std::printf("%s: %d failure(s)\n", mode.c_str(), g_fail);

@@ -147,3 +147,24 @@ def test_errors(ctx):
         shuffle_partition(ctx, cols, [0], 16_385, 100)
     with pytest.raises(WgError):
         shuffle_partition(ctx, cols, [3], 4, 100)
+
+
+def test_state_bytes_tracks_groups(ctx):
+    """wg_agg_state_bytes (HashAggregate.StateBytes, the bound the capped
+    partial aggregate flushes on): 0 before any row, groups x per-group state
+    after host and device batches, the same after finalize, 0 after reset."""
+    from wadjet_amd.gpu import AggSpec, HashAggregate
+    agg = HashAggregate(ctx, [L.INT64, L.INT64], [0], [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1)], group_hint=16)
+    assert agg.state_bytes() == 0
+    k = np.arange(1000, dtype=np.int64) % 300
+    agg.consume([Col(k, L.INT64), Col(k, L.INT64)], 1000, device=False)  # staged host batch
+    b1 = agg.state_bytes()
+    assert b1 > 0 and b1 % 300 == 0
+    k2 = (np.arange(5000, dtype=np.int64) % 700) + 10_000
+    agg.consume([Col(ctx.to_device(k2), L.INT64), Col(ctx.to_device(k2), L.INT64)], 5000)
+    assert agg.state_bytes() == b1 // 300 * 1000
+    agg.finalize()
+    assert agg.num_groups() == 1000 and agg.state_bytes() == b1 // 300 * 1000
+    agg.reset()
+    assert agg.state_bytes() == 0
+    agg.close()

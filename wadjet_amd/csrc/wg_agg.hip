@@ -1469,6 +1469,15 @@ wg_status wg_agg_footprint(wg_agg* h, uint64_t* bytes) {
   return WG_OK;
 }
 
+wg_status wg_agg_state_bytes(wg_agg* h, uint64_t* bytes) {
+  clear_error();
+  WG_CHECK(h && bytes, WG_ERR_INVALID, "null argument");
+  if (!h->finalized) WG_TRY(agg_flush_stage(h));
+  const uint64_t groups = h->finalized ? h->out_n : h->groups;
+  *bytes = groups * ((uint64_t)h->pg.stride * 8 + (uint64_t)h->n_sumd * 16);
+  return WG_OK;
+}
+
 wg_status wg_agg_set_tuning(wg_agg* h, int32_t lds_slots, int32_t block) {
   clear_error();
   WG_CHECK(h, WG_ERR_INVALID, "null handle");

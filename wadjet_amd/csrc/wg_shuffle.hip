@@ -13,8 +13,8 @@
 //            major, so partition p's rows from wavefront w start at
 //            offs[p * W + w] and the partitions come out contiguous;
 //   scatter  each wavefront walks its range again 64 rows at a time; a row's
-//            place among the step's rows of its partition comes from a
-//            ballot per distinct partition in the step (lanes in order), and
+//            place among the step's rows of its partition comes from one
+//            ballot per bit of the partition id (lanes in order), and
 //            the wavefront's LDS cursor per partition advances by the step's
 //            count, so every partition keeps input order — the stable
 //            grouping the reference builds with appends.
@@ -30,7 +30,6 @@
 #include "wg_core.hpp"
 
 namespace wg {
-namespace {
 
 constexpr uint64_t kFnvOffset = 14695981039346656037ull;  // partitioned_shuffle_sink.go:1023
 constexpr uint64_t kFnvPrime = 1099511628211ull;          // partitioned_shuffle_sink.go:1024
@@ -43,6 +42,7 @@ struct ShufParams {
   uint32_t np;
   uint32_t np_mask;    // np - 1 when np is a power of two, else 0
   uint32_t two32_mod;  // 2^32 mod np
+  int32_t pbits;       // bits of a partition id: 2^pbits >= np
   uint32_t W;          // wavefronts of the grid
   const uint32_t* sel;
   uint64_t n;
@@ -62,31 +62,53 @@ __device__ __forceinline__ uint64_t fnv_bytes(uint64_t h, uint64_t v) {
   return h;
 }
 
-// One row's hash (hashRowsIntoPartitions' type switch, :1057-1162): each key
-// column mixes its storage bytes in the reference's vectors — Int32/Port the
-// 4 bytes of Int32Data, Int64/IPv4 the 8 bytes of Int64Data (a compact uint32
-// IPv4 zero-extended), Float64 its 8 bits — or one 0xff byte when NULL; a
-// decimal column takes the default case, one 0x00 byte, NULL or not.
-__device__ __forceinline__ uint64_t shuffle_row_hash(const ShufParams& P, uint64_t row) {
-  uint64_t h = kFnvOffset;
+// The hashes of U rows (hashRowsIntoPartitions' type switch, :1057-1162),
+// key column by key column so that the U rows' loads of a column are in
+// flight together: each key column mixes its storage bytes in the
+// reference's vectors — Int32/Port the 4 bytes of Int32Data, Int64/IPv4 the
+// 8 bytes of Int64Data (a compact uint32 IPv4 zero-extended), Float64 its 8
+// bits — or one 0xff byte when NULL; a decimal column takes the default case,
+// one 0x00 byte, NULL or not.
+template <int U>
+__device__ __forceinline__ void shuffle_row_hashes(const ShufParams& P, const uint64_t (&row)[U], uint64_t (&h)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) h[u] = kFnvOffset;
   for (int k = 0; k < P.n_keys; ++k) {
-    const ColDesc& c = P.key[k];
+    const ColDesc c = P.key[k];
     if (c.type == WG_DECIMAL || c.type == WG_DECIMAL128) {
-      h = (h ^ 0x00ull) * kFnvPrime;
+#pragma unroll
+      for (int u = 0; u < U; ++u) h[u] = (h[u] ^ 0x00ull) * kFnvPrime;
       continue;
     }
-    if (!col_valid(c, row)) {
-      h = (h ^ 0xFFull) * kFnvPrime;
-      continue;
-    }
+    uint64_t v[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) ok[u] = col_valid(c, row[u]);
+    const bool narrow = c.type == WG_INT32 || c.type == WG_UINT16;
     switch (c.type) {
-      case WG_INT32: h = fnv_bytes<4>(h, (uint32_t)((const int32_t*)c.data)[row]); break;
-      case WG_UINT16: h = fnv_bytes<4>(h, ((const uint16_t*)c.data)[row]); break;
-      case WG_UINT32: h = fnv_bytes<8>(h, ((const uint32_t*)c.data)[row]); break;
-      default: h = fnv_bytes<8>(h, ((const uint64_t*)c.data)[row]); break;
+      case WG_INT32:
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = (uint32_t)((const int32_t*)c.data)[row[u]];
+        break;
+      case WG_UINT16:
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = ((const uint16_t*)c.data)[row[u]];
+        break;
+      case WG_UINT32:
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = ((const uint32_t*)c.data)[row[u]];
+        break;
+      default:
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = ((const uint64_t*)c.data)[row[u]];
+        break;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) h[u] = (h[u] ^ 0xFFull) * kFnvPrime;
+      else h[u] = narrow ? fnv_bytes<4>(h[u], v[u]) : fnv_bytes<8>(h[u], v[u]);
     }
   }
-  return h;
 }
 
 // h % np with 32-bit remainders only: h = hi * 2^32 + lo, so
@@ -100,6 +122,8 @@ __device__ __forceinline__ uint32_t part_of(const ShufParams& P, uint64_t h) {
   return (r_hi * P.two32_mod + r_lo) % np;
 }
 
+constexpr int kShufU = 4;  // 64-row steps whose loads are issued together
+
 __global__ void __launch_bounds__(256) shuffle_count_kernel(ShufParams P) {
   extern __shared__ uint32_t hist_all[];
   const int wave = threadIdx.x >> 6, lane = __lane_id();
@@ -109,11 +133,24 @@ __global__ void __launch_bounds__(256) shuffle_count_kernel(ShufParams P) {
   const uint64_t beg = (uint64_t)gw * P.rows_per_wave;
   const uint64_t end = min(beg + P.rows_per_wave, P.n);
   __builtin_amdgcn_wave_barrier();
-  for (uint64_t i = beg + lane; i < end; i += 64) {
-    const uint64_t row = P.sel ? (uint64_t)P.sel[i] : i;
-    const uint32_t p = P.n_keys ? part_of(P, shuffle_row_hash(P, row)) : 0;
-    P.part[i] = p;
-    atomicAdd(&hist[p], 1u);
+  for (uint64_t base = beg; base < end; base += 64 * kShufU) {
+    uint64_t row[kShufU], h[kShufU];
+    bool in[kShufU];
+#pragma unroll
+    for (int u = 0; u < kShufU; ++u) {
+      const uint64_t i = base + 64 * u + lane;
+      in[u] = i < end;
+      const uint64_t ii = in[u] ? i : beg;
+      row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
+    }
+    shuffle_row_hashes<kShufU>(P, row, h);
+#pragma unroll
+    for (int u = 0; u < kShufU; ++u) {
+      if (!in[u]) continue;
+      const uint32_t p = P.n_keys ? part_of(P, h[u]) : 0;
+      P.part[base + 64 * u + lane] = p;
+      atomicAdd(&hist[p], 1u);
+    }
   }
   __builtin_amdgcn_wave_barrier();
   if (gw < P.W)
@@ -131,32 +168,40 @@ __global__ void __launch_bounds__(256) shuffle_scatter_kernel(ShufParams P) {
   const uint64_t end = min(beg + P.rows_per_wave, P.n);
   const uint64_t lt = (1ull << lane) - 1;
   __builtin_amdgcn_wave_barrier();
-  for (uint64_t base = beg; base < end; base += 64) {
-    const uint64_t i = base + lane;
-    const bool in = i < end;
-    const uint32_t p = in ? P.part[i] : kNoPart;
-    // rank among this step's rows of the same partition, and their count;
-    // one round per distinct partition present (a uniform loop)
-    uint64_t left = __ballot(in);
-    uint32_t rank = 0, cnt = 0;
-    bool last = false;
-    while (left) {
-      const int l = __ffsll((long long)left) - 1;
-      const uint32_t lp = (uint32_t)__builtin_amdgcn_readlane((int)p, l);
-      const uint64_t m = __ballot(p == lp) & left;
-      if (p == lp) {
-        rank = (uint32_t)__popcll(m & lt);
-        cnt = (uint32_t)__popcll(m);
-        last = rank + 1 == cnt;
-      }
-      left &= ~m;
+  for (uint64_t base = beg; base < end; base += 64 * kShufU) {
+    uint32_t part[kShufU], src[kShufU];
+#pragma unroll
+    for (int u = 0; u < kShufU; ++u) {
+      const uint64_t i = base + 64 * u + lane;
+      const bool in = i < end;
+      part[u] = in ? P.part[i] : kNoPart;
+      src[u] = in ? (P.sel ? P.sel[i] : (uint32_t)i) : 0;
     }
-    // every lane reads its cursor before the last lane of each group moves it
-    const uint32_t b = in ? cur[p] : 0;
-    __builtin_amdgcn_wave_barrier();
-    if (in && last) cur[p] = b + cnt;
-    if (in) P.rows[b + rank] = P.sel ? P.sel[i] : (uint32_t)i;
-    __builtin_amdgcn_wave_barrier();
+    // the steps in row order; within a step, the lanes holding the same
+    // partition as this lane: one ballot per partition-id bit narrows the
+    // wave to them (lanes whose id agrees on every bit), so a row's place
+    // among the step's rows of its partition is the count of such lanes
+    // below it
+#pragma unroll
+    for (int u = 0; u < kShufU; ++u) {
+      const uint32_t p = part[u];
+      const bool in = p != kNoPart;
+      uint64_t same = __ballot(in);
+      for (int b = 0; b < P.pbits; ++b) {
+        const bool bit = (p >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        same &= bit ? bal : ~bal;
+      }
+      const uint32_t rank = (uint32_t)__popcll(same & lt);
+      const uint32_t cnt = (uint32_t)__popcll(same);
+      const bool last = rank + 1 == cnt;
+      // every lane reads its cursor before the last lane of each group moves it
+      const uint32_t b = in ? cur[p] : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (in && last) cur[p] = b + cnt;
+      if (in) P.rows[b + rank] = src[u];
+      __builtin_amdgcn_wave_barrier();
+    }
   }
 }
 
@@ -168,7 +213,6 @@ __global__ void shuffle_offsets_kernel(const uint32_t* offs, uint32_t np, uint32
     out[p] = p < np ? (uint64_t)offs[(uint64_t)p * W] : total;
 }
 
-}  // namespace
 }  // namespace wg
 
 using namespace wg;
@@ -202,16 +246,17 @@ wg_status wg_shuffle_partition(wg_ctx* ctx, const wg_column* cols, int32_t n_col
   P.np = np;
   P.np_mask = (np & (np - 1)) == 0 ? np - 1 : 0;
   P.two32_mod = (uint32_t)((1ull << 32) % np);
+  while ((1u << P.pbits) < np) ++P.pbits;
   P.sel = sel;
   P.n = n;
   if (n == 0) {
     for (uint32_t p = 0; p <= np; ++p) offsets[p] = 0;
     return WG_OK;
   }
-  // wavefronts: ~8 per CU for large batches, >= 1024 rows each, and a
+  // wavefronts: 32 per CU for large batches, >= 1024 rows each, and a
   // (partition x wavefront) count table of at most 2^25 entries
   const int wpb = np <= 4096 ? 4 : np <= 8192 ? 2 : 1;  // wavefronts per workgroup (LDS <= 64 KiB)
-  uint64_t W = std::min<uint64_t>((uint64_t)ctx->cus * 8, (n + 1023) / 1024);
+  uint64_t W = std::min<uint64_t>((uint64_t)ctx->cus * 32, (n + 1023) / 1024);
   W = std::min<uint64_t>(W, std::max<uint64_t>(1, (1ull << 25) / np));
   W = std::max<uint64_t>(W, 1);
   const uint64_t blocks = (W + wpb - 1) / wpb;

@@ -237,6 +237,11 @@ class HashAggregate:
         call("wg_agg_footprint", self.h, C.byref(n))
         return n.value
 
+    def state_bytes(self) -> int:
+        n = C.c_uint64()
+        call("wg_agg_state_bytes", self.h, C.byref(n))
+        return n.value
+
     def stats(self) -> dict:
         v = [C.c_uint64() for _ in range(4)]
         call("wg_agg_stats", self.h, *[C.byref(x) for x in v])
@@ -422,13 +427,17 @@ def gather(ctx: Context, col: Col, rows: DeviceBuffer, n: int, out: DeviceBuffer
 
 
 def shuffle_partition(ctx: Context, cols: Sequence[Col], key_cols: Sequence[int], num_parts: int, n_rows: int,
-                      sel: Optional[DeviceBuffer] = None, n_sel: int = 0, want_parts: bool = True):
+                      sel: Optional[DeviceBuffer] = None, n_sel: int = 0, want_parts: bool = True,
+                      parts: Optional[DeviceBuffer] = None, rows: Optional[DeviceBuffer] = None):
     """Routing of a partitioned shuffle sink on the device (wg_shuffle_partition):
     returns (parts device buffer or None, rows device buffer, offsets host
-    int64 array of num_parts + 1)."""
+    int64 array of num_parts + 1). parts / rows may be passed in (n active
+    entries of 4 bytes each)."""
     n = n_sel if sel is not None else n_rows
-    parts = ctx.alloc(max(n, 1) * 4) if want_parts else None
-    rows = ctx.alloc(max(n, 1) * 4)
+    if parts is None and want_parts:
+        parts = ctx.alloc(max(n, 1) * 4)
+    if rows is None:
+        rows = ctx.alloc(max(n, 1) * 4)
     offsets = np.zeros(max(num_parts, 1) + 1, np.uint64)
     arr = columns(cols)
     k = (C.c_int32 * max(len(key_cols), 1))(*key_cols)

@@ -2,6 +2,7 @@
 #include "exec.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cstring>
 #include <thread>
 
@@ -743,6 +744,200 @@ Status GpuBloomFilterOp::Execute(Context&, const BatchPtr& in, BatchPtr* out) {
   TRY(wrap(wg_memcpy_d2h(c, o->sel.data(), d_out_, got * 4), "download sel"));
   o->has_sel = true;
   *out = o;
+  return Status::OK();
+}
+
+// ------------------------------------------------------ GpuHashAggregate
+
+int64_t GpuHashAggregate::StateBytes() {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (!h_) return 0;
+  uint64_t b = 0;
+  if (wg_agg_state_bytes(h_, &b) != WG_OK) return 0;
+  return (int64_t)b;
+}
+
+// ------------------------------------------------------ GpuShuffleRouter
+
+GpuShuffleRouter::GpuShuffleRouter(wg_ctx* ctx, std::vector<std::string> keys, int num_parts)
+    : ctx_(ctx), keys_(std::move(keys)), num_parts_(num_parts), counts_(std::max(num_parts, 1), 0) {}
+
+GpuShuffleRouter::~GpuShuffleRouter() {
+  for (void* p : buf_) (void)wg_free(ctx_, p);
+}
+
+// grow-only device scratch slot i
+void* GpuShuffleRouter::scratch(size_t i, uint64_t bytes) {
+  if (buf_.size() <= i) {
+    buf_.resize(i + 1, nullptr);
+    cap_.resize(i + 1, 0);
+  }
+  if (cap_[i] < bytes) {
+    if (buf_[i]) (void)wg_free(ctx_, buf_[i]);
+    buf_[i] = nullptr;
+    cap_[i] = 0;
+    if (wg_malloc(ctx_, bytes, &buf_[i]) != WG_OK) return nullptr;
+    cap_[i] = bytes;
+  }
+  return buf_[i];
+}
+
+Status GpuShuffleRouter::Route(Context& ctx, const BatchPtr& b, std::vector<std::vector<uint32_t>>* per_part_rows,
+                               std::vector<int>* parts) {
+  if (ctx.cancelled) return Status::Error("context canceled");
+  if (!resolved_) {  // keyOnce: resolve key names against the first batch
+    for (auto& k : keys_) {
+      const int ci = b->column_index(k);
+      if (ci < 0) return Status::Error("partitioned shuffle: key \"" + k + "\" not in schema");
+      key_idx_.push_back(ci);
+    }
+    resolved_ = true;
+  }
+  const int np = std::max(num_parts_, 1);
+  per_part_rows->assign(np, {});
+  const uint64_t n = (uint64_t)b->active();
+  if (parts) parts->assign(n, 0);
+  if (n == 0) return Status::OK();
+  // key columns and Sel to the device
+  std::vector<wg_column> cols;
+  size_t slot = 0;
+  for (int ci : key_idx_) {
+    const Vector& v = *b->columns[ci];
+    const uint64_t bytes = (uint64_t)std::max(b->len, 1) * elem_bytes(v.type);
+    void* d = scratch(slot++, bytes);
+    if (!d) return Status::Error(std::string("wg_malloc: ") + wg_last_error());
+    TRY(wrap(wg_memcpy_h2d(ctx_, d, data_of(v), (uint64_t)b->len * elem_bytes(v.type)), "upload key"));
+    wg_column c = col_of(v);
+    c.data = d;
+    if (v.nulls.has_nulls()) {
+      void* dv = scratch(slot++, v.nulls.words.size() * 8);
+      if (!dv) return Status::Error(std::string("wg_malloc: ") + wg_last_error());
+      TRY(wrap(wg_memcpy_h2d(ctx_, dv, v.nulls.words.data(), v.nulls.words.size() * 8), "upload validity"));
+      c.validity = (const uint64_t*)dv;
+    }
+    cols.push_back(c);
+  }
+  const size_t s_sel = 2 * key_idx_.size(), s_part = s_sel + 1, s_rows = s_sel + 2;
+  uint32_t* d_sel = nullptr;
+  if (b->has_sel) {
+    d_sel = (uint32_t*)scratch(s_sel, n * 4);
+    if (!d_sel) return Status::Error(std::string("wg_malloc: ") + wg_last_error());
+    TRY(wrap(wg_memcpy_h2d(ctx_, d_sel, b->sel.data(), n * 4), "upload sel"));
+  }
+  uint32_t* d_part = parts ? (uint32_t*)scratch(s_part, n * 4) : nullptr;
+  uint32_t* d_rows = (uint32_t*)scratch(s_rows, n * 4);
+  if (!d_rows || (parts && !d_part)) return Status::Error(std::string("wg_malloc: ") + wg_last_error());
+  std::vector<int32_t> kidx(cols.size());
+  for (size_t k = 0; k < cols.size(); ++k) kidx[k] = (int32_t)k;
+  std::vector<uint64_t> offs(np + 1);
+  TRY(wrap(wg_shuffle_partition(ctx_, cols.data(), (int32_t)cols.size(), kidx.data(), (int32_t)kidx.size(),
+                                (uint64_t)b->len, d_sel, b->has_sel ? n : 0, num_parts_, d_part, d_rows, offs.data()),
+           "wg_shuffle_partition"));
+  std::vector<uint32_t> rows(n);
+  TRY(wrap(wg_memcpy_d2h(ctx_, rows.data(), d_rows, n * 4), "download rows"));
+  for (int p = 0; p < np; ++p) {
+    (*per_part_rows)[p].assign(rows.begin() + offs[p], rows.begin() + offs[p + 1]);
+    counts_[p] += (int64_t)(offs[p + 1] - offs[p]);
+  }
+  if (parts) {
+    std::vector<uint32_t> pp(n);
+    TRY(wrap(wg_memcpy_d2h(ctx_, pp.data(), d_part, n * 4), "download parts"));
+    for (uint64_t i = 0; i < n; ++i) (*parts)[i] = (int)pp[i];
+  }
+  return Status::OK();
+}
+
+// ------------------------------------------------------ CappedPartialAgg
+
+// parseAggFuncString (worker/executor_stage.go:251-300) for the functions the
+// GPU aggregate carries; false for anything else (the operator then passes
+// rows through, as it does for an unknown function).
+static bool gpu_agg_func(const std::string& name, wg_agg_func* f) {
+  std::string s;
+  for (char c : name)
+    if (c != ' ' && c != '\t' && c != '\n') s.push_back((char)std::tolower((unsigned char)c));
+  if (s == "sum") *f = WG_AGG_SUM;
+  else if (s == "count") *f = WG_AGG_COUNT;
+  else if (s == "min") *f = WG_AGG_MIN;
+  else if (s == "max") *f = WG_AGG_MAX;
+  else if (s == "avg") *f = WG_AGG_AVG;
+  else return false;
+  return true;
+}
+
+CappedPartialAgg::CappedPartialAgg(wg_ctx* ctx, std::vector<std::string> keys, std::vector<AggSpec> specs,
+                                   std::vector<std::string> partition_keys, int64_t cap_bytes)
+    : ctx_(ctx), group_by_(std::move(keys)), partition_keys_(std::move(partition_keys)),
+      cap_bytes_(cap_bytes <= 0 ? kDefaultCapBytes : cap_bytes) {
+  bool unknown = false;
+  for (auto& s : specs) {
+    wg_agg_func f = WG_AGG_SUM;
+    if (!gpu_agg_func(s.func, &f)) unknown = true;
+    aggs_.push_back(AggColumn{f, s.input_col, s.output_col, ""});
+  }
+  resolved_ = disabled_ = unknown;  // newCappedPartialAggPartitioned (:85-113)
+}
+
+// resolveAgainst (:117-151)
+void CappedPartialAgg::resolve_against(const RecordBatch& b) {
+  resolved_ = true;
+  auto present = [&](const std::string& name) {
+    for (auto& c : b.schema)
+      if (c.name == name) return true;
+    return false;
+  };
+  for (auto& k : partition_keys_)
+    if (!present(k)) {
+      disabled_ = true;
+      return;
+    }
+  std::vector<std::string> kept;
+  for (auto& k : group_by_) {
+    if (present(k)) kept.push_back(k);
+    else ++dropped_keys_;
+  }
+  group_by_ = kept;
+  std::vector<AggColumn> kept_aggs;
+  for (auto& a : aggs_)
+    if (present(a.input_col)) kept_aggs.push_back(a);
+  aggs_ = kept_aggs;
+  if (aggs_.empty() || group_by_.empty()) disabled_ = true;
+}
+
+Status CappedPartialAgg::Consume(Context& ctx, const BatchPtr& b, std::vector<BatchPtr>* flushed) {
+  flushed->clear();
+  if (!resolved_) resolve_against(*b);
+  in_rows += b->active();
+  if (disabled_) {  // raw passthrough
+    out_rows += b->active();
+    flushed->push_back(b);
+    return Status::OK();
+  }
+  if (!agg_) agg_ = std::make_unique<GpuHashAggregate>(ctx_, group_by_, aggs_);
+  TRY(agg_->Consume(ctx, b));
+  if (agg_->StateBytes() < cap_bytes_) return Status::OK();
+  return flush(ctx, flushed);
+}
+
+Status CappedPartialAgg::Drain(Context& ctx, std::vector<BatchPtr>* flushed) {
+  flushed->clear();
+  if (!agg_) return Status::OK();
+  return flush(ctx, flushed);
+}
+
+// flush (:207-234): finalize the epoch, collect its batches, start afresh
+Status CappedPartialAgg::flush(Context& ctx, std::vector<BatchPtr>* out) {
+  TRY(agg_->Finalize(ctx));
+  for (;;) {
+    BatchPtr b;
+    TRY(agg_->Next(ctx, &b));
+    if (!b) break;
+    out_rows += b->active();
+    out->push_back(b);
+  }
+  TRY(agg_->Close());
+  agg_.reset();
+  ++flushes;
   return Status::OK();
 }
 

@@ -10,6 +10,8 @@
 //   GpuHashJoin        Build(ctx, Source) / Probe()      (join.go:771, :1967)
 //   GpuHashJoinProbe   UnaryOperator + BoundedOutput     (join.go:2072-2139, :2306-2515)
 //   GpuBloomFilterOp   UnaryOperator + Cloneable         (bloom_filter_op.go:13-274)
+//   GpuShuffleRouter   partitioned shuffle routing       (worker/partitioned_shuffle_sink.go:199-290)
+//   CappedPartialAgg   exchange partial aggregation      (worker/shuffle_partial_agg.go:36-234)
 // and the CPU pieces a pipeline needs around them: KernelFilter (filter.go:347-425),
 // BatchSource, CollectSink and Pipeline{Source, Ops, Sink, Workers}
 // (pipeline.go:31-690). Errors are returned as Status (Go's `error`).
@@ -231,6 +233,8 @@ class GpuHashAggregate : public MergeableSink {
   Status AddPredicate(const KernelFilter& f);
   std::vector<batch::Column> OutputSchema() const;
   wg_agg* handle() const { return h_; }
+  // StateBytes (aggregate.go:4050-4054): bytes of live group state
+  int64_t StateBytes();
 
  private:
   Status resolve(const RecordBatch& b);  // typed dispatch on the first batch (resolveIndices)
@@ -333,6 +337,79 @@ class GpuBloomFilterOp : public UnaryOperator, public Cloneable {
   uint32_t* d_sel_ = nullptr;
   uint32_t* d_out_ = nullptr;
   uint64_t cap_rows_ = 0;
+};
+
+// ------------------------------------------------------------ shuffle
+
+// The routing half of partitionedShuffleSink (worker/partitioned_shuffle_sink.go:
+// 153-290): key columns resolved by name on the first batch (keyOnce), every
+// active row hashed to a partition on the device (hashRowsIntoPartitions,
+// :1039-1171, via wg_shuffle_partition) and, per partition, the batch's rows in
+// input order (perPartRows). Encoding and writing the partition files belongs
+// to the storage layer and stays with the caller.
+class GpuShuffleRouter {
+ public:
+  GpuShuffleRouter(wg_ctx* ctx, std::vector<std::string> keys, int num_parts);
+  ~GpuShuffleRouter();
+  // per_part_rows[p] = source rows (Sel values or positions) routed to p;
+  // parts (optional) = partition of the i-th active row
+  Status Route(Context&, const BatchPtr& b, std::vector<std::vector<uint32_t>>* per_part_rows,
+               std::vector<int>* parts = nullptr);
+  // rows routed to each partition so far (PartitionRowCounts, :1006-1019)
+  const std::vector<int64_t>& PartitionRowCounts() const { return counts_; }
+  int NumParts() const { return num_parts_; }
+
+ private:
+  void* scratch(size_t i, uint64_t bytes);
+  wg_ctx* ctx_;
+  std::vector<std::string> keys_;
+  int num_parts_;
+  std::vector<int> key_idx_;
+  bool resolved_ = false;
+  std::vector<int64_t> counts_;
+  std::vector<void*> buf_;
+  std::vector<uint64_t> cap_;
+};
+
+// distributed.AggSpec as the exchange carries it: the function by name.
+struct AggSpec {
+  std::string func;  // "sum", "min", "max", ... (parseAggFuncString)
+  std::string input_col;
+  std::string output_col;
+};
+
+// cappedPartialAgg (worker/shuffle_partial_agg.go:36-234): pre-combines a
+// shuffle task's rows on the exchange's keys before they are partitioned.
+// Name-preserving SUM/MIN/MAX specs; once the aggregate's state reaches
+// cap_bytes the epoch is flushed (Finalize + drain of Next) and a fresh one
+// starts, so memory stays bounded and flushed partials stay mergeable. On the
+// first batch keys and specs are intersected with its schema; with nothing
+// combinable left, an unknown function, or an absent partition key the
+// operator passes batches through unchanged.
+class CappedPartialAgg {
+ public:
+  static constexpr int64_t kDefaultCapBytes = 128ll << 20;  // defaultPartialAggCapBytes
+  CappedPartialAgg(wg_ctx* ctx, std::vector<std::string> keys, std::vector<AggSpec> specs,
+                   std::vector<std::string> partition_keys = {}, int64_t cap_bytes = 0);
+  // consume: returns the batches flushed by this call (the input itself when disabled)
+  Status Consume(Context&, const BatchPtr& b, std::vector<BatchPtr>* flushed);
+  // drain: flushes the last epoch at end of stream
+  Status Drain(Context&, std::vector<BatchPtr>* flushed);
+  int64_t in_rows = 0, out_rows = 0, flushes = 0;
+  bool Disabled() const { return disabled_; }
+  int DroppedKeys() const { return dropped_keys_; }
+
+ private:
+  void resolve_against(const RecordBatch& b);
+  Status flush(Context&, std::vector<BatchPtr>* out);
+  wg_ctx* ctx_;
+  std::vector<std::string> group_by_;
+  std::vector<AggColumn> aggs_;
+  std::vector<std::string> partition_keys_;
+  int64_t cap_bytes_;
+  std::unique_ptr<GpuHashAggregate> agg_;
+  bool resolved_ = false, disabled_ = false;
+  int dropped_keys_ = 0;
 };
 
 }  // namespace wadjet
