@@ -17,7 +17,9 @@
 //            ballot per bit of the partition id (lanes in order), and
 //            the wavefront's LDS cursor per partition advances by the step's
 //            count, so every partition keeps input order — the stable
-//            grouping the reference builds with appends.
+//            grouping the reference builds with appends. Up to 1024
+//            partitions a workgroup first counting-sorts tiles of 8192 rows
+//            in LDS and writes each partition's run of the tile together.
 //
 // Bytes per active row: key bytes read twice (count, then the stored 4-byte
 // partition instead of the keys) + 4 B partition written + 4 B row id
@@ -205,6 +207,141 @@ __global__ void __launch_bounds__(256) shuffle_scatter_kernel(ShufParams P) {
   }
 }
 
+// The same scatter for up to kTileMaxParts partitions, by workgroup tiles:
+// a tile of kTileRows consecutive rows is ranked (each wavefront a quarter of
+// it, with the ballot-per-bit ranks and a wavefront-private running count),
+// counting-sorted by partition in LDS, and written out partition run by
+// partition run, so the stores of a run are consecutive addresses (a few
+// dozen rows per partition per tile) instead of one 4-byte store per row per
+// partition front. The workgroup's range is its four wavefronts' ranges of
+// the count pass, so its cursor in partition p starts at offs[p * W + 4 b].
+constexpr uint32_t kTileRows = 8192;
+constexpr uint32_t kTileMaxParts = 1024;
+constexpr int kTileBlock = 256;
+constexpr int kTileQuarter = kTileRows / 4;       // rows of one wavefront per tile
+constexpr int kTileSteps = kTileQuarter / 64;     // 64-row steps of one wavefront per tile
+
+__host__ __device__ inline size_t tile_lds_bytes(uint32_t np) { return (size_t)24 * np + 4 * kTileRows; }
+
+// exclusive prefix sum of one value per thread over a 256-thread workgroup
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* tmp) {
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) tmp[wave] = x;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int w = 0; w < wave; ++w) off += tmp[w];
+  __syncthreads();
+  return off + x - v;
+}
+
+__global__ void __launch_bounds__(kTileBlock) shuffle_scatter_tile_kernel(ShufParams P) {
+  extern __shared__ uint32_t lds_t[];
+  __shared__ uint32_t scan_tmp[kTileBlock / 64];
+  const uint32_t np = P.np;
+  uint32_t* cntw = lds_t;         // [4][np] running counts, then wavefront prefixes
+  uint32_t* tstart = cntw + 4 * np;  // [np] where the tile's run of p starts
+  uint32_t* cur = tstart + np;       // [np] this workgroup's cursor in p
+  uint16_t* st_off = (uint16_t*)(cur + np);  // [kTileRows] row offset in the tile, partition-sorted
+  uint16_t* st_p = st_off + kTileRows;       // [kTileRows] its partition
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  const uint64_t lt = (1ull << lane) - 1;
+  for (uint32_t p = threadIdx.x; p < np; p += kTileBlock) cur[p] = P.offs[(uint64_t)p * P.W + 4ull * blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < 4 * np; i += kTileBlock) cntw[i] = 0;
+  const uint64_t beg = (uint64_t)blockIdx.x * 4 * P.rows_per_wave;
+  const uint64_t end = min(beg + 4 * P.rows_per_wave, P.n);
+  uint32_t* myc = cntw + (uint64_t)wave * np;
+  __syncthreads();
+  for (uint64_t tb = beg; tb < end; tb += kTileRows) {
+    const uint32_t trows = (uint32_t)min((uint64_t)kTileRows, end - tb);
+    // a. rank this wavefront's quarter: (partition << 16) | rank in the quarter
+    uint32_t pr[kTileSteps];
+#pragma unroll
+    for (int s = 0; s < kTileSteps; ++s) {
+      const uint32_t off = wave * kTileQuarter + s * 64 + lane;
+      pr[s] = off < trows ? P.part[tb + off] : kNoPart;
+    }
+#pragma unroll
+    for (int s = 0; s < kTileSteps; ++s) {
+      const uint32_t p = pr[s];
+      const bool in = p != kNoPart;
+      uint64_t same = __ballot(in);
+      for (int b = 0; b < P.pbits; ++b) {
+        const bool bit = (p >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        same &= bit ? bal : ~bal;
+      }
+      const uint32_t rank = (uint32_t)__popcll(same & lt);
+      const uint32_t cnt = (uint32_t)__popcll(same);
+      const uint32_t base = in ? myc[p] : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (in && rank + 1 == cnt) myc[p] = base + cnt;
+      __builtin_amdgcn_wave_barrier();
+      pr[s] = in ? (p << 16) | (base + rank) : kNoPart;
+    }
+    __syncthreads();
+    // b. per partition: wavefront prefixes, the tile's count and run start
+    uint32_t tot[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t p = 4 * threadIdx.x + q;
+      tot[q] = 0;
+      if (p < np) {
+        const uint32_t c0 = cntw[p], c1 = cntw[np + p], c2 = cntw[2 * np + p], c3 = cntw[3 * np + p];
+        cntw[p] = 0;
+        cntw[np + p] = c0;
+        cntw[2 * np + p] = c0 + c1;
+        cntw[3 * np + p] = c0 + c1 + c2;
+        tot[q] = c0 + c1 + c2 + c3;
+      }
+    }
+    const uint32_t ex = block_exclusive_sum(tot[0] + tot[1] + tot[2] + tot[3], scan_tmp);
+    {
+      uint32_t run = ex;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t p = 4 * threadIdx.x + q;
+        if (p < np) tstart[p] = run;
+        run += tot[q];
+      }
+    }
+    __syncthreads();
+    // c. counting-sort the tile's rows into LDS
+#pragma unroll
+    for (int s = 0; s < kTileSteps; ++s) {
+      if (pr[s] == kNoPart) continue;
+      const uint32_t p = pr[s] >> 16, r = pr[s] & 0xFFFFu;
+      const uint32_t pos = tstart[p] + myc[p] + r;
+      st_off[pos] = (uint16_t)(wave * kTileQuarter + s * 64 + lane);
+      st_p[pos] = (uint16_t)p;
+    }
+    __syncthreads();
+    // d. write the runs: consecutive positions of a partition are
+    // consecutive rows of its output
+    for (uint32_t j = threadIdx.x; j < trows; j += kTileBlock) {
+      const uint32_t p = st_p[j];
+      const uint64_t i = tb + st_off[j];
+      P.rows[cur[p] + (j - tstart[p])] = P.sel ? P.sel[i] : (uint32_t)i;
+    }
+    __syncthreads();
+    // e. advance the cursors, clear the running counts
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t p = 4 * threadIdx.x + q;
+      if (p < np) {
+        cur[p] += tot[q];
+        cntw[p] = cntw[np + p] = cntw[2 * np + p] = cntw[3 * np + p] = 0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // offsets[p] = offs[p * W] (the first wavefront's start in partition p),
 // offsets[np] = total rows.
 __global__ void shuffle_offsets_kernel(const uint32_t* offs, uint32_t np, uint32_t W, uint64_t total,
@@ -284,7 +421,12 @@ wg_status wg_shuffle_partition(wg_ctx* ctx, const wg_column* cols, int32_t n_col
   hipLaunchKernelGGL(shuffle_count_kernel, dim3(blocks), dim3(64 * wpb), lds, ctx->stream, P);
   WG_HIP(hipGetLastError());
   WG_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, P.counts, offs, (int)ncnt, ctx->stream));
-  hipLaunchKernelGGL(shuffle_scatter_kernel, dim3(blocks), dim3(64 * wpb), lds, ctx->stream, P);
+  if (np <= kTileMaxParts && wpb == 4) {
+    hipLaunchKernelGGL(shuffle_scatter_tile_kernel, dim3(blocks), dim3(kTileBlock), tile_lds_bytes(np), ctx->stream,
+                       P);
+  } else {
+    hipLaunchKernelGGL(shuffle_scatter_kernel, dim3(blocks), dim3(64 * wpb), lds, ctx->stream, P);
+  }
   WG_HIP(hipGetLastError());
   hipLaunchKernelGGL(shuffle_offsets_kernel, dim3((np + 256) / 256), dim3(256), 0, ctx->stream, offs, np,
                      (uint32_t)W, n, doff);
