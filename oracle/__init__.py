@@ -22,6 +22,11 @@ Pinning (how each function is checked, see tests/test_oracle.py):
                                                      (aggregate_partial_spill_test.go:289-392), the
                                                      numGroups offset Close() patches (:259-290),
                                                      bytes.Compare order of decimal-text keys
+  bloom_build / bloom_filter                         no false negatives, the size rule, bit positions
+                                                     recomputed per key (tests/test_bloom_oracle.py)
+  fnv1a64 / shuffle_hash / shuffle_partition         published FNV-1a 64 vectors, a byte-at-a-time FNV-1a
+                                                     over each row's key bytes, the reference's routing
+                                                     test (partitioned_shuffle_sink_test.go:19-78)
 """
 from __future__ import annotations
 
