@@ -471,10 +471,11 @@ __global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_b
 // Unpack the key column k of groups [off, off+cnt) (writeIntKeyToColumn,
 // aggregate_partial_spill.go:2006-2021; packedField.get, packed_hash.go:342-351).
 __global__ void agg_key_kernel(const uint64_t* dense, const uint8_t* dense_null, uint64_t off, uint64_t cnt,
-                               int stride, KeyDesc K, int k, void* out, uint8_t* valid_out) {
+                               int stride, KeyDesc K, int k, void* out, uint8_t* valid_out, const uint64_t* dict) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cnt; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t w = dense[(off + j) * stride];
+    uint64_t w = dense[(off + j) * stride];
     const bool is_null = dense_null[off + j] != 0;
+    if (dict && !is_null) w = dict[2 * w + K.word[k]];  // 128-bit keys: the id's {lo, hi} word
     int64_t v;
     if (K.n == 1) v = (int64_t)w;
     else if (K.narrow[k] == 1) v = (int64_t)(int32_t)(uint32_t)(w >> K.shift[k]);
@@ -655,7 +656,38 @@ static wg_status compile_key(wg_agg* h, int n_keys, const int32_t* key_cols) {
     bits += packed_width(t) * 8;
   }
   if (n_keys == 1) return WG_OK;
-  WG_CHECK(bits <= 64, WG_ERR_UNSUPPORTED, "composite key of %d bits: the GPU table packs <= 64", bits);
+  WG_CHECK(bits <= 128, WG_ERR_UNSUPPORTED, "composite key of %d bits: the GPU path packs <= 128", bits);
+  if (bits > 64) {
+    // {lo, hi} placement, widest first (buildPackedLayout, packed_hash.go:
+    // 384-425): 8-byte columns take whole words, then 4-byte columns fill the
+    // free half-words in (word, shift) order; the table keys on the pair's
+    // dictionary id, held in a per-row column after the batch's own
+    WG_CHECK(h->n_cols < kMaxCols, WG_ERR_UNSUPPORTED, "a 128-bit key needs a spare column slot (%d columns)",
+             h->n_cols);
+    KeyDesc W = K;
+    int next_wide = 0;
+    for (int i = 0; i < n_keys; ++i)
+      if (packed_width(W.key_type[i]) == 8) {
+        W.word[i] = (uint8_t)next_wide++;
+        W.shift[i] = 0;
+        W.narrow[i] = 0;
+      }
+    int slot = 0;
+    for (int i = 0; i < n_keys; ++i) {
+      if (packed_width(W.key_type[i]) != 4) continue;
+      W.word[i] = (uint8_t)(next_wide + slot / 2);
+      W.shift[i] = (uint8_t)(32 * (slot % 2));
+      W.narrow[i] = W.key_type[i] == WG_INT32 ? 1 : 2;
+      ++slot;
+    }
+    h->wide = true;
+    h->wkey = W;
+    K = KeyDesc{};
+    K.n = 1;
+    K.col[0] = h->n_cols;
+    K.key_type[0] = WG_INT64;
+    return WG_OK;
+  }
   // 8-byte columns take the whole word (only possible alone); 4-byte columns
   // fill the low then the high half (buildPackedLayout, packed_hash.go:384-425)
   int shift = 0;
@@ -677,6 +709,7 @@ uint64_t agg_footprint(const wg_agg* h) {
   b += h->dense_cap * ((uint64_t)h->pg.stride * 8 + (uint64_t)h->n_sumd * 16 + 1);
   for (void* p : h->stage.dev) (void)p;
   b += h->stage.cap_rows * 16 * (uint64_t)h->stage.dev.size();
+  if (h->wide) b += h->dict_cap * 32 + h->dict_keys_cap * 16 + h->ids_cap * 8;
   return b;
 }
 
@@ -1052,6 +1085,7 @@ wg_status agg_flush_stage(wg_agg* h) {
   const PredDesc none[2] = {{-1, 0, 0, 0.0}, {-1, 0, 0, 0.0}};
   const uint64_t n = S.rows;
   S.rows = 0;
+  if (h->wide) WG_TRY(wide_attach(h, cs, nullptr, n, n));
   // consume syncs on its counter read-back, after which the pinned buffers are free
   WG_TRY(agg_consume_device(h, cs, nullptr, n, none));
   for (int c = 0; c < h->n_cols; ++c)
@@ -1100,9 +1134,10 @@ static wg_status consume_host(wg_agg* h, const wg_column* cols, uint64_t n_rows,
   HostStage& S = h->stage;
   const uint64_t n = sel ? n_sel : n_rows;
   // multi-column keys: a NULL key is handed back before any state changes
-  if (h->key.n > 1) {
-    for (int k = 0; k < h->key.n; ++k) {
-      const wg_column& kc = cols[h->key.col[k]];
+  const KeyDesc& UK = h->wide ? h->wkey : h->key;
+  if (UK.n > 1) {
+    for (int k = 0; k < UK.n; ++k) {
+      const wg_column& kc = cols[UK.col[k]];
       if (!kc.validity) continue;
       for (uint64_t i = 0; i < n; ++i)
         if (!host_valid(kc, sel ? sel[i] : i)) {
@@ -1181,6 +1216,7 @@ wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types, c
   const uint64_t hint = std::max<uint64_t>(group_hint, 1024);
   h->group_hint = group_hint;
   st = alloc_table(h, next_pow2(hint + hint / 4));
+  if (st == WG_OK && h->wide) st = wide_init(h);
   if (st != WG_OK) {
     wg_agg_destroy(h);
     return st;
@@ -1215,6 +1251,7 @@ wg_status wg_agg_destroy(wg_agg* h) {
   for (uint64_t* p : h->stage.pvalid) (void)hipHostFree(p);
   for (void* p : h->stage.dev) (void)hipFree(p);
   for (uint64_t* p : h->stage.dvalid) (void)hipFree(p);
+  if (h->wide) (void)wide_free(h);
   delete h;
   return WG_OK;
 }
@@ -1249,9 +1286,10 @@ wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, cons
   WG_CHECK(mem == WG_MEM_DEVICE, WG_ERR_INVALID, "bad mem kind %d", mem);
   const uint64_t n = sel ? n_sel : n_rows;
   if (n == 0) return WG_OK;
-  if (h->key.n > 1) {
-    for (int k = 0; k < h->key.n; ++k) {
-      const wg_column& kc = cols[h->key.col[k]];
+  const KeyDesc& UK = h->wide ? h->wkey : h->key;
+  if (UK.n > 1) {
+    for (int k = 0; k < UK.n; ++k) {
+      const wg_column& kc = cols[UK.col[k]];
       if (!kc.validity) continue;
       uint64_t* flag = h->counters + CTR_STOPPED;
       WG_TRY(zero_dev(h->ctx, flag, 1));
@@ -1273,6 +1311,7 @@ wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, cons
     cs.inline_word[c] = -1;
   }
   WG_TRY(agg_flush_stage(h));
+  if (h->wide) WG_TRY(wide_attach(h, cs, sel, n, n_rows));
   return agg_consume_device(h, cs, sel, n, pd);
 }
 
@@ -1282,8 +1321,22 @@ wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
   WG_CHECK(!dst->finalized && !src->finalized, WG_ERR_STATE, "merge after finalize");
   WG_CHECK(dst->pg.stride == src->pg.stride && dst->pg.n_ops == src->pg.n_ops && dst->ctx == src->ctx,
            WG_ERR_INVALID, "merge of aggregates with different specs");
+  WG_CHECK(dst->wide == src->wide, WG_ERR_INVALID, "merge of aggregates with different keys");
   WG_TRY(agg_flush_stage(dst));
   WG_TRY(agg_flush_stage(src));
+  if (dst->wide) {
+    // src's keys are ids of its own dictionary: rewrite them as dst's ids
+    for (int k = 0; k < dst->wkey.n; ++k)
+      WG_CHECK(dst->wkey.word[k] == src->wkey.word[k] && dst->wkey.shift[k] == src->wkey.shift[k] &&
+                   dst->wkey.narrow[k] == src->wkey.narrow[k],
+               WG_ERR_INVALID, "merge of aggregates with different key layouts");
+    uint64_t* remapped = nullptr;
+    WG_TRY(wide_remap_slots(dst, src, &remapped));
+    const wg_status st = agg_merge_slots(dst, remapped, src->cap, src->base, src->groups);
+    (void)hipFree(remapped);
+    WG_TRY(st);
+    return wg_agg_reset(src);
+  }
   WG_TRY(agg_merge_slots(dst, src->slots, src->cap, src->base, src->groups));
   return wg_agg_reset(src);
 }
@@ -1347,14 +1400,15 @@ wg_status wg_agg_fetch(wg_agg* h, uint64_t offset, uint64_t count, void* const* 
   };
   wg_status st = WG_OK;
   const int g = grid_for(count, 256);
-  for (int k = 0; k < h->key.n && st == WG_OK; ++k) {
+  const KeyDesc& UK = h->wide ? h->wkey : h->key;
+  for (int k = 0; k < UK.n && st == WG_OK; ++k) {
     void* kout = key_out ? key_out[k] : nullptr;
     if (!kout) continue;
     hipLaunchKernelGGL(agg_key_kernel, dim3(g), dim3(256), 0, h->ctx->stream, (const uint64_t*)h->dense,
-                       (const uint8_t*)h->dense_null, offset, count, h->pg.stride, h->key, k, host ? tmp : kout,
-                       (uint8_t*)nullptr);
+                       (const uint8_t*)h->dense_null, offset, count, h->pg.stride, UK, k, host ? tmp : kout,
+                       (uint8_t*)nullptr, h->wide ? (const uint64_t*)h->dict_keys : (const uint64_t*)nullptr);
     if (hipGetLastError() != hipSuccess) { set_error("agg_key_kernel launch failed"); st = WG_ERR_DEVICE; break; }
-    st = finish(kout, tmp, count * type_bytes(h->key.key_type[k]));
+    st = finish(kout, tmp, count * type_bytes(UK.key_type[k]));
   }
   for (size_t a = 0; a < h->finals.size() && st == WG_OK; ++a) {
     void* vout = val_out ? val_out[a] : nullptr;
@@ -1460,6 +1514,7 @@ wg_status wg_agg_reset(wg_agg* h) {
   h->rows_since_norm = 0;
   h->finalized = false;
   h->out_n = 0;
+  if (h->wide) WG_TRY(wide_reset(h));
   return WG_OK;
 }
 

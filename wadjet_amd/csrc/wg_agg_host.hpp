@@ -105,6 +105,20 @@ struct wg_agg {
   uint64_t dense_cap = 0;
   // host staging
   wg::HostStage stage;
+  // composite keys of 65-128 bits: the key columns pack into a {lo, hi} pair
+  // (buildPackedLayout, packed_hash.go:384-425) and a device dictionary maps
+  // each distinct pair to a dense 64-bit id; the id is the table's key (`key`
+  // then names the per-row id column, index n_cols of the consumed ColSet)
+  bool wide = false;
+  wg::KeyDesc wkey{};              // the key columns' places in {lo, hi}
+  uint64_t* dict = nullptr;        // dict_cap slots of 4 words {state or id, lo, hi, -}
+  uint64_t dict_cap = 0;
+  uint64_t* dict_keys = nullptr;   // id -> {lo, hi}
+  uint64_t dict_keys_cap = 0;      // ids that fit
+  uint64_t dict_n = 0;             // ids handed out (as of the last read-back)
+  uint64_t* dict_ctr = nullptr;    // device: [0] ids handed out, [1] overflow flag
+  uint64_t* ids = nullptr;         // per-row ids of the batch being consumed
+  uint64_t ids_cap = 0;
 };
 
 namespace wg {
@@ -128,5 +142,13 @@ wg_status agg_merge_partials(wg_agg* h, const uint64_t* recs, uint64_t n);
 wg_status agg_merge_slots(wg_agg* h, const uint64_t* src, uint64_t src_cap, const uint64_t* src_base,
                           uint64_t src_groups);
 bool part_eligible(const wg_agg* h, const ColSet& cs);
+// keys wider than 64 bits (wg_agg_wide.hip)
+wg_status wide_init(wg_agg* h);
+wg_status wide_free(wg_agg* h);
+wg_status wide_reset(wg_agg* h);
+wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, uint64_t n_rows);
+wg_status wide_encode_pairs(wg_agg* h, const uint64_t* pairs, uint64_t n, uint64_t* ids_out);
+wg_status wide_dict_host(wg_agg* h, std::vector<uint64_t>* pairs);
+wg_status wide_remap_slots(wg_agg* dst, wg_agg* src, uint64_t** remapped);
 wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred);
 }  // namespace wg

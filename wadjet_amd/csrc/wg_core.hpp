@@ -156,6 +156,7 @@ struct KeyDesc {
   int32_t col[kMaxKeys];
   uint8_t shift[kMaxKeys];
   uint8_t narrow[kMaxKeys];  // 0 = whole word, 1 = 32 bits sign-extended, 2 = 32 bits zero-extended
+  uint8_t word[kMaxKeys];    // keys wider than 64 bits: 0 = lo, 1 = hi word of the {lo, hi} pair
   int32_t key_type[kMaxKeys];
 };
 

@@ -1008,6 +1008,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
   for (int c = 0; c < j->n_cols; ++c)
     WG_CHECK(j->types[c] == agg->col_types[n_probe_cols + c], WG_ERR_INVALID, "build column %d type", c);
   WG_CHECK(is_int_class(probe_cols[probe_key].type), WG_ERR_UNSUPPORTED, "probe key type");
+  WG_CHECK(!agg->wide, WG_ERR_UNSUPPORTED, "a GROUP BY key wider than 64 bits after a join is not on the GPU path");
   if (agg->key.n > 1)
     for (int k = 0; k < agg->key.n; ++k)
       WG_CHECK(!agg->col_nullable[agg->key.col[k]], WG_ERR_UNSUPPORTED, "nullable multi-column key after a join");

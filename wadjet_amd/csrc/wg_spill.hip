@@ -113,17 +113,23 @@ wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint
     }
     WG_HIP(hipStreamSynchronize(h->ctx->stream));
   }
-  const KeyDesc& K = h->key;
+  const KeyDesc& K = h->wide ? h->wkey : h->key;
+  std::vector<uint64_t> dict;  // 128-bit keys: id -> {lo, hi}
+  if (h->wide) WG_TRY(wide_dict_host(h, &dict));
+  // group column k of dense slot j
+  auto kv = [&](uint64_t j, int k) -> int64_t {
+    const uint64_t w = slots[j * S];
+    return key_value(K, h->wide ? dict[2 * w + K.word[k]] : w, k);
+  };
   const int na = (int)h->aggs.size();
   // sort keys, then the order of the groups
   std::vector<std::string> sk(n);
   for (uint64_t j = 0; j < n; ++j) {
     std::string& s = sk[j];
-    const uint64_t w = slots[j * S];
     for (int k = 0; k < K.n; ++k) {
       if (k) s.push_back('\0');
       if (nulls[j]) s += "<null>";
-      else s += std::to_string(key_value(K, w, k));
+      else s += std::to_string(kv(j, k));
     }
   }
   std::vector<uint32_t> order(n);
@@ -162,10 +168,10 @@ wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint
         o.u8(0);  // partialTagNull
       } else if (int32_tagged(K.key_type[k])) {
         o.u8(6);  // partialTagInt32
-        o.u32((uint32_t)(int32_t)key_value(K, s[0], k));
+        o.u32((uint32_t)(int32_t)kv(j, k));
       } else {
         o.u8(3);  // partialTagInt64
-        o.u64((uint64_t)key_value(K, s[0], k));
+        o.u64((uint64_t)kv(j, k));
       }
     }
     for (int a = 0; a < na; ++a) {
@@ -228,7 +234,7 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
   };
   WG_CHECK(need(5) && memcmp(p, "WAGS\x01", 5) == 0, WG_ERR_INVALID, "not a partial-state spill run");
   p += 5;
-  const KeyDesc& K = h->key;
+  const KeyDesc& K = h->wide ? h->wkey : h->key;
   const int na = (int)h->aggs.size();
   WG_CHECK(rd(4) == (uint64_t)K.n && ok, WG_ERR_INVALID, "run has another number of group columns");
   for (int k = 0; k < K.n && ok; ++k) {
@@ -255,12 +261,13 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
     for (int w = 0; w < S; ++w) slots[j * S + w] = slot_init_word(h->pg, w);
   for (uint64_t j = n; j < n + kSideSlots; ++j) slots[j * S + 1] = 0;  // a side slot is live once it has rows
   uint64_t g = 0;
+  std::vector<uint64_t> pairs;  // 128-bit keys: {lo, hi} per group, in table order
   for (uint64_t i = 0; i < n; ++i) {
     WG_CHECK(rd(1) == 1 && ok, WG_ERR_INVALID, "bad row marker in group %llu", (unsigned long long)i);
     const uint64_t sl = rd(4);
     WG_CHECK(need(sl), WG_ERR_INVALID, "truncated sort key");
     p += sl;
-    uint64_t key = 0;
+    uint64_t key = 0, key_hi = 0;
     bool null_key = false;
     for (int k = 0; k < K.n; ++k) {
       const uint64_t tag = rd(1);
@@ -272,13 +279,18 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
       WG_CHECK(ok, WG_ERR_INVALID, "group value tag %llu not on the GPU path", (unsigned long long)tag);
       uint64_t w = (uint64_t)v;
       if (K.n > 1 && K.narrow[k]) w = (w & 0xFFFFFFFFull) << K.shift[k];
-      key |= w;
+      if (h->wide && K.word[k]) key_hi |= w;
+      else key |= w;
+    }
+    if (h->wide) {  // the pair goes through the dictionary below
+      pairs.push_back(key);
+      pairs.push_back(key_hi);
     }
     WG_CHECK(!null_key || K.n == 1, WG_ERR_NULL_KEY, "NULL in a multi-column group key");
     // side slots: [n] the sentinel-valued key, [n + 1] the NULL key
-    const uint64_t j = null_key ? n + 1 : key == kEmpty ? n : g++;
+    const uint64_t j = null_key ? n + 1 : (key == kEmpty && !h->wide) ? n : g++;
     uint64_t* s = slots.data() + j * S;
-    s[0] = null_key || key == kEmpty ? kEmpty : key;
+    s[0] = null_key || (key == kEmpty && !h->wide) ? kEmpty : key;
     uint64_t* b = base.data() + j * (uint64_t)std::max(nb, 1);
     int64_t rows = -1;  // the group's row count, from an aggregate counted on word 1
     for (int a = 0; a < na; ++a) {
@@ -320,6 +332,22 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
     if (nb) std::copy(base.begin() + (n + k) * nb, base.begin() + (n + k + 1) * nb, base.begin() + (g + k) * nb);
   }
   WG_TRY(agg_flush_stage(h));
+  if (h->wide && g) {
+    // the pairs through this aggregate's dictionary: the table keys on ids
+    uint64_t* dp = nullptr;
+    WG_HIP(hipMalloc(&dp, g * 24));
+    uint64_t* did = dp + 2 * g;
+    std::vector<uint64_t> ids(g);
+    wg_status st = hipMemcpy(dp, pairs.data(), g * 16, hipMemcpyHostToDevice) == hipSuccess ? WG_OK : WG_ERR_DEVICE;
+    if (st == WG_OK) st = wide_encode_pairs(h, dp, g, did);
+    if (st == WG_OK && hipMemcpy(ids.data(), did, g * 8, hipMemcpyDeviceToHost) != hipSuccess) st = WG_ERR_DEVICE;
+    (void)hipFree(dp);
+    if (st != WG_OK) {
+      if (st == WG_ERR_DEVICE) set_error("dictionary encode of a run's keys failed");
+      return st;
+    }
+    for (uint64_t j = 0; j < g; ++j) slots[j * S] = ids[j];
+  }
   uint64_t* d_slots = nullptr;
   uint64_t* d_base = nullptr;
   WG_HIP(hipMalloc(&d_slots, (g + kSideSlots) * S * 8));
