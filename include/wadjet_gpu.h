@@ -150,10 +150,13 @@ typedef struct wg_agg wg_agg;
  * schema col_types[0..n_cols) (wg_type) with col_nullable[i] != 0 when the
  * column may carry a validity bitmap. Typed dispatch is resolved here, once,
  * like resolveIndices (aggregate.go:1296-1764): one int-class key -> 64-bit
- * key; 2-4 int-class keys -> packed composite key (packed_hash.go:386-472).
+ * key; 2-4 int-class keys -> packed composite key (packed_hash.go:386-472):
+ * one 64-bit word when the columns fit it, else the reference's 128-bit
+ * {lo, hi} layout (widest column first), dictionary-encoded on the device to
+ * a dense id per distinct key (a 128-bit key needs n_cols < 16).
  * group_hint sizes the table (InputRowHint/GroupNDVHint, aggregate.go:1464-1506);
  * the table grows when it is exceeded.
- * Keys wider than 8 bytes in total, string keys and other functions return
+ * Keys wider than 16 bytes in total, string keys and other functions return
  * WG_ERR_UNSUPPORTED (the caller keeps the CPU operator). */
 wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types,
                         const int32_t* col_scales, const uint8_t* col_nullable,
