@@ -208,6 +208,43 @@ def bench_shuffle(ctx, args, cols=None):
                 achieved_gbs=alg / (m / 1000) / 1e9, frac=alg / (m / 1000) / 1e9 / HBM_PEAK_GBS)
 
 
+# -------------------------------------------- GROUP BY on a 128-bit key
+
+def bench_wide(ctx, args):
+    """GROUP BY (a int64 uniform over 1M, b int64 uniform over 4) SUM(v),
+    COUNT(*) over 100M rows: a two-int64 key takes the reference's 128-bit
+    {lo, hi} layout, dictionary-encoded on the device."""
+    from wadjet_amd import _lib as L
+    from wadjet_amd.gpu import AggSpec, Col, HashAggregate
+    n = args.rows
+    log(f"[wide] generating {n} rows on device")
+    a = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, 21, 0, 999_999)
+    b = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, 22, 0, 3)
+    v = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, 23, -10**9, 10**9)
+    cols = [Col(a, L.INT64), Col(b, L.INT64), Col(v, L.INT64)]
+    agg = HashAggregate(ctx, [L.INT64, L.INT64, L.INT64], [0, 1], [AggSpec(L.SUM, 2), AggSpec(L.COUNT, -1)],
+                        group_hint=4_000_000)
+
+    def step():
+        agg.reset()
+        agg.consume(cols, n)
+        agg.finalize()
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    steps = max(1, args.steps)
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1000 / steps
+    g = agg.num_groups()
+    log(f"[wide] {n} rows, {g} groups: {ms:.3f} ms/step ({n / ms / 1e6:.1f} G rows/s)")
+    agg.close()
+    return dict(rows=n, groups=g, ms_per_step=ms, rows_per_s=n / (ms / 1000))
+
+
 # -------------------------------------------- 256 concurrent queries, one launch
 
 def bench_batched(ctx, args):
@@ -312,7 +349,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--qps", type=float, nargs="*", default=[100, 1000, 5000])
     ap.add_argument("--shuffle-parts", type=int, default=256)
-    ap.add_argument("--only", choices=["agg", "join", "batched", "served", "shuffle"], default=None)
+    ap.add_argument("--only", choices=["agg", "join", "batched", "served", "shuffle", "wide"], default=None)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -340,6 +377,8 @@ def main():
         extra["join"] = J
     if want("shuffle"):
         extra["shuffle"] = bench_shuffle(ctx, args, A["cols"] if A is not None else None)
+    if want("wide"):
+        extra["wide_key"] = bench_wide(ctx, args)
     if want("batched"):
         extra["batched"] = bench_batched(ctx, args)
     if want("served"):

@@ -116,7 +116,7 @@ struct wg_agg {
   uint64_t* dict_keys = nullptr;   // id -> {lo, hi}
   uint64_t dict_keys_cap = 0;      // ids that fit
   uint64_t dict_n = 0;             // ids handed out (as of the last read-back)
-  uint64_t* dict_ctr = nullptr;    // device: [0] ids handed out, [1] overflow flag
+  uint64_t* dict_ctr = nullptr;    // device: [0] ids handed out, [1] rows left without an id, [2] ids exhausted
   uint64_t* ids = nullptr;         // per-row ids of the batch being consumed
   uint64_t ids_cap = 0;
 };

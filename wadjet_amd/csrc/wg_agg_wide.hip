@@ -11,10 +11,10 @@
 // aggregation kernel keeps its one-word claims. Finalize and the spill run
 // turn ids back into the key columns through the id -> {lo, hi} array.
 //
-// Dictionary slot (4 words): state, lo, hi, unused. state is kDictFree,
-// kDictBusy while the claiming lane writes lo/hi, kDictDead for a claim
-// that found no id left, else the id (published with release order after
-// lo/hi, read with acquire order before them).
+// Dictionary slot (4 words, 32-byte aligned, all ones when free): state, lo,
+// hi, unused. state is kDictFree, kDictBusy while the claiming lane writes
+// lo/hi, kDictDead for a claim that found no id left, else the id (published
+// with release order after lo/hi). Ids are below every reserved state value.
 #include <algorithm>
 #include <vector>
 
@@ -26,7 +26,7 @@ constexpr uint64_t kDictFree = ~0ull;
 constexpr uint64_t kDictBusy = ~0ull - 1;
 constexpr uint64_t kDictDead = ~0ull - 2;
 constexpr uint64_t kNoId = ~0ull;
-constexpr int kDictMaxProbe = 4096;
+constexpr int kDictMaxProbe = 512;
 constexpr uint64_t kDictMaxSpin = 1ull << 22;
 
 struct WideDict {
@@ -34,29 +34,58 @@ struct WideDict {
   uint64_t mask;
   uint64_t* keys;     // id -> {lo, hi}
   uint64_t keys_cap;  // ids that fit
-  unsigned long long* ctr;  // [0] ids handed out, [1] overflow flag
+  unsigned long long* ctr;  // [0] ids handed out, [1] rows left without an id, [2] ids exhausted
 };
 
 __device__ __forceinline__ uint64_t wide_hash(uint64_t lo, uint64_t hi) {
   return mix64(lo ^ mix64(hi + 0x9E3779B97F4A7C15ull));
 }
 
-// The pair's id, inserting it if new; kNoId (and the overflow flag) when the
-// id space or the probe budget ran out — the host grows and runs again.
+// The pair's id, inserting it if new; kNoId (counted in ctr[1]) when the id
+// space or the probe budget ran out — the host grows and runs again.
+//
+// Every word of a slot changes once: state Free -> Busy -> id (or Dead), lo
+// and hi from the fill pattern (all ones) to the pair. A slot is first read
+// with plain loads; the read settles it when the state is an id (or Dead) and
+// neither pair word still shows the fill pattern, since a cached line may be
+// older than the claim but never holds a half-written pair otherwise. Any
+// other read (free, busy, an all-ones word) is settled with read-modify-write
+// atomics, which act at the device's coherence point: a CAS to claim a free
+// slot, fetch-ors of zero to read a published one, and for a busy slot a look
+// again in the next round — never a spin inside the round, so a claiming
+// lane of the same wavefront publishes first.
 __device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t hi) {
   const uint64_t lanes_below = (1ull << __lane_id()) - 1;
   uint64_t i = wide_hash(lo, hi) & D.mask;
   int probes = 0;
   uint64_t spins = 0;
+  bool settle = false;
   for (;;) {
     uint64_t* s = D.slots + i * 4;
-    uint64_t st = __hip_atomic_load(s, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t st, l2, h2;
+    if (!settle) {
+      const ulonglong2 a = *(const ulonglong2*)s;
+      const ulonglong2 b = *(const ulonglong2*)(s + 2);
+      st = a.x;
+      l2 = a.y;
+      h2 = b.x;
+      settle = !(st < kDictDead && l2 != kDictFree && h2 != kDictFree) && st != kDictDead;
+    }
     bool claimed = false;
-    if (st == kDictFree) {
-      uint64_t expected = kDictFree;
-      claimed = __hip_atomic_compare_exchange_strong(s, &expected, kDictBusy, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-      if (!claimed) st = expected;
+    if (settle) {
+      settle = false;
+      st = __hip_atomic_fetch_or(s, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      if (st == kDictFree && __hip_atomic_load(D.ctr + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (st == kDictFree) {
+        uint64_t expected = kDictFree;
+        claimed = __hip_atomic_compare_exchange_strong(s, &expected, kDictBusy, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        if (!claimed) st = expected;
+      }
+      if (!claimed && st < kDictDead) {
+        l2 = __hip_atomic_fetch_or(s + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        h2 = __hip_atomic_fetch_or(s + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     // this round's claimers take consecutive ids with one atomic per wavefront
     const uint64_t m = __ballot(claimed);
@@ -70,30 +99,28 @@ __device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t
     }
     if (claimed) {
       if (nid < D.keys_cap) {
-        __hip_atomic_store(s + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(s + 2, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        (void)__hip_atomic_exchange(s + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        (void)__hip_atomic_exchange(s + 2, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         D.keys[2 * nid] = lo;
         D.keys[2 * nid + 1] = hi;
-        __hip_atomic_store(s, nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        (void)__hip_atomic_exchange(s, nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return nid;
       }
-      __hip_atomic_store(s, kDictDead, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      atomicOr(D.ctr + 1, 1ull);
+      (void)__hip_atomic_exchange(s, kDictDead, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(D.ctr + 2, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      atomicAdd(D.ctr + 1, 1ull);
       return kNoId;
     }
-    if (st == kDictBusy) {  // the claimer is writing the pair: look again
+    if (st == kDictBusy) {  // being written: settle the same slot next round
+      settle = true;
       if (++spins > kDictMaxSpin) break;
       continue;
     }
-    if (st != kDictDead) {
-      const uint64_t l2 = __hip_atomic_load(s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint64_t h2 = __hip_atomic_load(s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (l2 == lo && h2 == hi) return st;
-    }
+    if (st < kDictDead && l2 == lo && h2 == hi) return st;
     if (++probes >= kDictMaxProbe) break;
     i = (i + 1) & D.mask;
   }
-  atomicOr(D.ctr + 1, 1ull);
+  atomicAdd(D.ctr + 1, 1ull);
   return kNoId;
 }
 
@@ -195,13 +222,13 @@ static wg_status dict_alloc(wg_agg* h, uint64_t cap, uint64_t keep) {
   h->dict_cap = cap;
   h->dict_keys_cap = cap / 2;
   h->dict_n = keep;
-  uint64_t c[2] = {keep, 0};
-  WG_HIP(hipMemcpy(h->dict_ctr, c, 16, hipMemcpyHostToDevice));
+  uint64_t c[3] = {keep, 0, 0};
+  WG_HIP(hipMemcpy(h->dict_ctr, c, 24, hipMemcpyHostToDevice));
   return WG_OK;
 }
 
 wg_status wide_init(wg_agg* h) {
-  WG_HIP(hipMalloc(&h->dict_ctr, 16));
+  WG_HIP(hipMalloc(&h->dict_ctr, 24));
   return dict_alloc(h, next_pow2(std::max<uint64_t>(2 * std::max<uint64_t>(h->group_hint, 1024), 64)), 0);
 }
 
@@ -216,7 +243,7 @@ wg_status wide_free(wg_agg* h) {
 
 wg_status wide_reset(wg_agg* h) {
   WG_TRY(fill(h, h->dict, h->dict_cap * 4, kDictFree));
-  WG_TRY(zero_dev(h->ctx, h->dict_ctr, 2));
+  WG_TRY(zero_dev(h->ctx, h->dict_ctr, 3));
   h->dict_n = 0;
   return WG_OK;
 }
@@ -227,19 +254,23 @@ wg_status wide_reset(wg_agg* h) {
 // idempotent, pairs already in keep their ids.
 template <typename F>
 static wg_status encode_until_fits(wg_agg* h, uint64_t n, F launch) {
-  for (int round = 0; round < 8; ++round) {
+  for (int round = 0; round < 16; ++round) {
     WG_TRY(launch(dict_of(h)));
-    uint64_t c[2];
-    WG_HIP(hipMemcpyAsync(c, h->dict_ctr, 16, hipMemcpyDeviceToHost, h->ctx->stream));
+    uint64_t c[3];
+    WG_HIP(hipMemcpyAsync(c, h->dict_ctr, 24, hipMemcpyDeviceToHost, h->ctx->stream));
     WG_HIP(hipStreamSynchronize(h->ctx->stream));
     if (!c[1]) {
       h->dict_n = c[0];
       return WG_OK;
     }
-    // ids [0, keys_cap) are valid and contiguous; later claims were dropped
+    // ids [0, keys_cap) are valid and contiguous; later claims were dropped.
+    // Room for every row left without an id (an upper bound of the new
+    // pairs), at least twice the ids, at most a 2^23-id step beyond 4x.
     const uint64_t keep = std::min<uint64_t>(c[0], h->dict_keys_cap);
-    const uint64_t want = std::max<uint64_t>(c[0] + std::min<uint64_t>(n, c[0]) / 4, keep + 1);
-    WG_TRY(dict_alloc(h, std::max<uint64_t>(h->dict_cap * 2, next_pow2(want * 2)), keep));
+    uint64_t ids = next_pow2(keep + std::min<uint64_t>(c[1], n));
+    ids = std::max<uint64_t>(ids, 2 * h->dict_keys_cap);
+    ids = std::min<uint64_t>(ids, std::max<uint64_t>(4 * h->dict_keys_cap, 1ull << 23));
+    WG_TRY(dict_alloc(h, 2 * ids, keep));
   }
   set_error("key dictionary did not converge");
   return WG_ERR_DEVICE;
