@@ -28,13 +28,15 @@ constexpr uint64_t kDictDead = ~0ull - 2;
 constexpr uint64_t kNoId = ~0ull;
 constexpr int kDictMaxProbe = 512;
 constexpr uint64_t kDictMaxSpin = 1ull << 22;
+constexpr uint64_t kIdBlock = 256;  // ids a wavefront takes from the shared counter at a time
+constexpr int kDictBlock = 256;     // threads per workgroup of the dictionary kernels
 
 struct WideDict {
   uint64_t* slots;
   uint64_t mask;
   uint64_t* keys;     // id -> {lo, hi}
   uint64_t keys_cap;  // ids that fit
-  unsigned long long* ctr;  // [0] ids handed out, [1] rows left without an id, [2] ids exhausted
+  unsigned long long* ctr;  // [0] ids reserved in blocks, [1] rows left without an id, [2] ids exhausted
 };
 
 __device__ __forceinline__ uint64_t wide_hash(uint64_t lo, uint64_t hi) {
@@ -54,7 +56,7 @@ __device__ __forceinline__ uint64_t wide_hash(uint64_t lo, uint64_t hi) {
 // slot, fetch-ors of zero to read a published one, and for a busy slot a look
 // again in the next round — never a spin inside the round, so a claiming
 // lane of the same wavefront publishes first.
-__device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t hi) {
+__device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t hi, unsigned long long* blk) {
   const uint64_t lanes_below = (1ull << __lane_id()) - 1;
   uint64_t i = wide_hash(lo, hi) & D.mask;
   int probes = 0;
@@ -75,7 +77,7 @@ __device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t
     if (settle) {
       settle = false;
       st = __hip_atomic_fetch_or(s, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      if (st == kDictFree && __hip_atomic_load(D.ctr + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (st == kDictFree && *(volatile const unsigned long long*)(D.ctr + 2)) break;  // a hint: ids ran out
       if (st == kDictFree) {
         uint64_t expected = kDictFree;
         claimed = __hip_atomic_compare_exchange_strong(s, &expected, kDictBusy, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
@@ -87,13 +89,25 @@ __device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t
         h2 = __hip_atomic_fetch_or(s + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    // this round's claimers take consecutive ids with one atomic per wavefront
+    // this round's claimers take consecutive ids of the wavefront's block
+    // (blk: the wavefront's next id and block end, in LDS); a new block
+    // comes from the shared counter, kIdBlock >= 64 ids at a time
     const uint64_t m = __ballot(claimed);
     uint64_t nid = 0;
     if (m) {
       const int leader = __ffsll((long long)m) - 1;
+      const unsigned long long cnt = (unsigned long long)__popcll(m);
       unsigned long long base = 0;
-      if (__lane_id() == leader) base = atomicAdd(D.ctr, (unsigned long long)__popcll(m));
+      if (__lane_id() == leader) {
+        unsigned long long b = blk[0], e = blk[1];
+        if (e - b < cnt) {
+          b = atomicAdd(D.ctr, (unsigned long long)kIdBlock);
+          e = b + kIdBlock;
+        }
+        base = b;
+        blk[0] = b + cnt;
+        blk[1] = e;
+      }
       base = __shfl(base, leader);
       nid = base + (uint64_t)__popcll(m & lanes_below);
     }
@@ -137,25 +151,58 @@ __device__ __forceinline__ void wide_pack(const KeyDesc& K, const ColSet& cs, ui
   }
 }
 
-__global__ void wide_encode_kernel(ColSet cs, KeyDesc K, const uint32_t* sel, uint64_t n, WideDict D,
-                                   uint64_t* ids) {
+// The wavefront's id block in LDS, empty at the start of a kernel.
+__device__ __forceinline__ unsigned long long* wave_block(unsigned long long (*blocks)[2]) {
+  unsigned long long* blk = blocks[threadIdx.x >> 6];
+  if (__lane_id() == 0) blk[0] = blk[1] = 0;
+  __builtin_amdgcn_wave_barrier();
+  return blk;
+}
+
+__global__ void __launch_bounds__(kDictBlock) wide_encode_kernel(ColSet cs, KeyDesc K, const uint32_t* sel,
+                                                                 uint64_t n, WideDict D, uint64_t* ids) {
+  __shared__ unsigned long long blocks[kDictBlock / 64][2];
+  unsigned long long* blk = wave_block(blocks);
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t row = sel ? (uint64_t)sel[i] : i;
     uint64_t lo, hi;
     wide_pack(K, cs, row, lo, hi);
-    ids[row] = dict_find_or_insert(D, lo, hi);
+    ids[row] = dict_find_or_insert(D, lo, hi, blk);
   }
 }
 
-__global__ void wide_pairs_kernel(const uint64_t* pairs, uint64_t n, WideDict D, uint64_t* ids) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    ids[i] = dict_find_or_insert(D, pairs[2 * i], pairs[2 * i + 1]);
+// Remap: the key word of every live slot of src's table (an id of src's
+// dictionary, whose pair is src_keys[2 id]) becomes the pair's id in D,
+// written into a copy of the table.
+__global__ void __launch_bounds__(kDictBlock) wide_remap_kernel(const uint64_t* src_slots, uint64_t cap, int stride,
+                                                                const uint64_t* src_keys, WideDict D,
+                                                                uint64_t* copy) {
+  __shared__ unsigned long long blocks[kDictBlock / 64][2];
+  unsigned long long* blk = wave_block(blocks);
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cap; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t w = src_slots[j * stride];
+    if (w == kEmpty) continue;
+    copy[j * stride] = dict_find_or_insert(D, src_keys[2 * w], src_keys[2 * w + 1], blk);
+  }
 }
 
-// Re-insert ids [0, n) into an empty table (distinct pairs, ids kept).
-__global__ void wide_rehash_kernel(const uint64_t* keys, uint64_t n, uint64_t* slots, uint64_t mask) {
-  for (uint64_t id = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; id < n; id += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t lo = keys[2 * id], hi = keys[2 * id + 1];
+// A run's pairs through the dictionary (ids[i] of pairs[2 i], pairs[2 i + 1]).
+__global__ void __launch_bounds__(kDictBlock) wide_pairs_kernel(const uint64_t* pairs, uint64_t n, WideDict D,
+                                                                uint64_t* ids) {
+  __shared__ unsigned long long blocks[kDictBlock / 64][2];
+  unsigned long long* blk = wave_block(blocks);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    ids[i] = dict_find_or_insert(D, pairs[2 * i], pairs[2 * i + 1], blk);
+}
+
+// Move the published slots of an old table (ids below `keep`) into an empty
+// one, ids kept.
+__global__ void wide_rehash_kernel(const uint64_t* old_slots, uint64_t old_cap, uint64_t keep, uint64_t* slots,
+                                   uint64_t mask) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < old_cap; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t id = old_slots[j * 4];
+    if (id >= kDictDead || id >= keep) continue;
+    const uint64_t lo = old_slots[j * 4 + 1], hi = old_slots[j * 4 + 2];
     uint64_t i = wide_hash(lo, hi) & mask;
     for (uint64_t p = 0; p <= mask; ++p) {
       uint64_t expected = kDictFree;
@@ -163,7 +210,7 @@ __global__ void wide_rehash_kernel(const uint64_t* keys, uint64_t n, uint64_t* s
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         slots[i * 4 + 1] = lo;
         slots[i * 4 + 2] = hi;
-        __hip_atomic_store(slots + i * 4, id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        slots[i * 4] = id;
         break;
       }
       i = (i + 1) & mask;
@@ -174,15 +221,6 @@ __global__ void wide_rehash_kernel(const uint64_t* keys, uint64_t n, uint64_t* s
 __global__ void fill_words_kernel(uint64_t* p, uint64_t n, uint64_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
-}
-
-// Rewrite the key word of every live slot of a table copy through map (ids of
-// another dictionary -> ids of this one).
-__global__ void wide_remap_kernel(uint64_t* slots, uint64_t cap, int stride, const uint64_t* map) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cap; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t w = slots[j * stride];
-    if (w != kEmpty) slots[j * stride] = map[w];
-  }
 }
 
 static WideDict dict_of(wg_agg* h) {
@@ -196,8 +234,9 @@ static wg_status fill(wg_agg* h, uint64_t* p, uint64_t n, uint64_t v) {
   return WG_OK;
 }
 
-// Table of `cap` slots and room for cap/2 ids; the ids [0, keep) of the old
-// arrays are kept.
+// Table of `cap` slots and room for cap/2 ids; the published pairs with ids
+// below `keep` move over from the old table (ids are handed out in blocks,
+// so [0, keep) may hold unused ids).
 static wg_status dict_alloc(wg_agg* h, uint64_t cap, uint64_t keep) {
   uint64_t* slots = nullptr;
   uint64_t* keys = nullptr;
@@ -210,8 +249,8 @@ static wg_status dict_alloc(wg_agg* h, uint64_t cap, uint64_t keep) {
   WG_TRY(fill(h, slots, cap * 4, kDictFree));
   if (keep) {
     WG_HIP(hipMemcpyAsync(keys, h->dict_keys, keep * 16, hipMemcpyDeviceToDevice, h->ctx->stream));
-    hipLaunchKernelGGL(wide_rehash_kernel, dim3(grid_for(keep, 256)), dim3(256), 0, h->ctx->stream, keys, keep, slots,
-                       cap - 1);
+    hipLaunchKernelGGL(wide_rehash_kernel, dim3(grid_for(h->dict_cap, 256)), dim3(256), 0, h->ctx->stream,
+                       (const uint64_t*)h->dict, h->dict_cap, keep, slots, cap - 1);
     WG_HIP(hipGetLastError());
   }
   WG_HIP(hipStreamSynchronize(h->ctx->stream));
@@ -249,9 +288,9 @@ wg_status wide_reset(wg_agg* h) {
 }
 
 // Run `launch` (an encode over the dictionary) until nothing overflows:
-// after an overflow the dictionary grows to hold every pair handed an id
-// attempt (the counter counts them) and the encode runs again — it is
-// idempotent, pairs already in keep their ids.
+// after an overflow the dictionary grows (room for every row left without an
+// id) and the encode runs again — it is idempotent, pairs already in keep
+// their ids.
 template <typename F>
 static wg_status encode_until_fits(wg_agg* h, uint64_t n, F launch) {
   for (int round = 0; round < 16; ++round) {
@@ -260,10 +299,10 @@ static wg_status encode_until_fits(wg_agg* h, uint64_t n, F launch) {
     WG_HIP(hipMemcpyAsync(c, h->dict_ctr, 24, hipMemcpyDeviceToHost, h->ctx->stream));
     WG_HIP(hipStreamSynchronize(h->ctx->stream));
     if (!c[1]) {
-      h->dict_n = c[0];
+      h->dict_n = std::min<uint64_t>(c[0], h->dict_keys_cap);  // ids are below this
       return WG_OK;
     }
-    // ids [0, keys_cap) are valid and contiguous; later claims were dropped.
+    // pairs with ids below keys_cap are published; later claims were dropped.
     // Room for every row left without an id (an upper bound of the new
     // pairs), at least twice the ids, at most a 2^23-id step beyond 4x.
     const uint64_t keep = std::min<uint64_t>(c[0], h->dict_keys_cap);
@@ -288,8 +327,8 @@ wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, ui
     const KeyDesc K = h->wkey;
     const ColSet c = cs;
     WG_TRY(encode_until_fits(h, n, [&](const WideDict& D) -> wg_status {
-      hipLaunchKernelGGL(wide_encode_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, h->ctx->stream, c, K, sel, n,
-                         D, h->ids);
+      hipLaunchKernelGGL(wide_encode_kernel, dim3(grid_for(n, kDictBlock, 8192)), dim3(kDictBlock), 0,
+                         h->ctx->stream, c, K, sel, n, D, h->ids);
       WG_HIP(hipGetLastError());
       return WG_OK;
     }));
@@ -305,8 +344,8 @@ wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, ui
 wg_status wide_encode_pairs(wg_agg* h, const uint64_t* pairs, uint64_t n, uint64_t* ids_out) {
   if (n == 0) return WG_OK;
   return encode_until_fits(h, n, [&](const WideDict& D) -> wg_status {
-    hipLaunchKernelGGL(wide_pairs_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, h->ctx->stream, pairs, n, D,
-                       ids_out);
+    hipLaunchKernelGGL(wide_pairs_kernel, dim3(grid_for(n, kDictBlock, 8192)), dim3(kDictBlock), 0, h->ctx->stream,
+                       pairs, n, D, ids_out);
     WG_HIP(hipGetLastError());
     return WG_OK;
   });
@@ -324,28 +363,22 @@ wg_status wide_dict_host(wg_agg* h, std::vector<uint64_t>* pairs) {
 // A copy of src's table whose keys are ids of dst's dictionary.
 wg_status wide_remap_slots(wg_agg* dst, wg_agg* src, uint64_t** remapped) {
   *remapped = nullptr;
-  const uint64_t n = src->dict_n;
-  uint64_t* map = nullptr;
   uint64_t* copy = nullptr;
   const uint64_t words = (src->cap + kSideSlots) * (uint64_t)src->pg.stride;
-  WG_HIP(hipMalloc(&map, std::max<uint64_t>(n, 1) * 8));
   WG_HIP(hipMalloc(&copy, words * 8));
-  wg_status st = wide_encode_pairs(dst, src->dict_keys, n, map);
-  if (st == WG_OK) {
-    if (hipMemcpyAsync(copy, src->slots, words * 8, hipMemcpyDeviceToDevice, dst->ctx->stream) != hipSuccess) {
-      set_error("hipMemcpyAsync");
-      st = WG_ERR_DEVICE;
-    }
+  wg_status st = WG_OK;
+  if (hipMemcpyAsync(copy, src->slots, words * 8, hipMemcpyDeviceToDevice, dst->ctx->stream) != hipSuccess) {
+    set_error("hipMemcpyAsync");
+    st = WG_ERR_DEVICE;
   }
-  if (st == WG_OK) {
-    hipLaunchKernelGGL(wide_remap_kernel, dim3(grid_for(src->cap, 256, 8192)), dim3(256), 0, dst->ctx->stream, copy,
-                       src->cap, src->pg.stride, (const uint64_t*)map);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(dst->ctx->stream) != hipSuccess) {
-      set_error("wide_remap_kernel");
-      st = WG_ERR_DEVICE;
-    }
-  }
-  (void)hipFree(map);
+  if (st == WG_OK)
+    st = encode_until_fits(dst, src->groups + 1, [&](const WideDict& D) -> wg_status {
+      hipLaunchKernelGGL(wide_remap_kernel, dim3(grid_for(src->cap, kDictBlock, 8192)), dim3(kDictBlock), 0,
+                         dst->ctx->stream, (const uint64_t*)src->slots, src->cap, src->pg.stride,
+                         (const uint64_t*)src->dict_keys, D, copy);
+      WG_HIP(hipGetLastError());
+      return WG_OK;
+    });
   if (st != WG_OK) {
     (void)hipFree(copy);
     return st;
