@@ -4,17 +4,30 @@
 // key (buildPackedLayout / packedKeyAt, internal/engine/exec/packed_hash.go:
 // 384-472) and probes a table of 24-byte entries (packedHashTable, :47-328).
 // Here such a key is dictionary-encoded before it reaches the group table:
-// a device hash table maps every distinct {lo, hi} pair to a dense 64-bit id
-// (handed out in blocks per wavefront), the batch's rows get a per-row id
-// column, and the aggregate runs on that id exactly as on a one-column int64
-// key — direct or partitioned consume, heavy hitters, merge — so every
-// aggregation kernel keeps its one-word claims. Finalize and the spill run
-// turn ids back into the key columns through the id -> {lo, hi} array.
+// a device hash table maps every distinct {lo, hi} pair to a 64-bit id, the
+// batch's rows get a per-row id column, and the aggregate runs on that id
+// exactly as on a one-column int64 key — direct or partitioned consume, heavy
+// hitters, merge — so every aggregation kernel keeps its one-word claims.
+// Finalize and the spill run turn ids back into the key columns through the
+// id -> {lo, hi} array.
 //
-// Dictionary slot (4 words, 32-byte aligned, all ones when free): state, lo,
-// hi, unused. state is kDictFree, kDictBusy while the claiming lane writes
-// lo/hi, kDictDead for a claim that found no id left, else the id (published
-// with release order after lo/hi). Ids are below every reserved state value.
+// Dictionary slot (4 words, 32-byte aligned, all ones when free):
+//   fp      63-bit fingerprint of the pair (the claim word: CAS from free)
+//   lo, hi  the pair
+//   id      the pair's id (all ones until assigned)
+// An encode is three kernels, so that no kernel reads a pair another lane of
+// the same kernel is still writing (no fences on the hot path):
+//   claim   every row probes from fp; a slot holding its fingerprint ends the
+//           probe (the same pair, or — checked below — another pair with the
+//           same fingerprint); a free slot is claimed by CAS and its pair
+//           written;
+//   assign  the new slots get ids (workgroup ranges of the table, one shared
+//           counter atomic per workgroup) and the id -> pair array;
+//   lookup  every row probes again with plain loads and takes the id of the
+//           slot holding its exact pair.
+// A pair the lookup does not find (its fingerprint was another pair's in the
+// same claim pass) is claimed in another round, where the earlier claim's
+// pair is visible and differs; a table past its probe budget grows first.
 #include <algorithm>
 #include <vector>
 
@@ -23,118 +36,67 @@
 namespace wg {
 
 constexpr uint64_t kDictFree = ~0ull;
-constexpr uint64_t kDictBusy = ~0ull - 1;
-constexpr uint64_t kDictDead = ~0ull - 2;
 constexpr uint64_t kNoId = ~0ull;
-constexpr int kDictMaxProbe = 512;
-constexpr uint64_t kDictMaxSpin = 1ull << 22;
-constexpr uint64_t kIdBlock = 256;  // ids a wavefront takes from the shared counter at a time
-constexpr int kDictBlock = 256;     // threads per workgroup of the dictionary kernels
+constexpr int kDictMaxProbe = 256;
+constexpr int kDictBlock = 256;           // threads per workgroup of the dictionary kernels
+constexpr uint64_t kAssignRange = 16384;  // table slots per workgroup of the assign pass
 
 struct WideDict {
   uint64_t* slots;
   uint64_t mask;
-  uint64_t* keys;     // id -> {lo, hi}
-  uint64_t keys_cap;  // ids that fit
-  unsigned long long* ctr;  // [0] ids reserved in blocks, [1] rows left without an id, [2] ids exhausted
+  uint64_t* keys;           // id -> {lo, hi}
+  unsigned long long* ctr;  // [0] ids handed out, [1] a claim ran out of probes, [2] a lookup missed
 };
 
-__device__ __forceinline__ uint64_t wide_hash(uint64_t lo, uint64_t hi) {
-  return mix64(lo ^ mix64(hi + 0x9E3779B97F4A7C15ull));
+__device__ __forceinline__ uint64_t wide_fp(uint64_t lo, uint64_t hi) {
+  return mix64(lo ^ mix64(hi + 0x9E3779B97F4A7C15ull)) >> 1;  // below the free pattern
 }
 
-// The pair's id, inserting it if new; kNoId (counted in ctr[1]) when the id
-// space or the probe budget ran out — the host grows and runs again.
-//
-// Every word of a slot changes once: state Free -> Busy -> id (or Dead), lo
-// and hi from the fill pattern (all ones) to the pair. A slot is first read
-// with plain loads; the read settles it when the state is an id (or Dead) and
-// neither pair word still shows the fill pattern, since a cached line may be
-// older than the claim but never holds a half-written pair otherwise. Any
-// other read (free, busy, an all-ones word) is settled with read-modify-write
-// atomics, which act at the device's coherence point: a CAS to claim a free
-// slot, fetch-ors of zero to read a published one, and for a busy slot a look
-// again in the next round — never a spin inside the round, so a claiming
-// lane of the same wavefront publishes first.
-__device__ uint64_t dict_find_or_insert(const WideDict& D, uint64_t lo, uint64_t hi, unsigned long long* blk) {
-  const uint64_t lanes_below = (1ull << __lane_id()) - 1;
-  uint64_t i = wide_hash(lo, hi) & D.mask;
-  int probes = 0;
-  uint64_t spins = 0;
-  bool settle = false;
-  for (;;) {
+// Claim pass for one pair.
+__device__ __forceinline__ void dict_claim(const WideDict& D, uint64_t lo, uint64_t hi) {
+  const uint64_t fp = wide_fp(lo, hi);
+  uint64_t i = fp & D.mask;
+  for (int p = 0; p < kDictMaxProbe; ++p) {
     uint64_t* s = D.slots + i * 4;
-    uint64_t st, l2, h2;
-    if (!settle) {
-      const ulonglong2 a = *(const ulonglong2*)s;
-      const ulonglong2 b = *(const ulonglong2*)(s + 2);
-      st = a.x;
-      l2 = a.y;
-      h2 = b.x;
-      settle = !(st < kDictDead && l2 != kDictFree && h2 != kDictFree) && st != kDictDead;
-    }
-    bool claimed = false;
-    if (settle) {
-      settle = false;
-      st = __hip_atomic_fetch_or(s, 0ull, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      if (st == kDictFree && *(volatile const unsigned long long*)(D.ctr + 2)) break;  // a hint: ids ran out
-      if (st == kDictFree) {
-        uint64_t expected = kDictFree;
-        claimed = __hip_atomic_compare_exchange_strong(s, &expected, kDictBusy, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-        if (!claimed) st = expected;
+    uint64_t st = *s;
+    if (st == kDictFree) {
+      uint64_t expected = kDictFree;
+      if (__hip_atomic_compare_exchange_strong(s, &expected, fp, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)) {
+        s[1] = lo;
+        s[2] = hi;
+        return;
       }
-      if (!claimed && st < kDictDead) {
-        l2 = __hip_atomic_fetch_or(s + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        h2 = __hip_atomic_fetch_or(s + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      st = expected;
     }
-    // this round's claimers take consecutive ids of the wavefront's block
-    // (blk: the wavefront's next id and block end, in LDS); a new block
-    // comes from the shared counter, kIdBlock >= 64 ids at a time
-    const uint64_t m = __ballot(claimed);
-    uint64_t nid = 0;
-    if (m) {
-      const int leader = __ffsll((long long)m) - 1;
-      const unsigned long long cnt = (unsigned long long)__popcll(m);
-      unsigned long long base = 0;
-      if (__lane_id() == leader) {
-        unsigned long long b = blk[0], e = blk[1];
-        if (e - b < cnt) {
-          b = atomicAdd(D.ctr, (unsigned long long)kIdBlock);
-          e = b + kIdBlock;
-        }
-        base = b;
-        blk[0] = b + cnt;
-        blk[1] = e;
-      }
-      base = __shfl(base, leader);
-      nid = base + (uint64_t)__popcll(m & lanes_below);
+    if (st == fp) {
+      // the same pair, unless this slot's pair reads fully written (neither
+      // word still the all-ones fill, so not half-written by a lane of this
+      // pass) and differs: then it is another pair with this fingerprint
+      const uint64_t l2 = s[1], h2 = s[2];
+      const bool settled = l2 != kDictFree && h2 != kDictFree;
+      if (!settled || (l2 == lo && h2 == hi)) return;
     }
-    if (claimed) {
-      if (nid < D.keys_cap) {
-        (void)__hip_atomic_exchange(s + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        (void)__hip_atomic_exchange(s + 2, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        D.keys[2 * nid] = lo;
-        D.keys[2 * nid + 1] = hi;
-        (void)__hip_atomic_exchange(s, nid, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        return nid;
-      }
-      (void)__hip_atomic_exchange(s, kDictDead, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(D.ctr + 2, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      atomicAdd(D.ctr + 1, 1ull);
-      return kNoId;
-    }
-    if (st == kDictBusy) {  // being written: settle the same slot next round
-      settle = true;
-      if (++spins > kDictMaxSpin) break;
-      continue;
-    }
-    if (st < kDictDead && l2 == lo && h2 == hi) return st;
-    if (++probes >= kDictMaxProbe) break;
     i = (i + 1) & D.mask;
   }
-  atomicAdd(D.ctr + 1, 1ull);
+  D.ctr[1] = 1;  // out of probes: the table grows
+}
+
+// Lookup pass for one pair: its id, or kNoId (flagged) when absent.
+__device__ __forceinline__ uint64_t dict_lookup(const WideDict& D, uint64_t lo, uint64_t hi) {
+  const uint64_t fp = wide_fp(lo, hi);
+  uint64_t i = fp & D.mask;
+  for (int p = 0; p < kDictMaxProbe; ++p) {
+    const uint64_t* s = D.slots + i * 4;
+    const ulonglong2 a = *(const ulonglong2*)s;
+    if (a.x == kDictFree) break;
+    if (a.x == fp) {
+      const ulonglong2 b = *(const ulonglong2*)(s + 2);
+      if (a.y == lo && b.x == hi) return b.y;
+    }
+    i = (i + 1) & D.mask;
+  }
+  D.ctr[2] = 1;
   return kNoId;
 }
 
@@ -151,66 +113,117 @@ __device__ __forceinline__ void wide_pack(const KeyDesc& K, const ColSet& cs, ui
   }
 }
 
-// The wavefront's id block in LDS, empty at the start of a kernel.
-__device__ __forceinline__ unsigned long long* wave_block(unsigned long long (*blocks)[2]) {
-  unsigned long long* blk = blocks[threadIdx.x >> 6];
-  if (__lane_id() == 0) blk[0] = blk[1] = 0;
-  __builtin_amdgcn_wave_barrier();
-  return blk;
-}
+// Pair sources of an encode: a batch's rows, a list of pairs, or the live
+// slots of another aggregate's table (whose keys are ids of its dictionary).
+struct RowsSrc {
+  ColSet cs;
+  KeyDesc K;
+  const uint32_t* sel;
+  uint64_t* ids;  // per row
+  __device__ bool get(uint64_t i, uint64_t& lo, uint64_t& hi) const {
+    wide_pack(K, cs, sel ? (uint64_t)sel[i] : i, lo, hi);
+    return true;
+  }
+  __device__ void put(uint64_t i, uint64_t id) const { ids[sel ? (uint64_t)sel[i] : i] = id; }
+};
 
-__global__ void __launch_bounds__(kDictBlock) wide_encode_kernel(ColSet cs, KeyDesc K, const uint32_t* sel,
-                                                                 uint64_t n, WideDict D, uint64_t* ids) {
-  __shared__ unsigned long long blocks[kDictBlock / 64][2];
-  unsigned long long* blk = wave_block(blocks);
+struct PairsSrc {
+  const uint64_t* pairs;
+  uint64_t* ids;
+  __device__ bool get(uint64_t i, uint64_t& lo, uint64_t& hi) const {
+    lo = pairs[2 * i];
+    hi = pairs[2 * i + 1];
+    return true;
+  }
+  __device__ void put(uint64_t i, uint64_t id) const { ids[i] = id; }
+};
+
+struct SlotsSrc {
+  const uint64_t* slots;  // the other table
+  int stride;
+  const uint64_t* keys;  // its id -> pair array
+  uint64_t* copy;        // a copy of its table, keys rewritten
+  __device__ bool get(uint64_t j, uint64_t& lo, uint64_t& hi) const {
+    const uint64_t w = slots[j * stride];
+    if (w == kEmpty) return false;
+    lo = keys[2 * w];
+    hi = keys[2 * w + 1];
+    return true;
+  }
+  __device__ void put(uint64_t j, uint64_t id) const { copy[j * stride] = id; }
+};
+
+template <typename Src>
+__global__ void __launch_bounds__(kDictBlock) dict_claim_kernel(Src src, uint64_t n, WideDict D) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t row = sel ? (uint64_t)sel[i] : i;
     uint64_t lo, hi;
-    wide_pack(K, cs, row, lo, hi);
-    ids[row] = dict_find_or_insert(D, lo, hi, blk);
+    if (src.get(i, lo, hi)) dict_claim(D, lo, hi);
   }
 }
 
-// Remap: the key word of every live slot of src's table (an id of src's
-// dictionary, whose pair is src_keys[2 id]) becomes the pair's id in D,
-// written into a copy of the table.
-__global__ void __launch_bounds__(kDictBlock) wide_remap_kernel(const uint64_t* src_slots, uint64_t cap, int stride,
-                                                                const uint64_t* src_keys, WideDict D,
-                                                                uint64_t* copy) {
-  __shared__ unsigned long long blocks[kDictBlock / 64][2];
-  unsigned long long* blk = wave_block(blocks);
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cap; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t w = src_slots[j * stride];
-    if (w == kEmpty) continue;
-    copy[j * stride] = dict_find_or_insert(D, src_keys[2 * w], src_keys[2 * w + 1], blk);
+template <typename Src>
+__global__ void __launch_bounds__(kDictBlock) dict_lookup_kernel(Src src, uint64_t n, WideDict D) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo, hi;
+    if (src.get(i, lo, hi)) src.put(i, dict_lookup(D, lo, hi));
   }
 }
 
-// A run's pairs through the dictionary (ids[i] of pairs[2 i], pairs[2 i + 1]).
-__global__ void __launch_bounds__(kDictBlock) wide_pairs_kernel(const uint64_t* pairs, uint64_t n, WideDict D,
-                                                                uint64_t* ids) {
-  __shared__ unsigned long long blocks[kDictBlock / 64][2];
-  unsigned long long* blk = wave_block(blocks);
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    ids[i] = dict_find_or_insert(D, pairs[2 * i], pairs[2 * i + 1], blk);
+__device__ __forceinline__ bool needs_id(const uint64_t* s) { return s[0] != kDictFree && s[3] == kNoId; }
+
+// Ids for the slots claimed since the last assign: each workgroup numbers
+// the new slots of its kAssignRange-slot range in slot order from a base
+// taken from the shared counter.
+__global__ void __launch_bounds__(kDictBlock) dict_assign_kernel(WideDict D, uint64_t cap) {
+  __shared__ uint32_t wave_tot[kDictBlock / 64];
+  __shared__ unsigned long long base;
+  constexpr uint64_t per = kAssignRange / kDictBlock;
+  const uint64_t beg = blockIdx.x * kAssignRange + threadIdx.x * per;
+  const uint64_t end = min(beg + per, cap);
+  uint32_t mine = 0;
+  for (uint64_t j = beg; j < end; ++j) mine += needs_id(D.slots + j * 4) ? 1u : 0u;
+  // exclusive prefix of `mine` over the workgroup
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  uint32_t x = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wave_tot[wave] = x;
+  __syncthreads();
+  uint32_t off = 0, total = 0;
+  for (int w = 0; w < kDictBlock / 64; ++w) {
+    if (w < wave) off += wave_tot[w];
+    total += wave_tot[w];
+  }
+  if (threadIdx.x == 0) base = total ? atomicAdd(D.ctr, (unsigned long long)total) : 0ull;
+  __syncthreads();
+  uint64_t id = base + off + x - mine;
+  for (uint64_t j = beg; j < end; ++j) {
+    uint64_t* s = D.slots + j * 4;
+    if (!needs_id(s)) continue;
+    s[3] = id;
+    D.keys[2 * id] = s[1];
+    D.keys[2 * id + 1] = s[2];
+    ++id;
+  }
 }
 
-// Move the published slots of an old table (ids below `keep`) into an empty
-// one, ids kept.
-__global__ void wide_rehash_kernel(const uint64_t* old_slots, uint64_t old_cap, uint64_t keep, uint64_t* slots,
-                                   uint64_t mask) {
+// Move every claimed slot of an old table into an empty one (fingerprints,
+// pairs and ids kept).
+__global__ void dict_rehash_kernel(const uint64_t* old_slots, uint64_t old_cap, uint64_t* slots, uint64_t mask) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < old_cap; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t id = old_slots[j * 4];
-    if (id >= kDictDead || id >= keep) continue;
-    const uint64_t lo = old_slots[j * 4 + 1], hi = old_slots[j * 4 + 2];
-    uint64_t i = wide_hash(lo, hi) & mask;
+    const uint64_t fp = old_slots[j * 4];
+    if (fp == kDictFree) continue;
+    uint64_t i = fp & mask;
     for (uint64_t p = 0; p <= mask; ++p) {
       uint64_t expected = kDictFree;
-      if (__hip_atomic_compare_exchange_strong(slots + i * 4, &expected, kDictBusy, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        slots[i * 4 + 1] = lo;
-        slots[i * 4 + 2] = hi;
-        slots[i * 4] = id;
+      if (__hip_atomic_compare_exchange_strong(slots + i * 4, &expected, fp, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)) {
+        slots[i * 4 + 1] = old_slots[j * 4 + 1];
+        slots[i * 4 + 2] = old_slots[j * 4 + 2];
+        slots[i * 4 + 3] = old_slots[j * 4 + 3];
         break;
       }
       i = (i + 1) & mask;
@@ -224,7 +237,7 @@ __global__ void fill_words_kernel(uint64_t* p, uint64_t n, uint64_t v) {
 }
 
 static WideDict dict_of(wg_agg* h) {
-  return WideDict{h->dict, h->dict_cap - 1, h->dict_keys, h->dict_keys_cap, (unsigned long long*)h->dict_ctr};
+  return WideDict{h->dict, h->dict_cap - 1, h->dict_keys, (unsigned long long*)h->dict_ctr};
 }
 
 static wg_status fill(wg_agg* h, uint64_t* p, uint64_t n, uint64_t v) {
@@ -234,23 +247,23 @@ static wg_status fill(wg_agg* h, uint64_t* p, uint64_t n, uint64_t v) {
   return WG_OK;
 }
 
-// Table of `cap` slots and room for cap/2 ids; the published pairs with ids
-// below `keep` move over from the old table (ids are handed out in blocks,
-// so [0, keep) may hold unused ids).
-static wg_status dict_alloc(wg_agg* h, uint64_t cap, uint64_t keep) {
+// A table of `cap` slots and an id array of as many entries; the claimed
+// slots of the old table move over with their ids.
+static wg_status dict_alloc(wg_agg* h, uint64_t cap) {
   uint64_t* slots = nullptr;
   uint64_t* keys = nullptr;
-  if (hipMalloc(&slots, cap * 32) != hipSuccess || hipMalloc(&keys, cap / 2 * 16) != hipSuccess) {
+  if (hipMalloc(&slots, cap * 32) != hipSuccess || hipMalloc(&keys, cap * 16) != hipSuccess) {
     (void)hipGetLastError();
     if (slots) (void)hipFree(slots);
     set_error("hipMalloc for a %llu-slot key dictionary failed", (unsigned long long)cap);
     return WG_ERR_OOM;
   }
   WG_TRY(fill(h, slots, cap * 4, kDictFree));
-  if (keep) {
-    WG_HIP(hipMemcpyAsync(keys, h->dict_keys, keep * 16, hipMemcpyDeviceToDevice, h->ctx->stream));
-    hipLaunchKernelGGL(wide_rehash_kernel, dim3(grid_for(h->dict_cap, 256)), dim3(256), 0, h->ctx->stream,
-                       (const uint64_t*)h->dict, h->dict_cap, keep, slots, cap - 1);
+  if (h->dict) {
+    if (h->dict_n)
+      WG_HIP(hipMemcpyAsync(keys, h->dict_keys, h->dict_n * 16, hipMemcpyDeviceToDevice, h->ctx->stream));
+    hipLaunchKernelGGL(dict_rehash_kernel, dim3(grid_for(h->dict_cap, 256)), dim3(256), 0, h->ctx->stream,
+                       (const uint64_t*)h->dict, h->dict_cap, slots, cap - 1);
     WG_HIP(hipGetLastError());
   }
   WG_HIP(hipStreamSynchronize(h->ctx->stream));
@@ -259,16 +272,15 @@ static wg_status dict_alloc(wg_agg* h, uint64_t cap, uint64_t keep) {
   h->dict = slots;
   h->dict_keys = keys;
   h->dict_cap = cap;
-  h->dict_keys_cap = cap / 2;
-  h->dict_n = keep;
-  uint64_t c[3] = {keep, 0, 0};
-  WG_HIP(hipMemcpy(h->dict_ctr, c, 24, hipMemcpyHostToDevice));
+  h->dict_keys_cap = cap;
   return WG_OK;
 }
 
 wg_status wide_init(wg_agg* h) {
   WG_HIP(hipMalloc(&h->dict_ctr, 24));
-  return dict_alloc(h, next_pow2(std::max<uint64_t>(2 * std::max<uint64_t>(h->group_hint, 1024), 64)), 0);
+  WG_TRY(zero_dev(h->ctx, h->dict_ctr, 3));
+  h->dict_n = 0;
+  return dict_alloc(h, next_pow2(std::max<uint64_t>(2 * std::max<uint64_t>(h->group_hint, 1024), 64)));
 }
 
 wg_status wide_free(wg_agg* h) {
@@ -287,29 +299,34 @@ wg_status wide_reset(wg_agg* h) {
   return WG_OK;
 }
 
-// Run `launch` (an encode over the dictionary) until nothing overflows:
-// after an overflow the dictionary grows (room for every row left without an
-// id) and the encode runs again — it is idempotent, pairs already in keep
-// their ids.
-template <typename F>
-static wg_status encode_until_fits(wg_agg* h, uint64_t n, F launch) {
-  for (int round = 0; round < 16; ++round) {
-    WG_TRY(launch(dict_of(h)));
+// claim -> assign -> lookup over n items of src, growing the table when a
+// claim ran out of probes or more than half of it is taken, and claiming
+// again when a lookup missed (a fingerprint shared by two pairs).
+template <typename Src>
+static wg_status dict_encode(wg_agg* h, const Src& src, uint64_t n) {
+  if (n == 0) return WG_OK;
+  const int grid = grid_for(n, kDictBlock, 8192);
+  for (int round = 0; round < 12; ++round) {
+    WG_TRY(zero_dev(h->ctx, h->dict_ctr + 1, 2));
+    hipLaunchKernelGGL(dict_claim_kernel<Src>, dim3(grid), dim3(kDictBlock), 0, h->ctx->stream, src, n, dict_of(h));
+    WG_HIP(hipGetLastError());
+    const uint64_t ranges = (h->dict_cap + kAssignRange - 1) / kAssignRange;
+    hipLaunchKernelGGL(dict_assign_kernel, dim3((unsigned)ranges), dim3(kDictBlock), 0, h->ctx->stream, dict_of(h),
+                       h->dict_cap);
+    WG_HIP(hipGetLastError());
     uint64_t c[3];
     WG_HIP(hipMemcpyAsync(c, h->dict_ctr, 24, hipMemcpyDeviceToHost, h->ctx->stream));
     WG_HIP(hipStreamSynchronize(h->ctx->stream));
-    if (!c[1]) {
-      h->dict_n = std::min<uint64_t>(c[0], h->dict_keys_cap);  // ids are below this
-      return WG_OK;
+    h->dict_n = c[0];
+    if (c[1] || 2 * c[0] > h->dict_cap) {  // out of probes, or over half full: grow and claim again
+      WG_TRY(dict_alloc(h, std::max<uint64_t>(2 * h->dict_cap, next_pow2(4 * c[0]))));
+      continue;
     }
-    // pairs with ids below keys_cap are published; later claims were dropped.
-    // Room for every row left without an id (an upper bound of the new
-    // pairs), at least twice the ids, at most a 2^23-id step beyond 4x.
-    const uint64_t keep = std::min<uint64_t>(c[0], h->dict_keys_cap);
-    uint64_t ids = next_pow2(keep + std::min<uint64_t>(c[1], n));
-    ids = std::max<uint64_t>(ids, 2 * h->dict_keys_cap);
-    ids = std::min<uint64_t>(ids, std::max<uint64_t>(4 * h->dict_keys_cap, 1ull << 23));
-    WG_TRY(dict_alloc(h, 2 * ids, keep));
+    hipLaunchKernelGGL(dict_lookup_kernel<Src>, dim3(grid), dim3(kDictBlock), 0, h->ctx->stream, src, n, dict_of(h));
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipMemcpyAsync(c, h->dict_ctr, 24, hipMemcpyDeviceToHost, h->ctx->stream));
+    WG_HIP(hipStreamSynchronize(h->ctx->stream));
+    if (!c[2]) return WG_OK;
   }
   set_error("key dictionary did not converge");
   return WG_ERR_DEVICE;
@@ -323,16 +340,7 @@ wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, ui
     WG_HIP(hipMalloc(&h->ids, std::max<uint64_t>(n_rows, 1) * 8));
     h->ids_cap = std::max<uint64_t>(n_rows, 1);
   }
-  if (n) {
-    const KeyDesc K = h->wkey;
-    const ColSet c = cs;
-    WG_TRY(encode_until_fits(h, n, [&](const WideDict& D) -> wg_status {
-      hipLaunchKernelGGL(wide_encode_kernel, dim3(grid_for(n, kDictBlock, 8192)), dim3(kDictBlock), 0,
-                         h->ctx->stream, c, K, sel, n, D, h->ids);
-      WG_HIP(hipGetLastError());
-      return WG_OK;
-    }));
-  }
+  WG_TRY(dict_encode(h, RowsSrc{cs, h->wkey, sel, h->ids}, n));
   const int ic = h->n_cols;
   cs.cols[ic] = ColDesc{h->ids, nullptr, WG_INT64, 0};
   cs.inline_word[ic] = -1;
@@ -342,13 +350,7 @@ wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, ui
 }
 
 wg_status wide_encode_pairs(wg_agg* h, const uint64_t* pairs, uint64_t n, uint64_t* ids_out) {
-  if (n == 0) return WG_OK;
-  return encode_until_fits(h, n, [&](const WideDict& D) -> wg_status {
-    hipLaunchKernelGGL(wide_pairs_kernel, dim3(grid_for(n, kDictBlock, 8192)), dim3(kDictBlock), 0, h->ctx->stream,
-                       pairs, n, D, ids_out);
-    WG_HIP(hipGetLastError());
-    return WG_OK;
-  });
+  return dict_encode(h, PairsSrc{pairs, ids_out}, n);
 }
 
 wg_status wide_dict_host(wg_agg* h, std::vector<uint64_t>* pairs) {
@@ -371,14 +373,7 @@ wg_status wide_remap_slots(wg_agg* dst, wg_agg* src, uint64_t** remapped) {
     set_error("hipMemcpyAsync");
     st = WG_ERR_DEVICE;
   }
-  if (st == WG_OK)
-    st = encode_until_fits(dst, src->groups + 1, [&](const WideDict& D) -> wg_status {
-      hipLaunchKernelGGL(wide_remap_kernel, dim3(grid_for(src->cap, kDictBlock, 8192)), dim3(kDictBlock), 0,
-                         dst->ctx->stream, (const uint64_t*)src->slots, src->cap, src->pg.stride,
-                         (const uint64_t*)src->dict_keys, D, copy);
-      WG_HIP(hipGetLastError());
-      return WG_OK;
-    });
+  if (st == WG_OK) st = dict_encode(dst, SlotsSrc{src->slots, src->pg.stride, src->dict_keys, copy}, src->cap);
   if (st != WG_OK) {
     (void)hipFree(copy);
     return st;
