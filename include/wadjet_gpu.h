@@ -283,6 +283,18 @@ typedef struct wg_join wg_join;
 wg_status wg_join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types,
                          const int32_t* build_scales, int32_t build_key, int32_t n_inline,
                          const int32_t* inline_cols, uint64_t build_hint, wg_join** out);
+/* The same join on two int-class key columns (tryEnableDualIntKey with
+ * dualIntHash / dualIntKeyFromVectors, join.go:592-661, 1621-1648; the chain
+ * walk that verifies both components, inlineDualIntProbe :2848-3130): a build
+ * row matches a probe row when both keys are equal as int64 (int32 columns
+ * widened), any number of build rows per pair (chains), and a row with a NULL
+ * in either key never matches. Each distinct (key0, key1) pair of the build
+ * gets an exact 64-bit id from a device dictionary; the table keys on it. The
+ * probe entry points then take the id column wg_join_dual_probe_key makes. */
+wg_status wg_join_create_dual(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types,
+                              const int32_t* build_scales, int32_t build_key0, int32_t build_key1,
+                              int32_t n_inline, const int32_t* inline_cols, uint64_t build_hint,
+                              wg_join** out);
 wg_status wg_join_destroy(wg_join* j);
 /* HashJoin.Build per batch (join.go:1019-1065): rows with a NULL key are
  * skipped. Build columns are retained on the device for the gathers. */
@@ -290,6 +302,16 @@ wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_row
 /* End of build (join.go:1126-1152). */
 wg_status wg_join_build_finish(wg_join* j);
 wg_status wg_join_build_rows(wg_join* j, uint64_t* rows);
+/* Probe key of a two-column-key join (after build_finish): for every active
+ * row (sel, or 0..n_rows) out_ids[row] = the id of its (key0, key1) pair, an id
+ * no build row has when the build lacks the pair; out_valid (bitmap words,
+ * (n_rows + 63) / 64) = key0 valid AND key1 valid. Pass
+ * {out_ids, out_valid, WG_INT64} as the probe key of wg_join_probe,
+ * wg_join_semi, wg_join_bloom_filter, or append it to the probe columns of
+ * wg_join_probe_agg. Device buffers. */
+wg_status wg_join_dual_probe_key(wg_join* j, const wg_column* probe_key0, const wg_column* probe_key1,
+                                 uint64_t n_rows, const uint32_t* sel, uint64_t n_sel, int64_t* out_ids,
+                                 uint64_t* out_valid);
 
 /* Resumable probe cursor (HashJoinProbe.res, join.go:2670-2836). */
 typedef struct wg_probe_cursor {

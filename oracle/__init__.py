@@ -600,3 +600,53 @@ def shuffle_row_bytes(vals, types, nulls=None) -> bytes:
         else:
             out += (int(v) & ((1 << (8 * w)) - 1)).to_bytes(w, "little")
     return bytes(out)
+
+
+# ------------------------------------------------ two-column join keys
+
+def _valid_at(valid, n):
+    if valid is None:
+        return np.ones(n, bool)
+    v = np.asarray(valid, np.uint64)
+    r = np.arange(n, dtype=np.int64)
+    return ((v[r >> 6] >> (r & 63).astype(np.uint64)) & np.uint64(1)) == 1
+
+
+def hash_join2(build_keys, probe_keys, build_valid=(None, None), probe_valid=(None, None), probe_sel=None):
+    """Inner join on two int-class keys (dualIntKeyFromVectors + the chain walk
+    that verifies both components, join.go:1621-1648, :2848-3130): every
+    (probe row, build row) with both keys equal as int64; a NULL in either key
+    never matches. Returns (probe_rows, build_rows) as uint32 arrays, probe
+    rows ascending, build rows ascending within a probe row."""
+    ba, bb = (np.asarray(k).astype(np.int64) for k in build_keys)
+    pa, pb = (np.asarray(k).astype(np.int64) for k in probe_keys)
+    bok = _valid_at(build_valid[0], len(ba)) & _valid_at(build_valid[1], len(ba))
+    pok = _valid_at(probe_valid[0], len(pa)) & _valid_at(probe_valid[1], len(pa))
+    index = {}
+    for r in np.nonzero(bok)[0]:
+        index.setdefault((int(ba[r]), int(bb[r])), []).append(int(r))
+    rows = np.arange(len(pa)) if probe_sel is None else np.asarray(probe_sel, np.int64)
+    out_p, out_b = [], []
+    for r in rows:
+        if not pok[r]:
+            continue
+        for b in index.get((int(pa[r]), int(pb[r])), ()):
+            out_p.append(int(r))
+            out_b.append(b)
+    return np.array(out_p, np.uint32), np.array(out_b, np.uint32)
+
+
+def semi_join2(build_keys, probe_keys, build_valid=(None, None), probe_valid=(None, None), probe_sel=None,
+               anti=False):
+    """Semi / anti join on two int-class keys: probe positions (ascending)
+    whose pair has a build row (anti: has none; a NULL in either probe key never
+    matches, so only the anti join keeps it)."""
+    ba, bb = (np.asarray(k).astype(np.int64) for k in build_keys)
+    pa, pb = (np.asarray(k).astype(np.int64) for k in probe_keys)
+    bok = _valid_at(build_valid[0], len(ba)) & _valid_at(build_valid[1], len(ba))
+    pok = _valid_at(probe_valid[0], len(pa)) & _valid_at(probe_valid[1], len(pa))
+    keys = {(int(x), int(y)) for x, y, ok in zip(ba, bb, bok) if ok}
+    rows = np.arange(len(pa)) if probe_sel is None else np.asarray(probe_sel, np.int64)
+    hit = np.array([bool(pok[r]) and (int(pa[r]), int(pb[r])) in keys for r in rows], bool)
+    keep = ~hit if anti else hit
+    return rows[keep].astype(np.uint32)

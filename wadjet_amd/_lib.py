@@ -102,6 +102,8 @@ SIGNATURES: dict[str, list] = {
     "wg_filter": [P, C.POINTER(Column), U64, P, U64, I32, I64, C.c_double, P, C.POINTER(U64)],
     "wg_join_create": [P, I32, C.POINTER(I32), C.POINTER(I32), I32, I32, C.POINTER(I32), U64, C.POINTER(P)],
     "wg_join_destroy": [P],
+    "wg_join_create_dual": [P, I32, C.POINTER(I32), C.POINTER(I32), I32, I32, I32, C.POINTER(I32), U64, C.POINTER(P)],
+    "wg_join_dual_probe_key": [P, C.POINTER(Column), C.POINTER(Column), U64, P, U64, P, P],
     "wg_join_build_append": [P, C.POINTER(Column), U64, I32],
     "wg_join_build_finish": [P],
     "wg_join_build_rows": [P, C.POINTER(U64)],

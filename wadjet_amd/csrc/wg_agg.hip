@@ -709,7 +709,7 @@ uint64_t agg_footprint(const wg_agg* h) {
   b += h->dense_cap * ((uint64_t)h->pg.stride * 8 + (uint64_t)h->n_sumd * 16 + 1);
   for (void* p : h->stage.dev) (void)p;
   b += h->stage.cap_rows * 16 * (uint64_t)h->stage.dev.size();
-  if (h->wide) b += h->dict_cap * 32 + h->dict_keys_cap * 16 + h->ids_cap * 8;
+  if (h->wide) b += kd_bytes(h->dict) + h->ids_cap * 8;
   return b;
 }
 
@@ -1406,7 +1406,7 @@ wg_status wg_agg_fetch(wg_agg* h, uint64_t offset, uint64_t count, void* const* 
     if (!kout) continue;
     hipLaunchKernelGGL(agg_key_kernel, dim3(g), dim3(256), 0, h->ctx->stream, (const uint64_t*)h->dense,
                        (const uint8_t*)h->dense_null, offset, count, h->pg.stride, UK, k, host ? tmp : kout,
-                       (uint8_t*)nullptr, h->wide ? (const uint64_t*)h->dict_keys : (const uint64_t*)nullptr);
+                       (uint8_t*)nullptr, h->wide ? (const uint64_t*)h->dict.keys : (const uint64_t*)nullptr);
     if (hipGetLastError() != hipSuccess) { set_error("agg_key_kernel launch failed"); st = WG_ERR_DEVICE; break; }
     st = finish(kout, tmp, count * type_bytes(UK.key_type[k]));
   }

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "wg_agg_dev.hpp"
+#include "wg_key_dict.hpp"
 
 namespace wg {
 
@@ -107,17 +108,12 @@ struct wg_agg {
   wg::HostStage stage;
   // composite keys of 65-128 bits: the key columns pack into a {lo, hi} pair
   // (buildPackedLayout, packed_hash.go:384-425) and a device dictionary maps
-  // each distinct pair to a dense 64-bit id; the id is the table's key (`key`
-  // then names the per-row id column, index n_cols of the consumed ColSet)
+  // each distinct pair to a 64-bit id; the id is the table's key (`key` then
+  // names the per-row id column, index n_cols of the consumed ColSet)
   bool wide = false;
-  wg::KeyDesc wkey{};              // the key columns' places in {lo, hi}
-  uint64_t* dict = nullptr;        // dict_cap slots of 4 words {state or id, lo, hi, -}
-  uint64_t dict_cap = 0;
-  uint64_t* dict_keys = nullptr;   // id -> {lo, hi}
-  uint64_t dict_keys_cap = 0;      // ids that fit
-  uint64_t dict_n = 0;             // ids handed out (as of the last read-back)
-  uint64_t* dict_ctr = nullptr;    // device: [0] ids handed out, [1] rows left without an id, [2] ids exhausted
-  uint64_t* ids = nullptr;         // per-row ids of the batch being consumed
+  wg::KeyDesc wkey{};      // the key columns' places in {lo, hi}
+  wg::KeyDict dict;
+  uint64_t* ids = nullptr;  // per-row ids of the batch being consumed
   uint64_t ids_cap = 0;
 };
 

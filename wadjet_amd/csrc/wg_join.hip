@@ -475,7 +475,7 @@ static wg_status join_reserve_rows(wg_join* j, uint64_t need) {
   if (need <= j->row_cap) return WG_OK;
   uint64_t cap = std::max<uint64_t>(need, j->row_cap * 2);
   cap = std::max<uint64_t>(cap, 1 << 16);
-  for (int c = 0; c < j->n_cols; ++c) {
+  for (int c = 0; c < j->n_ret; ++c) {
     void* d = nullptr;
     WG_HIP(hipMalloc(&d, cap * type_bytes(j->types[c])));
     if (j->col_data[c]) {
@@ -583,6 +583,33 @@ static ColDesc retained_col(const wg_join* j, int c) {
   return ColDesc{j->col_data[c], j->has_valid[c] ? j->col_valid[c] : nullptr, j->types[c], j->scales[c]};
 }
 
+// Validity of two-column keys: a row's pair is NULL when either key is
+// (dualIntKeyFromVectors, join.go:1630-1633); words [w0, w1) of the bitmaps.
+__global__ void and_words_kernel(const uint64_t* a, const uint64_t* b, uint64_t w0, uint64_t w1, uint64_t* out) {
+  for (uint64_t w = w0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < w1; w += (uint64_t)gridDim.x * blockDim.x)
+    out[w] = (a ? a[w] : ~0ull) & (b ? b[w] : ~0ull);
+}
+
+// The appended build rows [rows, rows + n) get their pairs' ids (new pairs
+// enter the dictionary) in the retained id column, and its validity.
+static wg_status dual_encode_build(wg_join* j, uint64_t n) {
+  ColSet cs{};
+  cs.n_cols = j->n_cols;
+  cs.n_probe = j->n_cols;
+  for (int c = 0; c < j->n_cols; ++c) {
+    cs.cols[c] = retained_col(j, c);
+    cs.inline_word[c] = -1;
+  }
+  const int ic = j->n_cols;
+  const uint64_t w0 = j->rows / 64, w1 = (j->rows + n + 63) / 64;
+  hipLaunchKernelGGL(and_words_kernel, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, j->ctx->stream,
+                     (const uint64_t*)j->col_valid[j->dkey.col[0]], (const uint64_t*)j->col_valid[j->dkey.col[1]], w0,
+                     w1, j->col_valid[ic]);
+  WG_HIP(hipGetLastError());
+  j->has_valid[ic] = true;
+  return kd_encode_rows(j->ctx, j->dict, cs, j->dkey, nullptr, n, true, (uint64_t*)j->col_data[ic], j->rows);
+}
+
 // Does the fused probe -> aggregate have the streamlined kernel's shape
 // (wg_join_agg.hpp)? Fills the parts of Q that describe the query.
 static bool lean_plan(const wg_join* j, const wg_agg* agg, const wg_column* probe_cols, int n_probe,
@@ -659,37 +686,63 @@ using namespace wg;
 
 extern "C" {
 
-wg_status wg_join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types, const int32_t* build_scales,
-                         int32_t build_key, int32_t n_inline, const int32_t* inline_cols, uint64_t build_hint,
-                         wg_join** out) {
+static wg_status join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types,
+                             const int32_t* build_scales, int32_t build_key, int32_t build_key2, int32_t n_inline,
+                             const int32_t* inline_cols, uint64_t build_hint, wg_join** out) {
   clear_error();
   WG_CHECK(ctx && out && build_types, WG_ERR_INVALID, "null argument");
-  WG_CHECK(n_build_cols > 0 && n_build_cols <= kMaxCols, WG_ERR_UNSUPPORTED, "%d build columns", n_build_cols);
-  WG_CHECK(build_key >= 0 && build_key < n_build_cols, WG_ERR_INVALID, "build key column %d", build_key);
-  WG_CHECK(is_int_class(build_types[build_key]), WG_ERR_UNSUPPORTED, "join key of type %d not on the GPU path",
-           build_types[build_key]);
+  const bool dual = build_key2 >= 0;
+  WG_CHECK(n_build_cols > 0 && n_build_cols + (dual ? 1 : 0) <= kMaxCols, WG_ERR_UNSUPPORTED, "%d build columns",
+           n_build_cols);
+  for (int key : {build_key, build_key2}) {
+    if (key < 0 && key == build_key2) continue;
+    WG_CHECK(key >= 0 && key < n_build_cols, WG_ERR_INVALID, "build key column %d", key);
+    WG_CHECK(is_int_class(build_types[key]), WG_ERR_UNSUPPORTED, "join key of type %d not on the GPU path",
+             build_types[key]);
+  }
   WG_CHECK(n_inline >= 0 && n_inline <= kMaxInline, WG_ERR_INVALID, "%d inline columns (max %d)", n_inline, kMaxInline);
   wg_join* j = new wg_join();
   j->ctx = ctx;
   j->n_cols = n_build_cols;
+  j->n_ret = n_build_cols;
   for (int c = 0; c < n_build_cols; ++c) {
-    WG_CHECK(build_types[c] >= WG_INT32 && build_types[c] <= WG_UINT16, WG_ERR_UNSUPPORTED, "build column type %d", build_types[c]);
+    if (!(build_types[c] >= WG_INT32 && build_types[c] <= WG_UINT16)) {
+      delete j;
+      WG_CHECK(false, WG_ERR_UNSUPPORTED, "build column type %d", build_types[c]);
+    }
     j->types[c] = build_types[c];
     j->scales[c] = build_scales ? build_scales[c] : 0;
   }
   j->key_col = build_key;
+  if (dual) {
+    // the table keys on the pair's id, a retained column after the batch's own
+    j->dual = true;
+    j->dkey.n = 2;
+    j->dkey.col[0] = build_key;
+    j->dkey.col[1] = build_key2;
+    j->dkey.word[1] = 1;
+    j->dkey.key_type[0] = build_types[build_key];
+    j->dkey.key_type[1] = build_types[build_key2];
+    j->key_col = n_build_cols;
+    j->types[n_build_cols] = WG_INT64;
+    j->n_ret = n_build_cols + 1;
+  }
   j->n_inline = n_inline;
   for (int i = 0; i < n_inline; ++i) {
     const int c = inline_cols[i];
-    WG_CHECK(c >= 0 && c < n_build_cols && j->types[c] != WG_DECIMAL128, WG_ERR_INVALID, "inline column %d", c);
+    if (!(c >= 0 && c < n_build_cols && j->types[c] != WG_DECIMAL128)) {
+      delete j;
+      WG_CHECK(false, WG_ERR_INVALID, "inline column %d", c);
+    }
     j->inline_cols[i] = c;
   }
   j->stride = (int32_t)next_pow2(2 + n_inline);
   WG_HIP(hipMalloc(&j->counter, 64));
   WG_TRY(zero_dev(ctx, j->counter, 8));
   wg_status st = join_alloc_table(j, next_pow2(std::max<uint64_t>(build_hint, 1024) * 2));
+  if (st == WG_OK && dual) st = kd_init(ctx, j->dict, build_hint);
   if (st != WG_OK) {
-    delete j;
+    wg_join_destroy(j);
     return st;
   }
   WG_HIP(hipStreamSynchronize(ctx->stream));
@@ -697,13 +750,29 @@ wg_status wg_join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build
   return WG_OK;
 }
 
+wg_status wg_join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types, const int32_t* build_scales,
+                         int32_t build_key, int32_t n_inline, const int32_t* inline_cols, uint64_t build_hint,
+                         wg_join** out) {
+  return join_create(ctx, n_build_cols, build_types, build_scales, build_key, -1, n_inline, inline_cols, build_hint,
+                     out);
+}
+
+wg_status wg_join_create_dual(wg_ctx* ctx, int32_t n_build_cols, const int32_t* build_types,
+                              const int32_t* build_scales, int32_t build_key0, int32_t build_key1, int32_t n_inline,
+                              const int32_t* inline_cols, uint64_t build_hint, wg_join** out) {
+  WG_CHECK(build_key1 >= 0 && build_key1 != build_key0, WG_ERR_INVALID, "second build key column %d", build_key1);
+  return join_create(ctx, n_build_cols, build_types, build_scales, build_key0, build_key1, n_inline, inline_cols,
+                     build_hint, out);
+}
+
 wg_status wg_join_destroy(wg_join* j) {
   if (!j) return WG_OK;
   (void)hipStreamSynchronize(j->ctx->stream);
-  for (int c = 0; c < j->n_cols; ++c) {
+  for (int c = 0; c < j->n_ret; ++c) {
     (void)hipFree(j->col_data[c]);
     (void)hipFree(j->col_valid[c]);
   }
+  if (j->dual) kd_free(j->dict);
   (void)hipFree(j->next);
   (void)hipFree(j->slots);
   (void)hipFree(j->counter);
@@ -733,6 +802,7 @@ wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_row
     WG_HIP(hipMemcpyAsync((char*)j->col_data[c] + j->rows * w, cols[c].data, n_rows * w, kind, j->ctx->stream));
     if (cols[c].validity) WG_TRY(append_validity(j, c, cols[c].validity, n_rows, mem != WG_MEM_HOST));
   }
+  if (j->dual) WG_TRY(dual_encode_build(j, n_rows));
   BuildParams B{};
   B.key = retained_col(j, j->key_col);
   for (int i = 0; i < j->n_inline; ++i) B.inl[i] = retained_col(j, j->inline_cols[i]);
@@ -748,6 +818,36 @@ wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_row
   WG_HIP(hipGetLastError());
   WG_HIP(hipStreamSynchronize(j->ctx->stream));
   j->rows += n_rows;
+  return WG_OK;
+}
+
+wg_status wg_join_dual_probe_key(wg_join* j, const wg_column* probe_key0, const wg_column* probe_key1,
+                                 uint64_t n_rows, const uint32_t* sel, uint64_t n_sel, int64_t* out_ids,
+                                 uint64_t* out_valid) {
+  clear_error();
+  WG_CHECK(j && probe_key0 && probe_key1 && out_ids && out_valid, WG_ERR_INVALID, "null argument");
+  WG_CHECK(j->dual, WG_ERR_STATE, "not a two-column-key join");
+  WG_CHECK(j->finished, WG_ERR_STATE, "probe before build_finish");
+  WG_CHECK(is_int_class(probe_key0->type) && is_int_class(probe_key1->type), WG_ERR_UNSUPPORTED,
+           "probe key types %d, %d", probe_key0->type, probe_key1->type);
+  const uint64_t n = sel ? n_sel : n_rows;
+  const uint64_t words = (n_rows + 63) / 64;
+  if (words) {
+    hipLaunchKernelGGL(and_words_kernel, dim3(grid_for(words, 256)), dim3(256), 0, j->ctx->stream,
+                       probe_key0->validity, probe_key1->validity, (uint64_t)0, words, out_valid);
+    WG_HIP(hipGetLastError());
+  }
+  ColSet cs{};
+  cs.n_cols = 2;
+  cs.n_probe = 2;
+  cs.cols[0] = ColDesc{probe_key0->data, probe_key0->validity, probe_key0->type, probe_key0->scale};
+  cs.cols[1] = ColDesc{probe_key1->data, probe_key1->validity, probe_key1->type, probe_key1->scale};
+  cs.inline_word[0] = cs.inline_word[1] = -1;
+  KeyDesc K = j->dkey;
+  K.col[0] = 0;
+  K.col[1] = 1;
+  WG_TRY(kd_lookup_rows(j->ctx, j->dict, cs, K, sel, n, true, (uint64_t*)out_ids, 0));
+  WG_HIP(hipStreamSynchronize(j->ctx->stream));
   return WG_OK;
 }
 

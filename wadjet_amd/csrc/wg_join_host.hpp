@@ -2,6 +2,7 @@
 // batched / served runners that probe a built dimension table).
 #pragma once
 #include "wg_join_dev.hpp"
+#include "wg_key_dict.hpp"
 
 struct wg_join {
   wg_ctx* ctx = nullptr;
@@ -9,6 +10,13 @@ struct wg_join {
   int32_t types[wg::kMaxCols] = {};
   int32_t scales[wg::kMaxCols] = {};
   int32_t key_col = 0;
+  // two-column keys (dualIntKeyFromVectors, join.go:1621-1648): each distinct
+  // (key0, key1) pair has an id in `dict`; the retained column n_cols holds
+  // the build rows' ids (NULL when either key is) and is the table's key
+  bool dual = false;
+  wg::KeyDesc dkey{};  // the two build key columns, lo = key0, hi = key1
+  wg::KeyDict dict;
+  int32_t n_ret = 0;  // retained columns: n_cols, plus the id column when dual
   int32_t n_inline = 0;
   int32_t inline_cols[wg::kMaxInline] = {};
   int32_t stride = 2;

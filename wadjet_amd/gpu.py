@@ -327,8 +327,9 @@ def filter_sel(ctx: Context, col: Col, n_rows: int, op: int, i64: int = 0, f64: 
 class HashJoin:
     """Inner hash join on one int-class key (wg_join_*): build, then probe."""
 
-    def __init__(self, ctx: Context, build_types: Sequence[int], build_key: int, inline: Sequence[int] = (),
+    def __init__(self, ctx: Context, build_types: Sequence[int], build_key, inline: Sequence[int] = (),
                  scales: Optional[Sequence[int]] = None, build_hint: int = 0):
+        """build_key: a column index, or a pair of them (two-column key)."""
         self.ctx = ctx
         self.types = list(build_types)
         n = len(build_types)
@@ -336,9 +337,23 @@ class HashJoin:
         s = (C.c_int32 * n)(*(scales or [0] * n))
         il = (C.c_int32 * max(len(inline), 1))(*inline)
         h = C.c_void_p()
-        call("wg_join_create", ctx.h, n, t, s, build_key, len(inline), il, build_hint, C.byref(h))
+        self.dual = isinstance(build_key, (tuple, list))
+        if self.dual:
+            call("wg_join_create_dual", ctx.h, n, t, s, build_key[0], build_key[1], len(inline), il, build_hint,
+                 C.byref(h))
+        else:
+            call("wg_join_create", ctx.h, n, t, s, build_key, len(inline), il, build_hint, C.byref(h))
         self.h = h
         ctx._adopt(self)
+
+    def dual_probe_key(self, key0: Col, key1: Col, n_rows: int, sel=None) -> Col:
+        """The id column a two-column-key join probes with (wg_join_dual_probe_key)."""
+        ids = self.ctx.alloc(max(n_rows, 1) * 8)
+        valid = self.ctx.alloc(max((n_rows + 63) // 64, 1) * 8)
+        sp, ns = (None, 0) if sel is None else (sel[0].ptr, int(sel[1]))
+        k0, k1 = key0.c(), key1.c()
+        call("wg_join_dual_probe_key", self.h, C.byref(k0), C.byref(k1), n_rows, sp, ns, ids.ptr, valid.ptr)
+        return Col(ids, L.INT64, valid)
 
     def close(self):
         if self.h and self.ctx.h:
