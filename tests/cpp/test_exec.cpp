@@ -325,6 +325,64 @@ static void test_bloom_filter_op(wg_ctx* g) {
   EXPECT(!half.Disabled(), "a filter rejecting ~half the rows must stay on");
 }
 
+static void test_dual_key_join_kats(wg_ctx* g) {
+  Context ctx;
+  // TestHashJoin_DualIntKey (dual_int_test.go:11-75): 2 of 3 probe rows match
+  auto build = make_batch({{"ps_partkey", TypeID::Int64}, {"ps_suppkey", TypeID::Int64}, {"ps_value", TypeID::Float64}}, 3);
+  for (int i = 0; i < 3; ++i) {
+    build->columns[0]->i64[i] = i + 1;
+    build->columns[1]->i64[i] = 10 * (i + 1);
+    build->columns[2]->f64[i] = 100.0 * (i + 1);
+  }
+  auto probe = make_batch({{"l_partkey", TypeID::Int64}, {"l_suppkey", TypeID::Int64}, {"l_qty", TypeID::Int64}}, 3);
+  const int64_t pk[] = {1, 2, 99}, sk[] = {10, 20, 99};
+  for (int i = 0; i < 3; ++i) {
+    probe->columns[0]->i64[i] = pk[i];
+    probe->columns[1]->i64[i] = sk[i];
+    probe->columns[2]->i64[i] = 5 * (i + 1);
+  }
+  {
+    GpuHashJoin hj(g, std::vector<std::string>{"l_partkey", "l_suppkey"},
+                   std::vector<std::string>{"ps_partkey", "ps_suppkey"});
+    BatchSource bs({build});
+    OK(hj.Build(ctx, &bs));
+    auto pr = hj.Probe();
+    BatchSource ps({probe});
+    CollectSink sink;
+    Pipeline p{&ps, {pr.get()}, &sink, 0};
+    OK(p.Run(ctx));
+    EXPECT(sink.rows() == 2, "dual-key inner join rows %d, want 2", sink.rows());
+    for (auto& ob : sink.batches)
+      for (int i = 0; i < ob->len; ++i)
+        EXPECT(ob->columns[0]->i64[i] == ob->columns[3]->i64[i] && ob->columns[1]->i64[i] == ob->columns[4]->i64[i],
+               "probe keys differ from build keys");
+  }
+  // TestHashJoin_DualIntKey_SemiAnti (:77-140): semi keeps 2, anti 1
+  auto build2 = make_batch({{"ps_partkey", TypeID::Int64}, {"ps_suppkey", TypeID::Int64}}, 2);
+  for (int i = 0; i < 2; ++i) {
+    build2->columns[0]->i64[i] = i + 1;
+    build2->columns[1]->i64[i] = 10 * (i + 1);
+  }
+  auto probe2 = make_batch({{"l_partkey", TypeID::Int64}, {"l_suppkey", TypeID::Int64}}, 3);
+  const int64_t pk2[] = {1, 3, 2}, sk2[] = {10, 30, 20};
+  for (int i = 0; i < 3; ++i) {
+    probe2->columns[0]->i64[i] = pk2[i];
+    probe2->columns[1]->i64[i] = sk2[i];
+  }
+  for (JoinType t : {JoinType::Semi, JoinType::Anti}) {
+    GpuHashJoin hj(g, std::vector<std::string>{"l_partkey", "l_suppkey"},
+                   std::vector<std::string>{"ps_partkey", "ps_suppkey"}, 0, t);
+    BatchSource bs({build2});
+    OK(hj.Build(ctx, &bs));
+    auto pr = hj.Probe();
+    BatchPtr out;
+    OK(pr->Execute(ctx, probe2, &out));
+    const size_t want = t == JoinType::Semi ? 2 : 1;
+    EXPECT(out && out->has_sel && out->sel.size() == want, "%s: %zu rows, want %zu",
+           t == JoinType::Semi ? "semi" : "anti", out ? out->sel.size() : (size_t)0, want);
+  }
+}
+
 // ------------------------------------------------------------ shuffle tests
 
 static uint64_t fnv1a64(const unsigned char* p, size_t n, uint64_t h = 14695981039346656037ull) {
@@ -509,6 +567,7 @@ int main(int argc, char** argv) {
     test_hash_join_kats(g);
     test_semi_anti_kats(g);
     test_bloom_filter_op(g);
+    test_dual_key_join_kats(g);
     test_capped_partial_agg(g);
     test_shuffle_router(g);
     wg_ctx_destroy(g);

@@ -451,13 +451,31 @@ void PushDownFiltersIntoGpuAggregate(Pipeline& p) {
 
 GpuHashJoin::GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint,
                          JoinType type)
-    : ctx_(ctx), probe_key_(std::move(probe_key)), build_key_(std::move(build_key)), hint_(build_hint), type_(type) {}
+    : GpuHashJoin(ctx, std::vector<std::string>{std::move(probe_key)}, std::vector<std::string>{std::move(build_key)},
+                  build_hint, type) {}
+
+GpuHashJoin::GpuHashJoin(wg_ctx* ctx, std::vector<std::string> probe_keys, std::vector<std::string> build_keys,
+                         uint64_t build_hint, JoinType type)
+    : ctx_(ctx), probe_keys_(std::move(probe_keys)), build_keys_(std::move(build_keys)), hint_(build_hint),
+      type_(type) {}
 
 GpuHashJoin::~GpuHashJoin() {
   if (j_) wg_join_destroy(j_);
 }
 
+Status GpuHashJoin::probe_key_idx(const RecordBatch& in, std::vector<int>* idx) const {
+  idx->clear();
+  for (auto& k : probe_keys_) {
+    const int ki = in.column_index(k);
+    if (ki < 0) return Status::Error("probe key \"" + k + "\" not in the probe schema");
+    idx->push_back(ki);
+  }
+  return Status::OK();
+}
+
 Status GpuHashJoin::Build(Context& ctx, Source* build) {
+  if (build_keys_.empty() || build_keys_.size() > 2 || build_keys_.size() != probe_keys_.size())
+    return Status::Error("a GPU join takes one or two key columns per side");
   TRY(build->Init(ctx));
   for (;;) {
     if (ctx.cancelled) return Status::Error("context canceled");
@@ -473,16 +491,24 @@ Status GpuHashJoin::Build(Context& ctx, Source* build) {
       for (auto& v : b->columns) c->columns.push_back(gather(*v, b->sel.data(), b->sel.size()));
     }
     if (!j_) {
-      const int ki = c->column_index(build_key_);
-      if (ki < 0) return Status::Error("build key \"" + build_key_ + "\" not in the build schema");
+      std::vector<int> ki;
+      for (auto& k : build_keys_) {
+        ki.push_back(c->column_index(k));
+        if (ki.back() < 0) return Status::Error("build key \"" + k + "\" not in the build schema");
+      }
       build_schema_ = c->schema;
       std::vector<int32_t> types, scales;
       for (auto& col : build_schema_) {
         types.push_back(wg_type_of(col.type));
         scales.push_back(col.scale);
       }
-      TRY(wrap(wg_join_create(ctx_, (int32_t)types.size(), types.data(), scales.data(), ki, 0, nullptr, hint_, &j_),
-               "wg_join_create"));
+      if (ki.size() == 1)
+        TRY(wrap(wg_join_create(ctx_, (int32_t)types.size(), types.data(), scales.data(), ki[0], 0, nullptr, hint_, &j_),
+                 "wg_join_create"));
+      else
+        TRY(wrap(wg_join_create_dual(ctx_, (int32_t)types.size(), types.data(), scales.data(), ki[0], ki[1], 0,
+                                     nullptr, hint_, &j_),
+                 "wg_join_create_dual"));
       for (auto& col : build_schema_) build_cols_.push_back(Vector::make(col.type, 0, col.scale));
     }
     std::vector<wg_column> cols;
@@ -512,21 +538,81 @@ Status GpuHashJoin::Build(Context& ctx, Source* build) {
 
 std::unique_ptr<GpuHashJoinProbe> GpuHashJoin::Probe() { return std::make_unique<GpuHashJoinProbe>(this); }
 
+// ------------------------------------------------------ DeviceProbeKey
+
+void DeviceProbeKey::Release() {
+  for (void*& p : buf_) {
+    if (p) (void)wg_free(ctx_, p);
+    p = nullptr;
+  }
+  if (sel_) (void)wg_free(ctx_, sel_);
+  sel_ = nullptr;
+  cap_ = 0;
+}
+
+Status DeviceProbeKey::Prepare(wg_ctx* ctx, wg_join* j, const std::vector<int>& key_idx, const RecordBatch& in,
+                               wg_column* key, const uint32_t** sel) {
+  const uint64_t n = (uint64_t)in.len;
+  if (n > cap_ || ctx != ctx_) {
+    Release();
+    ctx_ = ctx;
+    cap_ = std::max<uint64_t>(n, batch::kDefaultBatchSize);
+    const uint64_t vw = (cap_ + 63) / 64 * 8;
+    const uint64_t sizes[6] = {cap_ * 8, cap_ * 8, vw, vw, cap_ * 8, vw};
+    for (int i = 0; i < 6; ++i) TRY(wrap(wg_malloc(ctx, sizes[i], &buf_[i]), "wg_malloc"));
+    TRY(wrap(wg_malloc(ctx, cap_ * 4, (void**)&sel_), "wg_malloc"));
+  }
+  *sel = nullptr;
+  if (in.has_sel) {
+    if (!in.sel.empty()) TRY(wrap(wg_memcpy_h2d(ctx, sel_, in.sel.data(), in.sel.size() * 4), "upload sel"));
+    *sel = sel_;
+  }
+  wg_column cols[2];
+  for (size_t k = 0; k < key_idx.size(); ++k) {
+    const Vector& v = *in.columns[key_idx[k]];
+    TRY(wrap(wg_memcpy_h2d(ctx, buf_[k], data_of(v), n * elem_bytes(v.type)), "upload probe key"));
+    cols[k] = col_of(v);
+    cols[k].data = buf_[k];
+    cols[k].validity = nullptr;
+    if (v.nulls.has_nulls()) {
+      TRY(wrap(wg_memcpy_h2d(ctx, buf_[2 + k], v.nulls.words.data(), v.nulls.words.size() * 8), "upload validity"));
+      cols[k].validity = (const uint64_t*)buf_[2 + k];
+    }
+  }
+  if (key_idx.size() == 1) {
+    *key = cols[0];
+    return Status::OK();
+  }
+  TRY(wrap(wg_join_dual_probe_key(j, &cols[0], &cols[1], n, *sel, in.has_sel ? in.sel.size() : 0,
+                                  (int64_t*)buf_[4], (uint64_t*)buf_[5]),
+           "wg_join_dual_probe_key"));
+  *key = wg_column{buf_[4], (const uint64_t*)buf_[5], WG_INT64, 0};
+  return Status::OK();
+}
+
+// ------------------------------------------------------ GpuHashJoinProbe
+
 GpuHashJoinProbe::GpuHashJoinProbe(GpuHashJoin* j) : join_(j) {}
 
 GpuHashJoinProbe::~GpuHashJoinProbe() { (void)Close(); }
 
 Status GpuHashJoinProbe::Close() {
-  void* ps[] = {d_key_, d_valid_, d_sel_, d_out_p_, d_out_b_};
-  for (void* p : ps)
+  dkey_.Release();
+  for (uint32_t* p : {d_out_p_, d_out_b_})
     if (p) wg_free(join_->ctx_, p);
-  d_key_ = d_valid_ = nullptr;
-  d_key_ = nullptr;
-  d_valid_ = nullptr;
-  d_sel_ = nullptr;
   d_out_p_ = d_out_b_ = nullptr;
-  cap_rows_ = 0;
+  out_cap_ = 0;
   return Status::OK();
+}
+
+Status GpuHashJoinProbe::ensure_out(uint64_t rows) {
+  if (rows <= out_cap_) return Status::OK();
+  for (uint32_t* p : {d_out_p_, d_out_b_})
+    if (p) wg_free(join_->ctx_, p);
+  d_out_p_ = d_out_b_ = nullptr;
+  out_cap_ = rows;
+  TRY(wrap(wg_malloc(join_->ctx_, rows * 4, (void**)&d_out_p_), "wg_malloc"));
+  return wrap(wg_malloc(join_->ctx_, rows * 4, (void**)&d_out_b_), "wg_malloc");
 }
 
 Status GpuHashJoinProbe::Execute(Context& ctx, const BatchPtr& in, BatchPtr* out) {
@@ -534,23 +620,10 @@ Status GpuHashJoinProbe::Execute(Context& ctx, const BatchPtr& in, BatchPtr* out
   pending_ = false;
   if (join_->type_ != JoinType::Inner) return semi_anti(in, out);
   if (!join_->j_) return Status::OK();  // empty build
-  key_idx_ = in->column_index(join_->probe_key_);
-  if (key_idx_ < 0) return Status::Error("probe key \"" + join_->probe_key_ + "\" not in the probe schema");
-  const Vector& k = *in->columns[key_idx_];
-  const uint64_t n = (uint64_t)in->len;
-  wg_ctx* c = join_->ctx_;
-  if (n > cap_rows_) {
-    (void)Close();
-    cap_rows_ = std::max<uint64_t>(n, batch::kDefaultBatchSize);
-    TRY(wrap(wg_malloc(c, cap_rows_ * 8, &d_key_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, (cap_rows_ + 63) / 64 * 8, (void**)&d_valid_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_sel_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, kMaxProbeOutputRows * 4, (void**)&d_out_p_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, kMaxProbeOutputRows * 4, (void**)&d_out_b_), "wg_malloc"));
-  }
-  TRY(wrap(wg_memcpy_h2d(c, d_key_, data_of(k), n * elem_bytes(k.type)), "upload probe key"));
-  if (k.nulls.has_nulls()) TRY(wrap(wg_memcpy_h2d(c, d_valid_, k.nulls.words.data(), k.nulls.words.size() * 8), "upload validity"));
-  if (in->has_sel && !in->sel.empty()) TRY(wrap(wg_memcpy_h2d(c, d_sel_, in->sel.data(), in->sel.size() * 4), "upload sel"));
+  std::vector<int> ki;
+  TRY(join_->probe_key_idx(*in, &ki));
+  TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
+  TRY(ensure_out(kMaxProbeOutputRows));
   cur_ = in;
   cursor_ = wg_probe_cursor{0, -1, 0, 0};
   if (bounded_) {
@@ -602,8 +675,6 @@ Status GpuHashJoinProbe::NextOutput(Context&, BatchPtr* out) {
   return Status::OK();
 }
 
-// One bounded chunk: probe on the device, then gather the probe columns and the
-// retained build columns on the host (gatherVector / gatherBuildVector).
 // Semi / anti: the input batch with a new Sel of the probe rows that have
 // (semi) / have no (anti) build row; nullptr when no row remains
 // (join.go:3468-3478). An empty build keeps no row (semi) / every row (anti).
@@ -615,49 +686,31 @@ Status GpuHashJoinProbe::semi_anti(const BatchPtr& in, BatchPtr* out) {
     if (anti) *out = in;
     return Status::OK();
   }
-  key_idx_ = in->column_index(join_->probe_key_);
-  if (key_idx_ < 0) return Status::Error("probe key \"" + join_->probe_key_ + "\" not in the probe schema");
-  const Vector& k = *in->columns[key_idx_];
-  wg_ctx* c = join_->ctx_;
-  if ((uint64_t)in->len > cap_rows_) {
-    (void)Close();
-    cap_rows_ = std::max<uint64_t>((uint64_t)in->len, batch::kDefaultBatchSize);
-    TRY(wrap(wg_malloc(c, cap_rows_ * 8, &d_key_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, (cap_rows_ + 63) / 64 * 8, (void**)&d_valid_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_sel_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, std::max<uint64_t>(cap_rows_, kMaxProbeOutputRows) * 4, (void**)&d_out_p_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, kMaxProbeOutputRows * 4, (void**)&d_out_b_), "wg_malloc"));
-  }
-  TRY(wrap(wg_memcpy_h2d(c, d_key_, data_of(k), (uint64_t)in->len * elem_bytes(k.type)), "upload probe key"));
-  if (k.nulls.has_nulls())
-    TRY(wrap(wg_memcpy_h2d(c, d_valid_, k.nulls.words.data(), k.nulls.words.size() * 8), "upload validity"));
-  if (in->has_sel) TRY(wrap(wg_memcpy_h2d(c, d_sel_, in->sel.data(), in->sel.size() * 4), "upload sel"));
-  wg_column key = col_of(k);
-  key.data = d_key_;
-  key.validity = k.nulls.has_nulls() ? d_valid_ : nullptr;
+  std::vector<int> ki;
+  TRY(join_->probe_key_idx(*in, &ki));
+  TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
+  TRY(ensure_out(std::max<uint64_t>((uint64_t)in->len, kMaxProbeOutputRows)));
   uint64_t got = 0;
-  TRY(wrap(wg_join_semi(join_->j_, &key, (uint64_t)in->len, in->has_sel ? d_sel_ : nullptr,
-                        in->has_sel ? in->sel.size() : 0, anti ? 1 : 0, d_out_p_, &got),
+  TRY(wrap(wg_join_semi(join_->j_, &key_, (uint64_t)in->len, dsel_, in->has_sel ? in->sel.size() : 0, anti ? 1 : 0,
+                        d_out_p_, &got),
            "wg_join_semi"));
   if (got == 0) return Status::OK();
   auto o = std::make_shared<RecordBatch>(*in);  // columns shared, Sel replaced
   o->sel.resize(got);
-  TRY(wrap(wg_memcpy_d2h(c, o->sel.data(), d_out_p_, got * 4), "download sel"));
+  TRY(wrap(wg_memcpy_d2h(join_->ctx_, o->sel.data(), d_out_p_, got * 4), "download sel"));
   o->has_sel = true;
   *out = o;
   return Status::OK();
 }
 
+// One bounded chunk: probe on the device, then gather the probe columns and the
+// retained build columns on the host (gatherVector / gatherBuildVector).
 Status GpuHashJoinProbe::emit(BatchPtr* out) {
   *out = nullptr;
   const RecordBatch& in = *cur_;
-  const Vector& k = *in.columns[key_idx_];
-  wg_column key = col_of(k);
-  key.data = d_key_;
-  key.validity = k.nulls.has_nulls() ? d_valid_ : nullptr;
   uint64_t got = 0;
-  TRY(wrap(wg_join_probe(join_->j_, &key, (uint64_t)in.len, in.has_sel ? d_sel_ : nullptr,
-                         in.has_sel ? in.sel.size() : 0, d_out_p_, d_out_b_, kMaxProbeOutputRows, &cursor_, &got),
+  TRY(wrap(wg_join_probe(join_->j_, &key_, (uint64_t)in.len, dsel_, in.has_sel ? in.sel.size() : 0, d_out_p_,
+                         d_out_b_, kMaxProbeOutputRows, &cursor_, &got),
            "wg_join_probe"));
   if (in.has_sel && in.sel.empty()) cursor_.done = 1;
   if (got == 0) return Status::OK();
@@ -689,12 +742,8 @@ GpuBloomFilterOp::GpuBloomFilterOp(GpuHashJoin* j) : join_(j) {}
 GpuBloomFilterOp::~GpuBloomFilterOp() { (void)Close(); }
 
 Status GpuBloomFilterOp::Close() {
-  wg_ctx* c = join_->ctx_;
-  for (void* p : {d_key_, (void*)d_valid_, (void*)d_sel_, (void*)d_out_})
-    if (p) (void)wg_free(c, p);
-  d_key_ = nullptr;
-  d_valid_ = nullptr;
-  d_sel_ = nullptr;
+  dkey_.Release();
+  if (d_out_) (void)wg_free(join_->ctx_, d_out_);
   d_out_ = nullptr;
   cap_rows_ = 0;
   return Status::OK();
@@ -708,28 +757,21 @@ Status GpuBloomFilterOp::Execute(Context&, const BatchPtr& in, BatchPtr* out) {
     TRY(wrap(wg_join_bloom_build(join_->j_), "wg_join_bloom_build"));
     join_->bloom_built_ = true;
   }
-  const int ki = in->column_index(join_->probe_key_);
-  if (ki < 0) return Status::OK();  // key column not in this batch: pass every row (:66-76)
-  const Vector& k = *in->columns[ki];
+  std::vector<int> ki;
+  if (!join_->probe_key_idx(*in, &ki).ok()) return Status::OK();  // key column not in this batch: pass every row (:66-76)
   wg_ctx* c = join_->ctx_;
   if ((uint64_t)in->len > cap_rows_) {
-    (void)Close();
+    if (d_out_) (void)wg_free(c, d_out_);
+    d_out_ = nullptr;
     cap_rows_ = std::max<uint64_t>((uint64_t)in->len, batch::kDefaultBatchSize);
-    TRY(wrap(wg_malloc(c, cap_rows_ * 8, &d_key_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, (cap_rows_ + 63) / 64 * 8, (void**)&d_valid_), "wg_malloc"));
-    TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_sel_), "wg_malloc"));
     TRY(wrap(wg_malloc(c, cap_rows_ * 4, (void**)&d_out_), "wg_malloc"));
   }
-  TRY(wrap(wg_memcpy_h2d(c, d_key_, data_of(k), (uint64_t)in->len * elem_bytes(k.type)), "upload probe key"));
-  if (k.nulls.has_nulls())
-    TRY(wrap(wg_memcpy_h2d(c, d_valid_, k.nulls.words.data(), k.nulls.words.size() * 8), "upload validity"));
-  if (in->has_sel) TRY(wrap(wg_memcpy_h2d(c, d_sel_, in->sel.data(), in->sel.size() * 4), "upload sel"));
-  wg_column key = col_of(k);
-  key.data = d_key_;
-  key.validity = k.nulls.has_nulls() ? d_valid_ : nullptr;
+  wg_column key{};
+  const uint32_t* d_sel = nullptr;
+  TRY(dkey_.Prepare(c, join_->j_, ki, *in, &key, &d_sel));
   uint64_t got = 0;
-  TRY(wrap(wg_join_bloom_filter(join_->j_, &key, (uint64_t)in->len, in->has_sel ? d_sel_ : nullptr,
-                                in->has_sel ? in->sel.size() : 0, d_out_, &got),
+  TRY(wrap(wg_join_bloom_filter(join_->j_, &key, (uint64_t)in->len, d_sel, in->has_sel ? in->sel.size() : 0, d_out_,
+                                &got),
            "wg_join_bloom_filter"));
   if (got == 0) {
     *out = nullptr;  // every row rejected (:183-185); not counted toward the bypass

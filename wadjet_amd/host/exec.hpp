@@ -257,10 +257,32 @@ class GpuHashJoinProbe;
 // JoinType subset on the GPU path (join.go JoinType): inner, semi, anti.
 enum class JoinType { Inner, Semi, Anti };
 
+// The probe key of a batch on the device: the key column (one-column keys) or
+// the id column of its (key0, key1) pairs (two-column keys,
+// wg_join_dual_probe_key), and the batch's Sel.
+class DeviceProbeKey {
+ public:
+  ~DeviceProbeKey() { Release(); }
+  // key columns key_idx of `in` (one or two); *key / *sel name device memory
+  Status Prepare(wg_ctx* ctx, wg_join* j, const std::vector<int>& key_idx, const RecordBatch& in, wg_column* key,
+                 const uint32_t** sel);
+  void Release();
+
+ private:
+  wg_ctx* ctx_ = nullptr;
+  void* buf_[6] = {};  // key0, key1, validity0, validity1, ids, id validity
+  uint32_t* sel_ = nullptr;
+  uint64_t cap_ = 0;
+};
+
 class GpuHashJoin {
  public:
   GpuHashJoin(wg_ctx* ctx, std::string probe_key, std::string build_key, uint64_t build_hint = 0,
               JoinType type = JoinType::Inner);
+  // one or two key columns per side (NewHashJoin's LeftKeys / RightKeys, join.go:180-240;
+  // two int-class keys take the dual-key path, join.go:592-661)
+  GpuHashJoin(wg_ctx* ctx, std::vector<std::string> probe_keys, std::vector<std::string> build_keys,
+              uint64_t build_hint = 0, JoinType type = JoinType::Inner);
   ~GpuHashJoin();
   Status Build(Context& ctx, Source* build);  // join.go:771
   std::unique_ptr<GpuHashJoinProbe> Probe();   // join.go:1967
@@ -269,8 +291,9 @@ class GpuHashJoin {
 
  private:
   friend class GpuHashJoinProbe;
+  Status probe_key_idx(const RecordBatch& in, std::vector<int>* idx) const;  // resolve names; error if absent
   wg_ctx* ctx_;
-  std::string probe_key_, build_key_;
+  std::vector<std::string> probe_keys_, build_keys_;
   uint64_t hint_;
   JoinType type_;
   wg_join* j_ = nullptr;
@@ -299,17 +322,17 @@ class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, pub
  private:
   Status emit(BatchPtr* out);  // one bounded chunk of the current input
   Status semi_anti(const BatchPtr& in, BatchPtr* out);  // executeSemiAntiJoin, join.go:3245-3479
+  Status ensure_out(uint64_t rows);
   GpuHashJoin* join_;
   bool bounded_ = false, pending_ = false;
   BatchPtr cur_;
-  int key_idx_ = -1;
   wg_probe_cursor cursor_{};
-  void* d_key_ = nullptr;
-  uint64_t* d_valid_ = nullptr;
-  uint32_t* d_sel_ = nullptr;
+  DeviceProbeKey dkey_;
+  wg_column key_{};
+  const uint32_t* dsel_ = nullptr;
   uint32_t* d_out_p_ = nullptr;
   uint32_t* d_out_b_ = nullptr;
-  uint64_t cap_rows_ = 0;
+  uint64_t out_cap_ = 0;
 };
 
 // BloomFilterOp (bloom_filter_op.go:13-274): drops probe rows whose key cannot
@@ -332,9 +355,7 @@ class GpuBloomFilterOp : public UnaryOperator, public Cloneable {
   uint64_t total_checked_ = 0, total_passed_ = 0;
   int batches_seen_ = 0;
   bool disabled_ = false;
-  void* d_key_ = nullptr;
-  uint64_t* d_valid_ = nullptr;
-  uint32_t* d_sel_ = nullptr;
+  DeviceProbeKey dkey_;
   uint32_t* d_out_ = nullptr;
   uint64_t cap_rows_ = 0;
 };
