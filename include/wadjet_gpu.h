@@ -384,6 +384,57 @@ wg_status wg_join_digest(wg_join* j, const wg_column* probe_cols, const uint32_t
                          int32_t n_probe_sel, const int32_t* build_sel_cols, int32_t n_build_sel,
                          uint64_t digest_out[2]);
 
+/* ---------------------------------------------------------------- parquet */
+
+/* Parquet data pages decoded on the device (the reference's ColumnPageReader:
+ * decodeDataPageV1 / V2, decodeValues, resolveDictForRows, the RLE /
+ * bit-packing hybrid of rle.go — internal/storage/parquet/page_reader.go:
+ * 434-700, reader.go:1020-1060). The host parses page headers, decompresses,
+ * and scans the run headers of each hybrid stream (wg_pq_scan_runs); the
+ * device expands runs, bit-unpacks, resolves dictionary indices and places
+ * values and validity. */
+
+/* Run headers of one RLE / bit-packing hybrid stream of `count` values at
+ * bit width bit_width (host memory): runs[4 i..4 i+3] = {first value index,
+ * value count, RLE value or bit offset of a bit-packed run from `data`,
+ * 1 = bit-packed}. Call with runs = NULL to learn *n_runs; *used = bytes of
+ * the stream consumed; *ones (may be NULL) = values equal to 1 among the
+ * first `count` (the present rows of a definition-level stream).
+ * WG_ERR_INVALID for a stream that ends early. */
+wg_status wg_pq_scan_runs(const uint8_t* data, uint64_t len, int32_t bit_width, uint64_t count, uint32_t* runs,
+                          uint64_t cap, uint64_t* n_runs, uint64_t* used, uint64_t* ones);
+
+/* One hybrid stream on the device: its run table (from wg_pq_scan_runs) and
+ * bytes; `data` 8-byte aligned with 8 readable bytes past the stream. */
+typedef struct wg_pq_runs {
+  const uint32_t* runs;
+  uint32_t n_runs;
+  int32_t bit_width;
+  const uint8_t* data;
+} wg_pq_runs;
+
+/* One decompressed data page of a flat (non-repeated) column. */
+typedef struct wg_pq_page {
+  uint64_t num_rows;      /* rows of the page, NULLs included */
+  int32_t physical_type;  /* parquet Type: 1 INT32, 2 INT64, 5 DOUBLE */
+  int32_t encoding;       /* parquet Encoding: 0 PLAIN, 2 PLAIN_DICTIONARY, 8 RLE_DICTIONARY */
+  wg_pq_runs def;         /* definition levels (max level 1); n_runs = 0 for a required column */
+  const void* plain;      /* PLAIN: the non-NULL rows' values, packed */
+  wg_pq_runs idx;         /* dictionary encodings: the non-NULL rows' indices */
+  const void* dict;       /* the dictionary page's values (PLAIN, dict_n of them) */
+  uint64_t dict_n;
+} wg_pq_page;
+
+/* Decode one page into out[row0 .. row0 + num_rows) (4- or 8-byte values by
+ * physical type; NULL rows get 0) and, when out_valid != NULL, set the
+ * validity bits of its present rows (bit (row0 + r) of the zero-initialised
+ * bitmap words). Device pointers. WG_ERR_INVALID when a dictionary index is
+ * out of range (ValidateDictIndices, reader.go:1020-1031) or a present row has
+ * no index (a page whose index stream is empty must be all NULL).
+ * Synchronises. */
+wg_status wg_parquet_decode_page(wg_ctx* ctx, const wg_pq_page* page, void* out, uint64_t* out_valid,
+                                 uint64_t row0);
+
 /* ----------------------------------------------------------------- shuffle */
 
 /* Routing of a partitioned shuffle sink (partitionedShuffleSink.Consume,

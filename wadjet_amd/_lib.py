@@ -40,6 +40,15 @@ class ProbeCursor(C.Structure):
     _fields_ = [("pos", C.c_uint64), ("chain", C.c_int64), ("done", C.c_int32), ("reserved", C.c_int32)]
 
 
+class PqRuns(C.Structure):
+    _fields_ = [("runs", C.c_void_p), ("n_runs", C.c_uint32), ("bit_width", C.c_int32), ("data", C.c_void_p)]
+
+
+class PqPage(C.Structure):
+    _fields_ = [("num_rows", C.c_uint64), ("physical_type", C.c_int32), ("encoding", C.c_int32), ("def_", PqRuns),
+                ("plain", C.c_void_p), ("idx", PqRuns), ("dict", C.c_void_p), ("dict_n", C.c_uint64)]
+
+
 class BatchQuery(C.Structure):
     _fields_ = [("key", Column), ("val", Column), ("n_rows", C.c_uint64), ("est_groups", C.c_uint64),
                 ("row_begin", C.c_uint64), ("range_col", Column), ("range_lo", C.c_int64), ("range_hi", C.c_int64),
@@ -117,6 +126,8 @@ SIGNATURES: dict[str, list] = {
     "wg_join_bloom_filter": [P, C.POINTER(Column), U64, P, U64, P, C.POINTER(U64)],
     "wg_join_bloom_words": [P, C.POINTER(U64), P, U64],
     "wg_join_digest": [P, C.POINTER(Column), P, P, U64, C.POINTER(I32), I32, C.POINTER(I32), I32, C.POINTER(U64)],
+    "wg_pq_scan_runs": [P, U64, I32, U64, P, U64, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)],
+    "wg_parquet_decode_page": [P, P, P, P, U64],
     "wg_shuffle_partition": [P, C.POINTER(Column), I32, C.POINTER(I32), I32, U64, P, U64, I32, P, P, P],
     "wg_batch_create": [P, I32, U64, C.POINTER(P)],
     "wg_batch_destroy": [P],

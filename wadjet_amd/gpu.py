@@ -53,6 +53,9 @@ class Context:
     def sync(self):
         call("wg_ctx_sync", self.h)
 
+    def memset(self, buf: "DeviceBuffer", value: int, nbytes: Optional[int] = None):
+        call("wg_memset", self.h, buf.ptr, value, buf.nbytes if nbytes is None else nbytes)
+
     def timer_start(self):
         call("wg_ctx_timer_start", self.h)
 

@@ -135,3 +135,303 @@ def consume_parquet(agg, path: str, columns: Sequence[str], pred=None, batch_row
         agg.consume(cols, n, pred=pred, device=False)
         rows += n
     return rows
+
+
+# ------------------------------------------------------ device page decode
+#
+# The reference's own Parquet reader (internal/storage/parquet: footer.go,
+# thrift.go, page_reader.go, rle.go) decodes every page on the CPU. Here the
+# host reads a column chunk's bytes, parses the Thrift compact page headers,
+# decompresses with pyarrow's codecs and scans the run headers of each
+# RLE / bit-packing hybrid stream (wg_pq_scan_runs); the device decodes the
+# values (wg_parquet_decode_page): definition levels, PLAIN values,
+# dictionary indices resolved against the dictionary page.
+
+_CT_STOP, _CT_TRUE, _CT_FALSE, _CT_BYTE, _CT_I16, _CT_I32, _CT_I64, _CT_DOUBLE, _CT_BINARY, _CT_LIST, _CT_SET, \
+    _CT_MAP, _CT_STRUCT = range(13)
+
+PAGE_DATA, PAGE_INDEX, PAGE_DICTIONARY, PAGE_DATA_V2 = 0, 1, 2, 3
+ENC_PLAIN, ENC_PLAIN_DICTIONARY, ENC_RLE_DICTIONARY = 0, 2, 8
+PT_INT32, PT_INT64, PT_DOUBLE = 1, 2, 5
+
+
+class _Compact:
+    """Thrift compact protocol reader for page headers (thrift.go restates the
+    same protocol): field headers with deltas, zigzag varints, nested structs;
+    unknown fields are skipped."""
+
+    def __init__(self, buf: bytes, pos: int = 0):
+        self.b = buf
+        self.p = pos
+
+    def varint(self) -> int:
+        v, shift = 0, 0
+        while True:
+            c = self.b[self.p]
+            self.p += 1
+            v |= (c & 0x7F) << shift
+            shift += 7
+            if not c & 0x80:
+                return v
+
+    def zigzag(self) -> int:
+        v = self.varint()
+        return (v >> 1) ^ -(v & 1)
+
+    def value(self, t: int):
+        if t in (_CT_TRUE, _CT_FALSE):
+            return t == _CT_TRUE
+        if t == _CT_BYTE:
+            self.p += 1
+            return self.b[self.p - 1]
+        if t in (_CT_I16, _CT_I32, _CT_I64):
+            return self.zigzag()
+        if t == _CT_DOUBLE:
+            self.p += 8
+            return None
+        if t == _CT_BINARY:
+            n = self.varint()
+            self.p += n
+            return self.b[self.p - n:self.p]
+        if t in (_CT_LIST, _CT_SET):
+            h = self.b[self.p]
+            self.p += 1
+            size, et = h >> 4, h & 0x0F
+            if size == 15:
+                size = self.varint()
+            return [self._bool_elem() if et in (_CT_TRUE, _CT_FALSE) else self.value(et) for _ in range(size)]
+        if t == _CT_MAP:
+            size = self.varint()
+            if size:
+                kv = self.b[self.p]
+                self.p += 1
+                for _ in range(size):
+                    self.value(kv >> 4)
+                    self.value(kv & 0x0F)
+            return None
+        if t == _CT_STRUCT:
+            return self.struct()
+        raise ValueError(f"thrift compact type {t}")
+
+    def _bool_elem(self):
+        self.p += 1
+        return self.b[self.p - 1] == 1
+
+    def struct(self) -> dict:
+        out, fid = {}, 0
+        while True:
+            h = self.b[self.p]
+            self.p += 1
+            if h == _CT_STOP:
+                return out
+            delta, t = h >> 4, h & 0x0F
+            fid = fid + delta if delta else self.zigzag()
+            out[fid] = self.value(t)
+
+
+def _read_page_header(buf: bytes, pos: int):
+    r = _Compact(buf, pos)
+    h = r.struct()
+    return h, r.p
+
+
+def _align(n: int, a: int = 8) -> int:
+    return (n + a - 1) // a * a
+
+
+class _Arena:
+    """Host staging of one column chunk's streams, 8-byte aligned, each
+    followed by 16 readable padding bytes (the device reads whole words)."""
+
+    def __init__(self):
+        self.parts, self.size = [], 0
+
+    def add(self, data) -> int:
+        off = self.size
+        b = np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data.view(np.uint8).ravel()
+        self.parts.append((off, b))
+        self.size = _align(off + len(b) + 16)
+        return off
+
+    def pack(self) -> np.ndarray:
+        out = np.zeros(max(self.size, 8), np.uint8)
+        for off, b in self.parts:
+            out[off:off + len(b)] = b
+        return out
+
+
+def _scan_runs(buf, bit_width: int, count: int):
+    """Run table (uint32 x 4 per run), bytes used, and values equal to 1."""
+    import ctypes as C
+    arr = np.frombuffer(buf, np.uint8)
+    ptr = arr.ctypes.data_as(C.c_void_p) if len(arr) else None
+    nr, used, ones = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    L.call("wg_pq_scan_runs", ptr, len(arr), bit_width, count, None, 0, C.byref(nr), C.byref(used), C.byref(ones))
+    runs = np.zeros(max(nr.value, 1) * 4, np.uint32)
+    L.call("wg_pq_scan_runs", ptr, len(arr), bit_width, count, runs.ctypes.data_as(C.c_void_p), nr.value,
+           C.byref(nr), C.byref(used), C.byref(ones))
+    return runs, nr.value, used.value, ones.value
+
+
+def _wg_type_of(col_schema) -> Tuple[int, int, int]:
+    """(physical type id, wg type, scale) of a flat column."""
+    phys = {"INT32": PT_INT32, "INT64": PT_INT64, "DOUBLE": PT_DOUBLE}.get(col_schema.physical_type)
+    if phys is None:
+        raise L.UnsupportedError(f"parquet physical type {col_schema.physical_type} not on the GPU decode path")
+    lt = col_schema.logical_type
+    ltype = getattr(lt, "type", "NONE")
+    if ltype == "DECIMAL":
+        if phys != PT_INT64:
+            raise L.UnsupportedError("decimals stored as INT32 are not on the GPU decode path")
+        return phys, L.DECIMAL, col_schema.scale
+    if phys == PT_INT32:
+        lts = str(lt)
+        if "isSigned=false" in lts.replace(" ", "") and "bitWidth=32" in lts.replace(" ", ""):
+            return phys, L.UINT32, 0
+        return phys, L.INT32, 0
+    return phys, (L.INT64 if phys == PT_INT64 else L.FLOAT64), 0
+
+
+def decode_column_chunk(ctx, path: str, row_group: int, column: str, out=None, out_valid=None, row0: int = 0):
+    """Decode one column chunk on the device. Returns (rows, Col); with `out`
+    given (a device buffer of the column's width), pages land at row0."""
+    import ctypes as C
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+    from .gpu import Col
+    f = pq.ParquetFile(path)
+    md = f.metadata
+    ci = f.schema_arrow.get_field_index(column)
+    if ci < 0:
+        raise KeyError(column)
+    cs = f.schema.column(ci)
+    if cs.max_repetition_level != 0:
+        raise L.UnsupportedError("repeated columns are not on the GPU decode path")
+    max_def = cs.max_definition_level
+    phys, wgt, scale = _wg_type_of(cs)
+    width = 4 if phys == PT_INT32 else 8
+    cc = md.row_group(row_group).column(ci)
+    n_rows = md.row_group(row_group).num_rows
+    start = cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset else cc.data_page_offset
+    with open(path, "rb") as fh:
+        fh.seek(start)
+        raw = fh.read(cc.total_compressed_size)
+    codec = None if cc.compression == "UNCOMPRESSED" else pa.Codec(cc.compression.lower())
+
+    def inflate(b, size):
+        return bytes(b) if codec is None else codec.decompress(b, decompressed_size=size).to_pybytes()
+
+    arena = _Arena()
+    pages, dict_off, dict_n = [], None, 0
+    pos, rows_seen = 0, 0
+    while pos < len(raw) and rows_seen < n_rows:
+        h, pos = _read_page_header(raw, pos)
+        ptype, usize, csize = h[1], h[2], h[3]
+        body = raw[pos:pos + csize]
+        pos += csize
+        if ptype == PAGE_DICTIONARY:
+            dph = h[7]
+            dict_n = dph[1]
+            vals = inflate(body, usize)
+            if len(vals) < dict_n * width:
+                raise L.WgError("dictionary page shorter than its values")
+            dict_off = arena.add(vals[:dict_n * width])
+            continue
+        if ptype == PAGE_DATA:
+            dph = h[5]
+            nv, enc = dph[1], dph[2]
+            data = inflate(body, usize)
+            off = 0
+            levels = None
+            if max_def > 0:
+                ll = int.from_bytes(data[0:4], "little")
+                levels = data[4:4 + ll]
+                off = 4 + ll
+            values = data[off:]
+        elif ptype == PAGE_DATA_V2:
+            dph = h[8]
+            nv, enc = dph[1], dph[4]
+            dl, rl = dph.get(5, 0), dph.get(6, 0)
+            levels = body[rl:rl + dl] if max_def > 0 else None
+            rest = body[rl + dl:]
+            values = inflate(rest, usize - rl - dl) if dph.get(7, True) else bytes(rest)
+        else:
+            continue  # index pages carry no values
+        page = dict(rows=nv, enc=enc)
+        present = nv
+        if levels is not None:
+            runs, nr, _, present = _scan_runs(levels, 1, nv)
+            page["def"] = (arena.add(levels), arena.add(runs), nr)
+        if enc == ENC_PLAIN:
+            if len(values) < present * width:
+                raise L.WgError("PLAIN page shorter than its values")
+            page["plain"] = arena.add(values[:present * width])
+        elif enc in (ENC_PLAIN_DICTIONARY, ENC_RLE_DICTIONARY):
+            if dict_off is None:
+                raise L.WgError("dictionary-encoded page before a dictionary page")
+            bw = values[0]
+            runs, nr, _, _ = _scan_runs(values[1:], bw, present)
+            page["idx"] = (arena.add(values[1:]), arena.add(runs), nr, bw)
+        else:
+            raise L.UnsupportedError(f"parquet encoding {enc} not on the GPU decode path")
+        pages.append(page)
+        rows_seen += nv
+    host = arena.pack()
+    dev = ctx.to_device(host)
+    if out is None:
+        out = ctx.alloc(max(n_rows, 1) * width)
+        row0 = 0
+        if max_def > 0:
+            out_valid = ctx.alloc(max((n_rows + 63) // 64, 1) * 8)
+            ctx.memset(out_valid, 0)
+    base = dev.ptr.value if hasattr(dev.ptr, "value") else int(dev.ptr)
+    r0 = row0
+    if sum(pg["rows"] for pg in pages) != n_rows:
+        raise L.WgError(f"column chunk pages hold {sum(pg['rows'] for pg in pages)} rows, the row group {n_rows}")
+    for pg in pages:
+        P = L.PqPage()
+        P.num_rows = pg["rows"]
+        P.physical_type = phys
+        P.encoding = pg["enc"]
+        if "def" in pg:
+            d_off, r_off, nr = pg["def"]
+            P.def_ = L.PqRuns(base + r_off, nr, 1, base + d_off)
+        if "plain" in pg:
+            P.plain = base + pg["plain"]
+        if "idx" in pg:
+            d_off, r_off, nr, bw = pg["idx"]
+            P.idx = L.PqRuns(base + r_off, nr, bw, base + d_off)
+            P.dict = base + dict_off
+            P.dict_n = dict_n
+        L.call("wg_parquet_decode_page", ctx.h, C.byref(P), out.ptr,
+               out_valid.ptr if (out_valid is not None and max_def > 0) else None, r0)
+        r0 += pg["rows"]
+    dev.free()
+    return n_rows, Col(out, wgt, out_valid if max_def > 0 else None, scale)
+
+
+def read_parquet_gpu(ctx, path: str, columns: Sequence[str]):
+    """Every row group of the named columns decoded on the device into one
+    device column each: (rows, [Col])."""
+    import pyarrow.parquet as pq
+    from .gpu import Col
+    f = pq.ParquetFile(path)
+    md = f.metadata
+    n = md.num_rows
+    cols = []
+    for name in columns:
+        ci = f.schema_arrow.get_field_index(name)
+        cs = f.schema.column(ci)
+        phys, wgt, scale = _wg_type_of(cs)
+        width = 4 if phys == PT_INT32 else 8
+        out = ctx.alloc(max(n, 1) * width)
+        valid = None
+        if cs.max_definition_level > 0:
+            valid = ctx.alloc(max((n + 63) // 64, 1) * 8)
+            ctx.memset(valid, 0)
+        row0 = 0
+        for rg in range(md.num_row_groups):
+            got, _ = decode_column_chunk(ctx, path, rg, name, out=out, out_valid=valid, row0=row0)
+            row0 += got
+        cols.append(Col(out, wgt, valid, scale))
+    return n, cols
