@@ -435,6 +435,17 @@ typedef struct wg_pq_page {
 wg_status wg_parquet_decode_page(wg_ctx* ctx, const wg_pq_page* page, void* out, uint64_t* out_valid,
                                  uint64_t row0);
 
+/* Consecutive data pages of one column — a column chunk, or the chunks of
+ * several row groups, each page naming its own chunk's dictionary — decoded
+ * as one: page i lands at rows row0 + (rows of pages 0..i-1). The page array
+ * is host memory, everything it points to device memory; all pages of one
+ * physical type. Same results and errors as wg_parquet_decode_page on each
+ * page in turn, in three launches and one synchronisation (the reference
+ * decodes a chunk page by page, ColumnPageReader.ReadPage,
+ * page_reader.go:434-520). */
+wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t n_pages, void* out,
+                                  uint64_t* out_valid, uint64_t row0);
+
 /* ----------------------------------------------------------------- shuffle */
 
 /* Routing of a partitioned shuffle sink (partitionedShuffleSink.Consume,

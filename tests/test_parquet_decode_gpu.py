@@ -1,5 +1,5 @@
-"""Parquet pages decoded on the device (wg_parquet_decode_page, driven by
-wadjet_amd.parquet.read_parquet_gpu) against pyarrow's decode of the same
+"""Parquet pages decoded on the device (wg_parquet_decode_pages /
+wg_parquet_decode_page, driven by wadjet_amd.parquet) against pyarrow's decode of the same
 files: values and NULLs bit-exact for every page layout pyarrow writes —
 PLAIN and dictionary encodings (and the fallback from one to the other within
 a chunk), data page v1 and v2, each codec, nullable and required columns,
@@ -129,3 +129,69 @@ def test_gpu_decoded_columns_feed_the_aggregate(ctx, tmp_path):
     np.testing.assert_array_equal(ra["keys"][0][oa], rb["keys"][0][ob])
     for x, y in zip(ra["values"], rb["values"]):
         np.testing.assert_array_equal(x[oa], y[ob])
+
+
+def test_chunk_lands_at_an_unaligned_row_offset(ctx, tmp_path):
+    """A chunk decoded into a larger column at row 37: its values and
+    validity bits land there, and the rows around it are left alone."""
+    path = str(tmp_path / "off.parquet")
+    pq.write_table(table(30_000, 11, 0.3), path, row_group_size=10_000, data_page_size=8192)
+    t = pq.read_table(path, columns=["u64"]).combine_chunks()
+    arr = t.column("u64").chunk(0).slice(10_000, 10_000)
+    total, row0 = 10_100, 37
+    out = ctx.to_device(np.full(total, -1, np.int64))
+    words = (total + 63) // 64
+    sentinel = np.zeros(words, np.uint64)
+    sentinel[0] = np.uint64((1 << 37) - 1)  # rows before row0 already valid
+    valid = ctx.to_device(sentinel)
+    n, col = PQ.decode_column_chunk(ctx, path, 1, "u64", out=out, out_valid=valid, row0=row0)
+    assert n == 10_000
+    got = out.to_host(np.int64, total)
+    bits = valid.to_host(np.uint64, words)
+    v = ((bits[np.arange(total) >> 6] >> (np.arange(total) & 63).astype(np.uint64)) & np.uint64(1)) == 1
+    want_valid = ~arr.is_null().to_numpy(zero_copy_only=False)
+    np.testing.assert_array_equal(v[:row0], True)
+    np.testing.assert_array_equal(v[row0:row0 + n], want_valid)
+    np.testing.assert_array_equal(v[row0 + n:], False)
+    np.testing.assert_array_equal(got[:row0], -1)
+    np.testing.assert_array_equal(got[row0 + n:], -1)
+    want = arr.fill_null(0).to_numpy(zero_copy_only=False)
+    np.testing.assert_array_equal(got[row0:row0 + n], want)
+
+
+def test_out_of_range_dictionary_index_is_an_error(ctx, tmp_path):
+    """A page whose indices name entries past its dictionary is refused
+    (ValidateDictIndices): the staged page's dictionary is cut to one entry."""
+    path = str(tmp_path / "dict.parquet")
+    pq.write_table(pa.table({"k": pa.array(np.arange(5000) % 7, pa.int64())}), path)
+    st = PQ.StagedColumn(ctx, path, "k")
+    try:
+        assert st.pages[0].encoding in (2, 8) and st.pages[0].dict_n == 7
+        st.decode()  # intact: decodes
+        st.pages[0].dict_n = 1
+        with pytest.raises(L.WgError, match="out of range"):
+            st.decode()
+    finally:
+        st.free()
+
+
+def test_all_row_groups_decode_in_one_call(ctx, tmp_path):
+    """Many chunks, each with its own dictionary, staged together and decoded
+    by one call; chunks of 4097 rows straddle the 4096-row tiles."""
+    path = str(tmp_path / "many.parquet")
+    n = 450_000
+    rng = np.random.default_rng(5)
+    k = rng.integers(0, 3000, n)
+    t = pa.table({"k": pa.array(k, pa.int64(), mask=rng.random(n) < 0.05)})
+    pq.write_table(t, path, row_group_size=4097, data_page_size=1 << 20)
+    st = PQ.StagedColumn(ctx, path, "k")
+    try:
+        assert st.n_pages >= 100
+        col = st.decode()
+    finally:
+        st.free()
+    got = col.data.to_host(np.int64, n)
+    arr = pq.read_table(path).combine_chunks().column("k").chunk(0)
+    nul = arr.is_null().to_numpy(zero_copy_only=False)
+    np.testing.assert_array_equal(got[~nul], arr.fill_null(0).to_numpy()[~nul])
+    assert not np.any(got[nul])

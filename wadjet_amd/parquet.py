@@ -292,37 +292,17 @@ def _wg_type_of(col_schema) -> Tuple[int, int, int]:
     return phys, (L.INT64 if phys == PT_INT64 else L.FLOAT64), 0
 
 
-def decode_column_chunk(ctx, path: str, row_group: int, column: str, out=None, out_valid=None, row0: int = 0):
-    """Decode one column chunk on the device. Returns (rows, Col); with `out`
-    given (a device buffer of the column's width), pages land at row0."""
-    import ctypes as C
+def _chunk_pages(raw: bytes, compression: str, n_rows: int, max_def: int, width: int, arena: _Arena) -> list:
+    """The data pages of one column chunk's bytes, decompressed into `arena`:
+    per page its rows, encoding and the arena offsets of its streams
+    (ColumnPageReader.ReadPage, storage/parquet/page_reader.go:434-520)."""
     import pyarrow as pa
-    import pyarrow.parquet as pq
-    from .gpu import Col
-    f = pq.ParquetFile(path)
-    md = f.metadata
-    ci = f.schema_arrow.get_field_index(column)
-    if ci < 0:
-        raise KeyError(column)
-    cs = f.schema.column(ci)
-    if cs.max_repetition_level != 0:
-        raise L.UnsupportedError("repeated columns are not on the GPU decode path")
-    max_def = cs.max_definition_level
-    phys, wgt, scale = _wg_type_of(cs)
-    width = 4 if phys == PT_INT32 else 8
-    cc = md.row_group(row_group).column(ci)
-    n_rows = md.row_group(row_group).num_rows
-    start = cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset else cc.data_page_offset
-    with open(path, "rb") as fh:
-        fh.seek(start)
-        raw = fh.read(cc.total_compressed_size)
-    codec = None if cc.compression == "UNCOMPRESSED" else pa.Codec(cc.compression.lower())
+    codec = None if compression == "UNCOMPRESSED" else pa.Codec(compression.lower())
 
     def inflate(b, size):
         return bytes(b) if codec is None else codec.decompress(b, decompressed_size=size).to_pybytes()
 
-    arena = _Arena()
-    pages, dict_off, dict_n = [], None, 0
+    pages, dict_ref = [], None
     pos, rows_seen = 0, 0
     while pos < len(raw) and rows_seen < n_rows:
         h, pos = _read_page_header(raw, pos)
@@ -330,12 +310,11 @@ def decode_column_chunk(ctx, path: str, row_group: int, column: str, out=None, o
         body = raw[pos:pos + csize]
         pos += csize
         if ptype == PAGE_DICTIONARY:
-            dph = h[7]
-            dict_n = dph[1]
+            dict_n = h[7][1]
             vals = inflate(body, usize)
             if len(vals) < dict_n * width:
                 raise L.WgError("dictionary page shorter than its values")
-            dict_off = arena.add(vals[:dict_n * width])
+            dict_ref = (arena.add(vals[:dict_n * width]), dict_n)
             continue
         if ptype == PAGE_DATA:
             dph = h[5]
@@ -367,71 +346,116 @@ def decode_column_chunk(ctx, path: str, row_group: int, column: str, out=None, o
                 raise L.WgError("PLAIN page shorter than its values")
             page["plain"] = arena.add(values[:present * width])
         elif enc in (ENC_PLAIN_DICTIONARY, ENC_RLE_DICTIONARY):
-            if dict_off is None:
+            if dict_ref is None:
                 raise L.WgError("dictionary-encoded page before a dictionary page")
             bw = values[0]
             runs, nr, _, _ = _scan_runs(values[1:], bw, present)
             page["idx"] = (arena.add(values[1:]), arena.add(runs), nr, bw)
+            page["dict"] = dict_ref
         else:
             raise L.UnsupportedError(f"parquet encoding {enc} not on the GPU decode path")
         pages.append(page)
         rows_seen += nv
-    host = arena.pack()
-    dev = ctx.to_device(host)
-    if out is None:
-        out = ctx.alloc(max(n_rows, 1) * width)
-        row0 = 0
-        if max_def > 0:
-            out_valid = ctx.alloc(max((n_rows + 63) // 64, 1) * 8)
-            ctx.memset(out_valid, 0)
-    base = dev.ptr.value if hasattr(dev.ptr, "value") else int(dev.ptr)
-    r0 = row0
-    if sum(pg["rows"] for pg in pages) != n_rows:
-        raise L.WgError(f"column chunk pages hold {sum(pg['rows'] for pg in pages)} rows, the row group {n_rows}")
-    for pg in pages:
-        P = L.PqPage()
-        P.num_rows = pg["rows"]
-        P.physical_type = phys
-        P.encoding = pg["enc"]
-        if "def" in pg:
-            d_off, r_off, nr = pg["def"]
-            P.def_ = L.PqRuns(base + r_off, nr, 1, base + d_off)
-        if "plain" in pg:
-            P.plain = base + pg["plain"]
-        if "idx" in pg:
-            d_off, r_off, nr, bw = pg["idx"]
-            P.idx = L.PqRuns(base + r_off, nr, bw, base + d_off)
-            P.dict = base + dict_off
-            P.dict_n = dict_n
-        L.call("wg_parquet_decode_page", ctx.h, C.byref(P), out.ptr,
-               out_valid.ptr if (out_valid is not None and max_def > 0) else None, r0)
-        r0 += pg["rows"]
-    dev.free()
-    return n_rows, Col(out, wgt, out_valid if max_def > 0 else None, scale)
+    got = sum(pg["rows"] for pg in pages)
+    if got != n_rows:
+        raise L.WgError(f"column chunk pages hold {got} rows, the row group {n_rows}")
+    return pages
+
+
+class StagedColumn:
+    """One column's chunks (all row groups, or those named) parsed and
+    decompressed on the host and uploaded to HBM in one buffer: the input of
+    the device decode, which `decode()` runs in one wg_parquet_decode_pages
+    call. `page_bytes` = the decompressed stream bytes the decode reads."""
+
+    def __init__(self, ctx, path: str, column: str, row_groups: Optional[Sequence[int]] = None):
+        import pyarrow.parquet as pq
+        f = pq.ParquetFile(path)
+        md = f.metadata
+        ci = f.schema_arrow.get_field_index(column)
+        if ci < 0:
+            raise KeyError(column)
+        cs = f.schema.column(ci)
+        if cs.max_repetition_level != 0:
+            raise L.UnsupportedError("repeated columns are not on the GPU decode path")
+        self.ctx = ctx
+        self.max_def = cs.max_definition_level
+        self.phys, self.type, self.scale = _wg_type_of(cs)
+        self.width = 4 if self.phys == PT_INT32 else 8
+        rgs = list(range(md.num_row_groups)) if row_groups is None else list(row_groups)
+        arena = _Arena()
+        pages = []
+        self.rows = 0
+        with open(path, "rb") as fh:
+            for rg in rgs:
+                cc = md.row_group(rg).column(ci)
+                n_rows = md.row_group(rg).num_rows
+                start = (cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset
+                         else cc.data_page_offset)
+                fh.seek(start)
+                raw = fh.read(cc.total_compressed_size)
+                pages += _chunk_pages(raw, cc.compression, n_rows, self.max_def, self.width, arena)
+                self.rows += n_rows
+        self.page_bytes = sum(len(b) for _, b in arena.parts)
+        self.dev = ctx.to_device(arena.pack())
+        base = self.dev.ptr.value if hasattr(self.dev.ptr, "value") else int(self.dev.ptr)
+        self.pages = (L.PqPage * max(len(pages), 1))()
+        for P, pg in zip(self.pages, pages):
+            P.num_rows = pg["rows"]
+            P.physical_type = self.phys
+            P.encoding = pg["enc"]
+            if "def" in pg:
+                d_off, r_off, nr = pg["def"]
+                P.def_ = L.PqRuns(base + r_off, nr, 1, base + d_off)
+            if "plain" in pg:
+                P.plain = base + pg["plain"]
+            if "idx" in pg:
+                d_off, r_off, nr, bw = pg["idx"]
+                P.idx = L.PqRuns(base + r_off, nr, bw, base + d_off)
+                P.dict = base + pg["dict"][0]
+                P.dict_n = pg["dict"][1]
+        self.n_pages = len(pages)
+
+    def decode(self, out=None, out_valid=None, row0: int = 0):
+        """Decode every page on the device. With `out` given (a device buffer
+        of the column's width, and `out_valid` zeroed bitmap words for a
+        nullable column) the rows land at row0; else new buffers. -> Col."""
+        from .gpu import Col
+        if out is None:
+            out = self.ctx.alloc(max(self.rows, 1) * self.width)
+            row0 = 0
+            if self.max_def > 0:
+                out_valid = self.ctx.alloc(max((self.rows + 63) // 64, 1) * 8)
+                self.ctx.memset(out_valid, 0)
+        L.call("wg_parquet_decode_pages", self.ctx.h, self.pages, self.n_pages, out.ptr,
+               out_valid.ptr if (out_valid is not None and self.max_def > 0) else None, row0)
+        return Col(out, self.type, out_valid if self.max_def > 0 else None, self.scale)
+
+    def free(self):
+        if self.dev is not None:
+            self.dev.free()
+            self.dev = None
+
+
+def decode_column_chunk(ctx, path: str, row_group: int, column: str, out=None, out_valid=None, row0: int = 0):
+    """Decode one column chunk on the device. Returns (rows, Col); with `out`
+    given (a device buffer of the column's width), pages land at row0."""
+    st = StagedColumn(ctx, path, column, [row_group])
+    try:
+        return st.rows, st.decode(out, out_valid, row0)
+    finally:
+        st.free()
 
 
 def read_parquet_gpu(ctx, path: str, columns: Sequence[str]):
     """Every row group of the named columns decoded on the device into one
     device column each: (rows, [Col])."""
-    import pyarrow.parquet as pq
-    from .gpu import Col
-    f = pq.ParquetFile(path)
-    md = f.metadata
-    n = md.num_rows
-    cols = []
+    n, cols = 0, []
     for name in columns:
-        ci = f.schema_arrow.get_field_index(name)
-        cs = f.schema.column(ci)
-        phys, wgt, scale = _wg_type_of(cs)
-        width = 4 if phys == PT_INT32 else 8
-        out = ctx.alloc(max(n, 1) * width)
-        valid = None
-        if cs.max_definition_level > 0:
-            valid = ctx.alloc(max((n + 63) // 64, 1) * 8)
-            ctx.memset(valid, 0)
-        row0 = 0
-        for rg in range(md.num_row_groups):
-            got, _ = decode_column_chunk(ctx, path, rg, name, out=out, out_valid=valid, row0=row0)
-            row0 += got
-        cols.append(Col(out, wgt, valid, scale))
+        st = StagedColumn(ctx, path, name)
+        try:
+            cols.append(st.decode())
+            n = st.rows
+        finally:
+            st.free()
     return n, cols
