@@ -245,6 +245,76 @@ def bench_wide(ctx, args):
     return dict(rows=n, groups=g, ms_per_step=ms, rows_per_s=n / (ms / 1000))
 
 
+# -------------------------------------------- Parquet pages decoded on the device
+
+def bench_parquet(ctx, args):
+    """A 100M-row Parquet file (pyarrow's default writer, snappy, 4M-row row
+    groups) of k int64 uniform over 1M (dictionary, falling back to PLAIN as
+    the dictionary outgrows its page), g int32 uniform over 16 (dictionary,
+    4-bit indices), v float64 with 10 % NULLs (PLAIN, definition levels):
+    every page of every chunk decompressed on the host and staged in HBM once,
+    then the timed step decodes the three columns on the device
+    (wg_parquet_decode_pages, one call per column). Algorithmic bytes: the
+    decompressed page streams read once, the values and validity written."""
+    import tempfile
+
+    import numpy as np
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    from wadjet_amd import parquet as PQ
+    n = args.rows
+    rng = np.random.default_rng(7)
+    t = pa.table({"k": pa.array(rng.integers(0, 1_000_000, n), pa.int64()),
+                  "g": pa.array(rng.integers(0, 16, n).astype(np.int32), pa.int32()),
+                  "v": pa.array(rng.normal(size=n), pa.float64(), mask=rng.random(n) < 0.1)})
+    d = tempfile.mkdtemp(prefix="wg_pq_")
+    path = os.path.join(d, "bench.parquet")
+    log(f"[parquet] writing {n} rows")
+    pq.write_table(t, path, row_group_size=4_000_000)
+    del t
+    t0 = time.perf_counter()
+    staged = [PQ.StagedColumn(ctx, path, c) for c in ("k", "g", "v")]
+    stage_s = time.perf_counter() - t0
+    outs = [ctx.alloc(n * st.width) for st in staged]
+    valids = [ctx.alloc(((n + 63) // 64) * 8) if st.max_def > 0 else None for st in staged]
+
+    def step():
+        for st, o, vb in zip(staged, outs, valids):
+            if vb is not None:
+                ctx.memset(vb, 0)
+            st.decode(o, vb, 0)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    ctx.sync()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1000 / steps
+    page_bytes = sum(st.page_bytes for st in staged)
+    out_bytes = sum(n * st.width + (((n + 63) // 64) * 8 if st.max_def > 0 else 0) for st in staged)
+    alg = page_bytes + out_bytes
+    t0 = time.perf_counter()
+    pq.read_table(path, columns=["k", "g", "v"])
+    arrow_ms = (time.perf_counter() - t0) * 1000
+    pages = sum(st.n_pages for st in staged)
+    for st in staged:
+        st.free()
+    for b in outs + [v for v in valids if v is not None]:
+        b.free()
+    os.remove(path)
+    os.rmdir(d)
+    gbs = alg / (ms / 1000) / 1e9
+    log(f"[parquet] {n} rows x 3 columns, {pages} pages: {ms:.3f} ms/step ({n / ms / 1e6:.1f} G rows/s, "
+        f"{gbs:.0f} GB/s); host staging {stage_s:.2f} s; pyarrow read_table {arrow_ms:.0f} ms")
+    return dict(rows=n, columns=3, pages=pages, ms_per_step=ms, rows_per_s=n / (ms / 1000),
+                page_bytes=page_bytes, out_bytes=out_bytes, achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS,
+                host_stage_s=stage_s, pyarrow_read_ms=arrow_ms)
+
+
 # -------------------------------------------- 256 concurrent queries, one launch
 
 def bench_batched(ctx, args):
@@ -349,7 +419,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--qps", type=float, nargs="*", default=[100, 1000, 5000])
     ap.add_argument("--shuffle-parts", type=int, default=256)
-    ap.add_argument("--only", choices=["agg", "join", "batched", "served", "shuffle", "wide"], default=None)
+    ap.add_argument("--only", choices=["agg", "join", "batched", "served", "shuffle", "wide", "parquet"], default=None)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -379,6 +449,8 @@ def main():
         extra["shuffle"] = bench_shuffle(ctx, args, A["cols"] if A is not None else None)
     if want("wide"):
         extra["wide_key"] = bench_wide(ctx, args)
+    if want("parquet"):
+        extra["parquet_decode"] = bench_parquet(ctx, args)
     if want("batched"):
         extra["batched"] = bench_batched(ctx, args)
     if want("served"):

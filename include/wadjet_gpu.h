@@ -405,7 +405,8 @@ wg_status wg_pq_scan_runs(const uint8_t* data, uint64_t len, int32_t bit_width, 
                           uint64_t cap, uint64_t* n_runs, uint64_t* used, uint64_t* ones);
 
 /* One hybrid stream on the device: its run table (from wg_pq_scan_runs) and
- * bytes; `data` 8-byte aligned with 8 readable bytes past the stream. */
+ * bytes; `data` 8-byte aligned with 16 readable bytes past the stream (the
+ * decode reads whole 64-bit words). */
 typedef struct wg_pq_runs {
   const uint32_t* runs;
   uint32_t n_runs;
@@ -418,7 +419,8 @@ typedef struct wg_pq_page {
   uint64_t num_rows;      /* rows of the page, NULLs included */
   int32_t physical_type;  /* parquet Type: 1 INT32, 2 INT64, 5 DOUBLE */
   int32_t encoding;       /* parquet Encoding: 0 PLAIN, 2 PLAIN_DICTIONARY, 8 RLE_DICTIONARY */
-  wg_pq_runs def;         /* definition levels (max level 1); n_runs = 0 for a required column */
+  wg_pq_runs def;         /* definition levels (max level 1); n_runs = 0 for a required column or a page
+                             without NULLs (every row then has a value) */
   const void* plain;      /* PLAIN: the non-NULL rows' values, packed */
   wg_pq_runs idx;         /* dictionary encodings: the non-NULL rows' indices */
   const void* dict;       /* the dictionary page's values (PLAIN, dict_n of them) */

@@ -340,7 +340,8 @@ def _chunk_pages(raw: bytes, compression: str, n_rows: int, max_def: int, width:
         present = nv
         if levels is not None:
             runs, nr, _, present = _scan_runs(levels, 1, nv)
-            page["def"] = (arena.add(levels), arena.add(runs), nr)
+            if present < nv:  # a page without NULLs decodes as a required one
+                page["def"] = (arena.add(levels), arena.add(runs), nr)
         if enc == ENC_PLAIN:
             if len(values) < present * width:
                 raise L.WgError("PLAIN page shorter than its values")
