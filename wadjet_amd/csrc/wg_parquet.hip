@@ -115,27 +115,6 @@ __device__ __forceinline__ uint32_t run_search_wave(const PqRun* runs, uint32_t 
   return lo;
 }
 
-// A stream's current run, kept in registers while a wavefront walks its
-// segment; it only reloads when the rows cross into the next run.
-struct RunCursor {
-  uint32_t r, start, end, payload, packed;
-};
-
-__device__ __forceinline__ void cur_load(const wg_pq_runs& S, RunCursor& c, uint32_t r) {
-  const PqRun R = ((const PqRun*)S.runs)[r];
-  c.r = r;
-  c.start = R.start;
-  c.end = R.start + R.count;
-  c.payload = R.payload;
-  c.packed = R.packed;
-}
-
-// Forward to the run holding value i (the stream's last run when i lies past
-// its end).
-__device__ __forceinline__ void cur_seek(const wg_pq_runs& S, RunCursor& c, uint64_t i) {
-  while (i >= c.end && c.r + 1 < S.n_runs) cur_load(S, c, c.r + 1);
-}
-
 // bw bits of a bit-packed stream at bit offset `bit`.
 __device__ __forceinline__ uint64_t packed_bits(const uint8_t* data, uint64_t bit) {
   const uint64_t* w = (const uint64_t*)data;
@@ -174,20 +153,28 @@ __device__ __forceinline__ uint64_t chunk_present(const wg_pq_runs& D, uint32_t 
   return m;
 }
 
-// Index k of the index stream for a lane whose run cursor `c` (shared by the
-// wavefront) is at or before k's run.
-__device__ __forceinline__ bool index_at(const wg_pq_runs& S, const RunCursor& c, uint64_t k, uint32_t* d) {
-  if (k >= c.start && k < c.end) {
-    if (!c.packed) {
-      *d = c.payload;
-    } else {
-      const uint64_t v = packed_bits(S.data, (uint64_t)c.payload + (k - c.start) * (uint64_t)S.bit_width);
-      *d = (uint32_t)(v & ((1ull << S.bit_width) - 1));
-    }
-    return true;
+// Up to 64 consecutive runs of a stream held by a wavefront, lane l holding
+// run first + l, so the run of an index is found by a ballot instead of a
+// load; reloaded when the indices move past it.
+struct RunWindow {
+  uint32_t first, n;                     // first run held, runs held
+  uint32_t start, end, payload, packed;  // this lane's run
+  uint64_t limit;                        // first index past the window
+};
+
+__device__ __forceinline__ void win_load(const wg_pq_runs& S, RunWindow& w, uint32_t first, int lane) {
+  w.first = first;
+  w.n = min(64u, S.n_runs - first);
+  w.start = w.end = 0xFFFFFFFFu;
+  w.payload = w.packed = 0;
+  if ((uint32_t)lane < w.n) {
+    const PqRun R = ((const PqRun*)S.runs)[first + lane];
+    w.start = R.start;
+    w.end = R.start + R.count;
+    w.payload = R.payload;
+    w.packed = R.packed;
   }
-  const PqRun* runs = (const PqRun*)S.runs;
-  return run_value(S, runs[run_advance(runs, S.n_runs, c.r, k)], k, d);
+  w.limit = (uint32_t)__builtin_amdgcn_readlane((int)w.end, (int)w.n - 1);
 }
 
 constexpr uint32_t kPqChunks = kPqSeg / 64;  // 64-row chunks per segment
@@ -276,10 +263,9 @@ __global__ void pq_values_kernel(const PqPageDev* pages, const uint32_t* tile_pa
       my_before = my_j;
     }
   }
-  RunCursor ci{};
-  if (has_idx) {
-    cur_load(P.idx, ci, run_search_wave((const PqRun*)P.idx.runs, P.idx.n_runs, shfl64(my_before, 0), lane));
-  }
+  RunWindow iw{};
+  if (has_idx)
+    win_load(P.idx, iw, run_search_wave((const PqRun*)P.idx.runs, P.idx.n_runs, shfl64(my_before, 0), lane), lane);
   const uint64_t lt = (1ull << lane) - 1;  // lanes below this one
   T* dst = out + row0 + g.row_start;
   uint32_t nbad = 0;
@@ -297,13 +283,34 @@ __global__ void pq_values_kernel(const PqPageDev* pages, const uint32_t* tile_pa
       v[u] = 0;
       const bool has = (m[u] >> lane) & 1;
       const uint64_t k = kb[u] + (uint64_t)__popcll(m[u] & lt);
-      if (has_idx && m[u]) cur_seek(P.idx, ci, kb[u]);
-      if (has) {
-        if (!dict) {
-          v[u] = ((const T*)P.plain)[k];
-        } else {
+      if (!dict) {
+        if (has) v[u] = ((const T*)P.plain)[k];
+      } else if (m[u]) {
+        // the run of the chunk's first index, from the window
+        if (has_idx && kb[u] >= iw.limit && iw.first + iw.n < P.idx.n_runs)
+          win_load(P.idx, iw, run_search_wave((const PqRun*)P.idx.runs, P.idx.n_runs, kb[u], lane), lane);
+        const uint64_t held = __ballot((uint32_t)lane < iw.n && iw.start <= kb[u]);
+        const int h = held ? 63 - __clzll(held) : 0;
+        const uint32_t hs = (uint32_t)__builtin_amdgcn_readlane((int)iw.start, h);
+        const uint32_t he = (uint32_t)__builtin_amdgcn_readlane((int)iw.end, h);
+        const uint32_t hp = (uint32_t)__builtin_amdgcn_readlane((int)iw.payload, h);
+        const uint32_t hk = (uint32_t)__builtin_amdgcn_readlane((int)iw.packed, h);
+        if (has) {
           uint32_t d = 0;
-          if (has_idx && index_at(P.idx, ci, k, &d) && d < P.dict_n) v[u] = ((const T*)P.dict)[d];
+          bool got = false;
+          if (!has_idx) {
+          } else if (held && k >= hs && k < he) {
+            got = true;
+            if (!hk) d = hp;
+            else d = (uint32_t)(packed_bits(P.idx.data, (uint64_t)hp + (k - hs) * (uint64_t)P.idx.bit_width) &
+                                ((1ull << P.idx.bit_width) - 1));
+          } else {
+            // past the run of the chunk's first index: walk the runs from it
+            const PqRun* runs = (const PqRun*)P.idx.runs;
+            const uint32_t r0 = held ? iw.first + (uint32_t)h : 0u;
+            got = run_value(P.idx, runs[run_advance(runs, P.idx.n_runs, r0, k)], k, &d);
+          }
+          if (got && d < P.dict_n) v[u] = ((const T*)P.dict)[d];
           else ++nbad;  // no index stream, an index past it, or one past the dictionary
         }
       }
