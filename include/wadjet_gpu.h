@@ -302,6 +302,10 @@ wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_row
 /* End of build (join.go:1126-1152). */
 wg_status wg_join_build_finish(wg_join* j);
 wg_status wg_join_build_rows(wg_join* j, uint64_t* rows);
+/* The retained build column build_col as a device column (data and validity
+ * owned by the join, valid until wg_join_destroy or the next build append):
+ * the base of the build side's view columns (wg_view_flatten). */
+wg_status wg_join_build_column(wg_join* j, int32_t build_col, wg_column* out);
 /* Probe key of a two-column-key join (after build_finish): for every active
  * row (sel, or 0..n_rows) out_ids[row] = the id of its (key0, key1) pair, an id
  * no build row has when the build lacks the pair; out_valid (bitmap words,
@@ -360,6 +364,20 @@ wg_status wg_join_gather_build(wg_join* j, int32_t build_col, const uint32_t* bu
 /* Gather a device column at row indices (gatherVector, sort.go:611). */
 wg_status wg_gather(wg_ctx* ctx, const wg_column* col, const uint32_t* rows, uint64_t n,
                     void* out);
+/* Late materialization: the probe's (probe_row, build_row) pair arrays are the
+ * indices of view columns over the probe batch and the retained build columns
+ * (emitViewOutput, join.go:2517-2600; NewViewVector, batch/view.go:27-80),
+ * kept on the device until a consumer needs values. This flattens one view
+ * (Vector.Flatten, view.go:87-230): out[i] = base[indices[i]] and bit i of
+ * out_valid ((n + 63) / 64 words, all written; may be NULL) set, unless row i
+ * is NULL in the view's own bitmap own_valid (NULL = no own NULLs; the base
+ * is then not read) or the base row is NULL — out[i] = 0 and the bit clear.
+ * A view over a view composes (view.go:50-70) through this same call with
+ * base = the inner view's indices as a WG_UINT32 column whose validity is the
+ * inner view's own bitmap. Device pointers; WG_ERR_INVALID when an index is
+ * >= base_rows. Synchronises. */
+wg_status wg_view_flatten(wg_ctx* ctx, const wg_column* base, uint64_t base_rows, const uint32_t* indices,
+                          const uint64_t* own_valid, uint64_t n, void* out, uint64_t* out_valid);
 
 /* Fused probe -> aggregate: join the device-resident probe batch against the
  * build, and feed every matched (probe row, build row) pair to `agg` as one

@@ -27,6 +27,11 @@ Pinning (how each function is checked, see tests/test_oracle.py):
   fnv1a64 / shuffle_hash / shuffle_partition         published FNV-1a 64 vectors, a byte-at-a-time FNV-1a
                                                      over each row's key bytes, the reference's routing
                                                      test (partitioned_shuffle_sink_test.go:19-78)
+  view_flatten / view_compose                        the reference's view KATs (batch/view_test.go:53-93,
+                                                     :155-218): permutation + duplication over NULL base
+                                                     rows, own-NULL override, view-over-view composition,
+                                                     NaN, empty and single; flatten(compose) equals
+                                                     flatten(flatten) on random views
 """
 from __future__ import annotations
 
@@ -650,3 +655,47 @@ def semi_join2(build_keys, probe_keys, build_valid=(None, None), probe_valid=(No
     hit = np.array([bool(pok[r]) and (int(pa[r]), int(pb[r])) in keys for r in rows], bool)
     keep = ~hit if anti else hit
     return rows[keep].astype(np.uint32)
+
+
+# ------------------------------------- late-materialized (view) columns
+
+def bits_of(valid: np.ndarray) -> np.ndarray:
+    """Validity words (bit i % 64 of word i / 64, 1 = valid) of a bool array."""
+    valid = np.asarray(valid, bool)
+    b = np.packbits(valid, bitorder="little")
+    b = np.concatenate([b, np.zeros((-len(b)) % 8, np.uint8)])
+    return b.view(np.uint64).copy() if len(b) else np.zeros(0, np.uint64)
+
+
+def view_flatten(base_vals, base_valid, indices, own_valid=None):
+    """Vector.Flatten of a view (internal/engine/batch/view.go:87-230): row i
+    is NULL when it is NULL in the view's own bitmap (the base is then not
+    read, view.go:106-112) or the base row indices[i] is NULL; otherwise it is
+    base[indices[i]]. NULL rows hold 0 (newOwnedLike storage). Returns
+    (values, valid bool array)."""
+    vals = np.asarray(base_vals)
+    idx = np.asarray(indices, np.int64)
+    n = len(idx)
+    own = _valid_at(own_valid, n)
+    bok = _valid_at(base_valid, len(vals))
+    valid = np.zeros(n, bool)
+    out = np.zeros(n, vals.dtype) if vals.ndim == 1 else np.zeros((n,) + vals.shape[1:], vals.dtype)
+    for i in range(n):
+        if not own[i]:
+            continue
+        s = int(idx[i])
+        if bok[s]:
+            valid[i] = True
+            out[i] = vals[s]
+    return out, valid
+
+
+def view_compose(inner_indices, inner_own_valid, outer_indices):
+    """A view over a view (NewViewVectorReuse, view.go:43-80): the composed
+    indices are inner[outer[i]], and row i is own-NULL when the inner view's
+    row outer[i] is own-NULL; the base stays the inner view's base. Returns
+    (indices uint32, own valid bool array)."""
+    inner = np.asarray(inner_indices, np.uint32)
+    outer = np.asarray(outer_indices, np.int64)
+    own = _valid_at(inner_own_valid, len(inner))
+    return inner[outer].astype(np.uint32), own[outer]

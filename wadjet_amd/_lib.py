@@ -129,6 +129,8 @@ SIGNATURES: dict[str, list] = {
     "wg_pq_scan_runs": [P, U64, I32, U64, P, U64, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)],
     "wg_parquet_decode_page": [P, P, P, P, U64],
     "wg_parquet_decode_pages": [P, P, U64, P, P, U64],
+    "wg_view_flatten": [P, P, U64, P, P, U64, P, P],
+    "wg_join_build_column": [P, I32, P],
     "wg_shuffle_partition": [P, C.POINTER(Column), I32, C.POINTER(I32), I32, U64, P, U64, I32, P, P, P],
     "wg_batch_create": [P, I32, U64, C.POINTER(P)],
     "wg_batch_destroy": [P],

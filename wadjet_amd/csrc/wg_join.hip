@@ -885,6 +885,15 @@ wg_status wg_join_build_rows(wg_join* j, uint64_t* rows) {
   return WG_OK;
 }
 
+wg_status wg_join_build_column(wg_join* j, int32_t build_col, wg_column* out) {
+  clear_error();
+  WG_CHECK(j && out, WG_ERR_INVALID, "null argument");
+  WG_CHECK(build_col >= 0 && build_col < j->n_cols, WG_ERR_INVALID, "build column %d", build_col);
+  const ColDesc c = retained_col(j, build_col);
+  *out = wg_column{c.data, c.valid, c.type, c.scale};
+  return WG_OK;
+}
+
 wg_status wg_join_semi(wg_join* j, const wg_column* probe_key, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
                        int32_t anti, uint32_t* out_sel, uint64_t* n_out) {
   clear_error();

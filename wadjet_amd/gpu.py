@@ -120,6 +120,25 @@ class DeviceBuffer:
             pass
 
 
+class BorrowedBuffer:
+    """Device memory the library owns (a join's retained build column):
+    readable through the same calls as a DeviceBuffer, never freed here."""
+
+    def __init__(self, ctx: Context, ptr, nbytes: int):
+        self.ctx = ctx
+        self.ptr = C.c_void_p(ptr)
+        self.nbytes = nbytes
+
+    to_host = DeviceBuffer.to_host
+
+    def free(self):
+        pass
+
+
+TYPE_BYTES = {L.INT32: 4, L.INT64: 8, L.FLOAT64: 8, L.DECIMAL: 8, L.DECIMAL128: 16, L.IPV4: 8, L.UINT32: 4,
+              L.UINT16: 2}
+
+
 @dataclass
 class Col:
     """A column handed to the library: device buffer or host numpy array."""
@@ -132,7 +151,7 @@ class Col:
         def ptr(x):
             if x is None:
                 return None
-            if isinstance(x, DeviceBuffer):
+            if isinstance(x, (DeviceBuffer, BorrowedBuffer)):
                 return x.ptr
             return x.ctypes.data_as(C.c_void_p)
         return L.Column(ptr(self.data), ptr(self.validity), self.type, self.scale)
@@ -381,6 +400,16 @@ class HashJoin:
         call("wg_join_build_rows", self.h, C.byref(n))
         return n.value
 
+    def build_column(self, col: int) -> Col:
+        """The retained build column as a device column (the base of the
+        build side's view columns); owned by the join."""
+        c = L.Column()
+        call("wg_join_build_column", self.h, col, C.byref(c))
+        n = self.rows()
+        data = BorrowedBuffer(self.ctx, c.data, n * TYPE_BYTES[c.type])
+        valid = BorrowedBuffer(self.ctx, c.validity, ((n + 63) // 64) * 8) if c.validity else None
+        return Col(data, c.type, valid, c.scale)
+
     def probe(self, key: Col, n_rows: int, out_p: DeviceBuffer, out_b: DeviceBuffer, out_cap: int,
               cursor: Optional[L.ProbeCursor] = None, sel=None):
         """One bounded probe call; returns (pairs written, cursor)."""
@@ -442,6 +471,20 @@ class HashJoin:
 def gather(ctx: Context, col: Col, rows: DeviceBuffer, n: int, out: DeviceBuffer):
     cc = col.c()
     call("wg_gather", ctx.h, C.byref(cc), rows.ptr, n, out.ptr)
+
+
+def view_flatten(ctx: Context, base: Col, base_rows: int, indices: DeviceBuffer, n: int,
+                 own_valid: Optional[DeviceBuffer] = None, out: Optional[DeviceBuffer] = None,
+                 out_valid: Optional[DeviceBuffer] = None) -> Col:
+    """A view column (base, indices, own NULL bitmap) materialized on the
+    device (wg_view_flatten): Col of n rows with its validity words."""
+    w = TYPE_BYTES[base.type]
+    out = out if out is not None else ctx.alloc(max(n, 1) * w)
+    out_valid = out_valid if out_valid is not None else ctx.alloc(max((n + 63) // 64, 1) * 8)
+    cc = base.c()
+    call("wg_view_flatten", ctx.h, C.byref(cc), base_rows, indices.ptr, own_valid.ptr if own_valid is not None else None,
+         n, out.ptr, out_valid.ptr)
+    return Col(out, base.type, out_valid, base.scale)
 
 
 def shuffle_partition(ctx: Context, cols: Sequence[Col], key_cols: Sequence[int], num_parts: int, n_rows: int,
