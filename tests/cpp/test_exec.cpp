@@ -2,6 +2,8 @@
 // like the reference's own exec tests. `cpu` runs the host-only pieces
 // (KernelFilter, Pipeline, CollectSink); `gpu` runs GpuHashAggregate and
 // GpuHashJoin on device 0.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -553,9 +555,242 @@ static void test_shuffle_router(wg_ctx* g) {
   }
 }
 
+// ------------------------------------------------- late materialization
+
+static std::string cell(const Vector& v, int i) {
+  if (!v.nulls.valid(i)) return "null";
+  char buf[64];
+  switch (v.type) {
+    case TypeID::Int32: case TypeID::Port: case TypeID::Date: std::snprintf(buf, sizeof buf, "%d", v.i32[i]); break;
+    case TypeID::Float64: std::snprintf(buf, sizeof buf, "%.17g", v.f64[i]); break;
+    case TypeID::Decimal: std::snprintf(buf, sizeof buf, "%lld:%llu", (long long)v.dec[i].hi, (unsigned long long)v.dec[i].lo); break;
+    default: std::snprintf(buf, sizeof buf, "%lld", (long long)v.i64[i]); break;
+  }
+  return buf;
+}
+
+// the rows of flattened batches as text, sorted (pair order is unspecified)
+static std::vector<std::string> rows_of(const std::vector<BatchPtr>& bs) {
+  std::vector<std::string> out;
+  for (auto& b : bs)
+    for (int i = 0; i < b->len; ++i) {
+      std::string r;
+      for (size_t c = 0; c < b->columns.size(); ++c) r += b->schema[c].name + "=" + cell(*b->columns[c], i) + " ";
+      out.push_back(r);
+    }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+static std::vector<int64_t> i64_values(Vector& v) {
+  std::vector<int64_t> out;
+  for (int i = 0; i < v.len; ++i) out.push_back(v.nulls.valid(i) ? v.i64[i] : INT64_MIN);
+  return out;
+}
+
+static void test_view_vectors() {
+  const int64_t N = INT64_MIN;  // NULL marker in the expectations
+  // TestViewFlattenAllFlatTypes, int64 case (view_test.go:53-93)
+  auto base = Vector::make(TypeID::Int64, 4);
+  base->i64 = {1, 0, -9000000000000000ll, 42};
+  base->nulls.set_null(1, 4);
+  auto v = NewViewVector(base, {3, 0, 0, 2, 1, 3});
+  EXPECT(v->is_view() && v->len == 6, "a view of 6 rows");
+  v->Flatten();
+  EXPECT(!v->is_view() && i64_values(*v) == (std::vector<int64_t>{42, 1, 1, -9000000000000000ll, N, 42}),
+         "flatten through a permuting, duplicating view");
+  // TestViewOwnNullOverride (:155-169)
+  auto ints = Vector::make(TypeID::Int64, 2);
+  ints->i64 = {10, 20};
+  auto iv = NewViewVector(ints, {1, 0});
+  iv->nulls.set_null(0, 2);
+  iv->Flatten();
+  EXPECT(i64_values(*iv) == (std::vector<int64_t>{N, 10}), "own NULL overrides the index");
+  // TestViewComposition (:171-183)
+  auto b4 = Vector::make(TypeID::Int64, 4);
+  b4->i64 = {0, 10, 20, 30};
+  auto v1 = NewViewVector(b4, {3, 2, 1, 0});
+  v1->nulls.set_null(2, 4);
+  auto v2 = NewViewVector(v1, {2, 0, 1, 2});
+  EXPECT(v2->base == b4, "composition collapses to the owned base");
+  v2->Flatten();
+  EXPECT(i64_values(*v2) == (std::vector<int64_t>{N, 30, 20, N}), "composed view values");
+  // TestViewFlattenCopiesStorage (:199-211) and TestViewEmptyAndSingle (:213-219)
+  auto f = Vector::make(TypeID::Float64, 2);
+  f->f64 = {1.5, 2.5};
+  auto fv = NewViewVector(f, {1, 0});
+  fv->Flatten();
+  f->f64[0] = -1;
+  EXPECT(fv->f64[0] == 2.5 && fv->f64[1] == 1.5, "flatten copies, it does not alias");
+  auto one = Vector::make(TypeID::Int64, 1);
+  one->i64 = {5};
+  auto e = NewViewVector(one, {});
+  e->Flatten();
+  EXPECT(e->len == 0, "empty view");
+  auto s1 = NewViewVector(one, {0});
+  s1->Flatten();
+  EXPECT(i64_values(*s1) == std::vector<int64_t>{5}, "single-row view");
+  // TestBatchHasViewsFlattenViews (:280-319)
+  auto bt = make_batch({{"a", TypeID::Int64}, {"b", TypeID::Int64}}, 2);
+  EXPECT(!bt->HasViews(), "owned batch has no views");
+  bt->columns[1] = NewViewVector(b4, {0, 3});
+  EXPECT(bt->HasViews(), "batch with a view column");
+  bt->FlattenViews();
+  EXPECT(!bt->HasViews() && bt->columns[1]->i64[1] == 30, "FlattenViews");
+}
+
+// An operator that re-emits its input as views (a stand-in for a late-
+// materializing producer), for the pipeline's defensive flatten.
+struct ViewEmitter : UnaryOperator {
+  Status Init(Context&) override { return Status{}; }
+  Status Close() override { return Status{}; }
+  Status Execute(Context&, const BatchPtr& in, BatchPtr* out) override {
+    auto o = std::make_shared<RecordBatch>();
+    o->schema = in->schema;
+    o->len = in->len;
+    std::vector<uint32_t> idx(in->len);
+    for (int i = 0; i < in->len; ++i) idx[i] = (uint32_t)(in->len - 1 - i);
+    for (auto& c : in->columns) o->columns.push_back(NewViewVector(c, idx));
+    *out = o;
+    return Status{};
+  }
+};
+
+static void test_pipeline_flattens_views() {
+  Context ctx;
+  auto b = make_batch({{"k", TypeID::Int64}}, 3);
+  b->columns[0]->i64 = {1, 2, 3};
+  BatchSource src({b});
+  ViewEmitter op;
+  CollectSink sink;
+  Pipeline p{&src, {&op}, &sink, 0};
+  const int64_t before = LateMat.flattens.load();
+  OK(p.Run(ctx));
+  EXPECT(LateMat.flattens.load() > before, "no flatten for a sink that is not view-aware");
+  for (auto& ob : sink.batches) EXPECT(!ob->HasViews(), "the sink kept a view");
+  EXPECT(sink.rows() == 3 && sink.batches[0]->columns[0]->i64[0] == 3, "reversed rows through the view");
+}
+
+// late_mat_test.go: orders (o_orderkey, o_comment) and lineitem (l_orderkey,
+// l_price, l_ship); the reference's string columns are int64 codes here.
+static BatchPtr late_build() {
+  auto b = make_batch({{"o_orderkey", TypeID::Int64}, {"o_comment", TypeID::Int64}}, 3);
+  b->columns[0]->i64 = {1, 2, 3};
+  b->columns[1]->i64 = {101, 102, 100};
+  return b;
+}
+static BatchPtr late_probe() {
+  auto b = make_batch({{"l_orderkey", TypeID::Int64}, {"l_price", TypeID::Float64}, {"l_ship", TypeID::Int64}}, 4);
+  b->columns[0]->i64 = {2, 9, 1, 2};  // 9: no match; the second 2: 1:N with row 0
+  b->columns[1]->f64 = {20.5, 90.0, 10.0, 21.0};
+  b->columns[2]->i64 = {7, 8, 0, 9};
+  return b;
+}
+
+static std::vector<std::string> run_late_join(wg_ctx* g, bool late, JoinType t = JoinType::Inner,
+                                              bool* had_views = nullptr) {
+  Context ctx;
+  GpuHashJoin hj(g, "l_orderkey", "o_orderkey", 0, t);
+  BatchSource bs({late_build()});
+  OK(hj.Build(ctx, &bs));
+  auto pr = hj.Probe();
+  pr->LateMaterialize = late;
+  BatchPtr out;
+  OK(pr->Execute(ctx, late_probe(), &out));
+  if (!out) return {};
+  if (had_views) *had_views = out->HasViews();
+  out->FlattenViews();
+  EXPECT(!out->HasViews(), "FlattenViews left views");
+  return rows_of({out});
+}
+
+static void test_late_materialized_join(wg_ctx* g) {
+  // TestLateMatInnerJoinParity (late_mat_test.go:76-92)
+  const int64_t before = LateMat.batches_emitted.load();
+  bool views = true;
+  auto eager = run_late_join(g, false, JoinType::Inner, &views);
+  EXPECT(!views, "eager output contains views");
+  EXPECT(LateMat.batches_emitted.load() == before, "dormancy: the counter moved with the flag off");
+  views = false;
+  auto lazy = run_late_join(g, true, JoinType::Inner, &views);
+  EXPECT(views, "late-materialized output has no view columns");
+  EXPECT(LateMat.batches_emitted.load() > before, "engagement: the counter did not move with the flag on");
+  EXPECT(eager == lazy && lazy.size() == 3, "inner join parity (%zu vs %zu rows)", eager.size(), lazy.size());
+  // TestLateMatSemiAntiUnaffected (:119-149)
+  for (JoinType t : {JoinType::Semi, JoinType::Anti}) {
+    const int64_t b0 = LateMat.batches_emitted.load();
+    views = false;
+    run_late_join(g, true, t, &views);
+    EXPECT(!views && LateMat.batches_emitted.load() == b0, "semi/anti must not emit views");
+  }
+  // TestLateMatPipelineFlattensForSink (:151-185)
+  {
+    Context ctx;
+    GpuHashJoin hj(g, "l_orderkey", "o_orderkey");
+    BatchSource bs({late_build()});
+    OK(hj.Build(ctx, &bs));
+    auto pr = hj.Probe();
+    pr->LateMaterialize = true;
+    BatchSource ps({late_probe()});
+    CollectSink sink;
+    Pipeline p{&ps, {pr.get()}, &sink, 0};
+    const int64_t f0 = LateMat.flattens.load();
+    OK(p.Run(ctx));
+    EXPECT(LateMat.flattens.load() > f0, "the pipeline did not flatten for the sink");
+    for (auto& ob : sink.batches) EXPECT(!ob->HasViews(), "the sink kept a batch with live views");
+    EXPECT(sink.rows() == 3, "pipeline rows %d, want 3", sink.rows());
+  }
+  // TestLateMatChainComposition (:236-330): lineitem -> orders -> customer
+  for (bool late : {false, true}) {
+    Context ctx;
+    auto line = make_batch({{"l_orderkey", TypeID::Int64}, {"l_price", TypeID::Float64}, {"l_comment", TypeID::Int64}}, 4);
+    line->columns[0]->i64 = {1, 2, 1, 3};  // 3: no order
+    line->columns[1]->f64 = {10.0, 20.0, 11.0, 30.0};
+    line->columns[2]->i64 = {1, 2, 3, 4};
+    auto orders = make_batch({{"o_orderkey", TypeID::Int64}, {"o_custkey", TypeID::Int64}}, 2);
+    orders->columns[0]->i64 = {1, 2};
+    orders->columns[1]->i64 = {100, 200};
+    auto cust = make_batch({{"c_custkey", TypeID::Int64}, {"c_name", TypeID::Int64}}, 2);
+    cust->columns[0]->i64 = {100, 200};
+    cust->columns[1]->i64 = {7001, 7002};
+    GpuHashJoin hj1(g, "l_orderkey", "o_orderkey"), hj2(g, "o_custkey", "c_custkey");
+    BatchSource s1({orders}), s2({cust});
+    OK(hj1.Build(ctx, &s1));
+    OK(hj2.Build(ctx, &s2));
+    auto p1 = hj1.Probe(), p2 = hj2.Probe();
+    p1->LateMaterialize = p2->LateMaterialize = late;
+    const int64_t f0 = LateMat.flattens.load();
+    BatchPtr out1, out2;
+    OK(p1->Execute(ctx, line, &out1));
+    OK(p2->Execute(ctx, out1, &out2));
+    if (!out1 || !out2) {
+      EXPECT(false, "chain produced no output");
+      continue;
+    }
+    if (late) {
+      const int ck = out1->column_index("o_custkey"), pc = out1->column_index("l_price");
+      EXPECT(!out1->columns[ck]->is_view(), "probe 2 did not flatten its key column");
+      EXPECT(out1->columns[pc]->is_view(), "probe 2 flattened a pass-through column");
+      EXPECT(LateMat.flattens.load() - f0 == 1, "flatten events %lld, want 1 (probe 2's key)",
+             (long long)(LateMat.flattens.load() - f0));
+      EXPECT(out1->columns[pc]->base == line->columns[1], "the pass-through view does not compose to the scan batch");
+      const int pc2 = out2->column_index("l_price");
+      EXPECT(out2->columns[pc2]->is_view() && out2->columns[pc2]->base == line->columns[1],
+             "join 2's pass-through view does not compose to the scan batch");
+    }
+    out2->FlattenViews();
+    static std::vector<std::string> eager_rows;
+    auto rows = rows_of({out2});
+    if (!late) eager_rows = rows;
+    else EXPECT(rows == eager_rows && rows.size() == 3, "chain parity (%zu rows)", rows.size());
+  }
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   test_kernel_filter_and_pipeline();
+  test_view_vectors();
+  test_pipeline_flattens_views();
   if (mode == "gpu") {
     wg_ctx* g = nullptr;
     if (wg_ctx_create(0, &g) != WG_OK) {
@@ -570,6 +805,7 @@ int main(int argc, char** argv) {
     test_dual_key_join_kats(g);
     test_capped_partial_agg(g);
     test_shuffle_router(g);
+    test_late_materialized_join(g);
     wg_ctx_destroy(g);
   }
   std::printf("%s: %d failure(s)\n", mode.c_str(), g_fail);

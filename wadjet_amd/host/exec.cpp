@@ -25,9 +25,12 @@ static Status wrap(wg_status st, const char* what) {
 // ------------------------------------------------------------- batch helpers
 
 void batch::Bitmap::set_null(int i, int n) {
-  if (words.empty()) words.assign((n + 63) / 64, ~0ull);
+  const size_t need = (size_t)(std::max(n, i + 1) + 63) / 64;
+  if (words.size() < need) words.resize(need, ~0ull);  // a vector that grew keeps its earlier bits
   words[i >> 6] &= ~(1ull << (i & 63));
 }
+
+LateMatCounters LateMat;
 
 std::shared_ptr<Vector> Vector::make(TypeID t, int n, int scale) {
   auto v = std::make_shared<Vector>();
@@ -81,6 +84,62 @@ static int elem_bytes(TypeID t) {
 
 static wg_column col_of(const Vector& v) {
   return wg_column{data_of(v), v.nulls.has_nulls() ? v.nulls.words.data() : nullptr, wg_type_of(v.type), v.scale};
+}
+
+void Vector::Flatten() {
+  if (!base) return;
+  const int n = (int)indices.size();
+  auto flat = Vector::make(base->type, n, base->scale);
+  const int w = elem_bytes(base->type);
+  const char* s = (const char*)data_of(*base);
+  char* d = (char*)data_of(*flat);
+  for (int i = 0; i < n; ++i) {
+    const uint32_t si = indices[i];
+    if (!nulls.valid(i) || !base->nulls.valid((int)si)) {  // own NULL: the index is not read
+      flat->nulls.set_null(i, n);
+      continue;
+    }
+    memcpy(d + (size_t)i * w, s + (size_t)si * w, w);
+  }
+  type = flat->type;
+  len = n;
+  scale = flat->scale;
+  nulls = std::move(flat->nulls);
+  i32 = std::move(flat->i32);
+  i64 = std::move(flat->i64);
+  f64 = std::move(flat->f64);
+  dec = std::move(flat->dec);
+  base.reset();
+  indices.clear();
+}
+
+std::shared_ptr<Vector> batch::NewViewVector(const std::shared_ptr<Vector>& base, std::vector<uint32_t> indices) {
+  auto v = std::make_shared<Vector>();
+  v->type = base->type;
+  v->scale = base->scale;
+  v->len = (int)indices.size();
+  if (base->is_view()) {  // compose: fold the base view's indirection and own NULLs
+    for (size_t i = 0; i < indices.size(); ++i) {
+      const uint32_t x = indices[i];
+      if (!base->nulls.valid((int)x)) v->nulls.set_null((int)i, v->len);
+      indices[i] = base->indices[x];
+    }
+    v->base = base->base;
+  } else {
+    v->base = base;
+  }
+  v->indices = std::move(indices);
+  return v;
+}
+
+bool RecordBatch::HasViews() const {
+  for (auto& c : columns)
+    if (c->is_view()) return true;
+  return false;
+}
+
+void RecordBatch::FlattenViews() {
+  for (auto& c : columns) c->Flatten();
 }
 
 // copy rows[idx] of src into a fresh vector
@@ -190,7 +249,11 @@ int CollectSink::rows() const {
 // after every Execute before the next input (ChainDriver, pipeline.go:196-235)
 static Status push(Context& ctx, std::vector<UnaryOperator*>& ops, size_t i, Sink* sink, const BatchPtr& b) {
   if (ctx.cancelled) return Status::Error("context canceled");
-  if (i == ops.size()) return sink->Consume(ctx, b);
+  if (i == ops.size()) {
+    FlattenForConsumer(b.get(), sink);
+    return sink->Consume(ctx, b);
+  }
+  FlattenForConsumer(b.get(), ops[i]);
   BatchPtr out;
   TRY(ops[i]->Execute(ctx, b, &out));
   if (out) TRY(push(ctx, ops, i + 1, sink, out));
@@ -337,6 +400,7 @@ Status GpuHashAggregate::Consume(Context& ctx, const BatchPtr& b) {
   if (ctx.cancelled) return Status::Error("context canceled");
   std::lock_guard<std::mutex> lk(mu_);
   if (finalized_) return Status::Error("Consume after Finalize");
+  FlattenForConsumer(b.get(), this);  // not view-aware: reads typed storage
   if (!h_) TRY(resolve(*b));
   std::vector<wg_column> cols;
   for (int ci : key_idx_) cols.push_back(col_of(*b->columns[ci]));
@@ -482,6 +546,7 @@ Status GpuHashJoin::Build(Context& ctx, Source* build) {
     BatchPtr b;
     TRY(build->Next(ctx, &b));
     if (!b) break;
+    FlattenForConsumer(b.get(), nullptr);  // the build retains typed storage (flattenSource, late_mat.go:55-68)
     // the build keeps every row of a batch (a Sel is applied by compacting first)
     BatchPtr c = b;
     if (b->has_sel) {
@@ -622,6 +687,12 @@ Status GpuHashJoinProbe::Execute(Context& ctx, const BatchPtr& in, BatchPtr* out
   if (!join_->j_) return Status::OK();  // empty build
   std::vector<int> ki;
   TRY(join_->probe_key_idx(*in, &ki));
+  for (int k : ki)  // the keys go to the device as values; the other columns stay views
+    if (in->columns[k]->is_view()) {
+      in->FlattenColumn(k);
+      LateMat.flattens.fetch_add(1);
+    }
+  if (!LateMaterialize) FlattenForConsumer(in.get(), nullptr);  // the eager gather reads typed storage
   TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
   TRY(ensure_out(kMaxProbeOutputRows));
   cur_ = in;
@@ -647,6 +718,25 @@ Status GpuHashJoinProbe::Execute(Context& ctx, const BatchPtr& in, BatchPtr* out
   auto all = std::make_shared<RecordBatch>();
   all->schema = parts[0]->schema;
   for (size_t ci = 0; ci < all->schema.size(); ++ci) {
+    if (parts[0]->columns[ci]->is_view()) {
+      // views of one base: one view with the parts' indices and own NULLs in turn
+      auto v = std::make_shared<Vector>();
+      v->type = parts[0]->columns[ci]->type;
+      v->scale = parts[0]->columns[ci]->scale;
+      v->base = parts[0]->columns[ci]->base;
+      for (auto& p : parts) {
+        const Vector& s = *p->columns[ci];
+        const int old = (int)v->indices.size();
+        v->indices.insert(v->indices.end(), s.indices.begin(), s.indices.end());
+        v->len = (int)v->indices.size();
+        if (s.nulls.has_nulls())
+          for (int i = 0; i < s.len; ++i)
+            if (!s.nulls.valid(i)) v->nulls.set_null(old + i, v->len);
+      }
+      all->columns.push_back(v);
+      all->len = v->len;
+      continue;
+    }
     auto v = Vector::make(all->schema[ci].type, 0, all->schema[ci].scale);
     for (auto& p : parts) {
       const Vector& s = *p->columns[ci];
@@ -719,17 +809,23 @@ Status GpuHashJoinProbe::emit(BatchPtr* out) {
   TRY(wrap(wg_memcpy_d2h(join_->ctx_, br.data(), d_out_b_, got * 4), "download pairs"));
   auto o = std::make_shared<RecordBatch>();
   o->len = (int)got;
-  // output = probe columns, then build columns (joinOutputSchemaWithMapping, join.go:3898)
+  // output = probe columns, then build columns (joinOutputSchemaWithMapping, join.go:3898);
+  // late materialization: views at the pair indices (emitViewOutput, join.go:2517-2600)
   for (size_t ci = 0; ci < in.columns.size(); ++ci) {
     o->schema.push_back(in.schema[ci]);
-    o->columns.push_back(gather(*in.columns[ci], pr.data(), got));
+    o->columns.push_back(LateMaterialize ? NewViewVector(in.columns[ci], pr) : gather(*in.columns[ci], pr.data(), got));
   }
   for (size_t ci = 0; ci < join_->build_cols_.size(); ++ci) {
     Column c = join_->build_schema_[ci];
     for (auto& pc : in.schema)
       if (pc.name == c.name) c.name = "build." + c.name;  // qualify duplicate names
     o->schema.push_back(c);
-    o->columns.push_back(gather(*join_->build_cols_[ci], br.data(), got));
+    o->columns.push_back(LateMaterialize ? NewViewVector(join_->build_cols_[ci], br)
+                                         : gather(*join_->build_cols_[ci], br.data(), got));
+  }
+  if (LateMaterialize) {
+    LateMat.batches_emitted.fetch_add(1);
+    LateMat.view_columns.fetch_add((int64_t)o->columns.size());
   }
   *out = o;
   return Status::OK();
@@ -835,6 +931,7 @@ Status GpuShuffleRouter::Route(Context& ctx, const BatchPtr& b, std::vector<std:
     }
     resolved_ = true;
   }
+  for (int ci : key_idx_) b->FlattenColumn(ci);  // only the key columns are read
   const int np = std::max(num_parts_, 1);
   per_part_rows->assign(np, {});
   const uint64_t n = (uint64_t)b->active();
@@ -948,6 +1045,7 @@ void CappedPartialAgg::resolve_against(const RecordBatch& b) {
 
 Status CappedPartialAgg::Consume(Context& ctx, const BatchPtr& b, std::vector<BatchPtr>* flushed) {
   flushed->clear();
+  FlattenForConsumer(b.get(), nullptr);
   if (!resolved_) resolve_against(*b);
   in_rows += b->active();
   if (disabled_) {  // raw passthrough

@@ -8,7 +8,8 @@
 // the operators the GPU library backs:
 //   GpuHashAggregate   SinkSource + MergeableSink      (aggregate.go:98, :1075, :3855, :4202, :4646-4700)
 //   GpuHashJoin        Build(ctx, Source) / Probe()      (join.go:771, :1967)
-//   GpuHashJoinProbe   UnaryOperator + BoundedOutput     (join.go:2072-2139, :2306-2515)
+//   GpuHashJoinProbe   UnaryOperator + BoundedOutput     (join.go:2072-2139, :2306-2600; view output
+//                      + ViewAware                        late_mat.go, batch/view.go)
 //   GpuBloomFilterOp   UnaryOperator + Cloneable         (bloom_filter_op.go:13-274)
 //   GpuShuffleRouter   partitioned shuffle routing       (worker/partitioned_shuffle_sink.go:199-290)
 //   CappedPartialAgg   exchange partial aggregation      (worker/shuffle_partial_agg.go:36-234)
@@ -16,6 +17,7 @@
 // BatchSource, CollectSink and Pipeline{Source, Ops, Sink, Workers}
 // (pipeline.go:31-690). Errors are returned as Status (Go's `error`).
 #pragma once
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <mutex>
@@ -76,8 +78,22 @@ struct Vector {
   std::vector<int64_t> i64;
   std::vector<double> f64;
   std::vector<Int128> dec;  // DecimalData
+  // A view (late materialization, view.go:10-26) owns no storage: row i is
+  // row indices[i] of `base` (always an owned vector), NULL when the view's
+  // own bitmap `nulls` or the base says so.
+  std::shared_ptr<Vector> base;
+  std::vector<uint32_t> indices;
+  bool is_view() const { return base != nullptr; }
+  // Vector.Flatten (view.go:87-230): gather through the indices into owned
+  // storage in place (own-NULL rows do not read the base); no-op when owned.
+  void Flatten();
   static std::shared_ptr<Vector> make(TypeID t, int n, int scale = 0);
 };
+
+// NewViewVector (view.go:27-80): a view of `base` at `indices`; over a view it
+// composes (indices through the base view's, its own NULLs folded in) so the
+// result's base is always owned.
+std::shared_ptr<Vector> NewViewVector(const std::shared_ptr<Vector>& base, std::vector<uint32_t> indices);
 
 struct RecordBatch {
   std::vector<std::shared_ptr<Vector>> columns;
@@ -87,6 +103,9 @@ struct RecordBatch {
   bool has_sel = false;
   int column_index(const std::string& name) const;
   int active() const { return has_sel ? (int)sel.size() : len; }
+  bool HasViews() const;   // view.go:259-267
+  void FlattenViews();     // view.go:271-275
+  void FlattenColumn(int i) { columns[i]->Flatten(); }
 };
 
 }  // namespace batch
@@ -132,6 +151,35 @@ struct MergeableSink : SinkSource {  // operators.go:56-60
   virtual std::unique_ptr<SinkSource> CloneSink() = 0;
   virtual Status MergeSink(SinkSource* other) = 0;
 };
+
+// Late materialization (late_mat.go:11-68): process-wide counters that prove
+// the feature engaged, the ViewAware opt-in, and the defensive flatten in
+// front of every consumer that has not opted in.
+struct LateMatCounters {
+  std::atomic<int64_t> batches_emitted{0};  // probe output batches emitted as views
+  std::atomic<int64_t> view_columns{0};     // view columns across those batches
+  std::atomic<int64_t> flattens{0};         // FlattenViews calls made for consumers
+};
+extern LateMatCounters LateMat;
+
+struct ViewAware {  // late_mat.go:32-34
+  virtual ~ViewAware() = default;
+  virtual bool AcceptsViews() const = 0;
+};
+
+// FlattenForConsumer (late_mat.go:43-53)
+template <class T>
+void FlattenForConsumer(RecordBatch* b, T* consumer) {
+  if (!b || !b->HasViews()) return;
+  if (auto* va = dynamic_cast<ViewAware*>(consumer); va && va->AcceptsViews()) return;
+  b->FlattenViews();
+  LateMat.flattens.fetch_add(1);
+}
+inline void FlattenForConsumer(RecordBatch* b, std::nullptr_t) {
+  if (!b || !b->HasViews()) return;
+  b->FlattenViews();
+  LateMat.flattens.fetch_add(1);
+}
 
 struct BoundedOutputOperator {  // operators.go:86-90
   virtual ~BoundedOutputOperator() = default;
@@ -307,7 +355,7 @@ class GpuHashJoin {
 // MaxProbeOutputRows = 32768 (join.go:2044-2060)
 constexpr uint64_t kMaxProbeOutputRows = 32768;
 
-class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, public Cloneable {
+class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, public Cloneable, public ViewAware {
  public:
   explicit GpuHashJoinProbe(GpuHashJoin* j);
   ~GpuHashJoinProbe() override;
@@ -317,7 +365,17 @@ class GpuHashJoinProbe : public UnaryOperator, public BoundedOutputOperator, pub
   void EnableBoundedOutput() override { bounded_ = true; }
   bool HasPendingOutput() override { return pending_; }
   Status NextOutput(Context&, BatchPtr* out) override;
-  std::unique_ptr<UnaryOperator> Clone() override { return std::make_unique<GpuHashJoinProbe>(join_); }
+  std::unique_ptr<UnaryOperator> Clone() override {
+    auto c = std::make_unique<GpuHashJoinProbe>(join_);
+    c->LateMaterialize = LateMaterialize;
+    return c;
+  }
+  // Reads view inputs (its key columns flattened, the rest composed), join.go:2234.
+  bool AcceptsViews() const override { return true; }
+  // HashJoinProbe.LateMaterialize (join.go:2436-2442): inner-join output as
+  // view columns over the probe batch and the retained build columns at the
+  // pair indices instead of gathered copies. Off by default.
+  bool LateMaterialize = false;
 
  private:
   Status emit(BatchPtr* out);  // one bounded chunk of the current input
