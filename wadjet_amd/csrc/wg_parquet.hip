@@ -179,6 +179,7 @@ __device__ __forceinline__ void win_load(const wg_pq_runs& S, RunWindow& w, uint
 
 constexpr uint32_t kPqChunks = kPqSeg / 64;  // 64-row chunks per segment
 constexpr int kPqUnroll = 4;                 // chunks in flight per wavefront
+constexpr uint32_t kPqLdsDict = 2048;        // dictionaries up to this many entries are staged in LDS
 static_assert(kPqTile == 64 * 64, "the mask pass gives each lane of a wavefront one chunk of a tile");
 
 __device__ __forceinline__ uint64_t low_bits(uint64_t n) { return n >= 64 ? ~0ull : (1ull << n) - 1; }
@@ -246,6 +247,14 @@ __global__ void pq_values_kernel(const PqPageDev* pages, const uint32_t* tile_pa
   const PqSeg g = segment_of(pages, tile_page, wave);
   const wg_pq_page& P = g.P;
   const uint64_t s0 = g.s0, s1 = g.s1;
+  // a small dictionary is staged in LDS once for the workgroup (its four
+  // wavefronts decode one page), so an index resolves without a global load
+  __shared__ T sdict[kPqLdsDict];
+  const bool lds_dict = P.encoding != 0 && P.dict_n <= kPqLdsDict;
+  if (lds_dict) {
+    for (uint32_t e = threadIdx.x; e < (uint32_t)P.dict_n; e += blockDim.x) sdict[e] = ((const T*)P.dict)[e];
+    __syncthreads();
+  }
   if (s0 >= s1) return;
   const bool nullable = P.def.n_runs != 0;
   const bool dict = P.encoding != 0;
@@ -310,7 +319,7 @@ __global__ void pq_values_kernel(const PqPageDev* pages, const uint32_t* tile_pa
             const uint32_t r0 = held ? iw.first + (uint32_t)h : 0u;
             got = run_value(P.idx, runs[run_advance(runs, P.idx.n_runs, r0, k)], k, &d);
           }
-          if (got && d < P.dict_n) v[u] = ((const T*)P.dict)[d];
+          if (got && d < P.dict_n) v[u] = lds_dict ? sdict[d] : ((const T*)P.dict)[d];
           else ++nbad;  // no index stream, an index past it, or one past the dictionary
         }
       }
