@@ -250,7 +250,7 @@ __global__ void pq_values_kernel(const PqPageDev* pages, const uint32_t* tile_pa
   // a small dictionary is staged in LDS once for the workgroup (its four
   // wavefronts decode one page), so an index resolves without a global load
   __shared__ T sdict[kPqLdsDict];
-  const bool lds_dict = P.encoding != 0 && P.dict_n <= kPqLdsDict;
+  const bool lds_dict = P.encoding != 0 && P.dict != nullptr && P.dict_n <= kPqLdsDict;
   if (lds_dict) {
     for (uint32_t e = threadIdx.x; e < (uint32_t)P.dict_n; e += blockDim.x) sdict[e] = ((const T*)P.dict)[e];
     __syncthreads();
