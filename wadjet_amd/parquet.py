@@ -21,6 +21,7 @@ is quoted on: k int64 uniform over 1000 distinct values, v int64 uniform in
 from __future__ import annotations
 
 import hashlib
+import os
 from typing import Iterator, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -300,7 +301,8 @@ def _chunk_pages(raw: bytes, compression: str, n_rows: int, max_def: int, width:
     codec = None if compression == "UNCOMPRESSED" else pa.Codec(compression.lower())
 
     def inflate(b, size):
-        return bytes(b) if codec is None else codec.decompress(b, decompressed_size=size).to_pybytes()
+        # an Arrow buffer read in place (no copy into a Python bytes object)
+        return b if codec is None else memoryview(codec.decompress(b, decompressed_size=size)).cast("B")
 
     pages, dict_ref = [], None
     pos, rows_seen = 0, 0
@@ -384,22 +386,54 @@ class StagedColumn:
         self.phys, self.type, self.scale = _wg_type_of(cs)
         self.width = 4 if self.phys == PT_INT32 else 8
         rgs = list(range(md.num_row_groups)) if row_groups is None else list(row_groups)
-        arena = _Arena()
-        pages = []
-        self.rows = 0
-        with open(path, "rb") as fh:
-            for rg in rgs:
-                cc = md.row_group(rg).column(ci)
-                n_rows = md.row_group(rg).num_rows
-                start = (cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset
-                         else cc.data_page_offset)
+
+        def stage(rg):
+            # one chunk: read, decompress, scan its streams into its own arena
+            # (file reads, decompression and run scans release the GIL)
+            cc = md.row_group(rg).column(ci)
+            n_rows = md.row_group(rg).num_rows
+            start = (cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset
+                     else cc.data_page_offset)
+            with open(path, "rb") as fh:
                 fh.seek(start)
                 raw = fh.read(cc.total_compressed_size)
-                pages += _chunk_pages(raw, cc.compression, n_rows, self.max_def, self.width, arena)
-                self.rows += n_rows
-        self.page_bytes = sum(len(b) for _, b in arena.parts)
-        self.dev = ctx.to_device(arena.pack())
+            arena = _Arena()
+            return n_rows, arena, _chunk_pages(raw, cc.compression, n_rows, self.max_def, self.width, arena)
+
+        import concurrent.futures as cf
+        # the page walk holds the GIL: past two threads the contention costs more than the overlap gains
+        workers = max(1, min(len(rgs), 2, os.cpu_count() or 1))
+        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+            staged = list(ex.map(stage, rgs))
+        # the chunks' arenas laid end to end in one host buffer, one upload
+        offs, total = [], 0
+        for _, arena, _ in staged:
+            offs.append(total)
+            total += arena.size
+        host = np.zeros(max(total, 8), np.uint8)
+
+        def place(i):
+            for off, b in staged[i][1].parts:
+                host[offs[i] + off:offs[i] + off + len(b)] = b
+
+        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(place, range(len(staged))))
+        self.rows = sum(n for n, _, _ in staged)
+        self.page_bytes = sum(len(b) for _, arena, _ in staged for _, b in arena.parts)
+        self.dev = ctx.to_device(host)
+        del host
         base = self.dev.ptr.value if hasattr(self.dev.ptr, "value") else int(self.dev.ptr)
+        pages = []
+        for (_, _, chunk_pages), off in zip(staged, offs):
+            for pg in chunk_pages:
+                pg = dict(pg)
+                for key in ("def", "plain", "idx"):
+                    if key in pg:
+                        v = pg[key]
+                        pg[key] = (v + off) if key == "plain" else (v[0] + off, v[1] + off) + tuple(v[2:])
+                if "dict" in pg:
+                    pg["dict"] = (pg["dict"][0] + off, pg["dict"][1])
+                pages.append(pg)
         self.pages = (L.PqPage * max(len(pages), 1))()
         for P, pg in zip(self.pages, pages):
             P.num_rows = pg["rows"]
