@@ -491,11 +491,20 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
   PqPageDev* dtable = nullptr;
   // per 64-row chunk: present flags, present-row counts, their exclusive sums
   uint64_t* masks = nullptr;
+  unsigned long long* bad = nullptr;
+  // scratch goes back to the stream's pool on every exit, errors included
+  struct Release {
+    hipStream_t st;
+    void** p[3];
+    ~Release() {
+      for (void** q : p)
+        if (*q) (void)hipFreeAsync(*q, st);
+    }
+  } release{ctx->stream, {(void**)&dtable, (void**)&masks, (void**)&bad}};
   uint32_t* counts = nullptr;
   uint32_t* prefix = nullptr;
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
-  unsigned long long* bad = nullptr;
   WG_HIP(hipMallocAsync((void**)&bad, 8, ctx->stream));
   WG_HIP(hipMemsetAsync(bad, 0, 8, ctx->stream));
   WG_HIP(hipMallocAsync((void**)&dtable, upload.size(), ctx->stream));
@@ -533,9 +542,6 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
   }
   uint64_t nbad = 0;
   WG_HIP(hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
-  if (masks) WG_HIP(hipFreeAsync(masks, ctx->stream));
-  WG_HIP(hipFreeAsync(dtable, ctx->stream));
-  WG_HIP(hipFreeAsync(bad, ctx->stream));
   WG_HIP(hipStreamSynchronize(ctx->stream));
   WG_CHECK(nbad == 0, WG_ERR_INVALID, "%llu values with their dictionary index missing or out of range, or past "
            "a stream's end", (unsigned long long)nbad);

@@ -66,6 +66,13 @@ wg_status wg_view_flatten(wg_ctx* ctx, const wg_column* base, uint64_t base_rows
   WG_CHECK(base->data || base_rows == 0, WG_ERR_INVALID, "null base column");
   if (n == 0) return WG_OK;
   unsigned long long* bad = nullptr;
+  struct Release {  // the counter goes back to the stream's pool on every exit
+    hipStream_t st;
+    unsigned long long** p;
+    ~Release() {
+      if (*p) (void)hipFreeAsync(*p, st);
+    }
+  } release{ctx->stream, &bad};
   WG_HIP(hipMallocAsync((void**)&bad, 8, ctx->stream));
   WG_HIP(hipMemsetAsync(bad, 0, 8, ctx->stream));
   const int grid = grid_for(n, 256, 16384);
@@ -94,7 +101,6 @@ wg_status wg_view_flatten(wg_ctx* ctx, const wg_column* base, uint64_t base_rows
   WG_HIP(hipGetLastError());
   uint64_t nbad = 0;
   WG_HIP(hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
-  WG_HIP(hipFreeAsync(bad, ctx->stream));
   WG_HIP(hipStreamSynchronize(ctx->stream));
   WG_CHECK(nbad == 0, WG_ERR_INVALID, "%llu view indices past the base column's %llu rows", (unsigned long long)nbad,
            (unsigned long long)base_rows);
