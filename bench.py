@@ -176,6 +176,60 @@ def bench_join(ctx, args):
                 _keep=(j, agg, bcols, pcols))
 
 
+# -------------------------------------------- join output: views vs eager gathers
+
+def bench_join_output(ctx, args):
+    """The materializing inner join of the fused-join workload (build 10M, probe
+    200M, ~190M matches): the probe emitting its (probe row, build row) arrays
+    as the view columns' indices (late materialization, emitViewOutput) vs the
+    same probe followed by the eager gather of all five output columns (fk,
+    qty, key, p1, p2) with their validity (wg_view_flatten)."""
+    from wadjet_amd import workloads as W
+    from wadjet_amd.gpu import HashJoin, view_flatten
+    cfg = W.FkJoin(n_build=args.join_build, n_probe=args.join_probe)
+    log(f"[join_output] generating build {cfg.n_build} / probe {cfg.n_probe}")
+    bcols = W.gen_fk_join_build(ctx, cfg)
+    pcols = W.gen_fk_join_probe(ctx, cfg)
+    j = HashJoin(ctx, W.FKJ_BUILD_TYPES, 0, inline=[1, 2], scales=W.FKJ_BUILD_SCALES, build_hint=cfg.n_build)
+    j.build(bcols, cfg.n_build)
+    j.finish()
+    cap = cfg.n_probe + 1024  # unique build keys: at most one match per probe row
+    outp, outb = ctx.alloc(cap * 4), ctx.alloc(cap * 4)
+    bases = [(c, cfg.n_probe, outp) for c in pcols] + [(j.build_column(i), cfg.n_build, outb) for i in range(3)]
+    from wadjet_amd.gpu import TYPE_BYTES
+    outs = [(ctx.alloc(cap * TYPE_BYTES[c.type]), ctx.alloc(((cap + 63) // 64) * 8)) for c, _, _ in bases]
+
+    def probe():
+        got, cur = j.probe(pcols[0], cfg.n_probe, outp, outb, cap)
+        assert cur.done == 1
+        return got
+
+    def eager():
+        got = probe()
+        for (col, rows, idx), (o, ov) in zip(bases, outs):
+            view_flatten(ctx, col, rows, idx, got, out=o, out_valid=ov)
+        return got
+
+    res = {}
+    for name, fn in (("views", probe), ("eager", eager)):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        steps = max(1, args.steps)
+        for _ in range(steps):
+            got = fn()
+        ctx.sync()
+        res[name] = (time.perf_counter() - t0) * 1000 / steps
+    gathered = got * (8 + 4 + 8 + 4 + 8)
+    for b in [outp, outb] + [x for pair in outs for x in pair]:
+        b.free()
+    log(f"[join_output] {got} pairs: views {res['views']:.3f} ms, eager gathers {res['eager']:.3f} ms "
+        f"({gathered / 1e9:.2f} GB of gathered columns deferred)")
+    return dict(pairs=got, views_ms=res["views"], eager_ms=res["eager"], deferred_bytes=gathered,
+                pairs_per_s_views=got / (res["views"] / 1000), pairs_per_s_eager=got / (res["eager"] / 1000))
+
+
 # -------------------------------------------- shuffle routing of the GROUP BY rows
 
 def bench_shuffle(ctx, args, cols=None):
@@ -419,7 +473,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--qps", type=float, nargs="*", default=[100, 1000, 5000])
     ap.add_argument("--shuffle-parts", type=int, default=256)
-    ap.add_argument("--only", choices=["agg", "join", "batched", "served", "shuffle", "wide", "parquet"], default=None)
+    ap.add_argument("--only", choices=["agg", "join", "join_output", "batched", "served", "shuffle", "wide", "parquet"], default=None)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -445,6 +499,8 @@ def main():
         J = bench_join(ctx, args)
         J.pop("_keep")
         extra["join"] = J
+    if want("join_output"):
+        extra["join_output"] = bench_join_output(ctx, args)
     if want("shuffle"):
         extra["shuffle"] = bench_shuffle(ctx, args, A["cols"] if A is not None else None)
     if want("wide"):

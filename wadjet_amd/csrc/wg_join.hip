@@ -125,14 +125,37 @@ __device__ __forceinline__ const uint64_t* probe_slot(const JoinTable& J, const 
   return join_find(J, (uint64_t)load_int(pk, row));
 }
 
-__global__ void join_count_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, uint64_t pos0, uint64_t n,
-                                  uint64_t* cnt) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+// The materializing probe works on tiles of kProbeTile probe positions, one
+// workgroup each. Pass 1 looks every position up once and keeps its chain
+// head and match count (count in the low word, head in the high word), and
+// each tile's pair total; a scan over the tile totals gives every tile its
+// output offset; pass 3 scans its tile's counts in LDS and writes the pairs
+// from the kept heads, so the table is read once per probe row.
+constexpr uint64_t kProbeTile = 4096;
+
+__global__ void __launch_bounds__(256) join_count_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, uint64_t pos0,
+                                                         uint64_t n, uint64_t* hc, uint64_t* tile_cnt) {
+  const uint64_t t0 = (uint64_t)blockIdx.x * kProbeTile;
+  const uint64_t t1 = min(t0 + kProbeTile, n);
+  uint64_t local = 0;
+  for (uint64_t j = t0 + threadIdx.x; j < t1; j += blockDim.x) {
     const uint64_t i = pos0 + j;
     const uint64_t row = sel ? sel[i] : i;
     const uint64_t* s = probe_slot(J, pk, row);
-    cnt[j] = s ? slot_count(s) : 0;
+    const uint32_t c = s ? (uint32_t)slot_count(s) : 0u;
+    const int32_t h = s ? slot_head(s) : -1;
+    hc[j] = (uint64_t)c | ((uint64_t)(uint32_t)h << 32);
+    local += c;
   }
+  __shared__ uint64_t part[4];
+  for (int o = 32; o; o >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)local, o);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(local >> 32), o);
+    local += ((uint64_t)hi << 32) | lo;
+  }
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
 // First position whose pairs do not fit in `cap`. off is non-decreasing, so
@@ -142,19 +165,38 @@ __global__ void join_cut_kernel(const uint64_t* off, const uint64_t* cnt, uint64
     if (off[j] <= cap && off[j] + cnt[j] > cap) *cut = j;
 }
 
-__global__ void join_emit_kernel(JoinTable J, ColDesc pk, const uint32_t* sel, uint64_t pos0, uint64_t n,
-                                 const uint64_t* off, uint32_t* out_p, uint32_t* out_b) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = pos0 + j;
-    const uint64_t row = sel ? sel[i] : i;
-    const uint64_t* s = probe_slot(J, pk, row);
-    if (!s) continue;
-    uint64_t o = off[j];
-    for (int32_t b = slot_head(s); b >= 0; b = J.next[b]) {
-      out_p[o] = (uint32_t)row;
-      out_b[o] = (uint32_t)b;
-      ++o;
+// Pass 3: tile b's pairs at its scanned offset, positions in order. Only
+// positions whose pairs end at or before `room` write; the one position that
+// straddles `room` records itself and its offset in cut[0..1].
+__global__ void __launch_bounds__(256) join_emit_kernel(JoinTable J, const uint32_t* sel, uint64_t pos0, uint64_t n,
+                                                        const uint64_t* hc, const uint64_t* tile_off, uint64_t room,
+                                                        uint32_t* out_p, uint32_t* out_b, uint64_t* cut) {
+  using BlockScan = hipcub::BlockScan<uint32_t, 256>;
+  __shared__ typename BlockScan::TempStorage tmp;
+  const uint64_t t0 = (uint64_t)blockIdx.x * kProbeTile;
+  const uint64_t t1 = min(t0 + kProbeTile, n);
+  uint64_t base = tile_off[blockIdx.x];
+  for (uint64_t j0 = t0; j0 < t1; j0 += blockDim.x) {
+    const uint64_t j = j0 + threadIdx.x;
+    const uint64_t v = j < t1 ? hc[j] : 0;
+    const uint32_t c = (uint32_t)v;
+    uint32_t pre = 0, total = 0;
+    BlockScan(tmp).ExclusiveSum(c, pre, total);
+    const uint64_t o = base + pre;
+    if (c && o + c <= room) {
+      const uint64_t row = sel ? sel[pos0 + j] : pos0 + j;
+      int32_t b = (int32_t)(uint32_t)(v >> 32);
+      for (uint32_t k = 0; k < c && b >= 0; ++k) {
+        out_p[o + k] = (uint32_t)row;
+        out_b[o + k] = (uint32_t)b;
+        if (k + 1 < c) b = J.next[b];  // a chain's last link is not read
+      }
+    } else if (c && o <= room) {
+      cut[0] = j;
+      cut[1] = o;
     }
+    base += total;
+    __syncthreads();  // the scan storage is reused
   }
 }
 
@@ -665,14 +707,19 @@ static bool lean_plan(const wg_join* j, const wg_agg* agg, const wg_column* prob
 
 static wg_status ensure_probe_scratch(wg_join* j, uint64_t n) {
   if (n <= j->scratch_rows) return WG_OK;
+  const uint64_t tiles = (n + kProbeTile - 1) / kProbeTile;
   if (j->cnt) WG_HIP(hipFree(j->cnt));
   if (j->off) WG_HIP(hipFree(j->off));
-  WG_HIP(hipMalloc(&j->cnt, n * 8));
-  WG_HIP(hipMalloc(&j->off, n * 8));
+  j->cnt = j->off = nullptr;
+  j->scratch_rows = 0;
+  WG_HIP(hipMalloc(&j->cnt, n * 8));                // per position: match count | chain head << 32
+  WG_HIP(hipMalloc(&j->off, (2 * tiles + 2) * 8));  // per tile: pair count, then its offset
   size_t bytes = 0;
-  WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, j->cnt, j->off, (int)std::min<uint64_t>(n, 1u << 30), j->ctx->stream));
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, j->off, j->off + tiles, (int)tiles, j->ctx->stream));
   if (bytes > j->cub_bytes) {
     if (j->cub_tmp) WG_HIP(hipFree(j->cub_tmp));
+    j->cub_tmp = nullptr;
+    j->cub_bytes = 0;
     WG_HIP(hipMalloc(&j->cub_tmp, bytes));
     j->cub_bytes = bytes;
   }
@@ -737,7 +784,7 @@ static wg_status join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* b
     j->inline_cols[i] = c;
   }
   j->stride = (int32_t)next_pow2(2 + n_inline);
-  WG_HIP(hipMalloc(&j->counter, 64));
+  WG_HIP(hipMalloc(&j->counter, 128));  // 16 words: claims, digest, chain result, probe cut
   WG_TRY(zero_dev(ctx, j->counter, 8));
   wg_status st = join_alloc_table(j, next_pow2(std::max<uint64_t>(build_hint, 1024) * 2));
   if (st == WG_OK && dual) st = kd_init(ctx, j->dict, build_hint);
@@ -991,38 +1038,54 @@ wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
     }
   }
   const uint64_t pos = cursor->pos;
-  const uint64_t m = n - pos;
   const uint64_t room = out_cap - written;
+  // a window of positions per call keeps a bounded probe of a large batch
+  // linear: at most room pairs come out, and positions without a match
+  // produce none, so twice the room (and at least 64K positions) is examined
+  const uint64_t m = std::min<uint64_t>(n - pos, std::max<uint64_t>(2 * room, 1ull << 16));
   WG_TRY(ensure_probe_scratch(j, m));
-  const int g = grid_for(m, 256, 16384);
-  hipLaunchKernelGGL(join_count_kernel, dim3(g), dim3(256), 0, j->ctx->stream, J, pk, sel, pos, m, j->cnt);
+  const uint64_t tiles = (m + kProbeTile - 1) / kProbeTile;
+  uint64_t* hc = j->cnt;
+  uint64_t* tile_cnt = j->off;
+  uint64_t* tile_off = j->off + tiles;
+  hipLaunchKernelGGL(join_count_kernel, dim3((uint32_t)tiles), dim3(256), 0, j->ctx->stream, J, pk, sel, pos, m, hc,
+                     tile_cnt);
   WG_HIP(hipGetLastError());
   size_t bytes = j->cub_bytes;
-  WG_HIP(hipcub::DeviceScan::ExclusiveSum(j->cub_tmp, bytes, j->cnt, j->off, (int)m, j->ctx->stream));
-  uint64_t* cut_d = j->counter + 6;
-  const uint64_t big = ~0ull;
-  WG_HIP(hipMemcpyAsync(cut_d, &big, 8, hipMemcpyHostToDevice, j->ctx->stream));
-  hipLaunchKernelGGL(join_cut_kernel, dim3(g), dim3(256), 0, j->ctx->stream, (const uint64_t*)j->off,
-                     (const uint64_t*)j->cnt, m, room, cut_d);
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(j->cub_tmp, bytes, tile_cnt, tile_off, (int)tiles, j->ctx->stream));
+  uint64_t* cut_d = j->counter + 6;  // [0] first tile that does not fit, then the straddling position and offset
+  const uint64_t init[3] = {~0ull, ~0ull, 0};
+  WG_HIP(hipMemcpyAsync(cut_d, init, 24, hipMemcpyHostToDevice, j->ctx->stream));
+  hipLaunchKernelGGL(join_cut_kernel, dim3(grid_for(tiles, 256, 4096)), dim3(256), 0, j->ctx->stream,
+                     (const uint64_t*)tile_off, (const uint64_t*)tile_cnt, tiles, room, cut_d);
   WG_HIP(hipGetLastError());
-  uint64_t cut = 0;
-  WG_HIP(hipMemcpyAsync(&cut, cut_d, 8, hipMemcpyDeviceToHost, j->ctx->stream));
+  uint64_t cut_t = 0;
+  WG_HIP(hipMemcpyAsync(&cut_t, cut_d, 8, hipMemcpyDeviceToHost, j->ctx->stream));
   WG_HIP(hipStreamSynchronize(j->ctx->stream));
-  const uint64_t fit = cut == ~0ull ? m : cut;
-  uint64_t total = 0;
-  if (fit > 0) {
-    uint64_t lastoff = 0, lastcnt = 0;
-    WG_HIP(hipMemcpy(&lastoff, j->off + fit - 1, 8, hipMemcpyDeviceToHost));
-    WG_HIP(hipMemcpy(&lastcnt, j->cnt + fit - 1, 8, hipMemcpyDeviceToHost));
-    total = lastoff + lastcnt;
-    hipLaunchKernelGGL(join_emit_kernel, dim3(grid_for(fit, 256, 16384)), dim3(256), 0, j->ctx->stream, J, pk, sel, pos,
-                       fit, (const uint64_t*)j->off, out_probe_rows + written, out_build_rows + written);
-    WG_HIP(hipGetLastError());
+  const uint64_t emit_tiles = cut_t == ~0ull ? tiles : cut_t + 1;
+  hipLaunchKernelGGL(join_emit_kernel, dim3((uint32_t)emit_tiles), dim3(256), 0, j->ctx->stream, J, sel, pos, m,
+                     (const uint64_t*)hc, (const uint64_t*)tile_off, room, out_probe_rows + written,
+                     out_build_rows + written, cut_d + 1);
+  WG_HIP(hipGetLastError());
+  uint64_t fit = m, total = 0;
+  if (cut_t == ~0ull) {
+    uint64_t last[2] = {0, 0};
+    WG_HIP(hipMemcpyAsync(&last[0], tile_off + tiles - 1, 8, hipMemcpyDeviceToHost, j->ctx->stream));
+    WG_HIP(hipMemcpyAsync(&last[1], tile_cnt + tiles - 1, 8, hipMemcpyDeviceToHost, j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    total = last[0] + last[1];
+  } else {
+    uint64_t c2[2] = {0, 0};
+    WG_HIP(hipMemcpyAsync(c2, cut_d + 1, 16, hipMemcpyDeviceToHost, j->ctx->stream));
+    WG_HIP(hipStreamSynchronize(j->ctx->stream));
+    WG_CHECK(c2[0] < m, WG_ERR_DEVICE, "probe output cut not found");
+    fit = c2[0];
+    total = c2[1];
   }
   written += total;
   if (fit == m) {
-    cursor->pos = n;
-    cursor->done = 1;
+    cursor->pos = pos + m;
+    if (cursor->pos >= n) cursor->done = 1;
   } else if (fit > 0) {
     cursor->pos = pos + fit;
   } else {
