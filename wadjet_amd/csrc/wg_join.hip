@@ -138,14 +138,54 @@ __global__ void __launch_bounds__(256) join_count_kernel(JoinTable J, ColDesc pk
   const uint64_t t0 = (uint64_t)blockIdx.x * kProbeTile;
   const uint64_t t1 = min(t0 + kProbeTile, n);
   uint64_t local = 0;
-  for (uint64_t j = t0 + threadIdx.x; j < t1; j += blockDim.x) {
-    const uint64_t i = pos0 + j;
-    const uint64_t row = sel ? sel[i] : i;
-    const uint64_t* s = probe_slot(J, pk, row);
-    const uint32_t c = s ? (uint32_t)slot_count(s) : 0u;
-    const int32_t h = s ? slot_head(s) : -1;
-    hc[j] = (uint64_t)c | ((uint64_t)(uint32_t)h << 32);
-    local += c;
+  constexpr int U = 4;  // positions whose home slots are read together
+  struct Pair {
+    uint64_t k, w;
+  };
+  for (uint64_t j0 = t0; j0 < t1; j0 += (uint64_t)blockDim.x * U) {
+    uint64_t key[U], idx[U];
+    bool live[U];
+    Pair v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t j = j0 + (uint64_t)u * blockDim.x + threadIdx.x;
+      live[u] = false;
+      key[u] = 0;
+      if (j < t1) {
+        const uint64_t i = pos0 + j;
+        const uint64_t row = sel ? sel[i] : i;
+        live[u] = col_valid(pk, row);  // NULL keys never match
+        if (live[u]) key[u] = (uint64_t)load_int(pk, row);
+      }
+      idx[u] = key[u] == kEmpty ? J.mask + 1 : (mix64(key[u]) & J.mask);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (live[u]) __builtin_memcpy(&v[u], J.slots + idx[u] * (uint64_t)J.stride, 16);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t j = j0 + (uint64_t)u * blockDim.x + threadIdx.x;
+      if (j >= t1) continue;
+      uint64_t w1 = 0;
+      bool hit = false;
+      if (live[u]) {
+        if (key[u] == kEmpty) {
+          hit = (v[u].w >> 32) != 0;  // the side slot of the sentinel key
+          w1 = v[u].w;
+        } else if (v[u].k == key[u]) {
+          hit = true;
+          w1 = v[u].w;
+        } else if (v[u].k != kEmpty) {
+          const uint64_t* s = join_find_from(J, key[u], (idx[u] + 1) & J.mask);  // past a collision
+          hit = s != nullptr;
+          if (hit) w1 = s[1];
+        }
+      }
+      const uint32_t c = hit ? (uint32_t)(w1 >> 32) : 0u;
+      const int32_t h = hit ? (int32_t)(uint32_t)w1 : -1;
+      hc[j] = (uint64_t)c | ((uint64_t)(uint32_t)h << 32);
+      local += c;
+    }
   }
   __shared__ uint64_t part[4];
   for (int o = 32; o; o >>= 1) {

@@ -40,4 +40,29 @@ __device__ __forceinline__ const uint64_t* join_find(const JoinTable& J, uint64_
   return join_find_from(J, key, mix64(key) & J.mask);
 }
 
+// The same probe, reading each slot's key and head/count words in one access
+// (one round trip per slot instead of two); *w1 = the matching slot's second
+// word. false when the key has no build row.
+__device__ __forceinline__ bool join_find_w1(const JoinTable& J, uint64_t key, uint64_t* w1) {
+  struct Pair {
+    uint64_t k, w;
+  } v;
+  if (key == kEmpty) {
+    __builtin_memcpy(&v, J.slots + (J.mask + 1) * (uint64_t)J.stride, 16);
+    *w1 = v.w;
+    return (v.w >> 32) != 0;
+  }
+  uint64_t idx = mix64(key) & J.mask;
+  for (uint64_t p = 0; p <= J.mask; ++p) {
+    __builtin_memcpy(&v, J.slots + idx * (uint64_t)J.stride, 16);
+    if (v.k == key) {
+      *w1 = v.w;
+      return true;
+    }
+    if (v.k == kEmpty) return false;
+    idx = (idx + 1) & J.mask;
+  }
+  return false;
+}
+
 }  // namespace wg
