@@ -23,32 +23,49 @@ struct Word16 {
   uint64_t lo, hi;
 };
 
+// Each wavefront takes kViewU groups of 64 rows per step and issues their
+// base reads together (the reads are random when the indices are).
+constexpr int kViewU = 4;
+
 template <typename W>
 __global__ void view_flatten_kernel(const W* base, const uint64_t* base_valid, uint64_t base_rows, const uint32_t* idx,
                                     const uint64_t* own_valid, uint64_t n, W* out, uint64_t* out_valid,
                                     unsigned long long* bad) {
   const int lane = __lane_id();
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t wave0 = (blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u)) * kViewU;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kViewU;
   uint32_t nbad = 0;
-  for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
-    const uint64_t i = i0 + lane;
-    bool valid = false;
-    if (i < n) {
-      W x{};
-      const bool own = !own_valid || ((own_valid[i >> 6] >> (i & 63)) & 1ull);
-      if (own) {
-        const uint32_t s = idx[i];
-        if (s >= base_rows) {
-          ++nbad;
-        } else if (!base_valid || ((base_valid[s >> 6] >> (s & 63)) & 1ull)) {
-          valid = true;
-          x = base[s];
-        }
+  for (uint64_t g0 = wave0; g0 < n; g0 += stride) {
+    uint32_t s[kViewU];
+    bool take[kViewU], valid[kViewU];
+#pragma unroll
+    for (int u = 0; u < kViewU; ++u) {
+      const uint64_t i = g0 + 64ull * u + lane;
+      take[u] = false;
+      valid[u] = false;
+      s[u] = 0;
+      if (i < n && (!own_valid || ((own_valid[i >> 6] >> (i & 63)) & 1ull))) {
+        s[u] = idx[i];
+        if (s[u] >= base_rows) ++nbad;
+        else take[u] = true;
       }
-      out[i] = x;
     }
-    const uint64_t m = __ballot(valid);
-    if (out_valid && lane == 0) out_valid[i0 >> 6] = m;
+#pragma unroll
+    for (int u = 0; u < kViewU; ++u)
+      if (take[u]) valid[u] = !base_valid || ((base_valid[s[u] >> 6] >> (s[u] & 63)) & 1ull);
+    W x[kViewU];
+#pragma unroll
+    for (int u = 0; u < kViewU; ++u) {
+      x[u] = W{};
+      if (valid[u]) x[u] = base[s[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kViewU; ++u) {
+      const uint64_t i = g0 + 64ull * u + lane;
+      if (i < n) out[i] = x[u];
+      const uint64_t m = __ballot(valid[u]);
+      if (out_valid && lane == 0 && g0 + 64ull * u < n) out_valid[(g0 >> 6) + u] = m;
+    }
   }
   if (nbad) atomicAdd(bad, (unsigned long long)nbad);
 }
@@ -75,7 +92,7 @@ wg_status wg_view_flatten(wg_ctx* ctx, const wg_column* base, uint64_t base_rows
   } release{ctx->stream, &bad};
   WG_HIP(hipMallocAsync((void**)&bad, 8, ctx->stream));
   WG_HIP(hipMemsetAsync(bad, 0, 8, ctx->stream));
-  const int grid = grid_for(n, 256, 16384);
+  const int grid = grid_for((n + kViewU - 1) / kViewU, 256, 16384);
   const uint64_t* bv = base->validity;
   switch (type_bytes(base->type)) {
     case 2:
