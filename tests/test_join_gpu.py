@@ -292,3 +292,39 @@ def test_bloom_prefilter_matches_oracle(ctx, n_build):
     n = j.bloom_filter(Col(ctx.to_device(pk32), L.INT32), npr, out)
     np.testing.assert_array_equal(out.to_host(np.uint32, n), O.bloom_filter(want_words, pk32.astype(np.int64)))
     j.close()
+
+
+@pytest.mark.parametrize("cap", [5000, 70_000])
+def test_bounded_probe_windows_over_a_large_batch(ctx, cap):
+    """A bounded probe of a 300K-row batch with a selection vector examines a
+    window of positions per call (twice the room, at least 64K): every call
+    returns at most `cap` pairs, calls resume across windows and chains, and
+    the pairs are the oracle's multiset."""
+    print(f"[join] windows cap={cap}", flush=True)
+    rng = np.random.default_rng(cap + 1)
+    nb, npr = 30_000, 300_000
+    bk = rng.integers(0, 40_000, nb).astype(np.int64)
+    bk[:2000] = 5  # a chain longer than the smaller bound's room
+    pk = rng.integers(0, 60_000, npr).astype(np.int64)
+    pk[::997] = 5
+    sel = np.sort(rng.choice(npr, 250_000, replace=False)).astype(np.uint32)
+    j = HashJoin(ctx, [L.INT64], 0, build_hint=nb)
+    j.build([Col(ctx.to_device(bk), L.INT64)], nb)
+    j.finish()
+    want_p, want_b = O.hash_join((bk, O.INT64), (pk, O.INT64), probe_sel=sel, cap=npr * 40)
+    tp, tb = ctx.alloc(cap * 4), ctx.alloc(cap * 4)
+    key = Col(ctx.to_device(pk), L.INT64)
+    dsel = (ctx.to_device(sel), len(sel))
+    cur, calls = None, 0
+    ps, bs = [], []
+    while True:
+        k, cur = j.probe(key, npr, tp, tb, cap, cur, sel=dsel)
+        assert k <= cap
+        calls += 1
+        ps.append(tp.to_host(np.uint32, k))
+        bs.append(tb.to_host(np.uint32, k))
+        if cur.done:
+            break
+        assert calls < 10_000
+    p, b = np.concatenate(ps), np.concatenate(bs)
+    np.testing.assert_array_equal(pairs_set(p, b), pairs_set(want_p, want_b))
