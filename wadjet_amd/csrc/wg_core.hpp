@@ -71,6 +71,10 @@ struct wg_ctx {
   uint64_t hbm = 0;
   // Small pinned scratch for counter read-back.
   uint64_t* pinned = nullptr;
+  // Pinned staging for per-call tables (the Parquet page table), grown on
+  // demand; every call that fills it synchronises before returning.
+  void* upload = nullptr;
+  uint64_t upload_cap = 0;
 };
 
 namespace wg {

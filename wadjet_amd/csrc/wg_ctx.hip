@@ -79,6 +79,7 @@ wg_status wg_ctx_destroy(wg_ctx* ctx) {
   (void)hipEventDestroy(ctx->t0);
   (void)hipEventDestroy(ctx->t1);
   (void)hipHostFree(ctx->pinned);
+  if (ctx->upload) (void)hipHostFree(ctx->upload);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return WG_OK;

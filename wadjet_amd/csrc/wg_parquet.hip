@@ -478,9 +478,19 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
     tile_off[g + 1] = tile_off[g] + tail.tile0 + (tail.p.num_rows + kPqTile - 1) / kPqTile;
   }
   const size_t table_bytes = table.size() * sizeof(PqPageDev);
-  std::vector<uint8_t> upload(table_bytes + tile_off.back() * 4);
-  memcpy(upload.data(), table.data(), table_bytes);
-  uint32_t* map = (uint32_t*)(upload.data() + table_bytes);
+  const size_t upload_bytes = table_bytes + tile_off.back() * 4;
+  const size_t need = (upload_bytes + 7) / 8 * 8 + 8;  // + the error count read back
+  if (ctx->upload_cap < need) {
+    if (ctx->upload) WG_HIP(hipHostFree(ctx->upload));
+    ctx->upload = nullptr;
+    ctx->upload_cap = 0;
+    WG_HIP(hipHostMalloc(&ctx->upload, need, hipHostMallocDefault));
+    ctx->upload_cap = need;
+  }
+  uint8_t* upload = (uint8_t*)ctx->upload;
+  uint64_t* nbad_host = (uint64_t*)(upload + (upload_bytes + 7) / 8 * 8);
+  memcpy(upload, table.data(), table_bytes);
+  uint32_t* map = (uint32_t*)(upload + table_bytes);
   for (size_t g = 0; g < groups.size(); ++g) {
     const size_t last = g + 1 < groups.size() ? groups[g + 1].first : table.size();
     for (size_t e = groups[g].first; e < last; ++e) {
@@ -507,8 +517,8 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
   size_t scan_bytes = 0;
   WG_HIP(hipMallocAsync((void**)&bad, 8, ctx->stream));
   WG_HIP(hipMemsetAsync(bad, 0, 8, ctx->stream));
-  WG_HIP(hipMallocAsync((void**)&dtable, upload.size(), ctx->stream));
-  WG_HIP(hipMemcpyAsync(dtable, upload.data(), upload.size(), hipMemcpyHostToDevice, ctx->stream));
+  WG_HIP(hipMallocAsync((void**)&dtable, upload_bytes, ctx->stream));
+  WG_HIP(hipMemcpyAsync(dtable, upload, upload_bytes, hipMemcpyHostToDevice, ctx->stream));
   const uint32_t* dmap = (const uint32_t*)((const uint8_t*)dtable + table_bytes);
   if (any_nullable) {
     WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
@@ -540,9 +550,9 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
     WG_HIP(hipGetLastError());
     r0 += groups[g].second;
   }
-  uint64_t nbad = 0;
-  WG_HIP(hipMemcpyAsync(&nbad, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
+  WG_HIP(hipMemcpyAsync(nbad_host, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
   WG_HIP(hipStreamSynchronize(ctx->stream));
+  const uint64_t nbad = *nbad_host;
   WG_CHECK(nbad == 0, WG_ERR_INVALID, "%llu values with their dictionary index missing or out of range, or past "
            "a stream's end", (unsigned long long)nbad);
   return WG_OK;
