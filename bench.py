@@ -222,11 +222,22 @@ def bench_join_output(ctx, args):
         ctx.sync()
         res[name] = (time.perf_counter() - t0) * 1000 / steps
     gathered = got * (8 + 4 + 8 + 4 + 8)
+    for anti in (False, True):  # semi / anti probe of the same rows (ordered selection vectors)
+        j.semi(pcols[0], cfg.n_probe, outp, anti=anti)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps)):
+            kept = j.semi(pcols[0], cfg.n_probe, outp, anti=anti)
+        ctx.sync()
+        res["anti" if anti else "semi"] = (time.perf_counter() - t0) * 1000 / max(1, args.steps)
+        res["anti_rows" if anti else "semi_rows"] = kept
     for b in [outp, outb] + [x for pair in outs for x in pair]:
         b.free()
     log(f"[join_output] {got} pairs: views {res['views']:.3f} ms, eager gathers {res['eager']:.3f} ms "
-        f"({gathered / 1e9:.2f} GB of gathered columns deferred)")
+        f"({gathered / 1e9:.2f} GB of gathered columns deferred); semi {res['semi']:.3f} ms "
+        f"({res['semi_rows']} rows), anti {res['anti']:.3f} ms ({res['anti_rows']} rows)")
     return dict(pairs=got, views_ms=res["views"], eager_ms=res["eager"], deferred_bytes=gathered,
+                semi_ms=res["semi"], anti_ms=res["anti"], semi_rows=res["semi_rows"], anti_rows=res["anti_rows"],
                 pairs_per_s_views=got / (res["views"] / 1000), pairs_per_s_eager=got / (res["eager"] / 1000))
 
 
