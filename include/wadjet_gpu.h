@@ -327,8 +327,10 @@ typedef struct wg_probe_cursor {
 
 /* Inner-join probe of one device-resident probe batch (inlineIntProbe,
  * join.go:2670-2814): emits at most out_cap (probe_row, build_row) pairs,
- * compacted per wavefront; resumes from *cursor and updates it. Rows with a
- * NULL key never match. Pair order is unspecified; the pair multiset is the
+ * compacted per wavefront; resumes from *cursor and updates it. A call
+ * examines a window of probe positions (twice its output room, at least 64K),
+ * so it may return fewer than out_cap pairs, even none, before the input is
+ * done: call again until cursor->done. Rows with a NULL key never match. Pair order is unspecified; the pair multiset is the
  * reference's. probe_key is a device column; out_* are device buffers;
  * *n_out is written on the host. */
 wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
