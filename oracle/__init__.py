@@ -15,6 +15,10 @@ Pinning (how each function is checked, see tests/test_oracle.py):
   or_filter                                          numpy boolean masks
   or_hash_join                                       brute-force nested loop, reference KATs
                                                      (join_test.go:10-56, join_coverage_test.go:101-141)
+  or_join_agg_parallel                               or_hash_join's pairs aggregated by or_hash_agg over
+                                                     the materialized joined rows (exact Python-int
+                                                     products), at 1 and 7 threads; the matched-row count
+                                                     of the fk generator's closed form
   or_semi_join                                       brute-force membership loop with NULLs, reference
                                                      KATs (join_coverage_test.go:272-351, :711-735)
   or_digest_rows                                     order invariance + a Python restatement
@@ -107,6 +111,8 @@ def lib():
         L.or_digest_rows.argtypes = [P, U64, I32, P]
         L.or_mix64.argtypes = [U64]
         L.or_mix64.restype = U64
+        L.or_join_agg_parallel.argtypes = [P, P, P, U64, P, P, U64, I32, P, P, P, P, I64]
+        L.or_join_agg_parallel.restype = I64
         _lib = L
     return _lib
 
@@ -272,6 +278,27 @@ def hash_join(build_key, probe_key, probe_sel=None, cap=None):
     return op[:n], ob[:n]
 
 
+def join_agg(build_key, build_group, build_dec, probe_key, probe_mul, threads: int = 1, max_groups: int = 1 << 16):
+    """JOIN probe_key = build_key, then GROUP BY build_group SUM(probe_mul *
+    build_dec) as exact Int128, COUNT(*). Columns: (array, type[, validity[,
+    scale]]). Returns dict(n, keys int64 ascending, count, sum_lo uint64,
+    sum_hi int64)."""
+    ba, k1 = make_cols([build_key, build_group, build_dec])
+    pa, k2 = make_cols([probe_key, probe_mul])
+    nb, npr = len(build_key[0]), len(probe_key[0])
+    keys = np.zeros(max_groups, np.int64)
+    cnt = np.zeros(max_groups, np.int64)
+    lo = np.zeros(max_groups, np.uint64)
+    hi = np.zeros(max_groups, np.int64)
+    sz = C.sizeof(OrCol)
+    base_b, base_p = C.addressof(ba), C.addressof(pa)
+    n = lib().or_join_agg_parallel(base_b, base_b + sz, base_b + 2 * sz, nb, base_p, base_p + sz, npr, threads,
+                                   _ptr(keys), _ptr(cnt), _ptr(lo), _ptr(hi), max_groups)
+    if n < 0:
+        raise RuntimeError("oracle join_agg: group limit exceeded")
+    return dict(n=n, keys=keys[:n], count=cnt[:n], sum_lo=lo[:n], sum_hi=hi[:n])
+
+
 def semi_join(build_key, probe_key, probe_sel=None, anti=False):
     """build_key/probe_key: (array, type[, validity]). Probe positions, in
     input order, whose key is (semi) / is not (anti) among the non-NULL build
@@ -381,7 +408,10 @@ def partial_run(res, specs, key_types, names=None, type_ids=None) -> bytes:
                 has = bool(x["has_min"] if func == MIN else x["has_max"])
                 out += bytes([1 if has else 0])
                 if has:
-                    if is_f:
+                    if is_d:  # writeInt128: Hi, then Lo (aggregate_partial_spill.go:322-345, :702-707)
+                        v = int(x["min_i64"] if func == MIN else x["max_i64"])
+                        out += struct.pack("<qQ", -1 if v < 0 else 0, v & ((1 << 64) - 1))
+                    elif is_f:
                         out += struct.pack("<d", float(x["min_f64"] if func == MIN else x["max_f64"]))
                     else:
                         out += struct.pack("<q", int(x["min_i64"] if func == MIN else x["max_i64"]))
@@ -447,7 +477,11 @@ def read_partial_run(buf: bytes) -> dict:
                 (has,) = take("<B")
                 a["has"] = bool(has)
                 if has:
-                    (a["value"],) = take("<d") if is_f else take("<q")
+                    if is_d:  # readInt128 (aggregate_partial_spill.go:400-440, :709-717)
+                        hi, lo = take("<qQ")
+                        a["value"] = (hi << 64) | lo
+                    else:
+                        (a["value"],) = take("<d") if is_f else take("<q")
             accs.append(a)
         groups.append((sk, kv, accs))
     assert p == len(buf), "trailing bytes after the end marker"

@@ -13,6 +13,7 @@
  *   internal/engine/exec/kernel/compare.go:47-178   typed filter kernels
  *   internal/engine/exec/join.go:551-560,2670-2836  arenaAppendInt (LIFO chains) and inlineIntProbe
  */
+#define _GNU_SOURCE /* qsort_r */
 #include "oracle.h"
 
 #include <math.h>
@@ -674,4 +675,185 @@ int64_t or_semi_join(const or_col* build_key, uint64_t n_build, const or_col* pr
   free(tkey);
   free(used);
   return nout;
+}
+
+/* ------------------------------------------------ join -> aggregate */
+
+/* Inner join build_key = probe_key feeding GROUP BY (an int build column)
+ * SUM(probe int column x build DECIMAL column), COUNT(*): or_hash_join's
+ * build and probe (HashJoin.Build, join.go:551-560; inlineIntProbe,
+ * :2670-2836) with every matched pair fed to scatterSumDecimal
+ * (agg_scatter.go:330-348) and scatterCountStar (:387-406). The product
+ * (aggPreProject, plan.go:7900-7990, at the decimal's scale) is formed exactly
+ * in 128 bits and added with Int128.Add (decimal.go:77-85). The probe rows
+ * split into `threads` contiguous ranges, each aggregated into its own
+ * partial groups, folded in range order with Accumulator.Merge's Int128 add
+ * (kernel/types.go:106-175). Groups come out ascending by key. Returns the
+ * group count, or -1 past max_groups / out of memory. */
+typedef struct ja_job {
+  const int64_t* tkey;
+  const int64_t* thead;
+  const int64_t* next;
+  const uint8_t* used;
+  uint64_t tcap;
+  const or_col *bgroup, *bdec, *pkey, *pmul;
+  uint64_t beg, end;
+  int64_t max_groups;
+  int64_t* keys; /* partial groups, unordered */
+  int64_t* cnt;
+  uint64_t* lo;
+  int64_t* hi;
+  int64_t n;     /* -1: over max_groups */
+} ja_job;
+
+static int64_t ja_slot(ja_job* j, int64_t* gslot, uint64_t gcap, int64_t g) {
+  uint64_t i = or_mix64((uint64_t)g) & (gcap - 1);
+  while (gslot[i] >= 0 && j->keys[gslot[i]] != g) i = (i + 1) & (gcap - 1);
+  if (gslot[i] < 0) {
+    if (j->n >= j->max_groups) return -1;
+    gslot[i] = j->n;
+    j->keys[j->n] = g;
+    j->cnt[j->n] = 0;
+    j->lo[j->n] = 0;
+    j->hi[j->n] = 0;
+    ++j->n;
+  }
+  return gslot[i];
+}
+
+static void* ja_run(void* arg) {
+  ja_job* j = (ja_job*)arg;
+  uint64_t gcap = 16;
+  while (gcap < (uint64_t)j->max_groups * 2) gcap <<= 1;
+  int64_t* gslot = (int64_t*)malloc(gcap * 8);
+  if (!gslot) { j->n = -1; return NULL; }
+  memset(gslot, 0xFF, gcap * 8);
+  j->n = 0;
+  for (uint64_t row = j->beg; row < j->end; ++row) {
+    if (!valid_at(j->pkey, row)) continue;
+    int64_t k = int_at(j->pkey, row);
+    uint64_t i = or_mix64((uint64_t)k) & (j->tcap - 1);
+    while (j->used[i] && j->tkey[i] != k) i = (i + 1) & (j->tcap - 1);
+    if (!j->used[i]) continue;
+    for (int64_t b = j->thead[i]; b >= 0; b = j->next[b]) {
+      int64_t gi = ja_slot(j, gslot, gcap, int_at(j->bgroup, (uint64_t)b));
+      if (gi < 0) { j->n = -1; free(gslot); return NULL; }
+      j->cnt[gi] += 1;
+      if (!valid_at(j->pmul, row) || !valid_at(j->bdec, (uint64_t)b)) continue;
+      __int128 p = (__int128)int_at(j->pmul, row) * (__int128)int_at(j->bdec, (uint64_t)b);
+      or_i128_add(j->lo[gi], j->hi[gi], (uint64_t)p, (int64_t)(p >> 64), &j->lo[gi], &j->hi[gi]);
+    }
+  }
+  free(gslot);
+  return NULL;
+}
+
+static int cmp_i64_idx(const void* a, const void* b, void* keys) {
+  int64_t x = ((const int64_t*)keys)[*(const int64_t*)a], y = ((const int64_t*)keys)[*(const int64_t*)b];
+  return x < y ? -1 : x > y;
+}
+
+int64_t or_join_agg_parallel(const or_col* build_key, const or_col* build_group, const or_col* build_dec,
+                             uint64_t n_build, const or_col* probe_key, const or_col* probe_mul, uint64_t n_probe,
+                             int32_t threads, int64_t* keys_out, int64_t* count_out, uint64_t* sum_lo_out,
+                             int64_t* sum_hi_out, int64_t max_groups) {
+  if (threads < 1) threads = 1;
+  uint64_t tcap = 16;
+  while (tcap < n_build * 2 + 2) tcap <<= 1;
+  int64_t* tkey = (int64_t*)malloc(tcap * 8);
+  int64_t* thead = (int64_t*)malloc(tcap * 8);
+  int64_t* next = (int64_t*)malloc((n_build ? n_build : 1) * 8);
+  uint8_t* used = (uint8_t*)calloc(tcap, 1);
+  ja_job* jobs = (ja_job*)calloc(threads, sizeof(ja_job));
+  pthread_t* th = (pthread_t*)calloc(threads, sizeof(pthread_t));
+  int64_t ng = -1;
+  if (!tkey || !thead || !next || !used || !jobs || !th) goto out;
+  for (uint64_t r = 0; r < n_build; ++r) {
+    if (!valid_at(build_key, r)) continue;
+    int64_t k = int_at(build_key, r);
+    uint64_t i = or_mix64((uint64_t)k) & (tcap - 1);
+    while (used[i] && tkey[i] != k) i = (i + 1) & (tcap - 1);
+    if (!used[i]) {
+      used[i] = 1;
+      tkey[i] = k;
+      next[r] = -1;
+    } else {
+      next[r] = thead[i];
+    }
+    thead[i] = (int64_t)r;
+  }
+  for (int t = 0; t < threads; ++t) {
+    ja_job* j = &jobs[t];
+    *j = (ja_job){tkey, thead, next, used, tcap, build_group, build_dec, probe_key, probe_mul,
+                  n_probe * t / threads, n_probe * (t + 1) / threads, max_groups, NULL, NULL, NULL, NULL, 0};
+    j->keys = (int64_t*)malloc((size_t)max_groups * 8);
+    j->cnt = (int64_t*)malloc((size_t)max_groups * 8);
+    j->lo = (uint64_t*)malloc((size_t)max_groups * 8);
+    j->hi = (int64_t*)malloc((size_t)max_groups * 8);
+    pthread_create(&th[t], NULL, ja_run, j);
+  }
+  for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+  {
+    /* fold the partials in range order */
+    ja_job all = {0};
+    all.max_groups = max_groups;
+    all.keys = keys_out;
+    all.cnt = count_out;
+    all.lo = sum_lo_out;
+    all.hi = sum_hi_out;
+    uint64_t gcap = 16;
+    while (gcap < (uint64_t)max_groups * 2) gcap <<= 1;
+    int64_t* gslot = (int64_t*)malloc(gcap * 8);
+    if (!gslot) goto out;
+    memset(gslot, 0xFF, gcap * 8);
+    int ok = 1;
+    for (int t = 0; t < threads && ok; ++t) {
+      ja_job* j = &jobs[t];
+      if (j->n < 0) { ok = 0; break; }
+      for (int64_t q = 0; q < j->n; ++q) {
+        int64_t gi = ja_slot(&all, gslot, gcap, j->keys[q]);
+        if (gi < 0) { ok = 0; break; }
+        count_out[gi] += j->cnt[q];
+        or_i128_add(sum_lo_out[gi], sum_hi_out[gi], j->lo[q], j->hi[q], &sum_lo_out[gi], &sum_hi_out[gi]);
+      }
+    }
+    free(gslot);
+    if (ok) {
+      /* ascending by key */
+      int64_t n = all.n;
+      int64_t* idx = (int64_t*)malloc((size_t)(n ? n : 1) * 8 * 5);
+      if (!idx) goto out;
+      int64_t *k2 = idx + n, *c2 = idx + 2 * n, *h2 = idx + 3 * n;
+      uint64_t* l2 = (uint64_t*)(idx + 4 * n);
+      for (int64_t i = 0; i < n; ++i) idx[i] = i;
+      qsort_r(idx, (size_t)n, 8, cmp_i64_idx, keys_out);
+      for (int64_t i = 0; i < n; ++i) {
+        k2[i] = keys_out[idx[i]];
+        c2[i] = count_out[idx[i]];
+        l2[i] = sum_lo_out[idx[i]];
+        h2[i] = sum_hi_out[idx[i]];
+      }
+      memcpy(keys_out, k2, (size_t)n * 8);
+      memcpy(count_out, c2, (size_t)n * 8);
+      memcpy(sum_lo_out, l2, (size_t)n * 8);
+      memcpy(sum_hi_out, h2, (size_t)n * 8);
+      free(idx);
+      ng = n;
+    }
+  }
+out:
+  if (jobs)
+    for (int t = 0; t < threads; ++t) {
+      free(jobs[t].keys);
+      free(jobs[t].cnt);
+      free(jobs[t].lo);
+      free(jobs[t].hi);
+    }
+  free(jobs);
+  free(th);
+  free(tkey);
+  free(thead);
+  free(next);
+  free(used);
+  return ng;
 }

@@ -91,6 +91,15 @@ int64_t or_hash_agg_parallel(const or_col* cols, int32_t n_cols, uint64_t n_rows
                              const int32_t* agg_func, const int32_t* agg_col, int32_t n_aggs, int32_t threads,
                              int64_t* keys_out, or_acc* acc_out, int64_t max_groups);
 
+/* Inner join on an int key, then GROUP BY an int build column
+ * SUM(probe int column x build DECIMAL column) exactly in Int128, COUNT(*),
+ * with the probe rows split over `threads` threads (partials merged in range
+ * order). Groups ascending; returns their count, -1 past max_groups. */
+int64_t or_join_agg_parallel(const or_col* build_key, const or_col* build_group, const or_col* build_dec,
+                             uint64_t n_build, const or_col* probe_key, const or_col* probe_mul, uint64_t n_probe,
+                             int32_t threads, int64_t* keys_out, int64_t* count_out, uint64_t* sum_lo_out,
+                             int64_t* sum_hi_out, int64_t max_groups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -239,6 +239,67 @@ static void test_hash_join_kats(wg_ctx* g) {
   }
 }
 
+// Probe clones on pipeline workers share one join handle (and, with a Bloom
+// filter in front, its lazily built filter): Workers = 4 feeding a mergeable
+// GpuHashAggregate must equal the serial run and the host-computed answer.
+static void test_parallel_probe_into_aggregate(wg_ctx* g) {
+  auto build = make_batch({{"k", TypeID::Int64}, {"p", TypeID::Int32}}, 30000);
+  for (int i = 0; i < 30000; ++i) {
+    build->columns[0]->i64[i] = (i % 20000) * 13;  // keys below 10000 * 13 have two build rows (chains)
+    build->columns[1]->i32[i] = i % 37;
+  }
+  std::vector<BatchPtr> probes;
+  std::map<int64_t, std::pair<int64_t, int64_t>> want;  // p -> (sum q, count)
+  for (int m = 0; m < 48; ++m) {
+    auto x = make_batch({{"fk", TypeID::Int64}, {"q", TypeID::Int64}}, 2048);
+    for (int i = 0; i < 2048; ++i) {
+      const int64_t r = m * 2048 + i;
+      const int64_t key = ((r * 7919) % 26000) * 13;  // about 23 % miss the build
+      x->columns[0]->i64[i] = key;
+      x->columns[1]->i64[i] = r % 101;
+      for (int b = 0; b < 30000; b += 20000)
+        if (key / 13 < 20000 && (key / 13) + b < 30000) {
+          auto& w = want[((key / 13) + b) % 37];
+          w.first += r % 101;
+          w.second += 1;
+        }
+    }
+    probes.push_back(x);
+  }
+  for (int bloom = 0; bloom < 2; ++bloom)
+    for (int workers : {0, 4}) {
+      Context ctx;
+      GpuHashJoin hj(g, "fk", "k");
+      BatchSource bs({build});
+      OK(hj.Build(ctx, &bs));
+      auto pr = hj.Probe();
+      GpuBloomFilterOp bf(&hj);
+      BatchSource ps(probes);
+      GpuHashAggregate agg(g, {"p"}, {{WG_AGG_SUM, "q", "s"}, {WG_AGG_COUNT, "", "c"}});
+      std::vector<UnaryOperator*> ops;
+      if (bloom) ops.push_back(&bf);
+      ops.push_back(pr.get());
+      Pipeline p{&ps, ops, &agg, workers};
+      OK(p.Run(ctx));
+      int groups = 0;
+      for (;;) {
+        BatchPtr out;
+        OK(agg.Next(ctx, &out));
+        if (!out) break;
+        for (int i = 0; i < out->len; ++i, ++groups) {
+          const int64_t k = out->columns[0]->i32[i];
+          EXPECT(want.count(k), "unexpected group %lld", (long long)k);
+          EXPECT((int64_t)out->columns[1]->f64[i] == want[k].first, "sum of %lld (workers %d, bloom %d)",
+                 (long long)k, workers, bloom);
+          EXPECT(out->columns[2]->i64[i] == want[k].second, "count of %lld (workers %d, bloom %d)", (long long)k,
+                 workers, bloom);
+        }
+      }
+      EXPECT(groups == (int)want.size(), "groups %d want %zu (workers %d, bloom %d)", groups, want.size(), workers,
+             bloom);
+    }
+}
+
 // join_coverage_test.go:272-351 (TestSemiJoin / TestAntiJoin) and :711-735
 // (TestAntiJoinNoMatches), the string ids coded as ints; plus a NULL probe key.
 static void test_semi_anti_kats(wg_ctx* g) {
@@ -800,6 +861,7 @@ int main(int argc, char** argv) {
     test_hash_aggregate_kat(g);
     test_parallel_pushdown_equivalence(g);
     test_hash_join_kats(g);
+    test_parallel_probe_into_aggregate(g);
     test_semi_anti_kats(g);
     test_bloom_filter_op(g);
     test_dual_key_join_kats(g);

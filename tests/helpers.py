@@ -82,3 +82,25 @@ def compare(gpu: dict, ref: dict, aggs, types, scales=None, abs_sums=None):
         else:
             np.testing.assert_array_equal(st["sum_lo"][m].view(np.int64), acc["sum_i64"][m], err_msg=f"agg {a}: int sum")
             np.testing.assert_array_equal(got[m].view(np.uint64), want[m].view(np.uint64), err_msg=f"agg {a}: int finalization")
+
+
+def splitmix_at(seed: int, rows: np.ndarray) -> np.ndarray:
+    """SplitMix64 output at positions `rows` (uint64 arithmetic mod 2^64): the
+    stream every seeded device generator draws from (wadjet_gen.h)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (rows.astype(np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def fk_match_count(n: int, seed: int, permille: int, chunk: int = 1 << 24) -> int:
+    """Probe rows of the foreign-key generator that draw an existing build key
+    (closed form): row r matches iff splitmix(seed ^ 0x3C..3C, r) % 1000 <
+    permille; every other row gets a key outside the build's domain."""
+    s = (seed ^ 0x3C3C3C3C3C3C3C3C) & ((1 << 64) - 1)
+    total = 0
+    for lo in range(0, n, chunk):
+        r = splitmix_at(s, np.arange(lo, min(n, lo + chunk), dtype=np.uint64))
+        total += int(np.count_nonzero(r % np.uint64(1000) < np.uint64(permille)))
+    return total

@@ -293,6 +293,39 @@ def test_join_brute_force_with_duplicates_and_nulls():
         assert (np.diff(bs.astype(np.int64)) < 0).all()
 
 
+@pytest.mark.parametrize("threads", [1, 7])
+def test_join_agg_equals_aggregated_pairs(threads):
+    """or_join_agg_parallel == or_hash_join's pairs, each joined row's
+    qty * p2 formed as an exact Python int and summed per p1 group; duplicate
+    build keys (chains) and NULLs in every input column. A dropped carry or a
+    truncated 64-bit product would change the Int128 sums."""
+    from wadjet_amd.gpu import bitmap_from_mask
+    rng = np.random.default_rng(20 + threads)
+    nb, npr = 3000, 40_000
+    bk = rng.integers(0, 2500, nb).astype(np.int64)
+    g = rng.integers(-3, 9, nb).astype(np.int32)
+    d = rng.integers(-2**62, 2**62, nb).astype(np.int64)
+    pk = rng.integers(0, 3000, npr).astype(np.int64)
+    q = rng.integers(-2**31, 2**31 - 1, npr).astype(np.int32)
+    bm, pm, dm, qm = (rng.random(x) > 0.03 for x in (nb, npr, nb, npr))
+    r = O.join_agg((bk, O.INT64, bitmap_from_mask(bm)), (g, O.INT32), (d, O.DECIMAL, bitmap_from_mask(dm), 2),
+                   (pk, O.INT64, bitmap_from_mask(pm)), (q, O.INT32, bitmap_from_mask(qm)), threads=threads)
+    p, b = O.hash_join((bk, O.INT64, bitmap_from_mask(bm)), (pk, O.INT64, bitmap_from_mask(pm)), cap=npr * 20)
+    want = {}
+    for i, j in zip(p.tolist(), b.tolist()):
+        c, s = want.get(int(g[j]), (0, 0))
+        if qm[i] and dm[j]:
+            s += int(q[i]) * int(d[j])
+        want[int(g[j])] = (c + 1, s)
+    keys = sorted(want)
+    assert r["n"] == len(keys) and r["keys"].tolist() == keys
+    for x, k in enumerate(keys):
+        c, s = want[k]
+        assert r["count"][x] == c
+        assert O.i128_join(int(r["sum_lo"][x]), int(r["sum_hi"][x])) == s % (1 << 128) - ((s % (1 << 128)) >> 127 << 128)
+    assert sum(r["count"]) == len(p)
+
+
 def test_join_kats():
     # join_test.go:10-56: probe alice, bob, alice, unknown vs build alice, bob,
     # carol (strings coded 1, 2, 1, 9 / 1, 2, 3) -> 3 rows

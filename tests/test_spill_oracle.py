@@ -95,3 +95,57 @@ def test_oracle_aggregate_round_trips_through_the_run():
         want[(int(a[i]), int(b[i]))] = want.get((int(a[i]), int(b[i])), 0) + big[i]
     for k, accs in got.items():
         assert accs[4]["sum"] == want[k]
+
+
+def le(n, v, signed=False):
+    return int(v).to_bytes(n, "little", signed=signed)
+
+
+def test_run_bytes_known_answer():
+    """The run of two groups of GROUP BY k int64 with SUM(v int64), COUNT(*),
+    MIN(f float64), SUM(d DECIMAL(10,2)), MAX(d) assembled byte by byte from
+    the reference writer, each field at its place:
+      writeHeader  aggregate_partial_spill.go:171-220 (magic, numGroupCols,
+                   nameLen/name/typeID, numAggs, func/isFloat/isDecimal/decScale,
+                   numGroups)
+      Write        :226-257 (rowMarker, sortKeyLen, sortKey, tagged key values
+                   via writeKeyValue :465-536: tag 3 + int64 LE)
+      emitAcc      :302-351 (SUM/AVG: Count then SumI64 / SumF64 / SumDec as
+                   Hi, Lo; COUNT: Count; MIN/MAX: HasMin byte, then the value;
+                   a present decimal MIN/MAX is an Int128 Hi, Lo :702-707)
+      Close        :259-290 (endMarker 0x00, numGroups patched)
+    A swapped Count / Sum order, a missing has byte or an 8-byte decimal
+    MAX changes the bytes."""
+    specs = [(O.SUM, False, False, 0), (O.COUNT, False, False, 0), (O.MIN, True, False, 0),
+             (O.SUM, False, True, 2), (O.MAX, False, True, 0)]
+    rows = [[{"count": 3, "sum_i64": -7}, {"count": 3}, {"has_min": 1, "min_f64": -1.5},
+             {"count": 3, "sum_lo": 2**64 - 5, "sum_hi": 1}, {"has_max": 1, "max_i64": -12}],
+            [{"count": 1, "sum_i64": 40}, {"count": 1}, {"has_min": 0},
+             {"count": 0, "sum_lo": 0, "sum_hi": 0}, {"has_max": 0}]]
+    res = dict(n=2, keys=np.array([[10], [9]], np.int64), key_null=np.zeros((2, 1), bool), acc=acc_rows(rows))
+    want = bytearray(b"WAGS\x01")
+    want += le(4, 1)                                            # numGroupCols
+    want += le(2, 1) + b"k" + le(2, O.PARQUET_INT64)            # nameLen, name, typeID
+    want += le(4, 5)                                            # numAggs
+    for func, is_f, is_d, sc in specs:                          # func u32, isFloat, isDecimal, decScale i32
+        want += le(4, func) + bytes([is_f, is_d]) + le(4, sc, True)
+    want += le(4, 2)                                            # numGroups
+    # sort keys "10" < "9" (bytes.Compare of the decimal text)
+    want += b"\x01" + le(4, 2) + b"10" + b"\x03" + le(8, 10, True)
+    want += le(8, 3, True) + le(8, -7, True)                   # SUM: Count, SumI64
+    want += le(8, 3, True)                                      # COUNT
+    want += b"\x01" + struct.pack("<d", -1.5)                   # MIN: HasMin, MinF64
+    want += le(8, 3, True) + le(8, 1, True) + le(8, 2**64 - 5)  # SUM dec: Count, Hi, Lo
+    want += b"\x01" + le(8, -1, True) + le(8, 2**64 - 12)       # MAX dec: HasMax, Hi, Lo of -12
+    want += b"\x01" + le(4, 1) + b"9" + b"\x03" + le(8, 9, True)
+    want += le(8, 1, True) + le(8, 40, True)
+    want += le(8, 1, True)
+    want += b"\x00"                                             # HasMin false: no value
+    want += le(8, 0, True) + le(8, 0, True) + le(8, 0)
+    want += b"\x00"                                             # HasMax false
+    want += b"\x00"                                             # endMarker
+    run = O.partial_run(res, specs, [O.INT64], names=["k"], type_ids=[O.PARQUET_INT64])
+    assert run == bytes(want)
+    r = O.read_partial_run(bytes(want))
+    assert r["groups"][0][2][4] == {"has": True, "value": -12}
+    assert r["groups"][1][2][2] == {"has": False}

@@ -799,9 +799,9 @@ wg_status normalize(wg_agg* h) {
 }
 
 wg_status agg_read_counters(wg_agg* h, uint64_t* c) {
-  WG_HIP(hipMemcpyAsync(h->ctx->pinned, h->counters, CTR_N * 8, hipMemcpyDeviceToHost, h->ctx->stream));
+  WG_HIP(hipMemcpyAsync(h->pinned, h->counters, CTR_N * 8, hipMemcpyDeviceToHost, h->ctx->stream));
   WG_HIP(hipStreamSynchronize(h->ctx->stream));
-  memcpy(c, h->ctx->pinned, CTR_N * 8);
+  memcpy(c, h->pinned, CTR_N * 8);
   return WG_OK;
 }
 
@@ -1211,7 +1211,12 @@ wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types, c
     tmp.insert(tmp.end(), h->sumd_limbs.begin(), h->sumd_limbs.end());
     WG_HIP(hipMemcpy(h->d_sumd_word, tmp.data(), h->n_sumd * 8, hipMemcpyHostToDevice));
   }
-  WG_HIP(hipMalloc(&h->counters, CTR_N * 8));
+  if (hipMalloc(&h->counters, CTR_N * 8) != hipSuccess ||
+      hipHostMalloc((void**)&h->pinned, kAggPinnedWords * 8, hipHostMallocDefault) != hipSuccess) {
+    set_error("device or pinned allocation of the counters failed");
+    wg_agg_destroy(h);
+    return WG_ERR_OOM;
+  }
   WG_TRY(zero_dev(ctx, h->counters, CTR_N));
   const uint64_t hint = std::max<uint64_t>(group_hint, 1024);
   h->group_hint = group_hint;
@@ -1247,6 +1252,7 @@ wg_status wg_agg_destroy(wg_agg* h) {
   (void)hipFree(h->part_items);
   (void)hipFree(h->hot_buf);
   if (h->part_host) (void)hipHostFree(h->part_host);
+  if (h->pinned) (void)hipHostFree(h->pinned);
   for (void* p : h->stage.pinned) (void)hipHostFree(p);
   for (uint64_t* p : h->stage.pvalid) (void)hipHostFree(p);
   for (void* p : h->stage.dev) (void)hipFree(p);

@@ -331,6 +331,8 @@ enum Ctr : int {
   CTR_FLUSHED = 7,
   CTR_N = 8
 };
+// pinned words per aggregate handle: the CTR_N counters, then two work-item cursors
+constexpr int kAggPinnedWords = CTR_N + 8;
 
 // Linear probing stops after this many slots; a table that needs more is
 // past its load budget and the host grows it (the claim reports overflow).

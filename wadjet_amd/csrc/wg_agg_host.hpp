@@ -61,6 +61,9 @@ struct wg_agg {
   uint64_t cap = 0;
   uint64_t groups = 0;  // as of the last read-back
   uint64_t* counters = nullptr;  // device CTR_N words
+  // pinned read-back words of this handle (counters, work-item cursors): not
+  // the context's, so handles on one context may run on different threads
+  uint64_t* pinned = nullptr;
   uint32_t* ovf_rows = nullptr;
   uint64_t ovf_rows_cap = 0;
   uint64_t* ovf_part = nullptr;

@@ -988,7 +988,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     if (!replay && items_done >= n_items) break;
     WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
     const uint64_t start_item = replay ? 0 : items_done;
-    uint64_t* pin = ctx->pinned + 8;  // [8] start item, [9] items taken (counters use [0, 8))
+    uint64_t* pin = h->pinned + CTR_N;  // [0] start item, [1] items taken (the counters use [0, CTR_N))
     pin[0] = start_item;
     WG_HIP(hipMemcpyAsync(item_ctr, pin, 8, hipMemcpyHostToDevice, ctx->stream));
     Q.T = table_of(h);

@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/wadjet_gpu.h"
@@ -69,12 +70,12 @@ struct wg_ctx {
   int cus = 0;
   int lds_per_block = 0;
   uint64_t hbm = 0;
-  // Small pinned scratch for counter read-back.
-  uint64_t* pinned = nullptr;
   // Pinned staging for per-call tables (the Parquet page table), grown on
-  // demand; every call that fills it synchronises before returning.
+  // demand; a call holds upload_mu from filling it until it has synchronised,
+  // so calls on one context from several threads do not share it.
   void* upload = nullptr;
   uint64_t upload_cap = 0;
+  std::mutex upload_mu;
 };
 
 namespace wg {

@@ -68,7 +68,6 @@ wg_status wg_ctx_create(int device, wg_ctx** out) {
   WG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   WG_HIP(hipEventCreate(&c->t0));
   WG_HIP(hipEventCreate(&c->t1));
-  WG_HIP(hipHostMalloc((void**)&c->pinned, 4096, hipHostMallocDefault));
   *out = c;
   return WG_OK;
 }
@@ -78,7 +77,6 @@ wg_status wg_ctx_destroy(wg_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipEventDestroy(ctx->t0);
   (void)hipEventDestroy(ctx->t1);
-  (void)hipHostFree(ctx->pinned);
   if (ctx->upload) (void)hipHostFree(ctx->upload);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

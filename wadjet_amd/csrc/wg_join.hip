@@ -211,7 +211,8 @@ __global__ void join_cut_kernel(const uint64_t* off, const uint64_t* cnt, uint64
 __global__ void __launch_bounds__(256) join_emit_kernel(JoinTable J, const uint32_t* sel, uint64_t pos0, uint64_t n,
                                                         const uint64_t* hc, const uint64_t* tile_off, uint64_t room,
                                                         uint32_t* out_p, uint32_t* out_b, uint64_t* cut) {
-  using BlockScan = hipcub::BlockScan<uint32_t, 256>;
+  // 64-bit scan: 256 chains of up to 2^32 - 1 build rows each can pass 2^32
+  using BlockScan = hipcub::BlockScan<uint64_t, 256>;
   __shared__ typename BlockScan::TempStorage tmp;
   const uint64_t t0 = (uint64_t)blockIdx.x * kProbeTile;
   const uint64_t t1 = min(t0 + kProbeTile, n);
@@ -219,14 +220,14 @@ __global__ void __launch_bounds__(256) join_emit_kernel(JoinTable J, const uint3
   for (uint64_t j0 = t0; j0 < t1; j0 += blockDim.x) {
     const uint64_t j = j0 + threadIdx.x;
     const uint64_t v = j < t1 ? hc[j] : 0;
-    const uint32_t c = (uint32_t)v;
-    uint32_t pre = 0, total = 0;
+    const uint64_t c = (uint32_t)v;
+    uint64_t pre = 0, total = 0;
     BlockScan(tmp).ExclusiveSum(c, pre, total);
     const uint64_t o = base + pre;
     if (c && o + c <= room) {
       const uint64_t row = sel ? sel[pos0 + j] : pos0 + j;
       int32_t b = (int32_t)(uint32_t)(v >> 32);
-      for (uint32_t k = 0; k < c && b >= 0; ++k) {
+      for (uint64_t k = 0; k < c && b >= 0; ++k) {
         out_p[o + k] = (uint32_t)row;
         out_b[o + k] = (uint32_t)b;
         if (k + 1 < c) b = J.next[b];  // a chain's last link is not read
@@ -1118,7 +1119,8 @@ wg_status wg_join_probe(wg_join* j, const wg_column* probe_key, uint64_t n_rows,
     uint64_t c2[2] = {0, 0};
     WG_HIP(hipMemcpyAsync(c2, cut_d + 1, 16, hipMemcpyDeviceToHost, j->ctx->stream));
     WG_HIP(hipStreamSynchronize(j->ctx->stream));
-    WG_CHECK(c2[0] < m, WG_ERR_DEVICE, "probe output cut not found");
+    WG_CHECK(c2[0] < m && c2[0] / kProbeTile == cut_t, WG_ERR_DEVICE, "probe output cut not found in tile %llu",
+             (unsigned long long)cut_t);
     fit = c2[0];
     total = c2[1];
   }

@@ -480,6 +480,7 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
   const size_t table_bytes = table.size() * sizeof(PqPageDev);
   const size_t upload_bytes = table_bytes + tile_off.back() * 4;
   const size_t need = (upload_bytes + 7) / 8 * 8 + 8;  // + the error count read back
+  std::lock_guard<std::mutex> upload_lock(ctx->upload_mu);  // held to the synchronising return
   if (ctx->upload_cap < need) {
     if (ctx->upload) WG_HIP(hipHostFree(ctx->upload));
     ctx->upload = nullptr;

@@ -82,6 +82,13 @@ wg_status select_rows(wg_ctx* ctx, const Pred& pred, uint64_t n, const uint32_t*
   const uint64_t tiles = (n + kSelTile - 1) / kSelTile;
   const uint64_t words = (n + 63) / 64;
   uint32_t* counts = nullptr;  // tile counts, tile offsets, then the ballot words
+  struct Release {  // the scratch goes back to the stream's pool on every exit
+    hipStream_t st;
+    uint32_t** p;
+    ~Release() {
+      if (*p) (void)hipFreeAsync(*p, st);
+    }
+  } release{ctx->stream, &counts};
   WG_HIP(hipMallocAsync((void**)&counts, tiles * 8 + words * 8 + 8, ctx->stream));
   uint32_t* offsets = counts + tiles;
   uint64_t* masks = (uint64_t*)(((uintptr_t)(offsets + tiles) + 7) & ~(uintptr_t)7);
@@ -102,6 +109,7 @@ wg_status select_rows(wg_ctx* ctx, const Pred& pred, uint64_t n, const uint32_t*
                      masks, (uint32_t*)nullptr, (const uint32_t*)offsets, out_sel);
   WG_HIP(hipGetLastError());
   WG_HIP(hipFreeAsync(counts, ctx->stream));
+  counts = nullptr;
   WG_HIP(hipStreamSynchronize(ctx->stream));
   *n_out = acc;
   return WG_OK;

@@ -315,7 +315,12 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
         }
       } else {
         const uint64_t has = rd(1);
-        if (has) {
+        if (has && sp.is_decimal) {
+          // an Int128 MinDec / MaxDec (Hi, Lo: readAcc, aggregate_partial_spill.go:400-440); the GPU
+          // path emits MIN/MAX(DECIMAL) as NULL like the flat accumulators, so it is read and dropped
+          (void)rd(8);
+          (void)rd(8);
+        } else if (has) {
           const uint64_t v = rd(8);
           if (f.val_kind == OP_MIN || f.val_kind == OP_MAX) s[f.val_word] = f.is_float_in ? enc_f64_bits(v) : enc_i64((int64_t)v);
         }

@@ -693,7 +693,10 @@ Status GpuHashJoinProbe::Execute(Context& ctx, const BatchPtr& in, BatchPtr* out
       LateMat.flattens.fetch_add(1);
     }
   if (!LateMaterialize) FlattenForConsumer(in.get(), nullptr);  // the eager gather reads typed storage
-  TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
+  {
+    std::lock_guard<std::mutex> lock(join_->mu_);
+    TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
+  }
   TRY(ensure_out(kMaxProbeOutputRows));
   cur_ = in;
   cursor_ = wg_probe_cursor{0, -1, 0, 0};
@@ -778,12 +781,14 @@ Status GpuHashJoinProbe::semi_anti(const BatchPtr& in, BatchPtr* out) {
   }
   std::vector<int> ki;
   TRY(join_->probe_key_idx(*in, &ki));
-  TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
   TRY(ensure_out(std::max<uint64_t>((uint64_t)in->len, kMaxProbeOutputRows)));
   uint64_t got = 0;
+  std::unique_lock<std::mutex> lock(join_->mu_);
+  TRY(dkey_.Prepare(join_->ctx_, join_->j_, ki, *in, &key_, &dsel_));
   TRY(wrap(wg_join_semi(join_->j_, &key_, (uint64_t)in->len, dsel_, in->has_sel ? in->sel.size() : 0, anti ? 1 : 0,
                         d_out_p_, &got),
            "wg_join_semi"));
+  lock.unlock();
   if (got == 0) return Status::OK();
   auto o = std::make_shared<RecordBatch>(*in);  // columns shared, Sel replaced
   o->sel.resize(got);
@@ -799,9 +804,12 @@ Status GpuHashJoinProbe::emit(BatchPtr* out) {
   *out = nullptr;
   const RecordBatch& in = *cur_;
   uint64_t got = 0;
-  TRY(wrap(wg_join_probe(join_->j_, &key_, (uint64_t)in.len, dsel_, in.has_sel ? in.sel.size() : 0, d_out_p_,
-                         d_out_b_, kMaxProbeOutputRows, &cursor_, &got),
-           "wg_join_probe"));
+  {
+    std::lock_guard<std::mutex> lock(join_->mu_);
+    TRY(wrap(wg_join_probe(join_->j_, &key_, (uint64_t)in.len, dsel_, in.has_sel ? in.sel.size() : 0, d_out_p_,
+                           d_out_b_, kMaxProbeOutputRows, &cursor_, &got),
+             "wg_join_probe"));
+  }
   if (in.has_sel && in.sel.empty()) cursor_.done = 1;
   if (got == 0) return Status::OK();
   std::vector<uint32_t> pr(got), br(got);
@@ -849,6 +857,7 @@ Status GpuBloomFilterOp::Execute(Context&, const BatchPtr& in, BatchPtr* out) {
   *out = in;
   const uint64_t active = in ? (in->has_sel ? in->sel.size() : (uint64_t)in->len) : 0;
   if (active == 0 || disabled_ || !join_->j_) return Status::OK();
+  std::unique_lock<std::mutex> lock(join_->mu_);
   if (!join_->bloom_built_) {
     TRY(wrap(wg_join_bloom_build(join_->j_), "wg_join_bloom_build"));
     join_->bloom_built_ = true;
@@ -869,6 +878,7 @@ Status GpuBloomFilterOp::Execute(Context&, const BatchPtr& in, BatchPtr* out) {
   TRY(wrap(wg_join_bloom_filter(join_->j_, &key, (uint64_t)in->len, d_sel, in->has_sel ? in->sel.size() : 0, d_out_,
                                 &got),
            "wg_join_bloom_filter"));
+  lock.unlock();
   if (got == 0) {
     *out = nullptr;  // every row rejected (:183-185); not counted toward the bypass
     return Status::OK();

@@ -349,6 +349,10 @@ class GpuHashJoin {
   std::vector<std::shared_ptr<batch::Vector>> build_cols_;  // retained build side (host copy for gathers)
   uint64_t rows_ = 0;
   bool bloom_built_ = false;
+  // Probe and Bloom clones run on pipeline worker threads over this one
+  // handle, whose probe scratch and lazily built filter are per handle (the
+  // C-ABI serialises nothing): every wg_join_* call they make holds this.
+  std::mutex mu_;
   friend class GpuBloomFilterOp;
 };
 
