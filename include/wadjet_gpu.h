@@ -271,6 +271,18 @@ wg_status wg_filter(wg_ctx* ctx, const wg_column* col, uint64_t n_rows, const ui
                     uint64_t n_sel, int32_t op, int64_t i64, double f64, uint32_t* out_sel,
                     uint64_t* n_out);
 
+/* wg_filter without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results and errors as wg_filter, for at most 2^26 (67 108 864) positions
+ * per call (WG_ERR_INVALID beyond). Its scratch (about 8.1 MiB of tile counts,
+ * offsets and ballot words) is one static device array per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap
+ * on one GPU (several threads or streams) race. */
+wg_status wg_filter_noalloc(wg_ctx* ctx, const wg_column* col, uint64_t n_rows, const uint32_t* sel,
+                            uint64_t n_sel, int32_t op, int64_t i64, double f64, uint32_t* out_sel,
+                            uint64_t* n_out);
+
 /* -------------------------------------------------------------------- join */
 
 typedef struct wg_join wg_join;

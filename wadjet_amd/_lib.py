@@ -109,6 +109,7 @@ SIGNATURES: dict[str, list] = {
     "wg_agg_state_bytes": [P, C.POINTER(U64)],
     "wg_agg_merge_run": [P, P, U64],
     "wg_filter": [P, C.POINTER(Column), U64, P, U64, I32, I64, C.c_double, P, C.POINTER(U64)],
+    "wg_filter_noalloc": [P, C.POINTER(Column), U64, P, U64, I32, I64, C.c_double, P, C.POINTER(U64)],
     "wg_join_create": [P, I32, C.POINTER(I32), C.POINTER(I32), I32, I32, C.POINTER(I32), U64, C.POINTER(P)],
     "wg_join_destroy": [P],
     "wg_join_create_dual": [P, I32, C.POINTER(I32), C.POINTER(I32), I32, I32, I32, C.POINTER(I32), U64, C.POINTER(P)],

@@ -76,6 +76,8 @@ struct wg_ctx {
   void* upload = nullptr;
   uint64_t upload_cap = 0;
   std::mutex upload_mu;
+  // device address of this device's no-allocation filter scratch
+  uint64_t* noalloc_scratch = nullptr;
 };
 
 namespace wg {

@@ -36,6 +36,44 @@ struct FilterPred {
   __device__ bool operator()(uint64_t row) const { return pred_pass(col, row, op, ci, cf); }
 };
 
+// ------------------------------------------- the filter without allocation
+
+// Scratch of wg_filter_noalloc, one copy per device for the life of the
+// process: per kSelTile-row tile a count and an offset (u32), one ballot word
+// per 64 rows, and the selected-row total. Host code never names this array
+// in a device context: it uses the address hipGetSymbolAddress returns, taken
+// once per context at wg_ctx_create with the context's device current.
+constexpr uint64_t kNoAllocMaxRows = 1ull << 26;
+constexpr uint64_t kNoAllocTiles = kNoAllocMaxRows / kSelTile;
+constexpr uint64_t kNoAllocWords = kNoAllocTiles + kNoAllocMaxRows / 64 + 1;
+__device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+// Exclusive scan of the tile counts in one workgroup (a tile's offset and the
+// total stay on the device): each thread sums a run of consecutive tiles, a
+// scan over the 1024 run sums gives each run's start.
+__global__ void __launch_bounds__(1024) tile_scan_kernel(const uint32_t* counts, uint32_t* offsets, uint64_t tiles,
+                                                         uint64_t* total) {
+  __shared__ uint64_t part[1024];
+  const uint64_t per = (tiles + 1023) / 1024;
+  const uint64_t b = min((uint64_t)threadIdx.x * per, tiles), e = min(b + per, tiles);
+  uint64_t s = 0;
+  for (uint64_t i = b; i < e; ++i) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const uint64_t v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = part[threadIdx.x] - s;
+  for (uint64_t i = b; i < e; ++i) {
+    offsets[i] = (uint32_t)run;
+    run += counts[i];
+  }
+  if (threadIdx.x == 1023) *total = part[1023];
+}
+
 // Grid-stride streaming copy of 16-byte words.
 __global__ void __launch_bounds__(256) copy_words_kernel(ulonglong2* dst, const ulonglong2* src, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -68,6 +106,8 @@ wg_status wg_ctx_create(int device, wg_ctx** out) {
   WG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   WG_HIP(hipEventCreate(&c->t0));
   WG_HIP(hipEventCreate(&c->t1));
+  // this device's copy of the no-allocation filter's scratch
+  WG_HIP(hipGetSymbolAddress((void**)&c->noalloc_scratch, HIP_SYMBOL(scratch)));
   *out = c;
   return WG_OK;
 }
@@ -179,6 +219,44 @@ wg_status wg_filter(wg_ctx* ctx, const wg_column* col, uint64_t n_rows, const ui
   const uint64_t n = sel ? n_sel : n_rows;
   return select_rows(ctx, FilterPred{ColDesc{col->data, col->validity, col->type, col->scale}, op, i64, f64}, n, sel,
                      out_sel, n_out);
+}
+
+wg_status wg_filter_noalloc(wg_ctx* ctx, const wg_column* col, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
+                            int32_t op, int64_t i64, double f64, uint32_t* out_sel, uint64_t* n_out) {
+  clear_error();
+  WG_CHECK(ctx && col && out_sel && n_out, WG_ERR_INVALID, "null argument");
+  WG_CHECK(op >= WG_EQ && op <= WG_GE, WG_ERR_INVALID, "bad compare op %d", op);
+  WG_CHECK(col->type == WG_INT32 || col->type == WG_INT64 || col->type == WG_FLOAT64 || col->type == WG_UINT16 ||
+               col->type == WG_UINT32 || col->type == WG_IPV4,
+           WG_ERR_UNSUPPORTED, "no filter kernel for type %d", col->type);
+  WG_CHECK(ctx->noalloc_scratch, WG_ERR_STATE, "the context has no filter scratch");
+  const uint64_t n = sel ? n_sel : n_rows;
+  WG_CHECK(n <= kNoAllocMaxRows, WG_ERR_INVALID,
+           "wg_filter_noalloc takes at most %llu rows per call (%llu given): split the batch or use wg_filter",
+           (unsigned long long)kNoAllocMaxRows, (unsigned long long)n);
+  *n_out = 0;
+  if (n == 0) return WG_OK;
+  // every word read below is written first by this call: counts and ballots
+  // by pass 1 (for every tile / 64-row word below n), offsets and the total
+  // by the scan
+  const uint64_t tiles = (n + kSelTile - 1) / kSelTile;
+  uint32_t* counts = (uint32_t*)ctx->noalloc_scratch;
+  uint32_t* offsets = counts + kNoAllocTiles;
+  uint64_t* masks = ctx->noalloc_scratch + kNoAllocTiles;
+  uint64_t* total = masks + kNoAllocMaxRows / 64;
+  const FilterPred pred{ColDesc{col->data, col->validity, col->type, col->scale}, op, i64, f64};
+  hipLaunchKernelGGL((select_rows_kernel<false, FilterPred>), dim3(tiles), dim3(kSelBlock), 0, ctx->stream, pred, n,
+                     sel, masks, counts, (const uint32_t*)nullptr, (uint32_t*)nullptr);
+  WG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t*)counts, offsets, tiles,
+                     total);
+  WG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((select_rows_kernel<true, FilterPred>), dim3(tiles), dim3(kSelBlock), 0, ctx->stream, pred, n,
+                     sel, masks, (uint32_t*)nullptr, (const uint32_t*)offsets, out_sel);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipMemcpyAsync(n_out, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+  WG_HIP(hipStreamSynchronize(ctx->stream));
+  return WG_OK;
 }
 
 }  // extern "C"
