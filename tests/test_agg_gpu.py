@@ -271,3 +271,59 @@ def test_empty_input_and_reset(ctx):
     agg.consume(cols, 10)
     agg.finalize()
     assert agg.num_groups() > 0
+
+
+def test_dense_emission_then_merge_and_more_rows(ctx):
+    """A partitioned consume into an empty aggregate writes its finished groups
+    straight to the output; later work folds them back first: a MergeSink of
+    two such aggregates, a direct consume after one, and a spill run of the
+    result merged into a third aggregate all equal the oracle."""
+    n = 3_000_000
+    cols = W.gen_zipf_groupby(ctx, n, seed=43)
+    hcols = host_cols(cols, n)
+    aggs = [(a.func, a.col) for a in W.ZIPF_AGGS]
+    ref = oracle_groups(run_oracle(hcols, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_PRED))
+    sums = abs_sums(hcols, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_PRED)
+    half = n // 2
+    print("[agg] dense emission: merge of two dense aggregates", flush=True)
+    a1 = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=400_000)
+    a2 = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=400_000)
+    for a in (a1, a2):
+        a.set_strategy(2, 2048, 1024, 1 << 16)
+    sel1 = ctx.to_device(np.arange(0, half, dtype=np.uint32))
+    sel2 = ctx.to_device(np.arange(half, n, dtype=np.uint32))
+    a1.consume(cols, n, sel=(sel1, half), pred=W.ZIPF_PRED)
+    a2.consume(cols, n, sel=(sel2, n - half), pred=W.ZIPF_PRED)
+    a1.merge(a2)
+    a1.finalize()
+    compare(gpu_groups(a1), ref, aggs, W.ZIPF_TYPES, W.ZIPF_SCALES, sums)
+
+    print("[agg] dense emission: a direct consume after a dense one", flush=True)
+    a3 = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=400_000)
+    a3.set_strategy(2, 2048, 1024, 1 << 16)
+    a3.consume(cols, n, sel=(sel1, half), pred=W.ZIPF_PRED)
+    a3.set_strategy(1)
+    a3.consume(cols, n, sel=(sel2, n - half), pred=W.ZIPF_PRED)
+    a3.finalize()
+    compare(gpu_groups(a3), ref, aggs, W.ZIPF_TYPES, W.ZIPF_SCALES, sums)
+
+    print("[agg] dense emission: reset, again, spill run merged into a fresh aggregate", flush=True)
+    a3.reset()
+    a3.set_strategy(2, 2048, 1024, 1 << 16)
+    a3.consume(cols, n, pred=W.ZIPF_PRED)
+    a3.finalize()
+    compare(gpu_groups(a3), ref, aggs, W.ZIPF_TYPES, W.ZIPF_SCALES, sums)
+    a4 = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=400_000)
+    a4.set_strategy(2, 2048, 1024, 1 << 16)
+    a4.consume(cols, n, sel=(sel1, half), pred=W.ZIPF_PRED)  # dense rows pending when the run arrives
+    a5 = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES, group_hint=400_000)
+    a5.set_strategy(2, 2048, 1024, 1 << 16)
+    a5.consume(cols, n, sel=(sel2, n - half), pred=W.ZIPF_PRED)
+    a5.finalize()
+    a4.merge_run(a5.spill_run())
+    a4.finalize()
+    g4 = gpu_groups(a4)
+    assert g4["n"] == ref["n"]
+    np.testing.assert_array_equal(g4["keys"], ref["keys"])
+    for x in (a1, a2, a3, a4, a5):
+        x.close()

@@ -329,7 +329,8 @@ __global__ void agg_merge_table_kernel(const uint64_t* src, uint64_t src_cap, co
 constexpr int kCompactBlock = 256;
 
 __global__ void __launch_bounds__(kCompactBlock) agg_compact_kernel(const uint64_t* slots, uint64_t cap,
-                                                                   const uint64_t* base, int stride, int n_base_words,
+                                                                   uint64_t first, const uint64_t* base, int stride,
+                                                                   int n_base_words,
                                                                    uint64_t* out, uint64_t* out_base,
                                                                    uint8_t* out_null, uint64_t* out_n,
                                                                    uint64_t per_block) {
@@ -337,7 +338,7 @@ __global__ void __launch_bounds__(kCompactBlock) agg_compact_kernel(const uint64
   __shared__ uint8_t live_lane[kCompactBlock];  // per wave: lanes of its live slots, in order
   __shared__ unsigned long long s_base;
   const uint64_t total = cap + kSideSlots;
-  const uint64_t beg = (uint64_t)blockIdx.x * per_block;
+  const uint64_t beg = first + (uint64_t)blockIdx.x * per_block;
   const uint64_t end = min(beg + per_block, total);
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   // pass 1: count this block's live slots
@@ -782,6 +783,54 @@ wg_status agg_ensure_capacity(wg_agg* h, uint64_t extra_groups) {
   return grow_to(h, next_pow2(std::max<uint64_t>(need + need / 3, 1024)));
 }
 
+wg_status agg_ensure_dense(wg_agg* h, uint64_t rows, uint64_t keep) {
+  if (h->dense_cap >= rows) return WG_OK;
+  uint64_t* d = nullptr;
+  uint64_t* db = nullptr;
+  uint8_t* dn = nullptr;
+  const uint64_t S = h->pg.stride, B = (uint64_t)h->n_sumd * 2;
+  struct Release {  // a partial allocation is undone on every error path
+    uint64_t** a;
+    uint64_t** b;
+    uint8_t** c;
+    ~Release() {
+      if (*a) (void)hipFree(*a);
+      if (*b) (void)hipFree(*b);
+      if (*c) (void)hipFree(*c);
+    }
+  } release{&d, &db, &dn};
+  WG_HIP(hipMalloc(&d, rows * S * 8));
+  if (B) WG_HIP(hipMalloc(&db, rows * B * 8));
+  WG_HIP(hipMalloc(&dn, rows));
+  if (keep) {
+    WG_HIP(hipMemcpyAsync(d, h->dense, keep * S * 8, hipMemcpyDeviceToDevice, h->ctx->stream));
+    if (B) WG_HIP(hipMemcpyAsync(db, h->dense_base, keep * B * 8, hipMemcpyDeviceToDevice, h->ctx->stream));
+    WG_HIP(hipMemcpyAsync(dn, h->dense_null, keep, hipMemcpyDeviceToDevice, h->ctx->stream));
+  }
+  WG_HIP(hipStreamSynchronize(h->ctx->stream));
+  if (h->dense) WG_HIP(hipFree(h->dense));
+  if (h->dense_base) WG_HIP(hipFree(h->dense_base));
+  if (h->dense_null) WG_HIP(hipFree(h->dense_null));
+  h->dense = d;
+  h->dense_base = db;
+  h->dense_null = dn;
+  h->dense_cap = rows;
+  d = db = nullptr;
+  dn = nullptr;
+  return WG_OK;
+}
+
+wg_status agg_settle(wg_agg* h) {
+  if (h->n_dense == 0) return WG_OK;
+  // dense rows carry zero exact bases: their limbs hold the whole sums
+  const uint64_t n = h->n_dense;
+  WG_TRY(agg_ensure_capacity(h, n));
+  WG_TRY(agg_merge_partials(h, h->dense, n));
+  h->n_dense = 0;
+  h->dirty = true;
+  return WG_OK;
+}
+
 wg_status normalize(wg_agg* h) {
   if (h->n_sumd == 0 || h->rows_since_norm == 0) {
     h->rows_since_norm = 0;
@@ -1016,6 +1065,7 @@ static wg_status consume_range(wg_agg* h, const ColSet& cs, const uint32_t* sel,
 wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred) {
   WG_CHECK(!h->finalized, WG_ERR_STATE, "consume after finalize");
   if (n == 0) return WG_OK;
+  WG_TRY(agg_settle(h));
   WG_CHECK(n < (1ull << 31), WG_ERR_INVALID, "batches of 2^31 rows or more must be split by the caller");
   if (h->rows_since_norm + n >= (1ull << 32) - 1) WG_TRY(normalize(h));
   WG_TRY(agg_ensure_capacity(h, 0));
@@ -1211,13 +1261,13 @@ wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types, c
     tmp.insert(tmp.end(), h->sumd_limbs.begin(), h->sumd_limbs.end());
     WG_HIP(hipMemcpy(h->d_sumd_word, tmp.data(), h->n_sumd * 8, hipMemcpyHostToDevice));
   }
-  if (hipMalloc(&h->counters, CTR_N * 8) != hipSuccess ||
+  if (hipMalloc(&h->counters, kAggCounterWords * 8) != hipSuccess ||
       hipHostMalloc((void**)&h->pinned, kAggPinnedWords * 8, hipHostMallocDefault) != hipSuccess) {
     set_error("device or pinned allocation of the counters failed");
     wg_agg_destroy(h);
     return WG_ERR_OOM;
   }
-  WG_TRY(zero_dev(ctx, h->counters, CTR_N));
+  WG_TRY(zero_dev(ctx, h->counters, kAggCounterWords));
   const uint64_t hint = std::max<uint64_t>(group_hint, 1024);
   h->group_hint = group_hint;
   st = alloc_table(h, next_pow2(hint + hint / 4));
@@ -1330,6 +1380,8 @@ wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
   WG_CHECK(dst->wide == src->wide, WG_ERR_INVALID, "merge of aggregates with different keys");
   WG_TRY(agg_flush_stage(dst));
   WG_TRY(agg_flush_stage(src));
+  WG_TRY(agg_settle(dst));
+  WG_TRY(agg_settle(src));
   if (dst->wide) {
     // src's keys are ids of its own dictionary: rewrite them as dst's ids
     for (int k = 0; k < dst->wkey.n; ++k)
@@ -1352,21 +1404,20 @@ wg_status wg_agg_finalize(wg_agg* h) {
   WG_CHECK(h, WG_ERR_INVALID, "null handle");
   if (h->finalized) return WG_OK;
   WG_TRY(agg_flush_stage(h));
-  const uint64_t need = h->groups + kSideSlots;
-  if (h->dense_cap < need) {
-    WG_TRY(free_dense(h));
-    WG_HIP(hipMalloc(&h->dense, need * (uint64_t)h->pg.stride * 8));
-    if (h->n_sumd) WG_HIP(hipMalloc(&h->dense_base, need * (uint64_t)h->n_sumd * 16));
-    WG_HIP(hipMalloc(&h->dense_null, need));
-    h->dense_cap = need;
-  }
+  // rows [0, n_dense) already hold the groups a partitioned consume emitted;
+  // the table's live slots are compacted after them
+  const uint64_t need = h->n_dense + h->groups + kSideSlots;
+  WG_TRY(agg_ensure_dense(h, need, h->n_dense));
   uint64_t* out_n = h->counters + CTR_LDS_ROWS;  // reuse a counter word
-  WG_TRY(zero_dev(h->ctx, out_n, 1));
-  const uint64_t total = h->cap + kSideSlots;
+  h->pinned[0] = h->n_dense;
+  WG_HIP(hipMemcpyAsync(out_n, h->pinned, 8, hipMemcpyHostToDevice, h->ctx->stream));
+  // an untouched table: only its side slots can be live
+  const uint64_t first = h->dirty ? 0 : h->cap;
+  const uint64_t total = h->cap + kSideSlots - first;
   const uint64_t per_block = std::max<uint64_t>(4096, (total + 8191) / 8192);
   const uint64_t blocks = (total + per_block - 1) / per_block;
   hipLaunchKernelGGL(agg_compact_kernel, dim3((unsigned)blocks), dim3(kCompactBlock), 0, h->ctx->stream, h->slots,
-                     h->cap, (const uint64_t*)h->base, h->pg.stride, h->n_sumd * 2, h->dense, h->dense_base,
+                     h->cap, first, (const uint64_t*)h->base, h->pg.stride, h->n_sumd * 2, h->dense, h->dense_base,
                      h->dense_null, out_n, per_block);
   WG_HIP(hipGetLastError());
   uint64_t c[CTR_N];
@@ -1513,9 +1564,12 @@ wg_status wg_agg_reset(wg_agg* h) {
   WG_CHECK(h, WG_ERR_INVALID, "null handle");
   h->stage.rows = 0;
   if (h->dirty) WG_TRY(fill_table(h, h->slots, h->cap));
+  else if (h->side_dirty) WG_TRY(fill_table(h, h->slots + h->cap * (uint64_t)h->pg.stride, 0));
   if (h->base) WG_TRY(zero_dev(h->ctx, h->base, (h->cap + kSideSlots) * (uint64_t)h->n_sumd * 2));
-  WG_TRY(zero_dev(h->ctx, h->counters, CTR_N));
+  WG_TRY(zero_dev(h->ctx, h->counters, kAggCounterWords));
   h->dirty = false;
+  h->side_dirty = false;
+  h->n_dense = 0;
   h->groups = 0;
   h->rows_since_norm = 0;
   h->finalized = false;
@@ -1534,7 +1588,7 @@ wg_status wg_agg_state_bytes(wg_agg* h, uint64_t* bytes) {
   clear_error();
   WG_CHECK(h && bytes, WG_ERR_INVALID, "null argument");
   if (!h->finalized) WG_TRY(agg_flush_stage(h));
-  const uint64_t groups = h->finalized ? h->out_n : h->groups;
+  const uint64_t groups = h->finalized ? h->out_n : h->groups + h->n_dense;
   *bytes = groups * ((uint64_t)h->pg.stride * 8 + (uint64_t)h->n_sumd * 16);
   return WG_OK;
 }

@@ -331,7 +331,12 @@ enum Ctr : int {
   CTR_FLUSHED = 7,
   CTR_N = 8
 };
-// pinned words per aggregate handle: the CTR_N counters, then two work-item cursors
+// device counter words per aggregate handle: the CTR_N counters, then the
+// dense-output row count (outside the range the consume passes clear)
+constexpr int kCtrDense = CTR_N;
+constexpr int kAggCounterWords = CTR_N + 1;
+// pinned words per aggregate handle: the CTR_N counters, then work-item
+// cursors and the dense row count read back
 constexpr int kAggPinnedWords = CTR_N + 8;
 
 // Linear probing stops after this many slots; a table that needs more is
@@ -407,9 +412,9 @@ constexpr int kLdsProbes = 16;  // a key beyond this run of probes aggregates in
 
 // Returns a pointer to the slot's key word (word k at ptr[k * lds_slots]).
 __device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_mask, uint64_t key, uint64_t h,
-                                                      int shift = 40) {
+                                                      int shift = 40, int max_probes = kLdsProbes) {
   uint32_t idx = (uint32_t)(h >> shift) & (uint32_t)lds_mask;
-  for (int p = 0; p < kLdsProbes; ++p) {
+  for (int p = 0; p < max_probes; ++p) {
     uint64_t* s = lds + idx;
     uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == key) return s;
@@ -418,6 +423,27 @@ __device__ __forceinline__ uint64_t* lds_find_or_claim(uint64_t* lds, int lds_ma
       if (prev == kEmpty || prev == key) return s;
     }
     idx = (idx + 1) & (uint32_t)lds_mask;
+  }
+  return nullptr;
+}
+
+// The same claim with triangular probing (home, +1, +3, +6, ...: every slot of
+// a power-of-two table once in lds_mask + 1 probes). Linear probing near 3/4
+// load forms runs of occupied slots long enough to exhaust a probe window now
+// and then; with triangular steps each probe lands in an independent place,
+// so a window of max_probes slots is exhausted only when the table is full.
+__device__ __forceinline__ uint64_t* lds_find_or_claim_tri(uint64_t* lds, int lds_mask, uint64_t key, uint64_t h,
+                                                          int shift, int max_probes) {
+  uint32_t idx = (uint32_t)(h >> shift) & (uint32_t)lds_mask;
+  for (int p = 0; p < max_probes; ++p) {
+    uint64_t* s = lds + idx;
+    uint64_t k = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == key) return s;
+    if (k == kEmpty) {
+      uint64_t prev = at_cas<LdsT>(s, kEmpty, key);
+      if (prev == kEmpty || prev == key) return s;
+    }
+    idx = (idx + (uint32_t)p + 1) & (uint32_t)lds_mask;
   }
   return nullptr;
 }

@@ -68,7 +68,11 @@ struct wg_agg {
   uint64_t ovf_rows_cap = 0;
   uint64_t* ovf_part = nullptr;
   uint64_t ovf_part_cap = 0;
-  bool dirty = false;  // table not in the freshly-filled state
+  bool dirty = false;       // table slots not in the freshly-filled state
+  bool side_dirty = false;  // only the side slots written since the last fill
+  // groups a partitioned consume into an empty aggregate wrote straight to
+  // the dense output (rows [0, n_dense) of `dense`), not into the table
+  uint64_t n_dense = 0;
   // partitioned consume (high-cardinality batches)
   int32_t strategy = 0;  // 0 auto, 1 direct, 2 partitioned
   uint64_t group_hint = 0;
@@ -149,5 +153,10 @@ wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, ui
 wg_status wide_encode_pairs(wg_agg* h, const uint64_t* pairs, uint64_t n, uint64_t* ids_out);
 wg_status wide_dict_host(wg_agg* h, std::vector<uint64_t>* pairs);
 wg_status wide_remap_slots(wg_agg* dst, wg_agg* src, uint64_t** remapped);
+// Fold groups emitted to the dense output back into the table (before any
+// consume, merge or merge-run that must see them there).
+wg_status agg_settle(wg_agg* h);
+// The dense output arrays with room for `rows` groups, keeping the first `keep`.
+wg_status agg_ensure_dense(wg_agg* h, uint64_t rows, uint64_t keep);
 wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* sel, uint64_t n, const PredDesc* pred);
 }  // namespace wg

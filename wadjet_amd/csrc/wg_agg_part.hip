@@ -277,13 +277,11 @@ static CountFn typed_count_for(const ColSet& cs, const KeyDesc& K, const PredDes
   return nullptr;
 }
 
-// RW = record words: key + values (2, 3, 4 or 8 words; 3 and 4 differ by a
-// zero pad word, 8 pads up to seven values).
-template <int RW>
-__global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) {
-  constexpr int NV = RW - 1;
-  extern __shared__ uint32_t cur[];
-  const RowSpec<NV> S = make_spec<NV>(P);
+// RW = record words: key + values (2, 4 or 8 words; zero pad words after
+// the values).
+// Pass C's LDS: the workgroup's partition cursors, the hot-set lookup and the
+// hot accumulators (initial values); returns the hot layout.
+__device__ __forceinline__ HotLds scatter_setup(const PartParams& P, uint32_t* cur) {
   const uint32_t np = 1u << P.pbits;
   for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) cur[p] = P.offs[(uint64_t)p * gridDim.x + blockIdx.x];
   const HotLds HL = hot_lds(cur, np, P.hot_bits, P.hot_slots);
@@ -295,66 +293,189 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) 
     }
     hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)H));
   }
-  uint64_t* const spare = HL.acc + H + (threadIdx.x & 63);
   __syncthreads();
+  return HL;
+}
+
+// Pass C for U loaded rows of a thread: hot rows into the LDS accumulators,
+// side-slot rows into the table's side slots, cold rows to their partition.
+template <int RW, int U>
+__device__ __forceinline__ void scatter_rows(const PartParams& P, const HotLds& HL, uint32_t* cur,
+                                             const uint64_t (&row)[U], const uint64_t (&key)[U], const int (&kind)[U],
+                                             const uint64_t (&val)[U][RW - 1], const uint32_t (&vvalid)[U]) {
+  constexpr int NV = RW - 1;
+  const int H = P.hot_slots;
+  const int stride = P.pg.stride;
+  uint64_t hk[U];
+  uint32_t hot = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) hk[u] = key_hash(key[u]);
+  if (H) {
+    // hot rows aggregate here: the same LDS atomics on every lane
+    // (identity operands on the lane's spare slot for the others)
+    uint64_t* const spare = HL.acc + H + (threadIdx.x & 63);
+    uint64_t* w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int hi = kind[u] == 0 ? hot_find(HL, key[u], hk[u]) : -1;
+      w[u] = hi >= 0 ? HL.acc + hi : spare;
+      hot |= (uint32_t)(hi >= 0) << u;
+    }
+    apply_vals_lds_batch<NV, U>(w, HL.HW, P.pgv, val, P.vfloat, vvalid, hot);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (kind[u] < 0 || ((hot >> u) & 1u)) continue;
+    if (kind[u] != 0) {
+      // side slots: [0] the sentinel-valued key, [1] the NULL key
+      uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
+      apply_row<GlbT, false>(s, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
+      continue;
+    }
+    const uint32_t pos = atomicAdd(&cur[hk[u] >> (64 - P.pbits)], 1u);
+    uint64_t w[RW];
+    w[0] = key[u];
+#pragma unroll
+    for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
+    uint64_t* rec = P.recs + (uint64_t)pos * RW;
+    static_assert(RW % 2 == 0, "records are whole 16-byte pairs");
+#pragma unroll
+    for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)rec)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+  }
+}
+
+// This workgroup's partial hot slots, dense: [H][stride], key in word 0.
+__device__ __forceinline__ void scatter_flush_hot(const PartParams& P, const HotLds& HL) {
+  const int H = P.hot_slots, stride = P.pg.stride;
+  if (!H) return;
+  __syncthreads();
+  const uint32_t n_hot = min(*P.n_hot, (uint32_t)H);
+  uint64_t* dst = P.hot_part + (uint64_t)blockIdx.x * H * stride;
+  for (int i = threadIdx.x; i < H * stride; i += blockDim.x) {
+    const int h = i / stride, w = i % stride;
+    dst[i] = w ? HL.acc[(uint64_t)w * HL.HW + h] : ((uint32_t)h < n_hot ? P.hot_keys[h] : kEmpty);
+  }
+}
+
+// RW = record words: key + values (2, 4 or 8 words; zero pad words after
+// the values).
+template <int RW>
+__global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) {
+  constexpr int NV = RW - 1;
+  extern __shared__ uint32_t cur[];
+  const RowSpec<NV> S = make_spec<NV>(P);
+  const HotLds HL = scatter_setup(P, cur);
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  const int stride = P.pg.stride;
   for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kScatterU) {
     uint64_t row[kScatterU], key[kScatterU], val[kScatterU][NV];
     uint32_t vvalid[kScatterU];
     int kind[kScatterU];
     load_rows<NV, kScatterU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
-    uint64_t hk[kScatterU];
-    uint32_t hot = 0;
+    scatter_rows<RW, kScatterU>(P, HL, cur, row, key, kind, val, vvalid);
+  }
+  scatter_flush_hot(P, HL);
+}
+
+// Pass C for the shapes the typed pass A takes (keys of one or two non-null
+// 4/8-byte integer columns, at most one predicate on a non-null 4/8-byte
+// column) whose value columns are non-null 8-byte words (int64, DECIMAL,
+// float64, IPv4): loads typed by width, kScatterTU rows in flight per thread,
+// a value column that is also the predicate column read once. Writes exactly
+// what part_scatter_kernel writes for the same rows.
+constexpr int kScatterTU = 4;
+
+template <int W0, int W1, int PW, int RW>
+__global__ void __launch_bounds__(kPartBlock) part_scatter_typed_kernel(PartParams P) {
+  constexpr int U = kScatterTU, NV = RW - 1;
+  extern __shared__ uint32_t cur[];
+  const HotLds HL = scatter_setup(P, cur);
+  const char* d0 = (const char*)P.cs.cols[P.key.col[0]].data;
+  const char* d1 = (const char*)P.cs.cols[P.key.col[W1 ? 1 : 0]].data;
+  const char* dp = (const char*)P.cs.cols[PW ? P.pred[0].col : 0].data;
+  const char* dv[NV];
+  bool vp[NV];  // value column j is the predicate column
 #pragma unroll
-    for (int u = 0; u < kScatterU; ++u) hk[u] = key_hash(key[u]);
-    if (H) {
-      // hot rows aggregate here: the same LDS atomics on every lane
-      // (identity operands on the lane's spare slot for the others)
-      uint64_t* w[kScatterU];
+  for (int j = 0; j < NV; ++j) {
+    const bool on = j < P.n_vals;
+    dv[j] = (const char*)P.cs.cols[on ? P.val_col[j] : 0].data;
+    vp[j] = on && PW == 8 && P.val_col[j] == P.pred[0].col;
+  }
+  const bool sx0 = P.cs.cols[P.key.col[0]].type == WG_INT32;
+  const bool pf = PW && P.cs.cols[P.pred[0].col].type == WG_FLOAT64;
+  const int sh0 = P.key.shift[0], sh1 = P.key.shift[1];
+  const int op = P.pred[0].op;
+  const int64_t ci = P.pred[0].i64;
+  const double cf = P.pred[0].f64;
+  const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
+  const uint64_t end = min(beg + P.rows_per_wg, P.n);
+  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * U) {
+    uint64_t row[U], a[U], b[U], pv[U], key[U], val[U][NV];
+    uint32_t in = 0, vvalid[U];
+    int kind[U];
 #pragma unroll
-      for (int u = 0; u < kScatterU; ++u) {
-        const int hi = kind[u] == 0 ? hot_find(HL, key[u], hk[u]) : -1;
-        w[u] = hi >= 0 ? HL.acc + hi : spare;
-        hot |= (uint32_t)(hi >= 0) << u;
-      }
-      apply_vals_lds_batch<NV, kScatterU>(w, HL.HW, P.pgv, val, P.vfloat, vvalid, hot);
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
+      in |= (uint32_t)(i < end) << u;
+      const uint64_t ii = i < end ? i : end - 1;
+      row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
+      vvalid[u] = ~0u;
     }
+    // every load of the step in flight before any is used
 #pragma unroll
-    for (int u = 0; u < kScatterU; ++u) {
-      if (kind[u] < 0 || ((hot >> u) & 1u)) continue;
-      if (kind[u] != 0) {
-        // side slots: [0] the sentinel-valued key, [1] the NULL key
-        uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
-        apply_row<GlbT, false>(s, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
-        continue;
+    for (int u = 0; u < U; ++u) a[u] = load_w<W0>(d0, row[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[u] = W1 ? load_w<W1 ? W1 : 4>(d1, row[u]) : 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) pv[u] = PW ? load_w<PW ? PW : 4>(dp, row[u]) : 0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (j < P.n_vals && !vp[j]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) val[u][j] = load_w<8>(dv[j], row[u]);
       }
-      const uint32_t pos = atomicAdd(&cur[hk[u] >> (64 - P.pbits)], 1u);
-      uint64_t w[RW];
-      w[0] = key[u];
 #pragma unroll
-      for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
-      uint64_t* rec = P.recs + (uint64_t)pos * RW;
-      if (RW % 2 == 0) {
+    for (int j = 0; j < NV; ++j)
 #pragma unroll
-        for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)rec)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+      for (int u = 0; u < U; ++u) val[u][j] = j >= P.n_vals ? 0 : vp[j] ? pv[u] : val[u][j];
+    const uint32_t pass = in & pred_bits<PW, U>(pv, pf, op, ci, cf);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (W1) {  // two 4-byte columns packed into one word
+        key[u] = ((a[u] & 0xFFFFFFFFull) << sh0) | ((b[u] & 0xFFFFFFFFull) << sh1);
       } else {
-#pragma unroll
-        for (int j = 0; j < RW; ++j) rec[j] = w[j];  // 24-byte records are only 8-byte aligned
+        key[u] = (W0 == 4 && sx0) ? (uint64_t)(int64_t)(int32_t)(uint32_t)a[u] : a[u];
       }
+      kind[u] = !((pass >> u) & 1u) ? -1 : key[u] == kEmpty ? 1 : 0;
     }
+    scatter_rows<RW, U>(P, HL, cur, row, key, kind, val, vvalid);
   }
-  if (H) {
-    // this workgroup's partial hot slots, dense: [H][stride], key in word 0
-    __syncthreads();
-    const uint32_t n_hot = min(*P.n_hot, (uint32_t)H);
-    uint64_t* dst = P.hot_part + (uint64_t)blockIdx.x * H * stride;
-    for (int i = threadIdx.x; i < H * stride; i += blockDim.x) {
-      const int h = i / stride, w = i % stride;
-      dst[i] = w ? HL.acc[(uint64_t)w * HL.HW + h] : ((uint32_t)h < n_hot ? P.hot_keys[h] : kEmpty);
-    }
-  }
+  scatter_flush_hot(P, HL);
+}
+
+// Dense emission of finished groups. A partitioned consume that starts from an
+// empty aggregate owns every key of a partition that one workgroup aggregates
+// alone, and every hot key: no other item, pass or earlier consume holds them.
+// Those groups go straight to the finalize output array in slot format (key
+// word, accumulator words), with zero exact-Int128 bases and a valid key; the
+// global table only receives what must still be merged there (split
+// partitions, rows whose LDS claim failed, side slots).
+struct DenseOut {
+  uint64_t* rows;               // [cap][stride]
+  uint64_t* base;               // [cap][base_words], or null
+  uint8_t* null_key;            // [cap]
+  unsigned long long* count;    // rows emitted so far
+  uint64_t cap;
+  int32_t base_words;
+};
+
+// Row `pos` of the dense output from slot words w0..: `word(k)` returns word k.
+template <class F>
+__device__ __forceinline__ void dense_put(const DenseOut& D, uint64_t pos, int stride, F word) {
+  uint64_t* d = D.rows + pos * (uint64_t)stride;
+  for (int w = 0; w < stride; w += 2) *(ulonglong2*)(d + w) = make_ulonglong2(word(w), word(w + 1));
+  for (int w = 0; w < D.base_words; ++w) D.base[pos * (uint64_t)D.base_words + w] = 0;
+  D.null_key[pos] = 0;
 }
 
 // ------------------------------------------------------------ hot set
@@ -468,7 +589,7 @@ __global__ void __launch_bounds__(256) hot_collect_kernel(HotScratch X, uint32_t
 // into the global table (overflow partial list if the table is past budget).
 __global__ void __launch_bounds__(256) hot_merge_kernel(const uint64_t* part, uint32_t G, uint32_t H,
                                                        const uint32_t* n_hot, AggProgram pg, GlobalTable T,
-                                                       uint64_t* ovf_part) {
+                                                       uint64_t* ovf_part, DenseOut dense) {
   __shared__ uint64_t s[64];
   const int stride = pg.stride;
   const uint32_t h = blockIdx.x;
@@ -481,7 +602,12 @@ __global__ void __launch_bounds__(256) hot_merge_kernel(const uint64_t* part, ui
       if (src[1]) merge_partial<LdsT>(s, src, 1, pg);  // word 1: the slot's row count
     }
     __syncthreads();
-    if (threadIdx.x == 0 && s[1]) {
+    if (threadIdx.x == 0 && s[1] && dense.rows) {
+      // hot keys are never partitioned: this block holds all of the key's rows
+      s[0] = part[(uint64_t)h * stride];
+      const uint64_t pos = atomicAdd(dense.count, 1ull);  // < cap: the host reserved H rows
+      dense_put(dense, pos, stride, [&](int w) { return w < stride ? s[w] : 0ull; });
+    } else if (threadIdx.x == 0 && s[1]) {
       const uint64_t key = part[(uint64_t)h * stride];
       uint64_t* d = glb_find_or_claim(T, stride, key, key_hash(key), claims);
       if (d) {
@@ -530,9 +656,15 @@ struct PartAggParams {
   uint64_t* ovf_part;
   uint64_t ovf_part_cap;
   uint32_t vfloat;  // bit j: value word j holds float64 bits (else an int64)
+  DenseOut dense;   // rows == nullptr: merge every item into the global table
 };
 
 constexpr int kFlushU = 2;
+// A partition's keys fill its LDS table to about 3/4 of L. A key that finds
+// no LDS slot aggregates in the global table, and under dense emission one
+// such key makes finalize scan (and reset refill) the whole table: the probe
+// sequence is triangular, so 128 probes miss only in a table that is full.
+constexpr int kPartLdsProbes = 128;
 
 template <int RW>
 __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
@@ -596,22 +728,17 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
 #pragma unroll
       for (int u = 0; u < kAggU; ++u) {
         const uint64_t* src = P.recs + pos[u] * RW;
-        if (RW % 2 == 0) {
 #pragma unroll
-          for (int j = 0; j < RW / 2; ++j) {
-            const ulonglong2 x = ((const ulonglong2*)src)[j];
-            rec[u][2 * j] = x.x;
-            rec[u][2 * j + 1] = x.y;
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < RW; ++j) rec[u][j] = src[j];
+        for (int j = 0; j < RW / 2; ++j) {
+          const ulonglong2 x = ((const ulonglong2*)src)[j];
+          rec[u][2 * j] = x.x;
+          rec[u][2 * j + 1] = x.y;
         }
       }
       uint64_t* ls[kAggU];
 #pragma unroll
       for (int u = 0; u < kAggU; ++u)
-        ls[u] = in[u] ? lds_find_or_claim(lds, L - 1, rec[u][0], key_hash(rec[u][0]), 16) : nullptr;
+        ls[u] = in[u] ? lds_find_or_claim_tri(lds, L - 1, rec[u][0], key_hash(rec[u][0]), 16, kPartLdsProbes) : nullptr;
       // same LDS atomics on every lane (identity operands on the spare slot
       // where the row does not aggregate here)
       {
@@ -647,6 +774,30 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
       }
     }
     __syncthreads();
+    if (P.dense.rows && !split) {
+      // this workgroup holds every row of the item's keys: its LDS groups are
+      // final, written to the dense output behind one reservation
+      __shared__ uint32_t s_live;
+      __shared__ unsigned long long s_at;
+      if (threadIdx.x == 0) s_live = 0;
+      __syncthreads();
+      uint32_t mine = 0;
+      for (int i = threadIdx.x; i < L; i += blockDim.x) mine += lds[i] != kEmpty;
+      if (mine) atomicAdd(&s_live, mine);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        s_at = s_live ? atomicAdd(P.dense.count, (unsigned long long)s_live) : 0ull;
+        s_live = 0;
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        if (lds[i] == kEmpty) continue;
+        const uint64_t pos = s_at + atomicAdd(&s_live, 1u);  // < cap: the host reserved L rows per item
+        dense_put(P.dense, pos, stride, [&](int w) { return w < stride ? lds[(uint64_t)w * LW + i] : 0ull; });
+      }
+      __syncthreads();
+      continue;
+    }
     // merge the LDS table into the global table; first probes of kFlushU
     // slots per thread are in flight together
     for (int i0 = threadIdx.x; i0 < L; i0 += blockDim.x * kFlushU) {
@@ -719,12 +870,34 @@ static wg_status allow_lds(const void* kernel, size_t bytes) {
 
 using ScatterFn = void (*)(PartParams);
 static ScatterFn scatter_kernel_for(int rw) {
-  return rw == 2 ? part_scatter_kernel<2> : rw == 3 ? part_scatter_kernel<3> : rw == 4 ? part_scatter_kernel<4>
-                                                                                       : part_scatter_kernel<8>;
+  return rw == 2 ? part_scatter_kernel<2> : rw == 4 ? part_scatter_kernel<4> : part_scatter_kernel<8>;
 }
 
-static wg_status launch_scatter(int rw, dim3 grid, size_t lds, hipStream_t st, const PartParams& P) {
-  hipLaunchKernelGGL(scatter_kernel_for(rw), grid, dim3(kPartBlock), lds, st, P);
+// The typed pass C for this batch's shape, or null (the general kernel): the
+// typed pass A's shapes, value columns non-null and 8 bytes wide.
+static ScatterFn typed_scatter_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pred,
+                                   const std::vector<int>& vcols, int rw) {
+  if (!typed_count_for(cs, K, pred) || rw > 4) return nullptr;
+  for (int c : vcols) {
+    const ColDesc& d = cs.cols[c];
+    if (d.valid || !(d.type == WG_INT64 || d.type == WG_DECIMAL || d.type == WG_FLOAT64 || d.type == WG_IPV4))
+      return nullptr;
+  }
+  auto width = [](int t) { return (t == WG_INT32 || t == WG_UINT32) ? 4 : 8; };
+  const int w0 = width(cs.cols[K.col[0]].type), w1 = K.n > 1 ? 4 : 0;
+  const int pw = pred[0].col < 0 ? 0 : cs.cols[pred[0].col].type == WG_INT32 ? 4 : 8;
+#define WG_TS(A, B, C)                                                                     \
+  if (w0 == A && w1 == B && pw == C)                                                       \
+    return rw == 2 ? part_scatter_typed_kernel<A, B, C, 2> : part_scatter_typed_kernel<A, B, C, 4>;
+  WG_TS(4, 4, 0) WG_TS(4, 4, 4) WG_TS(4, 4, 8)
+  WG_TS(4, 0, 0) WG_TS(4, 0, 4) WG_TS(4, 0, 8)
+  WG_TS(8, 0, 0) WG_TS(8, 0, 4) WG_TS(8, 0, 8)
+#undef WG_TS
+  return nullptr;
+}
+
+static wg_status launch_scatter(ScatterFn fn, dim3 grid, size_t lds, hipStream_t st, const PartParams& P) {
+  hipLaunchKernelGGL(fn, grid, dim3(kPartBlock), lds, st, P);
   WG_HIP(hipGetLastError());
   return WG_OK;
 }
@@ -734,10 +907,6 @@ static wg_status launch_part_agg(int rw, int grid, int block, size_t lds, hipStr
     case 2:
       WG_TRY(allow_lds((const void*)part_agg_kernel<2>, lds));
       hipLaunchKernelGGL(part_agg_kernel<2>, dim3(grid), dim3(block), lds, st, Q);
-      break;
-    case 3:
-      WG_TRY(allow_lds((const void*)part_agg_kernel<3>, lds));
-      hipLaunchKernelGGL(part_agg_kernel<3>, dim3(grid), dim3(block), lds, st, Q);
       break;
     case 4:
       WG_TRY(allow_lds((const void*)part_agg_kernel<4>, lds));
@@ -773,7 +942,13 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   WG_CHECK((int)vcols.size() <= kMaxVals, WG_ERR_UNSUPPORTED, "partitioned consume carries at most %d value columns",
            kMaxVals);
   WG_CHECK(h->key.n <= 2, WG_ERR_UNSUPPORTED, "partitioned consume takes keys of at most two columns");
-  const int RW = vcols.size() < 2 ? 2 : vcols.size() < 3 ? 3 : vcols.size() < 4 ? 4 : 8;
+  // records of 2, 4 or 8 words: a power of two, so each record fills whole
+  // 16- or 32-byte sectors (three words padded to four: a 24-byte record
+  // straddles sectors and leaves them partly written at eviction)
+  const int RW = vcols.size() < 2 ? 2 : vcols.size() < 4 ? 4 : 8;
+  // an aggregate with no groups yet owns every key this consume sees: finished
+  // groups go straight to the dense output instead of the table
+  const bool dense = h->groups == 0 && h->n_dense == 0 && !h->dirty;
   const int L = h->part_lds_slots;
   // partitions of at most about 3/4 L groups each: fewer, larger partitions
   // make the scatter's writes denser, while linear probing in the LDS table
@@ -800,16 +975,17 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   const size_t scat_bytes = scatter_lds(H);
   CountFn count_fn = typed_count_for(cs, h->key, pred);
   if (!count_fn) count_fn = part_count_kernel;
+  ScatterFn scatter_fn = typed_scatter_for(cs, h->key, pred, vcols, RW);
+  if (!scatter_fn) scatter_fn = scatter_kernel_for(RW);
   if (scat_bytes > 64 * 1024) {
     WG_HIP(hipFuncSetAttribute((const void*)count_fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    WG_HIP(hipFuncSetAttribute((const void*)scatter_kernel_for(RW), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               150 * 1024));
+    WG_HIP(hipFuncSetAttribute((const void*)scatter_fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   }
   // one resident round of count/scatter workgroups (the two passes must use
   // the same row ranges, so both run at the occupancy of the lower one)
   int occ_c = 0, occ_s = 0;
   WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, count_fn, kPartBlock, hist_bytes));
-  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_kernel_for(RW), kPartBlock, scat_bytes));
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_fn, kPartBlock, scat_bytes));
   const int per_cu = std::max(1, std::min(occ_c, occ_s));
   const int G = ctx->cus * per_cu;
   const uint64_t R = ((n + G - 1) / G + 255) / 256 * 256;
@@ -919,9 +1095,10 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   P.recs = h->part_buf;
 
   // C: scatter (side-slot rows are folded here)
-  h->dirty = true;
+  if (dense) h->side_dirty = true;
+  else h->dirty = true;
   WG_HIP(hipEventRecord(h->pev[2], ctx->stream));
-  WG_TRY(launch_scatter(RW, dim3(G), scat_bytes, ctx->stream, P));
+  WG_TRY(launch_scatter(scatter_fn, dim3(G), scat_bytes, ctx->stream, P));
   WG_HIP(hipEventRecord(h->pev[3], ctx->stream));
 
   // D: work items (built while the scatter runs), largest first; partitions
@@ -954,6 +1131,18 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   Q.lds_slots = L;
   Q.item_ctr = item_ctr;
   Q.vfloat = P.vfloat;
+  DenseOut D{};
+  if (dense) {
+    // at most L groups per item a workgroup owns alone, and H hot keys
+    uint64_t bound = (uint64_t)H;
+    for (uint64_t i = 0; i < n_items; ++i)
+      if (!hitems[i].split) bound += std::min<uint64_t>(hitems[i].end - hitems[i].beg, (uint64_t)L);
+    WG_TRY(agg_ensure_dense(h, bound + kSideSlots, 0));
+    WG_TRY(zero_dev(ctx, h->counters + kCtrDense, 1));
+    D = DenseOut{h->dense, h->n_sumd ? h->dense_base : nullptr, h->dense_null,
+                 (unsigned long long*)(h->counters + kCtrDense), bound, h->n_sumd * 2};
+  }
+  Q.dense = D;
   const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
   const int per_cu_agg = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
   const int grid_max = ctx->cus * per_cu_agg;
@@ -1058,7 +1247,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
     WG_HIP(hipEventRecord(h->pev[4], ctx->stream));
     hipLaunchKernelGGL(hot_merge_kernel, dim3(H), dim3(256), 0, ctx->stream, (const uint64_t*)P.hot_part, (uint32_t)G,
-                       (uint32_t)H, P.n_hot, h->pg, table_of(h), h->ovf_part);
+                       (uint32_t)H, P.n_hot, h->pg, table_of(h), h->ovf_part, D);
     WG_HIP(hipGetLastError());
     WG_HIP(hipEventRecord(h->pev[5], ctx->stream));
     uint64_t c[CTR_N];
@@ -1071,6 +1260,13 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
       WG_TRY(grow_to(h, next_pow2(std::max<uint64_t>((h->groups + c[CTR_OVF_PART]) * 2, h->cap * 2))));
       WG_TRY(agg_merge_partials(h, h->ovf_part, c[CTR_OVF_PART]));
     }
+  }
+  if (dense) {
+    WG_HIP(hipMemcpyAsync(h->pinned + CTR_N + 2, h->counters + kCtrDense, 8, hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(hipStreamSynchronize(ctx->stream));
+    h->n_dense = h->pinned[CTR_N + 2];
+    WG_CHECK(h->n_dense <= D.cap, WG_ERR_DEVICE, "dense output overran its reservation");
+    if (h->groups) h->dirty = true;  // split items, LDS misses or replays used the table
   }
   return WG_OK;
 }

@@ -1230,6 +1230,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
   if (n == 0) return WG_OK;
   WG_CHECK(n < (1ull << 31), WG_ERR_INVALID, "probe batches of 2^31 rows or more must be split");
   WG_TRY(agg_flush_stage(agg));
+  WG_TRY(agg_settle(agg));
   if (agg->rows_since_norm + n >= (1ull << 31)) WG_TRY(normalize(agg));
   WG_TRY(agg_ensure_capacity(agg, 0));
   WG_TRY(ensure_overflow_buffers(agg));

@@ -221,6 +221,7 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
   clear_error();
   WG_CHECK(h && (run || size == 0), WG_ERR_INVALID, "null argument");
   WG_CHECK(!h->finalized, WG_ERR_STATE, "merge after finalize");
+  WG_TRY(agg_settle(h));
   const uint8_t* p = (const uint8_t*)run;
   const uint8_t* const end = p + size;
   bool ok = true;
