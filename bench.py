@@ -5,7 +5,7 @@ Headline (the JSON line's `value`): the 100M-row composite GROUP BY
 (a int32 Zipf(1.1) over 1M values, b int32 uniform over 16) with
 SUM(price DECIMAL(15,2)) into Int128, COUNT(*), MIN/MAX/AVG(f float64) and
 WHERE f > -0.5 fused. Inputs are generated into HBM before timing; one step =
-reset + consume + finalize.
+reset + consume + finalize + the output columns written on the device.
 
 Beside it, in the same line:
   join     10M-row build, 200M-row probe, fused JOIN fk = key GROUP BY p1
@@ -17,7 +17,7 @@ Beside it, in the same line:
 `roofline` is the headline's consume kernels: algorithmic bytes
 (24 B/row + 56 B/group) over their device time (HIP events on the library
 stream). `cpu_baseline` times the C restatement (oracle/) on this host's cores
-over a prefix of the same rows.
+over the same rows (`join.cpu_baseline` likewise for the join).
 """
 from __future__ import annotations
 
@@ -68,10 +68,18 @@ def bench_agg(ctx, args):
     if args.strategy >= 0:
         agg.set_strategy(args.strategy)
 
+    # the step ends with every output column on the device (keys, finalized
+    # values, validity): the Next drain, without the copy to the host
+    agg.reset()
+    agg.consume(cols, n, pred=W.ZIPF_PRED)
+    agg.finalize()
+    out = agg.output_buffers(agg.num_groups() + (agg.num_groups() >> 4) + 1024)
+
     def step():
         agg.reset()
         agg.consume(cols, n, pred=W.ZIPF_PRED)
         agg.finalize()
+        agg.fetch_device(*out)
 
     for i in range(args.warmup):
         step()
@@ -105,7 +113,23 @@ def bench_agg(ctx, args):
                 passes=passes[-1], per_pass=per_pass, cold_rows=cold, agg=agg)
 
 
-def cpu_agg(cols, n, sample_rows, threads):
+def cpu_threads(requested=0):
+    """Host threads for the CPU baselines: the cores this process may run on,
+    at most 16 (the share of the host a one-GPU job gets)."""
+    avail = len(os.sched_getaffinity(0))
+    return (requested or min(avail, 16)), avail
+
+
+def reps_summary(times, rows):
+    import statistics
+    return dict(reps=len(times), min_s=min(times), median_s=statistics.median(times), rows=rows)
+
+
+def cpu_agg(cols, n, sample_rows, threads, host_cpus, reps):
+    """The C restatement of HashAggregate (oracle/) on `threads` host threads:
+    each aggregates a contiguous row range (a morsel worker's partial), the
+    partials are merged in worker order (CloneSink / MergeSink), over the same
+    rows the GPU step reads. Generation excluded; the merge included."""
     import numpy as np
 
     import oracle as O
@@ -114,17 +138,19 @@ def cpu_agg(cols, n, sample_rows, threads):
     host = [(cols[0].data.to_host(np.int32, m), O.INT32), (cols[1].data.to_host(np.int32, m), O.INT32),
             (cols[2].data.to_host(np.int64, m), O.DECIMAL, None, 2), (cols[3].data.to_host(np.float64, m), O.FLOAT64)]
     aggs = [(a.func, a.col) for a in W.ZIPF_AGGS]
-    best = None
-    for rep in range(2):
+    times = []
+    for rep in range(reps):
         t0 = time.perf_counter()
-        r = O.hash_agg(host, W.ZIPF_KEYS, aggs, pred=(3, O.GT, 0, -0.5), threads=threads, max_groups=m)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-        log(f"[cpu] rep {rep}: {m} rows, {r['n']} groups, {dt:.2f} s on {threads} threads")
-    return dict(value=m / best, unit="rows/s", cores=threads, kind="port",
-                sample=f"first {m} rows of the same 100M-row input; C restatement of HashAggregate, {threads} "
-                       f"threads each aggregating a contiguous row range, partials merged in thread order "
-                       f"(the CloneSink/MergeSink shape); generation excluded, merge included")
+        r = O.hash_agg(host, W.ZIPF_KEYS, aggs, pred=(3, O.GT, 0, -0.5), threads=threads,
+                       max_groups=min(m, 8_000_000))
+        times.append(time.perf_counter() - t0)
+        log(f"[cpu] agg rep {rep}: {m} rows, {r['n']} groups, {times[-1]:.2f} s on {threads} threads")
+    st = reps_summary(times, m)
+    return dict(value=m / st["min_s"], unit="rows/s", cores=threads, host_cpus=host_cpus, kind="port", **st,
+                sample=f"{'all' if m == n else 'the first'} {m} rows of the same input; C restatement of "
+                       f"HashAggregate on {threads} threads, each aggregating a contiguous row range, partials "
+                       f"merged in worker order (CloneSink/MergeSink); value = rows / fastest of {reps} reps; "
+                       f"generation excluded, merge included")
 
 
 # ------------------------------------- join 10M build x 200M probe, then GROUP BY
@@ -166,14 +192,73 @@ def bench_join(ctx, args):
     alg = W.FKJ_BYTES_PER_PROBE_ROW * cfg.n_probe
     from wadjet_amd.profiling import load_traffic
     traffic = load_traffic(str(JOIN_PMC_SUMMARY), is_join_agg_kernel)
-    log(f"[join] build {build_ms:.2f} ms, probe+agg {ms:.3f} ms/step (kernel {k:.3f} ms), groups {agg.num_groups()}")
+    floor = line_floor(ctx, cfg.n_probe)
+    log(f"[join] build {build_ms:.2f} ms, probe+agg {ms:.3f} ms/step (kernel {k:.3f} ms), groups {agg.num_groups()}; "
+        f"random-line floor for {cfg.n_probe} lookups {floor['ms']:.3f} ms")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_join(bcols, pcols, cfg, *cpu_threads(args.cpu_threads), args.cpu_reps)
     return dict(rows_per_s=cfg.n_probe / (ms / 1000), ms_per_step=ms, kernel_ms=k, build_ms=build_ms,
                 build_rows_per_s=cfg.n_build / (build_ms / 1000), groups=agg.num_groups(),
                 roofline=dict(bound="hbm", achieved=alg / (k / 1000) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=alg / (k / 1000) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
                               kernel="join_agg_lean_kernel (fused probe -> GROUP BY)", traffic=traffic,
-                              traffic_source=JOIN_PMC_SUMMARY.name if traffic else None),
-                _keep=(j, agg, bcols, pcols))
+                              traffic_source=JOIN_PMC_SUMMARY.name if traffic else None,
+                              line_floor=floor, of_line_floor=floor["ms"] / k),
+                cpu_baseline=cpu, _keep=(j, agg, bcols, pcols))
+
+
+def line_floor(ctx, n_lookups):
+    """The random-line rate that bounds a hash probe beyond the L2: the
+    library's gather (gatherVector) of n_lookups uniformly random rows of a
+    1 GiB int64 column (the size of the join's slot array), each row one
+    128-byte line request, with a 4-byte index read and an 8-byte value
+    written per row."""
+    from wadjet_amd import _lib as L
+    from wadjet_amd.gpu import Col, gather
+    rows = 1 << 27
+    col = ctx.gen(L.GEN_UNIFORM_I64, "int64", rows, 91, 0, 1 << 40)
+    idx = ctx.gen(L.GEN_UNIFORM_I32, "int32", n_lookups, 92, 0, rows - 1)
+    out = ctx.alloc(n_lookups * 8)
+    gather(ctx, Col(col, L.INT64), idx, n_lookups, out)
+    ctx.sync()
+    best = None
+    for _ in range(3):
+        ctx.timer_start()
+        gather(ctx, Col(col, L.INT64), idx, n_lookups, out)
+        t = ctx.timer_stop()
+        best = t if best is None else min(best, t)
+    for b in (col, idx, out):
+        b.free()
+    return dict(ms=best, lookups=n_lookups, lines_per_s=n_lookups / (best / 1000), table_bytes=rows * 8,
+                kernel="gather_kernel (wg_gather)")
+
+
+def cpu_join(bcols, pcols, cfg, threads, host_cpus, reps):
+    """The C restatement of HashJoin.Build + the probe feeding GROUP BY p1
+    SUM(qty * p2) (oracle join_agg): the build serial, the 200M probe rows over
+    `threads` threads (morsel workers), partial groups merged in worker order."""
+    import numpy as np
+
+    import oracle as O
+    fk = pcols[0].data.to_host(np.int64, cfg.n_probe)
+    qty = pcols[1].data.to_host(np.int32, cfg.n_probe)
+    key = bcols[0].data.to_host(np.int64, cfg.n_build)
+    p1 = bcols[1].data.to_host(np.int32, cfg.n_build)
+    p2 = bcols[2].data.to_host(np.int64, cfg.n_build)
+    times = []
+    for rep in range(reps):
+        t0 = time.perf_counter()
+        r = O.join_agg((key, O.INT64), (p1, O.INT32), (p2, O.DECIMAL, None, 2), (fk, O.INT64), (qty, O.INT32),
+                       threads=threads)
+        times.append(time.perf_counter() - t0)
+        log(f"[cpu] join rep {rep}: {cfg.n_probe} probe rows, {r['n']} groups, {times[-1]:.2f} s on {threads} threads")
+    st = reps_summary(times, cfg.n_probe)
+    return dict(value=cfg.n_probe / st["min_s"], unit="probe rows/s", cores=threads, host_cpus=host_cpus,
+                kind="port", **st,
+                sample=f"all {cfg.n_build} build and {cfg.n_probe} probe rows; C restatement of HashJoin.Build "
+                       f"(serial) and the probe feeding the GROUP BY on {threads} threads; value = probe rows / "
+                       f"fastest of {reps} reps, build included")
 
 
 # -------------------------------------------- join output: views vs eager gathers
@@ -473,8 +558,9 @@ def main():
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--group-hint", type=int, default=6_000_000)
     ap.add_argument("--strategy", type=int, default=-1)
-    ap.add_argument("--cpu-rows", type=int, default=25_000_000)
+    ap.add_argument("--cpu-rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--join-build", type=int, default=10_000_000)
     ap.add_argument("--join-probe", type=int, default=200_000_000)
@@ -524,8 +610,7 @@ def main():
         extra["served"] = bench_served(ctx, args)
     cpu = None
     if A is not None and not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu = cpu_agg(A["cols"], A["n"], args.cpu_rows, threads)
+        cpu = cpu_agg(A["cols"], A["n"], args.cpu_rows, *cpu_threads(args.cpu_threads), args.cpu_reps)
 
     if A is not None:
         achieved = A["alg_bytes"] / (A["kernel_ms"] / 1000) / 1e9
@@ -556,6 +641,10 @@ def main():
                          "kernel_ms": A["kernel_ms"], "pass_ms": A["passes"], "alg_bytes": A["alg_bytes"],
                          "per_pass": A["per_pass"], "scattered_rows": A["cold_rows"]},
             "cpu_baseline": cpu,
+            "parity": {"tests": "tests/test_fullsize_gpu.py (this workload at 100M rows and the join at 10M x 200M, "
+                                "against the oracle)",
+                       "exact": "group keys, COUNT, MIN/MAX, Int128 DECIMAL sums, integer sums, the join's row count",
+                       "float_tolerance": "float64 SUM/AVG within 1e-12 x sum(|v|) of the group (x 1/count for AVG)"},
         }
     else:
         out = {"metric": "rows/s & %HBM roofline for hash-agg/join probe; batched vs plain qps + p99",

@@ -327,3 +327,50 @@ def test_dense_emission_then_merge_and_more_rows(ctx):
     np.testing.assert_array_equal(g4["keys"], ref["keys"])
     for x in (a1, a2, a3, a4, a5):
         x.close()
+
+
+@pytest.mark.parametrize("case", ["zipf", "nulls", "wide"])
+def test_fetch_to_device_columns_equals_host_fetch(ctx, case):
+    """wg_agg_fetch into device columns (one pass writing every key, value and
+    validity column) == the host fetch, for a composite key with Int128 and
+    float aggregates, a nullable single key with the NULL group, and a 128-bit
+    key through the dictionary; also for a window of the groups."""
+    print(f"[agg] device fetch {case}", flush=True)
+    rng = np.random.default_rng(71)
+    if case == "zipf":
+        n = 2_000_000
+        cols = W.gen_zipf_groupby(ctx, n, seed=5)
+        agg = run_gpu(ctx, cols, n, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, W.ZIPF_SCALES, pred=W.ZIPF_PRED,
+                      hint=200_000)
+    elif case == "nulls":
+        n = 300_000
+        k = rng.integers(-50, 50, n).astype(np.int64)
+        v = rng.integers(-10**9, 10**9, n).astype(np.int64)
+        m = rng.random(n) > 0.05
+        cols = [Col(ctx.to_device(k), L.INT64, ctx.to_device(bitmap_from_mask(m))), Col(ctx.to_device(v), L.INT64,
+                                                                                        ctx.to_device(bitmap_from_mask(m)))]
+        agg = run_gpu(ctx, cols, n, [L.INT64, L.INT64], [0], [AggSpec(L.SUM, 1), AggSpec(L.COUNT, 1),
+                                                               AggSpec(L.MIN, 1), AggSpec(L.AVG, 1)],
+                      nullable=[True, True])
+    else:
+        n = 500_000
+        a = rng.integers(0, 40_000, n).astype(np.int64)
+        b = rng.integers(-3, 3, n).astype(np.int64)
+        v = rng.integers(-10**6, 10**6, n).astype(np.int64)
+        cols = [Col(ctx.to_device(x), L.INT64) for x in (a, b, v)]
+        agg = run_gpu(ctx, cols, n, [L.INT64] * 3, [0, 1], [AggSpec(L.SUM, 2), AggSpec(L.COUNT, -1)])
+    host = agg.fetch()
+    g = host["n"]
+    ks, vs, vv = agg.output_buffers(g)
+    agg.fetch_device(ks, vs, vv)
+    for x, k in enumerate(agg.keys):
+        np.testing.assert_array_equal(ks[x].to_host(host["keys"][x].dtype, g), host["keys"][x])
+    for a in range(len(agg.aggs)):
+        np.testing.assert_array_equal(vs[a].to_host(host["values"][a].dtype, g).view(np.uint64),
+                                      host["values"][a].view(np.uint64))
+        np.testing.assert_array_equal(vv[a].to_host(np.uint8, g).astype(bool), host["valid"][a])
+    off, cnt = g // 3, g // 3
+    agg.fetch_device([None] * len(agg.keys), vs, [None] * len(agg.aggs), offset=off, count=cnt)
+    np.testing.assert_array_equal(vs[0].to_host(host["values"][0].dtype, cnt).view(np.uint64),
+                                  host["values"][0][off:off + cnt].view(np.uint64))
+    agg.close()

@@ -413,15 +413,11 @@ __device__ __forceinline__ double i128_to_f64(uint64_t lo, int64_t hi, int scale
   return f / p10;
 }
 
-__global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_base, uint64_t off, uint64_t cnt,
-                                  int stride, int n_sumd, FinalDesc fd, double* out_f, int64_t* out_i,
-                                  uint8_t* out_valid, int want_float) {
-  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cnt; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t* s = dense + (off + j) * stride;
+// Finalized value of one aggregate of the group whose slot words are s and
+// whose exact Int128 bases start at b (n_sumd pairs).
+__device__ __forceinline__ void decode_value(const uint64_t* s, const uint64_t* b, const FinalDesc& fd,
+                                             uint8_t& valid, double& vf, int64_t& vi) {
     const int64_t count = (int64_t)s[fd.count_word];
-    uint8_t valid = 1;
-    double vf = 0.0;
-    int64_t vi = 0;
     switch (fd.func) {
       case WG_AGG_COUNT:
         vi = count;
@@ -439,8 +435,8 @@ __global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_b
           uint64_t lo;
           int64_t hi;
           limbs_value(s + fd.val_word, fd.limbs, lo, hi);
-          const uint64_t* b = dense_base + ((off + j) * n_sumd + fd.sumd_index) * 2;
-          unsigned __int128 v = (((unsigned __int128)(uint64_t)hi << 64) | lo) + (((unsigned __int128)b[1] << 64) | b[0]);
+          const uint64_t* bb = b + fd.sumd_index * 2;
+          unsigned __int128 v = (((unsigned __int128)(uint64_t)hi << 64) | lo) + (((unsigned __int128)bb[1] << 64) | bb[0]);
           sum = i128_to_f64((uint64_t)v, (int64_t)(uint64_t)(v >> 64), fd.scale);
         }
         vf = fd.func == WG_AGG_AVG ? sum / (double)count : sum;
@@ -460,6 +456,16 @@ __global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_b
         break;
       }
     }
+}
+
+__global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_base, uint64_t off, uint64_t cnt,
+                                  int stride, int n_sumd, FinalDesc fd, double* out_f, int64_t* out_i,
+                                  uint8_t* out_valid, int want_float) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cnt; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint8_t valid = 1;
+    double vf = 0.0;
+    int64_t vi = 0;
+    decode_value(dense + (off + j) * stride, dense_base + (off + j) * n_sumd * 2, fd, valid, vf, vi);
     if (out_valid) out_valid[j] = valid;
     if (want_float) {
       if (out_f) out_f[j] = valid ? vf : 0.0;
@@ -471,25 +477,72 @@ __global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_b
 
 // Unpack the key column k of groups [off, off+cnt) (writeIntKeyToColumn,
 // aggregate_partial_spill.go:2006-2021; packedField.get, packed_hash.go:342-351).
+__device__ __forceinline__ int64_t decode_key(uint64_t w, bool is_null, const KeyDesc& K, int k, const uint64_t* dict) {
+  if (dict && !is_null) w = dict[2 * w + K.word[k]];  // 128-bit keys: the id's {lo, hi} word
+  int64_t v;
+  if (K.n == 1) v = (int64_t)w;
+  else if (K.narrow[k] == 1) v = (int64_t)(int32_t)(uint32_t)(w >> K.shift[k]);
+  else if (K.narrow[k] == 2) v = (int64_t)(uint32_t)(w >> K.shift[k]);
+  else v = (int64_t)w;
+  return is_null ? 0 : v;
+}
+
+__device__ __forceinline__ void store_key(void* out, uint64_t j, int type, int64_t v) {
+  switch (type) {
+    case WG_INT32: ((int32_t*)out)[j] = (int32_t)v; break;
+    case WG_UINT32: ((uint32_t*)out)[j] = (uint32_t)v; break;
+    case WG_UINT16: ((uint16_t*)out)[j] = (uint16_t)v; break;
+    default: ((int64_t*)out)[j] = v; break;
+  }
+}
+
 __global__ void agg_key_kernel(const uint64_t* dense, const uint8_t* dense_null, uint64_t off, uint64_t cnt,
                                int stride, KeyDesc K, int k, void* out, uint8_t* valid_out, const uint64_t* dict) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < cnt; j += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t w = dense[(off + j) * stride];
     const bool is_null = dense_null[off + j] != 0;
-    if (dict && !is_null) w = dict[2 * w + K.word[k]];  // 128-bit keys: the id's {lo, hi} word
-    int64_t v;
-    if (K.n == 1) v = (int64_t)w;
-    else if (K.narrow[k] == 1) v = (int64_t)(int32_t)(uint32_t)(w >> K.shift[k]);
-    else if (K.narrow[k] == 2) v = (int64_t)(uint32_t)(w >> K.shift[k]);
-    else v = (int64_t)w;
-    if (is_null) v = 0;
+    const int64_t v = decode_key(dense[(off + j) * stride], is_null, K, k, dict);
     if (valid_out) valid_out[j] = is_null ? 0 : 1;
-    if (out == nullptr) continue;
-    switch (K.key_type[k]) {
-      case WG_INT32: ((int32_t*)out)[j] = (int32_t)v; break;
-      case WG_UINT32: ((uint32_t*)out)[j] = (uint32_t)v; break;
-      case WG_UINT16: ((uint16_t*)out)[j] = (uint16_t)v; break;
-      default: ((int64_t*)out)[j] = v; break;
+    if (out) store_key(out, j, K.key_type[k], v);
+  }
+}
+
+// Every requested output column of groups [off, off+cnt) in one pass over the
+// dense rows: each thread reads its group's slot words once and writes the
+// keys and the finalized values (the Next drain of nextOwn,
+// aggregate.go:4226-4524, into device columns).
+struct EmitParams {
+  const uint64_t* dense;
+  const uint64_t* dense_base;
+  const uint8_t* dense_null;
+  const uint64_t* dict;
+  uint64_t off, cnt;
+  int32_t stride, n_sumd, n_keys, n_aggs;
+  KeyDesc K;
+  void* key_out[kMaxKeys];
+  FinalDesc fd[kMaxOps];
+  void* val_out[kMaxOps];
+  uint8_t* valid_out[kMaxOps];
+  uint32_t want_float;  // bit a: aggregate a emits float64
+};
+
+__global__ void __launch_bounds__(256) agg_emit_kernel(EmitParams E) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E.cnt; j += (uint64_t)gridDim.x * blockDim.x) {
+    // the row's first read brings its line in; the other words of the row hit the L1 cache
+    const uint64_t* s = E.dense + (E.off + j) * E.stride;
+    const bool is_null = E.dense_null[E.off + j] != 0;
+    const uint64_t key = s[0];
+    for (int k = 0; k < E.n_keys; ++k)
+      if (E.key_out[k]) store_key(E.key_out[k], j, E.K.key_type[k], decode_key(key, is_null, E.K, k, E.dict));
+    const uint64_t* b = E.dense_base + (E.off + j) * E.n_sumd * 2;
+    for (int a = 0; a < E.n_aggs; ++a) {
+      uint8_t valid = 1;
+      double vf = 0.0;
+      int64_t vi = 0;
+      decode_value(s, b, E.fd[a], valid, vf, vi);
+      if (E.valid_out[a]) E.valid_out[a][j] = valid;
+      if (!E.val_out[a]) continue;
+      if ((E.want_float >> a) & 1u) ((double*)E.val_out[a])[j] = valid ? vf : 0.0;
+      else ((int64_t*)E.val_out[a])[j] = valid ? vi : 0;
     }
   }
 }
@@ -1443,6 +1496,36 @@ wg_status wg_agg_fetch(wg_agg* h, uint64_t offset, uint64_t count, void* const* 
            (unsigned long long)offset, (unsigned long long)count, (unsigned long long)h->out_n);
   if (count == 0) return WG_OK;
   const bool host = mem == WG_MEM_HOST;
+  if (!host && (int)h->finals.size() <= kMaxOps) {
+    // device columns: one pass over the groups writes every output
+    const KeyDesc& K = h->wide ? h->wkey : h->key;
+    EmitParams E{};
+    E.dense = h->dense;
+    E.dense_base = h->dense_base;
+    E.dense_null = h->dense_null;
+    E.dict = h->wide ? (const uint64_t*)h->dict.keys : nullptr;
+    E.off = offset;
+    E.cnt = count;
+    E.stride = h->pg.stride;
+    E.n_sumd = h->n_sumd;
+    E.K = K;
+    E.n_keys = K.n;
+    for (int k = 0; k < K.n; ++k) E.key_out[k] = key_out ? key_out[k] : nullptr;
+    E.n_aggs = (int)h->finals.size();
+    for (int a = 0; a < E.n_aggs; ++a) {
+      const AggFinal& f = h->finals[a];
+      E.fd[a] = FinalDesc{f.func, f.count_word, f.val_kind, f.val_word, f.is_float_in, f.sumd_index, f.scale, f.limbs};
+      E.val_out[a] = val_out ? val_out[a] : nullptr;
+      E.valid_out[a] = valid_out ? valid_out[a] : nullptr;
+      const bool fl = f.func == WG_AGG_SUM || f.func == WG_AGG_AVG ||
+                      ((f.func == WG_AGG_MIN || f.func == WG_AGG_MAX) && f.is_float_in);
+      E.want_float |= (uint32_t)fl << a;
+    }
+    hipLaunchKernelGGL(agg_emit_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, h->ctx->stream, E);
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipStreamSynchronize(h->ctx->stream));
+    return WG_OK;
+  }
   void* tmp = nullptr;
   uint8_t* tmpv = nullptr;
   if (host) {

@@ -322,6 +322,25 @@ class HashAggregate:
             call("wg_agg_fetch_key_valid", self.h, 0, n, kvalid.ctypes.data_as(C.c_void_p), L.MEM_HOST)
         return dict(n=n, keys=keys, key_valid=kvalid.astype(bool), values=vals, valid=[v.astype(bool) for v in valid])
 
+    def output_buffers(self, n: int):
+        """Device buffers for n groups of every output column: (keys, values, valid)."""
+        ks = [self.ctx.alloc(max(n, 1) * np.dtype(NP_OF[self.types[k]]).itemsize) for k in self.keys]
+        vs = [self.ctx.alloc(max(n, 1) * 8) for _ in self.aggs]
+        vv = [self.ctx.alloc(max(n, 1)) for _ in self.aggs]
+        return ks, vs, vv
+
+    def fetch_device(self, keys, vals, valid, offset: int = 0, count: Optional[int] = None):
+        """Groups [offset, offset + count) into device buffers (None skips a
+        column): keys in their input type, values as float64 / int64, validity
+        bytes; one pass over the groups (the Next drain on the device)."""
+        count = self.num_groups() - offset if count is None else count
+        ptr = (lambda b: None if b is None else b.ptr)
+        kp = (C.c_void_p * max(len(keys), 1))(*[ptr(k) for k in keys])
+        vp = (C.c_void_p * max(len(vals), 1))(*[ptr(v) for v in vals])
+        vv = (C.c_void_p * max(len(valid), 1))(*[ptr(v) for v in valid])
+        if count:
+            call("wg_agg_fetch", self.h, offset, count, kp, vp, vv, L.MEM_DEVICE)
+
     def fetch_state(self, agg: int) -> dict:
         n = self.num_groups()
         cnt = np.empty(n, np.int64)
