@@ -413,11 +413,12 @@ __device__ __forceinline__ double i128_to_f64(uint64_t lo, int64_t hi, int scale
   return f / p10;
 }
 
-// Finalized value of one aggregate of the group whose slot words are s and
-// whose exact Int128 bases start at b (n_sumd pairs).
-__device__ __forceinline__ void decode_value(const uint64_t* s, const uint64_t* b, const FinalDesc& fd,
-                                             uint8_t& valid, double& vf, int64_t& vi) {
-    const int64_t count = (int64_t)s[fd.count_word];
+// Finalized value of one aggregate of a group: word(i) returns slot word i,
+// b points at the group's exact Int128 bases (n_sumd pairs).
+template <class Word>
+__device__ __forceinline__ void decode_value(Word word, const uint64_t* b, const FinalDesc& fd, uint8_t& valid,
+                                             double& vf, int64_t& vi) {
+    const int64_t count = (int64_t)word(fd.count_word);
     switch (fd.func) {
       case WG_AGG_COUNT:
         vi = count;
@@ -427,14 +428,19 @@ __device__ __forceinline__ void decode_value(const uint64_t* s, const uint64_t* 
         if (count == 0 || fd.val_kind == 0) { valid = 0; break; }
         double sum;
         if (fd.val_kind == OP_SUMI) {
-          vi = (int64_t)s[fd.val_word];
+          vi = (int64_t)word(fd.val_word);
           sum = (double)vi;
         } else if (fd.val_kind == OP_SUMF) {
-          sum = __longlong_as_double((long long)s[fd.val_word]);
+          sum = __longlong_as_double((long long)word(fd.val_word));
         } else {
+          uint64_t l[4] = {word(fd.val_word), word(fd.val_word + 1), 0, 0};
+          if (fd.limbs == 4) {
+            l[2] = word(fd.val_word + 2);
+            l[3] = word(fd.val_word + 3);
+          }
           uint64_t lo;
           int64_t hi;
-          limbs_value(s + fd.val_word, fd.limbs, lo, hi);
+          limbs_value(l, fd.limbs, lo, hi);
           const uint64_t* bb = b + fd.sumd_index * 2;
           unsigned __int128 v = (((unsigned __int128)(uint64_t)hi << 64) | lo) + (((unsigned __int128)bb[1] << 64) | bb[0]);
           sum = i128_to_f64((uint64_t)v, (int64_t)(uint64_t)(v >> 64), fd.scale);
@@ -445,7 +451,7 @@ __device__ __forceinline__ void decode_value(const uint64_t* s, const uint64_t* 
       case WG_AGG_MIN:
       case WG_AGG_MAX: {
         if (count == 0 || fd.val_kind == 0) { valid = 0; break; }
-        const uint64_t e = s[fd.val_word];
+        const uint64_t e = word(fd.val_word);
         const uint64_t init = fd.func == WG_AGG_MIN ? kMinInit : kMaxInit;
         if (fd.is_float_in) {
           if (e == init) vf = __longlong_as_double(0x7FF8000000000000ll);  // only NaN inputs
@@ -465,7 +471,8 @@ __global__ void agg_decode_kernel(const uint64_t* dense, const uint64_t* dense_b
     uint8_t valid = 1;
     double vf = 0.0;
     int64_t vi = 0;
-    decode_value(dense + (off + j) * stride, dense_base + (off + j) * n_sumd * 2, fd, valid, vf, vi);
+    const uint64_t* row = dense + (off + j) * stride;
+    decode_value([row](int i) { return row[i]; }, dense_base + (off + j) * n_sumd * 2, fd, valid, vf, vi);
     if (out_valid) out_valid[j] = valid;
     if (want_float) {
       if (out_f) out_f[j] = valid ? vf : 0.0;
@@ -525,20 +532,34 @@ struct EmitParams {
   uint32_t want_float;  // bit a: aggregate a emits float64
 };
 
+template <int S>
 __global__ void __launch_bounds__(256) agg_emit_kernel(EmitParams E) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < E.cnt; j += (uint64_t)gridDim.x * blockDim.x) {
-    // the row's first read brings its line in; the other words of the row hit the L1 cache
-    const uint64_t* s = E.dense + (E.off + j) * E.stride;
+    // the group's slot words in registers, read once (S / 2 16-byte loads);
+    // word i is picked by selects, so no word goes through scratch memory
+    const uint64_t* s = E.dense + (E.off + j) * S;
+    uint64_t w[S];
+#pragma unroll
+    for (int i = 0; i < S; i += 2) {
+      const ulonglong2 x = *(const ulonglong2*)(s + i);
+      w[i] = x.x;
+      w[i + 1] = x.y;
+    }
+    auto word = [&w](int i) {
+      uint64_t r = 0;
+#pragma unroll
+      for (int k = 0; k < S; ++k) r = i == k ? w[k] : r;
+      return r;
+    };
     const bool is_null = E.dense_null[E.off + j] != 0;
-    const uint64_t key = s[0];
     for (int k = 0; k < E.n_keys; ++k)
-      if (E.key_out[k]) store_key(E.key_out[k], j, E.K.key_type[k], decode_key(key, is_null, E.K, k, E.dict));
+      if (E.key_out[k]) store_key(E.key_out[k], j, E.K.key_type[k], decode_key(w[0], is_null, E.K, k, E.dict));
     const uint64_t* b = E.dense_base + (E.off + j) * E.n_sumd * 2;
     for (int a = 0; a < E.n_aggs; ++a) {
       uint8_t valid = 1;
       double vf = 0.0;
       int64_t vi = 0;
-      decode_value(s, b, E.fd[a], valid, vf, vi);
+      decode_value(word, b, E.fd[a], valid, vf, vi);
       if (E.valid_out[a]) E.valid_out[a][j] = valid;
       if (!E.val_out[a]) continue;
       if ((E.want_float >> a) & 1u) ((double*)E.val_out[a])[j] = valid ? vf : 0.0;
@@ -1521,7 +1542,13 @@ wg_status wg_agg_fetch(wg_agg* h, uint64_t offset, uint64_t count, void* const* 
                       ((f.func == WG_AGG_MIN || f.func == WG_AGG_MAX) && f.is_float_in);
       E.want_float |= (uint32_t)fl << a;
     }
-    hipLaunchKernelGGL(agg_emit_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, h->ctx->stream, E);
+    const dim3 grid(grid_for(count, 256, 16384)), block(256);
+    switch (h->pg.stride) {
+      case 2: hipLaunchKernelGGL(agg_emit_kernel<2>, grid, block, 0, h->ctx->stream, E); break;
+      case 4: hipLaunchKernelGGL(agg_emit_kernel<4>, grid, block, 0, h->ctx->stream, E); break;
+      case 8: hipLaunchKernelGGL(agg_emit_kernel<8>, grid, block, 0, h->ctx->stream, E); break;
+      default: hipLaunchKernelGGL(agg_emit_kernel<16>, grid, block, 0, h->ctx->stream, E); break;
+    }
     WG_HIP(hipGetLastError());
     WG_HIP(hipStreamSynchronize(h->ctx->stream));
     return WG_OK;

@@ -4,6 +4,10 @@
         per-kernel dispatch count / total / average / min / max duration of a
         `rocprofv3 --kernel-trace --stats` run (its *_results.db, or the
         *_kernel_stats.csv of --output-format csv)
+    python -m wadjet_amd.profiling counters <dir>... [--kernel K]... --command "<cmd>"
+        raw counters per kernel from several `rocprofv3 --pmc` runs (LDS bank
+        conflicts, L2 hits and atomics, HBM read / write request counts and
+        the bytes they move), with derived ratios
     python -m wadjet_amd.profiling pmc <FETCH_SIZE dir> <WRITE_SIZE dir>
             --copy-bytes B --steps K --command "<cmd>" --command "<cmd>"
         per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
@@ -156,6 +160,60 @@ def load_traffic(path: str, match, steps_key="# steps profiled:"):
     return total / steps if found else None
 
 
+def pmc_all(dirs):
+    """{kernel: {counter: value}} over several --pmc runs (each counter from
+    the run that collected it; the last dispatch of each kernel)."""
+    out = defaultdict(dict)
+    for d in dirs:
+        for path in _files(d, "_results.db"):
+            con = sqlite3.connect(path)
+            per = defaultdict(lambda: defaultdict(float))
+            for name, disp, c, v in con.execute("select name, dispatch_id, counter_name, counter_value "
+                                                "from pmc_events"):
+                per[(short_name(name), disp)][c] += float(v)
+            con.close()
+            last = {}
+            for (k, disp), cv in per.items():
+                if k not in last or disp > last[k][0]:
+                    last[k] = (disp, cv)
+            for k, (_, cv) in last.items():
+                out[k].update(cv)
+    return out
+
+
+DERIVED = [
+    ("LDS bank-conflict cycles / LDS active cycles", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"),
+    ("LDS wait cycles / wave cycles", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES"),
+    ("L2 hit rate (hits / requests)", "TCC_HIT_sum", "TCC_REQ_sum"),
+]
+
+
+def format_counters(data, kernels, commands, notes=()):
+    out = ["# rocprofv3 --pmc counters per kernel (last dispatch of each kernel in the run)",
+           "# command(s), run from the repository root:"]
+    out += [f"#   {c}" for c in commands]
+    out += [f"# {n}" for n in notes]
+    out.append("# bytes from requests: HBM read = TCC_EA0_RDREQ_128B x 128 + the other read requests x 64;"
+               " HBM write = TCC_EA0_WRREQ_64B x 64 + the other write requests x 32")
+    for k in sorted(data):
+        if kernels and not any(p in k for p in kernels):
+            continue
+        cv = data[k]
+        out.append(k[:100])
+        for c in sorted(cv):
+            out.append(f"   {c:34s} {cv[c]:18.0f}")
+        if "TCC_EA0_RDREQ_sum" in cv and "TCC_EA0_RDREQ_128B_sum" in cv:
+            rd = cv["TCC_EA0_RDREQ_128B_sum"] * 128 + (cv["TCC_EA0_RDREQ_sum"] - cv["TCC_EA0_RDREQ_128B_sum"]) * 64
+            out.append(f"   {'= HBM read bytes':34s} {rd:18.0f}")
+        if "TCC_EA0_WRREQ_sum" in cv and "TCC_EA0_WRREQ_64B_sum" in cv:
+            wr = cv["TCC_EA0_WRREQ_64B_sum"] * 64 + (cv["TCC_EA0_WRREQ_sum"] - cv["TCC_EA0_WRREQ_64B_sum"]) * 32
+            out.append(f"   {'= HBM write bytes':34s} {wr:18.0f}")
+        for label, a, b in DERIVED:
+            if a in cv and b in cv and cv[b]:
+                out.append(f"   = {label}: {cv[a] / cv[b]:.3f}")
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -168,8 +226,15 @@ def main():
     p.add_argument("--copy-bytes", type=int, required=True)
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--command", action="append", default=[])
+    q = sub.add_parser("counters")
+    q.add_argument("dirs", nargs="+")
+    q.add_argument("--kernel", action="append", default=[])
+    q.add_argument("--command", action="append", default=[])
+    q.add_argument("--note", action="append", default=[])
     a = ap.parse_args()
-    if a.cmd == "stats":
+    if a.cmd == "counters":
+        print(format_counters(pmc_all(a.dirs), a.kernel, a.command, a.note), end="")
+    elif a.cmd == "stats":
         print(format_stats(kernel_stats(a.dir), a.command), end="")
     else:
         print(pmc_summary(a.fetch_dir, a.write_dir, a.copy_bytes, a.steps, a.command), end="")
