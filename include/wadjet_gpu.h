@@ -534,19 +534,34 @@ wg_status wg_batch_destroy(wg_batch* b);
  * until wg_batch_result. Per query: GROUP BY key SUM(val), COUNT(*). */
 wg_status wg_batch_run(wg_batch* b, const wg_batch_query* qs, int32_t n_queries,
                        int32_t use_graph, int32_t plain);
-/* Autotune: time the candidate layouts/workgroup sizes on these queries and
- * keep the fastest per (rows, groups, key width) bucket. */
+/* Autotune: time candidate tunings on these queries (device time of the
+ * launch, HIP events) and keep the fastest: workgroup size, LDS threshold
+ * (queries estimated at or below it aggregate in an LDS table per tile,
+ * larger ones in their global table), rows per work item, and the partition
+ * fan-out (a query above the threshold whose groups split into at most that
+ * many key-hash partitions of at most the threshold each becomes one work
+ * item per (tile, partition), each aggregating its partition in LDS), and the
+ * LDS cache of the queries still above it. */
 wg_status wg_batch_autotune(wg_batch* b, const wg_batch_query* qs, int32_t n_queries);
 /* Per-query result: number of groups, then keys/sums(lo,hi as Int128)/counts. */
 wg_status wg_batch_result(wg_batch* b, int32_t q, uint64_t* n_groups, int64_t* keys,
                           uint64_t* sum_lo, int64_t* sum_hi, int64_t* counts, uint64_t cap);
-/* Chosen layout for query q: 1 = LDS table, 0 = global table; workgroup size. */
-wg_status wg_batch_choice(wg_batch* b, int32_t q, int32_t* use_lds, int32_t* block);
+/* Chosen layout for query q in the last run: 1 = LDS table(s) holding all
+ * its groups, 2 = global table behind an LDS cache of its frequent keys,
+ * 0 = global table only; workgroup size; key partitions per tile (1 = no
+ * fan-out). */
+wg_status wg_batch_choice(wg_batch* b, int32_t q, int32_t* use_lds, int32_t* block, int32_t* fanout);
 /* Current tuning (what wg_batch_autotune picked, or what was set): workgroup
- * size, LDS cardinality threshold (queries estimated at or below it use an
- * LDS table; 0 = never), rows per work item. */
-wg_status wg_batch_tuning(wg_batch* b, int32_t* block, int32_t* lds_threshold, uint64_t* tile_rows);
-wg_status wg_batch_set_tuning(wg_batch* b, int32_t block, int32_t lds_threshold, uint64_t tile_rows);
+ * size, LDS cardinality threshold (0 = never an LDS table), rows per work
+ * item (an upper bound: a batch halves it, down to 8K rows, until it has four
+ * work items per compute unit), largest partition fan-out (a power of two
+ * <= 64; 1 = none), and
+ * whether queries still above the threshold keep an LDS table as a cache of
+ * their frequent keys (skewed keys: the hot keys' rows aggregate in LDS). */
+wg_status wg_batch_tuning(wg_batch* b, int32_t* block, int32_t* lds_threshold, uint64_t* tile_rows,
+                          int32_t* max_fanout, int32_t* lds_cache);
+wg_status wg_batch_set_tuning(wg_batch* b, int32_t block, int32_t lds_threshold, uint64_t tile_rows,
+                              int32_t max_fanout, int32_t lds_cache);
 
 /* ---------------------------------------------------------------- serving */
 

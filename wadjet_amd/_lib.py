@@ -138,9 +138,9 @@ SIGNATURES: dict[str, list] = {
     "wg_batch_run": [P, C.POINTER(BatchQuery), I32, I32, I32],
     "wg_batch_autotune": [P, C.POINTER(BatchQuery), I32],
     "wg_batch_result": [P, I32, C.POINTER(U64), P, P, P, P, U64],
-    "wg_batch_choice": [P, I32, C.POINTER(I32), C.POINTER(I32)],
-    "wg_batch_tuning": [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(U64)],
-    "wg_batch_set_tuning": [P, I32, I32, U64],
+    "wg_batch_choice": [P, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)],
+    "wg_batch_tuning": [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(U64), C.POINTER(I32), C.POINTER(I32)],
+    "wg_batch_set_tuning": [P, I32, I32, U64, I32, I32],
     "wg_serve_create": [P, C.POINTER(Column), I32, U64, I32, I32, U64, I32, I32, C.POINTER(P)],
     "wg_serve_destroy": [P],
     "wg_serve_add_dim": [P, P, C.POINTER(I32)],

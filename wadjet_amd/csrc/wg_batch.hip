@@ -37,6 +37,8 @@ struct BQ {
   uint64_t max_probe;  // a claim farther than this from the key's home slot overflows
   uint32_t lds_slots;  // 0 = global table only
   int32_t dim_word;    // >= 0: group key = this word of the matched dimension slot
+  uint32_t fan_bits;   // the query's keys split over 2^fan_bits work items per tile (top hash bits)
+  uint32_t pad;
   ColDesc range;       // data == null: no window filter
   int64_t range_lo, range_hi;
   JoinTable dim;       // slots == null: no join
@@ -44,7 +46,7 @@ struct BQ {
 
 struct BWork {
   uint32_t q;
-  uint32_t pad;
+  uint32_t part;  // key partition of a fanned-out query (0 otherwise)
   uint64_t beg, end;
 };
 
@@ -91,6 +93,13 @@ __device__ __forceinline__ void bq_apply(uint64_t* s, uint32_t ws, int64_t v) {
   at_add<S>(s + 3 * ws, (uint64_t)(v >> 32));
 }
 
+constexpr int kBU = 4;  // rows per thread per step, their loads all in flight together
+
+// One work item = rows [beg, end) of one query, and with fan-out only the
+// rows whose group key hashes to the item's partition: the LDS table then
+// holds 1 / 2^fan_bits of the query's groups. Rows go to the LDS table when
+// it has one (flushed into the query's table at the end of the item with
+// one atomic per group word), else straight to the query's table.
 __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
   extern __shared__ uint64_t lds[];
   __shared__ int64_t s_item;
@@ -112,33 +121,65 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
       for (uint32_t i = threadIdx.x; i < L * kBStride; i += blockDim.x) lds[i] = i < L ? kEmpty : 0;
       __syncthreads();
     }
-    for (uint64_t r = W.beg + threadIdx.x; r < W.end; r += blockDim.x) {
-      if (!col_valid(q.key, r) || !col_valid(q.val, r)) continue;  // NULL keys / values drop out
-      if (q.range.data) {
-        if (!col_valid(q.range, r)) continue;
-        const int64_t t = load_int(q.range, r);
-        if (t < q.range_lo || t >= q.range_hi) continue;
+    const ColRes kc = resolve_col(q.key, true), vc = resolve_col(q.val, true);
+    const bool ranged = q.range.data != nullptr;
+    const ColRes rc = resolve_col(q.range, ranged);
+    const uint32_t fsh = 64 - q.fan_bits;
+    for (uint64_t base = W.beg; base < W.end; base += (uint64_t)blockDim.x * kBU) {
+      uint64_t row[kBU];
+      bool in[kBU];
+#pragma unroll
+      for (int u = 0; u < kBU; ++u) {
+        const uint64_t r = base + (uint64_t)u * blockDim.x + threadIdx.x;
+        in[u] = r < W.end;
+        row[u] = in[u] ? r : W.beg;
       }
-      uint64_t key = bq_key(q, r);
-      if (q.dim.slots) {
-        const uint64_t* js = join_find(q.dim, key);
-        if (!js) continue;
-        key = js[q.dim_word];
+      // every load of the step in flight before any value is used
+      RawVal kr[kBU], vr[kBU], rr[kBU];
+      uint64_t kv[kBU], vv[kBU], rv[kBU];
+#pragma unroll
+      for (int u = 0; u < kBU; ++u) {
+        kr[u] = load_res(kc, row[u]);
+        vr[u] = load_res(vc, row[u]);
+        rr[u] = load_res(rc, row[u]);
+        kv[u] = valid_res(kc, row[u]);
+        vv[u] = valid_res(vc, row[u]);
+        rv[u] = valid_res(rc, row[u]);
       }
-      const int64_t v = load_int(q.val, r);
-      if (L && key != kEmpty) {
-        uint64_t* ls = lds_find_or_claim(lds, (int)L - 1, key, mix64(key));
-        if (ls) {
-          bq_apply<LdsT>(ls, L, v);
+      uint64_t key[kBU], val[kBU], tval[kBU];
+      values_of<kBU>(kc.type, kr, key);
+      values_of<kBU>(vc.type, vr, val);
+      values_of<kBU>(rc.type, rr, tval);
+#pragma unroll
+      for (int u = 0; u < kBU; ++u) {
+        const uint64_t bit = row[u] & 63;
+        // NULL keys / values drop out, as do rows outside the window
+        bool ok = in[u] && ((kv[u] >> bit) & 1ull) && ((vv[u] >> bit) & 1ull);
+        if (ranged) ok = ok && ((rv[u] >> bit) & 1ull) && (int64_t)tval[u] >= q.range_lo && (int64_t)tval[u] < q.range_hi;
+        if (!ok) continue;
+        uint64_t k = key[u];
+        if (q.dim.slots) {
+          const uint64_t* js = join_find(q.dim, k);
+          if (!js) continue;
+          k = js[q.dim_word];
+        }
+        const uint64_t h = mix64(k);
+        if (q.fan_bits && (uint32_t)(h >> fsh) != W.part) continue;  // another item's partition
+        const int64_t v = (int64_t)val[u];
+        if (L && k != kEmpty) {
+          uint64_t* ls = lds_find_or_claim(lds, (int)L - 1, k, h);
+          if (ls) {
+            bq_apply<LdsT>(ls, L, v);
+            continue;
+          }
+        }
+        uint64_t* s = bq_slot(q, k);
+        if (!s) {
+          atomicAdd(&c->overflow, 1ull);  // rare: the query is re-run alone
           continue;
         }
+        bq_apply<GlbT>(s, 1, v);
       }
-      uint64_t* s = bq_slot(q, key);
-      if (!s) {
-        atomicAdd(&c->overflow, 1ull);  // rare: the query is re-run alone
-        continue;
-      }
-      bq_apply<GlbT>(s, 1, v);
     }
     if (L) {
       __syncthreads();
@@ -245,6 +286,33 @@ __global__ void batch_fill_kernel(uint64_t* arena, const uint64_t* n_words, BCou
   if (blockIdx.x == 0 && threadIdx.x < 2) work_ctrs[threadIdx.x] = 0;
 }
 
+// Every query's compacted results packed densely for one copy to the host:
+// query i's n_i groups land at dst[i] as four runs of n_i words (keys, sum lo,
+// sum hi, counts); an integer SUM keeps only its wrapped low word, so its hi
+// word is the low word's sign (scatterSumInt).
+struct PackItem {
+  uint64_t src_off;  // the query's output region in the arena's outputs
+  uint64_t n;
+  uint64_t dst;      // word offset of its four runs in the packed buffer
+  uint32_t wrap;     // 1: integer SUM
+  uint32_t pad;
+};
+
+__global__ void batch_pack_kernel(const PackItem* items, uint32_t n_items, const int64_t* keys, const uint64_t* lo,
+                                  const int64_t* hi, const int64_t* counts, uint64_t* out) {
+  for (uint32_t q = blockIdx.y; q < n_items; q += gridDim.y) {
+    const PackItem it = items[q];
+    uint64_t* d = out + it.dst;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < it.n; i += (uint64_t)gridDim.x * blockDim.x) {
+      const uint64_t l = lo[it.src_off + i];
+      d[i] = (uint64_t)keys[it.src_off + i];
+      d[it.n + i] = l;
+      d[2 * it.n + i] = it.wrap ? ((int64_t)l < 0 ? ~0ull : 0ull) : (uint64_t)hi[it.src_off + i];
+      d[3 * it.n + i] = (uint64_t)counts[it.src_off + i];
+    }
+  }
+}
+
 }  // namespace wg
 
 // ================================================================ host side
@@ -271,6 +339,11 @@ struct BatchArena {
   int64_t* d_counts = nullptr;
   void* pinned = nullptr;  // descriptor staging + counter read-back
   uint64_t pinned_bytes = 0;
+  PackItem* pack_items = nullptr;  // pinned: the pack table of the last download
+  PackItem* d_pack_items = nullptr;
+  uint64_t pack_cap = 0;
+  uint64_t* d_packed = nullptr;  // packed results of the last download
+  uint64_t packed_cap = 0;
   // A captured graph holds the device pointers above as kernel arguments:
   // every reallocation bumps `gen`, and a graph of an older generation is
   // re-captured before it is replayed.
@@ -285,8 +358,10 @@ struct Plan {
   std::vector<BQ> qs;
   std::vector<uint64_t> out_off;
   std::vector<int32_t> use_lds;
+  std::vector<uint32_t> fanout;
   std::vector<BWork> work;
   std::vector<CWork> cwork;
+  uint64_t tile_rows = 0;  // rows per work item this batch uses
   uint64_t arena_words = 0;
   uint32_t lds_bytes = 0;
 };
@@ -312,6 +387,8 @@ struct wg_batch {
   uint32_t lds_threshold = 1024;
   uint32_t lds_max_slots = 2048;
   uint64_t tile_rows = 16384;
+  uint32_t max_fanout = 1;  // key partitions per tile a query above the threshold may use
+  uint32_t lds_cache = 1;   // queries still above the threshold keep an LDS table as a cache of frequent keys
   bool tuned = false;
 };
 
@@ -343,10 +420,11 @@ static wg_status arena_init(BatchArena& A, uint64_t slots, uint64_t max_q) {
 static void arena_free(BatchArena& A) {
   if (A.graph) (void)hipGraphExecDestroy(A.graph);
   void* ps[] = {A.arena, A.d_qs, A.d_work, A.d_cwork, A.d_scalars, A.d_work_ctrs, A.d_ctr,
-                A.d_keys, A.d_sum_lo, A.d_sum_hi, A.d_counts};
+                A.d_keys, A.d_sum_lo, A.d_sum_hi, A.d_counts, A.d_pack_items, A.d_packed};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (A.pinned) (void)hipHostFree(A.pinned);
+  if (A.pack_items) (void)hipHostFree(A.pack_items);
   A = BatchArena{};
 }
 
@@ -356,6 +434,7 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
   P.qs.assign(n, BQ{});
   P.out_off.assign(n, 0);
   P.use_lds.assign(n, 0);
+  P.fanout.assign(n, 1);
   uint64_t slot_off = 0;
   uint32_t max_lds = 0;
   for (int i = 0; i < n; ++i) {
@@ -388,9 +467,24 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
     d.max_probe = kMaxProbe;
     P.out_off[i] = slot_off;  // output region = the query's slot range
     slot_off += cap + 1;
-    const bool lds = b->lds_threshold > 0 && est <= b->lds_threshold;
-    P.use_lds[i] = lds;
-    d.lds_slots = lds ? (uint32_t)std::min<uint64_t>(table_cap_for(est), b->lds_max_slots) : 0;
+    // an LDS table for queries up to the threshold; above it, a query whose
+    // groups split over at most max_fanout partitions of at most the
+    // threshold each (and that is not a join: its group key needs the
+    // dimension lookup first) fans out into that many items per tile
+    uint32_t fan_bits = 0;
+    if (b->lds_threshold > 0 && est > b->lds_threshold && !q.dim)
+      while ((1u << fan_bits) < b->max_fanout && (est >> fan_bits) > b->lds_threshold) ++fan_bits;
+    const uint64_t per_item = est >> fan_bits;
+    const bool lds = b->lds_threshold > 0 && per_item <= b->lds_threshold;
+    if (!lds) fan_bits = 0;
+    d.fan_bits = fan_bits;
+    P.use_lds[i] = lds ? 1 : b->lds_cache ? 2 : 0;
+    P.fanout[i] = 1u << fan_bits;
+    // an LDS table that cannot hold the query's groups still catches its
+    // frequent keys (they claim slots first): rows of keys that find no slot
+    // go to the query's table, the cached partials are flushed once per item
+    d.lds_slots = lds ? (uint32_t)std::min<uint64_t>(table_cap_for(per_item), b->lds_max_slots)
+                      : b->lds_cache ? b->lds_max_slots : 0;
     max_lds = std::max(max_lds, d.lds_slots);
   }
   WG_CHECK(slot_off <= A.slots, WG_ERR_OOM, "batch needs %llu table slots, the arena holds %llu",
@@ -398,6 +492,15 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
   for (int i = 0; i < n; ++i) P.qs[i].table = A.arena + P.out_off[i] * kBStride;
   P.arena_words = slot_off * kBStride;
   P.lds_bytes = max_lds * kBStride * 8;
+  // rows per work item: the tuned size, halved (down to 8K rows) until the
+  // batch has at least four items per compute unit, so a batch of a few
+  // queries still spreads over the whole device
+  uint64_t rows_items = 0;
+  for (int i = 0; i < n; ++i) rows_items += qs[i].n_rows * P.fanout[i];
+  uint64_t tile = b->tile_rows;
+  const uint64_t want_items = 4ull * (uint64_t)std::max(b->ctx->cus, 1);
+  while (tile > 8192 && rows_items / tile < want_items) tile >>= 1;
+  P.tile_rows = tile;
   // tiles, largest queries first so their tails do not trail the batch
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
@@ -405,8 +508,9 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
   P.work.clear();
   for (int i : order) {
     const uint64_t r0 = qs[i].row_begin, r1 = qs[i].row_begin + qs[i].n_rows;
-    for (uint64_t r = r0; r < r1; r += b->tile_rows)
-      P.work.push_back(BWork{(uint32_t)i, 0, r, std::min<uint64_t>(r1, r + b->tile_rows)});
+    for (uint64_t r = r0; r < r1; r += tile)
+      for (uint32_t f = 0; f < P.fanout[i]; ++f)  // a tile's partitions back to back: its rows stay in cache
+        P.work.push_back(BWork{(uint32_t)i, f, r, std::min<uint64_t>(r1, r + tile)});
   }
   P.cwork.clear();
   for (int i = 0; i < n; ++i) {
@@ -589,6 +693,63 @@ wg_status wg_batch_poll(wg_batch* b, int32_t* finished, int32_t wait) {
   return WG_OK;
 }
 
+// 1 when query q of the last batch was re-run alone (its results on the host).
+int32_t wg_batch_rerun_alone(const wg_batch* b, int32_t q) {
+  return q >= 0 && q < (int)b->fallback.size() ? b->fallback[q] : 0;
+}
+
+// After wg_batch_poll reported the batch finished: enqueue on the runner's
+// stream one pack of every query's results and one copy of them into `host`
+// (pinned, host_words words); query i's runs start at word offsets[i] (four
+// runs of n_groups[i] words: keys, sum lo, sum hi, counts). Queries that were
+// re-run alone (their results already on the host) get n = 0 here and are
+// read with wg_batch_result. WG_ERR_OOM (nothing enqueued) when `host` is too
+// small; *words = the words needed.
+wg_status wg_batch_download_async(wg_batch* b, uint64_t* host, uint64_t host_words, uint64_t* offsets,
+                                  uint64_t* words) {
+  WG_CHECK(!b->in_flight, WG_ERR_STATE, "batch still running");
+  BatchArena& A = b->main;
+  const int n = (int)b->n_groups.size();
+  uint64_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    offsets[i] = total;
+    if (!b->fallback[i]) total += 4 * b->n_groups[i];
+  }
+  *words = total;
+  WG_CHECK(total <= host_words, WG_ERR_OOM, "results need %llu words, the buffer holds %llu",
+           (unsigned long long)total, (unsigned long long)host_words);
+  if (total == 0) return WG_OK;
+  if (A.pack_cap < (uint64_t)n) {
+    if (A.pack_items) WG_HIP(hipHostFree(A.pack_items));
+    if (A.d_pack_items) WG_HIP(hipFree(A.d_pack_items));
+    A.pack_items = nullptr;
+    A.d_pack_items = nullptr;
+    A.pack_cap = 0;
+    WG_HIP(hipHostMalloc((void**)&A.pack_items, n * sizeof(PackItem), hipHostMallocDefault));
+    WG_HIP(hipMalloc(&A.d_pack_items, n * sizeof(PackItem)));
+    A.pack_cap = n;
+  }
+  if (A.packed_cap < total) {
+    if (A.d_packed) WG_HIP(hipFree(A.d_packed));
+    A.d_packed = nullptr;
+    A.packed_cap = 0;
+    WG_HIP(hipMalloc(&A.d_packed, total * 8));
+    A.packed_cap = total;
+  }
+  // the pack table of an earlier download is no longer read: downloads on this
+  // stream run in order, and the caller waits for each before the next
+  for (int i = 0; i < n; ++i)
+    A.pack_items[i] = PackItem{b->plan.out_off[i], b->fallback[i] ? 0 : b->n_groups[i], offsets[i],
+                               b->queries[i].val.type != WG_DECIMAL ? 1u : 0u, 0};
+  WG_HIP(hipMemcpyAsync(A.d_pack_items, A.pack_items, n * sizeof(PackItem), hipMemcpyHostToDevice, b->ctx->stream));
+  hipLaunchKernelGGL(batch_pack_kernel, dim3(64, std::min(n, 1024)), dim3(256), 0, b->ctx->stream,
+                     (const PackItem*)A.d_pack_items, (uint32_t)n, (const int64_t*)A.d_keys, (const uint64_t*)A.d_sum_lo,
+                     (const int64_t*)A.d_sum_hi, (const int64_t*)A.d_counts, A.d_packed);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipMemcpyAsync(host, A.d_packed, total * 8, hipMemcpyDeviceToHost, b->ctx->stream));
+  return WG_OK;
+}
+
 extern "C" {
 
 wg_status wg_batch_create(wg_ctx* ctx, int32_t max_queries, uint64_t max_groups_total, wg_batch** out) {
@@ -633,9 +794,11 @@ wg_status wg_batch_run(wg_batch* b, const wg_batch_query* qs, int32_t n_queries,
     reset_results(b, n_queries);
     b->queries.assign(qs, qs + n_queries);
     b->plan.use_lds.assign(n_queries, 0);
+    b->plan.fanout.assign(n_queries, 1);
     for (int i = 0; i < n_queries; ++i) {
       WG_TRY(run_solo(b, i, qs[i], qs[i].est_groups));
-      b->plan.use_lds[i] = b->lds_threshold > 0 && std::max<uint64_t>(qs[i].est_groups, 8) <= b->lds_threshold;
+      b->plan.use_lds[i] = b->lds_threshold > 0 && std::max<uint64_t>(qs[i].est_groups, 8) <= b->lds_threshold ? 1
+                           : b->lds_cache ? 2 : 0;
     }
     return WG_OK;
   }
@@ -676,26 +839,30 @@ wg_status wg_batch_result(wg_batch* b, int32_t q, uint64_t* n_groups, int64_t* k
   return WG_OK;
 }
 
-wg_status wg_batch_choice(wg_batch* b, int32_t q, int32_t* use_lds, int32_t* block) {
+wg_status wg_batch_choice(wg_batch* b, int32_t q, int32_t* use_lds, int32_t* block, int32_t* fanout) {
   WG_CHECK(b, WG_ERR_INVALID, "null handle");
   WG_CHECK(q >= 0 && q < (int)b->plan.use_lds.size(), WG_ERR_INVALID, "query %d", q);
   if (use_lds) *use_lds = b->plan.use_lds[q];
   if (block) *block = b->block;
+  if (fanout) *fanout = q < (int)b->plan.fanout.size() ? (int32_t)b->plan.fanout[q] : 1;
   return WG_OK;
 }
 
-// Autotune: time (workgroup size x LDS threshold x tile size) candidates on
-// these queries with HIP events and keep the fastest.
+// Autotune: time (workgroup size x LDS threshold x tile size x partition
+// fan-out) candidates on these queries with HIP events and keep the fastest.
 wg_status wg_batch_autotune(wg_batch* b, const wg_batch_query* qs, int32_t n_queries) {
   clear_error();
   WG_CHECK(b && qs, WG_ERR_INVALID, "null argument");
   WG_CHECK(n_queries > 0 && n_queries <= b->max_q, WG_ERR_INVALID, "%d queries", n_queries);
   WG_CHECK(!b->in_flight, WG_ERR_STATE, "a batch is in flight");
-  struct Cand { int32_t block; uint32_t thr; uint64_t tile; };
+  struct Cand { int32_t block; uint32_t thr; uint64_t tile; uint32_t fan; uint32_t cache; };
   std::vector<Cand> cands;
   for (int32_t blk : {256, 512, 1024})
     for (uint32_t thr : {0u, 256u, 1024u, 2048u})
-      for (uint64_t tile : {8192ull, 32768ull}) cands.push_back(Cand{blk, thr, tile});
+      for (uint64_t tile : {8192ull, 32768ull, 65536ull})
+        for (uint32_t fan : {1u, 4u, 16u, 64u})
+          for (uint32_t cache : {0u, 1u})
+            if (thr > 0 || (fan == 1 && cache == 0)) cands.push_back(Cand{blk, thr, tile, fan, cache});
   hipEvent_t e0, e1;
   WG_HIP(hipEventCreate(&e0));
   WG_HIP(hipEventCreate(&e1));
@@ -706,6 +873,8 @@ wg_status wg_batch_autotune(wg_batch* b, const wg_batch_query* qs, int32_t n_que
     b->block = c.block;
     b->lds_threshold = c.thr;
     b->tile_rows = c.tile;
+    b->max_fanout = c.fan;
+    b->lds_cache = c.cache;
     WG_TRY(plan_into(b, qs, n_queries, b->main, P));
     float t = 0.f;
     for (int rep = 0; rep < 3; ++rep) {
@@ -728,27 +897,37 @@ wg_status wg_batch_autotune(wg_batch* b, const wg_batch_query* qs, int32_t n_que
   b->block = best_c.block;
   b->lds_threshold = best_c.thr;
   b->tile_rows = best_c.tile;
+  b->max_fanout = best_c.fan;
+  b->lds_cache = best_c.cache;
   b->tuned = true;
   return WG_OK;
 }
 
-wg_status wg_batch_tuning(wg_batch* b, int32_t* block, int32_t* lds_threshold, uint64_t* tile_rows) {
+wg_status wg_batch_tuning(wg_batch* b, int32_t* block, int32_t* lds_threshold, uint64_t* tile_rows,
+                          int32_t* max_fanout, int32_t* lds_cache) {
   WG_CHECK(b, WG_ERR_INVALID, "null handle");
   if (block) *block = b->block;
   if (lds_threshold) *lds_threshold = (int32_t)b->lds_threshold;
   if (tile_rows) *tile_rows = b->tile_rows;
+  if (max_fanout) *max_fanout = (int32_t)b->max_fanout;
+  if (lds_cache) *lds_cache = (int32_t)b->lds_cache;
   return WG_OK;
 }
 
-wg_status wg_batch_set_tuning(wg_batch* b, int32_t block, int32_t lds_threshold, uint64_t tile_rows) {
+wg_status wg_batch_set_tuning(wg_batch* b, int32_t block, int32_t lds_threshold, uint64_t tile_rows,
+                              int32_t max_fanout, int32_t lds_cache) {
   clear_error();
   WG_CHECK(b, WG_ERR_INVALID, "null handle");
   WG_CHECK(block == 256 || block == 512 || block == 1024, WG_ERR_INVALID, "block %d", block);
   WG_CHECK(lds_threshold >= 0 && lds_threshold <= 2048, WG_ERR_INVALID, "lds_threshold %d", lds_threshold);
   WG_CHECK(tile_rows >= 1024 && tile_rows <= (1u << 24), WG_ERR_INVALID, "tile_rows");
+  WG_CHECK(max_fanout >= 1 && max_fanout <= 64 && (max_fanout & (max_fanout - 1)) == 0, WG_ERR_INVALID,
+           "max_fanout %d (a power of two <= 64)", max_fanout);
   b->block = block;
   b->lds_threshold = (uint32_t)lds_threshold;
   b->tile_rows = tile_rows;
+  b->max_fanout = (uint32_t)max_fanout;
+  b->lds_cache = lds_cache ? 1u : 0u;
   return WG_OK;
 }
 

@@ -27,6 +27,9 @@
 // asynchronous batch API of wg_batch.hip
 wg_status wg_batch_submit_async(wg_batch* b, const wg_batch_query* qs, int32_t n, int32_t use_graph);
 wg_status wg_batch_poll(wg_batch* b, int32_t* finished, int32_t wait);
+wg_status wg_batch_download_async(wg_batch* b, uint64_t* host, uint64_t host_words, uint64_t* offsets,
+                                  uint64_t* words);
+int32_t wg_batch_rerun_alone(const wg_batch* b, int32_t q);
 
 namespace wg {
 
@@ -44,12 +47,27 @@ struct Request {
   std::vector<uint64_t> lo;
 };
 
+// Results of a finished batch on their way to the host: one packed copy into
+// one of the lane's two pinned buffers, the requests completed when its event
+// fires. The lane itself is free again as soon as the copy is enqueued.
+struct Download {
+  std::vector<Request*> reqs;
+  std::vector<uint64_t> off, n;  // word offset and group count of each request's runs
+  hipEvent_t ev = nullptr;
+  int buf = 0;
+};
+
 struct Lane {
   wg_ctx* ctx = nullptr;
   wg_batch* batch = nullptr;
   std::vector<Request*> reqs;
   uint64_t footprint = 0;
   bool busy = false;
+  uint64_t* host[2] = {nullptr, nullptr};  // pinned result buffers
+  uint64_t host_words[2] = {0, 0};
+  int next_buf = 0;
+  std::deque<Download> downloads;  // in flight, oldest first (at most two)
+  std::vector<hipEvent_t> free_events;
 };
 
 static uint64_t footprint_of(uint64_t est) {
@@ -130,29 +148,114 @@ static wg_status to_batch_query(wg_serve* s, const wg_serve_query& q, wg_batch_q
   return WG_OK;
 }
 
-static void complete_lane(wg_serve* s, Lane& L) {
-  const auto now = Clock::now();
-  for (size_t i = 0; i < L.reqs.size(); ++i) {
-    Request* r = L.reqs[i];
-    uint64_t n = 0;
-    wg_status st = wg_batch_result(L.batch, (int32_t)i, &n, nullptr, nullptr, nullptr, nullptr, 0);
-    if (st == WG_OK) {
-      r->keys.resize(n);
-      r->lo.resize(n);
-      r->hi.resize(n);
-      r->cnt.resize(n);
-      st = wg_batch_result(L.batch, (int32_t)i, &n, r->keys.data(), r->lo.data(), r->hi.data(), r->cnt.data(), n);
-    }
-    r->status = st;
+// Hand the results of the oldest download of L to its requests (the copy has
+// completed) and complete them.
+static void finish_download(wg_serve* s, Lane& L) {
+  Download d = std::move(L.downloads.front());
+  L.downloads.pop_front();
+  const uint64_t* h = L.host[d.buf];
+  for (size_t i = 0; i < d.reqs.size(); ++i) {
+    Request* r = d.reqs[i];
+    const uint64_t n = d.n[i];
+    if (r == nullptr) continue;  // completed already (re-run alone)
+    const uint64_t* p = h + d.off[i];
+    r->keys.assign((const int64_t*)p, (const int64_t*)p + n);
+    r->lo.assign(p + n, p + 2 * n);
+    r->hi.assign((const int64_t*)(p + 2 * n), (const int64_t*)(p + 3 * n));
+    r->cnt.assign((const int64_t*)(p + 3 * n), (const int64_t*)(p + 4 * n));
   }
+  L.free_events.push_back(d.ev);
   std::lock_guard<std::mutex> lk(s->mu);
-  for (Request* r : L.reqs) {
-    r->done_at = Clock::now();
+  const auto now = Clock::now();
+  for (Request* r : d.reqs) {
+    if (r == nullptr) continue;
+    r->done_at = now;
     r->done = true;
   }
-  (void)now;
-  s->in_use_fp -= L.footprint;
-  s->last_done = Clock::now();
+  s->last_done = now;
+  s->cv_done.notify_all();
+}
+
+// Complete the requests of a lane whose batch finished with status st, or
+// start their download; the lane is free for the next batch on return.
+static void retire_lane(wg_serve* s, Lane& L, wg_status st) {
+  Download d;
+  d.reqs = L.reqs;
+  d.off.assign(L.reqs.size(), 0);
+  d.n.assign(L.reqs.size(), 0);
+  std::vector<Request*> now_done;
+  if (st == WG_OK) {
+    // requests whose query was re-run alone hold their results on the host already
+    for (size_t i = 0; i < L.reqs.size(); ++i) {
+      uint64_t n = 0;
+      wg_status e = wg_batch_result(L.batch, (int32_t)i, &n, nullptr, nullptr, nullptr, nullptr, 0);
+      d.n[i] = n;
+      const bool alone = wg_batch_rerun_alone(L.batch, (int32_t)i) != 0;
+      if (e == WG_OK && n && alone) {
+        Request* r = L.reqs[i];
+        r->keys.resize(n);
+        r->lo.resize(n);
+        r->hi.resize(n);
+        r->cnt.resize(n);
+        e = wg_batch_result(L.batch, (int32_t)i, &n, r->keys.data(), r->lo.data(), r->hi.data(), r->cnt.data(), n);
+      }
+      if (e != WG_OK || alone) {
+        L.reqs[i]->status = e;
+        now_done.push_back(L.reqs[i]);
+        d.reqs[i] = nullptr;
+      }
+    }
+    // a pinned buffer free of undelivered results
+    if (L.downloads.size() >= 2) {
+      (void)hipEventSynchronize(L.downloads.front().ev);
+      finish_download(s, L);
+    }
+    d.buf = L.next_buf;
+    L.next_buf ^= 1;
+    uint64_t words = 0;
+    st = wg_batch_download_async(L.batch, L.host[d.buf], L.host_words[d.buf], d.off.data(), &words);
+    if (st == WG_ERR_OOM) {  // grow this buffer (no download uses it now)
+      (void)hipHostFree(L.host[d.buf]);
+      L.host[d.buf] = nullptr;
+      L.host_words[d.buf] = 0;
+      const uint64_t want = std::max<uint64_t>(words + words / 2, 1 << 20);
+      if (hipHostMalloc((void**)&L.host[d.buf], want * 8, hipHostMallocDefault) == hipSuccess) {
+        L.host_words[d.buf] = want;
+        st = wg_batch_download_async(L.batch, L.host[d.buf], L.host_words[d.buf], d.off.data(), &words);
+      }
+    }
+    if (st == WG_OK) {
+      if (L.free_events.empty()) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) st = WG_ERR_DEVICE;
+        else L.free_events.push_back(e);
+      }
+    }
+    if (st == WG_OK) {
+      d.ev = L.free_events.back();
+      L.free_events.pop_back();
+      if (hipEventRecord(d.ev, L.ctx->stream) != hipSuccess) st = WG_ERR_DEVICE;
+    }
+  }
+  if (st != WG_OK) {  // the batch or its download failed: every request fails
+    for (Request*& r : d.reqs)
+      if (r) {
+        r->status = st;
+        now_done.push_back(r);
+        r = nullptr;
+      }
+  } else {
+    L.downloads.push_back(std::move(d));
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (st != WG_OK) s->engine_error = st;
+  const auto now = Clock::now();
+  for (Request* r : now_done) {
+    r->done_at = now;
+    r->done = true;
+  }
+  s->in_use_fp -= L.footprint;  // the tables are packed away: the arena takes the next batch
+  if (!now_done.empty()) s->last_done = now;
   L.reqs.clear();
   L.footprint = 0;
   L.busy = false;
@@ -163,17 +266,19 @@ static void engine_loop(wg_serve* s) {
   (void)hipSetDevice(s->ctx->device);
   for (;;) {
     bool progressed = false;
-    // 1. retire finished lanes
+    // 1. retire finished lanes (their results start downloading), deliver
+    //    downloads that completed
     for (Lane& L : s->lanes) {
-      if (!L.busy) continue;
-      int32_t fin = 0;
-      const wg_status st = wg_batch_poll(L.batch, &fin, 0);
-      if (st != WG_OK) {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->engine_error = st;
+      if (L.busy) {
+        int32_t fin = 0;
+        const wg_status st = wg_batch_poll(L.batch, &fin, 0);
+        if (fin || st != WG_OK) {
+          retire_lane(s, L, st);
+          progressed = true;
+        }
       }
-      if (fin || st != WG_OK) {
-        complete_lane(s, L);
+      while (!L.downloads.empty() && hipEventQuery(L.downloads.front().ev) == hipSuccess) {
+        finish_download(s, L);
         progressed = true;
       }
     }
@@ -225,7 +330,7 @@ static void engine_loop(wg_serve* s) {
     }
     // 3. sleep only when idle
     bool any_busy = false;
-    for (Lane& L : s->lanes) any_busy |= L.busy;
+    for (Lane& L : s->lanes) any_busy |= L.busy || !L.downloads.empty();
     std::unique_lock<std::mutex> lk(s->mu);
     if (s->stop && s->queue.empty() && !any_busy) return;
     if (!progressed) {
@@ -278,6 +383,12 @@ wg_status wg_serve_create(wg_ctx* ctx, const wg_column* cols, int32_t n_cols, ui
     // every lane's arena can hold the whole budget: the budget, not the arena, admits
     const uint64_t groups = std::max<uint64_t>(hbm_budget / 64 / 4, 1 << 14);
     WG_TRY(wg_batch_create(L.ctx, max_batch, groups, &L.batch));
+    // the runner's tuning for request batches of window scans (1024-thread
+    // workgroups, LDS tables up to 2048 groups and as a cache of frequent keys
+    // above that, 128K-row work items, up to 16 key partitions): the fastest
+    // of the candidates on GROUP BY port, GROUP BY address (Zipf keys) and
+    // join-then-GROUP BY batches over the flow table
+    WG_TRY(wg_batch_set_tuning(L.batch, 1024, 2048, 131072, 16, 1));
   }
   s->engine = std::thread(engine_loop, s);
   *out = s;
@@ -294,6 +405,10 @@ wg_status wg_serve_destroy(wg_serve* s) {
   if (s->engine.joinable()) s->engine.join();
   for (Lane& L : s->lanes) {
     (void)wg_batch_destroy(L.batch);
+    for (const Download& d : L.downloads) (void)hipEventDestroy(d.ev);
+    for (hipEvent_t e : L.free_events) (void)hipEventDestroy(e);
+    for (uint64_t* h : L.host)
+      if (h) (void)hipHostFree(h);
     (void)wg_ctx_destroy(L.ctx);
   }
   for (auto& kv : s->reqs) delete kv.second;

@@ -575,17 +575,19 @@ class Batch:
         call("wg_batch_autotune", self.h, qarr, n)
 
     def tuning(self):
-        b, t, r = C.c_int32(), C.c_int32(), C.c_uint64()
-        call("wg_batch_tuning", self.h, C.byref(b), C.byref(t), C.byref(r))
-        return dict(block=b.value, lds_threshold=t.value, tile_rows=r.value)
+        b, t, r, f, c = C.c_int32(), C.c_int32(), C.c_uint64(), C.c_int32(), C.c_int32()
+        call("wg_batch_tuning", self.h, C.byref(b), C.byref(t), C.byref(r), C.byref(f), C.byref(c))
+        return dict(block=b.value, lds_threshold=t.value, tile_rows=r.value, max_fanout=f.value, lds_cache=c.value)
 
-    def set_tuning(self, block: int, lds_threshold: int, tile_rows: int):
-        call("wg_batch_set_tuning", self.h, block, lds_threshold, tile_rows)
+    def set_tuning(self, block: int, lds_threshold: int, tile_rows: int, max_fanout: int = 1, lds_cache: int = 1):
+        call("wg_batch_set_tuning", self.h, block, lds_threshold, tile_rows, max_fanout, lds_cache)
 
     def choice(self, q: int):
-        u, b = C.c_int32(), C.c_int32()
-        call("wg_batch_choice", self.h, q, C.byref(u), C.byref(b))
-        return bool(u.value), b.value
+        """(layout, workgroup size, key partitions per tile) of query q; layout
+        1 = LDS table(s), 2 = global table behind an LDS cache, 0 = global."""
+        u, b, f = C.c_int32(), C.c_int32(), C.c_int32()
+        call("wg_batch_choice", self.h, q, C.byref(u), C.byref(b), C.byref(f))
+        return u.value, b.value, f.value
 
     def result(self, q: int) -> dict:
         n = C.c_uint64()

@@ -67,27 +67,71 @@ def gen_fk_join_probe(ctx: Context, cfg: FkJoin, n: int = None, row0: int = 0):
     return [Col(fk, L.INT64), Col(qty, L.INT32)]
 
 
-# --- many concurrent GROUP BY queries of uneven shapes ---------------
+# --- many concurrent GROUP BY / join queries of uneven shapes ---------
+QMIX_DIM_ROWS = 50_000                     # the dimension the join queries probe (unique keys)
+QMIX_DIM_GROUPS = (16, 256, 4096, 50_000)  # cardinalities of its four payload columns
+QMIX_DIM_MATCH_PERMILLE = 900              # probe keys that find a dimension row
+
+
 def plan_query_mix(n_queries: int = 256, seed: int = 4, max_rows: int = 4_000_000, min_rows: int = 1000,
-                 median_rows: int = 65_536, sigma: float = 1.39, min_groups: int = 10, max_groups: int = 100_000):
+                 median_rows: int = 65_536, sigma: float = 1.39, min_groups: int = 10, max_groups: int = 100_000,
+                 join_every: int = 4):
     """Per query: rows ~ lognormal clipped to [min_rows, max_rows], groups
     log-uniform in [min_groups, max_groups], key type cycling int64 / int32 /
-    uint32 (IPv4-like), value int64 or DECIMAL(18,2)."""
+    uint32 (IPv4-like), value int64 or DECIMAL(18,2). Every join_every-th
+    query (0: none) is JOIN dim ON key GROUP BY a dimension payload column
+    instead: int64 foreign keys, the payload whose cardinality is nearest the
+    drawn group count."""
     import numpy as np
     rng = np.random.default_rng(seed)
     rows = np.clip(np.exp(rng.normal(np.log(median_rows), sigma, n_queries)), min_rows, max_rows).astype(np.int64)
     groups = np.exp(rng.uniform(np.log(min_groups), np.log(max_groups), n_queries)).astype(np.int64)
     key_types = [(L.INT64, L.INT32, L.UINT32)[i % 3] for i in range(n_queries)]
     val_types = [(L.INT64, L.DECIMAL)[(i // 3) % 2] for i in range(n_queries)]
-    return [dict(rows=int(r), groups=int(g), key_type=k, val_type=v, seed=seed * 1000 + i)
-            for i, (r, g, k, v) in enumerate(zip(rows, groups, key_types, val_types))]
+    out = []
+    for i, (r, g, k, v) in enumerate(zip(rows, groups, key_types, val_types)):
+        q = dict(rows=int(r), groups=int(g), key_type=k, val_type=v, seed=seed * 1000 + i, kind="groupby")
+        if join_every and i % join_every == join_every - 1:
+            pay = int(np.argmin([abs(np.log(c) - np.log(g)) for c in QMIX_DIM_GROUPS]))
+            q.update(kind="join", key_type=L.INT64, payload=pay, groups=QMIX_DIM_GROUPS[pay])
+        out.append(q)
+    return out
 
 
-def gen_query_mix(ctx: Context, plan):
-    """Device columns of every planned query: list of (key Col, val Col, rows, est_groups)."""
+def gen_query_dim(ctx: Context, seed: int = 77):
+    """The dimension of the join queries: QMIX_DIM_ROWS unique int64 keys and
+    one int32 payload column per QMIX_DIM_GROUPS cardinality, built into a
+    HashJoin with every payload inline. Returns (HashJoin, device columns)."""
+    from .gpu import HashJoin
+    n = QMIX_DIM_ROWS
+    cols = [Col(ctx.gen(L.GEN_PERM_KEY_I64, "int64", n, seed, n), L.INT64)]
+    for j, g in enumerate(QMIX_DIM_GROUPS):
+        cols.append(Col(ctx.gen(L.GEN_UNIFORM_I32, "int32", n, seed + 1 + j, 0, g - 1), L.INT32))
+    jn = HashJoin(ctx, [L.INT64] + [L.INT32] * len(QMIX_DIM_GROUPS), 0, inline=list(range(1, 1 + len(QMIX_DIM_GROUPS))),
+                  build_hint=n)
+    jn.build(cols, n)
+    jn.finish()
+    return jn, cols
+
+
+def gen_query_mix(ctx: Context, plan, dim=None):
+    """Device columns of every planned query: list of (key Col, val Col, rows,
+    est_groups[, extras]) — a join query's extras hold dim=(HashJoin, payload)
+    and the dimension columns. `dim` (gen_query_dim's result) is built on
+    demand."""
     out = []
     for q in plan:
         n, g, s = q["rows"], q["groups"], q["seed"]
+        if q.get("kind") == "join":
+            if dim is None:
+                dim = gen_query_dim(ctx)
+            k = ctx.gen(L.GEN_FK_I64, "int64", n, s, QMIX_DIM_ROWS, QMIX_DIM_MATCH_PERMILLE)
+            v = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, s + 7, -10**9, 10**9) if q["val_type"] == L.INT64 else \
+                ctx.gen(L.GEN_UNIFORM_I64, "int64", n, s + 7, 0, 10**12)
+            vc = Col(v, L.INT64) if q["val_type"] == L.INT64 else Col(v, L.DECIMAL, None, 2)
+            out.append((Col(k, L.INT64), vc, n, g, dict(dim=(dim[0], q["payload"]), dim_cols=dim[1],
+                                                          payload=q["payload"])))
+            continue
         if q["key_type"] == L.INT64:
             k = ctx.gen(L.GEN_UNIFORM_I64, "int64", n, s, 0, g - 1)
         elif q["key_type"] == L.INT32:
