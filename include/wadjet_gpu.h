@@ -480,6 +480,54 @@ wg_status wg_parquet_decode_page(wg_ctx* ctx, const wg_pq_page* page, void* out,
 wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t n_pages, void* out,
                                   uint64_t* out_valid, uint64_t row0);
 
+/* One column chunk of a Parquet file, as the footer's ColumnMetaData gives it. */
+typedef struct wg_pq_chunk {
+  uint64_t offset;    /* file offset of the chunk's first page (its dictionary page when it has one) */
+  uint64_t bytes;     /* total_compressed_size: every page header and body of the chunk */
+  uint64_t num_rows;  /* rows of the chunk's row group */
+  int32_t codec;      /* CompressionCodec: 0 UNCOMPRESSED, 1 SNAPPY, 2 GZIP, 6 ZSTD */
+  int32_t reserved;   /* 0 */
+} wg_pq_chunk;
+
+/* The pages of one flat column's chunks, staged for wg_parquet_decode_pages. */
+typedef struct wg_pq_staged wg_pq_staged;
+
+/* Stage the chunks of one flat column of the file at `path` (several row
+ * groups' chunks of the same column, in row order): on up to `threads` host
+ * threads (0 = the usable cores, at most 16), each chunk is read, its Thrift
+ * compact page headers parsed, its pages decompressed and the run headers of
+ * each hybrid stream scanned (ColumnPageReader.ReadPage / decodeDataPageV1 /
+ * decodeDataPageV2, internal/storage/parquet/page_reader.go:434-560;
+ * Decompress, decompress.go:15-100); the streams go to device memory of ctx's
+ * device through pinned staging slots, overlapping the next chunks' host work.
+ * ctx = NULL stages into host memory instead (the page table then points at
+ * host memory): the host half on its own. physical_type 1 INT32, 2 INT64 or
+ * 5 DOUBLE; max_def 0 (required) or 1 (optional). Data pages v1 and v2,
+ * PLAIN or dictionary encoded, with RLE definition levels. WG_ERR_UNSUPPORTED
+ * for another codec, encoding or level encoding; WG_ERR_INVALID for a file
+ * that cannot be read, a malformed page, or chunks whose data pages do not
+ * hold their rows. Synchronises. */
+wg_status wg_pq_stage(wg_ctx* ctx, const char* path, const wg_pq_chunk* chunks, uint64_t n_chunks,
+                      int32_t physical_type, int32_t max_def, int32_t threads, wg_pq_staged** out);
+
+/* Rows, page count, the stream bytes the decode reads (levels, values or
+ * indices, dictionaries, run tables) and the staged memory's size. */
+wg_status wg_pq_staged_info(const wg_pq_staged* s, uint64_t* rows, uint64_t* n_pages, uint64_t* stream_bytes,
+                            uint64_t* staged_bytes);
+
+/* Copy the page table (min(cap, n_pages) pages; host memory whose pointers
+ * name the staged memory) into pages: the `pages` argument of
+ * wg_parquet_decode_pages. */
+wg_status wg_pq_staged_pages(const wg_pq_staged* s, wg_pq_page* pages, uint64_t cap);
+
+/* Release the staged memory; NULL is a no-op. */
+wg_status wg_pq_staged_free(wg_pq_staged* s);
+
+/* One page body decompressed (Decompress, decompress.go:15-100): codec as in
+ * wg_pq_chunk; dst_len must equal the page's uncompressed size.
+ * WG_ERR_INVALID for corrupt input or a size mismatch. Host memory. */
+wg_status wg_pq_decompress(int32_t codec, const uint8_t* src, uint64_t len, uint8_t* dst, uint64_t dst_len);
+
 /* ----------------------------------------------------------------- shuffle */
 
 /* Routing of a partitioned shuffle sink (partitionedShuffleSink.Consume,

@@ -6,7 +6,7 @@ from pathlib import Path
 
 import pytest
 
-BIN = Path(__file__).resolve().parent.parent / "wadjet_amd" / "lib" / "test_exec"
+BIN = Path(__file__).resolve().parent / "cpp" / "build" / "test_exec"
 
 
 def run(mode):

@@ -49,6 +49,11 @@ class PqPage(C.Structure):
                 ("plain", C.c_void_p), ("idx", PqRuns), ("dict", C.c_void_p), ("dict_n", C.c_uint64)]
 
 
+class PqChunk(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("bytes", C.c_uint64), ("num_rows", C.c_uint64), ("codec", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class BatchQuery(C.Structure):
     _fields_ = [("key", Column), ("val", Column), ("n_rows", C.c_uint64), ("est_groups", C.c_uint64),
                 ("row_begin", C.c_uint64), ("range_col", Column), ("range_lo", C.c_int64), ("range_hi", C.c_int64),
@@ -130,6 +135,11 @@ SIGNATURES: dict[str, list] = {
     "wg_pq_scan_runs": [P, U64, I32, U64, P, U64, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)],
     "wg_parquet_decode_page": [P, P, P, P, U64],
     "wg_parquet_decode_pages": [P, P, U64, P, P, U64],
+    "wg_pq_stage": [P, C.c_char_p, P, U64, I32, I32, I32, C.POINTER(C.c_void_p)],
+    "wg_pq_staged_info": [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)],
+    "wg_pq_staged_pages": [P, P, U64],
+    "wg_pq_staged_free": [P],
+    "wg_pq_decompress": [I32, P, U64, P, U64],
     "wg_view_flatten": [P, P, U64, P, P, U64, P, P],
     "wg_join_build_column": [P, I32, P],
     "wg_shuffle_partition": [P, C.POINTER(Column), I32, C.POINTER(I32), I32, U64, P, U64, I32, P, P, P],

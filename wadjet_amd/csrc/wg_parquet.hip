@@ -7,8 +7,9 @@
 // (internal/storage/parquet/page_reader.go:434-700, rle.go, reader.go:
 // 1020-1060). Here the host keeps the byte-level work that is sequential by
 // nature — page headers, decompression, and the run headers of each hybrid
-// stream (wg_pq_scan_runs, a few bytes per run of up to hundreds of values) —
-// and the device does the per-value work:
+// stream (a few bytes per run of up to hundreds of values), on a pool of
+// threads per column (wg_pq_stage, wg_pq_stage.cpp) — and the device does the
+// per-value work:
 //   count    present rows (definition level 1) of each 1024-row segment of
 //            a nullable page, read from the level runs (RLE value, or bits
 //            extracted from a bit-packed run);
@@ -361,72 +362,6 @@ __global__ void pq_values_kernel(const PqPageDev* pages, const uint32_t* tile_pa
 using namespace wg;
 
 extern "C" {
-
-wg_status wg_pq_scan_runs(const uint8_t* data, uint64_t len, int32_t bit_width, uint64_t count, uint32_t* runs,
-                          uint64_t cap, uint64_t* n_runs, uint64_t* used, uint64_t* ones) {
-  clear_error();
-  WG_CHECK(data || len == 0, WG_ERR_INVALID, "null data");
-  WG_CHECK(n_runs && used, WG_ERR_INVALID, "null argument");
-  WG_CHECK(bit_width >= 0 && bit_width <= 32, WG_ERR_INVALID, "bit width %d", bit_width);
-  WG_CHECK(count < (1ull << 32) && len < (1ull << 29), WG_ERR_UNSUPPORTED, "a stream of %llu values / %llu bytes",
-           (unsigned long long)count, (unsigned long long)len);
-  const int vbytes = (bit_width + 7) / 8;
-  uint64_t off = 0, got = 0, nr = 0, n1 = 0;
-  // RLE / bit-packing hybrid (rle.go:50-120): varint header; header & 1 = 0:
-  // RLE run of header >> 1 values, the value in ceil(bw / 8) bytes; else a
-  // bit-packed run of (header >> 1) groups of 8 values, bw bytes per group
-  while (got < count) {
-    WG_CHECK(off < len, WG_ERR_INVALID, "hybrid stream ends after %llu of %llu values", (unsigned long long)got,
-             (unsigned long long)count);
-    uint64_t header = 0;
-    int shift = 0;
-    for (;;) {
-      WG_CHECK(off < len && shift < 64, WG_ERR_INVALID, "bad run header varint");
-      const uint8_t b = data[off++];
-      header |= (uint64_t)(b & 0x7F) << shift;
-      shift += 7;
-      if (!(b & 0x80)) break;
-    }
-    uint64_t n;
-    uint32_t payload = 0, packed = 0;
-    if (header & 1) {
-      const uint64_t groups = header >> 1;
-      n = groups * 8;
-      payload = (uint32_t)(off * 8);
-      packed = 1;
-      WG_CHECK(off + groups * (uint64_t)bit_width <= len, WG_ERR_INVALID, "bit-packed run past the stream");
-      if (bit_width == 1) {  // ones among this run's values below `count`
-        const uint64_t take = std::min<uint64_t>(n, count - got);
-        for (uint64_t v = 0; v < take; ++v) n1 += (data[off + v / 8] >> (v % 8)) & 1u;
-      }
-      off += groups * (uint64_t)bit_width;
-    } else {
-      n = header >> 1;
-      WG_CHECK(off + vbytes <= len, WG_ERR_INVALID, "RLE value past the stream");
-      for (int i = 0; i < vbytes; ++i) payload |= (uint32_t)data[off + i] << (8 * i);
-      off += vbytes;
-      if (payload == 1) n1 += std::min<uint64_t>(n, count - got);
-    }
-    if (n == 0) continue;
-    if (nr < cap && runs) {
-      uint32_t* r = runs + 4 * nr;
-      r[0] = (uint32_t)got;
-      r[1] = (uint32_t)std::min<uint64_t>(n, count - got);
-      r[2] = payload;
-      r[3] = packed;
-    }
-    ++nr;
-    got += n;  // a bit-packed run may pad its last group past `count`
-  }
-  *n_runs = nr;
-  *used = off;
-  if (ones) *ones = n1;
-  if (nr > cap && runs) {
-    set_error("%llu runs do not fit %llu", (unsigned long long)nr, (unsigned long long)cap);
-    return WG_ERR_INVALID;
-  }
-  return WG_OK;
-}
 
 wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t n_pages, void* out,
                                   uint64_t* out_valid, uint64_t row0) {

@@ -91,7 +91,7 @@ def _column(arr) -> Tuple[np.ndarray, int, int, Optional[np.ndarray]]:
     valid = _validity_words(arr)
     if pa.types.is_decimal(t):
         if not pa.types.is_decimal128(t):
-            raise L.UnsupportedError(f"decimal type {t} not on the GPU path")
+            raise L.UnsupportedError("parquet", L.WG_ERR_UNSUPPORTED, f"decimal type {t} not on the GPU path")
         raw = np.frombuffer(arr.buffers()[1], dtype=np.uint64)[2 * arr.offset: 2 * (arr.offset + len(arr))]
         lo, hi = raw[0::2], raw[1::2]
         if t.precision <= 18:
@@ -102,7 +102,7 @@ def _column(arr) -> Tuple[np.ndarray, int, int, Optional[np.ndarray]]:
     m = {pa.int32(): (np.int32, L.INT32), pa.int64(): (np.int64, L.INT64), pa.uint32(): (np.uint32, L.UINT32),
          pa.uint16(): (np.uint16, L.UINT16), pa.float64(): (np.float64, L.FLOAT64)}
     if t not in m:
-        raise L.UnsupportedError(f"column type {t} not on the GPU path")
+        raise L.UnsupportedError("parquet", L.WG_ERR_UNSUPPORTED, f"column type {t} not on the GPU path")
     dt, wt = m[t]
     vals = arr.to_numpy(zero_copy_only=False)
     if valid is not None:  # nulls decode as NaN / object; store zeros under them
@@ -142,320 +142,97 @@ def consume_parquet(agg, path: str, columns: Sequence[str], pred=None, batch_row
 #
 # The reference's own Parquet reader (internal/storage/parquet: footer.go,
 # thrift.go, page_reader.go, rle.go) decodes every page on the CPU. Here the
-# host reads a column chunk's bytes, parses the Thrift compact page headers,
-# decompresses with pyarrow's codecs and scans the run headers of each
-# RLE / bit-packing hybrid stream (wg_pq_scan_runs); the device decodes the
-# values (wg_parquet_decode_page): definition levels, PLAIN values,
-# dictionary indices resolved against the dictionary page.
-
-_CT_STOP, _CT_TRUE, _CT_FALSE, _CT_BYTE, _CT_I16, _CT_I32, _CT_I64, _CT_DOUBLE, _CT_BINARY, _CT_LIST, _CT_SET, \
-    _CT_MAP, _CT_STRUCT = range(13)
+# footer comes from pyarrow's metadata (chunk offsets, sizes, codecs, types);
+# the library stages each column's chunks on a pool of host threads
+# (wg_pq_stage: page headers, decompression, run headers of each RLE /
+# bit-packing hybrid stream, pinned uploads) and the device decodes the values
+# (wg_parquet_decode_pages): definition levels, PLAIN values, dictionary
+# indices resolved against the dictionary page.
 
 PAGE_DATA, PAGE_INDEX, PAGE_DICTIONARY, PAGE_DATA_V2 = 0, 1, 2, 3
 ENC_PLAIN, ENC_PLAIN_DICTIONARY, ENC_RLE_DICTIONARY = 0, 2, 8
 PT_INT32, PT_INT64, PT_DOUBLE = 1, 2, 5
-
-
-class _Compact:
-    """Thrift compact protocol reader for page headers (thrift.go restates the
-    same protocol): field headers with deltas, zigzag varints, nested structs;
-    unknown fields are skipped."""
-
-    def __init__(self, buf: bytes, pos: int = 0):
-        self.b = buf
-        self.p = pos
-
-    def varint(self) -> int:
-        v, shift = 0, 0
-        while True:
-            c = self.b[self.p]
-            self.p += 1
-            v |= (c & 0x7F) << shift
-            shift += 7
-            if not c & 0x80:
-                return v
-
-    def zigzag(self) -> int:
-        v = self.varint()
-        return (v >> 1) ^ -(v & 1)
-
-    def value(self, t: int):
-        if t in (_CT_TRUE, _CT_FALSE):
-            return t == _CT_TRUE
-        if t == _CT_BYTE:
-            self.p += 1
-            return self.b[self.p - 1]
-        if t in (_CT_I16, _CT_I32, _CT_I64):
-            return self.zigzag()
-        if t == _CT_DOUBLE:
-            self.p += 8
-            return None
-        if t == _CT_BINARY:
-            n = self.varint()
-            self.p += n
-            return self.b[self.p - n:self.p]
-        if t in (_CT_LIST, _CT_SET):
-            h = self.b[self.p]
-            self.p += 1
-            size, et = h >> 4, h & 0x0F
-            if size == 15:
-                size = self.varint()
-            return [self._bool_elem() if et in (_CT_TRUE, _CT_FALSE) else self.value(et) for _ in range(size)]
-        if t == _CT_MAP:
-            size = self.varint()
-            if size:
-                kv = self.b[self.p]
-                self.p += 1
-                for _ in range(size):
-                    self.value(kv >> 4)
-                    self.value(kv & 0x0F)
-            return None
-        if t == _CT_STRUCT:
-            return self.struct()
-        raise ValueError(f"thrift compact type {t}")
-
-    def _bool_elem(self):
-        self.p += 1
-        return self.b[self.p - 1] == 1
-
-    def struct(self) -> dict:
-        out, fid = {}, 0
-        while True:
-            h = self.b[self.p]
-            self.p += 1
-            if h == _CT_STOP:
-                return out
-            delta, t = h >> 4, h & 0x0F
-            fid = fid + delta if delta else self.zigzag()
-            out[fid] = self.value(t)
-
-
-def _read_page_header(buf: bytes, pos: int):
-    r = _Compact(buf, pos)
-    h = r.struct()
-    return h, r.p
-
-
-def _align(n: int, a: int = 8) -> int:
-    return (n + a - 1) // a * a
-
-
-class _Arena:
-    """Host staging of one column chunk's streams, 8-byte aligned, each
-    followed by 16 readable padding bytes (the device reads whole words)."""
-
-    def __init__(self):
-        self.parts, self.size = [], 0
-
-    def add(self, data) -> int:
-        off = self.size
-        b = np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data.view(np.uint8).ravel()
-        self.parts.append((off, b))
-        self.size = _align(off + len(b) + 16)
-        return off
-
-    def pack(self) -> np.ndarray:
-        out = np.zeros(max(self.size, 8), np.uint8)
-        for off, b in self.parts:
-            out[off:off + len(b)] = b
-        return out
-
-
-def _scan_runs(buf, bit_width: int, count: int):
-    """Run table (uint32 x 4 per run), bytes used, and values equal to 1."""
-    import ctypes as C
-    arr = np.frombuffer(buf, np.uint8)
-    ptr = arr.ctypes.data_as(C.c_void_p) if len(arr) else None
-    nr, used, ones = C.c_uint64(), C.c_uint64(), C.c_uint64()
-    L.call("wg_pq_scan_runs", ptr, len(arr), bit_width, count, None, 0, C.byref(nr), C.byref(used), C.byref(ones))
-    runs = np.zeros(max(nr.value, 1) * 4, np.uint32)
-    L.call("wg_pq_scan_runs", ptr, len(arr), bit_width, count, runs.ctypes.data_as(C.c_void_p), nr.value,
-           C.byref(nr), C.byref(used), C.byref(ones))
-    return runs, nr.value, used.value, ones.value
+# CompressionCodec (parquet.thrift) by pyarrow's metadata name
+CODECS = {"UNCOMPRESSED": 0, "SNAPPY": 1, "GZIP": 2, "LZO": 3, "BROTLI": 4, "LZ4": 5, "ZSTD": 6, "LZ4_RAW": 7}
 
 
 def _wg_type_of(col_schema) -> Tuple[int, int, int]:
     """(physical type id, wg type, scale) of a flat column."""
     phys = {"INT32": PT_INT32, "INT64": PT_INT64, "DOUBLE": PT_DOUBLE}.get(col_schema.physical_type)
     if phys is None:
-        raise L.UnsupportedError(f"parquet physical type {col_schema.physical_type} not on the GPU decode path")
+        raise L.UnsupportedError("parquet", L.WG_ERR_UNSUPPORTED,
+                                 f"physical type {col_schema.physical_type} not on the GPU decode path")
     lt = col_schema.logical_type
     ltype = getattr(lt, "type", "NONE")
     if ltype == "DECIMAL":
         if phys != PT_INT64:
-            raise L.UnsupportedError("decimals stored as INT32 are not on the GPU decode path")
+            raise L.UnsupportedError("parquet", L.WG_ERR_UNSUPPORTED,
+                                     "decimals stored as INT32 are not on the GPU decode path")
         return phys, L.DECIMAL, col_schema.scale
     if phys == PT_INT32:
-        lts = str(lt)
-        if "isSigned=false" in lts.replace(" ", "") and "bitWidth=32" in lts.replace(" ", ""):
+        lts = str(lt).replace(" ", "")
+        if "isSigned=false" in lts and "bitWidth=32" in lts:
             return phys, L.UINT32, 0
         return phys, L.INT32, 0
     return phys, (L.INT64 if phys == PT_INT64 else L.FLOAT64), 0
 
 
-def _chunk_pages(raw: bytes, compression: str, n_rows: int, max_def: int, width: int, arena: _Arena) -> list:
-    """The data pages of one column chunk's bytes, decompressed into `arena`:
-    per page its rows, encoding and the arena offsets of its streams
-    (ColumnPageReader.ReadPage, storage/parquet/page_reader.go:434-520)."""
-    import pyarrow as pa
-    codec = None if compression == "UNCOMPRESSED" else pa.Codec(compression.lower())
-
-    def inflate(b, size):
-        # an Arrow buffer read in place (no copy into a Python bytes object)
-        return b if codec is None else memoryview(codec.decompress(b, decompressed_size=size)).cast("B")
-
-    pages, dict_ref = [], None
-    pos, rows_seen = 0, 0
-    while pos < len(raw) and rows_seen < n_rows:
-        h, pos = _read_page_header(raw, pos)
-        ptype, usize, csize = h[1], h[2], h[3]
-        body = raw[pos:pos + csize]
-        pos += csize
-        if ptype == PAGE_DICTIONARY:
-            dict_n = h[7][1]
-            vals = inflate(body, usize)
-            if len(vals) < dict_n * width:
-                raise L.WgError("dictionary page shorter than its values")
-            dict_ref = (arena.add(vals[:dict_n * width]), dict_n)
-            continue
-        if ptype == PAGE_DATA:
-            dph = h[5]
-            nv, enc = dph[1], dph[2]
-            data = inflate(body, usize)
-            off = 0
-            levels = None
-            if max_def > 0:
-                ll = int.from_bytes(data[0:4], "little")
-                levels = data[4:4 + ll]
-                off = 4 + ll
-            values = data[off:]
-        elif ptype == PAGE_DATA_V2:
-            dph = h[8]
-            nv, enc = dph[1], dph[4]
-            dl, rl = dph.get(5, 0), dph.get(6, 0)
-            levels = body[rl:rl + dl] if max_def > 0 else None
-            rest = body[rl + dl:]
-            values = inflate(rest, usize - rl - dl) if dph.get(7, True) else bytes(rest)
-        else:
-            continue  # index pages carry no values
-        page = dict(rows=nv, enc=enc)
-        present = nv
-        if levels is not None:
-            runs, nr, _, present = _scan_runs(levels, 1, nv)
-            if present < nv:  # a page without NULLs decodes as a required one
-                page["def"] = (arena.add(levels), arena.add(runs), nr)
-        if enc == ENC_PLAIN:
-            if len(values) < present * width:
-                raise L.WgError("PLAIN page shorter than its values")
-            page["plain"] = arena.add(values[:present * width])
-        elif enc in (ENC_PLAIN_DICTIONARY, ENC_RLE_DICTIONARY):
-            if dict_ref is None:
-                raise L.WgError("dictionary-encoded page before a dictionary page")
-            bw = values[0]
-            runs, nr, _, _ = _scan_runs(values[1:], bw, present)
-            page["idx"] = (arena.add(values[1:]), arena.add(runs), nr, bw)
-            page["dict"] = dict_ref
-        else:
-            raise L.UnsupportedError(f"parquet encoding {enc} not on the GPU decode path")
-        pages.append(page)
-        rows_seen += nv
-    got = sum(pg["rows"] for pg in pages)
-    if got != n_rows:
-        raise L.WgError(f"column chunk pages hold {got} rows, the row group {n_rows}")
-    return pages
+def column_chunks(path: str, column: str, row_groups: Optional[Sequence[int]] = None):
+    """(column schema, [wg_pq_chunk]) of one flat column, from the footer."""
+    import pyarrow.parquet as pq
+    f = pq.ParquetFile(path)
+    md = f.metadata
+    ci = f.schema_arrow.get_field_index(column)
+    if ci < 0:
+        raise KeyError(column)
+    cs = f.schema.column(ci)
+    if cs.max_repetition_level != 0:
+        raise L.UnsupportedError("parquet", L.WG_ERR_UNSUPPORTED, "repeated columns are not on the GPU decode path")
+    rgs = list(range(md.num_row_groups)) if row_groups is None else list(row_groups)
+    chunks = (L.PqChunk * max(len(rgs), 1))()
+    for ch, rg in zip(chunks, rgs):
+        cc = md.row_group(rg).column(ci)
+        ch.offset = (cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset
+                     else cc.data_page_offset)
+        ch.bytes = cc.total_compressed_size
+        ch.num_rows = md.row_group(rg).num_rows
+        ch.codec = CODECS.get(cc.compression, -1)
+    return cs, chunks, len(rgs)
 
 
 class StagedColumn:
-    """One column's chunks (all row groups, or those named) parsed and
-    decompressed on the host and uploaded to HBM in one buffer: the input of
-    the device decode, which `decode()` runs in one wg_parquet_decode_pages
-    call. `page_bytes` = the decompressed stream bytes the decode reads."""
+    """One column's chunks (all row groups, or those named) read, parsed,
+    decompressed and scanned by the library on host threads and staged in
+    HBM (wg_pq_stage): the input of the device decode, which `decode()` runs
+    in one wg_parquet_decode_pages call. ctx=None stages into host memory
+    (the host half alone; `decode()` then refuses). `page_bytes` = the stream
+    bytes the decode reads."""
 
-    def __init__(self, ctx, path: str, column: str, row_groups: Optional[Sequence[int]] = None):
-        import pyarrow.parquet as pq
-        f = pq.ParquetFile(path)
-        md = f.metadata
-        ci = f.schema_arrow.get_field_index(column)
-        if ci < 0:
-            raise KeyError(column)
-        cs = f.schema.column(ci)
-        if cs.max_repetition_level != 0:
-            raise L.UnsupportedError("repeated columns are not on the GPU decode path")
+    def __init__(self, ctx, path: str, column: str, row_groups: Optional[Sequence[int]] = None, threads: int = 0):
+        import ctypes as C
+        cs, chunks, n_chunks = column_chunks(path, column, row_groups)
         self.ctx = ctx
         self.max_def = cs.max_definition_level
         self.phys, self.type, self.scale = _wg_type_of(cs)
         self.width = 4 if self.phys == PT_INT32 else 8
-        rgs = list(range(md.num_row_groups)) if row_groups is None else list(row_groups)
-
-        def stage(rg):
-            # one chunk: read, decompress, scan its streams into its own arena
-            # (file reads, decompression and run scans release the GIL)
-            cc = md.row_group(rg).column(ci)
-            n_rows = md.row_group(rg).num_rows
-            start = (cc.dictionary_page_offset if cc.has_dictionary_page and cc.dictionary_page_offset
-                     else cc.data_page_offset)
-            with open(path, "rb") as fh:
-                fh.seek(start)
-                raw = fh.read(cc.total_compressed_size)
-            arena = _Arena()
-            return n_rows, arena, _chunk_pages(raw, cc.compression, n_rows, self.max_def, self.width, arena)
-
-        import concurrent.futures as cf
-        # the page walk holds the GIL: past two threads the contention costs more than the overlap gains
-        workers = max(1, min(len(rgs), 2, os.cpu_count() or 1))
-        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-            staged = list(ex.map(stage, rgs))
-        # the chunks' arenas laid end to end in one host buffer, one upload
-        offs, total = [], 0
-        for _, arena, _ in staged:
-            offs.append(total)
-            total += arena.size
-        host = np.zeros(max(total, 8), np.uint8)
-
-        def place(i):
-            for off, b in staged[i][1].parts:
-                host[offs[i] + off:offs[i] + off + len(b)] = b
-
-        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-            list(ex.map(place, range(len(staged))))
-        self.rows = sum(n for n, _, _ in staged)
-        self.page_bytes = sum(len(b) for _, arena, _ in staged for _, b in arena.parts)
-        self.dev = ctx.to_device(host)
-        del host
-        base = self.dev.ptr.value if hasattr(self.dev.ptr, "value") else int(self.dev.ptr)
-        pages = []
-        for (_, _, chunk_pages), off in zip(staged, offs):
-            for pg in chunk_pages:
-                pg = dict(pg)
-                for key in ("def", "plain", "idx"):
-                    if key in pg:
-                        v = pg[key]
-                        pg[key] = (v + off) if key == "plain" else (v[0] + off, v[1] + off) + tuple(v[2:])
-                if "dict" in pg:
-                    pg["dict"] = (pg["dict"][0] + off, pg["dict"][1])
-                pages.append(pg)
-        self.pages = (L.PqPage * max(len(pages), 1))()
-        for P, pg in zip(self.pages, pages):
-            P.num_rows = pg["rows"]
-            P.physical_type = self.phys
-            P.encoding = pg["enc"]
-            if "def" in pg:
-                d_off, r_off, nr = pg["def"]
-                P.def_ = L.PqRuns(base + r_off, nr, 1, base + d_off)
-            if "plain" in pg:
-                P.plain = base + pg["plain"]
-            if "idx" in pg:
-                d_off, r_off, nr, bw = pg["idx"]
-                P.idx = L.PqRuns(base + r_off, nr, bw, base + d_off)
-                P.dict = base + pg["dict"][0]
-                P.dict_n = pg["dict"][1]
-        self.n_pages = len(pages)
+        h = C.c_void_p()
+        L.call("wg_pq_stage", ctx.h if ctx is not None else None, os.fsencode(path), chunks, n_chunks, self.phys,
+               self.max_def, threads, C.byref(h))
+        self.h = h
+        rows, n_pages, stream_bytes, staged = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        L.call("wg_pq_staged_info", h, C.byref(rows), C.byref(n_pages), C.byref(stream_bytes), C.byref(staged))
+        self.rows, self.n_pages = rows.value, n_pages.value
+        self.page_bytes, self.staged_bytes = stream_bytes.value, staged.value
+        self.pages = (L.PqPage * max(self.n_pages, 1))()
+        L.call("wg_pq_staged_pages", h, self.pages, self.n_pages)
 
     def decode(self, out=None, out_valid=None, row0: int = 0):
         """Decode every page on the device. With `out` given (a device buffer
         of the column's width, and `out_valid` zeroed bitmap words for a
         nullable column) the rows land at row0; else new buffers. -> Col."""
         from .gpu import Col
+        if self.ctx is None:
+            raise L.WgError("wg_parquet_decode_pages", L.WG_ERR_INVALID, "staged in host memory")
         if out is None:
             out = self.ctx.alloc(max(self.rows, 1) * self.width)
             row0 = 0
@@ -467,9 +244,15 @@ class StagedColumn:
         return Col(out, self.type, out_valid if self.max_def > 0 else None, self.scale)
 
     def free(self):
-        if self.dev is not None:
-            self.dev.free()
-            self.dev = None
+        if getattr(self, "h", None):
+            L.call("wg_pq_staged_free", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def decode_column_chunk(ctx, path: str, row_group: int, column: str, out=None, out_valid=None, row0: int = 0):
