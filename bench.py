@@ -572,9 +572,20 @@ def bench_served(ctx, args):
             h = len(lat) // 2  # a p99 that holds over the run, not one that grows with it
             res[name] = dict(p50_ms=pct(lat, 50), p99_ms=pct(lat, 99), qps=st["qps"], launches=st["batches"],
                              max_in_flight=st["max_in_flight"], p99_first_half_ms=pct(lat[:h], 99),
-                             p99_second_half_ms=pct(lat[h:], 99), requests_per_launch=len(lat) / max(st["batches"], 1))
-        log(f"[served] {qps} q/s offered: served p50 {res['served']['p50_ms']:.2f} p99 {res['served']['p99_ms']:.2f} "
-            f"ms ({res['served']['qps']:.0f} q/s), plain p50 {res['plain']['p50_ms']:.2f} p99 "
+                             p99_second_half_ms=pct(lat[h:], 99), requests_per_launch=len(lat) / max(st["batches"], 1),
+                             device_busy_frac=st["device_busy_s"] / max(st["wall_s"], 1e-9),
+                             lane_overlap=st["lane_busy_s"] / max(st["device_busy_s"], 1e-9),
+                             device_ms_per_launch=st["lane_busy_s"] * 1000 / max(st["batches"], 1),
+                             host_submit_ms_per_launch=st["host_submit_s"] * 1000 / max(st["batches"], 1),
+                             host_retire_ms_per_launch=st["host_retire_s"] * 1000 / max(st["batches"], 1),
+                             host_deliver_ms_per_launch=st["host_deliver_s"] * 1000 / max(st["batches"], 1),
+                             engine_busy_frac=(st["host_submit_s"] + st["host_retire_s"] + st["host_deliver_s"])
+                             / max(st["wall_s"], 1e-9))
+        sv = res["served"]
+        log(f"[served] {qps} q/s offered: served p50 {sv['p50_ms']:.2f} p99 {sv['p99_ms']:.2f} "
+            f"ms ({sv['qps']:.0f} q/s; device busy {sv['device_busy_frac']:.0%}, "
+            f"{sv['device_ms_per_launch']:.2f} ms device / {sv['host_submit_ms_per_launch']:.3f} ms submit / "
+            f"{sv['host_retire_ms_per_launch']:.3f} ms retire per launch), plain p50 {res['plain']['p50_ms']:.2f} p99 "
             f"{res['plain']['p99_ms']:.2f} ms ({res['plain']['qps']:.0f} q/s)")
         out.append(res)
     served.close()

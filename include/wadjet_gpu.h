@@ -639,7 +639,12 @@ typedef struct wg_serve_stats {
   uint64_t max_in_flight; /* most lanes busy at once */
   uint64_t peak_footprint;/* largest admitted hash-table bytes at once */
   double p50_ms, p99_ms, mean_ms, max_ms;
-  double wall_s, qps;
+  double wall_s, qps;     /* first arrival to last completion; completed / wall_s */
+  double device_busy_s;   /* union over lanes of each batch's device interval (first to last operation) */
+  double lane_busy_s;     /* sum of those intervals (above device_busy_s while lanes overlap) */
+  double host_submit_s;   /* engine thread: planning, uploading and launching batches */
+  double host_retire_s;   /* engine thread: retiring finished batches, starting their result copies */
+  double host_deliver_s;  /* engine thread: handing copied results to their requests */
 } wg_serve_stats;
 
 /* cols[n_cols] are device-resident columns of n_rows rows; ts_col is a

@@ -69,7 +69,9 @@ class ServeQuery(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("completed", C.c_uint64), ("batches", C.c_uint64), ("max_in_flight", C.c_uint64),
                 ("peak_footprint", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
-                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("wall_s", C.c_double), ("qps", C.c_double)]
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("wall_s", C.c_double), ("qps", C.c_double),
+                ("device_busy_s", C.c_double), ("lane_busy_s", C.c_double), ("host_submit_s", C.c_double),
+                ("host_retire_s", C.c_double), ("host_deliver_s", C.c_double)]
 
 
 P = C.c_void_p

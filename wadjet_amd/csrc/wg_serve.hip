@@ -68,6 +68,7 @@ struct Lane {
   int next_buf = 0;
   std::deque<Download> downloads;  // in flight, oldest first (at most two)
   std::vector<hipEvent_t> free_events;
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // timing: before the batch's first and after its last device operation
 };
 
 static uint64_t footprint_of(uint64_t est) {
@@ -99,6 +100,9 @@ struct wg_serve {
   std::thread engine;
   // statistics
   uint64_t batches = 0, max_in_flight = 0, peak_fp = 0, in_use_fp = 0;
+  hipEvent_t origin = nullptr;                  // device time zero of the intervals below
+  std::vector<std::pair<double, double>> busy;  // per batch: device interval (ms from origin)
+  double host_submit_s = 0, host_retire_s = 0, host_deliver_s = 0;
   wg::Clock::time_point first_arrival{}, last_done{};
   bool any = false;
   wg_status engine_error = WG_OK;
@@ -151,6 +155,7 @@ static wg_status to_batch_query(wg_serve* s, const wg_serve_query& q, wg_batch_q
 // Hand the results of the oldest download of L to its requests (the copy has
 // completed) and complete them.
 static void finish_download(wg_serve* s, Lane& L) {
+  const auto t_enter = Clock::now();
   Download d = std::move(L.downloads.front());
   L.downloads.pop_front();
   const uint64_t* h = L.host[d.buf];
@@ -173,12 +178,17 @@ static void finish_download(wg_serve* s, Lane& L) {
     r->done = true;
   }
   s->last_done = now;
+  s->host_deliver_s += std::chrono::duration<double>(now - t_enter).count();
   s->cv_done.notify_all();
 }
 
 // Complete the requests of a lane whose batch finished with status st, or
 // start their download; the lane is free for the next batch on return.
 static void retire_lane(wg_serve* s, Lane& L, wg_status st) {
+  const auto t_enter = Clock::now();
+  float a_ms = 0, b_ms = 0;
+  const bool timed = st == WG_OK && hipEventElapsedTime(&a_ms, s->origin, L.t0) == hipSuccess &&
+                     hipEventElapsedTime(&b_ms, s->origin, L.t1) == hipSuccess;
   Download d;
   d.reqs = L.reqs;
   d.off.assign(L.reqs.size(), 0);
@@ -250,6 +260,8 @@ static void retire_lane(wg_serve* s, Lane& L, wg_status st) {
   std::lock_guard<std::mutex> lk(s->mu);
   if (st != WG_OK) s->engine_error = st;
   const auto now = Clock::now();
+  if (timed) s->busy.push_back({a_ms, b_ms});
+  s->host_retire_s += std::chrono::duration<double>(now - t_enter).count();
   for (Request* r : now_done) {
     r->done_at = now;
     r->done = true;
@@ -306,7 +318,14 @@ static void engine_loop(wg_serve* s) {
       for (size_t i = 0; i < take.size(); ++i) qs[i] = take[i]->bq;
       L.reqs = take;
       L.busy = true;
-      const wg_status st = wg_batch_submit_async(L.batch, qs.data(), (int32_t)qs.size(), 1);
+      const auto t_sub = Clock::now();
+      wg_status st = hipEventRecord(L.t0, L.ctx->stream) == hipSuccess ? WG_OK : WG_ERR_DEVICE;
+      if (st == WG_OK) st = wg_batch_submit_async(L.batch, qs.data(), (int32_t)qs.size(), 1);
+      if (st == WG_OK && hipEventRecord(L.t1, L.ctx->stream) != hipSuccess) st = WG_ERR_DEVICE;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->host_submit_s += std::chrono::duration<double>(Clock::now() - t_sub).count();
+      }
       if (st != WG_OK) {
         for (Request* r : take) r->status = st;
         std::lock_guard<std::mutex> lk(s->mu);
@@ -377,9 +396,14 @@ wg_status wg_serve_create(wg_ctx* ctx, const wg_column* cols, int32_t n_cols, ui
     if (m)
       WG_HIP(hipMemcpy2D(s->ts_index.data(), 8, cols[ts_col].data, kIndexStride * 8, 8, m, hipMemcpyDeviceToHost));
   }
+  WG_HIP(hipEventCreate(&s->origin));
+  WG_HIP(hipEventRecord(s->origin, ctx->stream));
+  WG_HIP(hipEventSynchronize(s->origin));
   s->lanes.resize(lanes);
   for (Lane& L : s->lanes) {
     WG_TRY(wg_ctx_create(ctx->device, &L.ctx));
+    WG_HIP(hipEventCreate(&L.t0));
+    WG_HIP(hipEventCreate(&L.t1));
     // every lane's arena can hold the whole budget: the budget, not the arena, admits
     const uint64_t groups = std::max<uint64_t>(hbm_budget / 64 / 4, 1 << 14);
     WG_TRY(wg_batch_create(L.ctx, max_batch, groups, &L.batch));
@@ -409,10 +433,13 @@ wg_status wg_serve_destroy(wg_serve* s) {
     for (hipEvent_t e : L.free_events) (void)hipEventDestroy(e);
     for (uint64_t* h : L.host)
       if (h) (void)hipHostFree(h);
+    if (L.t0) (void)hipEventDestroy(L.t0);
+    if (L.t1) (void)hipEventDestroy(L.t1);
     (void)wg_ctx_destroy(L.ctx);
   }
   for (auto& kv : s->reqs) delete kv.second;
   for (Request* r : s->queue) (void)r;
+  if (s->origin) (void)hipEventDestroy(s->origin);
   delete s;
   return WG_OK;
 }
@@ -541,6 +568,26 @@ wg_status wg_serve_stats_get(wg_serve* s, wg_serve_stats* out) {
     out->wall_s = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
     out->qps = out->wall_s > 0 ? lat.size() / out->wall_s : 0;
   }
+  // device intervals: their union (the device busy with some batch) and their sum
+  std::vector<std::pair<double, double>> iv = s->busy;
+  std::sort(iv.begin(), iv.end());
+  double uni = 0, sum = 0, cur_a = 0, cur_b = -1;
+  for (const auto& [a, b] : iv) {
+    sum += b - a;
+    if (a > cur_b) {
+      if (cur_b > cur_a) uni += cur_b - cur_a;
+      cur_a = a;
+      cur_b = b;
+    } else {
+      cur_b = std::max(cur_b, b);
+    }
+  }
+  if (cur_b > cur_a) uni += cur_b - cur_a;
+  out->device_busy_s = uni / 1000;
+  out->lane_busy_s = sum / 1000;
+  out->host_submit_s = s->host_submit_s;
+  out->host_retire_s = s->host_retire_s;
+  out->host_deliver_s = s->host_deliver_s;
   return WG_OK;
 }
 
@@ -561,6 +608,8 @@ wg_status wg_serve_reset(wg_serve* s) {
   s->max_in_flight = 0;
   s->peak_fp = 0;
   s->any = false;
+  s->busy.clear();
+  s->host_submit_s = s->host_retire_s = s->host_deliver_s = 0;
   return WG_OK;
 }
 

@@ -4,7 +4,7 @@
         per-kernel dispatch count / total / average / min / max duration of a
         `rocprofv3 --kernel-trace --stats` run (its *_results.db, or the
         *_kernel_stats.csv of --output-format csv)
-    python -m wadjet_amd.profiling counters <dir>... [--kernel K]... --command "<cmd>"
+    python -m wadjet_amd.profiling counters <dir>... [--kernel K]... [--pick largest] --command "<cmd>"
         raw counters per kernel from several `rocprofv3 --pmc` runs (LDS bank
         conflicts, L2 hits and atomics, HBM read / write request counts and
         the bytes they move), with derived ratios
@@ -160,9 +160,11 @@ def load_traffic(path: str, match, steps_key="# steps profiled:"):
     return total / steps if found else None
 
 
-def pmc_all(dirs):
+def pmc_all(dirs, pick: str = "last"):
     """{kernel: {counter: value}} over several --pmc runs (each counter from
-    the run that collected it; the last dispatch of each kernel)."""
+    the run that collected it): per kernel its last dispatch, or with
+    pick="largest" the dispatch whose counters sum highest in each run (the
+    one big launch of a run that also makes small ones)."""
     out = defaultdict(dict)
     for d in dirs:
         for path in _files(d, "_results.db"):
@@ -174,8 +176,9 @@ def pmc_all(dirs):
             con.close()
             last = {}
             for (k, disp), cv in per.items():
-                if k not in last or disp > last[k][0]:
-                    last[k] = (disp, cv)
+                key = sum(cv.values()) if pick == "largest" else disp
+                if k not in last or key > last[k][0]:
+                    last[k] = (key, cv)
             for k, (_, cv) in last.items():
                 out[k].update(cv)
     return out
@@ -188,8 +191,9 @@ DERIVED = [
 ]
 
 
-def format_counters(data, kernels, commands, notes=()):
-    out = ["# rocprofv3 --pmc counters per kernel (last dispatch of each kernel in the run)",
+def format_counters(data, kernels, commands, notes=(), pick: str = "last"):
+    which = "last dispatch" if pick == "last" else "largest dispatch"
+    out = [f"# rocprofv3 --pmc counters per kernel ({which} of each kernel in the run)",
            "# command(s), run from the repository root:"]
     out += [f"#   {c}" for c in commands]
     out += [f"# {n}" for n in notes]
@@ -231,9 +235,10 @@ def main():
     q.add_argument("--kernel", action="append", default=[])
     q.add_argument("--command", action="append", default=[])
     q.add_argument("--note", action="append", default=[])
+    q.add_argument("--pick", choices=["last", "largest"], default="last")
     a = ap.parse_args()
     if a.cmd == "counters":
-        print(format_counters(pmc_all(a.dirs), a.kernel, a.command, a.note), end="")
+        print(format_counters(pmc_all(a.dirs, a.pick), a.kernel, a.command, a.note, a.pick), end="")
     elif a.cmd == "stats":
         print(format_stats(kernel_stats(a.dir), a.command), end="")
     else:
