@@ -131,6 +131,16 @@ wg_status wg_ctx_info(wg_ctx* ctx, int32_t* cus, uint64_t* hbm_bytes, int32_t* l
 wg_status wg_ctx_timer_start(wg_ctx* ctx);
 wg_status wg_ctx_timer_stop(wg_ctx* ctx, float* ms);
 
+/* Failure injection, for tests of the error paths: the nth device allocation
+ * made through this context from now on (the library's own and wg_malloc's)
+ * fails as out of memory, WG_ERR_OOM, and with sticky != 0 every allocation
+ * after it fails too, as under real memory exhaustion. nth = 0 clears it. The
+ * reference's tests force the same paths with a small memory.Tracker budget
+ * (internal/memory) and FatalEvalPanic (pipeline.go:45-57). After any
+ * WG_ERR_OOM a handle stays valid — every buffer it held is kept, and destroy
+ * releases it; see wg_agg_consume for what the aggregate then holds. */
+wg_status wg_ctx_inject_alloc_failure(wg_ctx* ctx, int64_t nth, int32_t sticky);
+
 /* Device memory helpers so callers need no other GPU runtime. */
 wg_status wg_malloc(wg_ctx* ctx, uint64_t bytes, void** out);
 wg_status wg_free(wg_ctx* ctx, void* ptr);
@@ -171,7 +181,18 @@ wg_status wg_agg_destroy(wg_agg* h);
  * (filter.go:408-418).
  * mem = WG_MEM_HOST: the call copies what it needs before returning and
  * coalesces small batches into large device batches. mem = WG_MEM_DEVICE:
- * pointers are device-resident; the launch is enqueued on the ctx stream. */
+ * pointers are device-resident; the launch is enqueued on the ctx stream.
+ * Out of device memory (WG_ERR_OOM): when every allocation the call needed
+ * failed before its rows reached the aggregate (scratch, staging, capacity,
+ * the partitioned passes' buffers), the aggregate holds exactly what it held
+ * before and the call may be retried or the state handed over
+ * (wg_agg_spill_run). When the table could not grow while the batch was
+ * being applied, the message says the aggregate holds part of the call's
+ * input; wg_agg_consume, _merge, _merge_run, _finalize and
+ * wg_join_probe_agg then return WG_ERR_STATE until wg_agg_reset (the
+ * adapter fails the query or re-runs it elsewhere, as pipeline.go:45-57
+ * turns a failed operator into a query error). The same holds for
+ * wg_agg_merge, wg_agg_merge_run, wg_agg_finalize and wg_join_probe_agg. */
 wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows,
                          const uint32_t* sel, uint64_t n_sel, const wg_predicate* preds,
                          int32_t n_preds, int32_t mem);
@@ -220,7 +241,9 @@ wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint
  * rules (mergeFlatAccumRow, aggregate.go:5155-5228) applied on the device.
  * WG_ERR_INVALID when the run's aggregates or group columns differ. */
 wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size);
-/* Drop all groups and staged rows; keep the spec and the table memory. */
+/* Drop all groups and staged rows; keep the spec and the table memory. Also
+ * makes an aggregate usable again after a call left it holding part of a
+ * failed input. */
 wg_status wg_agg_reset(wg_agg* h);
 /* Device bytes held by this aggregate (table + state + staging). */
 wg_status wg_agg_footprint(wg_agg* h, uint64_t* bytes);
@@ -309,7 +332,9 @@ wg_status wg_join_create_dual(wg_ctx* ctx, int32_t n_build_cols, const int32_t* 
                               wg_join** out);
 wg_status wg_join_destroy(wg_join* j);
 /* HashJoin.Build per batch (join.go:1019-1065): rows with a NULL key are
- * skipped. Build columns are retained on the device for the gathers. */
+ * skipped. Build columns are retained on the device for the gathers. Every
+ * allocation happens before the call's rows reach the table: after
+ * WG_ERR_OOM no row of the call was added, and the call may be repeated. */
 wg_status wg_join_build_append(wg_join* j, const wg_column* cols, uint64_t n_rows, int32_t mem);
 /* End of build (join.go:1126-1152). */
 wg_status wg_join_build_finish(wg_join* j);

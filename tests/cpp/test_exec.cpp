@@ -847,6 +847,69 @@ static void test_late_materialized_join(wg_ctx* g) {
   }
 }
 
+static void test_allocation_failures_reach_the_pipeline(wg_ctx* g) {
+  // The reference forces its error paths with a small memory budget and
+  // FatalEvalPanic (pipeline.go:45-57). Here every device allocation a
+  // pipeline run makes is in turn made to fail: Run returns the error with
+  // its code (out of memory, so an adapter can fall back to the CPU
+  // operator), serial and with 4 workers, and the next run on the same
+  // context is exact.
+  std::vector<BatchPtr> src;
+  std::map<int64_t, std::pair<int64_t, int64_t>> want;  // k -> (sum v, count)
+  for (int m = 0; m < 20; ++m) {
+    auto x = make_batch({{"k", TypeID::Int64}, {"v", TypeID::Int64}}, 2048);
+    for (int i = 0; i < 2048; ++i) {
+      const int64_t r = m * 2048 + i;
+      x->columns[0]->i64[i] = (r * 7919) % 5000;
+      x->columns[1]->i64[i] = r * 3 - 1000;
+      auto& w = want[x->columns[0]->i64[i]];
+      w.first += x->columns[1]->i64[i];
+      w.second += 1;
+    }
+    src.push_back(x);
+  }
+  for (int workers : {0, 4}) {
+    int failures = 0;
+    bool completed = false;
+    for (int k = 1; k < 400 && !completed; ++k) {
+      BatchSource s(src);
+      GpuHashAggregate agg(g, {"k"}, {{WG_AGG_SUM, "v", "s"}, {WG_AGG_COUNT, "", "c"}}, 64);
+      Pipeline p{&s, {}, &agg, workers};
+      Context ctx;
+      wg_ctx_inject_alloc_failure(g, k, 0);
+      const Status st = p.Run(ctx);
+      int groups = 0;
+      Status st2;  // ok
+      std::map<int64_t, std::pair<int64_t, int64_t>> got;
+      if (st.ok()) {
+        for (;;) {  // the output drain allocates too
+          BatchPtr out;
+          st2 = agg.Next(ctx, &out);
+          if (!st2.ok() || !out) break;
+          for (int i = 0; i < out->len; ++i, ++groups)
+            got[out->columns[0]->i64[i]] = {(int64_t)out->columns[1]->f64[i], out->columns[2]->i64[i]};
+        }
+      }
+      wg_ctx_inject_alloc_failure(g, 0, 0);
+      const Status& bad = st.ok() ? st2 : st;
+      if (bad.ok()) {
+        completed = true;
+        EXPECT(groups == (int)want.size(), "workers %d: %d groups, want %zu", workers, groups, want.size());
+        for (auto& [key, sv] : want) {
+          auto it = got.find(key);
+          EXPECT(it != got.end() && it->second == sv, "workers %d: group %lld", workers, (long long)key);
+        }
+        continue;
+      }
+      ++failures;
+      EXPECT(bad.out_of_memory(), "workers %d, allocation %d: code %d: %s", workers, k, bad.code, bad.msg.c_str());
+      EXPECT(bad.msg.find("out of memory") != std::string::npos, "message: %s", bad.msg.c_str());
+    }
+    EXPECT(completed, "workers %d: the sequence never completed", workers);
+    EXPECT(failures >= 3, "workers %d: only %d allocations failed", workers, failures);
+  }
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   test_kernel_filter_and_pipeline();
@@ -868,6 +931,7 @@ int main(int argc, char** argv) {
     test_capped_partial_agg(g);
     test_shuffle_router(g);
     test_late_materialized_join(g);
+    test_allocation_failures_reach_the_pipeline(g);
     wg_ctx_destroy(g);
   }
   std::printf("%s: %d failure(s)\n", mode.c_str(), g_fail);

@@ -137,6 +137,7 @@ SIGNATURES: dict[str, list] = {
     "wg_pq_scan_runs": [P, U64, I32, U64, P, U64, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)],
     "wg_parquet_decode_page": [P, P, P, P, U64],
     "wg_parquet_decode_pages": [P, P, U64, P, P, U64],
+    "wg_ctx_inject_alloc_failure": [P, I64, I32],
     "wg_pq_stage": [P, C.c_char_p, P, U64, I32, I32, I32, C.POINTER(C.c_void_p)],
     "wg_pq_staged_info": [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)],
     "wg_pq_staged_pages": [P, P, U64],

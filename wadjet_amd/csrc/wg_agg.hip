@@ -795,15 +795,22 @@ static wg_status fill_table(wg_agg* h, uint64_t* slots, uint64_t cap) {
   return WG_OK;
 }
 
+// A filled table of cap slots, not yet attached to h.
+static wg_status new_table(wg_agg* h, uint64_t cap, uint64_t** out) {
+  uint64_t* s = nullptr;
+  WG_ALLOC(h->ctx, &s, (cap + kSideSlots) * (uint64_t)h->pg.stride * 8, "a group table");
+  const wg_status st = fill_table(h, s, cap);
+  if (st != WG_OK) {
+    (void)hipFree(s);
+    return st;
+  }
+  *out = s;
+  return WG_OK;
+}
+
 static wg_status alloc_table(wg_agg* h, uint64_t cap) {
   uint64_t* s = nullptr;
-  const uint64_t bytes = (cap + kSideSlots) * (uint64_t)h->pg.stride * 8;
-  if (hipMalloc(&s, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    set_error("hipMalloc(%llu) for a %llu-slot group table failed", (unsigned long long)bytes, (unsigned long long)cap);
-    return WG_ERR_OOM;
-  }
-  WG_TRY(fill_table(h, s, cap));
+  WG_TRY(new_table(h, cap, &s));
   h->slots = s;
   h->cap = cap;
   return WG_OK;
@@ -812,7 +819,7 @@ static wg_status alloc_table(wg_agg* h, uint64_t cap) {
 wg_status ensure_base(wg_agg* h) {
   if (h->n_sumd == 0 || h->base) return WG_OK;
   const uint64_t words = (h->cap + kSideSlots) * (uint64_t)h->n_sumd * 2;
-  WG_HIP(hipMalloc(&h->base, words * 8));
+  WG_ALLOC(h->ctx, &h->base, words * 8, "exact Int128 bases");
   WG_TRY(zero_dev(h->ctx, h->base, words));
   return WG_OK;
 }
@@ -827,15 +834,29 @@ GlobalTable table_of(wg_agg* h) {
 }
 
 // Grow to a table of new_cap slots, moving every live slot (and its base).
+// The new table and bases are allocated first: when that fails, h keeps the
+// table it had.
 wg_status grow_to(wg_agg* h, uint64_t new_cap) {
   if (new_cap <= h->cap) return WG_OK;
+  uint64_t* fresh = nullptr;
+  uint64_t* fresh_base = nullptr;
+  WG_TRY(new_table(h, new_cap, &fresh));
+  if (h->base) {
+    const uint64_t words = (new_cap + kSideSlots) * (uint64_t)h->n_sumd * 2;
+    wg_status st = dev_alloc(h->ctx, (void**)&fresh_base, words * 8, nullptr, "exact Int128 bases");
+    if (st == WG_OK) st = zero_dev(h->ctx, fresh_base, words);
+    if (st != WG_OK) {
+      (void)hipFree(fresh);
+      (void)hipFree(fresh_base);
+      return st;
+    }
+  }
   uint64_t* old = h->slots;
-  uint64_t old_cap = h->cap;
+  const uint64_t old_cap = h->cap;
   uint64_t* old_base = h->base;
-  h->slots = nullptr;
-  h->base = nullptr;
-  WG_TRY(alloc_table(h, new_cap));
-  if (old_base) WG_TRY(ensure_base(h));
+  h->slots = fresh;
+  h->cap = new_cap;
+  h->base = fresh_base;
   // the rehash claims slots: reset the groups counter first
   WG_TRY(zero_dev(h->ctx, h->counters + CTR_GROUPS, 1));
   GlobalTable T = table_of(h);
@@ -873,9 +894,9 @@ wg_status agg_ensure_dense(wg_agg* h, uint64_t rows, uint64_t keep) {
       if (*c) (void)hipFree(*c);
     }
   } release{&d, &db, &dn};
-  WG_HIP(hipMalloc(&d, rows * S * 8));
-  if (B) WG_HIP(hipMalloc(&db, rows * B * 8));
-  WG_HIP(hipMalloc(&dn, rows));
+  WG_ALLOC(h->ctx, &d, rows * S * 8, "the dense output");
+  if (B) WG_ALLOC(h->ctx, &db, rows * B * 8, "the dense output");
+  WG_ALLOC(h->ctx, &dn, rows, "the dense output");
   if (keep) {
     WG_HIP(hipMemcpyAsync(d, h->dense, keep * S * 8, hipMemcpyDeviceToDevice, h->ctx->stream));
     if (B) WG_HIP(hipMemcpyAsync(db, h->dense_base, keep * B * 8, hipMemcpyDeviceToDevice, h->ctx->stream));
@@ -935,17 +956,10 @@ wg_status ensure_overflow_buffers(wg_agg* h) { return ensure_overflow_for(h, h->
 // a flush may spill (flushes write without a further bound check).
 wg_status ensure_overflow_for(wg_agg* h, uint64_t grid, int32_t lds_slots) {
   const uint64_t rows_cap = std::max<uint64_t>(grid * h->tile_rows * 4, 1ull << 20);
-  if (h->ovf_rows_cap < rows_cap) {
-    if (h->ovf_rows) WG_HIP(hipFree(h->ovf_rows));
-    WG_HIP(hipMalloc(&h->ovf_rows, rows_cap * 4));
-    h->ovf_rows_cap = rows_cap;
-  }
+  if (h->ovf_rows_cap < rows_cap) WG_TRY(regrow(h->ctx, &h->ovf_rows, &h->ovf_rows_cap, rows_cap, 4, "overflow rows"));
   const uint64_t part_cap = grid * (uint64_t)std::max(lds_slots, 1);
-  if (h->ovf_part_cap < part_cap) {
-    if (h->ovf_part) WG_HIP(hipFree(h->ovf_part));
-    WG_HIP(hipMalloc(&h->ovf_part, part_cap * (uint64_t)h->pg.stride * 8));
-    h->ovf_part_cap = part_cap;
-  }
+  if (h->ovf_part_cap < part_cap)
+    WG_TRY(regrow(h->ctx, &h->ovf_part, &h->ovf_part_cap, part_cap, (uint64_t)h->pg.stride * 8, "overflow partials"));
   return WG_OK;
 }
 
@@ -1038,6 +1052,7 @@ static wg_status consume_once(wg_agg* h, const ColSet& cs, const uint32_t* sel, 
   }
   WG_HIP(hipEventRecord(h->ev0, h->ctx->stream));
   const size_t lds = h->lds_slots > 0 ? lds_table_bytes(h->lds_slots, h->pg.stride) : 0;
+  h->touched = true;
   WG_TRY(launch_consume(h->lds_slots > 0, nv, grid, h->block, lds, h->ctx->stream, P));
   WG_HIP(hipGetLastError());
   WG_HIP(hipEventRecord(h->ev1, h->ctx->stream));
@@ -1065,6 +1080,7 @@ wg_status agg_merge_slots(wg_agg* h, const uint64_t* src, uint64_t src_cap, cons
   T.max_probe = h->cap;
   WG_TRY(zero_dev(h->ctx, h->counters + CTR_STOPPED, 1));
   const uint64_t total = src_cap + kSideSlots;
+  h->touched = true;
   hipLaunchKernelGGL(agg_merge_table_kernel, dim3(grid_for(total, 256, 8192)), dim3(256), 0, h->ctx->stream, src,
                      src_cap, src_base, T, h->base, h->pg, (const int32_t*)h->d_sumd_word,
                      (const int32_t*)h->d_sumd_limbs, h->n_sumd);
@@ -1080,10 +1096,11 @@ wg_status agg_merge_slots(wg_agg* h, const uint64_t* src, uint64_t src_cap, cons
 wg_status agg_merge_partials(wg_agg* h, const uint64_t* src, uint64_t n_part) {
   uint64_t* recs = nullptr;
   const uint64_t bytes = n_part * (uint64_t)h->pg.stride * 8;
-  WG_HIP(hipMalloc(&recs, bytes));  // out of the way of later launches
+  WG_ALLOC(h->ctx, &recs, bytes, "overflow records");  // out of the way of later launches
   WG_HIP(hipMemcpyAsync(recs, src, bytes, hipMemcpyDeviceToDevice, h->ctx->stream));
   GlobalTable T = table_of(h);
   T.max_probe = h->cap;
+  h->touched = true;
   hipLaunchKernelGGL(agg_merge_part_kernel, dim3(grid_for(n_part, 256)), dim3(256), 0, h->ctx->stream, recs, n_part,
                      T, h->pg);
   WG_HIP(hipGetLastError());
@@ -1106,7 +1123,7 @@ static wg_status resolve_overflow(wg_agg* h, const ColSet& cs, const PredDesc* p
   if (n_part) WG_TRY(agg_merge_partials(h, h->ovf_part, n_part));
   if (n_rows) {
     uint32_t* rows = nullptr;
-    WG_HIP(hipMalloc(&rows, n_rows * 4));
+    WG_ALLOC(h->ctx, &rows, n_rows * 4, "replayed rows");
     WG_HIP(hipMemcpyAsync(rows, h->ovf_rows, n_rows * 4, hipMemcpyDeviceToDevice, h->ctx->stream));
     wg_status st = consume_range(h, cs, rows, n_rows, pred, depth + 1);
     (void)hipStreamSynchronize(h->ctx->stream);
@@ -1172,25 +1189,46 @@ wg_status agg_consume_device(wg_agg* h, const ColSet& cs, const uint32_t* sel, u
 
 namespace wg {
 
-static wg_status stage_init(wg_agg* h) {
-  HostStage& S = h->stage;
-  if (S.cap_rows) return WG_OK;
-  S.cap_rows = 1ull << 20;
+static void stage_release(HostStage& S) {
+  for (void* p : S.pinned) (void)hipHostFree(p);
+  for (uint64_t* p : S.pvalid) (void)hipHostFree(p);
+  for (void* p : S.dev) (void)hipFree(p);
+  for (uint64_t* p : S.dvalid) (void)hipFree(p);
+  S.pinned.clear();
+  S.pvalid.clear();
+  S.dev.clear();
+  S.dvalid.clear();
+  S.cap_rows = 0;
+}
+
+static wg_status stage_alloc(wg_agg* h, HostStage& S, uint64_t cap_rows) {
   S.pinned.assign(h->n_cols, nullptr);
   S.pvalid.assign(h->n_cols, nullptr);
   S.dev.assign(h->n_cols, nullptr);
   S.dvalid.assign(h->n_cols, nullptr);
   for (int c = 0; c < h->n_cols; ++c) {
-    const uint64_t bytes = S.cap_rows * type_bytes(h->col_types[c]);
-    WG_HIP(hipHostMalloc(&S.pinned[c], bytes, hipHostMallocDefault));
-    WG_HIP(hipMalloc(&S.dev[c], bytes));
+    const uint64_t bytes = cap_rows * type_bytes(h->col_types[c]);
+    WG_CHECK(hipHostMalloc(&S.pinned[c], bytes, hipHostMallocDefault) == hipSuccess, WG_ERR_OOM,
+             "pinned staging of %llu bytes failed", (unsigned long long)bytes);
+    WG_ALLOC(h->ctx, &S.dev[c], bytes, "host-batch staging");
     if (h->col_nullable[c]) {
-      WG_HIP(hipHostMalloc((void**)&S.pvalid[c], S.cap_rows / 8, hipHostMallocDefault));
-      WG_HIP(hipMalloc((void**)&S.dvalid[c], S.cap_rows / 8));
-      memset(S.pvalid[c], 0, S.cap_rows / 8);
+      WG_CHECK(hipHostMalloc((void**)&S.pvalid[c], cap_rows / 8, hipHostMallocDefault) == hipSuccess, WG_ERR_OOM,
+               "pinned staging of %llu bytes failed", (unsigned long long)(cap_rows / 8));
+      WG_ALLOC(h->ctx, &S.dvalid[c], cap_rows / 8, "host-batch staging");
+      memset(S.pvalid[c], 0, cap_rows / 8);
     }
   }
+  S.cap_rows = cap_rows;
   return WG_OK;
+}
+
+// The staging buffers of host batches, all or none.
+static wg_status stage_init(wg_agg* h) {
+  HostStage& S = h->stage;
+  if (S.cap_rows) return WG_OK;
+  const wg_status st = stage_alloc(h, S, 1ull << 20);
+  if (st != WG_OK) stage_release(S);
+  return st;
 }
 
 wg_status agg_flush_stage(wg_agg* h) {
@@ -1208,10 +1246,11 @@ wg_status agg_flush_stage(wg_agg* h) {
   }
   const PredDesc none[2] = {{-1, 0, 0, 0.0}, {-1, 0, 0, 0.0}};
   const uint64_t n = S.rows;
-  S.rows = 0;
   if (h->wide) WG_TRY(wide_attach(h, cs, nullptr, n, n));
-  // consume syncs on its counter read-back, after which the pinned buffers are free
+  // consume syncs on its counter read-back, after which the pinned buffers
+  // are free; the staged rows stay staged until they have been consumed
   WG_TRY(agg_consume_device(h, cs, nullptr, n, none));
+  S.rows = 0;
   for (int c = 0; c < h->n_cols; ++c)
     if (S.pvalid[c]) memset(S.pvalid[c], 0, (n + 63) / 64 * 8);
   return WG_OK;
@@ -1281,9 +1320,34 @@ static wg_status consume_host(wg_agg* h, const wg_column* cols, uint64_t n_rows,
       memcpy((char*)S.pinned[c] + d * w, (const char*)cols[c].data + row * w, w);
       if (S.pvalid[c] && host_valid(cols[c], row)) S.pvalid[c][d >> 6] |= 1ull << (d & 63);
     }
-    if (++S.rows == S.cap_rows) WG_TRY(agg_flush_stage(h));
+    if (++S.rows == S.cap_rows) {
+      const wg_status st = agg_flush_stage(h);
+      if (st != WG_OK) {
+        h->touched = true;  // part of this call's rows are staged or consumed
+        return st;
+      }
+    }
   }
   return WG_OK;
+}
+
+wg_status agg_guard_begin(wg_agg* h) {
+  WG_CHECK(!h->poisoned, WG_ERR_STATE,
+           "the aggregate holds part of an input whose call failed; only wg_agg_reset or wg_agg_destroy apply");
+  h->touched = false;
+  return WG_OK;
+}
+
+wg_status agg_guard_end(wg_agg* h, wg_status st) {
+  if (st != WG_OK && h->touched) {
+    h->poisoned = true;
+    h->dirty = true;  // the failed call may have left groups anywhere in the table: reset refills it all
+    const std::string why = wg_last_error();
+    set_error("%s (the aggregate now holds part of this call's input: only wg_agg_reset or wg_agg_destroy apply)",
+              why.c_str());
+  }
+  h->touched = false;
+  return st;
 }
 
 static wg_status free_dense(wg_agg* h) {
@@ -1329,17 +1393,24 @@ wg_status wg_agg_create(wg_ctx* ctx, int32_t n_cols, const int32_t* col_types, c
     return st;
   }
   if (h->n_sumd) {
-    if (hipMalloc(&h->d_sumd_word, h->n_sumd * 8) != hipSuccess) { delete h; set_error("hipMalloc"); return WG_ERR_OOM; }
+    st = dev_alloc(ctx, (void**)&h->d_sumd_word, h->n_sumd * 8, nullptr, "the DECIMAL sum layout");
+    if (st != WG_OK) {
+      wg_agg_destroy(h);
+      return st;
+    }
     h->d_sumd_limbs = h->d_sumd_word + h->n_sumd;
     std::vector<int32_t> tmp(h->sumd_word);
     tmp.insert(tmp.end(), h->sumd_limbs.begin(), h->sumd_limbs.end());
     WG_HIP(hipMemcpy(h->d_sumd_word, tmp.data(), h->n_sumd * 8, hipMemcpyHostToDevice));
   }
-  if (hipMalloc(&h->counters, kAggCounterWords * 8) != hipSuccess ||
-      hipHostMalloc((void**)&h->pinned, kAggPinnedWords * 8, hipHostMallocDefault) != hipSuccess) {
-    set_error("device or pinned allocation of the counters failed");
+  st = dev_alloc(ctx, (void**)&h->counters, kAggCounterWords * 8, nullptr, "the aggregate's counters");
+  if (st == WG_OK && hipHostMalloc((void**)&h->pinned, kAggPinnedWords * 8, hipHostMallocDefault) != hipSuccess) {
+    set_error("pinned allocation of the counters failed");
+    st = WG_ERR_OOM;
+  }
+  if (st != WG_OK) {
     wg_agg_destroy(h);
-    return WG_ERR_OOM;
+    return st;
   }
   WG_TRY(zero_dev(ctx, h->counters, kAggCounterWords));
   const uint64_t hint = std::max<uint64_t>(group_hint, 1024);
@@ -1377,20 +1448,14 @@ wg_status wg_agg_destroy(wg_agg* h) {
   (void)hipFree(h->hot_buf);
   if (h->part_host) (void)hipHostFree(h->part_host);
   if (h->pinned) (void)hipHostFree(h->pinned);
-  for (void* p : h->stage.pinned) (void)hipHostFree(p);
-  for (uint64_t* p : h->stage.pvalid) (void)hipHostFree(p);
-  for (void* p : h->stage.dev) (void)hipFree(p);
-  for (uint64_t* p : h->stage.dvalid) (void)hipFree(p);
+  stage_release(h->stage);
   if (h->wide) (void)wide_free(h);
   delete h;
   return WG_OK;
 }
 
-wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
-                         const wg_predicate* preds, int32_t n_preds, int32_t mem) {
-  clear_error();
-  WG_CHECK(h && cols, WG_ERR_INVALID, "null argument");
-  WG_CHECK(!h->finalized, WG_ERR_STATE, "consume after finalize");
+static wg_status consume_call(wg_agg* h, const wg_column* cols, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
+                              const wg_predicate* preds, int32_t n_preds, int32_t mem) {
   WG_CHECK(n_preds >= 0 && n_preds <= 2 && (n_preds == 0 || preds), WG_ERR_INVALID, "%d predicates (max 2)", n_preds);
   for (int c = 0; c < h->n_cols; ++c) {
     WG_CHECK(cols[c].type == h->col_types[c], WG_ERR_INVALID, "column %d: type %d, created as %d", c, cols[c].type, h->col_types[c]);
@@ -1445,10 +1510,16 @@ wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, cons
   return agg_consume_device(h, cs, sel, n, pd);
 }
 
-wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
+wg_status wg_agg_consume(wg_agg* h, const wg_column* cols, uint64_t n_rows, const uint32_t* sel, uint64_t n_sel,
+                         const wg_predicate* preds, int32_t n_preds, int32_t mem) {
   clear_error();
-  WG_CHECK(dst && src && dst != src, WG_ERR_INVALID, "bad handles");
-  WG_CHECK(!dst->finalized && !src->finalized, WG_ERR_STATE, "merge after finalize");
+  WG_CHECK(h && cols, WG_ERR_INVALID, "null argument");
+  WG_CHECK(!h->finalized, WG_ERR_STATE, "consume after finalize");
+  WG_TRY(agg_guard_begin(h));
+  return agg_guard_end(h, consume_call(h, cols, n_rows, sel, n_sel, preds, n_preds, mem));
+}
+
+static wg_status merge_call(wg_agg* dst, wg_agg* src) {
   WG_CHECK(dst->pg.stride == src->pg.stride && dst->pg.n_ops == src->pg.n_ops && dst->ctx == src->ctx,
            WG_ERR_INVALID, "merge of aggregates with different specs");
   WG_CHECK(dst->wide == src->wide, WG_ERR_INVALID, "merge of aggregates with different keys");
@@ -1473,10 +1544,16 @@ wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
   return wg_agg_reset(src);
 }
 
-wg_status wg_agg_finalize(wg_agg* h) {
+wg_status wg_agg_merge(wg_agg* dst, wg_agg* src) {
   clear_error();
-  WG_CHECK(h, WG_ERR_INVALID, "null handle");
-  if (h->finalized) return WG_OK;
+  WG_CHECK(dst && src && dst != src, WG_ERR_INVALID, "bad handles");
+  WG_CHECK(!dst->finalized && !src->finalized, WG_ERR_STATE, "merge after finalize");
+  WG_CHECK(!src->poisoned, WG_ERR_STATE, "merge of an aggregate that holds part of a failed call's input");
+  WG_TRY(agg_guard_begin(dst));
+  return agg_guard_end(dst, merge_call(dst, src));
+}
+
+static wg_status finalize_call(wg_agg* h) {
   WG_TRY(agg_flush_stage(h));
   // rows [0, n_dense) already hold the groups a partitioned consume emitted;
   // the table's live slots are compacted after them
@@ -1499,6 +1576,14 @@ wg_status wg_agg_finalize(wg_agg* h) {
   h->out_n = c[CTR_LDS_ROWS];
   h->finalized = true;
   return WG_OK;
+}
+
+wg_status wg_agg_finalize(wg_agg* h) {
+  clear_error();
+  WG_CHECK(h, WG_ERR_INVALID, "null handle");
+  if (h->finalized) return WG_OK;
+  WG_TRY(agg_guard_begin(h));
+  return agg_guard_end(h, finalize_call(h));
 }
 
 wg_status wg_agg_num_groups(wg_agg* h, uint64_t* out) {
@@ -1556,8 +1641,8 @@ wg_status wg_agg_fetch(wg_agg* h, uint64_t offset, uint64_t count, void* const* 
   void* tmp = nullptr;
   uint8_t* tmpv = nullptr;
   if (host) {
-    WG_HIP(hipMalloc(&tmp, count * 8));
-    WG_HIP(hipMalloc(&tmpv, count));
+    WG_ALLOC(h->ctx, &tmp, count * 8, "fetch staging");
+    WG_ALLOC(h->ctx, &tmpv, count, "fetch staging");
   }
   auto finish = [&](void* dst, const void* src, uint64_t bytes) -> wg_status {
     if (!host || !dst) return WG_OK;
@@ -1685,6 +1770,7 @@ wg_status wg_agg_reset(wg_agg* h) {
   h->finalized = false;
   h->out_n = 0;
   if (h->wide) WG_TRY(wide_reset(h));
+  h->poisoned = false;
   return WG_OK;
 }
 

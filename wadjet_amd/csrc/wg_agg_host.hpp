@@ -104,6 +104,12 @@ struct wg_agg {
   float st_kernel_ms = 0.f;  // consume-kernel time of the last consume (HIP events)
   uint32_t st_launches = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // failure handling: `touched` is set when a call starts changing the
+  // aggregate's contents; a call that fails after that leaves it `poisoned`
+  // (holding part of that call's input) and every later call but reset and
+  // destroy is refused
+  bool touched = false;
+  bool poisoned = false;
   // finalize output
   bool finalized = false;
   uint64_t out_n = 0;
@@ -155,6 +161,10 @@ wg_status wide_dict_host(wg_agg* h, std::vector<uint64_t>* pairs);
 wg_status wide_remap_slots(wg_agg* dst, wg_agg* src, uint64_t** remapped);
 // Fold groups emitted to the dense output back into the table (before any
 // consume, merge or merge-run that must see them there).
+// Entry and exit of a call that changes the aggregate: refuse a poisoned one;
+// poison it when the call failed after changing it.
+wg_status agg_guard_begin(wg_agg* h);
+wg_status agg_guard_end(wg_agg* h, wg_status st);
 wg_status agg_settle(wg_agg* h);
 // The dense output arrays with room for `rows` groups, keeping the first `keep`.
 wg_status agg_ensure_dense(wg_agg* h, uint64_t rows, uint64_t keep);

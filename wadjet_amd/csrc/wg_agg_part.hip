@@ -1010,11 +1010,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     // scratch: sample table, histogram, threshold/count, keys, partial slots
     const uint64_t words = (2ull << kSampleBits) + kHotHistBins / 2 + 1 + (uint64_t)H +
                            (uint64_t)G * H * h->pg.stride;
-    if (h->hot_buf_words < words) {
-      if (h->hot_buf) WG_HIP(hipFree(h->hot_buf));
-      WG_HIP(hipMalloc(&h->hot_buf, words * 8));
-      h->hot_buf_words = words;
-    }
+    if (h->hot_buf_words < words) WG_TRY(regrow(ctx, &h->hot_buf, &h->hot_buf_words, words, 8, "heavy-hitter scratch"));
     X.tab = h->hot_buf;
     X.hist = (uint32_t*)(X.tab + (2ull << kSampleBits));
     X.thr = X.hist + kHotHistBins;
@@ -1029,19 +1025,12 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   // scratch: counts, offsets, boundaries (device) and boundaries (pinned host)
   const uint64_t ncnt = (uint64_t)np * G;
   const uint64_t cnt_words = 2 * ncnt + np + 1;
-  if (h->part_cnt_cap < cnt_words) {
-    if (h->part_cnt) WG_HIP(hipFree(h->part_cnt));
-    WG_HIP(hipMalloc(&h->part_cnt, cnt_words * 4));
-    h->part_cnt_cap = cnt_words;
-  }
+  if (h->part_cnt_cap < cnt_words) WG_TRY(regrow(ctx, &h->part_cnt, &h->part_cnt_cap, cnt_words, 4, "partition counts"));
   size_t scan_bytes = 0;
   WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->part_cnt, h->part_cnt + ncnt, (int)ncnt,
                                           ctx->stream));
-  if (scan_bytes > h->part_scan_bytes) {
-    if (h->part_scan) WG_HIP(hipFree(h->part_scan));
-    WG_HIP(hipMalloc(&h->part_scan, scan_bytes));
-    h->part_scan_bytes = scan_bytes;
-  }
+  if (scan_bytes > h->part_scan_bytes)
+    WG_TRY(regrow(ctx, &h->part_scan, &h->part_scan_bytes, scan_bytes, 1, "partition scan scratch"));
   uint32_t* counts = h->part_cnt;
   uint32_t* offs = h->part_cnt + ncnt;
   uint32_t* dbound = h->part_cnt + 2 * ncnt;
@@ -1085,16 +1074,36 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   const uint64_t total = bound[np];
   h->st_glb = total;  // rows scattered to the partitions (the cold rows)
 
-  // scratch: partitioned records
+  // Everything the remaining passes need is allocated here, before the
+  // scatter changes any state: records, work items, overflow lists and the
+  // dense output. (A failure to grow the table while draining the items is
+  // the one allocation failure that can come later.)
   const uint64_t need = std::max<uint64_t>(total, 1) * RW;
-  if (h->part_buf_words < need) {
-    if (h->part_buf) WG_HIP(hipFree(h->part_buf));
-    WG_HIP(hipMalloc(&h->part_buf, need * 8));
-    h->part_buf_words = need;
-  }
+  if (h->part_buf_words < need) WG_TRY(regrow(ctx, &h->part_buf, &h->part_buf_words, need, 8, "partition records"));
   P.recs = h->part_buf;
+  uint64_t n_items = 0, dense_bound = (uint64_t)H;
+  for (uint32_t p = 0; p < np; ++p) {
+    const uint64_t len = bound[p + 1] - bound[p];
+    n_items += (len + chunk - 1) / chunk;
+    if (len <= chunk) dense_bound += std::min<uint64_t>(len, (uint64_t)L);
+  }
+  const uint64_t ibytes = std::max<uint64_t>(n_items, 1) * sizeof(PartItem) + 64;
+  if (h->part_items_bytes < ibytes)
+    WG_TRY(regrow(ctx, &h->part_items, &h->part_items_bytes, ibytes, 1, "partition work items"));
+  const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
+  const int per_cu_agg = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
+  const int grid_max = ctx->cus * per_cu_agg;
+  // overflow lists with headroom for every in-flight item (see the kernel)
+  const uint64_t need_rows = (uint64_t)grid_max * chunk * 2;
+  if (h->ovf_rows_cap < need_rows) WG_TRY(regrow(ctx, &h->ovf_rows, &h->ovf_rows_cap, need_rows, 4, "overflow rows"));
+  const uint64_t need_part = (uint64_t)grid_max * L * 2;
+  if (h->ovf_part_cap < need_part)
+    WG_TRY(regrow(ctx, &h->ovf_part, &h->ovf_part_cap, need_part, (uint64_t)h->pg.stride * 8, "overflow partials"));
+  // dense output: at most L groups per item a workgroup owns alone, and H hot keys
+  if (dense) WG_TRY(agg_ensure_dense(h, dense_bound + kSideSlots, 0));
 
   // C: scatter (side-slot rows are folded here)
+  h->touched = true;
   if (dense) h->side_dirty = true;
   else h->dirty = true;
   WG_HIP(hipEventRecord(h->pev[2], ctx->stream));
@@ -1103,7 +1112,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
 
   // D: work items (built while the scatter runs), largest first; partitions
   // above the chunk size are split
-  uint64_t n_items = 0;
+  n_items = 0;
   for (uint32_t p = 0; p < np; ++p) {
     const uint64_t b = bound[p], e = bound[p + 1];
     if (e <= b) continue;
@@ -1113,12 +1122,6 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   }
   std::stable_sort(hitems, hitems + n_items,
                    [](const PartItem& a, const PartItem& b) { return a.end - a.beg > b.end - b.beg; });
-  const uint64_t ibytes = std::max<uint64_t>(n_items, 1) * sizeof(PartItem) + 64;
-  if (h->part_items_bytes < ibytes) {
-    if (h->part_items) WG_HIP(hipFree(h->part_items));
-    WG_HIP(hipMalloc(&h->part_items, ibytes));
-    h->part_items_bytes = ibytes;
-  }
   unsigned long long* item_ctr = (unsigned long long*)((char*)h->part_items + ibytes - 64);
   if (n_items)
     WG_HIP(hipMemcpyAsync(h->part_items, hitems, n_items * sizeof(PartItem), hipMemcpyHostToDevice, ctx->stream));
@@ -1133,32 +1136,11 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   Q.vfloat = P.vfloat;
   DenseOut D{};
   if (dense) {
-    // at most L groups per item a workgroup owns alone, and H hot keys
-    uint64_t bound = (uint64_t)H;
-    for (uint64_t i = 0; i < n_items; ++i)
-      if (!hitems[i].split) bound += std::min<uint64_t>(hitems[i].end - hitems[i].beg, (uint64_t)L);
-    WG_TRY(agg_ensure_dense(h, bound + kSideSlots, 0));
     WG_TRY(zero_dev(ctx, h->counters + kCtrDense, 1));
     D = DenseOut{h->dense, h->n_sumd ? h->dense_base : nullptr, h->dense_null,
-                 (unsigned long long*)(h->counters + kCtrDense), bound, h->n_sumd * 2};
+                 (unsigned long long*)(h->counters + kCtrDense), dense_bound, h->n_sumd * 2};
   }
   Q.dense = D;
-  const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
-  const int per_cu_agg = std::max(1, std::min<int>((160 * 1024) / (int)lds_bytes, 2048 / h->part_block));
-  const int grid_max = ctx->cus * per_cu_agg;
-  // overflow lists with headroom for every in-flight item (see the kernel)
-  const uint64_t need_rows = (uint64_t)grid_max * chunk * 2;
-  if (h->ovf_rows_cap < need_rows) {
-    if (h->ovf_rows) WG_HIP(hipFree(h->ovf_rows));
-    WG_HIP(hipMalloc(&h->ovf_rows, need_rows * 4));
-    h->ovf_rows_cap = need_rows;
-  }
-  const uint64_t need_part = (uint64_t)grid_max * L * 2;
-  if (h->ovf_part_cap < need_part) {
-    if (h->ovf_part) WG_HIP(hipFree(h->ovf_part));
-    WG_HIP(hipMalloc(&h->ovf_part, need_part * (uint64_t)h->pg.stride * 8));
-    h->ovf_part_cap = need_part;
-  }
   Q.max_item_rows = chunk;
   Q.sel_chunk = chunk;
   // Items are drained in launches; a launch stops taking items when the
@@ -1166,7 +1148,13 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   // then grows the table, folds the partials in, replays the overflowed
   // positions and resumes with the items not yet taken.
   struct Pending { uint32_t* rows; uint64_t n; };
-  std::vector<Pending> replays;  // replay lists still to aggregate
+  struct Replays {  // replay lists still to aggregate, freed on every exit
+    std::vector<Pending> v;
+    ~Replays() {
+      for (auto& r : v) (void)hipFree(r.rows);
+    }
+  } pending;
+  std::vector<Pending>& replays = pending.v;
   uint64_t items_done = 0;
   bool first = true;
   for (int round = 0; round < 64; ++round) {
@@ -1217,7 +1205,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
         Pending& r = replays.back();
         const uint64_t skip = done_now * chunk;
         uint32_t* tail = nullptr;
-        WG_HIP(hipMalloc(&tail, (r.n - skip) * 4));
+        WG_ALLOC(h->ctx, &tail, (r.n - skip) * 4, "replayed rows");
         WG_HIP(hipMemcpyAsync(tail, r.rows + skip, (r.n - skip) * 4, hipMemcpyDeviceToDevice, ctx->stream));
         WG_HIP(hipStreamSynchronize(ctx->stream));
         WG_HIP(hipFree(r.rows));
@@ -1233,14 +1221,13 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
       if (npart) WG_TRY(agg_merge_partials(h, h->ovf_part, npart));
       if (nr) {
         uint32_t* rows = nullptr;
-        WG_HIP(hipMalloc(&rows, nr * 4));
+        WG_ALLOC(h->ctx, &rows, nr * 4, "replayed rows");
         WG_HIP(hipMemcpyAsync(rows, h->ovf_rows, nr * 4, hipMemcpyDeviceToDevice, ctx->stream));
         WG_HIP(hipStreamSynchronize(ctx->stream));
         replays.push_back(Pending{rows, nr});
       }
     }
   }
-  for (auto& r : replays) (void)hipFree(r.rows);
   WG_CHECK(replays.empty() && items_done >= n_items, WG_ERR_DEVICE, "partitioned consume did not converge");
   if (H) {
     // E: the hot keys' per-workgroup partials into the global table

@@ -32,7 +32,7 @@ wg_status wide_attach(wg_agg* h, ColSet& cs, const uint32_t* sel, uint64_t n, ui
     if (h->ids) WG_HIP(hipFree(h->ids));
     h->ids = nullptr;
     h->ids_cap = 0;
-    WG_HIP(hipMalloc(&h->ids, std::max<uint64_t>(n_rows, 1) * 8));
+    WG_ALLOC(h->ctx, &h->ids, std::max<uint64_t>(n_rows, 1) * 8, "group-key ids");
     h->ids_cap = std::max<uint64_t>(n_rows, 1);
   }
   WG_TRY(kd_encode_rows(h->ctx, h->dict, cs, h->wkey, sel, n, false, h->ids));
@@ -55,7 +55,7 @@ wg_status wide_remap_slots(wg_agg* dst, wg_agg* src, uint64_t** remapped) {
   *remapped = nullptr;
   uint64_t* copy = nullptr;
   const uint64_t words = (src->cap + kSideSlots) * (uint64_t)src->pg.stride;
-  WG_HIP(hipMalloc(&copy, words * 8));
+  WG_ALLOC(dst->ctx, &copy, words * 8, "a remapped group table");
   wg_status st = WG_OK;
   if (hipMemcpyAsync(copy, src->slots, words * 8, hipMemcpyDeviceToDevice, dst->ctx->stream) != hipSuccess) {
     set_error("hipMemcpyAsync");

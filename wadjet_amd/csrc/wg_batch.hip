@@ -394,27 +394,35 @@ struct wg_batch {
 
 namespace wg {
 
-static wg_status ensure_dev(void** p, uint64_t& cap, uint64_t need, uint64_t elem, uint64_t& gen) {
+static wg_status ensure_dev(wg_ctx* ctx, void** p, uint64_t& cap, uint64_t need, uint64_t elem, uint64_t& gen) {
   if (need <= cap && *p) return WG_OK;
-  if (*p) WG_HIP(hipFree(*p));
-  cap = std::max<uint64_t>(need, cap * 2);
-  WG_HIP(hipMalloc(p, cap * elem));
+  const uint64_t want = std::max<uint64_t>(need, cap * 2);
+  WG_TRY(regrow(ctx, p, &cap, want, elem, "batch descriptors"));
   ++gen;  // invalidates any graph that captured the old pointer
   return WG_OK;
 }
 
-static wg_status arena_init(BatchArena& A, uint64_t slots, uint64_t max_q) {
+static void arena_free(BatchArena& A);
+
+static wg_status arena_alloc(wg_ctx* ctx, BatchArena& A, uint64_t slots, uint64_t max_q) {
+  WG_ALLOC(ctx, &A.arena, (slots + 1) * kBStride * 8, "the batch arena");
+  WG_ALLOC(ctx, &A.d_scalars, 64, "batch counters");
+  WG_ALLOC(ctx, &A.d_work_ctrs, 64, "batch counters");
+  WG_ALLOC(ctx, &A.d_keys, slots * 8, "batch outputs");
+  WG_ALLOC(ctx, &A.d_sum_lo, slots * 8, "batch outputs");
+  WG_ALLOC(ctx, &A.d_sum_hi, slots * 8, "batch outputs");
+  WG_ALLOC(ctx, &A.d_counts, slots * 8, "batch outputs");
+  WG_TRY(ensure_dev(ctx, (void**)&A.d_qs, A.q_cap, max_q, sizeof(BQ), A.gen));
+  WG_ALLOC(ctx, &A.d_ctr, max_q * sizeof(BCounters), "batch counters");
   A.slots = slots;
-  WG_HIP(hipMalloc(&A.arena, (slots + 1) * kBStride * 8));
-  WG_HIP(hipMalloc(&A.d_scalars, 64));
-  WG_HIP(hipMalloc(&A.d_work_ctrs, 64));
-  WG_HIP(hipMalloc(&A.d_keys, slots * 8));
-  WG_HIP(hipMalloc(&A.d_sum_lo, slots * 8));
-  WG_HIP(hipMalloc(&A.d_sum_hi, slots * 8));
-  WG_HIP(hipMalloc(&A.d_counts, slots * 8));
-  WG_TRY(ensure_dev((void**)&A.d_qs, A.q_cap, max_q, sizeof(BQ), A.gen));
-  WG_HIP(hipMalloc(&A.d_ctr, max_q * sizeof(BCounters)));
   return WG_OK;
+}
+
+// An arena of `slots` table slots for up to max_q queries, all or nothing.
+static wg_status arena_init(wg_ctx* ctx, BatchArena& A, uint64_t slots, uint64_t max_q) {
+  const wg_status st = arena_alloc(ctx, A, slots, max_q);
+  if (st != WG_OK) arena_free(A);
+  return st;
 }
 
 static void arena_free(BatchArena& A) {
@@ -523,9 +531,9 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
 
 static wg_status upload(wg_ctx* ctx, BatchArena& A, const Plan& P) {
   const int n = (int)P.qs.size();
-  WG_TRY(ensure_dev((void**)&A.d_qs, A.q_cap, std::max(n, 1), sizeof(BQ), A.gen));
-  WG_TRY(ensure_dev((void**)&A.d_work, A.work_cap, std::max<size_t>(P.work.size(), 1), sizeof(BWork), A.gen));
-  WG_TRY(ensure_dev((void**)&A.d_cwork, A.cwork_cap, std::max<size_t>(P.cwork.size(), 1), sizeof(CWork), A.gen));
+  WG_TRY(ensure_dev(ctx, (void**)&A.d_qs, A.q_cap, std::max(n, 1), sizeof(BQ), A.gen));
+  WG_TRY(ensure_dev(ctx, (void**)&A.d_work, A.work_cap, std::max<size_t>(P.work.size(), 1), sizeof(BWork), A.gen));
+  WG_TRY(ensure_dev(ctx, (void**)&A.d_cwork, A.cwork_cap, std::max<size_t>(P.cwork.size(), 1), sizeof(CWork), A.gen));
   const uint64_t bytes = n * sizeof(BQ) + P.work.size() * sizeof(BWork) + P.cwork.size() * sizeof(CWork) + 64 +
                          n * sizeof(BCounters);
   if (bytes > A.pinned_bytes) {
@@ -620,7 +628,7 @@ static wg_status run_solo(wg_batch* b, int i, const wg_batch_query& q0, uint64_t
   const uint64_t need = table_cap_for(est) + 1;
   if (b->solo.slots < need) {
     arena_free(b->solo);
-    WG_TRY(arena_init(b->solo, std::max<uint64_t>(need, 1 << 16), 1));
+    WG_TRY(arena_init(b->ctx, b->solo, std::max<uint64_t>(need, 1 << 16), 1));
   }
   Plan P;
   WG_TRY(plan_into(b, &q, 1, b->solo, P));
@@ -726,14 +734,14 @@ wg_status wg_batch_download_async(wg_batch* b, uint64_t* host, uint64_t host_wor
     A.d_pack_items = nullptr;
     A.pack_cap = 0;
     WG_HIP(hipHostMalloc((void**)&A.pack_items, n * sizeof(PackItem), hipHostMallocDefault));
-    WG_HIP(hipMalloc(&A.d_pack_items, n * sizeof(PackItem)));
+    WG_ALLOC(b->ctx, &A.d_pack_items, n * sizeof(PackItem), "batch result packing");
     A.pack_cap = n;
   }
   if (A.packed_cap < total) {
     if (A.d_packed) WG_HIP(hipFree(A.d_packed));
     A.d_packed = nullptr;
     A.packed_cap = 0;
-    WG_HIP(hipMalloc(&A.d_packed, total * 8));
+    WG_ALLOC(b->ctx, &A.d_packed, total * 8, "batch result packing");
     A.packed_cap = total;
   }
   // the pack table of an earlier download is no longer read: downloads on this
@@ -761,7 +769,7 @@ wg_status wg_batch_create(wg_ctx* ctx, int32_t max_queries, uint64_t max_groups_
   b->max_q = max_queries;
   // each query's table is at most 4x its estimate (power of two of 2x), + side slot
   const uint64_t slots = std::max<uint64_t>(max_groups_total * 4 + (uint64_t)max_queries * 130, 1 << 16);
-  wg_status st = arena_init(b->main, slots, max_queries);
+  wg_status st = arena_init(ctx, b->main, slots, max_queries);
   if (st != WG_OK) {
     wg_batch_destroy(b);
     return st;

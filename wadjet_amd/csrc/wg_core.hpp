@@ -183,6 +183,19 @@ __device__ __forceinline__ bool pred_pass(const ColDesc& c, uint64_t row, int op
 
 // ---------------------------------------------------------------- helpers
 
+// Replace *p (room for *cap elements) by a buffer of `want` elements. The new
+// buffer is allocated before the old one is freed, so a failed allocation
+// leaves both *p and *cap as they were. The contents are not kept.
+template <typename T, typename C>
+wg_status regrow(wg_ctx* ctx, T** p, C* cap, uint64_t want, uint64_t elem_bytes, const char* what) {
+  T* fresh = nullptr;
+  WG_TRY(dev_alloc(ctx, (void**)&fresh, want * elem_bytes, nullptr, what));
+  if (*p) (void)hipFree(*p);
+  *p = fresh;
+  *cap = (C)want;
+  return WG_OK;
+}
+
 inline uint64_t next_pow2(uint64_t v) {
   uint64_t p = 1;
   while (p < v) p <<= 1;

@@ -18,6 +18,34 @@ void set_error(const char* fmt, ...) {
 }
 void clear_error() { g_err[0] = 0; }
 
+wg_status dev_alloc(wg_ctx* ctx, void** p, size_t bytes, hipStream_t stream, const char* what) {
+  *p = nullptr;
+  if (ctx) {
+    bool inject = ctx->failing.load();
+    if (!inject) {
+      const int64_t left = ctx->fail_at.load();
+      if (left > 0 && ctx->fail_at.fetch_sub(1) == 1) {
+        inject = true;
+        if (ctx->fail_sticky.load()) ctx->failing.store(true);
+      }
+    }
+    if (inject) {
+      set_error("device allocation of %llu bytes for %s failed: out of memory (injected)", (unsigned long long)bytes,
+                what);
+      return WG_ERR_OOM;
+    }
+  }
+  const hipError_t e = stream ? hipMallocAsync(p, bytes, stream) : hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    set_error("device allocation of %llu bytes for %s failed: %s", (unsigned long long)bytes, what,
+              hipGetErrorString(e));
+    return WG_ERR_OOM;
+  }
+  return WG_OK;
+}
+
 // --------------------------------------------------------------- the filter
 
 // Order-preserving stream compaction of the rows that pass `col <op> const`.
@@ -123,6 +151,15 @@ wg_status wg_ctx_destroy(wg_ctx* ctx) {
   return WG_OK;
 }
 
+wg_status wg_ctx_inject_alloc_failure(wg_ctx* ctx, int64_t nth, int32_t sticky) {
+  clear_error();
+  WG_CHECK(ctx && nth >= 0, WG_ERR_INVALID, "bad argument");
+  ctx->failing.store(false);
+  ctx->fail_sticky.store(sticky != 0);
+  ctx->fail_at.store(nth);
+  return WG_OK;
+}
+
 wg_status wg_ctx_sync(wg_ctx* ctx) {
   WG_CHECK(ctx, WG_ERR_INVALID, "ctx is null");
   WG_HIP(hipStreamSynchronize(ctx->stream));
@@ -155,12 +192,7 @@ wg_status wg_malloc(wg_ctx* ctx, uint64_t bytes, void** out) {
   WG_CHECK(ctx && out, WG_ERR_INVALID, "null argument");
   *out = nullptr;
   if (bytes == 0) return WG_OK;
-  if (hipMalloc(out, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    set_error("hipMalloc(%llu) failed", (unsigned long long)bytes);
-    return WG_ERR_OOM;
-  }
-  return WG_OK;
+  return dev_alloc(ctx, out, bytes, nullptr, "wg_malloc");
 }
 
 wg_status wg_free(wg_ctx* ctx, void* ptr) {

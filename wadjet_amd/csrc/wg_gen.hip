@@ -127,7 +127,7 @@ static wg_status zipf_cdf(int64_t n, double s, double** out) {
   for (auto& v : c) v /= acc;
   c[n - 1] = 2.0;  // every u < 1 lands at or before the last rank
   double* d = nullptr;
-  WG_HIP(hipMalloc(&d, n * 8));
+  WG_ALLOC(nullptr, &d, n * 8, "a Zipf CDF");
   WG_HIP(hipMemcpy(d, c.data(), n * 8, hipMemcpyHostToDevice));
   g_cdf[{n, s}] = d;
   *out = d;

@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
+#include <cstddef>
 #include <cstdint>
 #include <mutex>
 
@@ -38,6 +40,16 @@ void clear_error();
     if (_s != WG_OK) return _s;       \
   } while (0)
 
+// Every device allocation of the library: hipMalloc (stream = nullptr) or
+// hipMallocAsync on `stream`, behind the context's injected failures
+// (wg_ctx_inject_alloc_failure). *p is null after a failure, which is
+// WG_ERR_OOM with the request in the message.
+wg_status dev_alloc(wg_ctx* ctx, void** p, size_t bytes, hipStream_t stream, const char* what);
+
+#define WG_ALLOC(ctx, pp, bytes, what) WG_TRY(::wg::dev_alloc((ctx), (void**)(pp), (bytes), nullptr, (what)))
+#define WG_ALLOC_ASYNC(ctx, pp, bytes, stream, what) \
+  WG_TRY(::wg::dev_alloc((ctx), (void**)(pp), (bytes), (stream), (what)))
+
 }  // namespace wg
 
 struct wg_ctx {
@@ -56,4 +68,9 @@ struct wg_ctx {
   std::mutex upload_mu;
   // device address of this device's no-allocation filter scratch
   uint64_t* noalloc_scratch = nullptr;
+  // injected allocation failures: the fail_at-th allocation from the time of
+  // injection fails (0 = none), and every later one too when sticky
+  std::atomic<int64_t> fail_at{0};
+  std::atomic<bool> fail_sticky{false};
+  std::atomic<bool> failing{false};
 };

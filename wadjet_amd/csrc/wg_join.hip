@@ -532,11 +532,7 @@ namespace wg {
 static wg_status join_alloc_table(wg_join* j, uint64_t cap) {
   uint64_t* s = nullptr;
   const uint64_t words = (cap + 1) * (uint64_t)j->stride;
-  if (hipMalloc(&s, words * 8) != hipSuccess) {
-    (void)hipGetLastError();
-    set_error("hipMalloc for a %llu-slot join table failed", (unsigned long long)cap);
-    return WG_ERR_OOM;
-  }
+  WG_ALLOC(j->ctx, &s, words * 8, "a join table");
   hipLaunchKernelGGL(join_fill_kernel, dim3(grid_for(words, 256, 8192)), dim3(256), 0, j->ctx->stream, s, words, j->stride);
   WG_HIP(hipGetLastError());
   if (j->slots) {
@@ -560,7 +556,7 @@ static wg_status join_reserve_rows(wg_join* j, uint64_t need) {
   cap = std::max<uint64_t>(cap, 1 << 16);
   for (int c = 0; c < j->n_ret; ++c) {
     void* d = nullptr;
-    WG_HIP(hipMalloc(&d, cap * type_bytes(j->types[c])));
+    WG_ALLOC(j->ctx, &d, cap * type_bytes(j->types[c]), "a retained build column");
     if (j->col_data[c]) {
       WG_HIP(hipMemcpyAsync(d, j->col_data[c], j->rows * type_bytes(j->types[c]), hipMemcpyDeviceToDevice, j->ctx->stream));
       WG_HIP(hipStreamSynchronize(j->ctx->stream));
@@ -568,7 +564,7 @@ static wg_status join_reserve_rows(wg_join* j, uint64_t need) {
     }
     j->col_data[c] = d;
     uint64_t* v = nullptr;
-    WG_HIP(hipMalloc(&v, (cap + 63) / 64 * 8));
+    WG_ALLOC(j->ctx, &v, (cap + 63) / 64 * 8, "a retained validity bitmap");
     WG_HIP(hipMemsetAsync(v, 0xFF, (cap + 63) / 64 * 8, j->ctx->stream));
     if (j->col_valid[c]) {
       WG_HIP(hipMemcpyAsync(v, j->col_valid[c], (j->rows + 63) / 64 * 8, hipMemcpyDeviceToDevice, j->ctx->stream));
@@ -578,7 +574,7 @@ static wg_status join_reserve_rows(wg_join* j, uint64_t need) {
     j->col_valid[c] = v;
   }
   int32_t* nx = nullptr;
-  WG_HIP(hipMalloc(&nx, cap * 4));
+  WG_ALLOC(j->ctx, &nx, cap * 4, "the chain array");
   if (j->next) {
     WG_HIP(hipMemcpyAsync(nx, j->next, j->rows * 4, hipMemcpyDeviceToDevice, j->ctx->stream));
     WG_HIP(hipStreamSynchronize(j->ctx->stream));
@@ -753,15 +749,15 @@ static wg_status ensure_probe_scratch(wg_join* j, uint64_t n) {
   if (j->off) WG_HIP(hipFree(j->off));
   j->cnt = j->off = nullptr;
   j->scratch_rows = 0;
-  WG_HIP(hipMalloc(&j->cnt, n * 8));                // per position: match count | chain head << 32
-  WG_HIP(hipMalloc(&j->off, (2 * tiles + 2) * 8));  // per tile: pair count, then its offset
+  WG_ALLOC(j->ctx, &j->cnt, n * 8, "probe scratch");                // per position: match count | chain head << 32
+  WG_ALLOC(j->ctx, &j->off, (2 * tiles + 2) * 8, "probe scratch");  // per tile: pair count, then its offset
   size_t bytes = 0;
   WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, j->off, j->off + tiles, (int)tiles, j->ctx->stream));
   if (bytes > j->cub_bytes) {
     if (j->cub_tmp) WG_HIP(hipFree(j->cub_tmp));
     j->cub_tmp = nullptr;
     j->cub_bytes = 0;
-    WG_HIP(hipMalloc(&j->cub_tmp, bytes));
+    WG_ALLOC(j->ctx, &j->cub_tmp, bytes, "scan scratch");
     j->cub_bytes = bytes;
   }
   j->scratch_rows = n;
@@ -825,9 +821,10 @@ static wg_status join_create(wg_ctx* ctx, int32_t n_build_cols, const int32_t* b
     j->inline_cols[i] = c;
   }
   j->stride = (int32_t)next_pow2(2 + n_inline);
-  WG_HIP(hipMalloc(&j->counter, 128));  // 16 words: claims, digest, chain result, probe cut
-  WG_TRY(zero_dev(ctx, j->counter, 8));
-  wg_status st = join_alloc_table(j, next_pow2(std::max<uint64_t>(build_hint, 1024) * 2));
+  // 16 counter words: claims, digest, chain result, probe cut
+  wg_status st = dev_alloc(ctx, (void**)&j->counter, 128, nullptr, "the join's counters");
+  if (st == WG_OK) st = zero_dev(ctx, j->counter, 8);
+  if (st == WG_OK) st = join_alloc_table(j, next_pow2(std::max<uint64_t>(build_hint, 1024) * 2));
   if (st == WG_OK && dual) st = kd_init(ctx, j->dict, build_hint);
   if (st != WG_OK) {
     wg_join_destroy(j);
@@ -1002,9 +999,8 @@ wg_status wg_join_bloom_build(wg_join* j) {
   uint64_t words = 1;
   while (words * 64 < j->distinct * 10) words *= 2;
   if (words < 8) words = 8;
-  if (j->bloom) WG_HIP(hipFree(j->bloom));
-  j->bloom = nullptr;
-  WG_HIP(hipMalloc(&j->bloom, words * 8));
+  uint64_t cap_words = j->bloom ? j->bloom_mask + 1 : 0;
+  if (cap_words != words) WG_TRY(regrow(j->ctx, &j->bloom, &cap_words, words, 8, "the Bloom filter"));
   WG_TRY(zero_dev(j->ctx, j->bloom, words));
   j->bloom_mask = words - 1;
   hipLaunchKernelGGL(bloom_build_kernel, dim3(grid_for(j->cap + 1, 256, 8192)), dim3(256), 0, j->ctx->stream,
@@ -1207,10 +1203,8 @@ wg_status wg_join_digest(wg_join* j, const wg_column* probe_cols, const uint32_t
   return WG_OK;
 }
 
-wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_probe_cols, int32_t probe_key,
-                            uint64_t n_rows, const uint32_t* sel, uint64_t n_sel, wg_agg* agg) {
-  clear_error();
-  WG_CHECK(j && agg && probe_cols, WG_ERR_INVALID, "null argument");
+static wg_status probe_agg_call(wg_join* j, const wg_column* probe_cols, int32_t n_probe_cols, int32_t probe_key,
+                                uint64_t n_rows, const uint32_t* sel, uint64_t n_sel, wg_agg* agg) {
   WG_CHECK(j->finished, WG_ERR_STATE, "probe before build_finish");
   WG_CHECK(!agg->finalized, WG_ERR_STATE, "aggregate already finalized");
   WG_CHECK(agg->ctx == j->ctx, WG_ERR_INVALID, "join and aggregate on different contexts");
@@ -1291,8 +1285,16 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
     WG_HIP(hipEventCreate(&agg->ev1));
   }
   // work list: (sel, n) segments still to process; overflow rows become new segments
-  struct Seg { const uint32_t* sel; uint64_t n; uint32_t* owned; bool lean; };
-  std::vector<Seg> work{{sel, n, nullptr, lean}};
+  struct Seg { const uint32_t* sel; uint64_t n; bool lean; };
+  std::vector<Seg> work{{sel, n, lean}};
+  struct Owned {  // replay lists, freed on every exit
+    hipStream_t st;
+    std::vector<uint32_t*> v;
+    ~Owned() {
+      if (!v.empty()) (void)hipStreamSynchronize(st);
+      for (uint32_t* p : v) (void)hipFree(p);
+    }
+  } owned{agg->ctx->stream, {}};
   wg_status st = WG_OK;
   for (int guard = 0; !work.empty() && st == WG_OK && guard < 64; ++guard) {
     Seg seg = work.back();
@@ -1310,6 +1312,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
       WG_TRY(zero_dev(agg->ctx, agg->counters + CTR_OVF_ROWS, CTR_N - 1));
       const int grid = (int)std::min<uint64_t>((uint64_t)(seg.lean ? grid_l : grid_j), std::max<uint64_t>(P.n_tiles, 1));
       WG_HIP(hipEventRecord(agg->ev0, agg->ctx->stream));
+      agg->touched = true;
       if (seg.lean) {
         Q.T = P.T;
         Q.sel = P.sel;
@@ -1352,7 +1355,7 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
         if (np) {
           uint64_t* recs = nullptr;
           const uint64_t bytes = np * (uint64_t)agg->pg.stride * 8;
-          WG_HIP(hipMalloc(&recs, bytes));
+          WG_ALLOC(j->ctx, &recs, bytes, "overflow records");
           WG_HIP(hipMemcpyAsync(recs, agg->ovf_part, bytes, hipMemcpyDeviceToDevice, agg->ctx->stream));
           GlobalTable T = table_of(agg);
           T.max_probe = agg->cap;
@@ -1367,24 +1370,29 @@ wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_p
         }
         if (nr) {
           uint32_t* rows = nullptr;
-          WG_HIP(hipMalloc(&rows, nr * 4));
+          WG_ALLOC(j->ctx, &rows, nr * 4, "replayed rows");
+          owned.v.push_back(rows);
           WG_HIP(hipMemcpyAsync(rows, agg->ovf_rows, nr * 4, hipMemcpyDeviceToDevice, agg->ctx->stream));
           WG_HIP(hipStreamSynchronize(agg->ctx->stream));
-          work.push_back(Seg{rows, nr, rows, false});  // the general kernel takes every row shape
+          work.push_back(Seg{rows, nr, false});  // the general kernel takes every row shape
         }
       }
       WG_CHECK(tiles > 0 || c[CTR_OVF_ROWS] > 0, WG_ERR_DEVICE, "join probe launch made no progress");
       pos = std::min<uint64_t>(seg.n, pos + tiles * P.tile_rows);
       if (pos < seg.n) WG_TRY(agg_ensure_capacity(agg, 0));
     }
-    if (seg.owned) {
-      WG_HIP(hipStreamSynchronize(agg->ctx->stream));
-      WG_HIP(hipFree(seg.owned));
-    }
   }
   WG_CHECK(work.empty(), WG_ERR_DEVICE, "join aggregate overflow did not converge");
   agg->rows_since_norm += n;
   return st;
+}
+
+wg_status wg_join_probe_agg(wg_join* j, const wg_column* probe_cols, int32_t n_probe_cols, int32_t probe_key,
+                            uint64_t n_rows, const uint32_t* sel, uint64_t n_sel, wg_agg* agg) {
+  clear_error();
+  WG_CHECK(j && agg && probe_cols, WG_ERR_INVALID, "null argument");
+  WG_TRY(agg_guard_begin(agg));
+  return agg_guard_end(agg, probe_agg_call(j, probe_cols, n_probe_cols, probe_key, n_rows, sel, n_sel, agg));
 }
 
 }  // extern "C"

@@ -256,11 +256,11 @@ static wg_status fill(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v) {
 static wg_status dict_alloc(wg_ctx* ctx, KeyDict& d, uint64_t cap) {
   uint64_t* slots = nullptr;
   uint64_t* keys = nullptr;
-  if (hipMalloc(&slots, cap * 32) != hipSuccess || hipMalloc(&keys, cap * 16) != hipSuccess) {
-    (void)hipGetLastError();
-    if (slots) (void)hipFree(slots);
-    set_error("hipMalloc for a %llu-slot key dictionary failed", (unsigned long long)cap);
-    return WG_ERR_OOM;
+  WG_ALLOC(ctx, &slots, cap * 32, "a key dictionary");
+  const wg_status st = dev_alloc(ctx, (void**)&keys, cap * 16, nullptr, "a key dictionary");
+  if (st != WG_OK) {
+    (void)hipFree(slots);
+    return st;
   }
   WG_TRY(fill(ctx, slots, cap * 4, kDictFree));
   if (d.slots) {
@@ -279,7 +279,7 @@ static wg_status dict_alloc(wg_ctx* ctx, KeyDict& d, uint64_t cap) {
 }
 
 wg_status kd_init(wg_ctx* ctx, KeyDict& d, uint64_t hint) {
-  WG_HIP(hipMalloc(&d.ctr, 24));
+  WG_ALLOC(ctx, &d.ctr, 24, "the key dictionary's counters");
   WG_TRY(zero_dev(ctx, d.ctr, 3));
   d.n = 0;
   return dict_alloc(ctx, d, next_pow2(std::max<uint64_t>(2 * std::max<uint64_t>(hint, 1024), 64)));

@@ -451,15 +451,15 @@ wg_status wg_parquet_decode_pages(wg_ctx* ctx, const wg_pq_page* pages, uint64_t
   uint32_t* prefix = nullptr;
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
-  WG_HIP(hipMallocAsync((void**)&bad, 8, ctx->stream));
+  WG_ALLOC_ASYNC(ctx, &bad, 8, ctx->stream, "an error count");
   WG_HIP(hipMemsetAsync(bad, 0, 8, ctx->stream));
-  WG_HIP(hipMallocAsync((void**)&dtable, upload_bytes, ctx->stream));
+  WG_ALLOC_ASYNC(ctx, &dtable, upload_bytes, ctx->stream, "the page table");
   WG_HIP(hipMemcpyAsync(dtable, upload, upload_bytes, hipMemcpyHostToDevice, ctx->stream));
   const uint32_t* dmap = (const uint32_t*)((const uint8_t*)dtable + table_bytes);
   if (any_nullable) {
     WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                             (int)max_chunks, ctx->stream));
-    WG_HIP(hipMallocAsync((void**)&masks, max_chunks * 16 + scan_bytes, ctx->stream));
+    WG_ALLOC_ASYNC(ctx, &masks, max_chunks * 16 + scan_bytes, ctx->stream, "decode scratch");
     counts = (uint32_t*)(masks + max_chunks);
     prefix = counts + max_chunks;
     scan_tmp = prefix + max_chunks;

@@ -1034,8 +1034,7 @@ wg_status wg_pq_stage(wg_ctx* ctx, const char* path, const wg_pq_chunk* chunks, 
     WG_TRY(F.report());
   } else {
     WG_HIP(hipSetDevice(ctx->device));
-    WG_CHECK(hipMalloc((void**)&S->streams, total + 64) == hipSuccess, WG_ERR_OOM,
-             "hipMalloc(%llu) for the staged pages failed", (unsigned long long)(total + 64));
+    WG_TRY(dev_alloc(ctx, (void**)&S->streams, total + 64, nullptr, "staged Parquet pages"));
     workers = std::max(1, (int)std::min<uint64_t>((uint64_t)workers, std::max<uint64_t>(kSlotBudget / std::max<uint64_t>(slot, 1), 1)));
     std::lock_guard<std::mutex> upload_lock(ctx->upload_mu);  // the slots live in the context's pinned buffer
     const uint64_t need = slot * (uint64_t)workers;
@@ -1089,8 +1088,7 @@ wg_status wg_pq_stage(wg_ctx* ctx, const char* path, const wg_pq_chunk* chunks, 
   for (auto& p : plan)
     if (!p.runs.empty()) memcpy(joined.data() + p.run_offset, p.runs.data(), p.runs.size() * 4);
   if (ctx) {
-    WG_CHECK(hipMalloc((void**)&S->runs, run_alloc) == hipSuccess, WG_ERR_OOM, "hipMalloc(%llu) for the run tables failed",
-             (unsigned long long)run_alloc);
+    WG_TRY(dev_alloc(ctx, (void**)&S->runs, run_alloc, nullptr, "staged Parquet run tables"));
     WG_HIP(hipMemcpy(S->runs, joined.data(), run_alloc, hipMemcpyHostToDevice));
   } else {
     S->runs = (uint8_t*)aligned_alloc(256, (run_alloc + 255) & ~255ull);

@@ -89,7 +89,7 @@ wg_status select_rows(wg_ctx* ctx, const Pred& pred, uint64_t n, const uint32_t*
       if (*p) (void)hipFreeAsync(*p, st);
     }
   } release{ctx->stream, &counts};
-  WG_HIP(hipMallocAsync((void**)&counts, tiles * 8 + words * 8 + 8, ctx->stream));
+  WG_ALLOC_ASYNC(ctx, &counts, tiles * 8 + words * 8 + 8, ctx->stream, "selection scratch");
   uint32_t* offsets = counts + tiles;
   uint64_t* masks = (uint64_t*)(((uintptr_t)(offsets + tiles) + 7) & ~(uintptr_t)7);
   hipLaunchKernelGGL((select_rows_kernel<false, Pred>), dim3(tiles), dim3(kSelBlock), 0, ctx->stream, pred, n, sel,

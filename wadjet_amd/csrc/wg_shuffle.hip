@@ -410,7 +410,7 @@ wg_status wg_shuffle_partition(wg_ctx* ctx, const wg_column* cols, int32_t n_col
   const uint64_t cnt_bytes = (ncnt * 4 + 255) / 256 * 256;
   const uint64_t off_bytes = ((uint64_t)(np + 1) * 8 + 255) / 256 * 256;
   char* scratch = nullptr;
-  WG_HIP(hipMallocAsync((void**)&scratch, part_bytes + 2 * cnt_bytes + off_bytes + scan_bytes, ctx->stream));
+  WG_ALLOC_ASYNC(ctx, &scratch, part_bytes + 2 * cnt_bytes + off_bytes + scan_bytes, ctx->stream, "shuffle scratch");
   P.part = part ? part : (uint32_t*)scratch;
   P.counts = (uint32_t*)(scratch + part_bytes);
   uint32_t* offs = (uint32_t*)(scratch + part_bytes + cnt_bytes);

@@ -217,10 +217,7 @@ wg_status wg_agg_spill_run(wg_agg* h, const char* const* group_names, const uint
   return WG_OK;
 }
 
-wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
-  clear_error();
-  WG_CHECK(h && (run || size == 0), WG_ERR_INVALID, "null argument");
-  WG_CHECK(!h->finalized, WG_ERR_STATE, "merge after finalize");
+static wg_status merge_run_call(wg_agg* h, const void* run, uint64_t size) {
   WG_TRY(agg_settle(h));
   const uint8_t* p = (const uint8_t*)run;
   const uint8_t* const end = p + size;
@@ -341,7 +338,7 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
   if (h->wide && g) {
     // the pairs through this aggregate's dictionary: the table keys on ids
     uint64_t* dp = nullptr;
-    WG_HIP(hipMalloc(&dp, g * 24));
+    WG_ALLOC(h->ctx, &dp, g * 24, "spill-run groups");
     uint64_t* did = dp + 2 * g;
     std::vector<uint64_t> ids(g);
     wg_status st = hipMemcpy(dp, pairs.data(), g * 16, hipMemcpyHostToDevice) == hipSuccess ? WG_OK : WG_ERR_DEVICE;
@@ -356,9 +353,9 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
   }
   uint64_t* d_slots = nullptr;
   uint64_t* d_base = nullptr;
-  WG_HIP(hipMalloc(&d_slots, (g + kSideSlots) * S * 8));
+  WG_ALLOC(h->ctx, &d_slots, (g + kSideSlots) * S * 8, "spill-run groups");
   wg_status st = WG_OK;
-  if (nb && hipMalloc(&d_base, (g + kSideSlots) * nb * 8) != hipSuccess) st = WG_ERR_OOM;
+  if (nb) st = dev_alloc(h->ctx, (void**)&d_base, (g + kSideSlots) * nb * 8, nullptr, "spill-run bases");
   if (st == WG_OK && hipMemcpy(d_slots, slots.data(), (g + kSideSlots) * S * 8, hipMemcpyHostToDevice) != hipSuccess)
     st = WG_ERR_DEVICE;
   if (st == WG_OK && nb && hipMemcpy(d_base, base.data(), (g + kSideSlots) * nb * 8, hipMemcpyHostToDevice) != hipSuccess)
@@ -369,6 +366,14 @@ wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
   if (d_base) (void)hipFree(d_base);
   if (st != WG_OK && !wg_last_error()[0]) set_error("merging a spill run failed");
   return st;
+}
+
+wg_status wg_agg_merge_run(wg_agg* h, const void* run, uint64_t size) {
+  clear_error();
+  WG_CHECK(h && (run || size == 0), WG_ERR_INVALID, "null argument");
+  WG_CHECK(!h->finalized, WG_ERR_STATE, "merge after finalize");
+  WG_TRY(agg_guard_begin(h));
+  return agg_guard_end(h, merge_run_call(h, run, size));
 }
 
 }  // extern "C"

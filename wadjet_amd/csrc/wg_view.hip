@@ -90,7 +90,7 @@ wg_status wg_view_flatten(wg_ctx* ctx, const wg_column* base, uint64_t base_rows
       if (*p) (void)hipFreeAsync(*p, st);
     }
   } release{ctx->stream, &bad};
-  WG_HIP(hipMallocAsync((void**)&bad, 8, ctx->stream));
+  WG_ALLOC_ASYNC(ctx, &bad, 8, ctx->stream, "an error count");
   WG_HIP(hipMemsetAsync(bad, 0, 8, ctx->stream));
   const int grid = grid_for((n + kViewU - 1) / kViewU, 256, 16384);
   const uint64_t* bv = base->validity;

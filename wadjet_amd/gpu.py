@@ -53,6 +53,11 @@ class Context:
     def sync(self):
         call("wg_ctx_sync", self.h)
 
+    def inject_alloc_failure(self, nth: int, sticky: bool = False):
+        """The nth device allocation made through this context from now on
+        fails as out of memory (and every later one when sticky); 0 clears."""
+        call("wg_ctx_inject_alloc_failure", self.h, nth, 1 if sticky else 0)
+
     def memset(self, buf: "DeviceBuffer", value: int, nbytes: Optional[int] = None):
         call("wg_memset", self.h, buf.ptr, value, buf.nbytes if nbytes is None else nbytes)
 

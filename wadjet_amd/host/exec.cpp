@@ -14,7 +14,7 @@ using batch::Vector;
 
 static Status wrap(wg_status st, const char* what) {
   if (st == WG_OK) return Status::OK();
-  return Status::Error(std::string(what) + ": " + wg_last_error());
+  return Status::Error(std::string(what) + ": " + wg_last_error(), st);
 }
 #define TRY(x)                       \
   do {                               \
@@ -412,10 +412,12 @@ Status GpuHashAggregate::Consume(Context& ctx, const BatchPtr& b) {
 Status GpuHashAggregate::Finalize(Context&) {
   std::lock_guard<std::mutex> lk(mu_);
   if (finalized_) return Status::OK();
-  finalized_ = true;
-  if (!h_) return Status::OK();  // no input: no groups
-  TRY(wrap(wg_agg_finalize(h_), "wg_agg_finalize"));
-  return wrap(wg_agg_num_groups(h_, &groups_), "wg_agg_num_groups");
+  if (h_) {  // no input: no groups
+    TRY(wrap(wg_agg_finalize(h_), "wg_agg_finalize"));
+    TRY(wrap(wg_agg_num_groups(h_, &groups_), "wg_agg_num_groups"));
+  }
+  finalized_ = true;  // a failed finalize can be called again
+  return Status::OK();
 }
 
 std::vector<Column> GpuHashAggregate::OutputSchema() const {

@@ -33,9 +33,12 @@ namespace wadjet {
 struct Status {
   bool ok_ = true;
   std::string msg;
+  int code = 0;  // the library's wg_status for failures it reported, else -1
   static Status OK() { return {}; }
-  static Status Error(std::string m) { return Status{false, std::move(m)}; }
+  static Status Error(std::string m, int code = -1) { return Status{false, std::move(m), code}; }
   bool ok() const { return ok_; }
+  // device memory ran out (WG_ERR_OOM): the adapter may fall back to the CPU operator
+  bool out_of_memory() const { return !ok_ && code == WG_ERR_OOM; }
 };
 
 struct Context {
