@@ -484,7 +484,10 @@ def bench_batched(ctx, args):
     qarr = Batch.queries(data)
     b = Batch(ctx, args.batch_queries, max(sum(q["groups"] for q in plan), sum(q["groups"] for q in tune_plan)))
     t0 = time.perf_counter()
-    b.autotune(Batch.queries(tune_data), len(tune_plan))
+    if args.batch_tuning:  # a fixed tuning (profiling runs): no autotuner launches
+        b.set_tuning(*[int(x) for x in args.batch_tuning.split(",")])
+    else:
+        b.autotune(Batch.queries(tune_data), len(tune_plan))
     tune_s = time.perf_counter() - t0
     del tune_data
     tuning = b.tuning()
@@ -501,6 +504,8 @@ def bench_batched(ctx, args):
     n_lds = sum(1 for c in choices if c[0] == 1)
     n_cache = sum(1 for c in choices if c[0] == 2)
     n_fan = sum(1 for c in choices if c[2] > 1)
+    if args.batch_no_plain:  # profiling runs: the batched launches only
+        return dict(queries=len(plan), batched_ms=min(bt) * 1000, device_ms=min(dev), tuning=b.tuning())
     # one launch per query, serialised; latency of query i = its completion time
     b.run(qarr, len(plan), plain=True)  # warm
     pt = []
@@ -608,6 +613,10 @@ def main():
     ap.add_argument("--join-build", type=int, default=10_000_000)
     ap.add_argument("--join-probe", type=int, default=200_000_000)
     ap.add_argument("--batch-queries", type=int, default=256)
+    ap.add_argument("--batch-tuning", default="",
+                    help="block,lds_threshold,tile_rows,max_fanout,lds_cache: skip the autotuner (profiling runs)")
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="skip the one-query-per-launch comparison (profiling runs)")
     ap.add_argument("--serve-rows", type=int, default=500_000_000)
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)

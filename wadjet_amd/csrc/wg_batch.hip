@@ -538,8 +538,12 @@ static wg_status upload(wg_ctx* ctx, BatchArena& A, const Plan& P) {
                          n * sizeof(BCounters);
   if (bytes > A.pinned_bytes) {
     if (A.pinned) WG_HIP(hipHostFree(A.pinned));
-    A.pinned_bytes = std::max<uint64_t>(bytes * 2, 1 << 20);
-    WG_HIP(hipHostMalloc(&A.pinned, A.pinned_bytes, hipHostMallocDefault));
+    A.pinned = nullptr;
+    A.pinned_bytes = 0;
+    const uint64_t want = std::max<uint64_t>(bytes * 2, 1 << 20);
+    WG_CHECK(hipHostMalloc(&A.pinned, want, hipHostMallocDefault) == hipSuccess, WG_ERR_OOM,
+             "pinned staging of %llu bytes failed", (unsigned long long)want);
+    A.pinned_bytes = want;
   }
   // the previous use of this arena has completed, so its staging is free
   char* p = (char*)A.pinned;
