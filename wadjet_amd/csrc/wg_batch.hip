@@ -38,7 +38,7 @@ struct BQ {
   uint32_t lds_slots;  // 0 = global table only
   int32_t dim_word;    // >= 0: group key = this word of the matched dimension slot
   uint32_t fan_bits;   // the query's keys split over 2^fan_bits work items per tile (top hash bits)
-  uint32_t pad;
+  uint32_t wrap;       // 1: integer values, whose SUM wraps at 64 bits: one sum word instead of two limbs
   ColDesc range;       // data == null: no window filter
   int64_t range_lo, range_hi;
   JoinTable dim;       // slots == null: no join
@@ -86,11 +86,18 @@ __device__ __forceinline__ uint64_t* bq_slot(const BQ& q, uint64_t key) {
 
 // Word k of the slot is s[k * ws]: ws = 1 in the arena (array of structs),
 // the slot count in the LDS table (struct of arrays).
+// An integer SUM wraps at 64 bits (scatterSumInt), so word 2 takes the value
+// itself and word 3 stays 0; a DECIMAL SUM is exact: words 2 and 3 take the
+// low 32 bits (unsigned) and the high 32 bits (signed) as carry-save limbs.
 template <typename S>
-__device__ __forceinline__ void bq_apply(uint64_t* s, uint32_t ws, int64_t v) {
+__device__ __forceinline__ void bq_apply(uint64_t* s, uint32_t ws, int64_t v, bool wrap) {
   at_add<S>(s + ws, 1);
-  at_add<S>(s + 2 * ws, (uint64_t)v & 0xFFFFFFFFull);
-  at_add<S>(s + 3 * ws, (uint64_t)(v >> 32));
+  if (wrap) {
+    at_add<S>(s + 2 * ws, (uint64_t)v);
+  } else {
+    at_add<S>(s + 2 * ws, (uint64_t)v & 0xFFFFFFFFull);
+    at_add<S>(s + 3 * ws, (uint64_t)(v >> 32));
+  }
 }
 
 constexpr int kBU = 4;  // rows per thread per step, their loads all in flight together
@@ -169,7 +176,7 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
         if (L && k != kEmpty) {
           uint64_t* ls = lds_find_or_claim(lds, (int)L - 1, k, h);
           if (ls) {
-            bq_apply<LdsT>(ls, L, v);
+            bq_apply<LdsT>(ls, L, v, q.wrap);
             continue;
           }
         }
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
           atomicAdd(&c->overflow, 1ull);  // rare: the query is re-run alone
           continue;
         }
-        bq_apply<GlbT>(s, 1, v);
+        bq_apply<GlbT>(s, 1, v, q.wrap);
       }
     }
     if (L) {
@@ -192,8 +199,8 @@ __global__ void __launch_bounds__(1024) batch_agg_kernel(BatchParams P) {
           continue;
         }
         at_add<GlbT>(s + 1, ls[L]);
-        at_add<GlbT>(s + 2, ls[2 * L]);
-        at_add<GlbT>(s + 3, ls[3 * L]);
+        if (ls[2 * L]) at_add<GlbT>(s + 2, ls[2 * L]);  // words that stayed 0 cost no atomic
+        if (ls[3 * L]) at_add<GlbT>(s + 3, ls[3 * L]);
       }
       __syncthreads();
     }
@@ -473,6 +480,7 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
     }
     d.mask = cap - 1;
     d.max_probe = kMaxProbe;
+    d.wrap = q.val.type != WG_DECIMAL ? 1u : 0u;
     P.out_off[i] = slot_off;  // output region = the query's slot range
     slot_off += cap + 1;
     // an LDS table for queries up to the threshold; above it, a query whose
