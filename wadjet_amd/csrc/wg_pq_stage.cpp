@@ -377,7 +377,7 @@ wg_status decompress(int codec, const uint8_t* ip, uint64_t n, uint8_t* dst, uin
   switch (codec) {
     case kCodecNone:
       if (n < dst_len) return *why = "stored page shorter than its size", WG_ERR_INVALID;
-      memcpy(dst, ip, dst_len);
+      if (dst_len) memcpy(dst, ip, dst_len);
       return WG_OK;
     case kCodecSnappy: ok = snappy_decode(ip, n, dst, dst_len, why); break;
     case kCodecGzip: ok = gzip_decode(ip, n, dst, dst_len, why); break;
@@ -973,7 +973,7 @@ wg_status wg_pq_scan_runs(const uint8_t* data, uint64_t len, int32_t bit_width, 
   *n_runs = nr;
   if (ones) *ones = n1;
   if (runs) {
-    memcpy(runs, table.data(), std::min(nr, cap) * 16);
+    if (nr && cap) memcpy(runs, table.data(), std::min(nr, cap) * 16);
     WG_CHECK(nr <= cap, WG_ERR_INVALID, "%llu runs do not fit %llu", (unsigned long long)nr, (unsigned long long)cap);
   }
   return WG_OK;
@@ -1135,7 +1135,8 @@ wg_status wg_pq_staged_info(const wg_pq_staged* s, uint64_t* rows, uint64_t* n_p
 wg_status wg_pq_staged_pages(const wg_pq_staged* s, wg_pq_page* pages, uint64_t cap) {
   clear_error();
   WG_CHECK(s && (pages || cap == 0), WG_ERR_INVALID, "null argument");
-  memcpy(pages, s->pages.data(), std::min<uint64_t>(cap, s->pages.size()) * sizeof(wg_pq_page));
+  const uint64_t n = std::min<uint64_t>(cap, s->pages.size());
+  if (n) memcpy(pages, s->pages.data(), n * sizeof(wg_pq_page));
   return WG_OK;
 }
 
