@@ -38,7 +38,7 @@ JOIN_PMC_SUMMARY = ROOT / "profiles" / "join_probe_agg_200M_pmc.txt"
 
 
 def is_consume_kernel(name):
-    return any(s in name for s in ("part_count", "part_scatter", "part_agg", "part_bounds", "rocprim",
+    return any(s in name for s in ("part_l1", "part_tot", "part_l2", "part_agg", "rocprim",
                                    "agg_consume_kernel", "hot_"))
 
 
@@ -96,15 +96,16 @@ def bench_agg(ctx, args):
     ms_step = wall * 1000 / args.steps
     k_ms = sum(kern_ms) / len(kern_ms)
     alg = W.ZIPF_BYTES_PER_ROW * n + W.ZIPF_BYTES_PER_GROUP * groups
-    cold = agg.stats()["global_rows"]  # rows the scatter pass wrote to the partitions
+    cold = agg.stats()["global_rows"]  # rows the sorting passes wrote to the partitions
     log(f"[agg] {ms_step:.3f} ms/step, consume kernels {k_ms:.3f} ms, groups {groups}; "
-        f"passes (direct, count+scan, scatter, partition-agg) {passes[-1]}; scattered rows {cold}")
-    # each pass against its own algorithmic bytes: count reads key + predicate
-    # (16 B/row); scatter reads the 24 B row and writes a 24 B record per cold
-    # row; the partition aggregate reads the records and writes 56 B per group
+        f"passes (direct, sort level 1, sort level 2, partition-agg) {passes[-1]}; partitioned rows {cold}")
+    # each pass against its own algorithmic bytes: level 1 reads the 24 B row
+    # and writes a 24 B record (key, price, f) per cold row; level 2 reads and
+    # writes each record once; the partition aggregate reads the records and
+    # writes 56 B per group
     p = passes[-1]
     per_pass = {}
-    for name, ms, b in (("count", p[1], 16 * n), ("scatter", p[2], 24 * n + 24 * cold),
+    for name, ms, b in (("sort_level1", p[1], 24 * n + 24 * cold), ("sort_level2", p[2], 48 * cold),
                         ("partition_agg", p[3], 24 * cold + W.ZIPF_BYTES_PER_GROUP * groups)):
         if ms > 0:
             per_pass[name] = dict(ms=ms, alg_bytes=b, achieved=b / (ms / 1000) / 1e9,
@@ -688,10 +689,10 @@ def main():
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                          "traffic_source": AGG_PMC_SUMMARY.name if traffic else None,
-                         "kernels": "consume kernels of one step (direct, or count+scan, scatter, "
-                                    "partition-aggregate)",
+                         "kernels": "consume kernels of one step (direct, or heavy-hitter set + sort level 1 "
+                                    "+ scan, sort level 2, partition-aggregate)",
                          "kernel_ms": A["kernel_ms"], "pass_ms": A["passes"], "alg_bytes": A["alg_bytes"],
-                         "per_pass": A["per_pass"], "scattered_rows": A["cold_rows"]},
+                         "per_pass": A["per_pass"], "partitioned_rows": A["cold_rows"]},
             "cpu_baseline": cpu,
             "parity": {"tests": "tests/test_fullsize_gpu.py (this workload at 100M rows and the join at 10M x 200M, "
                                 "against the oracle)",

@@ -268,18 +268,20 @@ wg_status wg_agg_stats(wg_agg* h, uint64_t* lds_rows, uint64_t* global_rows,
 wg_status wg_agg_kernel_time(wg_agg* h, float* ms, uint32_t* launches);
 
 /* Device time (ms) of the last consume by phase: [0] direct consume kernel,
- * [1] partition count + scan, [2] partition scatter, [3] partition aggregate. */
+ * [1] partitioned consume pass 1 (heavy-hitter set, tiles sorted by level-1
+ * partition, partition totals + scan), [2] pass 2 (records gathered into
+ * contiguous final partitions), [3] partition aggregate. */
 wg_status wg_agg_pass_times(wg_agg* h, float* ms4);
 /* Consume strategy: 0 = pick by batch size and cardinality, 1 = direct
- * (shared table with LDS pre-aggregation), 2 = partitioned (count, scatter,
- * per-partition LDS aggregation); partition LDS table slots, workgroup size
+ * (shared table with LDS pre-aggregation), 2 = partitioned (two sorting
+ * passes by hash partition, per-partition LDS aggregation); partition LDS table slots, workgroup size
  * and the largest partition chunk one workgroup takes. */
 wg_status wg_agg_set_strategy(wg_agg* h, int32_t strategy, int32_t part_lds_slots, int32_t part_block,
                               uint64_t part_chunk_rows);
 /* Heavy hitters of the partitioned consume (no reference counterpart; results
  * do not depend on it): up to hot_slots keys that a sample of the batch sees
- * most often are aggregated in LDS during the scatter pass instead of being
- * partitioned. 0 disables; otherwise a power of two in [64, 2048]. */
+ * most often are aggregated in LDS during the first sorting pass instead of
+ * being partitioned. 0 disables; otherwise a power of two in [64, 2048]. */
 wg_status wg_agg_set_heavy_hitters(wg_agg* h, int32_t hot_slots);
 
 /* ------------------------------------------------------------------ filter */

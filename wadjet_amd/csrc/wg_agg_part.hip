@@ -4,31 +4,40 @@
 // atomics; when a batch holds millions of groups with a skewed (Zipf) key,
 // rows of warm keys queue up on the same slot and on the same L2/memory-side
 // atomic unit. This path brings the rows of a key together first, so every
-// key is aggregated by ONE workgroup in LDS:
-//   A  part_count_kernel    each workgroup owns a contiguous row range, filters
-//                           it, hashes the keys and counts rows per partition
-//                           (top bits of the hash) in an LDS histogram
-//   B  exclusive scan       over the [partition][workgroup] counts
-//   C  part_scatter_kernel  the same workgroups re-read their rows and write
-//                           (packed key, the value words the program reads)
-//                           to their slice of each partition (LDS cursors)
-//   D  part_agg_kernel      one workgroup per partition chunk aggregates it in
-//                           an LDS table and merges the table into the global
-//                           table: plain read-modify-writes for a partition it
-//                           owns alone, atomics for the chunks of a large
-//                           partition split over several workgroups
-// Rows whose key is NULL or equals the table sentinel go straight to the side
-// slots in pass C. Results are the same accumulator words as the direct path.
+// key is aggregated by ONE workgroup in LDS. A key's final partition is the
+// top pbits bits of its hash; pass 1 sorts by the top b1 of them (level 1),
+// pass 2 by the rest (level 2):
+//   1  part_l1_*kernel   each workgroup owns a contiguous row range and walks
+//                        it in tiles: a tile's cold records (packed key, the
+//                        value words the program reads) are ranked per
+//                        level-1 partition in LDS and written as one block
+//                        sorted by level-1 partition into the workgroup's own
+//                        region; per-final-partition counts go to a histogram
+//      part_tot_kernel   + exclusive scan: final partition boundaries
+//   2  part_l2_kernel    per (level-1 partition, tile range): gathers its
+//                        runs, ranks each record within its final partition
+//                        and writes it into that partition's range, so every
+//                        final partition is contiguous
+//   3  part_agg_kernel   one workgroup per partition chunk aggregates it in
+//                        an LDS table and writes its groups to the dense
+//                        output (a partition it owns alone) or merges the
+//                        table into the global table (atomics for the chunks
+//                        of a large partition split over several workgroups)
+// Both sorting passes write blocks, not scattered records: the writes of one
+// tile or round land in a few kilobytes that L2 completes into whole lines,
+// where one pass of scattered 32-byte records over thousands of partitions
+// leaves every line partly written. Rows whose key is NULL or equals the
+// table sentinel go straight to the side slots in pass 1. Results are the
+// same accumulator words as the direct path.
 //
 // Heavy hitters: under a skewed key (Zipf) a few hundred keys carry half the
-// rows. Before pass A a sample of the batch (kSampleTiles spread tiles) is
+// rows. Before pass 1 a sample of the batch (kSampleTiles spread tiles) is
 // counted and the keys seen most often (at most H of them) become the hot
-// set. Passes A and C look every row up in an LDS copy of the hot set: hot
-// rows are not counted or scattered but aggregated in an LDS table of H
-// slots inside pass C; each workgroup writes its H partial slots out, and
-// hot_merge_kernel folds them into the global table. Only the cold rows pay
-// for the scattered record writes and for pass D. The hot set is fixed before
-// pass A, so passes A and C classify every row the same way.
+// set. Pass 1 looks every row up in an LDS copy of the hot set: hot rows are
+// not written out but aggregated in an LDS table of H slots; each workgroup
+// writes its H partial slots out, and hot_merge_kernel folds them into the
+// global table (or the dense output). Only the cold rows pay for the record
+// traffic of passes 1-3.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -39,40 +48,41 @@
 
 namespace wg {
 
-constexpr int kCountU = kRowU;    // rows per thread per step of the count loop
-constexpr int kScatterU = kRowU2;  // and of the scatter loop (more values live per row)
 constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
-constexpr int kPartBlock = 512;  // count/scatter workgroup size
+constexpr int kPartBlock = 512;  // pass-1 workgroup size
 
 struct PartParams {
   ColSet cs;
   KeyDesc key;
   PredDesc pred[2];
-  AggProgram pg;  // for side-slot rows in pass C (original columns)
+  AggProgram pg;  // for side-slot rows in pass 1 (original columns)
   GlobalTable T;
   const uint32_t* sel;
   uint64_t row_base;  // always 0: rows are sel[i] or i
   uint64_t n;
-  uint64_t rows_per_wg;
-  int32_t pbits;
+  uint64_t rows_per_wg;   // a multiple of the tile rows
+  uint32_t tiles_per_wg;
+  int32_t pbits;          // final partition = top pbits bits of the key hash
+  int32_t b1;             // level-1 partition = top b1 of those (b1 <= 7)
   int32_t n_vals;
-  uint32_t* counts;      // [P][G]
-  const uint32_t* offs;  // [P][G] exclusive scan of counts
-  uint64_t* recs;        // partitioned records: key word, value words, zero padding
+  uint32_t* hist;         // [G][2^pbits] cold rows per workgroup and final partition
+  uint32_t* d1_off;       // [tiles][2^b1 + 1] per tile: run offsets, then its record count
+  uint32_t* d1_base;      // [tiles] first position of each tile's block
+  uint64_t* recs;         // pass-1 records: key word, value words, zero padding
   int32_t val_col[kMaxVals];
   // heavy hitters (hot_slots = 0: none)
   AggProgram pgv;            // pg with op columns remapped onto the value words
   uint32_t vfloat;           // bit j: value word j holds float64 bits
   const uint64_t* hot_keys;  // the hot set, *n_hot keys
   const uint32_t* n_hot;
-  int32_t hot_slots;         // H: LDS accumulator slots of pass C (>= *n_hot)
+  int32_t hot_slots;         // H: LDS accumulator slots of pass 1 (>= *n_hot)
   int32_t hot_bits;          // the LDS lookup of the hot set has 2^hot_bits entries
-  uint64_t* hot_part;        // [G][H][stride] per-workgroup partial slots of pass C
+  uint64_t* hot_part;        // [G][H][stride] per-workgroup partial slots of pass 1
 };
 
-// LDS of passes A and C: the partition counters (np u32), then the hot-set
-// lookup (2^hot_bits keys and their hot indices), then in pass C the hot
-// accumulators (struct of arrays, H slots + one spare slot per lane).
+// The hot set in LDS: a bit filter, the lookup (2^hot_bits keys and their hot
+// indices) and the hot accumulators (struct of arrays, H slots + one spare
+// slot per lane).
 // A bit filter over 16 hash bits answers most cold rows with one LDS read.
 struct HotLds {
   uint32_t* bits;  // kHotFilterBits bits: set for the hash of every hot key
@@ -130,42 +140,196 @@ __device__ __forceinline__ int hot_find(const HotLds& L, uint64_t key, uint64_t 
   return -1;
 }
 
-__global__ void __launch_bounds__(kPartBlock) part_count_kernel(PartParams P) {
-  extern __shared__ uint32_t hist[];
-  const RowSpec<0> S = make_spec<0>(P);
-  const uint32_t np = 1u << P.pbits;
-  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) hist[i] = 0;
-  const HotLds HL = hot_lds(hist, np, P.hot_bits, P.hot_slots);
-  if (P.hot_slots) hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)P.hot_slots));
-  __syncthreads();
-  const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
-  const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kCountU) {
-    uint64_t row[kCountU], key[kCountU], val[kCountU][1];
-    uint32_t vvalid[kCountU];
-    int kind[kCountU];
-    load_rows<0, kCountU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
-    // no per-row branch: rows that do not count (filtered, side slot, hot)
-    // add 0 to their bucket
-#pragma unroll
-    for (int u = 0; u < kCountU; ++u) {
-      const uint64_t hk = key_hash(key[u]);
-      const bool cold = kind[u] == 0 && (P.hot_slots == 0 || hot_find(HL, key[u], hk) < 0);
-      atomicAdd(&hist[hk >> (64 - P.pbits)], cold ? 1u : 0u);
+// ------------------------------------------------------------ pass 1
+//
+// Each workgroup owns a contiguous row range and walks it in tiles of
+// blockDim * U rows. Per tile: hot rows aggregate in LDS, side-slot rows go to
+// the table's side slots, and every cold row takes a rank within its level-1
+// partition (the top b1 bits of its partition index) from an LDS counter. One
+// wave scans the tile's counts; then each cold row writes its record at the
+// tile's base + its level-1 partition's offset + its rank. A tile's records
+// are one contiguous block sorted by level-1 partition, written right after
+// the previous tile's block in the workgroup's own region (which starts at
+// the position of its first row and can hold all of its rows): no count pass
+// is needed, and the writes of one tile fall in a few kilobytes that L2
+// assembles into whole lines before they reach HBM. The tile's offsets and
+// base are its descriptor. The workgroup also counts its cold rows per final
+// partition in an LDS histogram; pass 2's partition boundaries are the scan
+// of those counts.
+
+constexpr int kL1TU = 4;  // typed loads: tiles of kPartBlock * 4 = 2048 rows
+constexpr int kL1GU = 2;  // general loader: tiles of kPartBlock * 2 rows
+
+// LDS of pass 1 (u32 words): cnt[np1], off[np1 + 1] (the offsets, then the
+// tile's base position), hist[np], then the hot-set lookup and accumulators.
+struct L1Lds {
+  uint32_t* cnt;
+  uint32_t* off;
+  uint32_t* hist;
+};
+
+__host__ __device__ inline uint32_t l1_hot_offset(uint32_t np1, uint32_t np) { return (2 * np1 + 1 + np + 1) & ~1u; }
+
+__device__ __forceinline__ L1Lds l1_lds(uint32_t* lds, uint32_t np1) { return L1Lds{lds, lds + np1, lds + 2 * np1 + 1}; }
+
+// Zeroed counters and histogram, the hot accumulators at their initial values
+// and the hot-set lookup; returns the hot layout.
+__device__ __forceinline__ HotLds l1_setup(const PartParams& P, uint32_t* lds) {
+  const uint32_t ho = l1_hot_offset(1u << P.b1, 1u << P.pbits);
+  for (uint32_t i = threadIdx.x; i < ho; i += blockDim.x) lds[i] = 0;
+  const HotLds HL = hot_lds(lds + ho, 0, P.hot_bits, P.hot_slots);
+  const int H = P.hot_slots;
+  if (H) {
+    for (int w = 0; w < P.pg.stride; ++w) {
+      const uint64_t v = slot_init_word(P.pg, w);
+      for (int i = threadIdx.x; i < HL.HW; i += blockDim.x) HL.acc[(uint64_t)w * HL.HW + i] = v;
     }
+    hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)H));
   }
   __syncthreads();
-  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
+  return HL;
 }
 
-// Pass A for the common column shapes, with the type decisions taken at
-// compile time: a key of one or two non-null 4- or 8-byte integer columns and
-// at most one predicate on a non-null 4- or 8-byte column. W0 / W1 / PW are the
-// byte widths of key column 0, key column 1 (0: none) and the predicate column
-// (0: none); signedness, the operator and the key packing stay runtime values
-// (uniform, resolved outside the row loop). Counts exactly the rows
-// part_count_kernel counts.
-constexpr int kCountTU = 8;  // rows per thread per step
+// One tile of U loaded rows per thread: hot rows into the LDS accumulators,
+// side-slot rows into the table's side slots, cold rows ranked, scanned and
+// written as the tile's block. `run` (wave 0) is the next free position of
+// the workgroup's region. Every thread of the workgroup calls this once per
+// tile (it holds two barriers).
+template <int RW, int U>
+__device__ __forceinline__ void l1_tile(const PartParams& P, const HotLds& HL, const L1Lds& S, uint64_t tile,
+                                        uint32_t& run, const uint64_t (&row)[U], const uint64_t (&key)[U],
+                                        const int (&kind)[U], const uint64_t (&val)[U][RW - 1],
+                                        const uint32_t (&vvalid)[U]) {
+  constexpr int NV = RW - 1;
+  const int H = P.hot_slots;
+  const int stride = P.pg.stride;
+  const int b2 = P.pbits - P.b1;
+  const uint32_t np1 = 1u << P.b1;
+  uint64_t hk[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) hk[u] = key_hash(key[u]);
+  uint32_t hot = 0;
+  if (H) {
+    // hot rows aggregate here: the same LDS atomics on every lane
+    // (identity operands on the lane's spare slot for the others)
+    uint64_t* const spare = HL.acc + H + (threadIdx.x & 63);
+    uint64_t* w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int hi = kind[u] == 0 ? hot_find(HL, key[u], hk[u]) : -1;
+      w[u] = hi >= 0 ? HL.acc + hi : spare;
+      hot |= (uint32_t)(hi >= 0) << u;
+    }
+    apply_vals_lds_batch<NV, U>(w, HL.HW, P.pgv, val, P.vfloat, vvalid, hot);
+  }
+  uint32_t part[U], rank[U], cold = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    part[u] = (uint32_t)(hk[u] >> (64 - P.pbits));
+    rank[u] = 0;
+    if (kind[u] < 0 || ((hot >> u) & 1u)) continue;
+    if (kind[u] != 0) {
+      // side slots: [0] the sentinel-valued key, [1] the NULL key
+      uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
+      apply_row<GlbT, false>(s, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
+      continue;
+    }
+    cold |= 1u << u;
+    atomicAdd(&S.hist[part[u]], 1u);
+    rank[u] = atomicAdd(&S.cnt[part[u] >> b2], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // wave 0: exclusive scan of the np1 <= 128 counts, two per lane
+    const uint32_t l = threadIdx.x, i0 = 2 * l, i1 = 2 * l + 1;
+    const uint32_t c0 = i0 < np1 ? S.cnt[i0] : 0u, c1 = i1 < np1 ? S.cnt[i1] : 0u;
+    uint32_t s = c0 + c1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(s, d);
+      if (l >= (uint32_t)d) s += y;
+    }
+    const uint32_t e0 = s - c0 - c1;
+    uint32_t* desc = P.d1_off + tile * (np1 + 1);
+    if (i0 < np1) {
+      S.off[i0] = e0;
+      S.cnt[i0] = 0;
+      desc[i0] = e0;
+    }
+    if (i1 < np1) {
+      S.off[i1] = e0 + c0;
+      S.cnt[i1] = 0;
+      desc[i1] = e0 + c0;
+    }
+    const uint32_t total = __shfl(s, 63);
+    if (l == 0) {
+      S.off[np1] = run;
+      desc[np1] = total;
+      P.d1_base[tile] = run;
+    }
+    run += total;
+  }
+  __syncthreads();
+  const uint64_t base = S.off[np1];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (!((cold >> u) & 1u)) continue;
+    uint64_t w[RW];
+    w[0] = key[u];
+#pragma unroll
+    for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
+    uint64_t* rec = P.recs + (base + S.off[part[u] >> b2] + rank[u]) * RW;
+    static_assert(RW % 2 == 0, "records are whole 16-byte pairs");
+#pragma unroll
+    for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)rec)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+  }
+}
+
+// This workgroup's partial hot slots, dense: [H][stride], key in word 0.
+__device__ __forceinline__ void scatter_flush_hot(const PartParams& P, const HotLds& HL) {
+  const int H = P.hot_slots, stride = P.pg.stride;
+  if (!H) return;
+  __syncthreads();
+  const uint32_t n_hot = min(*P.n_hot, (uint32_t)H);
+  uint64_t* dst = P.hot_part + (uint64_t)blockIdx.x * H * stride;
+  for (int i = threadIdx.x; i < H * stride; i += blockDim.x) {
+    const int h = i / stride, w = i % stride;
+    dst[i] = w ? HL.acc[(uint64_t)w * HL.HW + h] : ((uint32_t)h < n_hot ? P.hot_keys[h] : kEmpty);
+  }
+}
+
+// End of pass 1: the workgroup's histogram row and its hot partials.
+__device__ __forceinline__ void l1_finish(const PartParams& P, const L1Lds& S, const HotLds& HL) {
+  const uint32_t np = 1u << P.pbits;
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.hist[(uint64_t)blockIdx.x * np + p] = S.hist[p];
+  scatter_flush_hot(P, HL);
+}
+
+// Pass 1 through the general row loader (any key, predicate and value
+// columns the partitioned consume takes). RW = record words: key + values
+// (2, 4 or 8 words; zero pad words after the values).
+template <int RW>
+__global__ void __launch_bounds__(kPartBlock) part_l1_kernel(PartParams P) {
+  constexpr int NV = RW - 1, U = kL1GU;
+  extern __shared__ uint32_t lds1[];
+  const RowSpec<NV> S = make_spec<NV>(P);
+  const HotLds HL = l1_setup(P, lds1);
+  const L1Lds T = l1_lds(lds1, 1u << P.b1);
+  const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
+  const uint64_t end = min(beg + P.rows_per_wg, P.n);
+  uint32_t run = (uint32_t)beg;  // the region of the workgroup starts at its first row's position
+  const uint64_t t0 = (uint64_t)blockIdx.x * P.tiles_per_wg;
+  for (uint32_t k = 0; k < P.tiles_per_wg; ++k) {
+    uint64_t row[U], key[U], val[U][NV];
+    uint32_t vvalid[U];
+    int kind[U];
+    load_rows<NV, U>(S, beg + (uint64_t)k * blockDim.x * U + threadIdx.x, blockDim.x, end, row, key, kind, val,
+                     vvalid);
+    l1_tile<RW, U>(P, HL, T, t0 + k, run, row, key, kind, val, vvalid);
+  }
+  l1_finish(P, T, HL);
+}
 
 template <int W>
 __device__ __forceinline__ uint64_t load_w(const char* base, uint64_t row) {
@@ -190,206 +354,22 @@ __device__ __forceinline__ uint32_t pred_bits(const uint64_t (&v)[U], bool is_fl
   return m;
 }
 
-template <int W0, int W1, int PW>
-__global__ void __launch_bounds__(kPartBlock) part_count_typed_kernel(PartParams P) {
-  constexpr int U = kCountTU;
-  extern __shared__ uint32_t hist[];
-  const uint32_t np = 1u << P.pbits;
-  for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) hist[i] = 0;
-  const HotLds HL = hot_lds(hist, np, P.hot_bits, P.hot_slots);
-  if (P.hot_slots) hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)P.hot_slots));
-  __syncthreads();
-  const char* d0 = (const char*)P.cs.cols[P.key.col[0]].data;
-  const char* d1 = (const char*)P.cs.cols[P.key.col[W1 ? 1 : 0]].data;
-  const char* dp = (const char*)P.cs.cols[PW ? P.pred[0].col : 0].data;
-  const bool sx0 = P.cs.cols[P.key.col[0]].type == WG_INT32;
-  const bool sx1 = W1 && P.cs.cols[P.key.col[1]].type == WG_INT32;
-  const bool pf = PW && P.cs.cols[P.pred[0].col].type == WG_FLOAT64;
-  const int sh0 = P.key.shift[0], sh1 = P.key.shift[1];
-  const int op = P.pred[0].op;
-  const int64_t ci = P.pred[0].i64;
-  const double cf = P.pred[0].f64;
-  const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
-  const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * U) {
-    uint64_t row[U], a[U], b[U], pv[U];
-    uint32_t in = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t i = base + (uint64_t)u * blockDim.x + threadIdx.x;
-      in |= (uint32_t)(i < end) << u;
-      const uint64_t ii = i < end ? i : end - 1;
-      row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
-    }
-    // every load of the step in flight before any is used
-#pragma unroll
-    for (int u = 0; u < U; ++u) a[u] = load_w<W0>(d0, row[u]);
-#pragma unroll
-    for (int u = 0; u < U; ++u) b[u] = W1 ? load_w<W1 ? W1 : 4>(d1, row[u]) : 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) pv[u] = PW ? load_w<PW ? PW : 4>(dp, row[u]) : 0;
-    const uint32_t pass = in & pred_bits<PW, U>(pv, pf, op, ci, cf);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      uint64_t key;
-      if (W1) {  // two 4-byte columns packed into one word
-        key = ((a[u] & 0xFFFFFFFFull) << sh0) | ((b[u] & 0xFFFFFFFFull) << sh1);
-      } else {
-        key = (W0 == 4 && sx0) ? (uint64_t)(int64_t)(int32_t)(uint32_t)a[u] : a[u];
-      }
-      const uint64_t hk = key_hash(key);
-      // rows that do not count (filtered, sentinel-valued key, hot) add 0
-      const bool cold = ((pass >> u) & 1u) && key != kEmpty && (P.hot_slots == 0 || hot_find(HL, key, hk) < 0);
-      atomicAdd(&hist[hk >> (64 - P.pbits)], cold ? 1u : 0u);
-    }
-    (void)sx1;
-  }
-  __syncthreads();
-  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) P.counts[(uint64_t)p * gridDim.x + blockIdx.x] = hist[p];
-}
-
-using CountFn = void (*)(PartParams);
-
-// The typed pass A for this batch's columns, or null (the general kernel).
-static CountFn typed_count_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pred) {
-  auto width = [](int t) { return (t == WG_INT32 || t == WG_UINT32) ? 4 : (t == WG_INT64 || t == WG_IPV4) ? 8 : 0; };
-  if (K.n > 2 || pred[1].col >= 0) return nullptr;
-  int w[2] = {0, 0};
-  for (int k = 0; k < K.n; ++k) {
-    const ColDesc& c = cs.cols[K.col[k]];
-    if (c.valid) return nullptr;
-    w[k] = width(c.type);
-    if (!w[k]) return nullptr;
-  }
-  if (K.n == 2 && (w[0] != 4 || w[1] != 4)) return nullptr;
-  int pw = 0;
-  if (pred[0].col >= 0) {
-    const ColDesc& c = cs.cols[pred[0].col];
-    if (c.valid) return nullptr;
-    pw = c.type == WG_FLOAT64 ? 8 : c.type == WG_INT32 ? 4 : (c.type == WG_INT64 || c.type == WG_IPV4) ? 8 : 0;
-    if (!pw) return nullptr;
-  }
-#define WG_TC(A, B, C) if (w[0] == A && w[1] == B && pw == C) return part_count_typed_kernel<A, B, C>;
-  WG_TC(4, 4, 0) WG_TC(4, 4, 4) WG_TC(4, 4, 8)
-  WG_TC(4, 0, 0) WG_TC(4, 0, 4) WG_TC(4, 0, 8)
-  WG_TC(8, 0, 0) WG_TC(8, 0, 4) WG_TC(8, 0, 8)
-#undef WG_TC
-  return nullptr;
-}
-
-// RW = record words: key + values (2, 4 or 8 words; zero pad words after
-// the values).
-// Pass C's LDS: the workgroup's partition cursors, the hot-set lookup and the
-// hot accumulators (initial values); returns the hot layout.
-__device__ __forceinline__ HotLds scatter_setup(const PartParams& P, uint32_t* cur) {
-  const uint32_t np = 1u << P.pbits;
-  for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) cur[p] = P.offs[(uint64_t)p * gridDim.x + blockIdx.x];
-  const HotLds HL = hot_lds(cur, np, P.hot_bits, P.hot_slots);
-  const int H = P.hot_slots;
-  if (H) {
-    for (int w = 0; w < P.pg.stride; ++w) {
-      const uint64_t v = slot_init_word(P.pg, w);
-      for (int i = threadIdx.x; i < HL.HW; i += blockDim.x) HL.acc[(uint64_t)w * HL.HW + i] = v;
-    }
-    hot_lookup_build(HL, P.hot_keys, min(*P.n_hot, (uint32_t)H));
-  }
-  __syncthreads();
-  return HL;
-}
-
-// Pass C for U loaded rows of a thread: hot rows into the LDS accumulators,
-// side-slot rows into the table's side slots, cold rows to their partition.
-template <int RW, int U>
-__device__ __forceinline__ void scatter_rows(const PartParams& P, const HotLds& HL, uint32_t* cur,
-                                             const uint64_t (&row)[U], const uint64_t (&key)[U], const int (&kind)[U],
-                                             const uint64_t (&val)[U][RW - 1], const uint32_t (&vvalid)[U]) {
-  constexpr int NV = RW - 1;
-  const int H = P.hot_slots;
-  const int stride = P.pg.stride;
-  uint64_t hk[U];
-  uint32_t hot = 0;
-#pragma unroll
-  for (int u = 0; u < U; ++u) hk[u] = key_hash(key[u]);
-  if (H) {
-    // hot rows aggregate here: the same LDS atomics on every lane
-    // (identity operands on the lane's spare slot for the others)
-    uint64_t* const spare = HL.acc + H + (threadIdx.x & 63);
-    uint64_t* w[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int hi = kind[u] == 0 ? hot_find(HL, key[u], hk[u]) : -1;
-      w[u] = hi >= 0 ? HL.acc + hi : spare;
-      hot |= (uint32_t)(hi >= 0) << u;
-    }
-    apply_vals_lds_batch<NV, U>(w, HL.HW, P.pgv, val, P.vfloat, vvalid, hot);
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if (kind[u] < 0 || ((hot >> u) & 1u)) continue;
-    if (kind[u] != 0) {
-      // side slots: [0] the sentinel-valued key, [1] the NULL key
-      uint64_t* s = P.T.slots + (P.T.mask + (kind[u] == 1 ? 1 : 2)) * (uint64_t)stride;
-      apply_row<GlbT, false>(s, 1, P.pg, P.cs, RowRef{row[u], 0, nullptr});
-      continue;
-    }
-    const uint32_t pos = atomicAdd(&cur[hk[u] >> (64 - P.pbits)], 1u);
-    uint64_t w[RW];
-    w[0] = key[u];
-#pragma unroll
-    for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
-    uint64_t* rec = P.recs + (uint64_t)pos * RW;
-    static_assert(RW % 2 == 0, "records are whole 16-byte pairs");
-#pragma unroll
-    for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)rec)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
-  }
-}
-
-// This workgroup's partial hot slots, dense: [H][stride], key in word 0.
-__device__ __forceinline__ void scatter_flush_hot(const PartParams& P, const HotLds& HL) {
-  const int H = P.hot_slots, stride = P.pg.stride;
-  if (!H) return;
-  __syncthreads();
-  const uint32_t n_hot = min(*P.n_hot, (uint32_t)H);
-  uint64_t* dst = P.hot_part + (uint64_t)blockIdx.x * H * stride;
-  for (int i = threadIdx.x; i < H * stride; i += blockDim.x) {
-    const int h = i / stride, w = i % stride;
-    dst[i] = w ? HL.acc[(uint64_t)w * HL.HW + h] : ((uint32_t)h < n_hot ? P.hot_keys[h] : kEmpty);
-  }
-}
-
-// RW = record words: key + values (2, 4 or 8 words; zero pad words after
-// the values).
-template <int RW>
-__global__ void __launch_bounds__(kPartBlock) part_scatter_kernel(PartParams P) {
-  constexpr int NV = RW - 1;
-  extern __shared__ uint32_t cur[];
-  const RowSpec<NV> S = make_spec<NV>(P);
-  const HotLds HL = scatter_setup(P, cur);
-  const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
-  const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kScatterU) {
-    uint64_t row[kScatterU], key[kScatterU], val[kScatterU][NV];
-    uint32_t vvalid[kScatterU];
-    int kind[kScatterU];
-    load_rows<NV, kScatterU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
-    scatter_rows<RW, kScatterU>(P, HL, cur, row, key, kind, val, vvalid);
-  }
-  scatter_flush_hot(P, HL);
-}
-
-// Pass C for the shapes the typed pass A takes (keys of one or two non-null
-// 4/8-byte integer columns, at most one predicate on a non-null 4/8-byte
-// column) whose value columns are non-null 8-byte words (int64, DECIMAL,
-// float64, IPv4): loads typed by width, kScatterTU rows in flight per thread,
-// a value column that is also the predicate column read once. Writes exactly
-// what part_scatter_kernel writes for the same rows.
-constexpr int kScatterTU = 4;
-
+// Pass 1 for the common column shapes, with the type decisions taken at
+// compile time: a key of one or two non-null 4- or 8-byte integer columns,
+// at most one predicate on a non-null 4- or 8-byte column, and value columns
+// that are non-null 8-byte words (int64, DECIMAL, float64, IPv4). W0 / W1 /
+// PW are the byte widths of key column 0, key column 1 (0: none) and the
+// predicate column (0: none); signedness, the operator and the key packing
+// stay runtime values (uniform, resolved outside the row loop). Loads are
+// typed by width, kL1TU rows in flight per thread, and a value column that is
+// also the predicate column is read once. Classifies and writes exactly what
+// part_l1_kernel does for the same rows.
 template <int W0, int W1, int PW, int RW>
-__global__ void __launch_bounds__(kPartBlock) part_scatter_typed_kernel(PartParams P) {
-  constexpr int U = kScatterTU, NV = RW - 1;
-  extern __shared__ uint32_t cur[];
-  const HotLds HL = scatter_setup(P, cur);
+__global__ void __launch_bounds__(kPartBlock) part_l1_typed_kernel(PartParams P) {
+  constexpr int U = kL1TU, NV = RW - 1;
+  extern __shared__ uint32_t lds1[];
+  const HotLds HL = l1_setup(P, lds1);
+  const L1Lds T = l1_lds(lds1, 1u << P.b1);
   const char* d0 = (const char*)P.cs.cols[P.key.col[0]].data;
   const char* d1 = (const char*)P.cs.cols[P.key.col[W1 ? 1 : 0]].data;
   const char* dp = (const char*)P.cs.cols[PW ? P.pred[0].col : 0].data;
@@ -409,7 +389,10 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_typed_kernel(PartPara
   const double cf = P.pred[0].f64;
   const uint64_t beg = (uint64_t)blockIdx.x * P.rows_per_wg;
   const uint64_t end = min(beg + P.rows_per_wg, P.n);
-  for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * U) {
+  uint32_t run = (uint32_t)beg;
+  const uint64_t t0 = (uint64_t)blockIdx.x * P.tiles_per_wg;
+  for (uint32_t k = 0; k < P.tiles_per_wg; ++k) {
+    const uint64_t base = beg + (uint64_t)k * blockDim.x * U;
     uint64_t row[U], a[U], b[U], pv[U], key[U], val[U][NV];
     uint32_t in = 0, vvalid[U];
     int kind[U];
@@ -421,7 +404,7 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_typed_kernel(PartPara
       row[u] = P.sel ? (uint64_t)P.sel[ii] : ii;
       vvalid[u] = ~0u;
     }
-    // every load of the step in flight before any is used
+    // every load of the tile in flight before any is used
 #pragma unroll
     for (int u = 0; u < U; ++u) a[u] = load_w<W0>(d0, row[u]);
 #pragma unroll
@@ -448,9 +431,50 @@ __global__ void __launch_bounds__(kPartBlock) part_scatter_typed_kernel(PartPara
       }
       kind[u] = !((pass >> u) & 1u) ? -1 : key[u] == kEmpty ? 1 : 0;
     }
-    scatter_rows<RW, U>(P, HL, cur, row, key, kind, val, vvalid);
+    l1_tile<RW, U>(P, HL, T, t0 + k, run, row, key, kind, val, vvalid);
   }
-  scatter_flush_hot(P, HL);
+  l1_finish(P, T, HL);
+}
+
+using L1Fn = void (*)(PartParams);
+
+// The typed pass 1 for this batch's columns, or null (the general kernel).
+static L1Fn typed_l1_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pred, const std::vector<int>& vcols,
+                         int rw) {
+  auto width = [](int t) { return (t == WG_INT32 || t == WG_UINT32) ? 4 : (t == WG_INT64 || t == WG_IPV4) ? 8 : 0; };
+  if (K.n > 2 || pred[1].col >= 0 || rw > 4) return nullptr;
+  int w[2] = {0, 0};
+  for (int k = 0; k < K.n; ++k) {
+    const ColDesc& c = cs.cols[K.col[k]];
+    if (c.valid) return nullptr;
+    w[k] = width(c.type);
+    if (!w[k]) return nullptr;
+  }
+  if (K.n == 2 && (w[0] != 4 || w[1] != 4)) return nullptr;
+  int pw = 0;
+  if (pred[0].col >= 0) {
+    const ColDesc& c = cs.cols[pred[0].col];
+    if (c.valid) return nullptr;
+    pw = c.type == WG_FLOAT64 ? 8 : c.type == WG_INT32 ? 4 : (c.type == WG_INT64 || c.type == WG_IPV4) ? 8 : 0;
+    if (!pw) return nullptr;
+  }
+  for (int c : vcols) {
+    const ColDesc& d = cs.cols[c];
+    if (d.valid || !(d.type == WG_INT64 || d.type == WG_DECIMAL || d.type == WG_FLOAT64 || d.type == WG_IPV4))
+      return nullptr;
+  }
+#define WG_TL(A, B, C)                                                                 \
+  if (w[0] == A && w[1] == B && pw == C)                                               \
+    return rw == 2 ? part_l1_typed_kernel<A, B, C, 2> : part_l1_typed_kernel<A, B, C, 4>;
+  WG_TL(4, 4, 0) WG_TL(4, 4, 4) WG_TL(4, 4, 8)
+  WG_TL(4, 0, 0) WG_TL(4, 0, 4) WG_TL(4, 0, 8)
+  WG_TL(8, 0, 0) WG_TL(8, 0, 4) WG_TL(8, 0, 8)
+#undef WG_TL
+  return nullptr;
+}
+
+static L1Fn general_l1_for(int rw) {
+  return rw == 2 ? part_l1_kernel<2> : rw == 4 ? part_l1_kernel<4> : part_l1_kernel<8>;
 }
 
 // Dense emission of finished groups. A partitioned consume that starts from an
@@ -513,13 +537,13 @@ __global__ void __launch_bounds__(256) hot_sample_kernel(PartParams P, HotScratc
   const uint64_t gap = P.n / kSampleTiles;
   for (uint64_t t = blockIdx.x; t < kSampleTiles; t += gridDim.x) {
     const uint64_t beg = t * gap, end = min(beg + kSampleTileRows, P.n);
-    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kCountU) {
-      uint64_t row[kCountU], key[kCountU], val[kCountU][1];
-      uint32_t vvalid[kCountU];
-      int kind[kCountU];
-      load_rows<0, kCountU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
+    for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kRowU) {
+      uint64_t row[kRowU], key[kRowU], val[kRowU][1];
+      uint32_t vvalid[kRowU];
+      int kind[kRowU];
+      load_rows<0, kRowU>(S, base + threadIdx.x, blockDim.x, end, row, key, kind, val, vvalid);
 #pragma unroll
-      for (int u = 0; u < kCountU; ++u) {
+      for (int u = 0; u < kRowU; ++u) {
         if (kind[u] != 0) continue;
         // at most kSampleTiles * kSampleTileRows keys in a table twice that size
         uint64_t idx = key_hash(key[u]) & mask;
@@ -624,13 +648,125 @@ __global__ void __launch_bounds__(256) hot_merge_kernel(const uint64_t* part, ui
   block_add(T.counters + CTR_GROUPS, claims);
 }
 
-// Device copy of the partition boundaries: bound[p] = first position of
-// partition p, bound[np] = total records.
-__global__ void part_bounds_kernel(const uint32_t* counts, const uint32_t* offs, uint32_t np, uint32_t G,
-                                   uint32_t* bound) {
+// Partition totals: tot[p] += the cold rows of kTotRows workgroups in
+// partition p (tot zeroed before; tot[np] stays 0, so the exclusive scan of
+// np + 1 entries ends with the total).
+constexpr uint32_t kTotRows = 16;
+
+__global__ void __launch_bounds__(256) part_tot_kernel(const uint32_t* hist, uint32_t G, uint32_t np, uint32_t* tot) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < np) bound[p] = offs[(uint64_t)p * G];
-  if (p == np) bound[np] = offs[(uint64_t)np * G - 1] + counts[(uint64_t)np * G - 1];
+  if (p >= np) return;
+  const uint32_t g0 = blockIdx.y * kTotRows, g1 = min(g0 + kTotRows, G);
+  uint32_t s = 0;
+  for (uint32_t g = g0; g < g1; ++g) s += hist[(uint64_t)g * np + p];
+  if (s) atomicAdd(tot + p, s);
+}
+
+// ------------------------------------------------------------ pass 2
+//
+// One workgroup per (level-1 partition p1, range of tpu pass-1 tiles). It
+// gathers p1's runs of those tiles (run positions from the tile descriptors;
+// a record's run is found by binary search over the runs' prefix sums in
+// LDS) in rounds of kL2Block * kL2U records, ranks each record within its
+// final partition (the level-2 bits below p1) with an LDS counter, reserves
+// each final partition's share of the round with one atomic on that
+// partition's fill cursor (which starts at its boundary), and writes the
+// records there. Final partition p then holds all of its records in
+// [bound[p], bound[p + 1]), each round's share of it as one contiguous run.
+constexpr int kL2Block = 512;
+constexpr int kL2U = 8;
+constexpr uint32_t kL2MaxTiles = 2 * kL2Block;  // two tile descriptors per thread
+constexpr uint32_t kMaxNp2 = 128;
+
+struct L2Params {
+  const uint64_t* in;       // pass-1 records
+  uint64_t* out;            // records by final partition
+  const uint32_t* d1_off;   // [nt1][np1 + 1] run offsets in each tile, then the tile's record count
+  const uint32_t* d1_base;  // [nt1] first position of each tile
+  uint32_t* fill;           // [np] next free position of each final partition
+  uint64_t nt1;
+  uint32_t tpu;
+  int32_t b1, pbits;
+};
+
+template <int RW>
+__global__ void __launch_bounds__(kL2Block) part_l2_kernel(L2Params P) {
+  using Scan = hipcub::BlockScan<uint32_t, kL2Block>;
+  __shared__ typename Scan::TempStorage scan_tmp;
+  __shared__ uint32_t pre[kL2MaxTiles + 1];
+  __shared__ uint32_t start[kL2MaxTiles];
+  __shared__ uint32_t cnt[kMaxNp2], gbase[kMaxNp2];
+  const uint32_t np1 = 1u << P.b1;
+  const int b2 = P.pbits - P.b1;
+  const uint32_t np2 = 1u << b2;
+  const uint32_t p1 = blockIdx.x % np1;
+  const uint64_t t0 = (uint64_t)(blockIdx.x / np1) * P.tpu;
+  const uint32_t nt = (uint32_t)min<uint64_t>(P.tpu, P.nt1 - t0);
+  uint32_t c[2], e[2], total;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t i = 2 * threadIdx.x + j;
+    c[j] = 0;
+    if (i < nt) {
+      const uint32_t* d = P.d1_off + (t0 + i) * (np1 + 1);
+      const uint32_t a = d[p1];
+      c[j] = d[p1 + 1] - a;  // d[np1] is the tile's record count
+      start[i] = P.d1_base[t0 + i] + a;
+    }
+  }
+  Scan(scan_tmp).ExclusiveSum(c, e, total);
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    if (2 * threadIdx.x + j < nt) pre[2 * threadIdx.x + j] = e[j];
+  if (threadIdx.x == 0) pre[nt] = total;
+  for (uint32_t q = threadIdx.x; q < np2; q += blockDim.x) cnt[q] = 0;
+  __syncthreads();
+  for (uint32_t r0 = 0; r0 < total; r0 += kL2Block * kL2U) {
+    uint64_t rec[kL2U][RW];
+    uint32_t q[kL2U], rank[kL2U], in = 0;
+#pragma unroll
+    for (int u = 0; u < kL2U; ++u) {
+      const uint32_t i = r0 + u * kL2Block + threadIdx.x;
+      if (i >= total) {
+#pragma unroll
+        for (int j = 0; j < RW; ++j) rec[u][j] = 0;
+        continue;
+      }
+      in |= 1u << u;
+      uint32_t lo = 0, hi = nt;  // pre[lo] <= i < pre[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= i) lo = mid;
+        else hi = mid;
+      }
+      const uint64_t* src = P.in + (uint64_t)(start[lo] + (i - pre[lo])) * RW;
+#pragma unroll
+      for (int j = 0; j < RW / 2; ++j) {
+        const ulonglong2 x = ((const ulonglong2*)src)[j];
+        rec[u][2 * j] = x.x;
+        rec[u][2 * j + 1] = x.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kL2U; ++u) {
+      q[u] = (uint32_t)(key_hash(rec[u][0]) >> (64 - P.pbits)) & (np2 - 1);
+      rank[u] = ((in >> u) & 1u) ? atomicAdd(&cnt[q[u]], 1u) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < np2; x += blockDim.x) {
+      const uint32_t k = cnt[x];
+      gbase[x] = k ? atomicAdd(P.fill + p1 * np2 + x, k) : 0u;
+      cnt[x] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kL2U; ++u) {
+      if (!((in >> u) & 1u)) continue;
+      uint64_t* dst = P.out + (uint64_t)(gbase[q[u]] + rank[u]) * RW;
+#pragma unroll
+      for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)dst)[j] = make_ulonglong2(rec[u][2 * j], rec[u][2 * j + 1]);
+    }
+  }
 }
 
 struct PartItem {
@@ -868,40 +1004,6 @@ static wg_status allow_lds(const void* kernel, size_t bytes) {
   return WG_OK;
 }
 
-using ScatterFn = void (*)(PartParams);
-static ScatterFn scatter_kernel_for(int rw) {
-  return rw == 2 ? part_scatter_kernel<2> : rw == 4 ? part_scatter_kernel<4> : part_scatter_kernel<8>;
-}
-
-// The typed pass C for this batch's shape, or null (the general kernel): the
-// typed pass A's shapes, value columns non-null and 8 bytes wide.
-static ScatterFn typed_scatter_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pred,
-                                   const std::vector<int>& vcols, int rw) {
-  if (!typed_count_for(cs, K, pred) || rw > 4) return nullptr;
-  for (int c : vcols) {
-    const ColDesc& d = cs.cols[c];
-    if (d.valid || !(d.type == WG_INT64 || d.type == WG_DECIMAL || d.type == WG_FLOAT64 || d.type == WG_IPV4))
-      return nullptr;
-  }
-  auto width = [](int t) { return (t == WG_INT32 || t == WG_UINT32) ? 4 : 8; };
-  const int w0 = width(cs.cols[K.col[0]].type), w1 = K.n > 1 ? 4 : 0;
-  const int pw = pred[0].col < 0 ? 0 : cs.cols[pred[0].col].type == WG_INT32 ? 4 : 8;
-#define WG_TS(A, B, C)                                                                     \
-  if (w0 == A && w1 == B && pw == C)                                                       \
-    return rw == 2 ? part_scatter_typed_kernel<A, B, C, 2> : part_scatter_typed_kernel<A, B, C, 4>;
-  WG_TS(4, 4, 0) WG_TS(4, 4, 4) WG_TS(4, 4, 8)
-  WG_TS(4, 0, 0) WG_TS(4, 0, 4) WG_TS(4, 0, 8)
-  WG_TS(8, 0, 0) WG_TS(8, 0, 4) WG_TS(8, 0, 8)
-#undef WG_TS
-  return nullptr;
-}
-
-static wg_status launch_scatter(ScatterFn fn, dim3 grid, size_t lds, hipStream_t st, const PartParams& P) {
-  hipLaunchKernelGGL(fn, grid, dim3(kPartBlock), lds, st, P);
-  WG_HIP(hipGetLastError());
-  return WG_OK;
-}
-
 static wg_status launch_part_agg(int rw, int grid, int block, size_t lds, hipStream_t st, const PartAggParams& Q) {
   switch (rw) {
     case 2:
@@ -951,44 +1053,46 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   const bool dense = h->groups == 0 && h->n_dense == 0 && !h->dirty;
   const int L = h->part_lds_slots;
   // partitions of at most about 3/4 L groups each: fewer, larger partitions
-  // make the scatter's writes denser, while linear probing in the LDS table
-  // still mostly ends within its probe limit
+  // mean fewer, longer runs in the sorting passes, while triangular probing
+  // in the LDS table still ends within its probe limit
   const uint64_t est = std::max<uint64_t>(std::max<uint64_t>(h->group_hint, h->groups), 1);
   int pbits = 6;
   while (pbits < 14 && (1ull << pbits) * (uint64_t)(3 * L / 4) < est) ++pbits;
   const uint32_t np = 1u << pbits;
+  const int b1 = (pbits + 1) / 2;  // level 1 <= 2^7 partitions, level 2 the rest
+  const uint32_t np1 = 1u << b1;
   // heavy hitters: batches large enough for a sample to find them
-  // (as many of the requested slots as fit the scatter pass's LDS)
+  // (as many of the requested slots as fit pass 1's LDS)
   int H = (h->part_hot > 0 && n >= (1ull << 20) && h->pg.stride <= 64) ? h->part_hot : 0;
   auto hot_bits_for = [](int hs) {
     int b = 0;
     while (hs && (1 << b) < 2 * hs) ++b;
     return b;
   };
-  auto scatter_lds = [&](int hs) {
-    return (size_t)np * 4 + (hs ? hot_lookup_bytes(hot_bits_for(hs)) + lds_table_bytes(hs, h->pg.stride) : 0);
+  auto l1_lds_bytes = [&](int hs) {
+    return (size_t)l1_hot_offset(np1, np) * 4 +
+           (hs ? hot_lookup_bytes(hot_bits_for(hs)) + lds_table_bytes(hs, h->pg.stride) : 0);
   };
-  while (H > 64 && scatter_lds(H) > 150 * 1024) H >>= 1;
-  if (scatter_lds(H) > 150 * 1024) H = 0;
+  while (H > 64 && l1_lds_bytes(H) > 150 * 1024) H >>= 1;
+  if (l1_lds_bytes(H) > 150 * 1024) H = 0;
   const int hot_bits = hot_bits_for(H);
-  const size_t hist_bytes = (size_t)np * 4 + (H ? hot_lookup_bytes(hot_bits) : 0);
-  const size_t scat_bytes = scatter_lds(H);
-  CountFn count_fn = typed_count_for(cs, h->key, pred);
-  if (!count_fn) count_fn = part_count_kernel;
-  ScatterFn scatter_fn = typed_scatter_for(cs, h->key, pred, vcols, RW);
-  if (!scatter_fn) scatter_fn = scatter_kernel_for(RW);
-  if (scat_bytes > 64 * 1024) {
-    WG_HIP(hipFuncSetAttribute((const void*)count_fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    WG_HIP(hipFuncSetAttribute((const void*)scatter_fn, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-  }
-  // one resident round of count/scatter workgroups (the two passes must use
-  // the same row ranges, so both run at the occupancy of the lower one)
-  int occ_c = 0, occ_s = 0;
-  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, count_fn, kPartBlock, hist_bytes));
-  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, scatter_fn, kPartBlock, scat_bytes));
-  const int per_cu = std::max(1, std::min(occ_c, occ_s));
-  const int G = ctx->cus * per_cu;
-  const uint64_t R = ((n + G - 1) / G + 255) / 256 * 256;
+  const size_t l1_bytes = l1_lds_bytes(H);
+  L1Fn l1_fn = typed_l1_for(cs, h->key, pred, vcols, RW);
+  const uint64_t T1 = (uint64_t)kPartBlock * (l1_fn ? kL1TU : kL1GU);  // rows per pass-1 tile
+  if (!l1_fn) l1_fn = general_l1_for(RW);
+  WG_TRY(allow_lds((const void*)l1_fn, l1_bytes));
+  // one resident round of pass-1 workgroups, each over a whole number of tiles
+  int occ = 0;
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, l1_fn, kPartBlock, l1_bytes));
+  const int G = ctx->cus * std::max(1, occ);
+  const uint64_t R = ((n + G - 1) / G + T1 - 1) / T1 * T1;
+  const uint64_t tpw = R / T1, nt1 = (uint64_t)G * tpw;
+  WG_CHECK((uint64_t)G * R < (1ull << 32), WG_ERR_UNSUPPORTED,
+           "partitioned consume of %llu rows: record positions need more than 32 bits", (unsigned long long)n);
+  // pass-2 units of tpu tiles: about two rounds of records if every row were cold
+  const uint64_t T2 = (uint64_t)kL2Block * kL2U;
+  const uint32_t tpu = (uint32_t)std::min<uint64_t>(kL2MaxTiles, std::max<uint64_t>(16, 2 * T2 * np1 / T1));
+  const uint64_t units = (nt1 + tpu - 1) / tpu;
 
   PartParams P{};
   P.cs = cs;
@@ -999,12 +1103,20 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   P.sel = sel;
   P.n = n;
   P.rows_per_wg = R;
+  P.tiles_per_wg = (uint32_t)tpw;
   P.pbits = pbits;
+  P.b1 = b1;
   P.n_vals = (int)vcols.size();
   for (size_t j = 0; j < vcols.size(); ++j) P.val_col[j] = vcols[j];
   P.pgv = pg2;
   for (size_t j = 0; j < vcols.size(); ++j)
     if (cs.cols[vcols[j]].type == WG_FLOAT64) P.vfloat |= 1u << j;
+
+  // Everything the passes need is allocated here, before pass 1 changes any
+  // state: hot-set scratch, histograms, descriptors, both record buffers,
+  // work items, overflow lists and the dense output. (A failure to grow the
+  // table while draining the items is the one allocation failure that can
+  // come later.)
   HotScratch X{};
   if (H) {
     // scratch: sample table, histogram, threshold/count, keys, partial slots
@@ -1021,22 +1133,30 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     P.hot_bits = hot_bits;
     P.hot_part = X.keys + H;
   }
-
-  // scratch: counts, offsets, boundaries (device) and boundaries (pinned host)
-  const uint64_t ncnt = (uint64_t)np * G;
-  const uint64_t cnt_words = 2 * ncnt + np + 1;
+  // u32 scratch: totals (8-byte aligned, zeroed as words), boundaries, fill
+  // cursors, histogram rows, tile descriptors
+  const uint64_t tot_words = ((uint64_t)np + 2) / 2 * 2;
+  const uint64_t o_bound = tot_words, o_fill = o_bound + np + 1, o_hist = o_fill + np;
+  const uint64_t o_d1 = o_hist + (uint64_t)G * np, o_base = o_d1 + nt1 * (np1 + 1);
+  const uint64_t cnt_words = o_base + nt1;
   if (h->part_cnt_cap < cnt_words) WG_TRY(regrow(ctx, &h->part_cnt, &h->part_cnt_cap, cnt_words, 4, "partition counts"));
   size_t scan_bytes = 0;
-  WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->part_cnt, h->part_cnt + ncnt, (int)ncnt,
-                                          ctx->stream));
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, h->part_cnt, h->part_cnt, (int)(np + 1), ctx->stream));
   if (scan_bytes > h->part_scan_bytes)
     WG_TRY(regrow(ctx, &h->part_scan, &h->part_scan_bytes, scan_bytes, 1, "partition scan scratch"));
-  uint32_t* counts = h->part_cnt;
-  uint32_t* offs = h->part_cnt + ncnt;
-  uint32_t* dbound = h->part_cnt + 2 * ncnt;
-  P.counts = counts;
-  P.offs = offs;
+  uint32_t* tot = h->part_cnt;
+  uint32_t* dbound = h->part_cnt + o_bound;
+  uint32_t* fill = h->part_cnt + o_fill;
+  P.hist = h->part_cnt + o_hist;
+  P.d1_off = h->part_cnt + o_d1;
+  P.d1_base = h->part_cnt + o_base;
   P.T = table_of(h);
+  // records: pass 1's regions (room for every row of each workgroup), then
+  // the partitioned records of pass 2 (room for every row)
+  const uint64_t need = ((uint64_t)G * R + n) * RW;
+  if (h->part_buf_words < need) WG_TRY(regrow(ctx, &h->part_buf, &h->part_buf_words, need, 8, "partition records"));
+  P.recs = h->part_buf;
+  uint64_t* recs2 = h->part_buf + (uint64_t)G * R * RW;
   // host side: boundaries, then the work items (largest possible count)
   const uint64_t chunk = h->part_chunk_rows;
   const uint64_t max_items = np + n / chunk + 1;
@@ -1044,50 +1164,13 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   if (h->part_host_bytes < host_bytes) {
     if (h->part_host) WG_HIP(hipHostFree(h->part_host));
     h->part_host = nullptr;
+    h->part_host_bytes = 0;
     WG_HIP(hipHostMalloc(&h->part_host, host_bytes, hipHostMallocDefault));
     h->part_host_bytes = host_bytes;
   }
   uint32_t* bound = (uint32_t*)h->part_host;
   PartItem* hitems = (PartItem*)((char*)h->part_host + ((np + 1) * 4 + 15) / 16 * 16);
-
-  if (!h->pev[0])
-    for (hipEvent_t& e : h->pev) WG_HIP(hipEventCreate(&e));
-  // A: count, B: scan, boundaries
-  WG_HIP(hipEventRecord(h->pev[0], ctx->stream));
-  if (H) {
-    hipLaunchKernelGGL(hot_reset_kernel, dim3(256), dim3(256), 0, ctx->stream, X);
-    hipLaunchKernelGGL(hot_sample_kernel, dim3(kSampleTiles), dim3(256), 0, ctx->stream, P, X);
-    hipLaunchKernelGGL(hot_hist_kernel, dim3(256), dim3(256), 0, ctx->stream, X);
-    hipLaunchKernelGGL(hot_threshold_kernel, dim3(1), dim3(kHotHistBins), 0, ctx->stream, X, (uint32_t)H);
-    hipLaunchKernelGGL(hot_collect_kernel, dim3(256), dim3(256), 0, ctx->stream, X, (uint32_t)H);
-    WG_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(count_fn, dim3(G), dim3(kPartBlock), hist_bytes, ctx->stream, P);
-  WG_HIP(hipGetLastError());
-  WG_HIP(hipcub::DeviceScan::ExclusiveSum(h->part_scan, scan_bytes, counts, offs, (int)ncnt, ctx->stream));
-  hipLaunchKernelGGL(part_bounds_kernel, dim3((np + 256) / 256), dim3(256), 0, ctx->stream, counts, offs, np,
-                     (uint32_t)G, dbound);
-  WG_HIP(hipGetLastError());
-  WG_HIP(hipEventRecord(h->pev[1], ctx->stream));
-  WG_HIP(hipMemcpyAsync(bound, dbound, (np + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
-  WG_HIP(hipStreamSynchronize(ctx->stream));
-  const uint64_t total = bound[np];
-  h->st_glb = total;  // rows scattered to the partitions (the cold rows)
-
-  // Everything the remaining passes need is allocated here, before the
-  // scatter changes any state: records, work items, overflow lists and the
-  // dense output. (A failure to grow the table while draining the items is
-  // the one allocation failure that can come later.)
-  const uint64_t need = std::max<uint64_t>(total, 1) * RW;
-  if (h->part_buf_words < need) WG_TRY(regrow(ctx, &h->part_buf, &h->part_buf_words, need, 8, "partition records"));
-  P.recs = h->part_buf;
-  uint64_t n_items = 0, dense_bound = (uint64_t)H;
-  for (uint32_t p = 0; p < np; ++p) {
-    const uint64_t len = bound[p + 1] - bound[p];
-    n_items += (len + chunk - 1) / chunk;
-    if (len <= chunk) dense_bound += std::min<uint64_t>(len, (uint64_t)L);
-  }
-  const uint64_t ibytes = std::max<uint64_t>(n_items, 1) * sizeof(PartItem) + 64;
+  const uint64_t ibytes = max_items * sizeof(PartItem) + 64;
   if (h->part_items_bytes < ibytes)
     WG_TRY(regrow(ctx, &h->part_items, &h->part_items_bytes, ibytes, 1, "partition work items"));
   const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
@@ -1099,27 +1182,73 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   const uint64_t need_part = (uint64_t)grid_max * L * 2;
   if (h->ovf_part_cap < need_part)
     WG_TRY(regrow(ctx, &h->ovf_part, &h->ovf_part_cap, need_part, (uint64_t)h->pg.stride * 8, "overflow partials"));
-  // dense output: at most L groups per item a workgroup owns alone, and H hot keys
-  if (dense) WG_TRY(agg_ensure_dense(h, dense_bound + kSideSlots, 0));
+  // dense output: at most L groups per partition a workgroup owns alone (and
+  // never more groups than rows), and H hot keys
+  if (dense)
+    WG_TRY(agg_ensure_dense(h, (uint64_t)H + std::min<uint64_t>(n, (uint64_t)np * L) + kSideSlots, 0));
+  if (!h->pev[0])
+    for (hipEvent_t& e : h->pev) WG_HIP(hipEventCreate(&e));
 
-  // C: scatter (side-slot rows are folded here)
+  // 1: hot set, tiles sorted by level-1 partition (side-slot rows and hot
+  // rows are folded here), partition totals and boundaries
   h->touched = true;
   if (dense) h->side_dirty = true;
   else h->dirty = true;
+  WG_HIP(hipEventRecord(h->pev[0], ctx->stream));
+  if (H) {
+    hipLaunchKernelGGL(hot_reset_kernel, dim3(256), dim3(256), 0, ctx->stream, X);
+    hipLaunchKernelGGL(hot_sample_kernel, dim3(kSampleTiles), dim3(256), 0, ctx->stream, P, X);
+    hipLaunchKernelGGL(hot_hist_kernel, dim3(256), dim3(256), 0, ctx->stream, X);
+    hipLaunchKernelGGL(hot_threshold_kernel, dim3(1), dim3(kHotHistBins), 0, ctx->stream, X, (uint32_t)H);
+    hipLaunchKernelGGL(hot_collect_kernel, dim3(256), dim3(256), 0, ctx->stream, X, (uint32_t)H);
+    WG_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(l1_fn, dim3(G), dim3(kPartBlock), l1_bytes, ctx->stream, P);
+  WG_HIP(hipGetLastError());
+  WG_TRY(zero_dev(ctx, (uint64_t*)tot, tot_words / 2));
+  hipLaunchKernelGGL(part_tot_kernel, dim3((np + 255) / 256, (G + kTotRows - 1) / kTotRows), dim3(256), 0,
+                     ctx->stream, (const uint32_t*)P.hist, (uint32_t)G, np, tot);
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipcub::DeviceScan::ExclusiveSum(h->part_scan, scan_bytes, tot, dbound, (int)(np + 1), ctx->stream));
+  WG_HIP(hipMemcpyAsync(fill, dbound, (uint64_t)np * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  WG_HIP(hipEventRecord(h->pev[1], ctx->stream));
+  WG_HIP(hipMemcpyAsync(bound, dbound, (np + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
   WG_HIP(hipEventRecord(h->pev[2], ctx->stream));
-  WG_TRY(launch_scatter(scatter_fn, dim3(G), scat_bytes, ctx->stream, P));
-  WG_HIP(hipEventRecord(h->pev[3], ctx->stream));
 
-  // D: work items (built while the scatter runs), largest first; partitions
-  // above the chunk size are split
-  n_items = 0;
+  // 2: every final partition contiguous (queued before the host waits for
+  // the boundaries, so it runs while the work items are built)
+  L2Params Q2{};
+  Q2.in = P.recs;
+  Q2.out = recs2;
+  Q2.d1_off = P.d1_off;
+  Q2.d1_base = P.d1_base;
+  Q2.fill = fill;
+  Q2.nt1 = nt1;
+  Q2.tpu = tpu;
+  Q2.b1 = b1;
+  Q2.pbits = pbits;
+  switch (RW) {
+    case 2: hipLaunchKernelGGL(part_l2_kernel<2>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
+    case 4: hipLaunchKernelGGL(part_l2_kernel<4>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
+    default: hipLaunchKernelGGL(part_l2_kernel<8>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
+  }
+  WG_HIP(hipGetLastError());
+  WG_HIP(hipEventRecord(h->pev[3], ctx->stream));
+  WG_HIP(hipEventSynchronize(h->pev[2]));
+  const uint64_t total = bound[np];
+  h->st_glb = total;  // rows written to the partitions (the cold rows)
+
+  // 3: work items, largest first; partitions above the chunk size are split
+  uint64_t n_items = 0, dense_bound = (uint64_t)H;
   for (uint32_t p = 0; p < np; ++p) {
     const uint64_t b = bound[p], e = bound[p + 1];
     if (e <= b) continue;
     const bool split = e - b > chunk;
+    if (!split) dense_bound += std::min<uint64_t>(e - b, (uint64_t)L);
     for (uint64_t x = b; x < e; x += chunk)
       hitems[n_items++] = PartItem{(uint32_t)x, (uint32_t)std::min<uint64_t>(e, x + chunk), split ? 1u : 0u, 0};
   }
+  WG_CHECK(n_items <= max_items && total <= n, WG_ERR_DEVICE, "partition boundaries out of range");
   std::stable_sort(hitems, hitems + n_items,
                    [](const PartItem& a, const PartItem& b) { return a.end - a.beg > b.end - b.beg; });
   unsigned long long* item_ctr = (unsigned long long*)((char*)h->part_items + ibytes - 64);
@@ -1128,7 +1257,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
 
   PartAggParams Q{};
   Q.pg = pg2;
-  Q.recs = P.recs;
+  Q.recs = recs2;
   Q.items = (const PartItem*)h->part_items;
   Q.n_items = (uint32_t)n_items;
   Q.lds_slots = L;
