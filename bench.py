@@ -65,8 +65,11 @@ def bench_agg(ctx, args):
     cols = W.gen_zipf_groupby(ctx, n)
     agg = HashAggregate(ctx, W.ZIPF_TYPES, W.ZIPF_KEYS, W.ZIPF_AGGS, scales=W.ZIPF_SCALES,
                         group_hint=args.group_hint)
-    if args.strategy >= 0:
-        agg.set_strategy(args.strategy)
+    if args.strategy >= 0 or args.part_lds_slots or args.part_block:
+        agg.set_strategy(max(args.strategy, 0), part_lds_slots=args.part_lds_slots or 2048,
+                         part_block=args.part_block or 1024)
+    if args.hot_slots >= 0:
+        agg.set_heavy_hitters(args.hot_slots)
 
     # the step ends with every output column on the device (keys, finalized
     # values, validity): the Next drain, without the copy to the host
@@ -607,6 +610,12 @@ def main():
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--group-hint", type=int, default=6_000_000)
     ap.add_argument("--strategy", type=int, default=-1)
+    ap.add_argument("--part-lds-slots", type=int, default=0,
+                    help="LDS table slots of the partition aggregate (default: the library's 2048)")
+    ap.add_argument("--part-block", type=int, default=0,
+                    help="workgroup size of the partition aggregate (default: the library's 1024)")
+    ap.add_argument("--hot-slots", type=int, default=-1,
+                    help="heavy-hitter slots of the partitioned consume (default: the library's 512)")
     ap.add_argument("--cpu-rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-reps", type=int, default=5)

@@ -49,7 +49,8 @@
 namespace wg {
 
 constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
-constexpr int kPartBlock = 512;  // pass-1 workgroup size
+constexpr int kL1Block = 512;       // pass-1 workgroup size, general loader
+constexpr int kL1TypedBlock = 512;  // and typed loads
 
 struct PartParams {
   ColSet cs;
@@ -157,20 +158,49 @@ __device__ __forceinline__ int hot_find(const HotLds& L, uint64_t key, uint64_t 
 // partition in an LDS histogram; pass 2's partition boundaries are the scan
 // of those counts.
 
-constexpr int kL1TU = 4;  // typed loads: tiles of kPartBlock * 4 = 2048 rows
-constexpr int kL1GU = 2;  // general loader: tiles of kPartBlock * 2 rows
+constexpr int kL1TU = 4;  // typed loads: tiles of kL1TypedBlock * 4 rows
+constexpr int kL1GU = 2;  // general loader: tiles of kL1Block * 2 rows
 
-// LDS of pass 1 (u32 words): cnt[np1], off[np1 + 1] (the offsets, then the
-// tile's base position), hist[np], then the hot-set lookup and accumulators.
+// A record is RW words: the packed key, then the value words the program
+// reads (zero words after them up to RW). Records are written and read in
+// blocks, so they are packed back to back: an odd RW is moved as 8-byte
+// words, an even one as 16-byte pairs.
+template <int RW>
+__device__ __forceinline__ void rec_store(uint64_t* p, const uint64_t (&w)[RW]) {
+  if (RW % 2 == 0) {
+#pragma unroll
+    for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)p)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) p[j] = w[j];
+  }
+}
+
+template <int RW>
+__device__ __forceinline__ void rec_load(const uint64_t* p, uint64_t (&w)[RW]) {
+  if (RW % 2 == 0) {
+#pragma unroll
+    for (int j = 0; j < RW / 2; ++j) {
+      const ulonglong2 x = ((const ulonglong2*)p)[j];
+      w[2 * j] = x.x;
+      w[2 * j + 1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < RW; ++j) w[j] = p[j];
+  }
+}
+
+// LDS of pass 1 (u32 words): cnt[3][np1] (the counters of tiles k % 3),
+// hist[np], then the hot-set lookup and accumulators.
 struct L1Lds {
   uint32_t* cnt;
-  uint32_t* off;
   uint32_t* hist;
 };
 
-__host__ __device__ inline uint32_t l1_hot_offset(uint32_t np1, uint32_t np) { return (2 * np1 + 1 + np + 1) & ~1u; }
+__host__ __device__ inline uint32_t l1_hot_offset(uint32_t np1, uint32_t np) { return (3 * np1 + np + 1) & ~1u; }
 
-__device__ __forceinline__ L1Lds l1_lds(uint32_t* lds, uint32_t np1) { return L1Lds{lds, lds + np1, lds + 2 * np1 + 1}; }
+__device__ __forceinline__ L1Lds l1_lds(uint32_t* lds, uint32_t np1) { return L1Lds{lds, lds + 3 * np1}; }
 
 // Zeroed counters and histogram, the hot accumulators at their initial values
 // and the hot-set lookup; returns the hot layout.
@@ -192,19 +222,24 @@ __device__ __forceinline__ HotLds l1_setup(const PartParams& P, uint32_t* lds) {
 
 // One tile of U loaded rows per thread: hot rows into the LDS accumulators,
 // side-slot rows into the table's side slots, cold rows ranked, scanned and
-// written as the tile's block. `run` (wave 0) is the next free position of
-// the workgroup's region. Every thread of the workgroup calls this once per
-// tile (it holds two barriers).
+// written as the tile's block. `run` is the next free position of the
+// workgroup's region (the same in every thread); k is the tile's index in the
+// workgroup, `tile` its descriptor index. Every thread of the workgroup calls
+// this once per tile: one barrier, after the ranks are taken. Each wave then
+// scans the counts itself; the counters rotate over three buffers, so the
+// buffer a tile zeroes was last read before this tile's barrier and is next
+// counted into after the next tile's barrier.
 template <int RW, int U>
-__device__ __forceinline__ void l1_tile(const PartParams& P, const HotLds& HL, const L1Lds& S, uint64_t tile,
-                                        uint32_t& run, const uint64_t (&row)[U], const uint64_t (&key)[U],
-                                        const int (&kind)[U], const uint64_t (&val)[U][RW - 1],
-                                        const uint32_t (&vvalid)[U]) {
+__device__ __forceinline__ void l1_tile(const PartParams& P, const HotLds& HL, const L1Lds& S, uint32_t k,
+                                        uint64_t tile, uint32_t& run, const uint64_t (&row)[U],
+                                        const uint64_t (&key)[U], const int (&kind)[U],
+                                        const uint64_t (&val)[U][RW - 1], const uint32_t (&vvalid)[U]) {
   constexpr int NV = RW - 1;
   const int H = P.hot_slots;
   const int stride = P.pg.stride;
   const int b2 = P.pbits - P.b1;
   const uint32_t np1 = 1u << P.b1;
+  uint32_t* const cnt = S.cnt + (k % 3) * np1;
   uint64_t hk[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) hk[u] = key_hash(key[u]);
@@ -236,41 +271,45 @@ __device__ __forceinline__ void l1_tile(const PartParams& P, const HotLds& HL, c
     }
     cold |= 1u << u;
     atomicAdd(&S.hist[part[u]], 1u);
-    rank[u] = atomicAdd(&S.cnt[part[u] >> b2], 1u);
+    rank[u] = atomicAdd(&cnt[part[u] >> b2], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    // wave 0: exclusive scan of the np1 <= 128 counts, two per lane
-    const uint32_t l = threadIdx.x, i0 = 2 * l, i1 = 2 * l + 1;
-    const uint32_t c0 = i0 < np1 ? S.cnt[i0] : 0u, c1 = i1 < np1 ? S.cnt[i1] : 0u;
-    uint32_t s = c0 + c1;
+  // exclusive scan of the np1 <= 128 counts, two per lane, in every wave
+  const uint32_t l = threadIdx.x & 63, i0 = 2 * l, i1 = 2 * l + 1;
+  const uint32_t c0 = i0 < np1 ? cnt[i0] : 0u, c1 = i1 < np1 ? cnt[i1] : 0u;
+  uint32_t s = c0 + c1;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(s, d);
-      if (l >= (uint32_t)d) s += y;
-    }
-    const uint32_t e0 = s - c0 - c1;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(s, d);
+    if (l >= (uint32_t)d) s += y;
+  }
+  const uint32_t e0 = s - c0 - c1, e1 = e0 + c0;
+  const uint32_t total = __shfl(s, 63);
+  if (threadIdx.x < 64) {
     uint32_t* desc = P.d1_off + tile * (np1 + 1);
+    uint32_t* next = S.cnt + ((k + 2) % 3) * np1;
     if (i0 < np1) {
-      S.off[i0] = e0;
-      S.cnt[i0] = 0;
       desc[i0] = e0;
+      next[i0] = 0;
     }
     if (i1 < np1) {
-      S.off[i1] = e0 + c0;
-      S.cnt[i1] = 0;
-      desc[i1] = e0 + c0;
+      desc[i1] = e1;
+      next[i1] = 0;
     }
-    const uint32_t total = __shfl(s, 63);
     if (l == 0) {
-      S.off[np1] = run;
       desc[np1] = total;
       P.d1_base[tile] = run;
     }
-    run += total;
   }
-  __syncthreads();
-  const uint64_t base = S.off[np1];
+  const uint64_t base = run;
+  run += total;
+  uint32_t off[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {  // every lane shuffles: the offset of the row's level-1 partition
+    const uint32_t p1 = part[u] >> b2;
+    const uint32_t v0 = __shfl(e0, (int)(p1 >> 1)), v1 = __shfl(e1, (int)(p1 >> 1));
+    off[u] = (p1 & 1u) ? v1 : v0;
+  }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     if (!((cold >> u) & 1u)) continue;
@@ -278,10 +317,7 @@ __device__ __forceinline__ void l1_tile(const PartParams& P, const HotLds& HL, c
     w[0] = key[u];
 #pragma unroll
     for (int j = 1; j < RW; ++j) w[j] = (j - 1 < P.n_vals) ? val[u][j - 1] : 0;
-    uint64_t* rec = P.recs + (base + S.off[part[u] >> b2] + rank[u]) * RW;
-    static_assert(RW % 2 == 0, "records are whole 16-byte pairs");
-#pragma unroll
-    for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)rec)[j] = make_ulonglong2(w[2 * j], w[2 * j + 1]);
+    rec_store<RW>(P.recs + (base + off[u] + rank[u]) * RW, w);
   }
 }
 
@@ -310,7 +346,7 @@ __device__ __forceinline__ void l1_finish(const PartParams& P, const L1Lds& S, c
 // columns the partitioned consume takes). RW = record words: key + values
 // (2, 4 or 8 words; zero pad words after the values).
 template <int RW>
-__global__ void __launch_bounds__(kPartBlock) part_l1_kernel(PartParams P) {
+__global__ void __launch_bounds__(kL1Block) part_l1_kernel(PartParams P) {
   constexpr int NV = RW - 1, U = kL1GU;
   extern __shared__ uint32_t lds1[];
   const RowSpec<NV> S = make_spec<NV>(P);
@@ -326,7 +362,7 @@ __global__ void __launch_bounds__(kPartBlock) part_l1_kernel(PartParams P) {
     int kind[U];
     load_rows<NV, U>(S, beg + (uint64_t)k * blockDim.x * U + threadIdx.x, blockDim.x, end, row, key, kind, val,
                      vvalid);
-    l1_tile<RW, U>(P, HL, T, t0 + k, run, row, key, kind, val, vvalid);
+    l1_tile<RW, U>(P, HL, T, k, t0 + k, run, row, key, kind, val, vvalid);
   }
   l1_finish(P, T, HL);
 }
@@ -365,7 +401,7 @@ __device__ __forceinline__ uint32_t pred_bits(const uint64_t (&v)[U], bool is_fl
 // also the predicate column is read once. Classifies and writes exactly what
 // part_l1_kernel does for the same rows.
 template <int W0, int W1, int PW, int RW>
-__global__ void __launch_bounds__(kPartBlock) part_l1_typed_kernel(PartParams P) {
+__global__ void __launch_bounds__(kL1TypedBlock) part_l1_typed_kernel(PartParams P) {
   constexpr int U = kL1TU, NV = RW - 1;
   extern __shared__ uint32_t lds1[];
   const HotLds HL = l1_setup(P, lds1);
@@ -431,7 +467,7 @@ __global__ void __launch_bounds__(kPartBlock) part_l1_typed_kernel(PartParams P)
       }
       kind[u] = !((pass >> u) & 1u) ? -1 : key[u] == kEmpty ? 1 : 0;
     }
-    l1_tile<RW, U>(P, HL, T, t0 + k, run, row, key, kind, val, vvalid);
+    l1_tile<RW, U>(P, HL, T, k, t0 + k, run, row, key, kind, val, vvalid);
   }
   l1_finish(P, T, HL);
 }
@@ -465,7 +501,8 @@ static L1Fn typed_l1_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pre
   }
 #define WG_TL(A, B, C)                                                                 \
   if (w[0] == A && w[1] == B && pw == C)                                               \
-    return rw == 2 ? part_l1_typed_kernel<A, B, C, 2> : part_l1_typed_kernel<A, B, C, 4>;
+    return rw == 2 ? part_l1_typed_kernel<A, B, C, 2>                                  \
+                   : rw == 3 ? part_l1_typed_kernel<A, B, C, 3> : part_l1_typed_kernel<A, B, C, 4>;
   WG_TL(4, 4, 0) WG_TL(4, 4, 4) WG_TL(4, 4, 8)
   WG_TL(4, 0, 0) WG_TL(4, 0, 4) WG_TL(4, 0, 8)
   WG_TL(8, 0, 0) WG_TL(8, 0, 4) WG_TL(8, 0, 8)
@@ -474,7 +511,7 @@ static L1Fn typed_l1_for(const ColSet& cs, const KeyDesc& K, const PredDesc* pre
 }
 
 static L1Fn general_l1_for(int rw) {
-  return rw == 2 ? part_l1_kernel<2> : rw == 4 ? part_l1_kernel<4> : part_l1_kernel<8>;
+  return rw == 2 ? part_l1_kernel<2> : rw == 3 ? part_l1_kernel<3> : rw == 4 ? part_l1_kernel<4> : part_l1_kernel<8>;
 }
 
 // Dense emission of finished groups. A partitioned consume that starts from an
@@ -739,13 +776,7 @@ __global__ void __launch_bounds__(kL2Block) part_l2_kernel(L2Params P) {
         if (pre[mid] <= i) lo = mid;
         else hi = mid;
       }
-      const uint64_t* src = P.in + (uint64_t)(start[lo] + (i - pre[lo])) * RW;
-#pragma unroll
-      for (int j = 0; j < RW / 2; ++j) {
-        const ulonglong2 x = ((const ulonglong2*)src)[j];
-        rec[u][2 * j] = x.x;
-        rec[u][2 * j + 1] = x.y;
-      }
+      rec_load<RW>(P.in + (uint64_t)(start[lo] + (i - pre[lo])) * RW, rec[u]);
     }
 #pragma unroll
     for (int u = 0; u < kL2U; ++u) {
@@ -762,9 +793,7 @@ __global__ void __launch_bounds__(kL2Block) part_l2_kernel(L2Params P) {
 #pragma unroll
     for (int u = 0; u < kL2U; ++u) {
       if (!((in >> u) & 1u)) continue;
-      uint64_t* dst = P.out + (uint64_t)(gbase[q[u]] + rank[u]) * RW;
-#pragma unroll
-      for (int j = 0; j < RW / 2; ++j) ((ulonglong2*)dst)[j] = make_ulonglong2(rec[u][2 * j], rec[u][2 * j + 1]);
+      rec_store<RW>(P.out + (uint64_t)(gbase[q[u]] + rank[u]) * RW, rec[u]);
     }
   }
 }
@@ -795,6 +824,30 @@ struct PartAggParams {
   DenseOut dense;   // rows == nullptr: merge every item into the global table
 };
 
+// Exclusive prefix sum of one u32 per thread over the workgroup (blockDim a
+// multiple of 64, at most 1024) and its total; wsum is 16 words of shared
+// memory. Every thread calls it (two barriers).
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t& total, uint32_t* wsum) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t s = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(s, d);
+    if (lane >= (uint32_t)d) s += y;
+  }
+  if (lane == 63) wsum[wave] = s;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < nw; ++w) {
+    const uint32_t x = wsum[w];
+    all += x;
+    before += w < wave ? x : 0u;
+  }
+  total = all;
+  __syncthreads();
+  return before + s - v;
+}
+
 constexpr int kFlushU = 2;
 // A partition's keys fill its LDS table to about 3/4 of L. A key that finds
 // no LDS slot aggregates in the global table, and under dense emission one
@@ -808,6 +861,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
   extern __shared__ uint64_t lds[];
   __shared__ int64_t s_item;
   __shared__ uint64_t s_init[16];
+  __shared__ uint32_t s_wsum[16];
   const int stride = P.pg.stride;
   const int L = P.lds_slots;
   const int LW = lds_word_stride(L);  // L hashed slots, then one spare slot per lane
@@ -863,13 +917,7 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
       }
 #pragma unroll
       for (int u = 0; u < kAggU; ++u) {
-        const uint64_t* src = P.recs + pos[u] * RW;
-#pragma unroll
-        for (int j = 0; j < RW / 2; ++j) {
-          const ulonglong2 x = ((const ulonglong2*)src)[j];
-          rec[u][2 * j] = x.x;
-          rec[u][2 * j + 1] = x.y;
-        }
+        rec_load<RW>(P.recs + pos[u] * RW, rec[u]);
       }
       uint64_t* ls[kAggU];
 #pragma unroll
@@ -912,24 +960,20 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(PartAggParams P) {
     __syncthreads();
     if (P.dense.rows && !split) {
       // this workgroup holds every row of the item's keys: its LDS groups are
-      // final, written to the dense output behind one reservation
-      __shared__ uint32_t s_live;
+      // final, written to the dense output behind one reservation; a block
+      // scan of the threads' live-slot counts numbers them (rows of adjacent
+      // threads are adjacent)
       __shared__ unsigned long long s_at;
-      if (threadIdx.x == 0) s_live = 0;
-      __syncthreads();
       uint32_t mine = 0;
       for (int i = threadIdx.x; i < L; i += blockDim.x) mine += lds[i] != kEmpty;
-      if (mine) atomicAdd(&s_live, mine);
+      uint32_t total;
+      const uint32_t pre = block_exclusive_sum(mine, total, s_wsum);
+      if (threadIdx.x == 0) s_at = total ? atomicAdd(P.dense.count, (unsigned long long)total) : 0ull;
       __syncthreads();
-      if (threadIdx.x == 0) {
-        s_at = s_live ? atomicAdd(P.dense.count, (unsigned long long)s_live) : 0ull;
-        s_live = 0;
-      }
-      __syncthreads();
+      uint64_t pos = s_at + pre;  // < cap: the host reserved L rows per item
       for (int i = threadIdx.x; i < L; i += blockDim.x) {
         if (lds[i] == kEmpty) continue;
-        const uint64_t pos = s_at + atomicAdd(&s_live, 1u);  // < cap: the host reserved L rows per item
-        dense_put(P.dense, pos, stride, [&](int w) { return w < stride ? lds[(uint64_t)w * LW + i] : 0ull; });
+        dense_put(P.dense, pos++, stride, [&](int w) { return w < stride ? lds[(uint64_t)w * LW + i] : 0ull; });
       }
       __syncthreads();
       continue;
@@ -1010,6 +1054,10 @@ static wg_status launch_part_agg(int rw, int grid, int block, size_t lds, hipStr
       WG_TRY(allow_lds((const void*)part_agg_kernel<2>, lds));
       hipLaunchKernelGGL(part_agg_kernel<2>, dim3(grid), dim3(block), lds, st, Q);
       break;
+    case 3:
+      WG_TRY(allow_lds((const void*)part_agg_kernel<3>, lds));
+      hipLaunchKernelGGL(part_agg_kernel<3>, dim3(grid), dim3(block), lds, st, Q);
+      break;
     case 4:
       WG_TRY(allow_lds((const void*)part_agg_kernel<4>, lds));
       hipLaunchKernelGGL(part_agg_kernel<4>, dim3(grid), dim3(block), lds, st, Q);
@@ -1044,10 +1092,9 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   WG_CHECK((int)vcols.size() <= kMaxVals, WG_ERR_UNSUPPORTED, "partitioned consume carries at most %d value columns",
            kMaxVals);
   WG_CHECK(h->key.n <= 2, WG_ERR_UNSUPPORTED, "partitioned consume takes keys of at most two columns");
-  // records of 2, 4 or 8 words: a power of two, so each record fills whole
-  // 16- or 32-byte sectors (three words padded to four: a 24-byte record
-  // straddles sectors and leaves them partly written at eviction)
-  const int RW = vcols.size() < 2 ? 2 : vcols.size() < 4 ? 4 : 8;
+  // records of 2, 3, 4 or 8 words (the key, the value words, zero words up to
+  // the record size), packed back to back in the sorted blocks
+  const int RW = vcols.size() < 2 ? 2 : vcols.size() == 2 ? 3 : vcols.size() == 3 ? 4 : 8;
   // an aggregate with no groups yet owns every key this consume sees: finished
   // groups go straight to the dense output instead of the table
   const bool dense = h->groups == 0 && h->n_dense == 0 && !h->dirty;
@@ -1078,12 +1125,13 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   const int hot_bits = hot_bits_for(H);
   const size_t l1_bytes = l1_lds_bytes(H);
   L1Fn l1_fn = typed_l1_for(cs, h->key, pred, vcols, RW);
-  const uint64_t T1 = (uint64_t)kPartBlock * (l1_fn ? kL1TU : kL1GU);  // rows per pass-1 tile
+  const int B1 = l1_fn ? kL1TypedBlock : kL1Block;
+  const uint64_t T1 = (uint64_t)B1 * (l1_fn ? kL1TU : kL1GU);  // rows per pass-1 tile
   if (!l1_fn) l1_fn = general_l1_for(RW);
   WG_TRY(allow_lds((const void*)l1_fn, l1_bytes));
   // one resident round of pass-1 workgroups, each over a whole number of tiles
   int occ = 0;
-  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, l1_fn, kPartBlock, l1_bytes));
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, l1_fn, B1, l1_bytes));
   const int G = ctx->cus * std::max(1, occ);
   const uint64_t R = ((n + G - 1) / G + T1 - 1) / T1 * T1;
   const uint64_t tpw = R / T1, nt1 = (uint64_t)G * tpw;
@@ -1203,7 +1251,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     hipLaunchKernelGGL(hot_collect_kernel, dim3(256), dim3(256), 0, ctx->stream, X, (uint32_t)H);
     WG_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(l1_fn, dim3(G), dim3(kPartBlock), l1_bytes, ctx->stream, P);
+  hipLaunchKernelGGL(l1_fn, dim3(G), dim3(B1), l1_bytes, ctx->stream, P);
   WG_HIP(hipGetLastError());
   WG_TRY(zero_dev(ctx, (uint64_t*)tot, tot_words / 2));
   hipLaunchKernelGGL(part_tot_kernel, dim3((np + 255) / 256, (G + kTotRows - 1) / kTotRows), dim3(256), 0,
@@ -1229,6 +1277,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   Q2.pbits = pbits;
   switch (RW) {
     case 2: hipLaunchKernelGGL(part_l2_kernel<2>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
+    case 3: hipLaunchKernelGGL(part_l2_kernel<3>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
     case 4: hipLaunchKernelGGL(part_l2_kernel<4>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
     default: hipLaunchKernelGGL(part_l2_kernel<8>, dim3(np1 * units), dim3(kL2Block), 0, ctx->stream, Q2); break;
   }
