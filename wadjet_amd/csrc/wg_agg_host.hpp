@@ -93,7 +93,7 @@ struct wg_agg {
   uint64_t* hot_buf = nullptr;     // hot-set selection scratch + per-workgroup partial slots
   uint64_t hot_buf_words = 0;
   float pass_ms[4] = {0, 0, 0, 0};  // direct consume, count+scan, scatter, partition aggregate
-  hipEvent_t pev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t pev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // tuning
   int32_t lds_slots = 1024;
   int32_t block = 512;

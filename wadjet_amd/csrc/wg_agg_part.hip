@@ -1335,6 +1335,10 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   std::vector<Pending>& replays = pending.v;
   uint64_t items_done = 0;
   bool first = true;
+  // With dense emission the hot keys go to the dense output and never touch
+  // the table: their merge is queued right behind the first aggregate launch,
+  // and its results are read back with that launch's counters.
+  bool hot_done = false, dense_read = false;
   for (int round = 0; round < 64; ++round) {
     const bool replay = !replays.empty();
     const uint32_t* rsel = replay ? replays.back().rows : nullptr;
@@ -1358,7 +1362,18 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     WG_HIP(hipEventRecord(h->pev[4], ctx->stream));
     if (left) WG_TRY(launch_part_agg(RW, grid, h->part_block, lds_bytes, ctx->stream, Q));
     WG_HIP(hipEventRecord(h->pev[5], ctx->stream));
+    const bool hot_now = H && dense && !hot_done;
+    if (hot_now) {
+      hipLaunchKernelGGL(hot_merge_kernel, dim3(H), dim3(256), 0, ctx->stream, (const uint64_t*)P.hot_part,
+                         (uint32_t)G, (uint32_t)H, P.n_hot, h->pg, table_of(h), h->ovf_part, D);
+      WG_HIP(hipGetLastError());
+      WG_HIP(hipEventRecord(h->pev[6], ctx->stream));
+      hot_done = true;
+    }
     WG_HIP(hipMemcpyAsync(pin + 1, item_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (dense)
+      WG_HIP(hipMemcpyAsync(pin + 2, h->counters + kCtrDense, 8, hipMemcpyDeviceToHost, ctx->stream));
+    dense_read = dense;
     uint64_t c[CTR_N];
     WG_TRY(agg_read_counters(h, c));  // synchronises the stream
     const uint64_t taken = pin[1];
@@ -1370,7 +1385,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
       h->pass_ms[2] = ms;
       first = false;
     }
-    WG_HIP(hipEventElapsedTime(&ms, h->pev[4], h->pev[5]));
+    WG_HIP(hipEventElapsedTime(&ms, h->pev[4], h->pev[hot_now ? 6 : 5]));
     h->pass_ms[3] += ms;
     h->groups = c[CTR_GROUPS];
     const uint64_t done_now = std::min<uint64_t>(taken, total_work);
@@ -1407,8 +1422,10 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     }
   }
   WG_CHECK(replays.empty() && items_done >= n_items, WG_ERR_DEVICE, "partitioned consume did not converge");
-  if (H) {
-    // E: the hot keys' per-workgroup partials into the global table
+  if (H && !hot_done) {
+    // E: the hot keys' per-workgroup partials into the global table (or the
+    // dense output when no aggregate launch ran)
+    dense_read = false;
     WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
     WG_HIP(hipEventRecord(h->pev[4], ctx->stream));
     hipLaunchKernelGGL(hot_merge_kernel, dim3(H), dim3(256), 0, ctx->stream, (const uint64_t*)P.hot_part, (uint32_t)G,
@@ -1427,8 +1444,10 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     }
   }
   if (dense) {
-    WG_HIP(hipMemcpyAsync(h->pinned + CTR_N + 2, h->counters + kCtrDense, 8, hipMemcpyDeviceToHost, ctx->stream));
-    WG_HIP(hipStreamSynchronize(ctx->stream));
+    if (!dense_read) {  // the last launch did not read the dense count back
+      WG_HIP(hipMemcpyAsync(h->pinned + CTR_N + 2, h->counters + kCtrDense, 8, hipMemcpyDeviceToHost, ctx->stream));
+      WG_HIP(hipStreamSynchronize(ctx->stream));
+    }
     h->n_dense = h->pinned[CTR_N + 2];
     WG_CHECK(h->n_dense <= D.cap, WG_ERR_DEVICE, "dense output overran its reservation");
     if (h->groups) h->dirty = true;  // split items, LDS misses or replays used the table
