@@ -374,3 +374,50 @@ def test_fetch_to_device_columns_equals_host_fetch(ctx, case):
     np.testing.assert_array_equal(vs[0].to_host(host["values"][0].dtype, cnt).view(np.uint64),
                                   host["values"][0][off:off + cnt].view(np.uint64))
     agg.close()
+
+
+@pytest.mark.parametrize("n_vals", [1, 3, 5])
+@pytest.mark.parametrize("null_key", [False, True])
+def test_partitioned_record_widths(ctx, n_vals, null_key):
+    """The partitioned consume with records of 2, 4 and 8 words (one, three
+    and five value columns) through the typed pass 1 (non-null 8-byte
+    columns) and the general one (a nullable key, an int32 value column),
+    with a skewed key so the heavy-hitter set takes part, against the oracle."""
+    rng = np.random.default_rng(300 + n_vals + 10 * null_key)
+    # five value columns: a batch large enough for the automatic choice of
+    # the partitioned consume, whose default 2048-slot LDS table the wide
+    # slots (128 bytes) halve to fit
+    n = 1_300_000 if n_vals < 5 else (1 << 22) + 100_000
+    k = (rng.pareto(1.2, n) * 1000).astype(np.int64) % 250_000
+    kvalid = rng.random(n) > 0.02
+    v = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    f = rng.standard_normal(n)
+    d = rng.integers(-10**13, 10**13, n).astype(np.int64)
+    i32 = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    ip = rng.integers(0, 2**32, n).astype(np.int64)
+    arrays = [k, v, f, d, i32, ip]
+    types = [L.INT64, L.INT64, L.FLOAT64, L.DECIMAL, L.INT32, L.IPV4]
+    scales = [0, 0, 0, 2, 0, 0]
+    aggs = {1: [AggSpec(L.SUM, 1), AggSpec(L.COUNT, -1)],
+            3: [AggSpec(L.SUM, 1), AggSpec(L.MIN, 2), AggSpec(L.SUM, 3), AggSpec(L.AVG, 1), AggSpec(L.COUNT, -1)],
+            5: [AggSpec(L.SUM, 1), AggSpec(L.MAX, 2), AggSpec(L.SUM, 3), AggSpec(L.MIN, 4), AggSpec(L.MAX, 5),
+                AggSpec(L.AVG, 2)]}[n_vals]
+    cols = [Col(ctx.to_device(k), L.INT64, ctx.to_device(bitmap_from_mask(kvalid)) if null_key else None)]
+    cols += [Col(ctx.to_device(a), t, None, s) for a, t, s in zip(arrays[1:], types[1:], scales[1:])]
+    print(f"[agg] partitioned record widths n_vals={n_vals} null_key={null_key}", flush=True)
+    agg = HashAggregate(ctx, types, [0], aggs, scales=scales, nullable=[null_key] + [False] * 5, group_hint=250_000)
+    if n_vals < 5:
+        agg.set_strategy(2, 2048, 1024, 1 << 15)
+    # with three value columns the rows arrive through a selection vector
+    hsel = np.flatnonzero(rng.random(n) < 0.7).astype(np.uint32) if n_vals == 3 else None
+    if hsel is None:
+        agg.consume(cols, n)
+    else:
+        agg.consume(cols, n, sel=ctx.to_device(hsel))
+    agg.finalize()
+    assert agg.stats()["global_rows"] > 0  # rows went through the partitions
+    hcols = [(k, L.INT64, bitmap_from_mask(kvalid) if null_key else None, 0)]
+    hcols += [(a, t, None, s) for a, t, s in zip(arrays[1:], types[1:], scales[1:])]
+    ref = oracle_groups(run_oracle(hcols, [0], aggs, sel=hsel))
+    compare(gpu_groups(agg), ref, [(a.func, a.col) for a in aggs], types, scales,
+            abs_sums(hcols, [0], aggs, sel=hsel))

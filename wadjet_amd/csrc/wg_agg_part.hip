@@ -1098,7 +1098,12 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   // an aggregate with no groups yet owns every key this consume sees: finished
   // groups go straight to the dense output instead of the table
   const bool dense = h->groups == 0 && h->n_dense == 0 && !h->dirty;
-  const int L = h->part_lds_slots;
+  // the partition table in LDS: the configured slots, halved until a table of
+  // this program's slot stride fits
+  int L = h->part_lds_slots;
+  while (L > 64 && lds_table_bytes(L, h->pg.stride) > 150 * 1024) L >>= 1;
+  WG_CHECK(lds_table_bytes(L, h->pg.stride) <= 150 * 1024, WG_ERR_UNSUPPORTED,
+           "a partition table of this aggregate's slot stride does not fit in LDS");
   // partitions of at most about 3/4 L groups each: fewer, larger partitions
   // mean fewer, longer runs in the sorting passes, while triangular probing
   // in the LDS table still ends within its probe limit
