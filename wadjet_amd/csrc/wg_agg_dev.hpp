@@ -975,7 +975,12 @@ __device__ __forceinline__ void apply_vals_lds_batch(uint64_t* const (&w)[U], in
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
-          add_limbs2<LdsT>(w[u] + op.word * ws, ws, ok ? (int64_t)pick(v[u], c1) : 0);
+          const int64_t x = ok ? (int64_t)pick(v[u], c1) : 0;
+          uint64_t* t = w[u] + op.word * ws;
+          at_add<LdsT>(t, (uint64_t)x & 0xFFFFFFFFull);
+          // the signed high limb is 0 for every value in [0, 2^32): a wave
+          // whose values all are skips its atomic (adding 0 changes nothing)
+          if (__any((x >> 32) != 0)) at_add<LdsT>(t + ws, (uint64_t)(x >> 32));
         }
         break;
       case OP_SUMD4:
