@@ -708,8 +708,16 @@ static wg_status compile_program(wg_agg* h) {
   WG_CHECK((int)ops.size() <= kMaxOps, WG_ERR_UNSUPPORTED, "too many accumulator ops (%d)", (int)ops.size());
   WG_CHECK(next <= 16, WG_ERR_UNSUPPORTED, "group state of %d words exceeds a 128-byte slot", next);
   h->n_words = next;
+  // ops sorted by kind (ROWCOUNT, the smallest kind, stays first): the device
+  // updates run one loop per kind over the kind's run of ops
+  std::stable_sort(ops.begin(), ops.end(), [](const OpDesc& a, const OpDesc& b) { return a.kind < b.kind; });
   h->pg.n_ops = (int)ops.size();
   for (size_t i = 0; i < ops.size(); ++i) h->pg.ops[i] = ops[i];
+  for (int k = 0; k <= OP_MAX; ++k) {
+    int e = 0;
+    for (const OpDesc& op : ops) e += op.kind <= k;
+    h->pg.kind_end[k] = e;
+  }
   h->pg.stride = (int)next_pow2((uint64_t)next);
   h->n_sumd = (int)h->sumd_word.size();
   return WG_OK;

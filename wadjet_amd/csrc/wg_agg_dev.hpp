@@ -17,6 +17,8 @@ struct AggProgram {
   int32_t n_ops;
   int32_t stride;  // words per slot (power of two), word 0 = key
   OpDesc ops[kMaxOps];
+  // ops are sorted by kind: those of kind k are ops[kind_end[k - 1], kind_end[k])
+  int32_t kind_end[OP_MAX + 1];
 };
 
 // Columns a row's values come from. For a plain aggregate every column is a
@@ -937,88 +939,96 @@ template <int NV, int U>
 __device__ __forceinline__ void apply_vals_lds_batch(uint64_t* const (&w)[U], int ws, const AggProgram& pg,
                                                      const uint64_t (&v)[U][NV], uint32_t vfloat,
                                                      const uint32_t (&vvalid)[U], uint32_t live) {
-  for (int o = 0; o < pg.n_ops; ++o) {
+  // one loop per op kind over that kind's run of the (kind-sorted) program:
+  // the kind is known at compile time in each loop body, so the updates of
+  // one kind are straight-line code with no per-op dispatch
+  int o = 0;
+  for (; o < pg.kind_end[OP_ROWCOUNT]; ++o) {
+    const int word = pg.ops[o].word;
+#pragma unroll
+    for (int u = 0; u < U; ++u) at_add<LdsT>(w[u] + word * ws, (live >> u) & 1u);
+  }
+  for (; o < pg.kind_end[OP_NNCOUNT]; ++o) {
     const OpDesc op = pg.ops[o];
-    const int c1 = op.col, c2 = op.col2;
-    const bool f1 = c1 >= 0 && ((vfloat >> c1) & 1u);
-    switch (op.kind) {
-      case OP_ROWCOUNT:
 #pragma unroll
-        for (int u = 0; u < U; ++u) at_add<LdsT>(w[u] + op.word * ws, (live >> u) & 1u);
-        break;
-      case OP_NNCOUNT:
+    for (int u = 0; u < U; ++u) {
+      const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u) &&
+                      (op.col2 < 0 || ((vvalid[u] >> op.col2) & 1u));
+      at_add<LdsT>(w[u] + op.word * ws, ok ? 1 : 0);
+    }
+  }
+  for (; o < pg.kind_end[OP_SUMI]; ++o) {
+    const OpDesc op = pg.ops[o];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u) && (c2 < 0 || ((vvalid[u] >> c2) & 1u));
-          at_add<LdsT>(w[u] + op.word * ws, ok ? 1 : 0);
-        }
-        break;
-      case OP_SUMI:
+    for (int u = 0; u < U; ++u) {
+      const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u) &&
+                      (op.col2 < 0 || ((vvalid[u] >> op.col2) & 1u));
+      uint64_t x = pick(v[u], op.col);
+      if (op.col2 >= 0) x *= pick(v[u], op.col2);
+      at_add<LdsT>(w[u] + op.word * ws, ok ? x : 0);
+    }
+  }
+  for (; o < pg.kind_end[OP_SUMF]; ++o) {
+    const OpDesc op = pg.ops[o];
+    const bool f1 = (vfloat >> op.col) & 1u;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u) && (c2 < 0 || ((vvalid[u] >> c2) & 1u));
-          uint64_t x = pick(v[u], c1);
-          if (c2 >= 0) x *= pick(v[u], c2);
-          at_add<LdsT>(w[u] + op.word * ws, ok ? x : 0);
-        }
-        break;
-      case OP_SUMF:
+    for (int u = 0; u < U; ++u) {
+      const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
+      const uint64_t b = pick(v[u], op.col);
+      const double x = f1 ? __longlong_as_double((long long)b) : (double)(int64_t)b;
+      at_addf<LdsT>(w[u] + op.word * ws, ok ? x : -0.0);
+    }
+  }
+  for (; o < pg.kind_end[OP_SUMD]; ++o) {
+    const OpDesc op = pg.ops[o];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
-          const uint64_t b = pick(v[u], c1);
-          const double x = f1 ? __longlong_as_double((long long)b) : (double)(int64_t)b;
-          at_addf<LdsT>(w[u] + op.word * ws, ok ? x : -0.0);
-        }
-        break;
-      case OP_SUMD:
+    for (int u = 0; u < U; ++u) {
+      const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
+      const int64_t x = ok ? (int64_t)pick(v[u], op.col) : 0;
+      uint64_t* t = w[u] + op.word * ws;
+      at_add<LdsT>(t, (uint64_t)x & 0xFFFFFFFFull);
+      // the signed high limb is 0 for every value in [0, 2^32): a wave
+      // whose values all are skips its atomic (adding 0 changes nothing)
+      if (__any((x >> 32) != 0)) at_add<LdsT>(t + ws, (uint64_t)(x >> 32));
+    }
+  }
+  for (; o < pg.kind_end[OP_SUMD4]; ++o) {
+    const OpDesc op = pg.ops[o];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
-          const int64_t x = ok ? (int64_t)pick(v[u], c1) : 0;
-          uint64_t* t = w[u] + op.word * ws;
-          at_add<LdsT>(t, (uint64_t)x & 0xFFFFFFFFull);
-          // the signed high limb is 0 for every value in [0, 2^32): a wave
-          // whose values all are skips its atomic (adding 0 changes nothing)
-          if (__any((x >> 32) != 0)) at_add<LdsT>(t + ws, (uint64_t)(x >> 32));
-        }
-        break;
-      case OP_SUMD4:
+    for (int u = 0; u < U; ++u) {
+      const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u) &&
+                      (op.col2 < 0 || ((vvalid[u] >> op.col2) & 1u));
+      const int64_t a = (int64_t)pick(v[u], op.col);
+      uint64_t lo;
+      int64_t hi;
+      if (op.col2 >= 0) {
+        const int64_t b = (int64_t)pick(v[u], op.col2);
+        lo = (uint64_t)a * (uint64_t)b;
+        hi = (int64_t)__mul64hi((long long)a, (long long)b);
+      } else {
+        lo = (uint64_t)a;
+        hi = a < 0 ? -1 : 0;
+      }
+      add_limbs4<LdsT>(w[u] + op.word * ws, ws, ok ? lo : 0, ok ? hi : 0);
+    }
+  }
+  for (; o < pg.kind_end[OP_MAX]; ++o) {  // MIN then MAX
+    const OpDesc op = pg.ops[o];
+    const bool f1 = (vfloat >> op.col) & 1u;
+    const bool is_min = op.kind == OP_MIN;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u) && (c2 < 0 || ((vvalid[u] >> c2) & 1u));
-          const int64_t a = (int64_t)pick(v[u], c1);
-          uint64_t lo;
-          int64_t hi;
-          if (c2 >= 0) {
-            const int64_t b = (int64_t)pick(v[u], c2);
-            lo = (uint64_t)a * (uint64_t)b;
-            hi = (int64_t)__mul64hi((long long)a, (long long)b);
-          } else {
-            lo = (uint64_t)a;
-            hi = a < 0 ? -1 : 0;
-          }
-          add_limbs4<LdsT>(w[u] + op.word * ws, ws, ok ? lo : 0, ok ? hi : 0);
-        }
-        break;
-      case OP_MIN: case OP_MAX:
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint64_t b = pick(v[u], c1);
-          bool ok = ((live >> u) & 1u) && ((vvalid[u] >> c1) & 1u);
-          uint64_t e;
-          if (f1) {
-            ok = ok && (b & 0x7FFFFFFFFFFFFFFFull) <= 0x7FF0000000000000ull;  // NaN never wins
-            e = enc_f64_bits(b);
-          } else {
-            e = enc_i64((int64_t)b);
-          }
-          if (op.kind == OP_MIN) at_min<LdsT>(w[u] + op.word * ws, ok ? e : kMinInit);
-          else at_max<LdsT>(w[u] + op.word * ws, ok ? e : kMaxInit);
-        }
-        break;
-      default:
-        break;
+    for (int u = 0; u < U; ++u) {
+      const uint64_t b = pick(v[u], op.col);
+      bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
+      uint64_t e;
+      if (f1) {
+        ok = ok && (b & 0x7FFFFFFFFFFFFFFFull) <= 0x7FF0000000000000ull;  // NaN never wins
+        e = enc_f64_bits(b);
+      } else {
+        e = enc_i64((int64_t)b);
+      }
+      if (is_min) at_min<LdsT>(w[u] + op.word * ws, ok ? e : kMinInit);
+      else at_max<LdsT>(w[u] + op.word * ws, ok ? e : kMaxInit);
     }
   }
 }
