@@ -565,10 +565,14 @@ def bench_served(ctx, args):
     plain = Server(ctx, cols, n, 3, plain=True)
     served.add_dim(j)
     plain.add_dim(j)
-    # warm both (code objects, graphs, staging buffers) before measuring
-    wq, _ = W.plan_flow_requests(n, 64, seed=7)
+    # warm both at the top offered rate with that point's request count (code
+    # objects, graphs, staging buffers and the pool of result buffers that a
+    # server under that load keeps), unmeasured
+    top = max(args.qps)
+    wq, _ = W.plan_flow_requests(n, max(50, min(args.serve_requests, int(top * args.serve_seconds))), seed=7)
     for s in (served, plain):
-        s.replay(wq, qps=200, seed=3)
+        s.replay(wq, qps=top, seed=3)
+        s.reset()
     out = []
     for qps in args.qps:
         n_req = max(50, min(args.serve_requests, int(qps * args.serve_seconds)))

@@ -443,6 +443,8 @@ static void arena_free(BatchArena& A) {
   A = BatchArena{};
 }
 
+constexpr uint64_t kLdsItemRows = 32768;
+
 static uint64_t table_cap_for(uint64_t est) { return next_pow2(std::max<uint64_t>(std::max<uint64_t>(est, 8) * 2, 64)); }
 
 static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t n, const BatchArena& A, Plan& P) {
@@ -524,9 +526,14 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
   P.work.clear();
   for (int i : order) {
     const uint64_t r0 = qs[i].row_begin, r1 = qs[i].row_begin + qs[i].n_rows;
-    for (uint64_t r = r0; r < r1; r += tile)
+    // a query aggregated in an LDS table (no dimension lookup) flushes each
+    // item's table into its global table with device atomics on the same
+    // few slots: its items take at least kLdsItemRows rows, so the flushes
+    // stay few even when the batch is too small to fill the device
+    const uint64_t ti = P.use_lds[i] == 1 && !qs[i].dim ? std::max<uint64_t>(tile, kLdsItemRows) : tile;
+    for (uint64_t r = r0; r < r1; r += ti)
       for (uint32_t f = 0; f < P.fanout[i]; ++f)  // a tile's partitions back to back: its rows stay in cache
-        P.work.push_back(BWork{(uint32_t)i, f, r, std::min<uint64_t>(r1, r + tile)});
+        P.work.push_back(BWork{(uint32_t)i, f, r, std::min<uint64_t>(r1, r + ti)});
   }
   P.cwork.clear();
   for (int i = 0; i < n; ++i) {

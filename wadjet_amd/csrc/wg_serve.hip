@@ -15,6 +15,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <random>
 #include <thread>
@@ -43,8 +44,8 @@ struct Request {
   Clock::time_point arrival, done_at;
   bool done = false;
   wg_status status = WG_OK;
-  std::vector<int64_t> keys, hi, cnt;
-  std::vector<uint64_t> lo;
+  uint64_t n_groups = 0;
+  std::vector<uint64_t> res;  // keys, sum low words, sum high words, counts: n_groups words each
 };
 
 // Results of a finished batch on their way to the host: one packed copy into
@@ -106,11 +107,35 @@ struct wg_serve {
   wg::Clock::time_point first_arrival{}, last_done{};
   bool any = false;
   wg_status engine_error = WG_OK;
+  // result buffers of released requests, by capacity in words: the engine
+  // fills reused (already faulted-in) memory instead of fresh pages
+  std::multimap<uint64_t, std::vector<uint64_t>> pool;
+  uint64_t pool_words = 0;
 };
 
 namespace wg {
 
 constexpr uint64_t kIndexStride = 4096;
+constexpr uint64_t kPoolMaxWords = (8ull << 30) / 8;  // at most 8 GiB of pooled result buffers
+
+// A pooled result buffer of at least `words` words and at most about four
+// times that (empty if none): the caller holds s->mu.
+static std::vector<uint64_t> take_buffer(wg_serve* s, uint64_t words) {
+  auto it = s->pool.lower_bound(words);
+  if (it == s->pool.end() || it->first > 4 * words + 16384) return {};
+  std::vector<uint64_t> v = std::move(it->second);
+  s->pool_words -= it->first;
+  s->pool.erase(it);
+  return v;
+}
+
+// Return a released request's result buffer to the pool: the caller holds s->mu.
+static void give_buffer(wg_serve* s, std::vector<uint64_t>&& v) {
+  const uint64_t c = v.capacity();
+  if (c == 0 || s->pool_words + c > kPoolMaxWords) return;
+  s->pool_words += c;
+  s->pool.emplace(c, std::move(v));
+}
 
 // Row range [r0, r1) that covers ts window [lo, hi) at index granularity; the
 // per-row window filter in the batch kernel makes it exact.
@@ -159,15 +184,19 @@ static void finish_download(wg_serve* s, Lane& L) {
   Download d = std::move(L.downloads.front());
   L.downloads.pop_front();
   const uint64_t* h = L.host[d.buf];
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t i = 0; i < d.reqs.size(); ++i)
+      if (d.reqs[i] && d.n[i]) d.reqs[i]->res = take_buffer(s, 4 * d.n[i]);
+  }
   for (size_t i = 0; i < d.reqs.size(); ++i) {
     Request* r = d.reqs[i];
     const uint64_t n = d.n[i];
     if (r == nullptr) continue;  // completed already (re-run alone)
+    // the download packs keys, sum words and counts of a request back to back
     const uint64_t* p = h + d.off[i];
-    r->keys.assign((const int64_t*)p, (const int64_t*)p + n);
-    r->lo.assign(p + n, p + 2 * n);
-    r->hi.assign((const int64_t*)(p + 2 * n), (const int64_t*)(p + 3 * n));
-    r->cnt.assign((const int64_t*)(p + 3 * n), (const int64_t*)(p + 4 * n));
+    r->res.assign(p, p + 4 * n);
+    r->n_groups = n;
   }
   L.free_events.push_back(d.ev);
   std::lock_guard<std::mutex> lk(s->mu);
@@ -203,11 +232,15 @@ static void retire_lane(wg_serve* s, Lane& L, wg_status st) {
       const bool alone = wg_batch_rerun_alone(L.batch, (int32_t)i) != 0;
       if (e == WG_OK && n && alone) {
         Request* r = L.reqs[i];
-        r->keys.resize(n);
-        r->lo.resize(n);
-        r->hi.resize(n);
-        r->cnt.resize(n);
-        e = wg_batch_result(L.batch, (int32_t)i, &n, r->keys.data(), r->lo.data(), r->hi.data(), r->cnt.data(), n);
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          r->res = take_buffer(s, 4 * n);
+        }
+        r->res.resize(4 * n);
+        uint64_t* q = r->res.data();
+        e = wg_batch_result(L.batch, (int32_t)i, &n, (int64_t*)q, q + n, (int64_t*)(q + 2 * n), (int64_t*)(q + 3 * n),
+                            n);
+        r->n_groups = n;
       }
       if (e != WG_OK || alone) {
         L.reqs[i]->status = e;
@@ -494,7 +527,7 @@ wg_status wg_serve_wait(wg_serve* s, uint64_t id, uint64_t* n_groups, double* la
     set_error("request %llu failed with status %d", (unsigned long long)id, r->status);
     return r->status;
   }
-  if (n_groups) *n_groups = r->keys.size();
+  if (n_groups) *n_groups = r->n_groups;
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->done_at - r->arrival).count();
   return WG_OK;
 }
@@ -507,12 +540,13 @@ wg_status wg_serve_result(wg_serve* s, uint64_t id, int64_t* keys, uint64_t* sum
   auto it = s->reqs.find(id);
   WG_CHECK(it != s->reqs.end() && it->second->done, WG_ERR_STATE, "request %llu not completed", (unsigned long long)id);
   Request* r = it->second;
-  const uint64_t n = r->keys.size();
+  const uint64_t n = r->n_groups;
   WG_CHECK(cap >= n, WG_ERR_INVALID, "result has %llu groups, capacity %llu", (unsigned long long)n, (unsigned long long)cap);
-  if (keys) memcpy(keys, r->keys.data(), n * 8);
-  if (sum_lo) memcpy(sum_lo, r->lo.data(), n * 8);
-  if (sum_hi) memcpy(sum_hi, r->hi.data(), n * 8);
-  if (counts) memcpy(counts, r->cnt.data(), n * 8);
+  const uint64_t* q = r->res.data();
+  if (keys) memcpy(keys, q, n * 8);
+  if (sum_lo) memcpy(sum_lo, q + n, n * 8);
+  if (sum_hi) memcpy(sum_hi, q + 2 * n, n * 8);
+  if (counts) memcpy(counts, q + 3 * n, n * 8);
   return WG_OK;
 }
 
@@ -602,7 +636,10 @@ wg_status wg_serve_reset(wg_serve* s) {
       if (!kv.second->done) return false;
     return true;
   });
-  for (auto& kv : s->reqs) delete kv.second;
+  for (auto& kv : s->reqs) {
+    give_buffer(s, std::move(kv.second->res));
+    delete kv.second;
+  }
   s->reqs.clear();
   s->batches = 0;
   s->max_in_flight = 0;
