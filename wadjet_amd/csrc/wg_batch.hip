@@ -332,12 +332,14 @@ namespace wg {
 struct BatchArena {
   uint64_t slots = 0;  // table slots the arena holds
   uint64_t* arena = nullptr;
-  BQ* d_qs = nullptr;
-  uint64_t q_cap = 0;
-  BWork* d_work = nullptr;
-  CWork* d_cwork = nullptr;
-  uint64_t work_cap = 0, cwork_cap = 0;
+  // the descriptors, one device block uploaded with one copy from a staging
+  // area of the same layout: [scalars: 64 bytes][queries][compact items][work items]
+  void* d_desc = nullptr;
+  uint64_t q_cap = 0, cwork_cap = 0, work_cap = 0;
   uint64_t* d_scalars = nullptr;  // [0] n_work, [1] n_cwork, [2] arena words, [3] n_q
+  BQ* d_qs = nullptr;
+  CWork* d_cwork = nullptr;
+  BWork* d_work = nullptr;
   unsigned long long* d_work_ctrs = nullptr;
   BCounters* d_ctr = nullptr;
   int64_t* d_keys = nullptr;
@@ -401,11 +403,31 @@ struct wg_batch {
 
 namespace wg {
 
-static wg_status ensure_dev(wg_ctx* ctx, void** p, uint64_t& cap, uint64_t need, uint64_t elem, uint64_t& gen) {
-  if (need <= cap && *p) return WG_OK;
-  const uint64_t want = std::max<uint64_t>(need, cap * 2);
-  WG_TRY(regrow(ctx, p, &cap, want, elem, "batch descriptors"));
-  ++gen;  // invalidates any graph that captured the old pointer
+static_assert(alignof(BQ) <= 8 && alignof(CWork) <= 8 && alignof(BWork) <= 8 && sizeof(BQ) % 8 == 0 &&
+                  sizeof(CWork) % 8 == 0,
+              "descriptor block layout");
+static uint64_t desc_off_cwork(const BatchArena& A) { return 64 + A.q_cap * sizeof(BQ); }
+static uint64_t desc_off_work(const BatchArena& A) { return desc_off_cwork(A) + A.cwork_cap * sizeof(CWork); }
+
+// A descriptor block for nq queries, nw work items and nc compact items.
+static wg_status ensure_desc(wg_ctx* ctx, BatchArena& A, uint64_t nq, uint64_t nw, uint64_t nc) {
+  if (A.d_desc && nq <= A.q_cap && nw <= A.work_cap && nc <= A.cwork_cap) return WG_OK;
+  BatchArena G = A;  // the capacities that grow double at least
+  if (nq > G.q_cap) G.q_cap = std::max<uint64_t>(nq, 2 * G.q_cap);
+  if (nw > G.work_cap) G.work_cap = std::max<uint64_t>(nw, 2 * G.work_cap);
+  if (nc > G.cwork_cap) G.cwork_cap = std::max<uint64_t>(nc, 2 * G.cwork_cap);
+  void* fresh = nullptr;
+  WG_TRY(dev_alloc(ctx, &fresh, desc_off_work(G) + G.work_cap * sizeof(BWork), nullptr, "batch descriptors"));
+  if (A.d_desc) (void)hipFree(A.d_desc);
+  A.q_cap = G.q_cap;
+  A.work_cap = G.work_cap;
+  A.cwork_cap = G.cwork_cap;
+  A.d_desc = fresh;
+  A.d_scalars = (uint64_t*)fresh;
+  A.d_qs = (BQ*)((char*)fresh + 64);
+  A.d_cwork = (CWork*)((char*)fresh + desc_off_cwork(A));
+  A.d_work = (BWork*)((char*)fresh + desc_off_work(A));
+  ++A.gen;  // invalidates any graph that captured the old pointers
   return WG_OK;
 }
 
@@ -413,13 +435,12 @@ static void arena_free(BatchArena& A);
 
 static wg_status arena_alloc(wg_ctx* ctx, BatchArena& A, uint64_t slots, uint64_t max_q) {
   WG_ALLOC(ctx, &A.arena, (slots + 1) * kBStride * 8, "the batch arena");
-  WG_ALLOC(ctx, &A.d_scalars, 64, "batch counters");
   WG_ALLOC(ctx, &A.d_work_ctrs, 64, "batch counters");
   WG_ALLOC(ctx, &A.d_keys, slots * 8, "batch outputs");
   WG_ALLOC(ctx, &A.d_sum_lo, slots * 8, "batch outputs");
   WG_ALLOC(ctx, &A.d_sum_hi, slots * 8, "batch outputs");
   WG_ALLOC(ctx, &A.d_counts, slots * 8, "batch outputs");
-  WG_TRY(ensure_dev(ctx, (void**)&A.d_qs, A.q_cap, max_q, sizeof(BQ), A.gen));
+  WG_TRY(ensure_desc(ctx, A, max_q, 1, 1));
   WG_ALLOC(ctx, &A.d_ctr, max_q * sizeof(BCounters), "batch counters");
   A.slots = slots;
   return WG_OK;
@@ -434,7 +455,7 @@ static wg_status arena_init(wg_ctx* ctx, BatchArena& A, uint64_t slots, uint64_t
 
 static void arena_free(BatchArena& A) {
   if (A.graph) (void)hipGraphExecDestroy(A.graph);
-  void* ps[] = {A.arena, A.d_qs, A.d_work, A.d_cwork, A.d_scalars, A.d_work_ctrs, A.d_ctr,
+  void* ps[] = {A.arena, A.d_desc, A.d_work_ctrs, A.d_ctr,
                 A.d_keys, A.d_sum_lo, A.d_sum_hi, A.d_counts, A.d_pack_items, A.d_packed};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -546,11 +567,10 @@ static wg_status plan_into(const wg_batch* b, const wg_batch_query* qs, int32_t 
 
 static wg_status upload(wg_ctx* ctx, BatchArena& A, const Plan& P) {
   const int n = (int)P.qs.size();
-  WG_TRY(ensure_dev(ctx, (void**)&A.d_qs, A.q_cap, std::max(n, 1), sizeof(BQ), A.gen));
-  WG_TRY(ensure_dev(ctx, (void**)&A.d_work, A.work_cap, std::max<size_t>(P.work.size(), 1), sizeof(BWork), A.gen));
-  WG_TRY(ensure_dev(ctx, (void**)&A.d_cwork, A.cwork_cap, std::max<size_t>(P.cwork.size(), 1), sizeof(CWork), A.gen));
-  const uint64_t bytes = n * sizeof(BQ) + P.work.size() * sizeof(BWork) + P.cwork.size() * sizeof(CWork) + 64 +
-                         n * sizeof(BCounters);
+  WG_TRY(ensure_desc(ctx, A, std::max(n, 1), std::max<size_t>(P.work.size(), 1), std::max<size_t>(P.cwork.size(), 1)));
+  // staging: the descriptor block's layout up to the last work item, then the counters' read-back
+  const uint64_t desc_bytes = desc_off_work(A) + P.work.size() * sizeof(BWork);
+  const uint64_t bytes = desc_bytes + 64 + n * sizeof(BCounters);
   if (bytes > A.pinned_bytes) {
     if (A.pinned) WG_HIP(hipHostFree(A.pinned));
     A.pinned = nullptr;
@@ -562,24 +582,17 @@ static wg_status upload(wg_ctx* ctx, BatchArena& A, const Plan& P) {
   }
   // the previous use of this arena has completed, so its staging is free
   char* p = (char*)A.pinned;
-  memcpy(p, P.qs.data(), n * sizeof(BQ));
-  WG_HIP(hipMemcpyAsync(A.d_qs, p, n * sizeof(BQ), hipMemcpyHostToDevice, ctx->stream));
-  p += n * sizeof(BQ);
-  memcpy(p, P.work.data(), P.work.size() * sizeof(BWork));
-  WG_HIP(hipMemcpyAsync(A.d_work, p, P.work.size() * sizeof(BWork), hipMemcpyHostToDevice, ctx->stream));
-  p += P.work.size() * sizeof(BWork);
-  memcpy(p, P.cwork.data(), P.cwork.size() * sizeof(CWork));
-  WG_HIP(hipMemcpyAsync(A.d_cwork, p, P.cwork.size() * sizeof(CWork), hipMemcpyHostToDevice, ctx->stream));
-  p += P.cwork.size() * sizeof(CWork);
   const uint64_t sc[4] = {P.work.size(), P.cwork.size(), P.arena_words, (uint64_t)n};
   memcpy(p, sc, sizeof sc);
-  WG_HIP(hipMemcpyAsync(A.d_scalars, p, sizeof sc, hipMemcpyHostToDevice, ctx->stream));
+  memcpy(p + 64, P.qs.data(), n * sizeof(BQ));
+  memcpy(p + desc_off_cwork(A), P.cwork.data(), P.cwork.size() * sizeof(CWork));
+  memcpy(p + desc_off_work(A), P.work.data(), P.work.size() * sizeof(BWork));
+  WG_HIP(hipMemcpyAsync(A.d_desc, p, desc_bytes, hipMemcpyHostToDevice, ctx->stream));
   return WG_OK;
 }
 
 static BCounters* counters_staging(BatchArena& A, const Plan& P) {
-  const int n = (int)P.qs.size();
-  char* p = (char*)A.pinned + n * sizeof(BQ) + P.work.size() * sizeof(BWork) + P.cwork.size() * sizeof(CWork) + 64;
+  char* p = (char*)A.pinned + (desc_off_work(A) + P.work.size() * sizeof(BWork) + 63) / 64 * 64;
   return (BCounters*)p;
 }
 
