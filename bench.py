@@ -396,7 +396,13 @@ def bench_wide(ctx, args):
     g = agg.num_groups()
     log(f"[wide] {n} rows, {g} groups: {ms:.3f} ms/step ({n / ms / 1e6:.1f} G rows/s)")
     agg.close()
-    return dict(rows=n, groups=g, ms_per_step=ms, rows_per_s=n / (ms / 1000))
+    # algorithmic bytes: the two key columns and the value column read once,
+    # each group's 128-bit key, sum and count written once
+    alg = n * 24 + g * 32
+    gbs = alg / (ms / 1000) / 1e9
+    return dict(rows=n, groups=g, ms_per_step=ms, rows_per_s=n / (ms / 1000),
+                roofline=dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=gbs / HBM_PEAK_GBS,
+                              alg_bytes=alg, kernel="whole step (dictionary claim pass + partitioned consume)"))
 
 
 # -------------------------------------------- Parquet pages decoded on the device

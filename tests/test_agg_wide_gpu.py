@@ -106,6 +106,34 @@ def test_shapes_extremes_round_trip(ctx, shape):
     check_wide(ctx, shape, 200_000, 40, seed=len(shape) * 7 + shape[0])
 
 
+@pytest.mark.parametrize("n_ones", [20_000, 300_000])
+def test_all_ones_key_words(ctx, n_ones):
+    """Pairs with an all-ones word (int64 key columns at -1) never read as
+    written while the claim pass runs: their rows take the verify pass, from
+    the pass's row list (up to 64K rows) or over every row (more)."""
+    print(f"[wide] all-ones key words in {n_ones} rows", flush=True)
+    rng = np.random.default_rng(77 + n_ones)
+    n = n_ones + 50_000
+    a = rng.integers(-1000, 1000, n).astype(np.int64)
+    a[rng.permutation(n)[:n_ones]] = -1
+    b = rng.integers(0, 8, n).astype(np.int64)
+    b[::7] = -1
+    v = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    d = rng.integers(0, 10**15, n).astype(np.int64)
+    f = rng.standard_normal(n)
+    types = [L.INT64, L.INT64, L.INT64, L.DECIMAL, L.FLOAT64]
+    scales = [0, 0, 0, 2, 0]
+    aggs = AGGS_OF(2)
+    agg = HashAggregate(ctx, types, [0, 1], aggs, scales=scales, group_hint=4096)
+    arrs = [a, b, v, d, f]
+    agg.consume([Col(ctx.to_device(x), t, None, sc) for x, t, sc in zip(arrs, types, scales)], n)
+    agg.finalize()
+    hcols = [(x, t, None, sc) for x, t, sc in zip(arrs, types, scales)]
+    ref = oracle_groups(run_oracle(hcols, [0, 1], aggs))
+    compare(gpu_groups(agg), ref, [(g.func, g.col) for g in aggs], types, scales, abs_sums(hcols, [0, 1], aggs))
+    agg.close()
+
+
 def test_many_groups_grow_the_dictionary(ctx):
     """~1.5M distinct pairs from a 1024 hint: the dictionary grows and the
     batch is encoded again; the table grows too."""

@@ -10,20 +10,28 @@
 // Dictionary slot (4 words, 32-byte aligned, all ones when free):
 //   fp      63-bit fingerprint of the pair (the claim word: CAS from free)
 //   lo, hi  the pair
-//   id      the pair's id (all ones until assigned)
-// An encode is three kernels, so that no kernel reads a pair another lane of
-// the same kernel is still writing (no fences on the hot path):
-//   claim   every row probes from fp; a slot holding its fingerprint ends the
-//           probe (the same pair, or — checked below — another pair with the
-//           same fingerprint); a free slot is claimed by CAS and its pair
-//           written;
-//   assign  the new slots get ids (workgroup ranges of the table, one shared
-//           counter atomic per workgroup) and the id -> pair array;
-//   lookup  every row probes again with plain loads and takes the id of the
-//           slot holding its exact pair.
-// A pair the lookup does not find (its fingerprint was another pair's in the
-// same claim pass) is claimed in another round, where the earlier claim's
-// pair is visible and differs; a table past its probe budget grows first.
+//   id      the pair's id
+// A pair claimed in slot i of a table gets id base + i: the claiming lane
+// knows it at once and writes the slot's pair and id and the id -> pair
+// entry. A table that grows moves its slots, ids included, into one twice
+// as large whose base is the old base + old cap, so ids are never reused.
+// An encode is one claim pass, so that no kernel reads a pair another lane
+// of the same kernel is still writing (no fences on the hot path):
+//   claim   every row probes from fp; a free slot is claimed by CAS (the row
+//           takes its id); a slot holding the row's fingerprint and a fully
+//           written pair (neither word the all-ones fill) ends the probe when
+//           the pair is the row's (the row takes the slot's id: the id word,
+//           or base + slot while the claimer has not written it) and goes on
+//           when it is another pair with the same fingerprint; a slot holding
+//           the fingerprint and a pair not yet readable gives the row the
+//           slot's id for now and puts the row on the verify list;
+//   verify  the listed rows probe again with plain loads and take the id of
+//           the slot holding their exact pair (every claim of the pass is
+//           visible now).
+// A listed pair the verify pass does not find (its fingerprint was another
+// pair's) is claimed in another round, where the earlier claim's pair is
+// visible and differs; a table past its probe budget or half full grows
+// first, and the claim pass runs again over every row.
 #include <algorithm>
 #include <vector>
 
@@ -36,21 +44,26 @@ namespace wg {
 constexpr uint64_t kDictFree = ~0ull;
 constexpr int kDictMaxProbe = 256;
 constexpr int kDictBlock = 256;           // threads per workgroup of the dictionary kernels
-constexpr uint64_t kAssignRange = 16384;  // table slots per workgroup of the assign pass
+constexpr uint64_t kVerifyCap = 1 << 16;  // rows a claim pass can list for the verify pass
 
 struct WideDict {
   uint64_t* slots;
   uint64_t mask;
+  uint64_t base;            // id of slot i: base + i
   uint64_t* keys;           // id -> {lo, hi}
-  unsigned long long* ctr;  // [0] ids handed out, [1] a claim ran out of probes, [2] a lookup missed
+  unsigned long long* ctr;  // [0] claims, [1] a claim ran out of probes, [2] a lookup missed, [3] rows to verify
+  uint64_t* vlist;
+  uint64_t vcap;
 };
 
 __device__ __forceinline__ uint64_t wide_fp(uint64_t lo, uint64_t hi) {
   return mix64(lo ^ mix64(hi + 0x9E3779B97F4A7C15ull)) >> 1;  // below the free pattern
 }
 
-// Claim pass for one pair.
-__device__ __forceinline__ void dict_claim(const WideDict& D, uint64_t lo, uint64_t hi) {
+// Claim pass for one pair: 0 found (id exact), 1 claimed (id exact), 2 its
+// fingerprint's slot does not show its pair yet (id to verify), 3 out of
+// probes.
+__device__ __forceinline__ int dict_claim(const WideDict& D, uint64_t lo, uint64_t hi, uint64_t& id) {
   const uint64_t fp = wide_fp(lo, hi);
   uint64_t i = fp & D.mask;
   for (int p = 0; p < kDictMaxProbe; ++p) {
@@ -60,23 +73,28 @@ __device__ __forceinline__ void dict_claim(const WideDict& D, uint64_t lo, uint6
       uint64_t expected = kDictFree;
       if (__hip_atomic_compare_exchange_strong(s, &expected, fp, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT)) {
+        id = D.base + i;
+        *(ulonglong2*)(s + 2) = make_ulonglong2(hi, id);
         s[1] = lo;
-        s[2] = hi;
-        return;
+        *(ulonglong2*)(D.keys + 2 * id) = make_ulonglong2(lo, hi);
+        return 1;
       }
       st = expected;
     }
     if (st == fp) {
-      // the same pair, unless this slot's pair reads fully written (neither
-      // word still the all-ones fill, so not half-written by a lane of this
-      // pass) and differs: then it is another pair with this fingerprint
-      const uint64_t l2 = s[1], h2 = s[2];
-      const bool settled = l2 != kDictFree && h2 != kDictFree;
-      if (!settled || (l2 == lo && h2 == hi)) return;
+      // the id word is all ones only while this pass's claimer has not
+      // written it, i.e. for a slot of this table claimed now: base + i
+      const uint64_t l2 = s[1];
+      const ulonglong2 b = *(const ulonglong2*)(s + 2);
+      const bool settled = l2 != kDictFree && b.x != kDictFree;
+      if (!settled || (l2 == lo && b.x == hi)) {
+        id = b.y != kDictFree ? b.y : D.base + i;
+        return settled ? 0 : 2;
+      }
     }
     i = (i + 1) & D.mask;
   }
-  D.ctr[1] = 1;  // out of probes: the table grows
+  return 3;
 }
 
 // Lookup pass for one pair: its id, or kNoId (flagged when FLAG) when absent.
@@ -157,11 +175,51 @@ struct SlotsSrc {
   __device__ void put(uint64_t j, uint64_t id) const { copy[j * stride] = id; }
 };
 
+// Claim pass over n items: every item's id (exact, or listed to verify);
+// claims counted per workgroup into ctr[0].
 template <typename Src>
 __global__ void __launch_bounds__(kDictBlock) dict_claim_kernel(Src src, uint64_t n, WideDict D) {
+  __shared__ uint32_t claims;
+  if (threadIdx.x == 0) claims = 0;
+  __syncthreads();
+  uint32_t mine = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t lo, hi, id = kNoId;
+    if (!src.get(i, lo, hi)) continue;
+    const int r = dict_claim(D, lo, hi, id);
+    if (r == 3) {
+      D.ctr[1] = 1;  // out of probes: the table grows and the pass runs again
+      continue;
+    }
+    mine += r == 1;
+    // rows to verify, one counter atomic per wave that has any (pairs with
+    // an all-ones word never read as written, so a batch of such keys costs
+    // one atomic per wave, not per row); past the list's end the host
+    // verifies every item
+    const uint64_t lm = __ballot(r == 2);  // the same in every active lane
+    if (lm) {
+      const int lead = __builtin_ctzll(lm);
+      const uint32_t lane = __lane_id();
+      unsigned long long at = 0;
+      if (lane == (uint32_t)lead) at = atomicAdd(D.ctr + 3, (unsigned long long)__popcll(lm));
+      at = __shfl(at, lead) + __popcll(lm & ((1ull << lane) - 1));
+      if (r == 2 && at < D.vcap) D.vlist[at] = i;
+    }
+    src.put(i, id);
+  }
+  if (mine) atomicAdd(&claims, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && claims) atomicAdd(D.ctr, (unsigned long long)claims);
+}
+
+// Verify pass over the listed items (list) or over all n items (list null).
+template <typename Src>
+__global__ void __launch_bounds__(kDictBlock) dict_verify_kernel(Src src, const uint64_t* list, uint64_t n,
+                                                                 WideDict D) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = list ? list[k] : k;
     uint64_t lo, hi;
-    if (src.get(i, lo, hi)) dict_claim(D, lo, hi);
+    if (src.get(i, lo, hi)) src.put(i, dict_lookup<true>(D, lo, hi));
   }
 }
 
@@ -170,47 +228,6 @@ __global__ void __launch_bounds__(kDictBlock) dict_lookup_kernel(Src src, uint64
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t lo, hi;
     if (src.get(i, lo, hi)) src.put(i, dict_lookup<FLAG>(D, lo, hi));
-  }
-}
-
-__device__ __forceinline__ bool needs_id(const uint64_t* s) { return s[0] != kDictFree && s[3] == kNoId; }
-
-// Ids for the slots claimed since the last assign: each workgroup numbers
-// the new slots of its kAssignRange-slot range in slot order from a base
-// taken from the shared counter.
-__global__ void __launch_bounds__(kDictBlock) dict_assign_kernel(WideDict D, uint64_t cap) {
-  __shared__ uint32_t wave_tot[kDictBlock / 64];
-  __shared__ unsigned long long base;
-  constexpr uint64_t per = kAssignRange / kDictBlock;
-  const uint64_t beg = blockIdx.x * kAssignRange + threadIdx.x * per;
-  const uint64_t end = min(beg + per, cap);
-  uint32_t mine = 0;
-  for (uint64_t j = beg; j < end; ++j) mine += needs_id(D.slots + j * 4) ? 1u : 0u;
-  // exclusive prefix of `mine` over the workgroup
-  const int lane = __lane_id(), wave = threadIdx.x >> 6;
-  uint32_t x = mine;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d);
-    if (lane >= d) x += y;
-  }
-  if (lane == 63) wave_tot[wave] = x;
-  __syncthreads();
-  uint32_t off = 0, total = 0;
-  for (int w = 0; w < kDictBlock / 64; ++w) {
-    if (w < wave) off += wave_tot[w];
-    total += wave_tot[w];
-  }
-  if (threadIdx.x == 0) base = total ? atomicAdd(D.ctr, (unsigned long long)total) : 0ull;
-  __syncthreads();
-  uint64_t id = base + off + x - mine;
-  for (uint64_t j = beg; j < end; ++j) {
-    uint64_t* s = D.slots + j * 4;
-    if (!needs_id(s)) continue;
-    s[3] = id;
-    D.keys[2 * id] = s[1];
-    D.keys[2 * id + 1] = s[2];
-    ++id;
   }
 }
 
@@ -241,7 +258,7 @@ __global__ void fill_words_kernel(uint64_t* p, uint64_t n, uint64_t v) {
 }
 
 static WideDict dict_of(const KeyDict& d) {
-  return WideDict{d.slots, d.cap - 1, d.keys, (unsigned long long*)d.ctr};
+  return WideDict{d.slots, d.cap - 1, d.base, d.keys, (unsigned long long*)d.ctr, d.vlist, d.vcap};
 }
 
 static wg_status fill(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v) {
@@ -251,20 +268,22 @@ static wg_status fill(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t v) {
   return WG_OK;
 }
 
-// A table of `cap` slots and an id array of as many entries; the claimed
-// slots of the old table move over with their ids.
+// A table of `cap` slots whose ids start after the old table's, and an id
+// array over the whole id space; the claimed slots of the old table move
+// over with their ids.
 static wg_status dict_alloc(wg_ctx* ctx, KeyDict& d, uint64_t cap) {
+  const uint64_t base = d.slots ? d.base + d.cap : 0;
   uint64_t* slots = nullptr;
   uint64_t* keys = nullptr;
   WG_ALLOC(ctx, &slots, cap * 32, "a key dictionary");
-  const wg_status st = dev_alloc(ctx, (void**)&keys, cap * 16, nullptr, "a key dictionary");
+  const wg_status st = dev_alloc(ctx, (void**)&keys, (base + cap) * 16, nullptr, "a key dictionary");
   if (st != WG_OK) {
     (void)hipFree(slots);
     return st;
   }
   WG_TRY(fill(ctx, slots, cap * 4, kDictFree));
   if (d.slots) {
-    if (d.n) WG_HIP(hipMemcpyAsync(keys, d.keys, d.n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    WG_HIP(hipMemcpyAsync(keys, d.keys, d.n * 16, hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(dict_rehash_kernel, dim3(grid_for(d.cap, 256)), dim3(256), 0, ctx->stream,
                        (const uint64_t*)d.slots, d.cap, slots, cap - 1);
     WG_HIP(hipGetLastError());
@@ -275,13 +294,17 @@ static wg_status dict_alloc(wg_ctx* ctx, KeyDict& d, uint64_t cap) {
   d.slots = slots;
   d.keys = keys;
   d.cap = cap;
+  d.base = base;
+  d.n = base + cap;
   return WG_OK;
 }
 
 wg_status kd_init(wg_ctx* ctx, KeyDict& d, uint64_t hint) {
-  WG_ALLOC(ctx, &d.ctr, 24, "the key dictionary's counters");
-  WG_TRY(zero_dev(ctx, d.ctr, 3));
-  d.n = 0;
+  WG_ALLOC(ctx, &d.ctr, 32, "the key dictionary's counters");
+  WG_TRY(zero_dev(ctx, d.ctr, 4));
+  WG_ALLOC(ctx, &d.vlist, kVerifyCap * 8, "the key dictionary's verify list");
+  d.vcap = kVerifyCap;
+  d.claimed = 0;
   return dict_alloc(ctx, d, next_pow2(std::max<uint64_t>(2 * std::max<uint64_t>(hint, 1024), 64)));
 }
 
@@ -289,45 +312,47 @@ void kd_free(KeyDict& d) {
   (void)hipFree(d.slots);
   (void)hipFree(d.keys);
   (void)hipFree(d.ctr);
+  (void)hipFree(d.vlist);
   d = KeyDict{};
 }
 
 wg_status kd_reset(wg_ctx* ctx, KeyDict& d) {
   WG_TRY(fill(ctx, d.slots, d.cap * 4, kDictFree));
-  WG_TRY(zero_dev(ctx, d.ctr, 3));
-  d.n = 0;
+  WG_TRY(zero_dev(ctx, d.ctr, 4));
+  d.claimed = 0;
+  d.base = 0;  // every id is free again (the id array holds at least cap entries)
+  d.n = d.cap;
   return WG_OK;
 }
 
-uint64_t kd_bytes(const KeyDict& d) { return d.cap * 48; }
+uint64_t kd_bytes(const KeyDict& d) { return d.cap * 32 + d.n * 16 + d.vcap * 8; }
 
-// claim -> assign -> lookup over n items of src, growing the table when a
-// claim ran out of probes or more than half of it is taken, and claiming
-// again when a lookup missed (a fingerprint shared by two pairs).
+// claim (-> verify the listed items) over n items of src, growing the table
+// when a claim ran out of probes or more than half of it is taken, and
+// claiming again when a verify missed (a fingerprint shared by two pairs).
 template <typename Src>
 static wg_status dict_encode(wg_ctx* ctx, KeyDict& d, const Src& src, uint64_t n) {
   if (n == 0) return WG_OK;
   const int grid = grid_for(n, kDictBlock, 8192);
   for (int round = 0; round < 12; ++round) {
-    WG_TRY(zero_dev(ctx, d.ctr + 1, 2));
+    WG_TRY(zero_dev(ctx, d.ctr, 4));
     hipLaunchKernelGGL(dict_claim_kernel<Src>, dim3(grid), dim3(kDictBlock), 0, ctx->stream, src, n, dict_of(d));
     WG_HIP(hipGetLastError());
-    const uint64_t ranges = (d.cap + kAssignRange - 1) / kAssignRange;
-    hipLaunchKernelGGL(dict_assign_kernel, dim3((unsigned)ranges), dim3(kDictBlock), 0, ctx->stream, dict_of(d),
-                       d.cap);
-    WG_HIP(hipGetLastError());
-    uint64_t c[3];
-    WG_HIP(hipMemcpyAsync(c, d.ctr, 24, hipMemcpyDeviceToHost, ctx->stream));
+    uint64_t c[4];
+    WG_HIP(hipMemcpyAsync(c, d.ctr, 32, hipMemcpyDeviceToHost, ctx->stream));
     WG_HIP(hipStreamSynchronize(ctx->stream));
-    d.n = c[0];
-    if (c[1] || 2 * c[0] > d.cap) {  // out of probes, or over half full: grow and claim again
-      WG_TRY(dict_alloc(ctx, d, std::max<uint64_t>(2 * d.cap, next_pow2(4 * c[0]))));
+    d.claimed += c[0];
+    if (c[1] || 2 * d.claimed > d.cap) {  // out of probes, or over half full: grow and claim again
+      WG_TRY(dict_alloc(ctx, d, std::max<uint64_t>(2 * d.cap, next_pow2(4 * d.claimed))));
       continue;
     }
-    hipLaunchKernelGGL((dict_lookup_kernel<Src, true>), dim3(grid), dim3(kDictBlock), 0, ctx->stream, src, n,
-                       dict_of(d));
+    if (!c[3]) return WG_OK;
+    const bool listed = c[3] <= d.vcap;
+    const uint64_t m = listed ? c[3] : n;
+    hipLaunchKernelGGL(dict_verify_kernel<Src>, dim3(grid_for(m, kDictBlock, 8192)), dim3(kDictBlock), 0,
+                       ctx->stream, src, listed ? (const uint64_t*)d.vlist : nullptr, m, dict_of(d));
     WG_HIP(hipGetLastError());
-    WG_HIP(hipMemcpyAsync(c, d.ctr, 24, hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(hipMemcpyAsync(c, d.ctr, 32, hipMemcpyDeviceToHost, ctx->stream));
     WG_HIP(hipStreamSynchronize(ctx->stream));
     if (!c[2]) return WG_OK;
   }

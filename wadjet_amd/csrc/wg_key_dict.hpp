@@ -13,9 +13,14 @@ struct ColSet;
 struct KeyDict {
   uint64_t* slots = nullptr;  // cap slots of 4 words {fingerprint, lo, hi, id}
   uint64_t cap = 0;
-  uint64_t* keys = nullptr;   // id -> {lo, hi}, cap entries
-  uint64_t n = 0;             // ids handed out (as of the last read-back)
-  uint64_t* ctr = nullptr;    // device: [0] ids handed out, [1] a claim ran out of probes, [2] a lookup missed
+  uint64_t base = 0;          // id of a pair claimed in slot i of this table: base + i
+  uint64_t* keys = nullptr;   // id -> {lo, hi}, n entries
+  uint64_t n = 0;             // id space: base + cap (ids of earlier, smaller tables below base)
+  uint64_t claimed = 0;       // taken slots of this table (as of the last read-back)
+  uint64_t* ctr = nullptr;    // device: [0] claims of a pass, [1] a claim ran out of probes, [2] a lookup
+                              // missed, [3] rows to verify
+  uint64_t* vlist = nullptr;  // device: the rows a claim pass could not settle (vcap entries)
+  uint64_t vcap = 0;
 };
 
 constexpr uint64_t kNoId = ~0ull;  // the id of a pair not in the dictionary
