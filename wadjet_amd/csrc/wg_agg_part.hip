@@ -50,7 +50,10 @@ namespace wg {
 
 constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
 constexpr int kL1Block = 512;       // pass-1 workgroup size, general loader
-constexpr int kL1TypedBlock = 512;  // and typed loads
+// and typed loads: 768 threads at up to 80 registers (amdgpu_waves_per_eu(6)
+// below), so two workgroups with their LDS heavy-hitter tables keep six
+// waves per SIMD resident
+constexpr int kL1TypedBlock = 768;
 
 struct PartParams {
   ColSet cs;
@@ -401,7 +404,8 @@ __device__ __forceinline__ uint32_t pred_bits(const uint64_t (&v)[U], bool is_fl
 // also the predicate column is read once. Classifies and writes exactly what
 // part_l1_kernel does for the same rows.
 template <int W0, int W1, int PW, int RW>
-__global__ void __launch_bounds__(kL1TypedBlock) part_l1_typed_kernel(PartParams P) {
+__global__ void __launch_bounds__(kL1TypedBlock) __attribute__((amdgpu_waves_per_eu(6)))
+part_l1_typed_kernel(PartParams P) {
   constexpr int U = kL1TU, NV = RW - 1;
   extern __shared__ uint32_t lds1[];
   const HotLds HL = l1_setup(P, lds1);
