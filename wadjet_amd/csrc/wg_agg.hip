@@ -1449,6 +1449,10 @@ wg_status wg_agg_destroy(wg_agg* h) {
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   for (hipEvent_t e : h->pev)
     if (e) (void)hipEventDestroy(e);
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    (void)hipStreamDestroy(h->side);
+  }
   (void)hipFree(h->part_cnt);
   (void)hipFree(h->part_scan);
   (void)hipFree(h->part_buf);

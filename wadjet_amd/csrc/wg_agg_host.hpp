@@ -94,6 +94,9 @@ struct wg_agg {
   uint64_t hot_buf_words = 0;
   float pass_ms[4] = {0, 0, 0, 0};  // direct consume, count+scan, scatter, partition aggregate
   hipEvent_t pev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // the partitioned consume's copies of its boundaries and work items, on a
+  // stream of their own so that pass 2 never waits on them
+  hipStream_t side = nullptr;
   // tuning
   int32_t lds_slots = 1024;
   int32_t block = 512;
