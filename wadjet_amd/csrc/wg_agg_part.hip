@@ -572,11 +572,24 @@ __global__ void hot_reset_kernel(HotScratch X) {
 
 // Count the keys of kSampleTiles tiles of kSampleTileRows rows, spread over
 // the batch (same filtering and key packing as the passes).
+// Each workgroup counts its tile's keys in an LDS table first (one slot per
+// distinct key of the tile), then adds the tile's counts to the sample
+// table: one global claim and add per distinct key and tile, so the most
+// frequent keys do not serialise on their sample slot.
+constexpr uint32_t kSampleLdsSlots = 2 * (uint32_t)kSampleTileRows;
+
 __global__ void __launch_bounds__(256) hot_sample_kernel(PartParams P, HotScratch X) {
+  __shared__ uint64_t skey[kSampleLdsSlots];
+  __shared__ uint32_t scnt[kSampleLdsSlots];
   const RowSpec<0> S = make_spec<0>(P);
   const uint64_t mask = (1ull << kSampleBits) - 1;
   const uint64_t gap = P.n / kSampleTiles;
   for (uint64_t t = blockIdx.x; t < kSampleTiles; t += gridDim.x) {
+    for (uint32_t i = threadIdx.x; i < kSampleLdsSlots; i += blockDim.x) {
+      skey[i] = kEmpty;
+      scnt[i] = 0;
+    }
+    __syncthreads();
     const uint64_t beg = t * gap, end = min(beg + kSampleTileRows, P.n);
     for (uint64_t base = beg; base < end; base += (uint64_t)blockDim.x * kRowU) {
       uint64_t row[kRowU], key[kRowU], val[kRowU][1];
@@ -586,19 +599,35 @@ __global__ void __launch_bounds__(256) hot_sample_kernel(PartParams P, HotScratc
 #pragma unroll
       for (int u = 0; u < kRowU; ++u) {
         if (kind[u] != 0) continue;
-        // at most kSampleTiles * kSampleTileRows keys in a table twice that size
-        uint64_t idx = key_hash(key[u]) & mask;
-        for (uint64_t p = 0; p <= mask; ++p) {
-          uint64_t* s = X.tab + 2 * idx;
-          const uint64_t prev = at_cas<GlbT>(s, kEmpty, key[u]);
+        // at most kSampleTileRows keys in a table twice that size: every probe ends
+        uint32_t idx = (uint32_t)(key_hash(key[u]) >> 40) & (kSampleLdsSlots - 1);
+        for (;;) {
+          const uint64_t prev = at_cas<LdsT>(skey + idx, kEmpty, key[u]);
           if (prev == kEmpty || prev == key[u]) {
-            at_add<GlbT>(s + 1, 1);
+            atomicAdd(scnt + idx, 1u);
             break;
           }
-          idx = (idx + 1) & mask;
+          idx = (idx + 1) & (kSampleLdsSlots - 1);
         }
       }
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kSampleLdsSlots; i += blockDim.x) {
+      const uint64_t k = skey[i];
+      if (k == kEmpty) continue;
+      // at most kSampleTiles * kSampleTileRows keys in a table twice that size
+      uint64_t idx = key_hash(k) & mask;
+      for (uint64_t p = 0; p <= mask; ++p) {
+        uint64_t* s = X.tab + 2 * idx;
+        const uint64_t prev = at_cas<GlbT>(s, kEmpty, k);
+        if (prev == kEmpty || prev == k) {
+          at_add<GlbT>(s + 1, scnt[i]);
+          break;
+        }
+        idx = (idx + 1) & mask;
+      }
+    }
+    __syncthreads();
   }
 }
 
