@@ -72,6 +72,31 @@ def test_served_equals_alone_equals_oracle(ctx):
     plain.close()
 
 
+def test_results_after_reset_reuse_buffers(ctx):
+    """A reset returns the requests' result buffers to the server's pool; the
+    next runs deliver into those buffers (in another order, so sizes differ
+    from the first owner's) and every result equals the first run's."""
+    n = 2_000_000
+    cols = W.gen_flow_table(ctx, n, seed=11)
+    dim = W.gen_flow_dim(ctx, 20_000)
+    j = HashJoin(ctx, W.FLOW_DIM_TYPES, 0, inline=[1], build_hint=20_000)
+    j.build(dim, 20_000)
+    j.finish()
+    qs, _ = W.plan_flow_requests(n, 40, seed=8, scale=0.05)
+    print(f"[serve] {len(qs)} requests, three runs with resets", flush=True)
+    s = Server(ctx, cols, n, 3, lanes=3, hbm_budget=1 << 30, max_batch=8)
+    s.add_dim(j)
+    _, ids = s.replay(qs, qps=3000, seed=1)
+    first = [s.result(int(i)) for i in ids]
+    for rep in range(2):
+        s.reset()
+        order = list(range(len(qs)))[::-1] if rep == 0 else list(np.random.default_rng(rep).permutation(len(qs)))
+        _, ids2 = s.replay([qs[k] for k in order], qps=3000, seed=2 + rep)
+        for pos, k in enumerate(order):
+            same(s.result(int(ids2[pos])), first[k], k)
+    s.close()
+
+
 def test_admission_respects_the_budget(ctx):
     """A tiny HBM budget forces requests through one at a time; results stay exact."""
     n = 1_000_000
