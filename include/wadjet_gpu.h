@@ -163,7 +163,7 @@ typedef struct wg_agg wg_agg;
  * key; 2-4 int-class keys -> packed composite key (packed_hash.go:386-472):
  * one 64-bit word when the columns fit it, else the reference's 128-bit
  * {lo, hi} layout (widest column first), dictionary-encoded on the device to
- * a dense id per distinct key (a 128-bit key needs n_cols < 16).
+ * a 64-bit id per distinct key (a 128-bit key needs n_cols < 16).
  * group_hint sizes the table (InputRowHint/GroupNDVHint, aggregate.go:1464-1506);
  * the table grows when it is exceeded.
  * Keys wider than 16 bytes in total, string keys and other functions return
