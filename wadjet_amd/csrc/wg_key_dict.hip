@@ -29,7 +29,7 @@
 //           the slot holding their exact pair (every claim of the pass is
 //           visible now).
 // A listed pair the verify pass does not find (its fingerprint was another
-// pair's) is claimed in another round, where the earlier claim's pair is
+// pair's) is claimed by a further claim pass, where the earlier claim's pair is
 // visible and differs; a table past its probe budget or half full grows
 // first, and the claim pass runs again over every row.
 #include <algorithm>
