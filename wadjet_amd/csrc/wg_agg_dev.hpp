@@ -970,13 +970,20 @@ __device__ __forceinline__ void apply_vals_lds_batch(uint64_t* const (&w)[U], in
   }
   for (; o < pg.kind_end[OP_SUMF]; ++o) {
     const OpDesc op = pg.ops[o];
-    const bool f1 = (vfloat >> op.col) & 1u;
+    // the column's type is the same in every lane: one branch per op, so a
+    // float64 column pays for no int64 -> float64 conversion
+    if ((vfloat >> op.col) & 1u) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
-      const uint64_t b = pick(v[u], op.col);
-      const double x = f1 ? __longlong_as_double((long long)b) : (double)(int64_t)b;
-      at_addf<LdsT>(w[u] + op.word * ws, ok ? x : -0.0);
+      for (int u = 0; u < U; ++u) {
+        const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
+        at_addf<LdsT>(w[u] + op.word * ws, ok ? __longlong_as_double((long long)pick(v[u], op.col)) : -0.0);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
+        at_addf<LdsT>(w[u] + op.word * ws, ok ? (double)(int64_t)pick(v[u], op.col) : -0.0);
+      }
     }
   }
   for (; o < pg.kind_end[OP_SUMD]; ++o) {
@@ -1014,21 +1021,32 @@ __device__ __forceinline__ void apply_vals_lds_batch(uint64_t* const (&w)[U], in
   }
   for (; o < pg.kind_end[OP_MAX]; ++o) {  // MIN then MAX
     const OpDesc op = pg.ops[o];
-    const bool f1 = (vfloat >> op.col) & 1u;
     const bool is_min = op.kind == OP_MIN;
+    // encoded operands, one branch per op on the column's (lane-uniform) type
+    uint64_t e[U];
+    uint32_t ok = 0;
+    if ((vfloat >> op.col) & 1u) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t b = pick(v[u], op.col);
-      bool ok = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u);
-      uint64_t e;
-      if (f1) {
-        ok = ok && (b & 0x7FFFFFFFFFFFFFFFull) <= 0x7FF0000000000000ull;  // NaN never wins
-        e = enc_f64_bits(b);
-      } else {
-        e = enc_i64((int64_t)b);
+      for (int u = 0; u < U; ++u) {
+        const uint64_t b = pick(v[u], op.col);
+        const bool k = ((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u) &&
+                       (b & 0x7FFFFFFFFFFFFFFFull) <= 0x7FF0000000000000ull;  // NaN never wins
+        ok |= (uint32_t)k << u;
+        e[u] = enc_f64_bits(b);
       }
-      if (is_min) at_min<LdsT>(w[u] + op.word * ws, ok ? e : kMinInit);
-      else at_max<LdsT>(w[u] + op.word * ws, ok ? e : kMaxInit);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ok |= (uint32_t)(((live >> u) & 1u) && ((vvalid[u] >> op.col) & 1u)) << u;
+        e[u] = enc_i64((int64_t)pick(v[u], op.col));
+      }
+    }
+    if (is_min) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) at_min<LdsT>(w[u] + op.word * ws, ((ok >> u) & 1u) ? e[u] : kMinInit);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) at_max<LdsT>(w[u] + op.word * ws, ((ok >> u) & 1u) ? e[u] : kMaxInit);
     }
   }
 }
