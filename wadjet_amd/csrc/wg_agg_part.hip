@@ -684,17 +684,25 @@ __global__ void __launch_bounds__(256) hot_collect_kernel(HotScratch X, uint32_t
 __global__ void __launch_bounds__(256) hot_merge_kernel(const uint64_t* part, uint32_t G, uint32_t H,
                                                        const uint32_t* n_hot, AggProgram pg, GlobalTable T,
                                                        uint64_t* ovf_part, DenseOut dense) {
-  __shared__ uint64_t s[64];
+  // one accumulator slot per wave (fewer lanes on each LDS word), folded
+  // into the first at the end
+  __shared__ uint64_t sw[4][64];
+  uint64_t* const s = sw[0];
   const int stride = pg.stride;
   const uint32_t h = blockIdx.x;
   uint32_t claims = 0;
   if (h < min(*n_hot, H)) {  // uniform per block
-    if (threadIdx.x < (unsigned)stride) s[threadIdx.x] = slot_init_word(pg, threadIdx.x);
+    if (threadIdx.x < 4u * 64u) sw[threadIdx.x >> 6][threadIdx.x & 63] = slot_init_word(pg, threadIdx.x & 63);
     __syncthreads();
+    uint64_t* const mine = sw[threadIdx.x >> 6];
     for (uint32_t g = threadIdx.x; g < G; g += blockDim.x) {
       const uint64_t* src = part + ((uint64_t)g * H + h) * stride;
-      if (src[1]) merge_partial<LdsT>(s, src, 1, pg);  // word 1: the slot's row count
+      if (src[1]) merge_partial<LdsT>(mine, src, 1, pg);  // word 1: the slot's row count
     }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 1; q < 4; ++q)
+        if (sw[q][1]) merge_partial<LdsT>(s, sw[q], 1, pg);
     __syncthreads();
     if (threadIdx.x == 0 && s[1] && dense.rows) {
       // hot keys are never partitioned: this block holds all of the key's rows
