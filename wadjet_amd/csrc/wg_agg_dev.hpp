@@ -334,12 +334,14 @@ enum Ctr : int {
   CTR_N = 8
 };
 // device counter words per aggregate handle: the CTR_N counters, then the
-// dense-output row count (outside the range the consume passes clear)
+// dense-output row count (outside the range the consume passes clear) and the
+// partition aggregate's work-item cursor, so that one copy reads them all back
 constexpr int kCtrDense = CTR_N;
-constexpr int kAggCounterWords = CTR_N + 1;
-// pinned words per aggregate handle: the CTR_N counters, then work-item
-// cursors and the dense row count read back
+constexpr int kCtrItem = CTR_N + 1;
+constexpr int kAggCounterWords = CTR_N + 2;
+// pinned words per aggregate handle: the counter words read back
 constexpr int kAggPinnedWords = CTR_N + 8;
+static_assert(kAggPinnedWords >= kAggCounterWords, "every counter word has a pinned word");
 
 // Linear probing stops after this many slots; a table that needs more is
 // past its load budget and the host grows it (the claim reports overflow).

@@ -889,6 +889,16 @@ __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t& to
   return before + s - v;
 }
 
+// Counters of one partition-aggregate launch: the per-launch words cleared,
+// the work-item cursor set to the first item to take, the dense row count
+// cleared when asked (once per consume, before anything writes the output).
+__global__ void part_round_begin_kernel(uint64_t* ctr, uint64_t start_item, int zero_dense) {
+  const int t = threadIdx.x;
+  if (t >= CTR_OVF_ROWS && t < CTR_N) ctr[t] = 0;
+  if (t == kCtrDense && zero_dense) ctr[t] = 0;
+  if (t == kCtrItem) ctr[t] = start_item;
+}
+
 constexpr int kFlushU = 2;
 // A partition's keys fill its LDS table to about 3/4 of L. A key that finds
 // no LDS slot aggregates in the global table, and under dense emission one
@@ -1265,7 +1275,7 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   }
   uint32_t* bound = (uint32_t*)h->part_host;
   PartItem* hitems = (PartItem*)((char*)h->part_host + ((np + 1) * 4 + 15) / 16 * 16);
-  const uint64_t ibytes = max_items * sizeof(PartItem) + 64;
+  const uint64_t ibytes = max_items * sizeof(PartItem);
   if (h->part_items_bytes < ibytes)
     WG_TRY(regrow(ctx, &h->part_items, &h->part_items_bytes, ibytes, 1, "partition work items"));
   const size_t lds_bytes = lds_table_bytes(L, h->pg.stride);
@@ -1350,7 +1360,6 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   WG_CHECK(n_items <= max_items && total <= n, WG_ERR_DEVICE, "partition boundaries out of range");
   std::stable_sort(hitems, hitems + n_items,
                    [](const PartItem& a, const PartItem& b) { return a.end - a.beg > b.end - b.beg; });
-  unsigned long long* item_ctr = (unsigned long long*)((char*)h->part_items + ibytes - 64);
   if (n_items) {  // uploaded on the side stream while pass 2 runs
     WG_HIP(hipMemcpyAsync(h->part_items, hitems, n_items * sizeof(PartItem), hipMemcpyHostToDevice, h->side));
     WG_HIP(hipEventRecord(h->pev[7], h->side));
@@ -1363,11 +1372,10 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   Q.items = (const PartItem*)h->part_items;
   Q.n_items = (uint32_t)n_items;
   Q.lds_slots = L;
-  Q.item_ctr = item_ctr;
+  Q.item_ctr = (unsigned long long*)(h->counters + kCtrItem);
   Q.vfloat = P.vfloat;
   DenseOut D{};
   if (dense) {
-    WG_TRY(zero_dev(ctx, h->counters + kCtrDense, 1));
     D = DenseOut{h->dense, h->n_sumd ? h->dense_base : nullptr, h->dense_null,
                  (unsigned long long*)(h->counters + kCtrDense), dense_bound, h->n_sumd * 2};
   }
@@ -1392,17 +1400,30 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   // the table: their merge is queued right behind the first aggregate launch,
   // and its results are read back with that launch's counters.
   bool hot_done = false, dense_read = false;
+  // the dense row count is cleared by the first launch's counter set-up
+  bool dense_clear = dense;
+  auto begin_round = [&](uint64_t start_item) -> wg_status {
+    hipLaunchKernelGGL(part_round_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, h->counters, start_item,
+                       dense_clear ? 1 : 0);
+    WG_HIP(hipGetLastError());
+    dense_clear = false;
+    return WG_OK;
+  };
+  // every counter word to the pinned words in one copy (synchronises the stream)
+  auto read_back = [&](uint64_t* c) -> wg_status {
+    WG_HIP(hipMemcpyAsync(h->pinned, h->counters, kAggCounterWords * 8, hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(c, h->pinned, CTR_N * 8);
+    return WG_OK;
+  };
   for (int round = 0; round < 64; ++round) {
     const bool replay = !replays.empty();
     const uint32_t* rsel = replay ? replays.back().rows : nullptr;
     const uint64_t rn = replay ? replays.back().n : 0;
     const uint64_t total_work = replay ? (rn + chunk - 1) / chunk : n_items;
     if (!replay && items_done >= n_items) break;
-    WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
     const uint64_t start_item = replay ? 0 : items_done;
-    uint64_t* pin = h->pinned + CTR_N;  // [0] start item, [1] items taken (the counters use [0, CTR_N))
-    pin[0] = start_item;
-    WG_HIP(hipMemcpyAsync(item_ctr, pin, 8, hipMemcpyHostToDevice, ctx->stream));
+    WG_TRY(begin_round(start_item));
     Q.T = table_of(h);
     Q.sel = rsel;
     Q.n_sel = rn;
@@ -1423,13 +1444,10 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
       WG_HIP(hipEventRecord(h->pev[6], ctx->stream));
       hot_done = true;
     }
-    WG_HIP(hipMemcpyAsync(pin + 1, item_ctr, 8, hipMemcpyDeviceToHost, ctx->stream));
-    if (dense)
-      WG_HIP(hipMemcpyAsync(pin + 2, h->counters + kCtrDense, 8, hipMemcpyDeviceToHost, ctx->stream));
     dense_read = dense;
     uint64_t c[CTR_N];
-    WG_TRY(agg_read_counters(h, c));  // synchronises the stream
-    const uint64_t taken = pin[1];
+    WG_TRY(read_back(c));
+    const uint64_t taken = h->pinned[kCtrItem];
     float ms = 0.f;
     if (first) {
       WG_HIP(hipEventElapsedTime(&ms, h->pev[0], h->pev[1]));
@@ -1478,15 +1496,15 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
   if (H && !hot_done) {
     // E: the hot keys' per-workgroup partials into the global table (or the
     // dense output when no aggregate launch ran)
-    dense_read = false;
-    WG_TRY(zero_dev(ctx, h->counters + CTR_OVF_ROWS, CTR_N - 1));
+    WG_TRY(begin_round(0));
     WG_HIP(hipEventRecord(h->pev[4], ctx->stream));
     hipLaunchKernelGGL(hot_merge_kernel, dim3(H), dim3(256), 0, ctx->stream, (const uint64_t*)P.hot_part, (uint32_t)G,
                        (uint32_t)H, P.n_hot, h->pg, table_of(h), h->ovf_part, D);
     WG_HIP(hipGetLastError());
     WG_HIP(hipEventRecord(h->pev[5], ctx->stream));
     uint64_t c[CTR_N];
-    WG_TRY(agg_read_counters(h, c));
+    WG_TRY(read_back(c));
+    dense_read = dense;
     float ms = 0.f;
     WG_HIP(hipEventElapsedTime(&ms, h->pev[4], h->pev[5]));
     h->pass_ms[3] += ms;
@@ -1497,11 +1515,8 @@ wg_status agg_consume_partitioned(wg_agg* h, const ColSet& cs, const uint32_t* s
     }
   }
   if (dense) {
-    if (!dense_read) {  // the last launch did not read the dense count back
-      WG_HIP(hipMemcpyAsync(h->pinned + CTR_N + 2, h->counters + kCtrDense, 8, hipMemcpyDeviceToHost, ctx->stream));
-      WG_HIP(hipStreamSynchronize(ctx->stream));
-    }
-    h->n_dense = h->pinned[CTR_N + 2];
+    // no launch ran: nothing was written to the dense output
+    h->n_dense = dense_read ? h->pinned[kCtrDense] : 0;
     WG_CHECK(h->n_dense <= D.cap, WG_ERR_DEVICE, "dense output overran its reservation");
     if (h->groups) h->dirty = true;  // split items, LDS misses or replays used the table
   }
