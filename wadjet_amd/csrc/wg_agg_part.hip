@@ -48,7 +48,7 @@
 
 namespace wg {
 
-constexpr int kAggU = 4;         // records per thread per step of the aggregate loop
+constexpr int kAggU = 2;         // records per thread per step of the aggregate loop
 constexpr int kL1Block = 512;       // pass-1 workgroup size, general loader
 // and typed loads: 768 threads at up to 80 registers (amdgpu_waves_per_eu(6)
 // below), so two workgroups with their LDS heavy-hitter tables keep six
